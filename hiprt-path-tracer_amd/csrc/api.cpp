@@ -1,0 +1,374 @@
+// api.cpp -- the extern "C" entry points of libmipt (declared in include/mipt.h).
+// Each one validates its arguments, runs the C++ implementation and turns any
+// failure into a negative status plus a message for mipt_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "internal.h"
+
+namespace mipt {
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        fail(e == hipErrorOutOfMemory ? MIPT_ERR_OUT_OF_MEMORY : MIPT_ERR_HIP,
+             std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DeviceBuffer::release()
+{
+    if (ptr)
+        (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+void DeviceBuffer::reserve(size_t n)
+{
+    if (n <= bytes && ptr)
+        return;
+    release();
+    if (n == 0)
+        n = 1;
+    HIP_CHECK(hipMalloc(&ptr, n));
+    bytes = n;
+}
+
+namespace {
+
+template <class F> int guarded(Ctx* ctx, F&& f)
+{
+    try
+    {
+        if (ctx)
+        {
+            int cur = -1;
+            if (hipGetDevice(&cur) == hipSuccess && cur != ctx->device)
+                HIP_CHECK(hipSetDevice(ctx->device));
+        }
+        f();
+        return MIPT_OK;
+    }
+    catch (const Error& e)
+    {
+        if (ctx)
+            ctx->last_error = e.msg;
+        return e.code;
+    }
+    catch (const std::bad_alloc&)
+    {
+        if (ctx)
+            ctx->last_error = "host allocation failed";
+        return MIPT_ERR_OUT_OF_MEMORY;
+    }
+}
+
+template <class T> void upload(DeviceBuffer& buf, const T* host, size_t count)
+{
+    buf.reserve(sizeof(T) * (count > 0 ? count : 1));
+    if (count > 0)
+        HIP_CHECK(hipMemcpy(buf.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice));
+}
+
+} // namespace
+} // namespace mipt
+
+using mipt::Ctx;
+
+extern "C" {
+
+const char* mipt_version(void) { return "mipt 0.1 (gfx950)"; }
+
+void mipt_material_default(mipt_material* m)
+{
+    if (!m)
+        return;
+    std::memset(m, 0, sizeof(*m));
+    auto set3 = [](float* d, float v) { d[0] = d[1] = d[2] = v; };
+    // Defaults of CPUMaterial, src/HostDeviceCommon/Material/MaterialCPU.h:213-330.
+    set3(m->emission, 0.0f);
+    m->emission_strength = 1.0f;
+    m->global_emissive_factor = 1.0f;
+    m->emissive_texture_used = 0;
+    set3(m->base_color, 1.0f);
+    m->roughness = 0.3f;
+    m->oren_nayar_sigma = 0.34906585039886591538f;
+    m->metallic = 0.0f;
+    m->metallic_F90_falloff_exponent = 5.0f;
+    set3(m->metallic_F82, 1.0f);
+    set3(m->metallic_F90, 1.0f);
+    m->anisotropy = 0.0f;
+    m->anisotropy_rotation = 0.0f;
+    m->second_roughness_weight = 0.0f;
+    m->second_roughness = 0.5f;
+    m->do_metallic_energy_compensation = 1;
+    m->specular = 1.0f;
+    m->specular_tint = 1.0f;
+    set3(m->specular_color, 1.0f);
+    m->specular_darkening = 0.0f;
+    m->do_specular_energy_compensation = 1;
+    m->coat = 0.0f;
+    set3(m->coat_medium_absorption, 1.0f);
+    m->coat_medium_thickness = 5.0f;
+    m->coat_roughness = 0.0f;
+    m->coat_roughening = 1.0f;
+    m->coat_darkening = 1.0f;
+    m->coat_anisotropy = 0.0f;
+    m->coat_anisotropy_rotation = 0.0f;
+    m->coat_ior = 1.5f;
+    m->do_coat_energy_compensation = 1;
+    m->sheen = 0.0f;
+    m->sheen_roughness = 0.5f;
+    set3(m->sheen_color, 1.0f);
+    m->ior = 1.40f;
+    m->specular_transmission = 0.0f;
+    m->diffuse_transmission = 0.0f;
+    m->absorption_at_distance = 1.0f;
+    set3(m->absorption_color, 1.0f);
+    m->dispersion_scale = 0.0f;
+    m->dispersion_abbe_number = 20.0f;
+    m->thin_walled = 0;
+    m->do_glass_energy_compensation = 1;
+    m->thin_film = 0.0f;
+    m->thin_film_ior = 1.3f;
+    m->thin_film_thickness = 500.0f;
+    m->thin_film_kappa_3 = 0.0f;
+    m->thin_film_hue_shift_degrees = 0.0f;
+    m->thin_film_base_ior_override = 1.0f;
+    m->thin_film_do_ior_override = 0;
+    m->alpha_opacity = 1.0f;
+    m->dielectric_priority = 0;
+}
+
+int mipt_create(int hip_device, mipt_ctx** out)
+{
+    if (!out)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    Ctx* ctx = nullptr;
+    try
+    {
+        ctx = new Ctx;
+    }
+    catch (const std::bad_alloc&)
+    {
+        return MIPT_ERR_OUT_OF_MEMORY;
+    }
+    ctx->device = hip_device;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || hip_device < 0 || hip_device >= count)
+    {
+        delete ctx;
+        return MIPT_ERR_HIP;
+    }
+    if (hipSetDevice(hip_device) != hipSuccess)
+    {
+        delete ctx;
+        return MIPT_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, hip_device) != hipSuccess)
+    {
+        delete ctx;
+        return MIPT_ERR_HIP;
+    }
+    ctx->cu_count = prop.multiProcessorCount;
+    *out = reinterpret_cast<mipt_ctx*>(ctx);
+    return MIPT_OK;
+}
+
+void mipt_destroy(mipt_ctx* c)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    delete ctx;
+}
+
+const char* mipt_last_error(const mipt_ctx* c)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int mipt_set_scene(mipt_ctx* c, const mipt_scene_desc* s)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!s || s->triangle_count < 0 || s->vertex_count < 0 || s->material_count < 0 ||
+            s->emissive_triangle_count < 0)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_scene: negative counts or null scene");
+        if (s->triangle_count > 0 && (!s->triangles_vertex_indices || !s->vertices_positions))
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_scene: missing geometry arrays");
+        if (s->triangle_count >= (1 << 27))
+            mipt::fail(MIPT_ERR_UNSUPPORTED, "mipt_set_scene: more than 2^27 triangles");
+        for (int64_t i = 0; i < 3 * (int64_t)s->triangle_count; i++)
+        {
+            int32_t v = s->triangles_vertex_indices[i];
+            if (v < 0 || v >= s->vertex_count)
+                mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_scene: vertex index out of range at " +
+                                                          std::to_string(i));
+        }
+        if (s->material_indices)
+            for (int32_t i = 0; i < s->triangle_count; i++)
+                if (s->material_indices[i] < 0 || s->material_indices[i] >= s->material_count)
+                    mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_scene: material index out of range");
+        for (int32_t i = 0; i < s->emissive_triangle_count; i++)
+            if (s->emissive_triangles_primitive_indices[i] < 0 ||
+                s->emissive_triangles_primitive_indices[i] >= s->triangle_count)
+                mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_scene: emissive index out of range");
+        ctx->scene_set = false;
+        ctx->bvh.valid = false;
+        mipt::upload(ctx->vertices, s->vertices_positions, 3 * (size_t)s->vertex_count);
+        mipt::upload(ctx->indices, s->triangles_vertex_indices, 3 * (size_t)s->triangle_count);
+        if (s->has_vertex_normals && s->vertex_normals)
+        {
+            mipt::upload(ctx->has_normals, s->has_vertex_normals, (size_t)s->vertex_count);
+            mipt::upload(ctx->vertex_normals, s->vertex_normals, 3 * (size_t)s->vertex_count);
+        }
+        else
+            ctx->has_normals.release(), ctx->vertex_normals.release();
+        if (s->material_indices)
+            mipt::upload(ctx->material_indices, s->material_indices, (size_t)s->triangle_count);
+        else
+            ctx->material_indices.release();
+        mipt::upload(ctx->emissive_prims, s->emissive_triangles_primitive_indices, (size_t)s->emissive_triangle_count);
+        ctx->materials_host.assign(s->materials, s->materials + (s->materials ? s->material_count : 0));
+        ctx->triangle_count = s->triangle_count;
+        ctx->vertex_count = s->vertex_count;
+        ctx->material_count = s->material_count;
+        ctx->emissive_count = s->emissive_triangle_count;
+        ctx->scene_set = true;
+    });
+}
+
+int mipt_build_bvh(mipt_ctx* c, int32_t builder)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->scene_set)
+            mipt::fail(MIPT_ERR_NO_SCENE, "mipt_build_bvh: no scene");
+        if (builder != MIPT_BVH_LBVH)
+            mipt::fail(MIPT_ERR_UNSUPPORTED, "mipt_build_bvh: unknown builder " + std::to_string(builder));
+        mipt::build_lbvh(*ctx);
+    });
+}
+
+int mipt_get_bvh_info(const mipt_ctx* c, mipt_bvh_info* out)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    if (!ctx || !out)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (!ctx->bvh.valid)
+        return MIPT_ERR_NO_BVH;
+    const mipt::Bvh& b = ctx->bvh;
+    out->builder = b.builder;
+    out->triangle_count = b.triangle_count;
+    out->node_count = b.node_count;
+    out->leaf_count = b.leaf_count;
+    out->max_depth = b.max_depth;
+    out->max_leaf_size = b.max_leaf_size;
+    out->node_bytes = (int64_t)b.node_count * (int64_t)sizeof(mipt::BvhNode);
+    out->triangle_bytes = (int64_t)b.triangle_count * (int64_t)sizeof(mipt::TriRecord);
+    out->build_ms = b.build_ms;
+    out->sah_cost = b.sah_cost;
+    return MIPT_OK;
+}
+
+int mipt_set_trace_config(mipt_ctx* c, int32_t block_size, int32_t lds_stack)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (block_size != 0 && block_size != 64 && block_size != 128 && block_size != 256)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (lds_stack != 0 && lds_stack != 8 && lds_stack != 16 && lds_stack != 24 && lds_stack != 32)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (block_size)
+        ctx->trace_cfg.block_size = block_size;
+    if (lds_stack)
+        ctx->trace_cfg.lds_stack = lds_stack;
+    return MIPT_OK;
+}
+
+int mipt_trace_closest(mipt_ctx* c, const mipt_ray* d_rays, mipt_hit* d_hits, uint32_t n, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_rays || !d_hits)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::trace_closest(*ctx, d_rays, d_hits, nullptr, n, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_trace_closest_counted(mipt_ctx* c, const mipt_ray* d_rays, mipt_hit* d_hits, uint32_t* d_counts, uint32_t n,
+                               void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_rays || !d_hits || !d_counts)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::trace_closest(*ctx, d_rays, d_hits, d_counts, n, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_trace_any(mipt_ctx* c, const mipt_ray* d_rays, uint8_t* d_occluded, uint32_t n, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_rays || !d_occluded)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::trace_any(*ctx, d_rays, d_occluded, n, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_camera_rays(mipt_ctx* c, const mipt_camera* cam, int32_t w, int32_t h, uint32_t sample_number,
+                     uint32_t random_number, int32_t freeze_random, mipt_ray* d_rays, uint32_t* d_rng_state,
+                     void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !cam || w <= 0 || h <= 0 || !d_rays)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::camera_rays(*ctx, *cam, w, h, sample_number, random_number, freeze_random, d_rays, d_rng_state,
+                          reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_bounce_rays(mipt_ctx* c, const mipt_ray* d_primary, const mipt_hit* d_hits, uint32_t* d_rng_state,
+                     mipt_ray* d_out, uint32_t n, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_primary || !d_hits || !d_rng_state || !d_out)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::bounce_rays(*ctx, d_primary, d_hits, d_rng_state, d_out, n, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+uint32_t mipt_host_random_number(uint32_t pass_index)
+{
+    // Xorshift32 seeded with 42 (src/HostDeviceCommon/Xorshift.h:22-25), one
+    // draw per pass (src/Renderer/GPURenderer.cpp:750-751).
+    uint32_t x = 42u;
+    for (uint32_t i = 0; i <= pass_index; i++)
+    {
+        x ^= x << 13;
+        x ^= x >> 17;
+        x ^= x << 5;
+    }
+    return x;
+}
+
+} // extern "C"

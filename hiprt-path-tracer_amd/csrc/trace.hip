@@ -1,0 +1,326 @@
+// trace.hip -- closest-hit and any-hit BVH traversal for gfx950.
+//
+// Replaces hiprtGeomTraversalClosest / hiprtGeomTraversalAnyHit as used by
+// trace_main_path_ray, evaluate_shadow_ray_occluded and
+// evaluate_bsdf_light_sample_ray (src/Device/includes/Intersect.h:200-324,
+// :544-643). CDNA4 has no ray-tracing units, so this is a plain stackful
+// traversal:
+//   * one 64-byte BvhNode fetch per step tests both child boxes;
+//   * leaf children are intersected immediately (Moller-Trumbore, exact
+//     reference arithmetic), interior children are visited near-first and the
+//     far one pushed;
+//   * the per-lane stack lives in LDS, laid out [depth][lane] so the 64 lanes
+//     of a wave touch 64 consecutive dwords (no bank conflicts); entries
+//     deeper than the LDS stack spill to a per-thread global array;
+//   * a persistent grid sized from occupancy; each wave pulls batches of 64
+//     rays from a global counter so that long rays do not stall a whole block.
+//
+// Results are exact: the box test is widened (boxes padded at build time and
+// the slab interval relatively widened by 2^-16) so it never rejects a node
+// holding the closest Moller-Trumbore hit, and ties in t go to the smaller
+// primitive index. Hit IDs and t are therefore bit-identical to a brute-force
+// loop over all triangles with the same arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <utility>
+
+#include "internal.h"
+#include "pt_math.h"
+
+namespace mipt {
+namespace {
+
+enum Mode { CLOSEST = 0, ANY = 1, CLOSEST_COUNTED = 2 };
+
+constexpr float WIDEN_LO = 1.0f - 1.52587890625e-05f; // 1 - 2^-16
+constexpr float WIDEN_HI = 1.0f + 1.52587890625e-05f;
+
+__device__ __forceinline__ float safe_inv(float d)
+{
+    const float tiny = 1.0e-20f;
+    return 1.0f / (fabsf(d) > tiny ? d : copysignf(tiny, d));
+}
+
+// Slab test of one box; returns the widened entry distance or +inf on a miss.
+__device__ __forceinline__ float box_enter(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                           pt::f3 inv, pt::f3 oinv, float tfar_limit)
+{
+    float tx0 = __builtin_fmaf(lox, inv.x, -oinv.x), tx1 = __builtin_fmaf(hix, inv.x, -oinv.x);
+    float ty0 = __builtin_fmaf(loy, inv.y, -oinv.y), ty1 = __builtin_fmaf(hiy, inv.y, -oinv.y);
+    float tz0 = __builtin_fmaf(loz, inv.z, -oinv.z), tz1 = __builtin_fmaf(hiz, inv.z, -oinv.z);
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tfar_limit));
+    return (tn * WIDEN_LO <= tf * WIDEN_HI) ? tn : INFINITY;
+}
+
+__global__ void k_reset_counter(uint32_t* c) { c[0] = 0u; }
+
+template <int BLOCK, int STACK, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nodes, const TriRecord* __restrict__ tris,
+                                                 int32_t root, const float4* __restrict__ rays, int4* __restrict__ hits,
+                                                 uint8_t* __restrict__ occluded, uint2* __restrict__ counts, uint32_t n,
+                                                 int32_t* __restrict__ spill, uint32_t spill_stride,
+                                                 uint32_t* __restrict__ work_counter, uint32_t max_steps)
+{
+    __shared__ int32_t stack[STACK * BLOCK];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+
+    for (;;)
+    {
+        uint32_t base = 0;
+        if (lane == 0)
+            base = atomicAdd(work_counter, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n)
+            break;
+        const uint32_t r = base + lane;
+        if (r >= n)
+            continue;
+
+        const float4 ra = rays[2 * (size_t)r];
+        const float4 rb = rays[2 * (size_t)r + 1];
+        const pt::f3 o = pt::mk(ra.x, ra.y, ra.z);
+        const pt::f3 d = pt::mk(rb.x, rb.y, rb.z);
+        const int32_t last_prim = __float_as_int(ra.w);
+        const pt::f3 inv = pt::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+        const pt::f3 oinv = pt::mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+
+        float best_t = rb.w;
+        int32_t best_prim = -1;
+        float best_u = 0.0f, best_v = 0.0f;
+        uint32_t n_nodes = 0, n_tris = 0;
+        bool done = false;
+
+        int32_t node = root;
+        int sp = 0;
+        // A tree traversal visits each interior node at most once; the cap
+        // only guards against a malformed node array.
+        for (uint32_t step = 0; !done && step < max_steps; step++)
+        {
+            const BvhNode* np = nodes + node;
+            const float4 a = np->a, b = np->b, c = np->c;
+            const int4 refs = np->d;
+            if (MODE == CLOSEST_COUNTED)
+                n_nodes++;
+            const float t0 = box_enter(a.x, a.y, a.z, a.w, b.x, b.y, inv, oinv, best_t);
+            const float t1 = box_enter(b.z, b.w, c.x, c.y, c.z, c.w, inv, oinv, best_t);
+            const int32_t ch[2] = {refs.x, refs.y};
+            const float tc[2] = {t0, t1};
+
+            // Leaf children first.
+            for (int s = 0; s < 2 && !done; s++)
+            {
+                if (ch[s] >= 0 || tc[s] == INFINITY)
+                    continue;
+                const int32_t first = leaf_first(ch[s]);
+                const int32_t cnt = leaf_count(ch[s]);
+                for (int32_t k = 0; k < cnt; k++)
+                {
+                    const TriRecord* tr = tris + first + k;
+                    const float4 v0 = tr->v0, e1 = tr->e1, e2 = tr->e2;
+                    const int32_t prim = __float_as_int(v0.w);
+                    if (MODE == CLOSEST_COUNTED)
+                        n_tris++;
+                    float t, u, v;
+                    if (!pt::mt_intersect(pt::mk(v0.x, v0.y, v0.z), pt::mk(e1.x, e1.y, e1.z),
+                                          pt::mk(e2.x, e2.y, e2.z), o, d, t, u, v))
+                        continue;
+                    if (prim == last_prim)
+                        continue;
+                    if (t < best_t || (t == best_t && prim < best_prim))
+                    {
+                        best_t = t;
+                        best_prim = prim;
+                        best_u = u;
+                        best_v = v;
+                        if (MODE == ANY)
+                        {
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (done)
+                break;
+
+            // Interior children: near first, far pushed.
+            const bool h0 = ch[0] >= 0 && t0 != INFINITY;
+            const bool h1 = ch[1] >= 0 && t1 != INFINITY;
+            int32_t next = -1;
+            if (h0 && h1)
+            {
+                const bool first0 = t0 <= t1;
+                next = first0 ? ch[0] : ch[1];
+                const int32_t far = first0 ? ch[1] : ch[0];
+                if (sp < STACK)
+                    stack[sp * BLOCK + threadIdx.x] = far;
+                else
+                    spill[(size_t)(sp - STACK) * spill_stride + gtid] = far;
+                sp++;
+            }
+            else if (h0)
+                next = ch[0];
+            else if (h1)
+                next = ch[1];
+
+            if (next < 0)
+            {
+                if (sp == 0)
+                    break;
+                sp--;
+                next = sp < STACK ? stack[sp * BLOCK + threadIdx.x]
+                                  : spill[(size_t)(sp - STACK) * spill_stride + gtid];
+            }
+            node = next;
+        }
+
+        if (MODE == ANY)
+        {
+            occluded[r] = best_prim >= 0 ? 1 : 0;
+        }
+        else
+        {
+            int4 h;
+            if (best_prim >= 0)
+                h = make_int4(best_prim, __float_as_int(best_t), __float_as_int(best_u), __float_as_int(best_v));
+            else
+                h = make_int4(-1, __float_as_int(-1.0f), 0, 0);
+            hits[r] = h;
+            if (MODE == CLOSEST_COUNTED)
+                counts[r] = make_uint2(n_nodes, n_tris);
+        }
+    }
+}
+
+__global__ void k_all_miss(int4* __restrict__ hits, uint8_t* __restrict__ occluded, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    if (hits)
+        hits[i] = make_int4(-1, __float_as_int(-1.0f), 0, 0);
+    if (occluded)
+        occluded[i] = 0;
+}
+
+using TraceFn = void (*)(const BvhNode*, const TriRecord*, int32_t, const float4*, int4*, uint8_t*, uint2*,
+                         uint32_t, int32_t*, uint32_t, uint32_t*, uint32_t);
+
+template <int BLOCK, int STACK>
+TraceFn pick_mode(int mode)
+{
+    switch (mode)
+    {
+    case CLOSEST: return k_trace<BLOCK, STACK, CLOSEST>;
+    case ANY: return k_trace<BLOCK, STACK, ANY>;
+    default: return k_trace<BLOCK, STACK, CLOSEST_COUNTED>;
+    }
+}
+
+template <int BLOCK>
+TraceFn pick_stack(int stack, int mode)
+{
+    switch (stack)
+    {
+    case 8: return pick_mode<BLOCK, 8>(mode);
+    case 16: return pick_mode<BLOCK, 16>(mode);
+    case 24: return pick_mode<BLOCK, 24>(mode);
+    default: return pick_mode<BLOCK, 32>(mode);
+    }
+}
+
+TraceFn pick(int block, int stack, int mode)
+{
+    switch (block)
+    {
+    case 64: return pick_stack<64>(stack, mode);
+    case 128: return pick_stack<128>(stack, mode);
+    default: return pick_stack<256>(stack, mode);
+    }
+}
+
+struct Launch {
+    TraceFn fn;
+    unsigned grid;
+};
+
+Launch plan(Ctx& ctx, int mode, uint32_t n)
+{
+    const int block = ctx.trace_cfg.block_size;
+    const int stack = ctx.trace_cfg.lds_stack;
+    TraceFn fn = pick(block, stack, mode);
+    static std::map<std::pair<TraceFn, int>, int> occupancy;
+    auto key = std::make_pair(fn, ctx.device);
+    auto it = occupancy.find(key);
+    int per_cu = 0;
+    if (it == occupancy.end())
+    {
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), block, 0));
+        if (per_cu < 1)
+            per_cu = 1;
+        occupancy[key] = per_cu;
+    }
+    else
+        per_cu = it->second;
+    const uint64_t max_grid = (uint64_t)per_cu * (uint64_t)ctx.cu_count;
+    const uint64_t need = ((uint64_t)n + block - 1) / block;
+    Launch l;
+    l.fn = fn;
+    l.grid = (unsigned)(need < max_grid ? need : max_grid);
+    if (l.grid == 0)
+        l.grid = 1;
+    return l;
+}
+
+void launch(Ctx& ctx, int mode, const mipt_ray* rays, mipt_hit* hits, uint8_t* occluded, uint32_t* counts, uint32_t n,
+            hipStream_t stream)
+{
+    if (n == 0)
+        return;
+    if (!ctx.bvh.valid)
+        fail(MIPT_ERR_NO_BVH, "no BVH: call mipt_build_bvh first");
+    if (ctx.bvh.triangle_count == 0)
+    {
+        hipLaunchKernelGGL(k_all_miss, dim3((n + 255) / 256), dim3(256), 0, stream, reinterpret_cast<int4*>(hits),
+                           occluded, n);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const Launch l = plan(ctx, mode, n);
+    const int block = ctx.trace_cfg.block_size;
+    const uint32_t threads = l.grid * (uint32_t)block;
+    // Spill entries: the stack never holds more than max_depth entries.
+    const int spill_depth = ctx.bvh.max_depth - ctx.trace_cfg.lds_stack;
+    size_t spill_bytes = sizeof(uint32_t); // work counter lives at the front
+    size_t spill_off = 256;
+    if (spill_depth > 0)
+        spill_bytes = spill_off + sizeof(int32_t) * (size_t)spill_depth * threads;
+    else
+        spill_bytes = spill_off;
+    ctx.spill.reserve(spill_bytes);
+    uint32_t* counter = ctx.spill.as<uint32_t>();
+    int32_t* spill = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ctx.spill.ptr) + spill_off);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, counter);
+    hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(block), 0, stream, ctx.bvh.nodes.as<BvhNode>(),
+                       ctx.bvh.tris.as<TriRecord>(), ctx.bvh.root, reinterpret_cast<const float4*>(rays),
+                       reinterpret_cast<int4*>(hits), occluded, reinterpret_cast<uint2*>(counts), n, spill, threads,
+                       counter, (uint32_t)ctx.bvh.node_count + 1u);
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace
+
+void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n, hipStream_t stream)
+{
+    launch(ctx, counts ? CLOSEST_COUNTED : CLOSEST, rays, hits, nullptr, counts, n, stream);
+}
+
+void trace_any(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, uint32_t n, hipStream_t stream)
+{
+    launch(ctx, ANY, rays, nullptr, occluded, nullptr, n, stream);
+}
+
+} // namespace mipt

@@ -1,0 +1,335 @@
+"""ctypes binding of libmipt.so (include/mipt.h) for tests and bench.py.
+
+This module is plumbing: it loads the C-ABI library, mirrors its structs and
+moves arrays between numpy and device memory (torch tensors on the HIP
+device). All compute happens in libmipt's HIP kernels; if the library is not
+built, loading fails loudly -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PACKAGE_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PACKAGE_DIR, "libmipt.so")
+
+MIPT_OK = 0
+MIPT_BVH_LBVH = 0
+
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("last_prim", "<i4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+
+_F3 = C.c_float * 3
+
+
+class Camera(C.Structure):
+    """mipt_camera = HIPRTCamera (src/HostDeviceCommon/HIPRTCamera.h:16-27)."""
+
+    _fields_ = [
+        ("inverse_view", (C.c_float * 4) * 4),
+        ("inverse_projection", (C.c_float * 4) * 4),
+        ("view_projection", (C.c_float * 4) * 4),
+        ("position", _F3),
+        ("vertical_fov", C.c_float),
+        ("sensor_width", C.c_int32),
+        ("sensor_height", C.c_int32),
+        ("do_jittering", C.c_int32),
+    ]
+
+
+_MATERIAL_FIELDS = [
+    ("emission", _F3), ("emission_strength", C.c_float), ("global_emissive_factor", C.c_float),
+    ("emissive_texture_used", C.c_int32), ("base_color", _F3), ("roughness", C.c_float),
+    ("oren_nayar_sigma", C.c_float), ("metallic", C.c_float), ("metallic_F90_falloff_exponent", C.c_float),
+    ("metallic_F82", _F3), ("metallic_F90", _F3), ("anisotropy", C.c_float), ("anisotropy_rotation", C.c_float),
+    ("second_roughness_weight", C.c_float), ("second_roughness", C.c_float),
+    ("do_metallic_energy_compensation", C.c_int32), ("specular", C.c_float), ("specular_tint", C.c_float),
+    ("specular_color", _F3), ("specular_darkening", C.c_float), ("do_specular_energy_compensation", C.c_int32),
+    ("coat", C.c_float), ("coat_medium_absorption", _F3), ("coat_medium_thickness", C.c_float),
+    ("coat_roughness", C.c_float), ("coat_roughening", C.c_float), ("coat_darkening", C.c_float),
+    ("coat_anisotropy", C.c_float), ("coat_anisotropy_rotation", C.c_float), ("coat_ior", C.c_float),
+    ("do_coat_energy_compensation", C.c_int32), ("sheen", C.c_float), ("sheen_roughness", C.c_float),
+    ("sheen_color", _F3), ("ior", C.c_float), ("specular_transmission", C.c_float),
+    ("diffuse_transmission", C.c_float), ("absorption_at_distance", C.c_float), ("absorption_color", _F3),
+    ("dispersion_scale", C.c_float), ("dispersion_abbe_number", C.c_float), ("thin_walled", C.c_int32),
+    ("do_glass_energy_compensation", C.c_int32), ("thin_film", C.c_float), ("thin_film_ior", C.c_float),
+    ("thin_film_thickness", C.c_float), ("thin_film_kappa_3", C.c_float),
+    ("thin_film_hue_shift_degrees", C.c_float), ("thin_film_base_ior_override", C.c_float),
+    ("thin_film_do_ior_override", C.c_int32), ("alpha_opacity", C.c_float), ("dielectric_priority", C.c_int32),
+]
+
+
+class Material(C.Structure):
+    """mipt_material = CPUMaterial (src/HostDeviceCommon/Material/MaterialCPU.h:213-330)."""
+
+    _fields_ = _MATERIAL_FIELDS
+
+
+class SceneDesc(C.Structure):
+    """mipt_scene_desc = struct Scene (src/Scene/SceneParser.h:109-170)."""
+
+    _fields_ = [
+        ("triangles_vertex_indices", C.POINTER(C.c_int32)),
+        ("triangle_count", C.c_int32),
+        ("vertices_positions", C.POINTER(C.c_float)),
+        ("vertex_count", C.c_int32),
+        ("has_vertex_normals", C.POINTER(C.c_uint8)),
+        ("vertex_normals", C.POINTER(C.c_float)),
+        ("material_indices", C.POINTER(C.c_int32)),
+        ("materials", C.POINTER(Material)),
+        ("material_count", C.c_int32),
+        ("emissive_triangles_primitive_indices", C.POINTER(C.c_int32)),
+        ("emissive_triangle_count", C.c_int32),
+    ]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [
+        ("builder", C.c_int32), ("triangle_count", C.c_int32), ("node_count", C.c_int32),
+        ("leaf_count", C.c_int32), ("max_depth", C.c_int32), ("max_leaf_size", C.c_int32),
+        ("node_bytes", C.c_int64), ("triangle_bytes", C.c_int64), ("build_ms", C.c_double),
+        ("sah_cost", C.c_double),
+    ]
+
+
+class GeneratedScene(C.Structure):
+    _fields_ = [("desc", SceneDesc), ("camera", Camera), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+# Every extern "C" symbol of include/mipt.h with its ctypes signature.
+_SIGNATURES = {
+    "mipt_version": (C.c_char_p, []),
+    "mipt_material_default": (None, [C.POINTER(Material)]),
+    "mipt_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mipt_destroy": (None, [C.c_void_p]),
+    "mipt_last_error": (C.c_char_p, [C.c_void_p]),
+    "mipt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "mipt_build_bvh": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mipt_get_bvh_info": (C.c_int, [C.c_void_p, C.POINTER(BvhInfo)]),
+    "mipt_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mipt_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mipt_trace_closest_counted": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                             C.c_void_p]),
+    "mipt_set_trace_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "mipt_camera_rays": (C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
+                                   C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mipt_bounce_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.c_void_p]),
+    "mipt_scene_generate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.POINTER(GeneratedScene))]),
+    "mipt_scene_free": (None, [C.POINTER(GeneratedScene)]),
+    "mipt_host_random_number": (C.c_uint32, [C.c_uint32]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmipt.so; raises if it is not built (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libmipt.so not built at {path}: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def host_random_number(pass_index: int) -> int:
+    return int(load_library().mipt_host_random_number(pass_index))
+
+
+def default_material() -> Material:
+    m = Material()
+    load_library().mipt_material_default(C.byref(m))
+    return m
+
+
+class MiptError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mipt error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# Procedural scenes
+# ---------------------------------------------------------------------------
+class Scene:
+    """Host arrays of a scene (numpy), in the layout of struct Scene."""
+
+    def __init__(self, vertices, indices, material_indices=None, materials=None, emissive=None,
+                 camera=None, width=0, height=0):
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+        self.material_indices = None if material_indices is None else np.ascontiguousarray(material_indices,
+                                                                                              dtype=np.int32)
+        self.materials = materials
+        self.emissive = np.zeros(0, np.int32) if emissive is None else np.ascontiguousarray(emissive, np.int32)
+        self.camera = camera
+        self.width = width
+        self.height = height
+
+    @property
+    def triangle_count(self):
+        return int(self.indices.shape[0])
+
+    def desc(self) -> SceneDesc:
+        d = SceneDesc()
+        d.triangles_vertex_indices = self.indices.ctypes.data_as(C.POINTER(C.c_int32))
+        d.triangle_count = self.indices.shape[0]
+        d.vertices_positions = self.vertices.ctypes.data_as(C.POINTER(C.c_float))
+        d.vertex_count = self.vertices.shape[0]
+        d.has_vertex_normals = None
+        d.vertex_normals = None
+        if self.material_indices is not None and self.materials is not None:
+            d.material_indices = self.material_indices.ctypes.data_as(C.POINTER(C.c_int32))
+            d.materials = C.cast(self.materials, C.POINTER(Material))
+            d.material_count = len(self.materials)
+        d.emissive_triangles_primitive_indices = self.emissive.ctypes.data_as(C.POINTER(C.c_int32))
+        d.emissive_triangle_count = self.emissive.shape[0]
+        return d
+
+
+def generate_scene(config_id: int, width: int, height: int) -> Scene:
+    """Procedural scene of configuration 1 or 2, generated by libmipt."""
+    lib = load_library()
+    out = C.POINTER(GeneratedScene)()
+    rc = lib.mipt_scene_generate(config_id, width, height, C.byref(out))
+    if rc != MIPT_OK:
+        raise MiptError(rc, f"mipt_scene_generate({config_id})")
+    try:
+        g = out.contents
+        d = g.desc
+        T, V, M, E = d.triangle_count, d.vertex_count, d.material_count, d.emissive_triangle_count
+        idx = np.ctypeslib.as_array(d.triangles_vertex_indices, shape=(3 * T,)).copy()
+        verts = np.ctypeslib.as_array(d.vertices_positions, shape=(3 * V,)).copy()
+        mat_idx = np.ctypeslib.as_array(d.material_indices, shape=(T,)).copy()
+        mats = (Material * M)()
+        C.memmove(mats, d.materials, C.sizeof(Material) * M)
+        emissive = np.ctypeslib.as_array(d.emissive_triangles_primitive_indices, shape=(E,)).copy() if E else None
+        cam = Camera()
+        C.memmove(C.byref(cam), C.byref(g.camera), C.sizeof(Camera))
+        return Scene(verts, idx, mat_idx, mats, emissive, cam, g.width, g.height)
+    finally:
+        lib.mipt_scene_free(out)
+
+
+# ---------------------------------------------------------------------------
+# Device memory helpers (torch is only used as the HIP allocator)
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+
+    return torch
+
+
+def to_device(arr: np.ndarray):
+    torch = _torch()
+    raw = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+    return torch.from_numpy(raw.copy()).to("cuda")
+
+
+def empty_device(nbytes: int):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device="cuda")
+
+
+def from_device(t, dtype) -> np.ndarray:
+    return t.cpu().numpy().view(dtype)
+
+
+def ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+class Context:
+    """One mipt context on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.mipt_create(device, C.byref(h))
+        if rc != MIPT_OK:
+            raise MiptError(rc, "mipt_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mipt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != MIPT_OK:
+            msg = self.lib.mipt_last_error(self.h)
+            raise MiptError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_scene(self, scene: Scene):
+        d = scene.desc()
+        self._check(self.lib.mipt_set_scene(self.h, C.byref(d)), "mipt_set_scene")
+
+    def build_bvh(self, builder: int = MIPT_BVH_LBVH):
+        self._check(self.lib.mipt_build_bvh(self.h, builder), "mipt_build_bvh")
+
+    def bvh_info(self) -> BvhInfo:
+        info = BvhInfo()
+        self._check(self.lib.mipt_get_bvh_info(self.h, C.byref(info)), "mipt_get_bvh_info")
+        return info
+
+    def set_trace_config(self, block_size: int = 0, lds_stack: int = 0):
+        self._check(self.lib.mipt_set_trace_config(self.h, block_size, lds_stack), "mipt_set_trace_config")
+
+    def trace_closest(self, d_rays, d_hits, n: int, stream=None):
+        self._check(self.lib.mipt_trace_closest(self.h, ptr(d_rays), ptr(d_hits), n, stream),
+                    "mipt_trace_closest")
+
+    def trace_closest_counted(self, d_rays, d_hits, d_counts, n: int, stream=None):
+        self._check(self.lib.mipt_trace_closest_counted(self.h, ptr(d_rays), ptr(d_hits), ptr(d_counts), n, stream),
+                    "mipt_trace_closest_counted")
+
+    def trace_any(self, d_rays, d_occ, n: int, stream=None):
+        self._check(self.lib.mipt_trace_any(self.h, ptr(d_rays), ptr(d_occ), n, stream), "mipt_trace_any")
+
+    def camera_rays(self, cam: Camera, w: int, h: int, sample_number: int, random_number: int, freeze: int,
+                    d_rays, d_rng, stream=None):
+        self._check(self.lib.mipt_camera_rays(self.h, C.byref(cam), w, h, sample_number, random_number, freeze,
+                                              ptr(d_rays), ptr(d_rng) if d_rng is not None else None, stream),
+                    "mipt_camera_rays")
+
+    def bounce_rays(self, d_primary, d_hits, d_rng, d_out, n: int, stream=None):
+        self._check(self.lib.mipt_bounce_rays(self.h, ptr(d_primary), ptr(d_hits), ptr(d_rng), ptr(d_out), n,
+                                              stream), "mipt_bounce_rays")
+
+    # Convenience wrappers taking and returning host arrays (not for timing).
+    def trace_closest_host(self, rays: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        d_rays = to_device(rays)
+        d_hits = empty_device(16 * n)
+        self.trace_closest(d_rays, d_hits, n)
+        _torch().cuda.synchronize()
+        return from_device(d_hits, HIT_DTYPE)[:n]
+
+    def trace_any_host(self, rays: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        d_rays = to_device(rays)
+        d_occ = empty_device(n)
+        self.trace_any(d_rays, d_occ, n)
+        _torch().cuda.synchronize()
+        return from_device(d_occ, np.uint8)[:n]

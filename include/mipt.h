@@ -1,0 +1,251 @@
+/*
+ * mipt.h -- C ABI of the MI355X path-tracing core (libmipt.so).
+ *
+ * This is the drop-in boundary for the GPU kernel path of HIPRT-Path-Tracer.
+ * Every entry point names the reference interface it replaces (paths relative
+ * to the reference repository root). Plain C types only: host pointers are
+ * copied, device pointers are caller-owned HIP allocations, `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Errors: every int-returning call returns MIPT_OK (0) or a negative
+ * MIPT_ERR_* code; mipt_last_error() gives the message. No exception crosses
+ * the ABI. A context is not thread-safe; work is serialised on the stream
+ * passed in.
+ */
+#ifndef MIPT_H
+#define MIPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIPT_OK 0
+#define MIPT_ERR_INVALID_ARGUMENT -1
+#define MIPT_ERR_HIP -2
+#define MIPT_ERR_NO_SCENE -3
+#define MIPT_ERR_NO_BVH -4
+#define MIPT_ERR_OUT_OF_MEMORY -5
+#define MIPT_ERR_UNSUPPORTED -6
+
+/* ------------------------------------------------------------------ */
+/* Ray queries                                                          */
+/* ------------------------------------------------------------------ */
+
+/* 32-byte ray. Replaces hiprtRay {origin, minT, direction, maxT} plus the
+ * FilterFunctionPayload::last_hit_primitive_index self-intersection filter
+ * (src/Device/includes/Intersect.h:38-56, src/Device/functions/FilterFunction.h:37-45).
+ * minT is always 0 in the reference; `tmax` is exclusive: a hit needs t < tmax. */
+typedef struct mipt_ray {
+    float o[3];
+    int32_t last_prim; /* primitive the ray leaves, -1 for none */
+    float d[3];
+    float tmax;        /* +inf for path rays; t_max - 1e-4 for shadow rays (Intersect.h:285) */
+} mipt_ray;
+
+/* 16-byte hit. Replaces hiprtHit {primID, t, uv} (normal is recomputed from
+ * the triangle, src/Renderer/Triangle.h:54). Miss = {-1, -1.0f, 0, 0}. */
+typedef struct mipt_hit {
+    int32_t prim;
+    float t;
+    float u;
+    float v;
+} mipt_hit;
+
+/* ------------------------------------------------------------------ */
+/* Camera: field-for-field mirror of HIPRTCamera                        */
+/* (src/HostDeviceCommon/HIPRTCamera.h:16-27), matrices row-major.      */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_camera {
+    float inverse_view[4][4];
+    float inverse_projection[4][4];
+    float view_projection[4][4];
+    float position[3];
+    float vertical_fov;
+    int32_t sensor_width;
+    int32_t sensor_height;
+    int32_t do_jittering;
+} mipt_camera;
+
+/* ------------------------------------------------------------------ */
+/* Materials: mirror of CPUMaterial (src/HostDeviceCommon/Material/     */
+/* MaterialCPU.h:213-330). Same names, same defaults (see               */
+/* mipt_material_default). Packed on upload the way                     */
+/* CPUMaterial::pack_to_GPU (MaterialCPU.h:31-150) packs it.            */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_material {
+    float emission[3];
+    float emission_strength;
+    float global_emissive_factor;
+    int32_t emissive_texture_used;
+    float base_color[3];
+    float roughness;
+    float oren_nayar_sigma;
+    float metallic;
+    float metallic_F90_falloff_exponent;
+    float metallic_F82[3];
+    float metallic_F90[3];
+    float anisotropy;
+    float anisotropy_rotation;
+    float second_roughness_weight;
+    float second_roughness;
+    int32_t do_metallic_energy_compensation;
+    float specular;
+    float specular_tint;
+    float specular_color[3];
+    float specular_darkening;
+    int32_t do_specular_energy_compensation;
+    float coat;
+    float coat_medium_absorption[3];
+    float coat_medium_thickness;
+    float coat_roughness;
+    float coat_roughening;
+    float coat_darkening;
+    float coat_anisotropy;
+    float coat_anisotropy_rotation;
+    float coat_ior;
+    int32_t do_coat_energy_compensation;
+    float sheen;
+    float sheen_roughness;
+    float sheen_color[3];
+    float ior;
+    float specular_transmission;
+    float diffuse_transmission;
+    float absorption_at_distance;
+    float absorption_color[3];
+    float dispersion_scale;
+    float dispersion_abbe_number;
+    int32_t thin_walled;
+    int32_t do_glass_energy_compensation;
+    float thin_film;
+    float thin_film_ior;
+    float thin_film_thickness;
+    float thin_film_kappa_3;
+    float thin_film_hue_shift_degrees;
+    float thin_film_base_ior_override;
+    int32_t thin_film_do_ior_override;
+    float alpha_opacity;
+    int32_t dielectric_priority;
+} mipt_material;
+
+/* Fills `m` with the CPUMaterial defaults (MaterialCPU.h:213-330). */
+void mipt_material_default(mipt_material* m);
+
+/* ------------------------------------------------------------------ */
+/* Scene upload: mirror of struct Scene (src/Scene/SceneParser.h:109-170)*/
+/* All pointers are HOST arrays, copied by mipt_set_scene.              */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_scene_desc {
+    const int32_t* triangles_vertex_indices;    /* int[3*triangle_count] */
+    int32_t triangle_count;
+    const float* vertices_positions;            /* float[3*vertex_count] */
+    int32_t vertex_count;
+    const uint8_t* has_vertex_normals;          /* uint8[vertex_count] or NULL */
+    const float* vertex_normals;                /* float[3*vertex_count] or NULL */
+    const int32_t* material_indices;            /* int[triangle_count] or NULL */
+    const mipt_material* materials;             /* [material_count] or NULL */
+    int32_t material_count;
+    const int32_t* emissive_triangles_primitive_indices; /* int[emissive_triangle_count] */
+    int32_t emissive_triangle_count;
+} mipt_scene_desc;
+
+/* BVH builders (replace hiprtBuildGeometry, src/HIPRT-Orochi/HIPRTScene.h:87-150). */
+#define MIPT_BVH_LBVH 0
+
+typedef struct mipt_bvh_info {
+    int32_t builder;
+    int32_t triangle_count;
+    int32_t node_count;        /* 64-byte nodes written */
+    int32_t leaf_count;
+    int32_t max_depth;         /* deepest leaf, root = 0 */
+    int32_t max_leaf_size;
+    int64_t node_bytes;
+    int64_t triangle_bytes;
+    double build_ms;           /* device time of the build kernels */
+    double sah_cost;           /* SAH cost (C_node = 1, C_tri = 1), relative to root area */
+} mipt_bvh_info;
+
+/* ------------------------------------------------------------------ */
+/* Context                                                              */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_ctx mipt_ctx;
+
+/* Replaces HIPRTOrochiCtx::init (src/HIPRT-Orochi/HIPRTOrochiCtx.h:40-106). */
+int mipt_create(int hip_device, mipt_ctx** out);
+void mipt_destroy(mipt_ctx* ctx);
+const char* mipt_last_error(const mipt_ctx* ctx);
+const char* mipt_version(void);
+
+/* Replaces GPURenderer::set_scene (src/Renderer/GPURenderer.cpp:952-960 ->
+ * set_hiprt_scene_from_scene :822-940): uploads geometry, materials
+ * (packed), emissive list. Does not build the BVH. */
+int mipt_set_scene(mipt_ctx* ctx, const mipt_scene_desc* scene);
+
+/* Replaces HIPRTGeometry::build_bvh (HIPRTScene.h:87-150). Synchronous. */
+int mipt_build_bvh(mipt_ctx* ctx, int32_t builder);
+int mipt_get_bvh_info(const mipt_ctx* ctx, mipt_bvh_info* out);
+
+/* Closest hit over the scene BVH: replaces hiprtGeomTraversalClosest
+ * (used by trace_main_path_ray, src/Device/includes/Intersect.h:200-270, and
+ * evaluate_bsdf_light_sample_ray :544-643). Ties in t go to the smaller
+ * primitive index. d_rays/d_hits are device pointers of n records. */
+int mipt_trace_closest(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* d_hits, uint32_t n,
+                       void* stream);
+
+/* Any hit: replaces hiprtGeomTraversalAnyHit in evaluate_shadow_ray_occluded
+ * (Intersect.h:276-324). d_occluded[i] = 1 if some hit has t < tmax. */
+int mipt_trace_any(mipt_ctx* ctx, const mipt_ray* d_rays, uint8_t* d_occluded, uint32_t n,
+                   void* stream);
+
+/* Instrumented closest hit: same result as mipt_trace_closest and, per ray,
+ * d_counts[2*i] = 64-byte nodes fetched, d_counts[2*i+1] = triangles tested.
+ * Used for the bytes-per-ray accounting; slower than the plain kernel. */
+int mipt_trace_closest_counted(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* d_hits,
+                               uint32_t* d_counts, uint32_t n, void* stream);
+
+/* Traversal launch configuration (the autotuned knobs). block_size in
+ * {64,128,256}; lds_stack in {8,16,24,32}. 0 keeps the current value. */
+int mipt_set_trace_config(mipt_ctx* ctx, int32_t block_size, int32_t lds_stack);
+
+/* ------------------------------------------------------------------ */
+/* Camera rays (the ray-generation half of CameraRays,                  */
+/* src/Device/kernels/CameraRays.h:155-172 + HIPRTCamera.h:31-52).       */
+/* Writes w*h rays, pix = x + y*w, y = 0 the bottom row, and the        */
+/* per-pixel xorshift32 state after the jitter draws.                   */
+/* ------------------------------------------------------------------ */
+int mipt_camera_rays(mipt_ctx* ctx, const mipt_camera* cam, int32_t w, int32_t h,
+                     uint32_t sample_number, uint32_t random_number, int32_t freeze_random,
+                     mipt_ray* d_rays, uint32_t* d_rng_state, void* stream);
+
+/* Incoherent bounce rays: from each primary hit, one cosine-weighted
+ * direction about the (back-face fixed) shading normal
+ * (src/Device/includes/Sampling.h:180-201, Intersect.h:142-176), drawn from
+ * d_rng_state (updated). Misses copy the primary ray. */
+int mipt_bounce_rays(mipt_ctx* ctx, const mipt_ray* d_primary, const mipt_hit* d_hits,
+                     uint32_t* d_rng_state, mipt_ray* d_out, uint32_t n, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Procedural scenes (replace the ASSIMP SceneParser, which is out of   */
+/* scope). config 1: Cornell box + 20 icospheres (102,428 triangles);  */
+/* config 2: fBm terrain + 400 spheres + 4096 emitters (1,040,384).     */
+/* Seeded PCG32 (state 0x2026120200000000 + config, stream = config).  */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_generated_scene {
+    mipt_scene_desc desc;   /* pointers owned by the generated scene */
+    mipt_camera camera;     /* camera for (width, height) */
+    int32_t width, height;
+} mipt_generated_scene;
+
+int mipt_scene_generate(int32_t config_id, int32_t width, int32_t height, mipt_generated_scene** out);
+void mipt_scene_free(mipt_generated_scene* scene);
+
+/* Host xorshift32 seeded with 42, one draw per pass: the reference's
+ * per-pass random_number (src/Renderer/GPURenderer.cpp:750-751). */
+uint32_t mipt_host_random_number(uint32_t pass_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

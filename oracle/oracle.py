@@ -1,0 +1,191 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU checker.
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product. Arrays use the same
+32-byte ray / 16-byte hit records as libmipt.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("last_prim", "<i4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("inverse_view", (C.c_float * 4) * 4),
+        ("inverse_projection", (C.c_float * 4) * 4),
+        ("view_projection", (C.c_float * 4) * 4),
+        ("position", C.c_float * 3),
+        ("vertical_fov", C.c_float),
+        ("sensor_width", C.c_int32),
+        ("sensor_height", C.c_int32),
+        ("do_jittering", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_SIG = {
+    "orc_wang_hash": (C.c_uint32, [C.c_uint32]),
+    "orc_xorshift_fill": (None, [C.c_uint32, _P, C.c_int32]),
+    "orc_rand01_fill": (None, [C.c_uint32, _P, C.c_int32]),
+    "orc_random_index": (C.c_int32, [C.POINTER(C.c_uint32), C.c_int32]),
+    "orc_pixel_seed": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_mt_intersect": (C.c_int32, [_P, _P, _P, _P, _P, _P, _P]),
+    "orc_trace_brute": (None, [_P, _P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32]),
+    "orc_octree_build": (C.c_void_p, [_P, _P, C.c_int32, C.c_int32, C.c_int32]),
+    "orc_octree_free": (None, [C.c_void_p]),
+    "orc_octree_trace": (None, [C.c_void_p, _P, _P, C.c_int32, C.c_int32, C.c_int32]),
+    "orc_octree_stats": (None, [C.c_void_p, _P]),
+    "orc_camera_ray": (None, [_P, C.c_float, C.c_float, C.c_int32, C.c_int32, _P]),
+    "orc_camera_rays": (None, [_P, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32, _P, _P]),
+    "orc_hit_attributes": (None, [_P, _P, _P, _P, _P, _P, _P]),
+    "orc_bounce_rays": (None, [_P, _P, _P, _P, _P, C.c_int32, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built at {LIB_PATH}: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def wang_hash(x: int) -> int:
+    return int(lib().orc_wang_hash(x))
+
+
+def xorshift_seq(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().orc_xorshift_fill(seed, _p(out), n)
+    return out
+
+
+def rand01_seq(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().orc_rand01_fill(seed, _p(out), n)
+    return out
+
+
+def random_index(state: int, n: int):
+    s = C.c_uint32(state)
+    r = lib().orc_random_index(C.byref(s), n)
+    return int(r), int(s.value)
+
+
+def pixel_seed(pix: int, sample: int, random_number: int, freeze: int = 0) -> int:
+    return int(lib().orc_pixel_seed(pix, sample, random_number, freeze))
+
+
+def mt_intersect(a, b, c, ray: np.ndarray):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    c = np.ascontiguousarray(c, np.float32)
+    r = np.ascontiguousarray(ray).reshape(1).astype(RAY_DTYPE)
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = lib().orc_mt_intersect(_p(a), _p(b), _p(c), _p(r), C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+def trace_brute(vertices, indices, rays, any_hit=False, nthreads=0):
+    verts = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(indices, np.int32).reshape(-1)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    lib().orc_trace_brute(_p(verts), _p(idx), idx.size // 3, _p(rays), _p(hits), rays.shape[0], int(any_hit),
+                          nthreads)
+    return hits
+
+
+class Octree:
+    """The reference CPU renderer's octree of k-DOPs (max depth 32, <= 8 tris/leaf)."""
+
+    def __init__(self, vertices, indices, max_depth=32, leaf_max=8):
+        self.verts = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+        self.idx = np.ascontiguousarray(indices, np.int32).reshape(-1)
+        self.h = lib().orc_octree_build(_p(self.verts), _p(self.idx), self.idx.size // 3, max_depth, leaf_max)
+
+    def trace(self, rays, any_hit=False, nthreads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(rays.shape[0], HIT_DTYPE)
+        lib().orc_octree_trace(self.h, _p(rays), _p(hits), rays.shape[0], int(any_hit), nthreads)
+        return hits
+
+    def stats(self):
+        s = np.zeros(4, np.int64)
+        lib().orc_octree_stats(self.h, _p(s))
+        return dict(nodes=int(s[0]), leaves=int(s[1]), max_depth=int(s[2]), tri_refs=int(s[3]))
+
+    def close(self):
+        if self.h:
+            lib().orc_octree_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _cam(camera) -> Camera:
+    c = Camera()
+    C.memmove(C.byref(c), C.byref(camera), C.sizeof(Camera))
+    return c
+
+
+def camera_ray(camera, x, y, resx, resy):
+    c = _cam(camera)
+    out = np.zeros(1, RAY_DTYPE)
+    lib().orc_camera_ray(C.byref(c), x, y, resx, resy, _p(out))
+    return out[0]
+
+
+def camera_rays(camera, w, h, sample_number, random_number, freeze=0):
+    c = _cam(camera)
+    rays = np.zeros(w * h, RAY_DTYPE)
+    rng = np.zeros(w * h, np.uint32)
+    lib().orc_camera_rays(C.byref(c), w, h, sample_number, random_number, freeze, _p(rays), _p(rng))
+    return rays, rng
+
+
+def hit_attributes(vertices, indices, ray, hit):
+    verts = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(indices, np.int32).reshape(-1)
+    r = np.ascontiguousarray(np.asarray(ray).reshape(1), RAY_DTYPE)
+    h = np.ascontiguousarray(np.asarray(hit).reshape(1), HIT_DTYPE)
+    p, gn, sn = (np.zeros(3, np.float32) for _ in range(3))
+    lib().orc_hit_attributes(_p(verts), _p(idx), _p(r), _p(h), _p(p), _p(gn), _p(sn))
+    return p, gn, sn
+
+
+def bounce_rays(vertices, indices, primary, hits, rng_state):
+    verts = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(indices, np.int32).reshape(-1)
+    primary = np.ascontiguousarray(primary, RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    rng = np.ascontiguousarray(rng_state, np.uint32).copy()
+    out = np.zeros(primary.shape[0], RAY_DTYPE)
+    lib().orc_bounce_rays(_p(verts), _p(idx), _p(primary), _p(hits), _p(rng), primary.shape[0], _p(out))
+    return out, rng
