@@ -80,6 +80,18 @@ template <class T> void upload(DeviceBuffer& buf, const T* host, size_t count)
 
 using mipt::Ctx;
 
+namespace {
+// Message of the last failure that had no context to hold it (mipt_create).
+thread_local std::string g_no_ctx_error;
+
+int create_fail(Ctx* ctx, const char* what, hipError_t e)
+{
+    g_no_ctx_error = std::string(what) + ": " + hipGetErrorString(e);
+    delete ctx;
+    return e == hipErrorOutOfMemory ? MIPT_ERR_OUT_OF_MEMORY : MIPT_ERR_HIP;
+}
+} // namespace
+
 extern "C" {
 
 const char* mipt_version(void) { return "mipt 0.1 (gfx950)"; }
@@ -161,23 +173,22 @@ int mipt_create(int hip_device, mipt_ctx** out)
     }
     ctx->device = hip_device;
     int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || hip_device < 0 || hip_device >= count)
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess)
+        return create_fail(ctx, "hipGetDeviceCount", e);
+    if (hip_device < 0 || hip_device >= count)
     {
+        g_no_ctx_error = "device " + std::to_string(hip_device) + " out of range (" + std::to_string(count) +
+                         " devices)";
         delete ctx;
-        return MIPT_ERR_HIP;
+        return MIPT_ERR_INVALID_ARGUMENT;
     }
-    if (hipSetDevice(hip_device) != hipSuccess)
-    {
-        delete ctx;
-        return MIPT_ERR_HIP;
-    }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, hip_device) != hipSuccess)
-    {
-        delete ctx;
-        return MIPT_ERR_HIP;
-    }
-    ctx->cu_count = prop.multiProcessorCount;
+    if ((e = hipSetDevice(hip_device)) != hipSuccess)
+        return create_fail(ctx, "hipSetDevice", e);
+    int cus = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hip_device)) != hipSuccess)
+        return create_fail(ctx, "hipDeviceGetAttribute(MultiprocessorCount)", e);
+    ctx->cu_count = cus;
     *out = reinterpret_cast<mipt_ctx*>(ctx);
     return MIPT_OK;
 }
@@ -195,7 +206,7 @@ void mipt_destroy(mipt_ctx* c)
 const char* mipt_last_error(const mipt_ctx* c)
 {
     const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
-    return ctx ? ctx->last_error.c_str() : "null context";
+    return ctx ? ctx->last_error.c_str() : g_no_ctx_error.c_str();
 }
 
 int mipt_set_scene(mipt_ctx* c, const mipt_scene_desc* s)

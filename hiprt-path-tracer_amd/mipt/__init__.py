@@ -134,6 +134,13 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"libmipt.so not built at {path}: run __graft_entry__.build()")
+    # torch ships its own libamdhip64.so.7. Load it first so that libmipt's
+    # NEEDED entry binds to that same runtime: two HIP runtimes in one process
+    # leave the second one without a usable device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
@@ -262,7 +269,8 @@ class Context:
         h = C.c_void_p()
         rc = self.lib.mipt_create(device, C.byref(h))
         if rc != MIPT_OK:
-            raise MiptError(rc, "mipt_create failed")
+            msg = self.lib.mipt_last_error(None)
+            raise MiptError(rc, f"mipt_create failed: {msg.decode() if msg else ''}")
         self.h = h
 
     def close(self):

@@ -175,6 +175,8 @@ typedef struct mipt_ctx mipt_ctx;
 /* Replaces HIPRTOrochiCtx::init (src/HIPRT-Orochi/HIPRTOrochiCtx.h:40-106). */
 int mipt_create(int hip_device, mipt_ctx** out);
 void mipt_destroy(mipt_ctx* ctx);
+/* Message of the context's last failure; with ctx == NULL, of the calling
+ * thread's last mipt_create failure. */
 const char* mipt_last_error(const mipt_ctx* ctx);
 const char* mipt_version(void);
 
