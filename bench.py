@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -54,6 +56,17 @@ def main():
     import mipt
 
     assert torch.cuda.is_available(), "bench.py needs a HIP GPU"
+    if args.calibrate_copy:
+        nbytes = 1 << 30
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
+        return
     W, H = 1920, 1080
     n = W * H
     scene = mipt.generate_scene(2, W, H)
@@ -121,6 +134,15 @@ def main():
                          f"(max depth 32, <=8 tris/leaf), OpenMP dynamic over rays; {cpu_s:.2f} s; "
                          f"hit-ID agreement with GPU {agree:.6f}"}
 
+    # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # (scaled by a plain copy), recorded in profiles/ for this kernel and
+    # launch configuration; null when the configuration differs.
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "trace_closest_config2_rinc.json")
+    if os.path.exists(prof) and not (args.block or args.stack):
+        with open(prof) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -139,7 +161,8 @@ def main():
                    "block": args.block or 128,
                    "lds_stack": args.stack or 16},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/trace_closest_config2_rinc.md" if traffic else None,
                      "bytes_per_ray": round(b_ray, 1), "mean_nodes_per_ray": round(mean_nodes, 2),
                      "mean_tris_per_ray": round(mean_tris, 2), "mean_launch_ms": round(mean_launch_s * 1e3, 4)},
         "cpu_baseline": cpu,
