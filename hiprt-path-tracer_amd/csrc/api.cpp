@@ -75,6 +75,78 @@ template <class T> void upload(DeviceBuffer& buf, const T* host, size_t count)
         HIP_CHECK(hipMemcpy(buf.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice));
 }
 
+// Scene data the passes need beyond the traversal: unpacked materials,
+// triangle areas (ThreadFunctions.cpp:221-242: |cross(AB, AC)| * 0.5) and the
+// energy-compensation LUT.
+void prepare_scene_for_render(Ctx& ctx)
+{
+    RenderState& R = ctx.render;
+    if (R.scene_ready)
+        return;
+    if (!ctx.scene_set)
+        fail(MIPT_ERR_NO_SCENE, "no scene: call mipt_set_scene first");
+    if (!ctx.bvh.valid)
+        fail(MIPT_ERR_NO_BVH, "no BVH: call mipt_build_bvh first");
+    if (ctx.triangle_count > 0 && (ctx.materials_host.empty() || !ctx.material_indices.ptr))
+        fail(MIPT_ERR_INVALID_ARGUMENT, "the path-tracing passes need materials and material indices");
+    std::vector<float> mats(32 * (ctx.materials_host.size() > 0 ? ctx.materials_host.size() : 1), 0.0f);
+    for (size_t i = 0; i < ctx.materials_host.size(); i++)
+    {
+        std::string why = unpack_material(ctx.materials_host[i], mats.data() + 32 * i);
+        if (!why.empty())
+            fail(MIPT_ERR_UNSUPPORTED, "material " + std::to_string(i) + " uses the " + why +
+                                           ", which the path-tracing passes do not implement yet");
+    }
+    upload(R.materials, mats.data(), mats.size());
+    std::vector<float> areas(ctx.host_vertices.size() ? ctx.triangle_count : 0);
+    for (int32_t t = 0; t < (int32_t)areas.size(); t++)
+    {
+        const float* v[3];
+        for (int k = 0; k < 3; k++)
+            v[k] = ctx.host_vertices.data() + 3 * (size_t)ctx.host_indices[3 * (size_t)t + k];
+        float ab[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+        float ac[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+        float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+        areas[t] = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) * 0.5f;
+    }
+    upload(R.areas, areas.data(), areas.size());
+    upload(R.lut_conductor, conductor_lut_data(), 128 * 128);
+    R.scene_ready = true;
+}
+
+void allocate_render_buffers(Ctx& ctx)
+{
+    RenderState& R = ctx.render;
+    const size_t n = (size_t)R.settings.width * (size_t)R.settings.height;
+    R.pixels = (uint32_t)n;
+    R.gb_prim.reserve(4 * n);
+    R.gb_pos.reserve(16 * n);
+    R.gb_gn.reserve(4 * n);
+    R.gb_sn.reserve(4 * n);
+    R.gb_mat.reserve(4 * n);
+    for (DeviceBuffer* b : {&R.throughput, &R.thr_prev, &R.radiance, &R.vtx_pos, &R.vtx_sn, &R.vtx_gn, &R.vtx_view,
+                            &R.nee_light, &R.nee_bsdf})
+        b->reserve(16 * n);
+    R.rng.reserve(4 * n);
+    R.flags.reserve(4 * n);
+    R.shadow_rays.reserve(32 * n);
+    R.shadow_occ.reserve(n);
+    R.bsdf_rays.reserve(32 * n);
+    R.bsdf_hits.reserve(16 * n);
+    R.cont_rays.reserve(32 * n);
+    R.cont_hits.reserve(16 * n);
+    R.fb.reserve(12 * n);
+    HIP_CHECK(hipMemset(R.fb.ptr, 0, 12 * n));
+    if (R.settings.accumulate_fp64)
+    {
+        R.fb64.reserve(24 * n);
+        HIP_CHECK(hipMemset(R.fb64.ptr, 0, 24 * n));
+    }
+    R.counters.reserve(64);
+    HIP_CHECK(hipMemset(R.counters.ptr, 0, 64));
+    HIP_CHECK(hipMemset(R.gb_prim.ptr, 0xff, 4 * n));
+}
+
 } // namespace
 } // namespace mipt
 
@@ -254,6 +326,10 @@ int mipt_set_scene(mipt_ctx* c, const mipt_scene_desc* s)
             ctx->material_indices.release();
         mipt::upload(ctx->emissive_prims, s->emissive_triangles_primitive_indices, (size_t)s->emissive_triangle_count);
         ctx->materials_host.assign(s->materials, s->materials + (s->materials ? s->material_count : 0));
+        ctx->host_vertices.assign(s->vertices_positions, s->vertices_positions + 3 * (size_t)s->vertex_count);
+        ctx->host_indices.assign(s->triangles_vertex_indices,
+                                 s->triangles_vertex_indices + 3 * (size_t)s->triangle_count);
+        ctx->render.scene_ready = false;
         ctx->triangle_count = s->triangle_count;
         ctx->vertex_count = s->vertex_count;
         ctx->material_count = s->material_count;
@@ -365,6 +441,141 @@ int mipt_bounce_rays(mipt_ctx* c, const mipt_ray* d_primary, const mipt_hit* d_h
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         mipt::bounce_rays(*ctx, d_primary, d_hits, d_rng_state, d_out, n, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+void mipt_render_settings_default(mipt_render_settings* s)
+{
+    if (!s)
+        return;
+    std::memset(s, 0, sizeof(*s));
+    s->width = 512;
+    s->height = 512;
+    s->nb_bounces = 4;
+    s->do_russian_roulette = 1;
+    // RenderSettings.h:107 evaluates min(5, nb_bounces / 2) while nb_bounces
+    // is still 0, so Russian roulette starts at bounce 0.
+    s->russian_roulette_min_depth = 0;
+    s->russian_roulette_throughput_clamp = 10.0f;
+    s->freeze_random = 0;
+    s->enable_direct = 1;
+    s->microfacet_regularization_tau_0 = 30.0f;
+    s->microfacet_regularization_min_roughness = 0.0f;
+    s->do_microfacet_regularization = 1;
+    s->accumulate_fp64 = 1;
+}
+
+int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !s || s->width <= 0 || s->height <= 0 || s->nb_bounces < 0 ||
+        (int64_t)s->width * (int64_t)s->height > (1ll << 28))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        ctx->render.settings = *s;
+        ctx->render.configured = false;
+        mipt::prepare_scene_for_render(*ctx);
+        mipt::allocate_render_buffers(*ctx);
+        ctx->render.configured = true;
+    });
+}
+
+int mipt_set_camera(mipt_ctx* c, const mipt_camera* cam)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !cam)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    ctx->render.camera = *cam;
+    ctx->render.camera_set = true;
+    return MIPT_OK;
+}
+
+int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t random_number, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (pass != MIPT_PASS_CAMERA_RAYS && pass != MIPT_PASS_PATH_TRACE))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->render.configured)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_launch_pass: call mipt_set_render_settings first");
+        if (!ctx->render.camera_set)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_launch_pass: call mipt_set_camera first");
+        mipt::prepare_scene_for_render(*ctx);
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (pass == MIPT_PASS_CAMERA_RAYS)
+            mipt::camera_rays_pass(*ctx, sample_number, random_number, s);
+        else
+            mipt::path_trace_pass(*ctx, sample_number, random_number, s);
+    });
+}
+
+int mipt_render_sample(mipt_ctx* c, uint32_t sample_number, void* stream)
+{
+    int rc = mipt_launch_pass(c, MIPT_PASS_CAMERA_RAYS, sample_number, mipt_host_random_number(2 * sample_number),
+                              stream);
+    if (rc != MIPT_OK)
+        return rc;
+    return mipt_launch_pass(c, MIPT_PASS_PATH_TRACE, sample_number, mipt_host_random_number(2 * sample_number + 1),
+                            stream);
+}
+
+int mipt_read_framebuffer(mipt_ctx* c, float* host_rgb, double* host_rgb64)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->render.configured)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no framebuffer: call mipt_set_render_settings first");
+        HIP_CHECK(hipDeviceSynchronize());
+        const size_t n = ctx->render.pixels;
+        if (host_rgb)
+            HIP_CHECK(hipMemcpy(host_rgb, ctx->render.fb.ptr, 12 * n, hipMemcpyDeviceToHost));
+        if (host_rgb64)
+        {
+            if (!ctx->render.settings.accumulate_fp64)
+                mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "FP64 framebuffer requested but accumulate_fp64 is off");
+            HIP_CHECK(hipMemcpy(host_rgb64, ctx->render.fb64.ptr, 24 * n, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int mipt_framebuffer_device(mipt_ctx* c, float** d_rgb, double** d_rgb64)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !ctx->render.configured)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (d_rgb)
+        *d_rgb = ctx->render.fb.as<float>();
+    if (d_rgb64)
+        *d_rgb64 = ctx->render.settings.accumulate_fp64 ? ctx->render.fb64.as<double>() : nullptr;
+    return MIPT_OK;
+}
+
+int mipt_read_gbuffer_prims(mipt_ctx* c, int32_t* host_prims)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !host_prims)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->render.configured)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no G-buffer: call mipt_set_render_settings first");
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(host_prims, ctx->render.gb_prim.ptr, 4 * (size_t)ctx->render.pixels,
+                            hipMemcpyDeviceToHost));
+    });
+}
+
+int mipt_rejected_samples(mipt_ctx* c, uint32_t* out_count)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !out_count)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->render.configured)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no render state");
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out_count, ctx->render.counters.ptr, 4, hipMemcpyDeviceToHost));
     });
 }
 

@@ -58,6 +58,28 @@ struct Bvh {
     DeviceBuffer tris;            // TriRecord[triangle_count] in leaf order
 };
 
+// Per-resolution render buffers and settings of the path-tracing passes.
+struct RenderState {
+    bool configured = false;
+    bool camera_set = false;
+    mipt_render_settings settings{};
+    mipt_camera camera{};
+    uint32_t pixels = 0;
+    // scene-derived
+    DeviceBuffer materials;     // pt::Material[material_count]
+    DeviceBuffer areas;         // float[T]
+    DeviceBuffer lut_conductor; // float[128*128]
+    bool scene_ready = false;
+    // G-buffer (CameraRays output)
+    DeviceBuffer gb_prim, gb_pos, gb_gn, gb_sn, gb_mat;
+    // path state
+    DeviceBuffer throughput, thr_prev, radiance, vtx_pos, vtx_sn, vtx_gn, vtx_view, nee_light, nee_bsdf, rng, flags;
+    // rays of one bounce
+    DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
+    // framebuffer
+    DeviceBuffer fb, fb64, counters;
+};
+
 struct Ctx {
     int device = 0;
     int cu_count = 0;
@@ -75,6 +97,8 @@ struct Ctx {
     DeviceBuffer material_indices;// int[T]
     DeviceBuffer emissive_prims;  // int[E]
     std::vector<mipt_material> materials_host;
+    std::vector<float> host_vertices;   // host copies of the geometry
+    std::vector<int32_t> host_indices;
     bool scene_set = false;
 
     Bvh bvh;
@@ -82,7 +106,19 @@ struct Ctx {
 
     // Traversal spill stack (entries beyond the LDS stack), sized on demand.
     DeviceBuffer spill;
+
+    RenderState render;
 };
+
+// materials.cpp: CPUMaterial -> the 8-bit packed form -> unpacked floats
+// (pt::Material layout, 32 floats); returns an empty string or the reason a
+// material uses a lobe this build does not implement.
+std::string unpack_material(const mipt_material& in, float out[32]);
+const float* conductor_lut_data(); // 128 x 128, luts.cpp
+
+// pathtrace.hip
+void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
 
 // bvh_build.hip
 void build_lbvh(Ctx& ctx);

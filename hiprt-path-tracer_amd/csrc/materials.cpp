@@ -1,0 +1,75 @@
+// materials.cpp -- CPUMaterial -> packed GPU material -> unpacked floats.
+//
+// The reference uploads every material in an 8-bit packed form
+// (CPUMaterial::pack_to_GPU, src/HostDeviceCommon/Material/MaterialCPU.h:31-150;
+// packers in src/HostDeviceCommon/Packing.h:56-203) and every kernel reads
+// the unpacked values. The round trip is done once here, on the host, with
+// the same binary32 arithmetic: truncation u8(x * 255) on the way in;
+// u8 * (1/255) on the way out, except the green channel of a 3x8-bit colour,
+// which the reference unpacks as float(g << 8) * (1 / (255 << 8)).
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/mipt.h"
+
+namespace mipt {
+namespace {
+
+const float INV_255 = 1.0f / 255.0f;
+const float INV_255_SHL_8 = 1.0f / (float)(255 << 8);
+
+uint8_t q8(float x) { return static_cast<unsigned char>(x * 255.0f); }
+float u8f(uint8_t u) { return static_cast<float>(u) * INV_255; }
+void color3(const float in[3], float out[3])
+{
+    uint32_t r = q8(in[0]), g = q8(in[1]), b = q8(in[2]);
+    out[0] = static_cast<float>(r) * INV_255;
+    out[1] = static_cast<float>(g << 8) * INV_255_SHL_8;
+    out[2] = static_cast<float>(b) * INV_255;
+}
+
+} // namespace
+
+// Output layout = pt::Material (csrc/shading.h), 32 floats.
+std::string unpack_material(const mipt_material& m, float o[32])
+{
+    std::memset(o, 0, 32 * sizeof(float));
+    // emission is stored at full precision, pre-multiplied by its strength
+    // and the global factor (MaterialCPU.h:64)
+    for (int k = 0; k < 3; k++)
+        o[k] = m.emission[k] * m.emission_strength * m.global_emissive_factor;
+    color3(m.base_color, o + 3);
+    o[6] = u8f(q8(m.roughness));
+    o[7] = u8f(q8(m.metallic));
+    color3(m.metallic_F82, o + 8);
+    color3(m.metallic_F90, o + 11);
+    o[14] = m.metallic_F90_falloff_exponent;
+    o[15] = u8f(q8(m.anisotropy));
+    o[16] = u8f(q8(m.anisotropy_rotation));
+    o[17] = u8f(q8(m.second_roughness_weight));
+    o[18] = u8f(q8(m.second_roughness));
+    o[19] = u8f(q8(m.specular));
+    o[20] = u8f(q8(m.coat));
+    o[21] = u8f(q8(m.sheen));
+    o[22] = u8f(q8(m.specular_transmission));
+    o[23] = u8f(q8(m.diffuse_transmission));
+    o[24] = u8f(q8(m.thin_film));
+    o[25] = m.ior;
+    uint32_t flags = (m.do_metallic_energy_compensation ? 1u : 0u) | (m.do_specular_energy_compensation ? 2u : 0u) |
+                     (m.do_coat_energy_compensation ? 4u : 0u) | (m.do_glass_energy_compensation ? 8u : 0u);
+    std::memcpy(&o[26], &flags, 4);
+
+    // Lobes the passes implement: the two GGX metal lobes and the Lambertian
+    // diffuse lobe. Anything with weight on another lobe is refused.
+    const float metallic = o[7];
+    if (o[20] > 0.0f) return "coat lobe";
+    if (o[21] > 0.0f) return "sheen lobe";
+    if (o[22] > 0.0f) return "glass (specular transmission) lobe";
+    if (o[23] > 0.0f) return "diffuse transmission lobe";
+    if (o[24] > 0.0f) return "thin-film Fresnel";
+    if ((1.0f - metallic) * o[19] > 0.0f) return "dielectric specular lobe";
+    return "";
+}
+
+} // namespace mipt

@@ -1,0 +1,604 @@
+// pathtrace.hip -- the per-pass launch surface of the reference renderer on
+// the wavefront design:
+//
+//   CAMERA_RAYS  (CameraRays, src/Device/kernels/CameraRays.h:83-213)
+//       k_camera_rays -> closest-hit traversal -> k_gbuffer_write
+//   PATH_TRACE   (MegaKernel, src/Device/kernels/Megakernel.h:23-144)
+//       k_path_init, then per bounce:
+//         k_shade   : NEE light sample + MIS BSDF sample + continuation sample
+//                     and Russian roulette (all RNG draws, in the reference's order)
+//         traversal : any hit (shadow rays), closest hit (BSDF-to-light rays,
+//                     continuation rays)
+//         k_resolve : combines the ray results into the path radiance
+//                     (Lights.h:123-226, :364-423) and builds the next vertex
+//                     (Intersect.h:222-245)
+//       k_accumulate: framebuffer sum (PathTracing.h:176-217) + FP64 mirror
+//
+// Each path carries its xorshift32 state between kernels, so the per-path
+// RNG sequence is the megakernel's. Rays of a bounce live in fixed per-path
+// slots; an inactive slot has tmax = -1 and leaves the traversal at the root.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+#include "shading.h"
+
+namespace mipt {
+namespace {
+
+using pt::Color;
+using pt::f3;
+
+constexpr uint32_t FLAG_ALIVE = 1u;      // path is at a surface vertex
+constexpr uint32_t FLAG_NEE_LIGHT = 2u;  // shadow ray pending with a contribution
+constexpr uint32_t FLAG_NEE_BSDF = 4u;   // BSDF-to-light ray pending
+constexpr uint32_t FLAG_CONTINUE = 8u;   // continuation ray traced this bounce
+constexpr uint32_t FLAG_DIRECT = 16u;    // direct lighting estimated this bounce
+
+struct PtData {
+    // scene
+    const float* verts;
+    const int32_t* idx;
+    const int32_t* mat_idx;
+    const pt::Material* mats;
+    const int32_t* emissive;
+    const float* areas;
+    int32_t emissive_count;
+    // camera
+    float cam_pos[3];
+    // settings
+    int32_t width, height;
+    int32_t nb_bounces;
+    int32_t do_rr, rr_min_depth;
+    float rr_clamp;
+    int32_t freeze, enable_direct;
+    uint32_t sample_number;
+    uint32_t random_number;
+    pt::BsdfGlobals g;
+    // G-buffer
+    int32_t* gb_prim;
+    float4* gb_pos;
+    uint32_t* gb_gn;
+    uint32_t* gb_sn;
+    int32_t* gb_mat;
+    // path state
+    float4* throughput;   // xyz
+    float4* radiance;     // xyz = ray_color
+    float4* vtx_pos;      // xyz = inter point, w = accumulated roughness
+    float4* vtx_sn;       // xyz = shading normal, w = int bits of prim
+    float4* vtx_gn;       // xyz = geometric normal, w = int bits of material
+    float4* vtx_view;     // xyz = view direction (towards the viewer)
+    float4* nee_light;    // xyz = pending light-sample contribution
+    float4* nee_bsdf;     // xyz = bsdf_color * cos, w = bsdf pdf
+    uint32_t* rng;
+    uint32_t* flags;
+    // rays of the current bounce
+    float4* shadow_rays;
+    uint8_t* shadow_occ;
+    float4* bsdf_rays;
+    int4* bsdf_hits;
+    float4* cont_rays;
+    int4* cont_hits;
+    // output
+    float* fb;        // rgb sum
+    double* fb64;     // rgb sum (may be null)
+    uint32_t* rejected;
+};
+
+__device__ __forceinline__ f3 ld3(float4 v) { return pt::mk(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 st3(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ Color ldc(float4 v) { return pt::col(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 stc(Color c, float w) { return make_float4(c.r, c.g, c.b, w); }
+
+__device__ __forceinline__ void tri_verts(const PtData& d, int32_t prim, f3& a, f3& b, f3& c)
+{
+    const int32_t* t = d.idx + 3 * (int64_t)prim;
+    const float* v0 = d.verts + 3 * (int64_t)t[0];
+    const float* v1 = d.verts + 3 * (int64_t)t[1];
+    const float* v2 = d.verts + 3 * (int64_t)t[2];
+    a = pt::mk(v0[0], v0[1], v0[2]);
+    b = pt::mk(v1[0], v1[1], v1[2]);
+    c = pt::mk(v2[0], v2[1], v2[2]);
+}
+
+__device__ __forceinline__ void write_inactive(float4* rays, uint32_t i)
+{
+    rays[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    rays[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+}
+
+__device__ __forceinline__ void write_ray(float4* rays, uint32_t i, f3 o, int32_t last, f3 dir, float tmax)
+{
+    rays[2 * (size_t)i] = make_float4(o.x, o.y, o.z, __int_as_float(last));
+    rays[2 * (size_t)i + 1] = make_float4(dir.x, dir.y, dir.z, tmax);
+}
+
+// Intersect.h:142-176
+__device__ __forceinline__ void fix_backfacing(f3 view, f3& gn, f3& sn)
+{
+    if (pt::dot(view, gn) < 0.0f)
+    {
+        gn = -1.0f * gn;
+        sn = -1.0f * sn;
+    }
+    if (pt::dot(view, sn) < 0.0f)
+        sn = -1.0f * sn;
+    f3 refl = pt::reflect_ray(view, sn);
+    if (pt::dot(refl, gn) <= 0.0f)
+    {
+        const float eps = 0.01f;
+        refl = refl - pt::normalize((pt::dot(refl, gn) - eps) * gn);
+        sn = pt::normalize(view + refl);
+    }
+}
+
+// Hit attributes of trace_main_path_ray for meshes without vertex normals
+// (Intersect.h:222-245).
+__device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, int32_t prim, float t, f3& p, f3& gn,
+                                               f3& sn)
+{
+    f3 a, b, c;
+    tri_verts(d, prim, a, b, c);
+    p = o + t * dir;
+    gn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
+    sn = gn;
+    fix_backfacing(-1.0f * dir, gn, sn);
+}
+
+// ---------------------------------------------------------------------------
+// CAMERA_RAYS: G-buffer write after the closest hit (CameraRays.h:176-201).
+// ---------------------------------------------------------------------------
+__global__ void k_gbuffer_write(PtData d, const float4* __restrict__ rays, const int4* __restrict__ hits)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= (uint32_t)(d.width * d.height))
+        return;
+    const float4 ra = rays[2 * (size_t)pix], rb = rays[2 * (size_t)pix + 1];
+    const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
+    const int4 h = hits[pix];
+    if (h.x >= 0)
+    {
+        f3 p, gn, sn;
+        hit_attributes(d, o, dir, h.x, __int_as_float(h.y), p, gn, sn);
+        d.gb_gn[pix] = pt::oct_pack(gn);
+        d.gb_sn[pix] = pt::oct_pack(sn);
+        d.gb_mat[pix] = d.mat_idx[h.x];
+        d.gb_pos[pix] = st3(p, 0.0f);
+    }
+    else
+        d.gb_pos[pix] = st3(o + dir, 0.0f);
+    d.gb_prim[pix] = h.x >= 0 ? h.x : -1;
+}
+
+// ---------------------------------------------------------------------------
+// PATH_TRACE
+// ---------------------------------------------------------------------------
+
+// Megakernel.h:52-86: seed, first vertex from the G-buffer.
+__global__ void k_path_init(PtData d)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= (uint32_t)(d.width * d.height))
+        return;
+    d.rng[pix] = pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0);
+    d.throughput[pix] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    d.radiance[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int32_t prim = d.gb_prim[pix];
+    if (prim < 0)
+    {
+        d.flags[pix] = 0u; // camera ray missed: no envmap in this build's scenes
+        return;
+    }
+    const f3 p = ld3(d.gb_pos[pix]);
+    const f3 gn = pt::normalize(pt::oct_unpack(d.gb_gn[pix]));
+    const f3 sn = pt::normalize(pt::oct_unpack(d.gb_sn[pix]));
+    const f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
+    // GBufferDevice.h:19-22: view = normalize(camera - hit); the path ray is
+    // normalize(-view) and the BSDF view direction is -ray.direction.
+    const f3 ray_dir = pt::normalize(-1.0f * pt::normalize(cam - p));
+    d.vtx_pos[pix] = st3(p, 0.0f);
+    d.vtx_sn[pix] = st3(sn, __int_as_float(prim));
+    d.vtx_gn[pix] = st3(gn, __int_as_float(d.gb_mat[pix]));
+    d.vtx_view[pix] = st3(-1.0f * ray_dir, 0.0f);
+    d.flags[pix] = FLAG_ALIVE;
+}
+
+// Lights.h:123-226 (NEE with MIS, uniform emissive-triangle sampling) and
+// PathTracing.h:81-101 (next bounce + Russian roulette).
+__global__ void k_shade(PtData d, int32_t bounce)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= (uint32_t)(d.width * d.height))
+        return;
+    uint32_t flags = d.flags[pix];
+    if (!(flags & FLAG_ALIVE))
+    {
+        write_inactive(d.shadow_rays, pix);
+        write_inactive(d.bsdf_rays, pix);
+        write_inactive(d.cont_rays, pix);
+        return;
+    }
+    flags &= FLAG_ALIVE;
+    const float4 vp = d.vtx_pos[pix], vs = d.vtx_sn[pix], vg = d.vtx_gn[pix];
+    const f3 p = ld3(vp), sn = ld3(vs), gn = ld3(vg), view = ld3(d.vtx_view[pix]);
+    const int32_t prim = __float_as_int(vs.w);
+    const pt::Material m = d.mats[__float_as_int(vg.w)];
+    float path_roughness = vp.w;
+    pt::Xorshift32 rng{d.rng[pix]};
+
+    bool shadow_active = false, bsdf_active = false;
+    if (bounce > 0 || d.enable_direct)
+    {
+        flags |= FLAG_DIRECT;
+        d.nee_light[pix] = stc(pt::col(0.0f), 0.0f);
+    }
+    // Without emissive triangles sample_emissive_geometry returns before any
+    // draw (Lights.h:313-321): neither the light nor the MIS BSDF sample runs.
+    if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0)
+    {
+        // ---- light sample (LightUtils.h:248-277, :142-199)
+        Color light_mis = pt::col(0.0f);
+        {
+            const int32_t e = rng.random_index(d.emissive_count);
+            const int32_t lt = d.emissive[e];
+            f3 A, B, C;
+            tri_verts(d, lt, A, B, C);
+            float r1 = rng();
+            float r2 = rng();
+            if (r2 > r1)
+            {
+                r1 *= 0.5f;
+                r2 -= r1;
+            }
+            else
+            {
+                r2 *= 0.5f;
+                r1 -= r2;
+            }
+            const f3 AB = B - A, AC = C - A;
+            const f3 nrm = pt::cross(AB, AC);
+            const float len = sqrtf(pt::dot(nrm, nrm));
+            const f3 point = (A + r1 * AB) + r2 * AC;
+            const f3 lnormal = pt::vdiv(nrm, len);
+            const float area = 0.5f * len;
+            float area_pdf = 1.0f / area;
+            area_pdf /= (float)d.emissive_count;
+            const Color emission = d.mats[d.mat_idx[lt]].emission;
+            if (area_pdf > 0.0f)
+            {
+                const f3 to = point - p;
+                const float dist = sqrtf(pt::dot(to, to));
+                const f3 dirn = pt::vdiv(to, dist);
+                pt::BsdfCtx c{view, sn, gn, dirn, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness};
+                float bsdf_pdf;
+                const Color f = pt::principled_eval(d.g, m, c, bsdf_pdf);
+                if (bsdf_pdf > 0.0f)
+                {
+                    const float cos_l = pt::maxf_(0.0f, pt::dot(lnormal, -1.0f * dirn));
+                    if (cos_l > 1.0e-5f)
+                    {
+                        float lpdf = cos_l < 1.0e-8f ? 0.0f : area_pdf * (dist * dist) / cos_l;
+                        const float w = lpdf == 0.0f ? 0.0f : lpdf / (1.0f * lpdf + 1.0f * bsdf_pdf);
+                        const float cosine = fabsf(pt::dot(sn, dirn));
+                        light_mis = f * cosine * emission * w / lpdf / 1.0f;
+                        write_ray(d.shadow_rays, pix, p, prim, dirn, dist - 1.0e-4f);
+                        shadow_active = true;
+                        flags |= FLAG_NEE_LIGHT;
+                    }
+                }
+            }
+        }
+        d.nee_light[pix] = stc(light_mis, 0.0f);
+        // ---- BSDF sample for MIS (Lights.h:174-223)
+        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness};
+        f3 dir;
+        float bpdf;
+        const Color f = pt::principled_sample(d.g, m, c, dir, bpdf, rng);
+        if (bpdf > 0.0f)
+        {
+            const float cosine = fabsf(pt::dot(sn, dir));
+            d.nee_bsdf[pix] = stc(f * cosine, bpdf);
+            write_ray(d.bsdf_rays, pix, p, prim, dir, 1.0e35f - 1.0e-4f);
+            bsdf_active = true;
+            flags |= FLAG_NEE_BSDF;
+        }
+    }
+    if (!shadow_active)
+        write_inactive(d.shadow_rays, pix);
+    if (!bsdf_active)
+        write_inactive(d.bsdf_rays, pix);
+
+    // ---- continuation (PathTracing.h:26-101)
+    bool cont = false;
+    {
+        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, path_roughness};
+        f3 dir;
+        float bpdf;
+        const Color f = pt::principled_sample(d.g, m, c, dir, bpdf, rng);
+        // RayPayload::accumulate_roughness (RayPayload.h)
+        if (c.info == pt::FROM_DIFFUSE || c.info == pt::FROM_DIFFUSE_TRANSMISSION)
+            path_roughness = 1.0f;
+        else if (c.info == pt::FROM_FIRST_METAL || c.info == pt::FROM_SPECULAR)
+            path_roughness = pt::maxf_(m.roughness, path_roughness);
+        else if (c.info == pt::FROM_SECOND_METAL)
+            path_roughness = pt::maxf_(m.second_roughness, path_roughness);
+        if (bpdf > 0.0f)
+        {
+            Color thr = ldc(d.throughput[pix]);
+            const Color att = f * fabsf(pt::dot(dir, sn)) / bpdf;
+            bool survive = true;
+            if (bounce >= d.rr_min_depth && d.do_rr)
+            {
+                float sp = pt::max_component(thr);
+                sp = pt::minf_(sp, 1.0f);
+                if (rng() > sp)
+                    survive = false;
+                else
+                {
+                    float scale = 1.0f / sp;
+                    if (d.rr_clamp > 0.0f)
+                        scale = pt::minf_(scale, d.rr_clamp);
+                    thr = thr * scale;
+                }
+            }
+            if (survive)
+            {
+                thr = thr * pt::col(1.0f); // dispersion filter (no dispersive material)
+                thr = thr * att;
+                thr = pt::col(pt::maxf_(thr.r, 1.0e-5f), pt::maxf_(thr.g, 1.0e-5f), pt::maxf_(thr.b, 1.0e-5f));
+                if (!pt::is_black(thr) && bounce < d.nb_bounces)
+                {
+                    d.throughput[pix] = stc(thr, 0.0f);
+                    write_ray(d.cont_rays, pix, p, prim, dir, INFINITY);
+                    cont = true;
+                    flags |= FLAG_CONTINUE;
+                }
+            }
+        }
+    }
+    if (!cont)
+        write_inactive(d.cont_rays, pix);
+    // Keep the pre-update throughput for the direct-light estimate of this
+    // vertex: it was stored before the continuation changed it.
+    d.vtx_pos[pix] = st3(p, path_roughness);
+    d.rng[pix] = rng.state;
+    d.flags[pix] = flags;
+}
+
+__global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= (uint32_t)(d.width * d.height))
+        return;
+    const uint32_t flags = d.flags[pix];
+    if (!(flags & FLAG_ALIVE))
+        return;
+    Color radiance = ldc(d.radiance[pix]);
+    if (flags & FLAG_DIRECT)
+    {
+        // estimate_direct_lighting (Lights.h:364-398)
+        Color light_mis = pt::col(0.0f);
+        if ((flags & FLAG_NEE_LIGHT) && !d.shadow_occ[pix])
+            light_mis = ldc(d.nee_light[pix]);
+        Color bsdf_mis = pt::col(0.0f);
+        if (flags & FLAG_NEE_BSDF)
+        {
+            const int4 h = d.bsdf_hits[pix];
+            if (h.x >= 0)
+            {
+                const int32_t hp = h.x;
+                const Color emission = d.mats[d.mat_idx[hp]].emission;
+                const float4 rb = d.bsdf_rays[2 * (size_t)pix + 1];
+                const f3 dir = pt::mk(rb.x, rb.y, rb.z);
+                f3 a, b, c;
+                tri_verts(d, hp, a, b, c);
+                const f3 hgn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
+                const float cos_l = pt::maxf_(0.0f, pt::dot(hgn, -1.0f * dir));
+                if (!pt::is_black(emission) && cos_l > 0.0f)
+                {
+                    float pdf_area = 1.0f / d.areas[hp];
+                    pdf_area /= (float)d.emissive_count;
+                    const float t = __int_as_float(h.y);
+                    const float lpdf = cos_l < 1.0e-8f ? 0.0f : pdf_area * (t * t) / cos_l;
+                    const float4 nb = d.nee_bsdf[pix];
+                    const float bpdf = nb.w;
+                    const float w = bpdf == 0.0f ? 0.0f : bpdf / (1.0f * bpdf + 1.0f * lpdf);
+                    bsdf_mis = ldc(nb) * emission * w / bpdf;
+                }
+            }
+        }
+        const Color direct = light_mis + bsdf_mis;
+        Color total = pt::col(0.0f);
+        if (bounce == 0)
+            total = total + d.mats[__float_as_int(d.vtx_gn[pix].w)].emission;
+        total = total + ((direct / 1.0f) + pt::col(0.0f)) * ldc(thr_prev[pix]);
+        radiance = radiance + total;
+        d.radiance[pix] = stc(radiance, 0.0f);
+    }
+    if (!(flags & FLAG_CONTINUE))
+    {
+        d.flags[pix] = 0u;
+        return;
+    }
+    // next vertex (trace_main_path_ray, Intersect.h:222-245)
+    const int4 h = d.cont_hits[pix];
+    if (h.x < 0)
+    {
+        d.flags[pix] = 0u; // miss: no envmap contribution
+        return;
+    }
+    const float4 ra = d.cont_rays[2 * (size_t)pix], rb = d.cont_rays[2 * (size_t)pix + 1];
+    const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
+    f3 p, gn, sn;
+    hit_attributes(d, o, dir, h.x, __int_as_float(h.y), p, gn, sn);
+    d.vtx_pos[pix] = st3(p, d.vtx_pos[pix].w);
+    d.vtx_sn[pix] = st3(sn, __int_as_float(h.x));
+    d.vtx_gn[pix] = st3(gn, __int_as_float(d.mat_idx[h.x]));
+    d.vtx_view[pix] = st3(-1.0f * dir, 0.0f);
+    d.flags[pix] = FLAG_ALIVE;
+}
+
+__global__ void k_snapshot_throughput(PtData d, float4* __restrict__ out)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix < (uint32_t)(d.width * d.height))
+        out[pix] = d.throughput[pix];
+}
+
+// SanityCheck.h:78-106 + PathTracing.h:176-217: a NaN/inf/negative sample is
+// counted and dropped (contributes 0 at sample 0), otherwise summed.
+__global__ void k_accumulate(PtData d)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= (uint32_t)(d.width * d.height))
+        return;
+    float4 r = d.radiance[pix];
+    bool bad = false;
+    const float v[3] = {r.x, r.y, r.z};
+    for (int k = 0; k < 3; k++)
+        bad |= v[k] < 0.0f || isnan(v[k]) || isinf(v[k]);
+    if (bad)
+    {
+        atomicAdd(d.rejected, 1u);
+        r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (d.sample_number != 0)
+            return;
+    }
+    float* fb = d.fb + 3 * (size_t)pix;
+    if (d.sample_number == 0)
+    {
+        fb[0] = r.x;
+        fb[1] = r.y;
+        fb[2] = r.z;
+    }
+    else
+    {
+        fb[0] += r.x;
+        fb[1] += r.y;
+        fb[2] += r.z;
+    }
+    if (d.fb64)
+    {
+        double* f64 = d.fb64 + 3 * (size_t)pix;
+        if (d.sample_number == 0)
+        {
+            f64[0] = r.x;
+            f64[1] = r.y;
+            f64[2] = r.z;
+        }
+        else
+        {
+            f64[0] += (double)r.x;
+            f64[1] += (double)r.y;
+            f64[2] += (double)r.z;
+        }
+    }
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// MicrofacetRegularization::consistent_tau (MicrofacetRegularization.h:148-161),
+// evaluated once per pass on the host in binary32.
+float consistent_tau(float tau_0, uint32_t sample_number)
+{
+    const float pi = 3.14159265358979323846f;
+    const float four_pi = 12.5663706143591729539f;
+    const float two_pi = 6.28318530717958647693f;
+    float p = powf((float)(int)(sample_number + 1), -1.0f / 6.0f);
+    float inner = p * sqrtf(four_pi * tau_0 - 1.0f) / (two_pi * tau_0 - 1.0f);
+    return 1.0f / (2.0f * pi * (1.0f - cosf(atanf(inner))));
+}
+
+PtData make_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
+{
+    RenderState& R = ctx.render;
+    PtData d{};
+    d.verts = ctx.vertices.as<float>();
+    d.idx = ctx.indices.as<int32_t>();
+    d.mat_idx = ctx.material_indices.as<int32_t>();
+    d.mats = R.materials.as<pt::Material>();
+    d.emissive = ctx.emissive_prims.as<int32_t>();
+    d.areas = R.areas.as<float>();
+    d.emissive_count = ctx.emissive_count;
+    for (int k = 0; k < 3; k++) d.cam_pos[k] = R.camera.position[k];
+    d.width = R.settings.width;
+    d.height = R.settings.height;
+    d.nb_bounces = R.settings.nb_bounces;
+    d.do_rr = R.settings.do_russian_roulette;
+    d.rr_min_depth = R.settings.russian_roulette_min_depth;
+    d.rr_clamp = R.settings.russian_roulette_throughput_clamp;
+    d.freeze = R.settings.freeze_random;
+    d.enable_direct = R.settings.enable_direct;
+    d.sample_number = sample_number;
+    d.random_number = random_number;
+    d.g.ggx_conductor_ess = R.lut_conductor.as<float>();
+    d.g.lut_size = 128;
+    d.g.consistent_tau = consistent_tau(R.settings.microfacet_regularization_tau_0, sample_number);
+    d.g.min_roughness = R.settings.microfacet_regularization_min_roughness;
+    d.g.regularize = R.settings.do_microfacet_regularization;
+    d.gb_prim = R.gb_prim.as<int32_t>();
+    d.gb_pos = R.gb_pos.as<float4>();
+    d.gb_gn = R.gb_gn.as<uint32_t>();
+    d.gb_sn = R.gb_sn.as<uint32_t>();
+    d.gb_mat = R.gb_mat.as<int32_t>();
+    d.throughput = R.throughput.as<float4>();
+    d.radiance = R.radiance.as<float4>();
+    d.vtx_pos = R.vtx_pos.as<float4>();
+    d.vtx_sn = R.vtx_sn.as<float4>();
+    d.vtx_gn = R.vtx_gn.as<float4>();
+    d.vtx_view = R.vtx_view.as<float4>();
+    d.nee_light = R.nee_light.as<float4>();
+    d.nee_bsdf = R.nee_bsdf.as<float4>();
+    d.rng = R.rng.as<uint32_t>();
+    d.flags = R.flags.as<uint32_t>();
+    d.shadow_rays = R.shadow_rays.as<float4>();
+    d.shadow_occ = R.shadow_occ.as<uint8_t>();
+    d.bsdf_rays = R.bsdf_rays.as<float4>();
+    d.bsdf_hits = R.bsdf_hits.as<int4>();
+    d.cont_rays = R.cont_rays.as<float4>();
+    d.cont_hits = R.cont_hits.as<int4>();
+    d.fb = R.fb.as<float>();
+    d.fb64 = R.settings.accumulate_fp64 ? R.fb64.as<double>() : nullptr;
+    d.rejected = R.counters.as<uint32_t>();
+    return d;
+}
+
+} // namespace
+
+void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+{
+    RenderState& R = ctx.render;
+    const int32_t W = R.settings.width, H = R.settings.height;
+    const uint32_t n = (uint32_t)W * (uint32_t)H;
+    camera_rays(ctx, R.camera, W, H, sample_number, random_number, R.settings.freeze_random,
+                R.cont_rays.as<mipt_ray>(), nullptr, s);
+    trace_closest(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), nullptr, n, s);
+    PtData d = make_data(ctx, sample_number, random_number);
+    hipLaunchKernelGGL(k_gbuffer_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, d.cont_rays, d.cont_hits);
+    HIP_CHECK(hipGetLastError());
+}
+
+void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+{
+    RenderState& R = ctx.render;
+    const uint32_t n = (uint32_t)R.settings.width * (uint32_t)R.settings.height;
+    PtData d = make_data(ctx, sample_number, random_number);
+    const unsigned g = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d);
+    for (int32_t bounce = 0; bounce <= R.settings.nb_bounces; bounce++)
+    {
+        hipLaunchKernelGGL(k_snapshot_throughput, dim3(g), dim3(256), 0, s, d, R.thr_prev.as<float4>());
+        hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce);
+        trace_any(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), n, s);
+        trace_closest(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), nullptr, n, s);
+        if (bounce < R.settings.nb_bounces)
+            trace_closest(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), nullptr, n, s);
+        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)R.thr_prev.as<float4>());
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, d);
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace mipt

@@ -248,8 +248,10 @@ void scene_config1(Builder& B, mipt_camera& cam, int32_t w, int32_t h)
         for (int c = 0; c < 3; c++) m.emission[c] = (float)(15.0 + rng.uniform(-1.0, 1.0));
         int32_t mat = B.add_material(m);
         double x0 = q == 0 ? -0.5 : 0.1;
+        // Wound so that cross(p1 - p0, p2 - p0) points down (-y), into the
+        // box: back-facing lights receive no light samples.
         V3 p0 = {x0, 0.999, -0.3}, p1 = {x0 + 0.4, 0.999, -0.3}, p2 = {x0 + 0.4, 0.999, 0.3}, p3 = {x0, 0.999, 0.3};
-        B.add_quad(p0, p3, p2, p1, mat);
+        B.add_quad(p0, p1, p2, p3, mat);
     }
     const Mesh sphere = icosphere(4);
     std::vector<V3> centres;

@@ -96,6 +96,23 @@ class BvhInfo(C.Structure):
     ]
 
 
+class RenderSettings(C.Structure):
+    """mipt_render_settings (subset of HIPRTRenderSettings, RenderSettings.h:26-278)."""
+
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("nb_bounces", C.c_int32),
+        ("do_russian_roulette", C.c_int32), ("russian_roulette_min_depth", C.c_int32),
+        ("russian_roulette_throughput_clamp", C.c_float), ("freeze_random", C.c_int32),
+        ("enable_direct", C.c_int32), ("microfacet_regularization_tau_0", C.c_float),
+        ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
+        ("accumulate_fp64", C.c_int32),
+    ]
+
+
+MIPT_PASS_CAMERA_RAYS = 0
+MIPT_PASS_PATH_TRACE = 1
+
+
 class GeneratedScene(C.Structure):
     _fields_ = [("desc", SceneDesc), ("camera", Camera), ("width", C.c_int32), ("height", C.c_int32)]
 
@@ -122,6 +139,15 @@ _SIGNATURES = {
     "mipt_scene_generate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.POINTER(GeneratedScene))]),
     "mipt_scene_free": (None, [C.POINTER(GeneratedScene)]),
     "mipt_host_random_number": (C.c_uint32, [C.c_uint32]),
+    "mipt_render_settings_default": (None, [C.POINTER(RenderSettings)]),
+    "mipt_set_render_settings": (C.c_int, [C.c_void_p, C.POINTER(RenderSettings)]),
+    "mipt_set_camera": (C.c_int, [C.c_void_p, C.POINTER(Camera)]),
+    "mipt_launch_pass": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "mipt_render_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mipt_read_framebuffer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mipt_framebuffer_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "mipt_read_gbuffer_prims": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mipt_rejected_samples": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -156,6 +182,14 @@ def exported_symbols():
 
 def host_random_number(pass_index: int) -> int:
     return int(load_library().mipt_host_random_number(pass_index))
+
+
+def default_render_settings(**overrides) -> RenderSettings:
+    rs = RenderSettings()
+    load_library().mipt_render_settings_default(C.byref(rs))
+    for k, v in overrides.items():
+        setattr(rs, k, v)
+    return rs
 
 
 def default_material() -> Material:
@@ -324,6 +358,42 @@ class Context:
     def bounce_rays(self, d_primary, d_hits, d_rng, d_out, n: int, stream=None):
         self._check(self.lib.mipt_bounce_rays(self.h, ptr(d_primary), ptr(d_hits), ptr(d_rng), ptr(d_out), n,
                                               stream), "mipt_bounce_rays")
+
+    # Path-tracing passes
+    def set_render_settings(self, settings: RenderSettings):
+        self.settings = settings
+        self._check(self.lib.mipt_set_render_settings(self.h, C.byref(settings)), "mipt_set_render_settings")
+
+    def set_camera(self, cam: Camera):
+        self._check(self.lib.mipt_set_camera(self.h, C.byref(cam)), "mipt_set_camera")
+
+    def launch_pass(self, pass_id: int, sample_number: int, random_number: int, stream=None):
+        self._check(self.lib.mipt_launch_pass(self.h, pass_id, sample_number, random_number, stream),
+                    "mipt_launch_pass")
+
+    def render_sample(self, sample_number: int, stream=None):
+        self._check(self.lib.mipt_render_sample(self.h, sample_number, stream), "mipt_render_sample")
+
+    def read_framebuffer(self, fp64: bool = False):
+        n = self.settings.width * self.settings.height
+        fb = np.zeros(3 * n, np.float32)
+        fb64 = np.zeros(3 * n, np.float64) if fp64 else None
+        self._check(self.lib.mipt_read_framebuffer(self.h, fb.ctypes.data_as(C.c_void_p),
+                                                   fb64.ctypes.data_as(C.c_void_p) if fp64 else None),
+                    "mipt_read_framebuffer")
+        return (fb, fb64) if fp64 else fb
+
+    def read_gbuffer_prims(self):
+        n = self.settings.width * self.settings.height
+        out = np.zeros(n, np.int32)
+        self._check(self.lib.mipt_read_gbuffer_prims(self.h, out.ctypes.data_as(C.c_void_p)),
+                    "mipt_read_gbuffer_prims")
+        return out
+
+    def rejected_samples(self) -> int:
+        v = C.c_uint32()
+        self._check(self.lib.mipt_rejected_samples(self.h, C.byref(v)), "mipt_rejected_samples")
+        return int(v.value)
 
     # Convenience wrappers taking and returning host arrays (not for timing).
     def trace_closest_host(self, rays: np.ndarray) -> np.ndarray:

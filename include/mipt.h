@@ -228,6 +228,68 @@ int mipt_camera_rays(mipt_ctx* ctx, const mipt_camera* cam, int32_t w, int32_t h
 int mipt_bounce_rays(mipt_ctx* ctx, const mipt_ray* d_primary, const mipt_hit* d_hits,
                      uint32_t* d_rng_state, mipt_ray* d_out, uint32_t n, void* stream);
 
+
+/* ------------------------------------------------------------------ */
+/* Path-tracing passes: the per-pass launch surface of the renderer.    */
+/* ------------------------------------------------------------------ */
+
+/* Render settings: the subset of HIPRTRenderSettings
+ * (src/HostDeviceCommon/RenderSettings.h:26-278), MicrofacetRegularizationSettings
+ * and the kernel options the passes read. Defaults: mipt_render_settings_default. */
+typedef struct mipt_render_settings {
+    int32_t width, height;              /* render_resolution */
+    int32_t nb_bounces;                 /* RenderSettings.h:101 */
+    int32_t do_russian_roulette;        /* :103, default 1 */
+    int32_t russian_roulette_min_depth; /* :107, 0 as the reference initialises it */
+    float russian_roulette_throughput_clamp; /* :117, default 10 */
+    int32_t freeze_random;              /* :123 */
+    int32_t enable_direct;              /* :46, default 1 */
+    float microfacet_regularization_tau_0;  /* MicrofacetRegularizationSettings.h, 30 */
+    float microfacet_regularization_min_roughness; /* 0 */
+    int32_t do_microfacet_regularization;   /* PrincipledBSDFKernelOptions.h:171, 1 */
+    int32_t accumulate_fp64;            /* keep an FP64 mirror of the framebuffer sum */
+} mipt_render_settings;
+
+void mipt_render_settings_default(mipt_render_settings* s);
+
+/* Allocates the G-buffer, path state and framebuffer for width x height and
+ * packs the scene's materials (replaces GPURenderer::resize + the render-data
+ * wiring of update_render_data, GPURenderer.cpp:441, :769-820). Fails with
+ * MIPT_ERR_UNSUPPORTED if a material uses a BSDF lobe the passes do not
+ * implement (coat, sheen, specular, glass, diffuse transmission, thin film). */
+int mipt_set_render_settings(mipt_ctx* ctx, const mipt_render_settings* s);
+
+/* Replaces GPURenderer::set_camera (GPURenderer.cpp:1102). */
+int mipt_set_camera(mipt_ctx* ctx, const mipt_camera* cam);
+
+#define MIPT_PASS_CAMERA_RAYS 0 /* CameraRays (CameraRays.h:83): G-buffer fill */
+#define MIPT_PASS_PATH_TRACE 1  /* MegaKernel (Megakernel.h:23): 1 spp, accumulate */
+
+/* One pass with the given sample_number (render_settings.sample_number) and
+ * random_number (render_data.random_number). Asynchronous on `stream`. */
+int mipt_launch_pass(mipt_ctx* ctx, int32_t pass, uint32_t sample_number, uint32_t random_number, void* stream);
+
+/* One full sample: CAMERA_RAYS then PATH_TRACE, with random numbers from the
+ * host stream (xorshift32 seeded with 42): draw 2*sample_number for the camera
+ * pass and 2*sample_number + 1 for the path pass. */
+int mipt_render_sample(mipt_ctx* ctx, uint32_t sample_number, void* stream);
+
+/* Framebuffer: ColorRGB32F[W*H] SUM of samples (not the mean), pix = x + y*W,
+ * y = 0 the bottom row (PathTracing.h:176-217). host_rgb: float[3*W*H] or NULL;
+ * host_rgb64: double[3*W*H] or NULL (needs accumulate_fp64). Synchronous. */
+int mipt_read_framebuffer(mipt_ctx* ctx, float* host_rgb, double* host_rgb64);
+
+/* Device pointers of the framebuffer sums (valid until the next
+ * mipt_set_render_settings). */
+int mipt_framebuffer_device(mipt_ctx* ctx, float** d_rgb, double** d_rgb64);
+
+/* G-buffer readback (first-hit primitive per pixel, -1 on a miss). */
+int mipt_read_gbuffer_prims(mipt_ctx* ctx, int32_t* host_prims);
+
+/* Samples dropped by the NaN/negative check (SanityCheck.h:78-106) since the
+ * last mipt_set_render_settings. */
+int mipt_rejected_samples(mipt_ctx* ctx, uint32_t* out_count);
+
 /* ------------------------------------------------------------------ */
 /* Procedural scenes (replace the ASSIMP SceneParser, which is out of   */
 /* scope). config 1: Cornell box + 20 icospheres (102,428 triangles);  */

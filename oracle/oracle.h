@@ -108,6 +108,114 @@ void orc_hit_attributes(const float* verts, const int32_t* idx, const orc_ray* r
 void orc_bounce_rays(const float* verts, const int32_t* idx, const orc_ray* primary,
                      const orc_hit* hits, uint32_t* rng_state, int32_t n, orc_ray* out);
 
+/* ---- Path tracing (pathtrace.c) ------------------------------------------ */
+
+/* CPUMaterial fields (src/HostDeviceCommon/Material/MaterialCPU.h:213-330),
+ * in declaration order. */
+typedef struct {
+    float emission[3];
+    float emission_strength;
+    float global_emissive_factor;
+    int32_t emissive_texture_used;
+    float base_color[3];
+    float roughness;
+    float oren_nayar_sigma;
+    float metallic;
+    float metallic_F90_falloff_exponent;
+    float metallic_F82[3];
+    float metallic_F90[3];
+    float anisotropy;
+    float anisotropy_rotation;
+    float second_roughness_weight;
+    float second_roughness;
+    int32_t do_metallic_energy_compensation;
+    float specular;
+    float specular_tint;
+    float specular_color[3];
+    float specular_darkening;
+    int32_t do_specular_energy_compensation;
+    float coat;
+    float coat_medium_absorption[3];
+    float coat_medium_thickness;
+    float coat_roughness;
+    float coat_roughening;
+    float coat_darkening;
+    float coat_anisotropy;
+    float coat_anisotropy_rotation;
+    float coat_ior;
+    int32_t do_coat_energy_compensation;
+    float sheen;
+    float sheen_roughness;
+    float sheen_color[3];
+    float ior;
+    float specular_transmission;
+    float diffuse_transmission;
+    float absorption_at_distance;
+    float absorption_color[3];
+    float dispersion_scale;
+    float dispersion_abbe_number;
+    int32_t thin_walled;
+    int32_t do_glass_energy_compensation;
+    float thin_film;
+    float thin_film_ior;
+    float thin_film_thickness;
+    float thin_film_kappa_3;
+    float thin_film_hue_shift_degrees;
+    float thin_film_base_ior_override;
+    int32_t thin_film_do_ior_override;
+    float alpha_opacity;
+    int32_t dielectric_priority;
+} orc_cpu_material;
+
+typedef struct {
+    int32_t width, height;
+    int32_t nb_bounces;
+    int32_t do_russian_roulette;
+    int32_t russian_roulette_min_depth;
+    float russian_roulette_throughput_clamp;
+    int32_t freeze_random;
+    int32_t enable_direct;
+    float microfacet_regularization_tau_0;
+    float microfacet_regularization_min_roughness;
+    int32_t do_microfacet_regularization;
+    int32_t accumulate_fp64;
+} orc_render_settings;
+
+typedef struct {
+    const float* vertices;          /* float[3V] */
+    const int32_t* indices;         /* int[3T] */
+    int32_t triangle_count;
+    const int32_t* material_indices;/* int[T] */
+    const orc_cpu_material* materials;
+    int32_t material_count;
+    const int32_t* emissive;        /* int[E] */
+    int32_t emissive_count;
+    const float* lut_conductor;     /* 128 x 128 E_ss, row 0 = roughness 1 */
+} orc_render_scene;
+
+/* One sample of every pixel: CameraRays (CameraRays.h:83-213) then MegaKernel
+ * (Megakernel.h:23-144) per pixel, accumulated into fb (float[3WH], sum) and
+ * fb64 (double[3WH], may be NULL); gb_prims (int[WH], may be NULL) receives
+ * the first-hit primitives. Returns the number of dropped NaN/negative
+ * samples. `tree` must be built over the same geometry. */
+int32_t orc_render_sample(const orc_render_scene* scene, const orc_octree* tree, const orc_camera* cam,
+                          const orc_render_settings* s, uint32_t sample_number, uint32_t rn_camera,
+                          uint32_t rn_path, float* fb, double* fb64, int32_t* gb_prims, int32_t nthreads);
+
+float orc_consistent_tau(float tau_0, uint32_t sample_number);
+
+/* Principled BSDF (metal + diffuse lobes) of one material, restating
+ * principled_bsdf_eval / _sample (Principled.h:1620-1715, :1802-1939). Vectors
+ * are float[3] in world space; reg_mode 0/1/2 = none/classic/MIS. */
+void orc_bsdf_eval(const orc_cpu_material* m, const float* lut, float consistent_tau, int32_t reg_mode,
+                   float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light,
+                   float* out_rgb, float* out_pdf);
+void orc_bsdf_sample(const orc_cpu_material* m, const float* lut, float consistent_tau, int32_t reg_mode,
+                     float path_roughness, const float* view, const float* sn, const float* gn, uint32_t* rng_state,
+                     float* out_dir, float* out_rgb, float* out_pdf);
+uint32_t orc_oct_pack(const float* n);
+void orc_oct_unpack(uint32_t p, float* out);
+
 #ifdef __cplusplus
 }
 #endif
