@@ -11,9 +11,11 @@ Prints ONE JSON line: Mrays/s over the timed steps, the roofline of the
 traversal kernel (algorithmic bytes per launch / mean launch time, HIP events
 on the launch stream) and the CPU baseline (the oracle's restatement of the
 reference CPU renderer's octree traversal, OpenMP over all host cores, on the
-same rays).
+same rays). A second object, "plain_render", times the plain path tracer on
+the 512x512 Cornell-box scene (16 x 1-spp passes, 4 bounces) beside the
+oracle's CPU megakernel restatement.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-render]
 """
 import argparse
 import json
@@ -40,6 +42,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the plain-render timing leg")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -143,6 +146,8 @@ def main():
         with open(prof) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
+    render = None if args.no_render else plain_render_leg(mipt, torch, np, args.no_cpu)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -169,9 +174,61 @@ def main():
         "bvh": {"nodes": info.node_count, "leaves": info.leaf_count, "max_depth": info.max_depth,
                 "sah_cost": round(info.sah_cost, 2), "build_ms": round(info.build_ms, 2)},
         "hit_fraction": round(float((hits["prim"] >= 0).mean()), 4),
+        "plain_render": render,
     }
     print(json.dumps(out))
     ctx.close()
+
+
+def plain_render_leg(mipt, torch, np, no_cpu):
+    """Plain path tracing of the 512x512 Cornell-box scene (config 1): 16
+    one-sample passes of CameraRays + path trace, 4 bounces, uniform light
+    NEE + MIS. GPU wall time per 16-spp frame; the oracle's per-pixel
+    megakernel (OpenMP over the host cores) on 2 of the 16 passes."""
+    W = H = 512
+    spp = 16
+    scene = mipt.generate_scene(1, W, H)
+    settings = mipt.default_render_settings(width=W, height=H, nb_bounces=4)
+    ctx = mipt.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    ctx.set_render_settings(settings)
+    ctx.set_camera(scene.camera)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    for s in range(2):
+        ctx.render_sample(s, sh)
+    stream.synchronize()
+    frames = 3
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        for s in range(spp):
+            ctx.render_sample(s, sh)
+    stream.synchronize()
+    gpu_frame_ms = (time.perf_counter() - t0) / frames * 1e3
+    fb = ctx.read_framebuffer()
+    ctx.close()
+    out = {"workload": "config 1 (102,428 tris) 512x512, 16 x 1-spp passes, 4 bounces, uniform NEE + MIS",
+           "gpu_ms_per_frame": round(gpu_frame_ms, 3),
+           "gpu_ms_per_pass": round(gpu_frame_ms / spp, 4),
+           "gpu_Mpaths_per_s": round(W * H * spp / gpu_frame_ms / 1e3, 2),
+           "mean_radiance": [round(float(x), 5) for x in (fb.reshape(-1, 3) / spp).mean(axis=0)]}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        R = oracle.Renderer(scene, settings)
+        passes = 2
+        tc = time.perf_counter()
+        for s in range(passes):
+            R.render_sample(s, mipt.host_random_number(2 * s), mipt.host_random_number(2 * s + 1), nthreads=cores)
+        cpu_pass_s = (time.perf_counter() - tc) / passes
+        out["cpu_baseline"] = {"ms_per_pass": round(cpu_pass_s * 1e3, 1),
+                               "ms_per_frame_16spp": round(cpu_pass_s * spp * 1e3, 1), "cores": cores,
+                               "kind": "port", "sample": f"{passes} of the 16 passes, full 512x512"}
+        out["gpu_speedup_vs_cpu"] = round(cpu_pass_s * spp * 1e3 / gpu_frame_ms, 1)
+    return out
 
 
 if __name__ == "__main__":

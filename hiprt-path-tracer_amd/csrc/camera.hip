@@ -129,7 +129,7 @@ __global__ void k_bounce_rays(const float* __restrict__ verts, const int32_t* __
     }
     const float theta = 6.28318530717958647693f * r1;
     const float s = sqrtf(1.0f - r2 * r2);
-    const pt::f3 sp = pt::mk(s * cosf(theta), s * sinf(theta), r2);
+    const pt::f3 sp = pt::mk(s * pt::cos_fixed(theta), s * pt::sin_fixed(theta), r2);
     const pt::f3 nd = pt::normalize(sn + sp);
     rng_state[i] = rng.state;
     out[2 * (size_t)i] = make_float4(p.x, p.y, p.z, __int_as_float(prim));

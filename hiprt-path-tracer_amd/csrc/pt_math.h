@@ -36,6 +36,98 @@ PT_HD f3 fmin3(f3 a, f3 b) { return mk(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a
 PT_HD f3 fmax3(f3 a, f3 b) { return mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
 
 // ---------------------------------------------------------------------------
+// sin, cos and pow as a fixed sequence of binary32 operations. The device
+// library and a host libm round sinf/cosf/powf differently in the last
+// place; with these, the kernels and any host restatement of the same steps
+// produce identical bits. Accuracy: a few ulp for |x| < 6000 (sin, cos) and
+// for x >= 0 (pow); the renderer passes angles in [0, 2 pi] and x in [0, 1].
+// ---------------------------------------------------------------------------
+
+// Cody & Waite reduction x = k pi/2 + r with pi/2 split in three parts
+// (1.5703125 and 0x1.fb6p-12 have 8 and 12 significant bits, so k times
+// either is exact for |k| < 4096), then the Taylor polynomials of sin and
+// cos on |r| <= pi/4 (truncation error < 2e-9).
+PT_HD void sincos_fixed(float x, float& s_out, float& c_out)
+{
+    const float k = rintf(x * 0x1.45f306p-1f);
+    float r = x - k * 0x1.92p+0f;
+    r = r - k * 0x1.fb6p-12f;
+    r = r - k * -0x1.777a5cp-25f;
+    const float r2 = r * r;
+    const float sp = r + r * r2 * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f)));
+    const float cp = 1.0f + r2 * (-0.5f + r2 * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f + r2 * -0x1.27e4fcp-22f))));
+    const int q = (int)k & 3;
+    s_out = q == 0 ? sp : q == 1 ? cp : q == 2 ? -sp : -cp;
+    c_out = q == 0 ? cp : q == 1 ? -sp : q == 2 ? -cp : sp;
+}
+PT_HD float sin_fixed(float x)
+{
+    float s, c;
+    sincos_fixed(x, s, c);
+    return s;
+}
+PT_HD float cos_fixed(float x)
+{
+    float s, c;
+    sincos_fixed(x, s, c);
+    return c;
+}
+
+// log2 of a positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(t), t = (m - 1) / (m + 1), |t| < 0.172, by its series to t^9.
+PT_HD float log2_fixed(float x)
+{
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0x1.6a09e6p-1f)
+    {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    const float t = (m - 1.0f) / (m + 1.0f);
+    const float t2 = t * t;
+    const float ln_m = t * (2.0f + t2 * (0x1.555556p-1f + t2 * (0x1.99999ap-2f + t2 * (0x1.24924ap-2f + t2 * 0x1.c71c72p-3f))));
+    return (float)e + ln_m * 0x1.715476p+0f;
+}
+
+// 2^v = 2^n e^(f ln 2), n = rint(v), |f| <= 1/2, by the Taylor series of exp to g^7.
+PT_HD float exp2_fixed(float v)
+{
+    if (v < -150.0f)
+        return 0.0f;
+    if (v > 128.0f)
+        return INFINITY;
+    const float n = rintf(v);
+    const float g = (v - n) * 0x1.62e43p-1f;
+    const float p = 1.0f + g * (1.0f + g * (0.5f + g * (0x1.555556p-3f + g * (0x1.555556p-5f + g * (0x1.111112p-7f + g * (0x1.6c16c2p-10f + g * 0x1.a01a02p-13f))))));
+    return ldexpf(p, (int)n);
+}
+
+// x^y for x >= 0. Integral exponents up to 64 in magnitude by square and
+// multiply (the F82 Fresnel falloff exponent is 5 by default).
+PT_HD float pow_fixed(float x, float y)
+{
+    if (y == rintf(y) && fabsf(y) <= 64.0f)
+    {
+        int n = (int)fabsf(y);
+        float r = 1.0f, b = x;
+        while (n)
+        {
+            if (n & 1)
+                r = r * b;
+            b = b * b;
+            n >>= 1;
+        }
+        return y < 0.0f ? 1.0f / r : r;
+    }
+    if (x == 0.0f)
+        return y > 0.0f ? 0.0f : INFINITY;
+    if (x == 1.0f)
+        return 1.0f;
+    return exp2_fixed(y * log2_fixed(x));
+}
+
+// ---------------------------------------------------------------------------
 // RNG: the reference's per-pixel streams (src/Device/includes/Hash.h:11-19,
 // src/HostDeviceCommon/Xorshift.h:18-66).
 // ---------------------------------------------------------------------------

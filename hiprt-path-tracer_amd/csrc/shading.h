@@ -3,8 +3,9 @@
 // functions shared by the path-tracing kernels.
 //
 // Every function restates the reference's arithmetic operation for operation
-// (compiled with -ffp-contract=off) so that the only differences against the
-// CPU restatement come from transcendentals (cosf, sinf, powf, ...).
+// (compiled with -ffp-contract=off). sin, cos and pow go through the
+// fixed-sequence versions of pt_math.h, so a CPU restatement of the same
+// steps reproduces every result bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -121,7 +122,7 @@ PT_HD void build_rotated_onb(f3 N, f3& T, f3& B, float rotation)
 {
     f3 up = fabsf(N.z) < 0.9999999f ? mk(0.0f, 0.0f, 1.0f) : mk(1.0f, 0.0f, 0.0f);
     T = normalize(cross(up, N));
-    float c = cosf(rotation), s = sinf(rotation);
+    float c = cos_fixed(rotation), s = sin_fixed(rotation);
     T = (vmul(T, c) + vmul(cross(N, T), s)) + vmul(vmul(N, dot(N, T)), 1.0f - c);
     B = cross(N, T);
 }
@@ -147,7 +148,7 @@ PT_HD f3 cosine_sample_around_normal(f3 n, Xorshift32& rng)
     }
     float theta = TWO_PI_F * r1;
     float s = sqrtf(1.0f - r2 * r2);
-    f3 sp = mk(s * cosf(theta), s * sinf(theta), r2);
+    f3 sp = mk(s * cos_fixed(theta), s * sin_fixed(theta), r2);
     return normalize(n + sp);
 }
 
@@ -300,8 +301,8 @@ PT_HD f3 ggx_vndf_sample(f3 v, float ax, float ay, Xorshift32& rng)
     f3 T2 = cross(Vh, T1);
     float r = sqrtf(r1);
     float phi = TWO_PI_F * r2;
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float t1 = r * cos_fixed(phi);
+    float t2 = r * sin_fixed(phi);
     float s = 0.5f * (1.0f + Vh.z);
     t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
     f3 Nh = (vmul(T1, t1) + vmul(T2, t2)) + vmul(Vh, sqrtf(maxf_(0.0f, 1.0f - t1 * t1 - t2 * t2)));
@@ -332,13 +333,13 @@ PT_HD float pow6(float x)
 // Fresnel.h:138-168 (Adobe F82-tint).
 PT_HD Color f82_tint_fresnel(Color F0, Color F82, Color F90, float falloff, float cos_theta)
 {
-    Color base = F0 + (F90 - F0) * powf(1.0f - cos_theta, falloff);
+    Color base = F0 + (F90 - F0) * pow_fixed(1.0f - cos_theta, falloff);
     if (max_component(base) < 1.0e-8f)
         return col(0.0f);
     float lazanyi = cos_theta * pow6(1.0f - cos_theta);
     const float cos_theta_max = 1.0f / 7.0f;
     const float denom_a = cos_theta_max * pow6(1.0f - cos_theta_max);
-    Color nume_a = (F0 + (F90 - F0) * powf(1.0f - cos_theta_max, falloff)) * (col(1.0f) - F82);
+    Color nume_a = (F0 + (F90 - F0) * pow_fixed(1.0f - cos_theta_max, falloff)) * (col(1.0f) - F82);
     Color a = nume_a / denom_a;
     Color F = base - a * lazanyi;
     return col(clampf_(0.0f, 1.0f, F.r), clampf_(0.0f, 1.0f, F.g), clampf_(0.0f, 1.0f, F.b));

@@ -248,9 +248,50 @@ typedef struct mipt_render_settings {
     float microfacet_regularization_min_roughness; /* 0 */
     int32_t do_microfacet_regularization;   /* PrincipledBSDFKernelOptions.h:171, 1 */
     int32_t accumulate_fp64;            /* keep an FP64 mirror of the framebuffer sum */
+    /* DirectLightSamplingStrategy / ...BaseStrategy (kernel options in the
+     * reference, DirectLightSamplingOptions.h:20-35): MIPT_LSS_* below.
+     * Defaults: MIS light + BSDF, uniform. */
+    int32_t direct_light_sampling_strategy;
+    int32_t direct_light_sampling_base_strategy;
 } mipt_render_settings;
 
+#define MIPT_LSS_MIS_LIGHT_BSDF 3 /* LSS_MIS_LIGHT_BSDF: one light sample + one BSDF sample, balance heuristic */
+#define MIPT_LSS_RESTIR_DI 5      /* LSS_RESTIR_DI: ReSTIR DI reservoir at bounce 0, MIS light + BSDF after */
+#define MIPT_LSS_BASE_UNIFORM 0   /* LSS_BASE_UNIFORM: emissive triangle picked uniformly */
+#define MIPT_LSS_BASE_POWER 1     /* LSS_BASE_POWER: picked by emitted power (alias table) */
+
 void mipt_render_settings_default(mipt_render_settings* s);
+
+/* ReSTIR DI settings: ReSTIRDISettings (src/HostDeviceCommon/ReSTIR/ReSTIRDISettings.h:
+ * 17-163) and its ReSTIRCommonSettings (ReSTIRCommonSettings.h). The kernel
+ * options of ReSTIRDIOptions.h are fixed at the reference defaults:
+ * pairwise MIS bias correction with visibility, visibility reuse, no
+ * visibility in the initial or spatial target functions, no light
+ * presampling. Later bounces use MIS light + BSDF
+ * (RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF). Defaults:
+ * mipt_restir_di_settings_default (the reference's). */
+typedef struct mipt_restir_di_settings {
+    int32_t number_of_initial_light_candidates; /* 4 */
+    int32_t number_of_initial_bsdf_candidates;  /* 1 */
+    int32_t do_temporal_reuse_pass;             /* 1 */
+    int32_t do_spatial_reuse_pass;              /* 1 */
+    int32_t spatial_number_of_passes;           /* 1 */
+    int32_t spatial_reuse_radius;               /* 16 px */
+    int32_t spatial_reuse_neighbor_count;       /* 5, at most 32 */
+    int32_t m_cap;                              /* 3; 0 = no cap */
+    int32_t use_confidence_weights;             /* 1 */
+    int32_t use_normal_similarity_heuristic;    /* 1 */
+    float normal_similarity_angle_precomp;      /* cos(37.5 deg) = 0.906307787 */
+    int32_t use_plane_distance_heuristic;       /* 1 */
+    float plane_distance_threshold;             /* 0.1 */
+    int32_t reject_using_geometric_normals;     /* 1 */
+    int32_t do_final_shading_visibility;        /* 1 */
+} mipt_restir_di_settings;
+
+void mipt_restir_di_settings_default(mipt_restir_di_settings* s);
+/* Takes effect at the next pass; allocates the reservoir buffers when the
+ * strategy is MIPT_LSS_RESTIR_DI. */
+int mipt_set_restir_di_settings(mipt_ctx* ctx, const mipt_restir_di_settings* s);
 
 /* Allocates the G-buffer, path state and framebuffer for width x height and
  * packs the scene's materials (replaces GPURenderer::resize + the render-data
@@ -264,15 +305,47 @@ int mipt_set_camera(mipt_ctx* ctx, const mipt_camera* cam);
 
 #define MIPT_PASS_CAMERA_RAYS 0 /* CameraRays (CameraRays.h:83): G-buffer fill */
 #define MIPT_PASS_PATH_TRACE 1  /* MegaKernel (Megakernel.h:23): 1 spp, accumulate */
+#define MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES 2 /* ReSTIR_DI_InitialCandidates (DI/InitialCandidates.h:478) */
+#define MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE 3     /* ReSTIR_DI_TemporalReuse (DI/TemporalReuse.h:50) */
+/* ReSTIR_DI_SpatialReuse (DI/SpatialReuse.h:43) as spatial pass number i
+ * (common_spatial_pass.spatial_pass_index), i in [0, spatial_number_of_passes). */
+#define MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i) (16 + (i))
 
 /* One pass with the given sample_number (render_settings.sample_number) and
- * random_number (render_data.random_number). Asynchronous on `stream`. */
+ * random_number (render_data.random_number). Asynchronous on `stream`.
+ * sample_number 0 resets the accumulation (need_to_reset): the camera pass
+ * then clears the G-buffers and the ReSTIR DI reservoirs. The ReSTIR DI
+ * passes route their reservoir buffers as ReSTIRDIRenderPass does
+ * (src/Renderer/RenderPasses/ReSTIRDIRenderPass.cpp:442-677); the last
+ * ReSTIR pass of a frame selects the reservoirs the path pass shades. */
 int mipt_launch_pass(mipt_ctx* ctx, int32_t pass, uint32_t sample_number, uint32_t random_number, void* stream);
 
-/* One full sample: CAMERA_RAYS then PATH_TRACE, with random numbers from the
- * host stream (xorshift32 seeded with 42): draw 2*sample_number for the camera
- * pass and 2*sample_number + 1 for the path pass. */
+/* One full sample with the host random stream (xorshift32 seeded with 42)
+ * drawn once per pass in the order the GPU renderer draws it: CAMERA_RAYS
+ * (FillGBufferRenderPass.cpp:91); with ReSTIR DI, INITIAL_CANDIDATES (one
+ * draw, ReSTIRDIRenderPass.cpp:449), TEMPORAL_REUSE (two draws, the second
+ * being the unused permutation-sampling bits, :469-470) and each spatial pass
+ * (one draw, :532); then PATH_TRACE (MegaKernelRenderPass.cpp:53). Sample s
+ * uses draws [s*D, (s+1)*D), D = mipt_draws_per_sample(ctx). */
 int mipt_render_sample(mipt_ctx* ctx, uint32_t sample_number, void* stream);
+int32_t mipt_draws_per_sample(const mipt_ctx* ctx);
+
+/* Power alias table of the emissive triangles (GPURenderer.cpp:221-295 with
+ * Utils::compute_alias_table, src/Utils/Utils.cpp:63-131), built when the
+ * base strategy is POWER. Host copies: probas/alias of emissive_triangle_count
+ * entries (either may be NULL), *sum = sum of the powers. */
+int mipt_get_power_alias_table(mipt_ctx* ctx, float* probas, int32_t* alias, float* sum);
+
+/* Vose alias table of n non-negative weights, normalised in FP64
+ * (Utils::compute_alias_table, Utils.cpp:63-131). Host utility. */
+int mipt_compute_alias_table(const float* weights, int32_t n, float total_sum, float* out_probas, int32_t* out_alias);
+
+/* Reservoir readback: which = 0 initial-candidates buffer, 1 / 2 the spatial
+ * output buffers, 3 the reservoirs the path pass shades. Per pixel 8 words:
+ * float point[3], float target_function, int32 M, int32 emissive_triangle,
+ * float UCW, int32 flags (ReSTIRDIReservoir, DI/Reservoir.h:52-187, without
+ * weight_sum, which no later pass reads). Synchronous. */
+int mipt_read_restir_reservoirs(mipt_ctx* ctx, int32_t which, void* host_out);
 
 /* Framebuffer: ColorRGB32F[W*H] SUM of samples (not the mean), pix = x + y*W,
  * y = 0 the bottom row (PathTracing.h:176-217). host_rgb: float[3*W*H] or NULL;

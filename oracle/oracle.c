@@ -7,6 +7,7 @@
  * (relative to the reference repository root) whose behaviour it restates.
  */
 #include "oracle.h"
+#include "fmath.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -703,7 +704,7 @@ static f3 cosine_around_normal(f3 n, uint32_t* st)
     }
     float theta = 6.28318530717958647693f * r1;
     float s = sqrtf(1.0f - r2 * r2);
-    f3 sp = mk(s * cosf(theta), s * sinf(theta), r2);
+    f3 sp = mk(s * orc_cosf(theta), s * orc_sinf(theta), r2);
     return normalize(add(n, sp));
 }
 
@@ -727,5 +728,23 @@ void orc_bounce_rays(const float* verts, const int32_t* idx, const orc_ray* prim
         out[i].d[0] = d.x; out[i].d[1] = d.y; out[i].d[2] = d.z;
         out[i].last_prim = hits[i].prim;
         out[i].tmax = INFINITY;
+    }
+}
+
+/* Element-wise evaluation of the fixed-sequence math functions of fmath.h,
+ * for their accuracy tests: fn 0 = sin, 1 = cos, 2 = pow(x, y), 3 = log2,
+ * 4 = exp2. */
+void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n)
+{
+    for (int32_t i = 0; i < n; i++)
+    {
+        switch (fn)
+        {
+        case 0: out[i] = orc_sinf(x[i]); break;
+        case 1: out[i] = orc_cosf(x[i]); break;
+        case 2: out[i] = orc_powf(x[i], y[i]); break;
+        case 3: out[i] = orc_log2f(x[i]); break;
+        default: out[i] = orc_exp2f(x[i]); break;
+        }
     }
 }

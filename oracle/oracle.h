@@ -214,6 +214,8 @@ void orc_bsdf_sample(const orc_cpu_material* m, const float* lut, float consiste
                      float path_roughness, const float* view, const float* sn, const float* gn, uint32_t* rng_state,
                      float* out_dir, float* out_rgb, float* out_pdf);
 uint32_t orc_oct_pack(const float* n);
+/* fmath.h functions element-wise: fn 0 sin, 1 cos, 2 pow(x, y), 3 log2, 4 exp2. */
+void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n);
 void orc_oct_unpack(uint32_t p, float* out);
 
 #ifdef __cplusplus

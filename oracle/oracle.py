@@ -74,6 +74,7 @@ _SIG = {
     "orc_oct_unpack": (None, [C.c_uint32, _P]),
     "orc_bsdf_eval": (None, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P, _P, _P]),
     "orc_bsdf_sample": (None, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_fmath_eval": (None, [C.c_int32, _P, _P, _P, C.c_int32]),
 }
 
 _lib = None
@@ -264,6 +265,18 @@ class Renderer:
 
 def consistent_tau(tau_0, sample_number):
     return float(lib().orc_consistent_tau(tau_0, sample_number))
+
+
+FMATH = {"sin": 0, "cos": 1, "pow": 2, "log2": 3, "exp2": 4}
+
+
+def fmath(name, x, y=None):
+    """The fixed-sequence sin/cos/pow/log2/exp2 of oracle/fmath.h, element-wise."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float32)
+    out = np.zeros_like(x)
+    lib().orc_fmath_eval(FMATH[name], _p(x), _p(y), _p(out), x.size)
+    return out
 
 
 def oct_pack(n):

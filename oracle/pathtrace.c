@@ -9,6 +9,7 @@
  * function names the reference file:line it restates.
  */
 #include "oracle.h"
+#include "fmath.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -169,7 +170,7 @@ static void build_rotated_onb(v3 N, v3* T, v3* B, float rot)
 {
     v3 up = fabsf(N.z) < 0.9999999f ? V(0.0f, 0.0f, 1.0f) : V(1.0f, 0.0f, 0.0f);
     v3 t = vnorm(vcross(up, N));
-    float c = cosf(rot), s = sinf(rot);
+    float c = orc_cosf(rot), s = orc_sinf(rot);
     t = vadd(vadd(vscale(t, c), vscale(vcross(N, t), s)), vscale(vscale(N, vdot(N, t)), 1.0f - c));
     *T = t;
     *B = vcross(N, t);
@@ -190,7 +191,7 @@ static v3 cosine_around_normal(v3 n, uint32_t* st)
     }
     float theta = M_TWO_PI_F * r1;
     float s = sqrtf(1.0f - r2 * r2);
-    return vnorm(vadd(n, V(s * cosf(theta), s * sinf(theta), r2)));
+    return vnorm(vadd(n, V(s * orc_cosf(theta), s * orc_sinf(theta), r2)));
 }
 
 /* ------------------------------------------------------------------ */
@@ -247,13 +248,13 @@ static float pow6(float x) { float x2 = x * x, x4 = x2 * x2; return x4 * x2; }
 
 static rgb f82(rgb F0, rgb F82c, rgb F90, float e, float c)
 {
-    rgb base = cadd(F0, cscale(csub(F90, F0), powf(1.0f - c, e)));
+    rgb base = cadd(F0, cscale(csub(F90, F0), orc_powf(1.0f - c, e)));
     if (cmax(base) < 1.0e-8f)
         return C1(0.0f);
     float laz = c * pow6(1.0f - c);
     const float cmx = 1.0f / 7.0f;
     const float denom_a = cmx * pow6(1.0f - cmx);
-    rgb nume_a = cmul(cadd(F0, cscale(csub(F90, F0), powf(1.0f - cmx, e))), csub(C1(1.0f), F82c));
+    rgb nume_a = cmul(cadd(F0, cscale(csub(F90, F0), orc_powf(1.0f - cmx, e))), csub(C1(1.0f), F82c));
     rgb a = cdivs(nume_a, denom_a);
     rgb F = csub(base, cscale(a, laz));
     return C3(hclamp(0.0f, 1.0f, F.r), hclamp(0.0f, 1.0f, F.g), hclamp(0.0f, 1.0f, F.b));
@@ -322,7 +323,7 @@ static v3 vndf_sample(v3 v, float ax, float ay, uint32_t* st)
     v3 T1 = lensq > 0.0f ? vdivs(V(-Vh.y, Vh.x, 0.0f), sqrtf(lensq)) : V(1.0f, 0.0f, 0.0f);
     v3 T2 = vcross(Vh, T1);
     float r = sqrtf(r1), phi = M_TWO_PI_F * r2;
-    float t1 = r * cosf(phi), t2 = r * sinf(phi);
+    float t1 = r * orc_cosf(phi), t2 = r * orc_sinf(phi);
     float s = 0.5f * (1.0f + Vh.z);
     t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
     v3 Nh = vadd(vadd(vscale(T1, t1), vscale(T2, t2)), vscale(Vh, sqrtf(hmax(0.0f, 1.0f - t1 * t1 - t2 * t2))));

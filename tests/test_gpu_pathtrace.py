@@ -62,8 +62,10 @@ def _check(mipt_mod, oracle_mod, scene, settings, spp):
     assert np.array_equal(prims, cprims), "first-hit primitive IDs differ"
     frac, diverged = _compare(fb, cfb)
     frac64, diverged64 = _compare(fb64, cfb64)
+    identical = int((fb.reshape(-1, 3) == cfb.reshape(-1, 3)).all(axis=1).sum())
     print(f"\n{settings.width}x{settings.height} {spp} spp: {frac * 100:.4f} % pixels within 1e-4 "
-          f"({diverged} divergent), fp64 {frac64 * 100:.4f} % ({diverged64}); rejected gpu {rej} cpu {crej}")
+          f"({diverged} divergent, {identical} of {fb.size // 3} bit-identical), fp64 {frac64 * 100:.4f} % "
+          f"({diverged64}); rejected gpu {rej} cpu {crej}")
     assert frac >= MIN_PASS
     assert frac64 >= MIN_PASS
     assert rej == crej

@@ -2,8 +2,9 @@
 camera rays, closest hit, any hit and bounce rays against the oracle.
 
 Bar: bit-exact hit primitive IDs and bit-exact t/u/v against the oracle's
-brute-force loop (identical input ray bits); camera rays bit-exact; bounce
-directions (which go through cosf/sinf) within 2e-6 of the oracle's.
+brute-force loop (identical input ray bits); camera rays and bounce rays
+bit-exact (sin/cos of the cosine sampling are the fixed-sequence versions on
+both sides).
 """
 import numpy as np
 import pytest
@@ -99,13 +100,12 @@ def test_closest_hit_scene1_bit_exact(mipt_mod, oracle_mod, torch_mod, scene1):
     # the identical ray bits.
     _assert_hits_equal(hits2[sub], oracle_mod.trace_brute(scene1.vertices, scene1.indices, bounce[sub]))
     _assert_hits_equal(hits2, tree.trace(bounce))
-    # The GPU bounce-ray generator against the oracle's (cosf/sinf may
-    # differ in the last ulp between the device and host libm).
+    # The GPU bounce-ray generator against the oracle's.
     _, rng = oracle_mod.camera_rays(scene1.camera, w, h, 0, mipt_mod.host_random_number(0))
     want_bounce, want_rng = oracle_mod.bounce_rays(scene1.vertices, scene1.indices, rays, hits, rng)
     assert np.array_equal(bounce["last_prim"], want_bounce["last_prim"])
     assert _same_bits(bounce, want_bounce, "o")
-    assert np.abs(bounce["d"] - want_bounce["d"]).max() <= 2e-6
+    assert _same_bits(bounce, want_bounce, "d")
     assert np.array_equal(mipt_mod.from_device(d_rng, np.uint32), want_rng)
     ctx.close()
 

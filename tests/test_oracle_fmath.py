@@ -1,0 +1,74 @@
+"""Accuracy of the fixed-sequence sin/cos/pow (oracle/fmath.h; the kernels
+evaluate the same sequences, so renders agree bit for bit).
+
+Each function is compared with the correctly rounded binary32 value of the
+exact result (float64 libm, then rounded): the error bound is a few ulp over
+the ranges the renderer uses, and the special values are exact."""
+import numpy as np
+import pytest
+
+
+def ulp_err(got, exact64):
+    want = exact64.astype(np.float32)
+    spacing = np.spacing(np.abs(want)).astype(np.float64)
+    return np.abs(got.astype(np.float64) - exact64) / spacing
+
+
+@pytest.mark.parametrize("name,fn", [("sin", np.sin), ("cos", np.cos)])
+def test_sincos_ulp(oracle_mod, name, fn):
+    rng = np.random.default_rng(1)
+    # the renderer's range: 2 pi u for u in [0, 1), plus wider arguments
+    x = np.concatenate([np.linspace(0, 2 * np.pi, 200001), rng.uniform(-1000, 1000, 200000),
+                        rng.uniform(-1e-3, 1e-3, 1000)]).astype(np.float32)
+    got = oracle_mod.fmath(name, x)
+    exact = fn(x.astype(np.float64))
+    # absolute error near the zeros, relative (ulp) elsewhere
+    err = np.abs(got.astype(np.float64) - exact)
+    ok = (ulp_err(got, exact) <= 4) | (err <= 1.2e-7)
+    assert ok.all(), (x[~ok][:5], got[~ok][:5], exact[~ok][:5])
+    assert (np.abs(got) <= 1.0).all()
+
+
+def test_sincos_special_values(oracle_mod):
+    assert oracle_mod.fmath("sin", [0.0])[0] == 0.0
+    assert oracle_mod.fmath("cos", [0.0])[0] == 1.0
+    s = oracle_mod.fmath("sin", np.float32([np.pi / 2, np.pi, 3 * np.pi / 2]))
+    assert abs(s[0] - 1) < 1e-7 and abs(s[1]) < 2e-7 and abs(s[2] + 1) < 1e-7
+    # sin^2 + cos^2 = 1 to rounding
+    x = np.linspace(0, 6.3, 5001, dtype=np.float32)
+    s, c = oracle_mod.fmath("sin", x), oracle_mod.fmath("cos", x)
+    assert np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1).max() < 3e-7
+
+
+def test_log2_exp2_ulp(oracle_mod):
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(1e-6, 1, 100000), rng.uniform(1, 1e6, 100000)]).astype(np.float32)
+    got = oracle_mod.fmath("log2", x)
+    exact = np.log2(x.astype(np.float64))
+    ok = (ulp_err(got, exact) <= 4) | (np.abs(got - exact) <= 2e-7)
+    assert ok.all()
+    assert oracle_mod.fmath("log2", np.float32([1.0, 2.0, 0.5, 1024.0])).tolist() == [0.0, 1.0, -1.0, 10.0]
+    v = rng.uniform(-100, 100, 100000).astype(np.float32)
+    got = oracle_mod.fmath("exp2", v)
+    assert (ulp_err(got, np.exp2(v.astype(np.float64))) <= 4).all()
+    assert oracle_mod.fmath("exp2", np.float32([0.0, 1.0, -3.0])).tolist() == [1.0, 2.0, 0.125]
+
+
+@pytest.mark.parametrize("y", [0.0, 1.0, 2.0, 5.0, 7.0, -2.0, 0.5, 1.7, 3.3, 4.999])
+def test_pow_ulp(oracle_mod, y):
+    x = np.linspace(0, 1, 100001, dtype=np.float32)
+    if y < 0:
+        x = x[1:]
+    got = oracle_mod.fmath("pow", x, np.float32(y))
+    exact = np.power(x.astype(np.float64), y)
+    tiny = np.abs(exact) < 1e-30
+    # exp2(y log2 x) in binary32: the rounding of y log2 x scales the error by
+    # |y log2 x| (integral y is exact square-and-multiply: a few ulp)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = 1.0 + np.nan_to_num(np.abs(y * np.log2(x.astype(np.float64))), posinf=0.0)
+    ok = (ulp_err(got, exact) <= 4 * amp) | tiny
+    assert ok.all(), (x[~ok][:3], got[~ok][:3], exact[~ok][:3])
+    if y == 0.0:
+        assert (got == 1.0).all()
+    if y == 1.0:
+        assert np.array_equal(got, x)
