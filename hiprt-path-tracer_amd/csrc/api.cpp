@@ -111,6 +111,7 @@ void prepare_scene_for_render(Ctx& ctx)
     }
     upload(R.areas, areas.data(), areas.size());
     upload(R.lut_conductor, conductor_lut_data(), 128 * 128);
+    build_power_alias_table(ctx);
     R.scene_ready = true;
 }
 
@@ -144,7 +145,26 @@ void allocate_render_buffers(Ctx& ctx)
     }
     R.counters.reserve(64);
     HIP_CHECK(hipMemset(R.counters.ptr, 0, 64));
+    // The G-buffer starts in its reset state: no hit, material -1.
     HIP_CHECK(hipMemset(R.gb_prim.ptr, 0xff, 4 * n));
+    HIP_CHECK(hipMemset(R.gb_mat.ptr, 0xff, 4 * n));
+    HIP_CHECK(hipMemset(R.gb_gn.ptr, 0, 4 * n));
+    HIP_CHECK(hipMemset(R.gb_sn.ptr, 0, 4 * n));
+    HIP_CHECK(hipMemset(R.gb_pos.ptr, 0, 16 * n));
+    allocate_restir_buffers(ctx);
+    if (restir_needs_prev_gbuffer(R))
+    {
+        HIP_CHECK(hipMemset(R.pg_prim.ptr, 0xff, 4 * n));
+        HIP_CHECK(hipMemset(R.pg_mat.ptr, 0xff, 4 * n));
+        HIP_CHECK(hipMemset(R.pg_gn.ptr, 0, 4 * n));
+        HIP_CHECK(hipMemset(R.pg_sn.ptr, 0, 4 * n));
+        HIP_CHECK(hipMemset(R.pg_pos.ptr, 0, 16 * n));
+    }
+    restir_reset(ctx, nullptr);
+    R.last_output = 1;
+    R.restir_shade_buffer = 1;
+    R.temporal_output = 0;
+    HIP_CHECK(hipDeviceSynchronize());
 }
 
 } // namespace
@@ -261,6 +281,8 @@ int mipt_create(int hip_device, mipt_ctx** out)
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hip_device)) != hipSuccess)
         return create_fail(ctx, "hipDeviceGetAttribute(MultiprocessorCount)", e);
     ctx->cu_count = cus;
+    mipt_render_settings_default(&ctx->render.settings);
+    mipt_restir_di_settings_default(&ctx->render.restir);
     *out = reinterpret_cast<mipt_ctx*>(ctx);
     return MIPT_OK;
 }
@@ -463,13 +485,56 @@ void mipt_render_settings_default(mipt_render_settings* s)
     s->microfacet_regularization_min_roughness = 0.0f;
     s->do_microfacet_regularization = 1;
     s->accumulate_fp64 = 1;
+    s->direct_light_sampling_strategy = MIPT_LSS_MIS_LIGHT_BSDF;
+    s->direct_light_sampling_base_strategy = MIPT_LSS_BASE_UNIFORM;
+}
+
+void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
+{
+    if (!s)
+        return;
+    std::memset(s, 0, sizeof(*s));
+    // ReSTIRDISettings() (ReSTIRDISettings.h:58-121) and ReSTIRDIInitialCandidatesSettings (:17-19)
+    s->number_of_initial_light_candidates = 4;
+    s->number_of_initial_bsdf_candidates = 1;
+    s->do_temporal_reuse_pass = 1;
+    s->do_spatial_reuse_pass = 1;
+    s->spatial_number_of_passes = 1;
+    s->spatial_reuse_radius = 16;
+    s->spatial_reuse_neighbor_count = 5;
+    s->m_cap = 3;
+    s->use_confidence_weights = 1;
+    s->use_normal_similarity_heuristic = 1;
+    s->normal_similarity_angle_precomp = 0.906307787f;
+    s->use_plane_distance_heuristic = 1;
+    s->plane_distance_threshold = 0.1f;
+    s->reject_using_geometric_normals = 1;
+    s->do_final_shading_visibility = 1;
+}
+
+int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !s || s->number_of_initial_light_candidates < 0 || s->number_of_initial_bsdf_candidates < 0 ||
+        s->spatial_number_of_passes < 1 || s->spatial_reuse_radius < 0 || s->spatial_reuse_neighbor_count < 0 ||
+        s->spatial_reuse_neighbor_count > 32 || s->m_cap < 0)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        ctx->render.restir = *s;
+        if (ctx->render.configured)
+            mipt::allocate_render_buffers(*ctx);
+    });
 }
 
 int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
     if (!ctx || !s || s->width <= 0 || s->height <= 0 || s->nb_bounces < 0 ||
-        (int64_t)s->width * (int64_t)s->height > (1ll << 28))
+        (int64_t)s->width * (int64_t)s->height > (1ll << 28) ||
+        (s->direct_light_sampling_strategy != MIPT_LSS_MIS_LIGHT_BSDF &&
+         s->direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI) ||
+        (s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_UNIFORM &&
+         s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_POWER))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         ctx->render.settings = *s;
@@ -493,7 +558,10 @@ int mipt_set_camera(mipt_ctx* c, const mipt_camera* cam)
 int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t random_number, void* stream)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
-    if (!ctx || (pass != MIPT_PASS_CAMERA_RAYS && pass != MIPT_PASS_PATH_TRACE))
+    const bool spatial = pass >= MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0) && pass < MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(64);
+    if (!ctx || (pass != MIPT_PASS_CAMERA_RAYS && pass != MIPT_PASS_PATH_TRACE &&
+                 pass != MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES && pass != MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE &&
+                 !spatial))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         if (!ctx->render.configured)
@@ -504,19 +572,112 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
         if (pass == MIPT_PASS_CAMERA_RAYS)
             mipt::camera_rays_pass(*ctx, sample_number, random_number, s);
-        else
+        else if (pass == MIPT_PASS_PATH_TRACE)
             mipt::path_trace_pass(*ctx, sample_number, random_number, s);
+        else if (pass == MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES)
+            mipt::restir_initial_pass(*ctx, sample_number, random_number, s);
+        else if (pass == MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE)
+            mipt::restir_temporal_pass(*ctx, sample_number, random_number, s);
+        else
+            mipt::restir_spatial_pass(*ctx, pass - MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0), sample_number, random_number,
+                                      s);
     });
+}
+
+int32_t mipt_draws_per_sample(const mipt_ctx* c)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    const mipt::RenderState& R = ctx->render;
+    int32_t draws = 2;
+    if (R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
+    {
+        draws += 1;
+        if (R.restir.do_temporal_reuse_pass)
+            draws += 2;
+        if (R.restir.do_spatial_reuse_pass)
+            draws += R.restir.spatial_number_of_passes;
+    }
+    return draws;
 }
 
 int mipt_render_sample(mipt_ctx* c, uint32_t sample_number, void* stream)
 {
-    int rc = mipt_launch_pass(c, MIPT_PASS_CAMERA_RAYS, sample_number, mipt_host_random_number(2 * sample_number),
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    const uint32_t draws = (uint32_t)mipt_draws_per_sample(c);
+    uint32_t k = sample_number * draws;
+    int rc = mipt_launch_pass(c, MIPT_PASS_CAMERA_RAYS, sample_number, mipt_host_random_number(k++), stream);
+    const mipt::RenderState& R = ctx->render;
+    if (rc == MIPT_OK && R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
+    {
+        rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, sample_number, mipt_host_random_number(k++),
                               stream);
+        if (rc == MIPT_OK && R.restir.do_temporal_reuse_pass)
+        {
+            // the second draw is the permutation-sampling bits, unused here
+            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, sample_number, mipt_host_random_number(k),
+                                  stream);
+            k += 2;
+        }
+        for (int32_t i = 0; rc == MIPT_OK && R.restir.do_spatial_reuse_pass && i < R.restir.spatial_number_of_passes;
+             i++)
+            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i), sample_number,
+                                  mipt_host_random_number(k++), stream);
+    }
     if (rc != MIPT_OK)
         return rc;
-    return mipt_launch_pass(c, MIPT_PASS_PATH_TRACE, sample_number, mipt_host_random_number(2 * sample_number + 1),
-                            stream);
+    return mipt_launch_pass(c, MIPT_PASS_PATH_TRACE, sample_number, mipt_host_random_number(k), stream);
+}
+
+int mipt_get_power_alias_table(mipt_ctx* c, float* probas, int32_t* alias, float* sum)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->render.scene_ready)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no alias table: call mipt_set_render_settings first");
+        const mipt::RenderState& R = ctx->render;
+        if (probas)
+            std::memcpy(probas, R.alias_probas_host.data(), sizeof(float) * R.alias_probas_host.size());
+        if (alias)
+            std::memcpy(alias, R.alias_alias_host.data(), sizeof(int32_t) * R.alias_alias_host.size());
+        if (sum)
+            *sum = R.alias_sum;
+    });
+}
+
+int mipt_compute_alias_table(const float* weights, int32_t n, float total_sum, float* out_probas, int32_t* out_alias)
+{
+    if (n < 0 || (n > 0 && (!weights || !out_probas || !out_alias)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    try
+    {
+        mipt::compute_alias_table(weights, n, total_sum, out_probas, out_alias);
+    }
+    catch (const std::bad_alloc&)
+    {
+        return MIPT_ERR_OUT_OF_MEMORY;
+    }
+    return MIPT_OK;
+}
+
+int mipt_read_restir_reservoirs(mipt_ctx* c, int32_t which, void* host_out)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !host_out || which < 0 || which > 3)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::RenderState& R = ctx->render;
+        const int32_t b = which == 3 ? R.restir_shade_buffer : which;
+        if (!R.configured || !R.reservoirs[b].ptr)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no ReSTIR DI reservoirs: the strategy is not MIPT_LSS_RESTIR_DI");
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(host_out, R.reservoirs[b].ptr, 32 * (size_t)R.pixels, hipMemcpyDeviceToHost));
+    });
 }
 
 int mipt_read_framebuffer(mipt_ctx* c, float* host_rgb, double* host_rgb64)

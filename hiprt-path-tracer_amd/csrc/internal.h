@@ -78,6 +78,21 @@ struct RenderState {
     DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
     // framebuffer
     DeviceBuffer fb, fb64, counters;
+    // power alias table of the emissive triangles (LSS_BASE_POWER)
+    DeviceBuffer alias_probas, alias_alias;
+    std::vector<float> alias_probas_host;
+    std::vector<int32_t> alias_alias_host;
+    float alias_sum = 0.0f;
+    // ReSTIR DI (LSS_RESTIR_DI)
+    mipt_restir_di_settings restir{};
+    DeviceBuffer pg_prim, pg_pos, pg_gn, pg_sn, pg_mat; // previous-frame G-buffer
+    DeviceBuffer reservoirs[3];   // initial candidates, spatial output 1, spatial output 2
+    int32_t last_output = 1;      // m_last_restir_output_reservoirs (ReSTIRDIRenderPass.cpp:251)
+    int32_t restir_shade_buffer = 1; // restir_output_reservoirs of the current frame
+    int32_t temporal_output = 0;  // buffer the temporal pass of this frame wrote
+    bool odd_frame = false;
+    // ReSTIR DI pass scratch
+    DeviceBuffer rs_state, rs_rays, rs_occ, rs_hits, rs_aux;
 };
 
 struct Ctx {
@@ -119,6 +134,22 @@ const float* conductor_lut_data(); // 128 x 128, luts.cpp
 // pathtrace.hip
 void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
 void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+float consistent_tau(float tau_0, uint32_t sample_number);
+
+// lights.cpp: Utils::compute_alias_table (src/Utils/Utils.cpp:63-131) and the
+// emissive power list (GPURenderer.cpp:221-295).
+void compute_alias_table(const float* weights, int32_t n, float total_sum, float* out_probas, int32_t* out_alias);
+void build_power_alias_table(Ctx& ctx);
+
+// restir.hip: the ReSTIR DI passes and their buffers.
+bool restir_needs_prev_gbuffer(const RenderState& R);
+void allocate_restir_buffers(Ctx& ctx);
+void restir_reset(Ctx& ctx, hipStream_t stream);
+float4* restir_reservoir_buffer(Ctx& ctx, int32_t which);
+void restir_initial_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+void restir_temporal_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+void restir_spatial_pass(Ctx& ctx, int32_t pass_index, uint32_t sample_number, uint32_t random_number,
+                         hipStream_t stream);
 
 // bvh_build.hip
 void build_lbvh(Ctx& ctx);

@@ -2,11 +2,12 @@
 // the wavefront design:
 //
 //   CAMERA_RAYS  (CameraRays, src/Device/kernels/CameraRays.h:83-213)
-//       k_camera_rays -> closest-hit traversal -> k_gbuffer_write
+//       [reset at sample 0] -> k_camera_rays -> closest-hit traversal -> k_gbuffer_write
 //   PATH_TRACE   (MegaKernel, src/Device/kernels/Megakernel.h:23-144)
 //       k_path_init, then per bounce:
-//         k_shade   : NEE light sample + MIS BSDF sample + continuation sample
-//                     and Russian roulette (all RNG draws, in the reference's order)
+//         k_shade   : direct light (NEE light sample + MIS BSDF sample, or the
+//                     ReSTIR DI reservoir at bounce 0), continuation sample and
+//                     Russian roulette (all RNG draws, in the reference's order)
 //         traversal : any hit (shadow rays), closest hit (BSDF-to-light rays,
 //                     continuation rays)
 //         k_resolve : combines the ray results into the path radiance
@@ -22,14 +23,10 @@
 #include <cmath>
 #include <vector>
 
-#include "internal.h"
-#include "shading.h"
+#include "pt_device.h"
 
 namespace mipt {
 namespace {
-
-using pt::Color;
-using pt::f3;
 
 constexpr uint32_t FLAG_ALIVE = 1u;      // path is at a surface vertex
 constexpr uint32_t FLAG_NEE_LIGHT = 2u;  // shadow ray pending with a contribution
@@ -37,124 +34,48 @@ constexpr uint32_t FLAG_NEE_BSDF = 4u;   // BSDF-to-light ray pending
 constexpr uint32_t FLAG_CONTINUE = 8u;   // continuation ray traced this bounce
 constexpr uint32_t FLAG_DIRECT = 16u;    // direct lighting estimated this bounce
 
-struct PtData {
-    // scene
-    const float* verts;
-    const int32_t* idx;
-    const int32_t* mat_idx;
-    const pt::Material* mats;
-    const int32_t* emissive;
-    const float* areas;
-    int32_t emissive_count;
-    // camera
-    float cam_pos[3];
-    // settings
-    int32_t width, height;
-    int32_t nb_bounces;
-    int32_t do_rr, rr_min_depth;
-    float rr_clamp;
-    int32_t freeze, enable_direct;
-    uint32_t sample_number;
-    uint32_t random_number;
-    pt::BsdfGlobals g;
-    // G-buffer
-    int32_t* gb_prim;
-    float4* gb_pos;
-    uint32_t* gb_gn;
-    uint32_t* gb_sn;
-    int32_t* gb_mat;
-    // path state
-    float4* throughput;   // xyz
-    float4* radiance;     // xyz = ray_color
-    float4* vtx_pos;      // xyz = inter point, w = accumulated roughness
-    float4* vtx_sn;       // xyz = shading normal, w = int bits of prim
-    float4* vtx_gn;       // xyz = geometric normal, w = int bits of material
-    float4* vtx_view;     // xyz = view direction (towards the viewer)
-    float4* nee_light;    // xyz = pending light-sample contribution
-    float4* nee_bsdf;     // xyz = bsdf_color * cos, w = bsdf pdf
-    uint32_t* rng;
-    uint32_t* flags;
-    // rays of the current bounce
-    float4* shadow_rays;
-    uint8_t* shadow_occ;
-    float4* bsdf_rays;
-    int4* bsdf_hits;
-    float4* cont_rays;
-    int4* cont_hits;
-    // output
-    float* fb;        // rgb sum
-    double* fb64;     // rgb sum (may be null)
-    uint32_t* rejected;
+// ---------------------------------------------------------------------------
+// CAMERA_RAYS
+// ---------------------------------------------------------------------------
+struct GBufferView {
+    int32_t* prim;
+    float4* pos;
+    uint32_t* gn;
+    uint32_t* sn;
+    int32_t* mat;
 };
 
-__device__ __forceinline__ f3 ld3(float4 v) { return pt::mk(v.x, v.y, v.z); }
-__device__ __forceinline__ float4 st3(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
-__device__ __forceinline__ Color ldc(float4 v) { return pt::col(v.x, v.y, v.z); }
-__device__ __forceinline__ float4 stc(Color c, float w) { return make_float4(c.r, c.g, c.b, w); }
-
-__device__ __forceinline__ void tri_verts(const PtData& d, int32_t prim, f3& a, f3& b, f3& c)
+// reset_render (CameraRays.h:20-59): the G-buffers go back to "no hit". The
+// reset normal bits are 0 and the reset material index -1; neither is read
+// before a hit overwrites it.
+__global__ void k_gbuffer_reset(GBufferView g, uint32_t n)
 {
-    const int32_t* t = d.idx + 3 * (int64_t)prim;
-    const float* v0 = d.verts + 3 * (int64_t)t[0];
-    const float* v1 = d.verts + 3 * (int64_t)t[1];
-    const float* v2 = d.verts + 3 * (int64_t)t[2];
-    a = pt::mk(v0[0], v0[1], v0[2]);
-    b = pt::mk(v1[0], v1[1], v1[2]);
-    c = pt::mk(v2[0], v2[1], v2[2]);
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= n)
+        return;
+    g.prim[pix] = -1;
+    g.gn[pix] = 0u;
+    g.sn[pix] = 0u;
+    g.pos[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    g.mat[pix] = -1;
 }
 
-__device__ __forceinline__ void write_inactive(float4* rays, uint32_t i)
-{
-    rays[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
-    rays[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
-}
-
-__device__ __forceinline__ void write_ray(float4* rays, uint32_t i, f3 o, int32_t last, f3 dir, float tmax)
-{
-    rays[2 * (size_t)i] = make_float4(o.x, o.y, o.z, __int_as_float(last));
-    rays[2 * (size_t)i + 1] = make_float4(dir.x, dir.y, dir.z, tmax);
-}
-
-// Intersect.h:142-176
-__device__ __forceinline__ void fix_backfacing(f3 view, f3& gn, f3& sn)
-{
-    if (pt::dot(view, gn) < 0.0f)
-    {
-        gn = -1.0f * gn;
-        sn = -1.0f * sn;
-    }
-    if (pt::dot(view, sn) < 0.0f)
-        sn = -1.0f * sn;
-    f3 refl = pt::reflect_ray(view, sn);
-    if (pt::dot(refl, gn) <= 0.0f)
-    {
-        const float eps = 0.01f;
-        refl = refl - pt::normalize((pt::dot(refl, gn) - eps) * gn);
-        sn = pt::normalize(view + refl);
-    }
-}
-
-// Hit attributes of trace_main_path_ray for meshes without vertex normals
-// (Intersect.h:222-245).
-__device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, int32_t prim, float t, f3& p, f3& gn,
-                                               f3& sn)
-{
-    f3 a, b, c;
-    tri_verts(d, prim, a, b, c);
-    p = o + t * dir;
-    gn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
-    sn = gn;
-    fix_backfacing(-1.0f * dir, gn, sn);
-}
-
-// ---------------------------------------------------------------------------
-// CAMERA_RAYS: G-buffer write after the closest hit (CameraRays.h:176-201).
-// ---------------------------------------------------------------------------
-__global__ void k_gbuffer_write(PtData d, const float4* __restrict__ rays, const int4* __restrict__ hits)
+// CameraRays.h:117-124 (previous-frame copy) and :176-201 (G-buffer write
+// after the closest hit).
+__global__ void k_gbuffer_write(PtData d, const float4* __restrict__ rays, const int4* __restrict__ hits,
+                                GBufferView prev, int32_t copy_prev)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= (uint32_t)(d.width * d.height))
         return;
+    if (copy_prev)
+    {
+        prev.gn[pix] = d.gb_gn[pix];
+        prev.sn[pix] = d.gb_sn[pix];
+        prev.mat[pix] = d.gb_mat[pix];
+        prev.pos[pix] = d.gb_pos[pix];
+        prev.prim[pix] = d.gb_prim[pix];
+    }
     const float4 ra = rays[2 * (size_t)pix], rb = rays[2 * (size_t)pix + 1];
     const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
     const int4 h = hits[pix];
@@ -192,8 +113,8 @@ __global__ void k_path_init(PtData d)
         return;
     }
     const f3 p = ld3(d.gb_pos[pix]);
-    const f3 gn = pt::normalize(pt::oct_unpack(d.gb_gn[pix]));
-    const f3 sn = pt::normalize(pt::oct_unpack(d.gb_sn[pix]));
+    const f3 gn = gb_normal(d.gb_gn[pix]);
+    const f3 sn = gb_normal(d.gb_sn[pix]);
     const f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
     // GBufferDevice.h:19-22: view = normalize(camera - hit); the path ray is
     // normalize(-view) and the BSDF view direction is -ray.direction.
@@ -205,8 +126,43 @@ __global__ void k_path_init(PtData d)
     d.flags[pix] = FLAG_ALIVE;
 }
 
-// Lights.h:123-226 (NEE with MIS, uniform emissive-triangle sampling) and
-// PathTracing.h:81-101 (next bounce + Russian roulette).
+// evaluate_ReSTIR_DI_reservoir (ReSTIR/DI/FinalShading.h:16-101) without the
+// shadow ray: returns the contribution and whether a shadow ray must decide it.
+__device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t pix, const pt::Material& m, f3 p, f3 sn,
+                                                      f3 gn, f3 view, bool& need_shadow, f3& dir_out, float& dist_out)
+{
+    need_shadow = false;
+    const float4 ra = d.restir_out[2 * (size_t)pix], rb = d.restir_out[2 * (size_t)pix + 1];
+    const float ucw = rb.z;
+    const int32_t tri = __float_as_int(rb.y);
+    // A NaN weight sum leaves UCW = NaN with no sample selected (triangle -1):
+    // nothing to shade.
+    if (ucw <= 0.0f || tri < 0)
+        return pt::col(0.0f);
+    const uint32_t flags = (uint32_t)__float_as_int(rb.w);
+    const f3 to = ld3(ra) - p;
+    const float dist = sqrtf(pt::dot(to, to));
+    const f3 dir = pt::vdiv(to, dist);
+    dir_out = dir;
+    dist_out = dist;
+    if (!(flags & pt::RESTIR_FLAG_UNOCCLUDED) && d.final_shading_visibility)
+        need_shadow = true;
+    pt::BsdfCtx c{view, sn, gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::REGULARIZATION_MIS, 0.0f};
+    float bpdf;
+    const Color f = pt::principled_eval(d.g, m, c, bpdf);
+    float cosine = pt::dot(sn, dir);
+    if (flags & pt::FROM_GLASS_REFRACT)
+        cosine = fabsf(cosine);
+    if (!(cosine > 0.0f))
+        return pt::col(0.0f);
+    const Color emission = d.mats[d.mat_idx[tri]].emission;
+    float conv = cos_at_light(pt::normalize(tri_normal_unnormalized(d, tri)), -1.0f * dir);
+    conv /= dist * dist;
+    return (((f * ucw) * emission) * cosine) * conv;
+}
+
+// Lights.h:123-226 (NEE with MIS), Lights.h:261-294 (ReSTIR DI at bounce 0)
+// and PathTracing.h:81-101 (next bounce + Russian roulette).
 __global__ void k_shade(PtData d, int32_t bounce)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -236,39 +192,29 @@ __global__ void k_shade(PtData d, int32_t bounce)
     }
     // Without emissive triangles sample_emissive_geometry returns before any
     // draw (Lights.h:313-321): neither the light nor the MIS BSDF sample runs.
-    if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0)
+    if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0 && d.restir && bounce == 0)
     {
-        // ---- light sample (LightUtils.h:248-277, :142-199)
+        bool need_shadow;
+        f3 dir;
+        float dist;
+        const Color contrib = restir_final_shading(d, pix, m, p, sn, gn, view, need_shadow, dir, dist);
+        d.nee_light[pix] = stc(contrib, 0.0f);
+        flags |= FLAG_NEE_LIGHT;
+        if (need_shadow)
+        {
+            write_ray(d.shadow_rays, pix, p, prim, dir, dist - 1.0e-4f);
+            shadow_active = true;
+        }
+    }
+    else if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0)
+    {
+        // ---- light sample (LightUtils.h:248-309, :142-199)
         Color light_mis = pt::col(0.0f);
         {
-            const int32_t e = rng.random_index(d.emissive_count);
-            const int32_t lt = d.emissive[e];
-            f3 A, B, C;
-            tri_verts(d, lt, A, B, C);
-            float r1 = rng();
-            float r2 = rng();
-            if (r2 > r1)
+            const LightSample ls = sample_emissive(d, rng);
+            if (ls.area_pdf > 0.0f)
             {
-                r1 *= 0.5f;
-                r2 -= r1;
-            }
-            else
-            {
-                r2 *= 0.5f;
-                r1 -= r2;
-            }
-            const f3 AB = B - A, AC = C - A;
-            const f3 nrm = pt::cross(AB, AC);
-            const float len = sqrtf(pt::dot(nrm, nrm));
-            const f3 point = (A + r1 * AB) + r2 * AC;
-            const f3 lnormal = pt::vdiv(nrm, len);
-            const float area = 0.5f * len;
-            float area_pdf = 1.0f / area;
-            area_pdf /= (float)d.emissive_count;
-            const Color emission = d.mats[d.mat_idx[lt]].emission;
-            if (area_pdf > 0.0f)
-            {
-                const f3 to = point - p;
+                const f3 to = ls.point - p;
                 const float dist = sqrtf(pt::dot(to, to));
                 const f3 dirn = pt::vdiv(to, dist);
                 pt::BsdfCtx c{view, sn, gn, dirn, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness};
@@ -276,13 +222,13 @@ __global__ void k_shade(PtData d, int32_t bounce)
                 const Color f = pt::principled_eval(d.g, m, c, bsdf_pdf);
                 if (bsdf_pdf > 0.0f)
                 {
-                    const float cos_l = pt::maxf_(0.0f, pt::dot(lnormal, -1.0f * dirn));
+                    const float cos_l = pt::maxf_(0.0f, pt::dot(ls.normal, -1.0f * dirn));
                     if (cos_l > 1.0e-5f)
                     {
-                        float lpdf = cos_l < 1.0e-8f ? 0.0f : area_pdf * (dist * dist) / cos_l;
+                        float lpdf = cos_l < 1.0e-8f ? 0.0f : ls.area_pdf * (dist * dist) / cos_l;
                         const float w = lpdf == 0.0f ? 0.0f : lpdf / (1.0f * lpdf + 1.0f * bsdf_pdf);
                         const float cosine = fabsf(pt::dot(sn, dirn));
-                        light_mis = f * cosine * emission * w / lpdf / 1.0f;
+                        light_mis = f * cosine * ls.emission * w / lpdf / 1.0f;
                         write_ray(d.shadow_rays, pix, p, prim, dirn, dist - 1.0e-4f);
                         shadow_active = true;
                         flags |= FLAG_NEE_LIGHT;
@@ -360,8 +306,8 @@ __global__ void k_shade(PtData d, int32_t bounce)
     }
     if (!cont)
         write_inactive(d.cont_rays, pix);
-    // Keep the pre-update throughput for the direct-light estimate of this
-    // vertex: it was stored before the continuation changed it.
+    // The direct-light estimate of this vertex uses the pre-update throughput,
+    // snapshotted before this kernel.
     d.vtx_pos[pix] = st3(p, path_roughness);
     d.rng[pix] = rng.state;
     d.flags[pix] = flags;
@@ -398,8 +344,7 @@ __global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ t
                 const float cos_l = pt::maxf_(0.0f, pt::dot(hgn, -1.0f * dir));
                 if (!pt::is_black(emission) && cos_l > 0.0f)
                 {
-                    float pdf_area = 1.0f / d.areas[hp];
-                    pdf_area /= (float)d.emissive_count;
+                    const float pdf_area = emissive_hit_area_pdf(d, d.areas[hp], emission);
                     const float t = __int_as_float(h.y);
                     const float lpdf = cos_l < 1.0e-8f ? 0.0f : pdf_area * (t * t) / cos_l;
                     const float4 nb = d.nee_bsdf[pix];
@@ -499,6 +444,8 @@ __global__ void k_accumulate(PtData d)
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+} // namespace
+
 // MicrofacetRegularization::consistent_tau (MicrofacetRegularization.h:148-161),
 // evaluated once per pass on the host in binary32.
 float consistent_tau(float tau_0, uint32_t sample_number)
@@ -511,7 +458,7 @@ float consistent_tau(float tau_0, uint32_t sample_number)
     return 1.0f / (2.0f * pi * (1.0f - cosf(atanf(inner))));
 }
 
-PtData make_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
+PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
 {
     RenderState& R = ctx.render;
     PtData d{};
@@ -522,6 +469,10 @@ PtData make_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.emissive = ctx.emissive_prims.as<int32_t>();
     d.areas = R.areas.as<float>();
     d.emissive_count = ctx.emissive_count;
+    d.light_base = R.settings.direct_light_sampling_base_strategy;
+    d.alias_probas = R.alias_probas.as<float>();
+    d.alias_alias = R.alias_alias.as<int32_t>();
+    d.alias_sum = R.alias_sum;
     for (int k = 0; k < 3; k++) d.cam_pos[k] = R.camera.position[k];
     d.width = R.settings.width;
     d.height = R.settings.height;
@@ -559,24 +510,39 @@ PtData make_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.bsdf_hits = R.bsdf_hits.as<int4>();
     d.cont_rays = R.cont_rays.as<float4>();
     d.cont_hits = R.cont_hits.as<int4>();
+    d.restir = R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI;
+    d.final_shading_visibility = R.restir.do_final_shading_visibility;
+    d.restir_out = d.restir ? restir_reservoir_buffer(ctx, R.restir_shade_buffer) : nullptr;
     d.fb = R.fb.as<float>();
     d.fb64 = R.settings.accumulate_fp64 ? R.fb64.as<double>() : nullptr;
     d.rejected = R.counters.as<uint32_t>();
     return d;
 }
 
-} // namespace
-
 void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
 {
     RenderState& R = ctx.render;
     const int32_t W = R.settings.width, H = R.settings.height;
     const uint32_t n = (uint32_t)W * (uint32_t)H;
+    const bool keep_prev = restir_needs_prev_gbuffer(R);
+    GBufferView cur{R.gb_prim.as<int32_t>(), R.gb_pos.as<float4>(), R.gb_gn.as<uint32_t>(), R.gb_sn.as<uint32_t>(),
+                    R.gb_mat.as<int32_t>()};
+    GBufferView prev{R.pg_prim.as<int32_t>(), R.pg_pos.as<float4>(), R.pg_gn.as<uint32_t>(), R.pg_sn.as<uint32_t>(),
+                     R.pg_mat.as<int32_t>()};
+    if (sample_number == 0)
+    {
+        // need_to_reset: G-buffers and ReSTIR DI reservoirs start over.
+        hipLaunchKernelGGL(k_gbuffer_reset, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur, n);
+        if (keep_prev)
+            hipLaunchKernelGGL(k_gbuffer_reset, dim3(blocks_for(n, 256)), dim3(256), 0, s, prev, n);
+        restir_reset(ctx, s);
+    }
     camera_rays(ctx, R.camera, W, H, sample_number, random_number, R.settings.freeze_random,
                 R.cont_rays.as<mipt_ray>(), nullptr, s);
     trace_closest(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), nullptr, n, s);
-    PtData d = make_data(ctx, sample_number, random_number);
-    hipLaunchKernelGGL(k_gbuffer_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, d.cont_rays, d.cont_hits);
+    PtData d = make_pt_data(ctx, sample_number, random_number);
+    hipLaunchKernelGGL(k_gbuffer_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, d.cont_rays, d.cont_hits, prev,
+                       (int32_t)keep_prev);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -584,7 +550,7 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
 {
     RenderState& R = ctx.render;
     const uint32_t n = (uint32_t)R.settings.width * (uint32_t)R.settings.height;
-    PtData d = make_data(ctx, sample_number, random_number);
+    PtData d = make_pt_data(ctx, sample_number, random_number);
     const unsigned g = blocks_for(n, 256);
     hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d);
     for (int32_t bounce = 0; bounce <= R.settings.nb_bounces; bounce++)

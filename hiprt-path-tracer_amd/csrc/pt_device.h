@@ -1,0 +1,238 @@
+// pt_device.h -- the per-pass kernel argument (the HIPRTRenderData subset the
+// passes read) and the surface / light-sampling device functions shared by the
+// path-tracing and ReSTIR DI kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+#include "shading.h"
+
+namespace mipt {
+
+using pt::Color;
+using pt::f3;
+
+// Passed by value to every pass kernel, like HIPRTRenderData
+// (src/HostDeviceCommon/RenderData.h:93-129).
+struct PtData {
+    // scene
+    const float* verts;
+    const int32_t* idx;
+    const int32_t* mat_idx;
+    const pt::Material* mats;
+    const int32_t* emissive;
+    const float* areas;
+    int32_t emissive_count;
+    // light sampling (LSS_BASE_UNIFORM / LSS_BASE_POWER)
+    int32_t light_base;
+    const float* alias_probas;
+    const int32_t* alias_alias;
+    float alias_sum;
+    // camera
+    float cam_pos[3];
+    // settings
+    int32_t width, height;
+    int32_t nb_bounces;
+    int32_t do_rr, rr_min_depth;
+    float rr_clamp;
+    int32_t freeze, enable_direct;
+    uint32_t sample_number;
+    uint32_t random_number;
+    pt::BsdfGlobals g;
+    // G-buffer
+    int32_t* gb_prim;
+    float4* gb_pos;
+    uint32_t* gb_gn;
+    uint32_t* gb_sn;
+    int32_t* gb_mat;
+    // path state
+    float4* throughput;   // xyz
+    float4* radiance;     // xyz = ray_color
+    float4* vtx_pos;      // xyz = inter point, w = accumulated roughness
+    float4* vtx_sn;       // xyz = shading normal, w = int bits of prim
+    float4* vtx_gn;       // xyz = geometric normal, w = int bits of material
+    float4* vtx_view;     // xyz = view direction (towards the viewer)
+    float4* nee_light;    // xyz = pending light-sample contribution
+    float4* nee_bsdf;     // xyz = bsdf_color * cos, w = bsdf pdf
+    uint32_t* rng;
+    uint32_t* flags;
+    // rays of the current bounce
+    float4* shadow_rays;
+    uint8_t* shadow_occ;
+    float4* bsdf_rays;
+    int4* bsdf_hits;
+    float4* cont_rays;
+    int4* cont_hits;
+    // ReSTIR DI at bounce 0 (LSS_RESTIR_DI): the reservoirs to shade
+    int32_t restir;
+    int32_t final_shading_visibility;
+    const float4* restir_out; // 2 float4 per pixel
+    // output
+    float* fb;        // rgb sum
+    double* fb64;     // rgb sum (may be null)
+    uint32_t* rejected;
+};
+
+PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number);
+
+__device__ __forceinline__ f3 ld3(float4 v) { return pt::mk(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 st3(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ Color ldc(float4 v) { return pt::col(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 stc(Color c, float w) { return make_float4(c.r, c.g, c.b, w); }
+
+__device__ __forceinline__ void tri_verts(const PtData& d, int32_t prim, f3& a, f3& b, f3& c)
+{
+    const int32_t* t = d.idx + 3 * (int64_t)prim;
+    const float* v0 = d.verts + 3 * (int64_t)t[0];
+    const float* v1 = d.verts + 3 * (int64_t)t[1];
+    const float* v2 = d.verts + 3 * (int64_t)t[2];
+    a = pt::mk(v0[0], v0[1], v0[2]);
+    b = pt::mk(v1[0], v1[1], v1[2]);
+    c = pt::mk(v2[0], v2[1], v2[2]);
+}
+
+// get_triangle_normal_not_normalized (LightUtils.h:19-31)
+__device__ __forceinline__ f3 tri_normal_unnormalized(const PtData& d, int32_t prim)
+{
+    f3 a, b, c;
+    tri_verts(d, prim, a, b, c);
+    return pt::cross(b - a, c - a);
+}
+
+// Ray slots: float4 (origin, last_prim bits), float4 (direction, tmax).
+__device__ __forceinline__ void write_inactive(float4* rays, size_t i)
+{
+    rays[2 * i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    rays[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+}
+
+__device__ __forceinline__ void write_ray(float4* rays, size_t i, f3 o, int32_t last, f3 dir, float tmax)
+{
+    rays[2 * i] = make_float4(o.x, o.y, o.z, __int_as_float(last));
+    rays[2 * i + 1] = make_float4(dir.x, dir.y, dir.z, tmax);
+}
+
+// Intersect.h:142-176
+__device__ __forceinline__ void fix_backfacing(f3 view, f3& gn, f3& sn)
+{
+    if (pt::dot(view, gn) < 0.0f)
+    {
+        gn = -1.0f * gn;
+        sn = -1.0f * sn;
+    }
+    if (pt::dot(view, sn) < 0.0f)
+        sn = -1.0f * sn;
+    f3 refl = pt::reflect_ray(view, sn);
+    if (pt::dot(refl, gn) <= 0.0f)
+    {
+        const float eps = 0.01f;
+        refl = refl - pt::normalize((pt::dot(refl, gn) - eps) * gn);
+        sn = pt::normalize(view + refl);
+    }
+}
+
+// Hit attributes of trace_main_path_ray for meshes without vertex normals
+// (Intersect.h:222-245).
+__device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, int32_t prim, float t, f3& p, f3& gn,
+                                               f3& sn)
+{
+    f3 a, b, c;
+    tri_verts(d, prim, a, b, c);
+    p = o + t * dir;
+    gn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
+    sn = gn;
+    fix_backfacing(-1.0f * dir, gn, sn);
+}
+
+// Octahedral24BitNormal::unpack (Packing.h:237-360): decode then normalize.
+__device__ __forceinline__ f3 gb_normal(uint32_t p) { return pt::normalize(pt::oct_unpack(p)); }
+
+// ColorRGB32F::luminance (Color.h:90)
+__device__ __forceinline__ float luminance(Color c) { return 0.3086f * c.r + 0.6094f * c.g + 0.0820f * c.b; }
+
+// compute_cosine_term_at_light_source (PDFConversion.h:19-30), back faces rejected.
+__device__ __forceinline__ float cos_at_light(f3 n, f3 minus_dir) { return pt::maxf_(0.0f, pt::dot(n, minus_dir)); }
+
+// area_to_solid_angle_pdf (PDFConversion.h:32-38)
+__device__ __forceinline__ float area_to_solid_angle(float area_pdf, float dist, float cos_l)
+{
+    if (cos_l < 1.0e-8f)
+        return 0.0f;
+    return area_pdf * (dist * dist) / cos_l;
+}
+
+// balance_heuristic (Sampling.h:103-116)
+__device__ __forceinline__ float balance_heuristic(float pa, float na, float pb, float nb)
+{
+    if (pa == 0.0f)
+        return 0.0f;
+    return pa / (na * pa + nb * pb);
+}
+
+// LightSampleInformation (src/HostDeviceCommon/LightSampleInformation.h:14-30)
+struct LightSample {
+    int32_t tri;
+    f3 normal;
+    float area;
+    Color emission;
+    f3 point;
+    float area_pdf;
+};
+
+// sample_one_emissive_triangle_uniform / _power (LightUtils.h:248-309) with
+// sample_point_on_generic_triangle (:163-199, Heitz 2019 square-to-triangle
+// :142-156) and DeviceAliasTable::sample (AliasTable.h:13-22). Requires
+// emissive_count > 0.
+__device__ __forceinline__ LightSample sample_emissive(const PtData& d, pt::Xorshift32& rng)
+{
+    int32_t e = rng.random_index(d.emissive_count);
+    if (d.light_base == MIPT_LSS_BASE_POWER)
+    {
+        const float p = d.alias_probas[e];
+        if (rng() > p)
+            e = d.alias_alias[e];
+    }
+    LightSample s;
+    s.tri = d.emissive[e];
+    f3 A, B, C;
+    tri_verts(d, s.tri, A, B, C);
+    float r1 = rng();
+    float r2 = rng();
+    if (r2 > r1)
+    {
+        r1 *= 0.5f;
+        r2 -= r1;
+    }
+    else
+    {
+        r2 *= 0.5f;
+        r1 -= r2;
+    }
+    const f3 AB = B - A, AC = C - A;
+    const f3 nrm = pt::cross(AB, AC);
+    const float len = sqrtf(pt::dot(nrm, nrm));
+    s.point = (A + r1 * AB) + r2 * AC;
+    s.normal = pt::vdiv(nrm, len);
+    s.area = 0.5f * len;
+    s.emission = d.mats[d.mat_idx[s.tri]].emission;
+    s.area_pdf = 1.0f / s.area;
+    if (d.light_base == MIPT_LSS_BASE_POWER)
+        s.area_pdf *= (luminance(s.emission) * s.area) / d.alias_sum;
+    else
+        s.area_pdf /= (float)d.emissive_count;
+    return s;
+}
+
+// pdf_of_emissive_triangle_hit_area_measure (LightUtils.h:51-76)
+__device__ __forceinline__ float emissive_hit_area_pdf(const PtData& d, float light_area, Color emission)
+{
+    float pdf = 1.0f / light_area;
+    if (d.light_base == MIPT_LSS_BASE_POWER)
+        pdf *= (luminance(emission) * light_area) / d.alias_sum;
+    else
+        pdf /= (float)d.emissive_count;
+    return pdf;
+}
+
+} // namespace mipt

@@ -1,0 +1,994 @@
+// restir.hip -- ReSTIR DI passes on the wavefront design.
+//
+// Each reference kernel is one per-pixel function that traces rays inline
+// (src/Device/kernels/ReSTIR/DI/InitialCandidates.h:478-538,
+// TemporalReuse.h:50-296, SpatialReuse.h:43-322). Here every ray query is a
+// batched traversal launch between two kernels, and the per-pixel state
+// (reservoir in progress, xorshift32 state) lives in a scratch buffer in
+// between, so the RNG draws happen in the reference's order:
+//
+//   INITIAL  k_ic_start (light candidates, first BSDF candidate ray)
+//            -> closest hit -> k_ic_bsdf (x number_of_initial_bsdf_candidates)
+//            -> any hit (visibility reuse) -> k_ic_finish
+//   TEMPORAL k_tp_start (target functions without visibility, 2 shadow rays)
+//            -> any hit -> k_tp_finish (pairwise MIS, two combines)
+//   SPATIAL  k_sp_start (neighbour choice and heuristics, neighbour combines,
+//            one shadow ray per neighbour for the canonical MIS weight)
+//            -> any hit -> k_sp_center (canonical combine, normalisation)
+//            -> any hit (visibility reuse) -> k_sp_finish
+//
+// Fixed kernel options (ReSTIRDIOptions.h defaults): pairwise MIS with
+// visibility in the bias correction, visibility reuse, no visibility in the
+// initial or spatial target functions, no light presampling, no envmap, the
+// camera static between frames (accumulation: the temporal neighbour is the
+// pixel itself, UtilsTemporal.h:37-41).
+//
+// Reservoir record (ReSTIRDIReservoir without weight_sum, which no later pass
+// reads): float4 (point_on_light_source, target_function),
+// float4 (M bits, emissive_triangle_index bits, UCW, flags bits).
+#include <hip/hip_runtime.h>
+
+#include "pt_device.h"
+
+namespace mipt {
+namespace {
+
+constexpr int32_t PHASE_SKIP = 0;     // no primary hit: the reference returns without writing
+constexpr int32_t PHASE_PENDING = 1;  // a BSDF candidate ray is in flight
+constexpr int32_t PHASE_DONE = 2;     // reservoir finished, visibility ray (maybe) in flight
+constexpr int32_t PHASE_COPY = 3;     // temporal: output = initial candidates
+
+struct Res {
+    int32_t M;
+    float wsum;
+    float ucw;
+    f3 point;
+    int32_t tri;
+    float target;
+    uint32_t flags;
+};
+
+__device__ __forceinline__ Res res_empty()
+{
+    Res r;
+    r.M = 0;
+    r.wsum = 0.0f;
+    r.ucw = 0.0f;
+    r.point = pt::mk(0.0f, 0.0f, 0.0f);
+    r.tri = -1;
+    r.target = 0.0f;
+    r.flags = 0u;
+    return r;
+}
+
+__device__ __forceinline__ Res res_load(const float4* buf, size_t pix)
+{
+    const float4 a = buf[2 * pix], b = buf[2 * pix + 1];
+    Res r;
+    r.point = ld3(a);
+    r.target = a.w;
+    r.M = __float_as_int(b.x);
+    r.tri = __float_as_int(b.y);
+    r.ucw = b.z;
+    r.flags = (uint32_t)__float_as_int(b.w);
+    r.wsum = 0.0f;
+    return r;
+}
+
+__device__ __forceinline__ void res_store(float4* buf, size_t pix, const Res& r)
+{
+    buf[2 * pix] = st3(r.point, r.target);
+    buf[2 * pix + 1] = make_float4(__int_as_float(r.M), __int_as_float(r.tri), r.ucw, __int_as_float((int32_t)r.flags));
+}
+
+// Scratch state of a pixel between kernels: 4 float4.
+struct RsData {
+    float4* state;      // 4 per pixel
+    float4* rays;       // slots of 2 float4, slot k of pixel p at k * n + p
+    const uint8_t* occ; // per slot
+    const int4* hits;   // per pixel
+    float* aux;         // 2 * K per pixel (spatial)
+    const float4* in;   // reservoirs read
+    float4* out;        // reservoirs written
+    const float4* init; // initial candidates (temporal)
+    // previous-frame G-buffer (temporal)
+    const int32_t* pg_prim;
+    const float4* pg_pos;
+    const uint32_t* pg_gn;
+    const uint32_t* pg_sn;
+    const int32_t* pg_mat;
+    uint32_t n;
+    mipt_restir_di_settings s;
+    int32_t pass_index;
+    int32_t kill_after; // spatial: visibility reuse after this pass
+};
+
+__device__ __forceinline__ void state_store(const RsData& r, uint32_t pix, const Res& res, uint32_t rng, int32_t phase,
+                                            float x0, float x1, uint32_t x2)
+{
+    float4* s = r.state + 4 * (size_t)pix;
+    s[0] = st3(res.point, res.target);
+    s[1] = make_float4(__int_as_float(res.M), __int_as_float(res.tri), res.wsum, __int_as_float((int32_t)res.flags));
+    s[2] = make_float4(res.ucw, __int_as_float((int32_t)rng), __int_as_float(phase), x0);
+    s[3] = make_float4(x1, __int_as_float((int32_t)x2), 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ void state_load(const RsData& r, uint32_t pix, Res& res, uint32_t& rng, int32_t& phase,
+                                           float& x0, float& x1, uint32_t& x2)
+{
+    const float4* s = r.state + 4 * (size_t)pix;
+    const float4 a = s[0], b = s[1], c = s[2], d = s[3];
+    res.point = ld3(a);
+    res.target = a.w;
+    res.M = __float_as_int(b.x);
+    res.tri = __float_as_int(b.y);
+    res.wsum = b.z;
+    res.flags = (uint32_t)__float_as_int(b.w);
+    res.ucw = c.x;
+    rng = (uint32_t)__float_as_int(c.y);
+    phase = __float_as_int(c.z);
+    x0 = c.w;
+    x1 = d.x;
+    x2 = (uint32_t)__float_as_int(d.y);
+}
+
+// ReSTIRDIReservoir::add_one_candidate (DI/Reservoir.h:54-61)
+__device__ __forceinline__ void add_candidate(Res& r, f3 point, int32_t tri, float target, uint32_t flags, float weight,
+                                              pt::Xorshift32& rng)
+{
+    r.M++;
+    r.wsum += weight;
+    if (rng() < weight / r.wsum)
+    {
+        r.point = point;
+        r.tri = tri;
+        r.target = target;
+        r.flags = flags;
+    }
+}
+
+// combine_with (DI/Reservoir.h:77-101)
+__device__ __forceinline__ bool combine_with(Res& r, const Res& other, float mis, float target, float jacobian,
+                                             pt::Xorshift32& rng)
+{
+    if (other.ucw <= 0.0f)
+    {
+        r.M += other.M;
+        return false;
+    }
+    const float w = mis * target * other.ucw * jacobian;
+    r.M += other.M;
+    r.wsum += w;
+    if (rng() < w / r.wsum)
+    {
+        r.point = other.point;
+        r.tri = other.tri;
+        r.flags = other.flags;
+        r.target = target;
+        return true;
+    }
+    return false;
+}
+
+// end (DI/Reservoir.h:103-109)
+__device__ __forceinline__ void res_end(Res& r) { r.ucw = r.wsum == 0.0f ? 0.0f : 1.0f / r.target * r.wsum; }
+
+// end_with_normalization (DI/Reservoir.h:111-121)
+__device__ __forceinline__ void res_end_normalized(Res& r, float num, float den)
+{
+    if (r.wsum == 0.0f || r.wsum < 1.0e-10f || r.wsum > 1.0e10f || den == 0.0f || num == 0.0f)
+        r.ucw = 0.0f;
+    else
+        r.ucw = 1.0f / r.target * r.wsum * num / den;
+    r.M = r.M < 1000000 ? r.M : 1000000;
+}
+
+// ReSTIRSurface (ReSTIR/Surface.h:13-41). The view direction is taken from
+// the current frame's hit position also for a previous-frame surface
+// (Surface.h:62, get_view_direction of the current G-buffer).
+struct Surface {
+    int32_t mat;
+    int32_t prim;
+    f3 view, sn, gn, p;
+};
+
+__device__ __forceinline__ Surface surface_current(const PtData& d, uint32_t pix)
+{
+    Surface s;
+    s.mat = d.gb_mat[pix];
+    s.prim = d.gb_prim[pix];
+    const f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
+    s.p = ld3(d.gb_pos[pix]);
+    s.view = pt::normalize(cam - s.p);
+    s.sn = gb_normal(d.gb_sn[pix]);
+    s.gn = gb_normal(d.gb_gn[pix]);
+    return s;
+}
+
+__device__ __forceinline__ Surface surface_previous(const PtData& d, const RsData& r, uint32_t pix)
+{
+    Surface s;
+    s.mat = r.pg_mat[pix];
+    s.prim = r.pg_prim[pix];
+    const f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
+    s.view = pt::normalize(cam - ld3(d.gb_pos[pix]));
+    s.sn = gb_normal(r.pg_sn[pix]);
+    s.gn = gb_normal(r.pg_gn[pix]);
+    s.p = ld3(r.pg_pos[pix]);
+    return s;
+}
+
+// ReSTIR_DI_evaluate_target_function<false> (DI/TargetFunction.h:49-98): the
+// luminance of f * Le * cos * G. The visibility factor, when wanted, is
+// applied by the caller from a shadow ray (origin s.p, direction dir_out,
+// tmax dist_out - 1e-4), traced only when the value is non-zero.
+__device__ __forceinline__ float target_function(const PtData& d, const Surface& s, f3 point, int32_t tri, uint32_t flags,
+                                                 f3& dir_out, float& dist_out)
+{
+    dist_out = 0.0f;
+    if (tri == -1 || s.mat < 0)
+        return 0.0f;
+    const f3 to = point - s.p;
+    const float dist = sqrtf(pt::dot(to, to));
+    const f3 dir = pt::vdiv(to, dist);
+    dir_out = dir;
+    dist_out = dist;
+    const float cosine = pt::dot(s.sn, dir);
+    if (cosine <= 0.0f)
+        return 0.0f;
+    const pt::Material m = d.mats[s.mat];
+    pt::BsdfCtx c{s.view, s.sn, s.gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::NO_REGULARIZATION, 0.0f};
+    float bpdf;
+    const Color f = pt::principled_eval(d.g, m, c, bpdf);
+    const Color emission = d.mats[d.mat_idx[tri]].emission;
+    float geometry = cos_at_light(pt::normalize(tri_normal_unnormalized(d, tri)), -1.0f * dir);
+    geometry /= dist * dist;
+    return luminance(((f * emission) * cosine) * geometry);
+}
+
+// target with visibility from a traced shadow ray; NaN stays NaN as in
+// `target *= visible`.
+__device__ __forceinline__ float apply_visibility(float target, bool occluded) { return target * (occluded ? 0.0f : 1.0f); }
+
+__device__ __forceinline__ bool is_emissive(const PtData& d, int32_t mat)
+{
+    if (mat < 0)
+        return false;
+    const Color e = d.mats[mat].emission;
+    const float nz = 1.0e-10f;
+    return !(e.r < nz && e.r > -nz) || !(e.g < nz && e.g > -nz) || !(e.b < nz && e.b > -nz);
+}
+
+// ---------------------------------------------------------------------------
+// INITIAL CANDIDATES
+// ---------------------------------------------------------------------------
+
+// Samples BSDF candidates from index `cand` on until one needs a ray
+// (returns true, ray written to slot 0) or none is left (false).
+__device__ __forceinline__ bool next_bsdf_candidate(const PtData& d, const RsData& r, uint32_t pix, const Surface& s,
+                                                    const pt::Material& m, int32_t& cand, pt::Xorshift32& rng,
+                                                    float& bpdf_out, uint32_t& info_out)
+{
+    const int32_t nb = r.s.number_of_initial_bsdf_candidates;
+    while (cand < nb)
+    {
+        pt::BsdfCtx c{s.view, s.sn, s.gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f};
+        f3 dir;
+        float bpdf;
+        pt::principled_sample(d.g, m, c, dir, bpdf, rng);
+        if (bpdf > 0.0f)
+        {
+            write_ray(r.rays, pix, s.p, s.prim, dir, 1.0e35f - 1.0e-4f);
+            bpdf_out = bpdf;
+            info_out = c.info;
+            return true;
+        }
+        cand++;
+    }
+    return false;
+}
+
+// reservoir.end(), M = 1 (InitialCandidates.h:465-472), then the visibility
+// reuse test (ReSTIR/Utils.h:25-67) as a ray in slot 1.
+__device__ __forceinline__ void finish_initial(const RsData& r, uint32_t pix, const Surface& s, Res& res,
+                                               uint32_t& need_vis)
+{
+    res_end(res);
+    res.M = 1;
+    need_vis = 0u;
+    write_inactive(r.rays, (size_t)r.n + pix);
+    if (res.ucw <= 0.0f && res.wsum <= 0.0f)
+        return;
+    if (res.flags & pt::RESTIR_FLAG_UNOCCLUDED)
+        return;
+    const f3 to = res.point - s.p;
+    const float dist = sqrtf(pt::dot(to, to));
+    const f3 dir = pt::vdiv(to, dist);
+    write_ray(r.rays, (size_t)r.n + pix, s.p, s.prim, dir, dist - 1.0e-4f);
+    need_vis = 1u;
+}
+
+__global__ void k_ic_start(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    write_inactive(r.rays, pix);
+    pt::Xorshift32 rng{pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
+    if (d.gb_prim[pix] == -1)
+    {
+        write_inactive(r.rays, (size_t)r.n + pix);
+        state_store(r, pix, res_empty(), 0u, PHASE_SKIP, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const Surface s = surface_current(d, pix);
+    const pt::Material m = d.mats[s.mat];
+    Res res = res_empty();
+    const int32_t nl = r.s.number_of_initial_light_candidates;
+    const int32_t nb = r.s.number_of_initial_bsdf_candidates;
+    // sample_light_candidates (InitialCandidates.h:180-298)
+    for (int32_t i = 0; i < nl; i++)
+    {
+        float cosine = 0.0f, pdf = 0.0f, dist = 0.0f;
+        f3 dir = pt::mk(0.0f, 0.0f, 0.0f);
+        f3 point = pt::mk(0.0f, 0.0f, 0.0f);
+        int32_t tri = -1;
+        // sample_fresh_light_candidate (:99-178): envmap candidate probability 0
+        if (rng() > 0.0f)
+        {
+            const LightSample ls = sample_emissive(d, rng);
+            tri = ls.tri;
+            point = ls.point;
+            pdf = ls.area_pdf;
+            if (pdf > 0.0f)
+            {
+                const f3 to = point - s.p;
+                dist = sqrtf(pt::dot(to, to));
+                dir = pt::vdiv(to, dist);
+                cosine = pt::maxf_(0.0f, pt::dot(s.sn, dir));
+                pdf *= (1.0f - 0.0f);
+            }
+            else
+            {
+                const f3 to = point - s.p;
+                dist = sqrtf(pt::dot(to, to));
+                dir = pt::vdiv(to, dist);
+            }
+        }
+        float weight = 0.0f, sample_target = 0.0f;
+        if (cosine > 0.0f && pdf > 0.0f)
+        {
+            pt::BsdfCtx c{s.view, s.sn, s.gn, dir, pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f};
+            float bsdf_pdf;
+            pt::principled_eval(d.g, m, c, bsdf_pdf);
+            f3 tdir;
+            float tdist;
+            const float target = target_function(d, s, point, tri, 0u, tdir, tdist);
+            if (!(bsdf_pdf <= 0.0f))
+            {
+                f3 ln = tri_normal_unnormalized(d, tri);
+                const float nl_len = sqrtf(pt::dot(ln, ln));
+                ln = pt::vdiv(ln, nl_len);
+                float lpdf = area_to_solid_angle(pdf / (1.0f - 0.0f), dist, cos_at_light(ln, -1.0f * dir));
+                lpdf *= (1.0f - 0.0f);
+                const float mis = balance_heuristic(lpdf, (float)nl, bsdf_pdf, (float)nb);
+                weight = mis * target / pdf;
+                sample_target = target;
+            }
+        }
+        add_candidate(res, point, tri, sample_target, 0u, weight, rng);
+    }
+    int32_t cand = 0;
+    float bpdf = 0.0f;
+    uint32_t info = 0u;
+    if (next_bsdf_candidate(d, r, pix, s, m, cand, rng, bpdf, info))
+    {
+        write_inactive(r.rays, (size_t)r.n + pix);
+        state_store(r, pix, res, rng.state, PHASE_PENDING, bpdf, __int_as_float(cand), info);
+        return;
+    }
+    uint32_t need_vis;
+    finish_initial(r, pix, s, res, need_vis);
+    state_store(r, pix, res, rng.state, PHASE_DONE, 0.0f, 0.0f, need_vis);
+}
+
+// sample_bsdf_candidates (InitialCandidates.h:300-440) after the closest hit.
+__global__ void k_ic_bsdf(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    Res res;
+    uint32_t rng_state, info;
+    int32_t phase;
+    float bpdf, candf;
+    state_load(r, pix, res, rng_state, phase, bpdf, candf, info);
+    if (phase != PHASE_PENDING)
+        return;
+    int32_t cand = __float_as_int(candf);
+    pt::Xorshift32 rng{rng_state};
+    const Surface s = surface_current(d, pix);
+    const pt::Material m = d.mats[s.mat];
+    const int32_t nl = r.s.number_of_initial_light_candidates;
+    const int32_t nb = r.s.number_of_initial_bsdf_candidates;
+    const float4 ra = r.rays[2 * (size_t)pix], rb = r.rays[2 * (size_t)pix + 1];
+    const f3 o = ld3(ra), dir = ld3(rb);
+    const int4 h = r.hits[pix];
+    if (h.x >= 0)
+    {
+        const int32_t hp = h.x;
+        const float t = __int_as_float(h.y);
+        const Color emission = d.mats[d.mat_idx[hp]].emission;
+        if (!pt::is_black(emission))
+        {
+            const f3 point = o + t * dir;
+            const uint32_t flags = (pt::RESTIR_FLAG_UNOCCLUDED | info) & 0xffu;
+            f3 tdir;
+            float tdist;
+            const float target = target_function(d, s, point, hp, flags, tdir, tdist);
+            const f3 hgn = pt::normalize(pt::normalize(tri_normal_unnormalized(d, hp)));
+            const float cos_l = cos_at_light(hgn, -1.0f * dir);
+            float lpdf = 0.0f;
+            if (!(pt::dot(dir, s.sn) < 0.0f))
+                lpdf = area_to_solid_angle(emissive_hit_area_pdf(d, d.areas[hp], emission), t, cos_l);
+            if (!(bpdf <= 0.0f))
+            {
+                lpdf *= (1.0f - 0.0f);
+                const float mis = balance_heuristic(bpdf, (float)nb, lpdf, (float)nl);
+                float pdf_area = bpdf;
+                pdf_area /= (t * t);
+                pdf_area *= cos_l;
+                const float weight = mis * target / pdf_area;
+                add_candidate(res, point, hp, target, flags, weight, rng);
+            }
+        }
+    }
+    cand++;
+    if (next_bsdf_candidate(d, r, pix, s, m, cand, rng, bpdf, info))
+    {
+        state_store(r, pix, res, rng.state, PHASE_PENDING, bpdf, __int_as_float(cand), info);
+        return;
+    }
+    write_inactive(r.rays, pix);
+    uint32_t need_vis;
+    finish_initial(r, pix, s, res, need_vis);
+    state_store(r, pix, res, rng.state, PHASE_DONE, 0.0f, 0.0f, need_vis);
+}
+
+__global__ void k_ic_finish(RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    Res res;
+    uint32_t rng_state, need_vis;
+    int32_t phase;
+    float x0, x1;
+    state_load(r, pix, res, rng_state, phase, x0, x1, need_vis);
+    if (phase != PHASE_DONE)
+        return;
+    if (need_vis)
+    {
+        if (r.occ[(size_t)r.n + pix])
+            res.ucw = -1.0f;
+        else
+            res.flags |= pt::RESTIR_FLAG_UNOCCLUDED;
+    }
+    res_store(r.out, pix, res);
+}
+
+// ---------------------------------------------------------------------------
+// TEMPORAL REUSE (accumulation: the temporal neighbour is the pixel itself)
+// ---------------------------------------------------------------------------
+__global__ void k_tp_start(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    write_inactive(r.rays, pix);
+    write_inactive(r.rays, (size_t)r.n + pix);
+    if (d.gb_prim[pix] == -1)
+    {
+        state_store(r, pix, res_empty(), 0u, PHASE_SKIP, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const Res tn = res_load(r.in, pix);
+    if (d.freeze || tn.M == 0)
+    {
+        state_store(r, pix, res_empty(), 0u, PHASE_COPY, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const Surface prev = surface_previous(d, r, pix);
+    if (is_emissive(d, prev.mat))
+    {
+        state_store(r, pix, res_empty(), 0u, PHASE_COPY, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const Surface center = surface_current(d, pix);
+    const Res init = res_load(r.init, pix);
+    uint32_t rays = 0u;
+    float tf_center = 0.0f;
+    if (tn.ucw > 0.0f)
+    {
+        f3 dir;
+        float dist;
+        tf_center = target_function(d, center, tn.point, tn.tri, tn.flags, dir, dist);
+        if (tf_center != 0.0f)
+        {
+            write_ray(r.rays, pix, center.p, center.prim, dir, dist - 1.0e-4f);
+            rays |= 1u;
+        }
+    }
+    f3 dir;
+    float dist;
+    const float tf_csn = target_function(d, prev, init.point, init.tri, init.flags, dir, dist);
+    if (tf_csn != 0.0f)
+    {
+        write_ray(r.rays, (size_t)r.n + pix, prev.p, prev.prim, dir, dist - 1.0e-4f);
+        rays |= 2u;
+    }
+    state_store(r, pix, res_empty(), 0u, PHASE_DONE, tf_center, tf_csn, rays);
+}
+
+// TemporalReuse.h:100-296 with the pairwise MIS weights of
+// TemporalMISWeight.h:164-254.
+__global__ void k_tp_finish(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    Res unused;
+    uint32_t rng_state, rays;
+    int32_t phase;
+    float tf_center, tf_csn;
+    state_load(r, pix, unused, rng_state, phase, tf_center, tf_csn, rays);
+    if (phase == PHASE_SKIP)
+        return;
+    const Res init = res_load(r.init, pix);
+    if (phase == PHASE_COPY)
+    {
+        res_store(r.out, pix, init);
+        return;
+    }
+    const Res tn = res_load(r.in, pix);
+    if (rays & 1u)
+        tf_center = apply_visibility(tf_center, r.occ[pix] != 0);
+    if (rays & 2u)
+        tf_csn = apply_visibility(tf_csn, r.occ[(size_t)r.n + pix] != 0);
+    pt::Xorshift32 rng{pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
+    Res out = res_empty();
+    const bool cw = r.s.use_confidence_weights != 0;
+    const float tn_M = cw ? (float)tn.M : 1.0f;
+    const float center_M = cw ? (float)init.M : 1.0f;
+    const float conf_sum = cw ? (float)tn.M : 1.0f;
+    // temporal neighbour
+    const float tf_at_neighbor = tn.target;
+    const float nume = tf_at_neighbor * tn_M;
+    const float denom = tf_at_neighbor * conf_sum + tf_center * center_M;
+    const float mi = denom == 0.0f ? 0.0f : nume / denom;
+    const float tf_csc = init.target;
+    const float nume_mc = tf_csc * center_M;
+    const float denom_mc = tf_csn * conf_sum + tf_csc * center_M;
+    const float conf_mult = cw ? tn_M / conf_sum : 1.0f;
+    float mc = 0.0f;
+    if (denom_mc != 0.0f)
+        mc += nume_mc / denom_mc * conf_mult;
+    if (combine_with(out, tn, mi, tf_center, 1.0f, rng))
+        out.flags |= pt::RESTIR_FLAG_UNOCCLUDED;
+    // initial candidates
+    const float mis_init = mc == 0.0f ? 1.0f : mc;
+    if (combine_with(out, init, mis_init, init.target, 1.0f, rng))
+        out.flags |= pt::RESTIR_FLAG_UNOCCLUDED;
+    res_end_normalized(out, 1.0f, 1.0f);
+    if (r.s.m_cap > 0)
+        out.M = out.M < r.s.m_cap ? out.M : r.s.m_cap;
+    res_store(r.out, pix, out);
+}
+
+// ---------------------------------------------------------------------------
+// SPATIAL REUSE
+// ---------------------------------------------------------------------------
+
+// get_spatial_neighbor_pixel_index (UtilsSpatial.h:177-259), neighbour
+// `i` < K, uniform disk sample (Sampling.h:50-57); -1 outside the image.
+__device__ __forceinline__ int32_t spatial_neighbor(const PtData& d, const RsData& r, int32_t cx, int32_t cy,
+                                                    pt::Xorshift32& nrng)
+{
+    const float u = nrng();
+    const float v = nrng();
+    const float rad = (float)r.s.spatial_reuse_radius * sqrtf(v);
+    float sn_, cs_;
+    pt::sincos_fixed(6.28318530717958647693f * u, sn_, cs_);
+    const float ox = rad * cs_, oy = rad * sn_;
+    const int32_t nx = cx + (int32_t)roundf(ox);
+    const int32_t ny = cy + (int32_t)roundf(oy);
+    if (nx < 0 || nx >= d.width || ny < 0 || ny >= d.height)
+        return -1;
+    return nx + ny * d.width;
+}
+
+// check_neighbor_similarity_heuristics (NeighborSimilarity.h:46-107), current frame.
+__device__ __forceinline__ bool similar(const PtData& d, const RsData& r, int32_t npix, uint32_t cpix, f3 cpoint,
+                                        f3 cnormal)
+{
+    if ((uint32_t)npix == cpix)
+        return true;
+    if (d.gb_prim[npix] == -1)
+        return false;
+    const f3 np = ld3(d.gb_pos[npix]);
+    const f3 nn = r.s.reject_using_geometric_normals ? gb_normal(d.gb_gn[npix]) : gb_normal(d.gb_sn[npix]);
+    bool plane = true, normal = true;
+    if (r.s.use_plane_distance_heuristic)
+        plane = fabsf(pt::dot(np - cpoint, cnormal)) < r.s.plane_distance_threshold;
+    if (r.s.use_normal_similarity_heuristic)
+        normal = pt::dot(cnormal, nn) > r.s.normal_similarity_angle_precomp;
+    return plane && normal;
+}
+
+__global__ void k_sp_start(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    const int32_t K = r.s.spatial_reuse_neighbor_count;
+    for (int32_t k = 0; k < K; k++)
+        write_inactive(r.rays, (size_t)k * r.n + pix);
+    if (d.gb_prim[pix] == -1)
+    {
+        state_store(r, pix, res_empty(), 0u, PHASE_SKIP, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const int32_t cx = (int32_t)(pix % (uint32_t)d.width), cy = (int32_t)(pix / (uint32_t)d.width);
+    pt::Xorshift32 rng{pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
+    const Res center = res_load(r.in, pix);
+    const Surface cs = surface_current(d, pix);
+    // setup_adaptive_directional_spatial_reuse (UtilsSpatial.h:17-31): the
+    // neighbour stream is seeded from this pixel's stream.
+    const uint32_t nseed = rng.next();
+    const f3 cnormal = r.s.reject_using_geometric_normals ? cs.gn : cs.sn;
+    // count_valid_spatial_neighbors (UtilsSpatial.h:302-339)
+    int32_t valid_count = 0, valid_M = 0;
+    uint32_t cache = 0u;
+    {
+        pt::Xorshift32 nrng{nseed};
+        for (int32_t i = 0; i < K; i++)
+        {
+            const int32_t np = spatial_neighbor(d, r, cx, cy, nrng);
+            if (np == -1)
+                continue;
+            if (!similar(d, r, np, pix, cs.p, cnormal))
+                continue;
+            valid_M += __float_as_int(r.in[2 * (size_t)np + 1].x);
+            valid_count++;
+            cache |= 1u << i;
+        }
+    }
+    const bool cw = r.s.use_confidence_weights != 0;
+    const float center_M = cw ? (float)center.M : 1.0f;
+    const float conf_sum = cw ? (float)valid_M : 1.0f;
+    const float vdt = cw ? 1.0f : (float)valid_count;
+    const bool update_mc = center.M > 0 && center.ucw > 0.0f;
+    const float tf_csc = center.target;
+    const float nume_mc = tf_csc / vdt * center_M;
+    float* aux = r.aux + 2 * (size_t)K * pix;
+    for (int32_t i = 0; i < K; i++)
+    {
+        aux[2 * i] = 0.0f;
+        aux[2 * i + 1] = -1.0f;
+    }
+    Res out = res_empty();
+    pt::Xorshift32 nrng{nseed};
+    const int32_t start = valid_M == 0 ? K : 0;
+    for (int32_t i = start; i < K; i++)
+    {
+        if (!(cache & (1u << i)))
+        {
+            nrng.next();
+            nrng.next();
+            continue;
+        }
+        const int32_t np = spatial_neighbor(d, r, cx, cy, nrng);
+        if (np == -1)
+            continue;
+        const Res nres = res_load(r.in, np);
+        float tf_center = 0.0f;
+        if (nres.ucw > 0.0f)
+        {
+            f3 dir;
+            float dist;
+            tf_center = target_function(d, cs, nres.point, nres.tri, nres.flags, dir, dist);
+        }
+        // ReSTIRSpatialResamplingMISWeight<PAIRWISE_MIS> (SpatialMISWeight.h:118-234), neighbour
+        const float res_M = cw ? (float)nres.M : 1.0f;
+        const float nume = nres.target * res_M;
+        const float denom = nres.target * conf_sum + tf_center / vdt * center_M;
+        const float mi = denom == 0.0f ? 0.0f : nume / denom;
+        if (update_mc)
+        {
+            const Surface ns = surface_current(d, (uint32_t)np);
+            f3 dir;
+            float dist;
+            const float tf_csn = target_function(d, ns, center.point, center.tri, center.flags, dir, dist);
+            const float conf_mult = cw ? (conf_sum == 0.0f ? 0.0f : res_M / conf_sum) : 1.0f;
+            aux[2 * i] = tf_csn;
+            aux[2 * i + 1] = conf_mult;
+            if (tf_csn != 0.0f)
+                write_ray(r.rays, (size_t)i * r.n + pix, ns.p, ns.prim, dir, dist - 1.0e-4f);
+        }
+        if (combine_with(out, nres, mi / vdt, tf_center, 1.0f, rng))
+            out.flags &= ~pt::RESTIR_FLAG_UNOCCLUDED;
+    }
+    state_store(r, pix, out, rng.state, PHASE_PENDING, nume_mc, conf_sum, (uint32_t)valid_count);
+}
+
+__global__ void k_sp_center(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    write_inactive(r.rays, pix);
+    Res out;
+    uint32_t rng_state, valid_count;
+    int32_t phase;
+    float nume_mc, conf_sum;
+    state_load(r, pix, out, rng_state, phase, nume_mc, conf_sum, valid_count);
+    if (phase == PHASE_SKIP)
+        return;
+    const int32_t K = r.s.spatial_reuse_neighbor_count;
+    // valid_neighbor_division_term (SpatialMISWeight.h:150): 1 with
+    // confidence weights, else the number of valid neighbours.
+    const float vdt = r.s.use_confidence_weights ? 1.0f : (float)valid_count;
+    const float* aux = r.aux + 2 * (size_t)K * pix;
+    float mc = 0.0f;
+    for (int32_t i = 0; i < K; i++)
+    {
+        const float conf_mult = aux[2 * i + 1];
+        if (conf_mult < 0.0f)
+            continue;
+        float tf_csn = aux[2 * i];
+        if (tf_csn != 0.0f)
+            tf_csn = apply_visibility(tf_csn, r.occ[(size_t)i * r.n + pix] != 0);
+        const float denom_mc = tf_csn * conf_sum + nume_mc;
+        if (denom_mc != 0.0f)
+            mc += nume_mc / denom_mc / vdt * conf_mult;
+    }
+    pt::Xorshift32 rng{rng_state};
+    const Res center = res_load(r.in, pix);
+    const float tf_center = center.ucw > 0.0f ? center.target : 0.0f;
+    const float mis = mc == 0.0f ? 1.0f : mc;
+    if (combine_with(out, center, mis, tf_center, 1.0f, rng))
+        out.flags |= center.flags & pt::RESTIR_FLAG_UNOCCLUDED;
+    res_end_normalized(out, 1.0f, 1.0f);
+    uint32_t need_vis = 0u;
+    if (r.kill_after && !(out.ucw <= 0.0f && out.wsum <= 0.0f) && !(out.flags & pt::RESTIR_FLAG_UNOCCLUDED))
+    {
+        const f3 p = ld3(d.gb_pos[pix]);
+        const f3 to = out.point - p;
+        const float dist = sqrtf(pt::dot(to, to));
+        const f3 dir = pt::vdiv(to, dist);
+        write_ray(r.rays, pix, p, d.gb_prim[pix], dir, dist - 1.0e-4f);
+        need_vis = 1u;
+    }
+    state_store(r, pix, out, rng.state, PHASE_DONE, 0.0f, 0.0f, need_vis);
+}
+
+__global__ void k_sp_finish(RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    Res out;
+    uint32_t rng_state, need_vis;
+    int32_t phase;
+    float x0, x1;
+    state_load(r, pix, out, rng_state, phase, x0, x1, need_vis);
+    if (phase != PHASE_DONE)
+        return;
+    if (need_vis)
+    {
+        if (r.occ[pix])
+            out.ucw = -1.0f;
+        else
+            out.flags |= pt::RESTIR_FLAG_UNOCCLUDED;
+    }
+    if (r.s.m_cap > 0)
+        out.M = out.M < r.s.m_cap ? out.M : r.s.m_cap;
+    res_store(r.out, pix, out);
+}
+
+__global__ void k_reservoir_clear(float4* buf, uint32_t n)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix < n)
+        res_store(buf, pix, res_empty());
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+RsData make_rs_data(Ctx& ctx)
+{
+    RenderState& R = ctx.render;
+    RsData r{};
+    r.state = R.rs_state.as<float4>();
+    r.rays = R.rs_rays.as<float4>();
+    r.occ = R.rs_occ.as<uint8_t>();
+    r.hits = R.rs_hits.as<int4>();
+    r.aux = R.rs_aux.as<float>();
+    r.pg_prim = R.pg_prim.as<int32_t>();
+    r.pg_pos = R.pg_pos.as<float4>();
+    r.pg_gn = R.pg_gn.as<uint32_t>();
+    r.pg_sn = R.pg_sn.as<uint32_t>();
+    r.pg_mat = R.pg_mat.as<int32_t>();
+    r.n = R.pixels;
+    r.s = R.restir;
+    return r;
+}
+
+// configure_output_buffer (ReSTIRDIRenderPass.cpp:659-677) after the last
+// ReSTIR pass of a frame.
+void end_frame(RenderState& R, int32_t output)
+{
+    R.restir_shade_buffer = output;
+    R.last_output = output;
+    R.odd_frame = !R.odd_frame;
+}
+
+void check_ready(Ctx& ctx)
+{
+    RenderState& R = ctx.render;
+    if (R.settings.direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "ReSTIR DI passes need direct_light_sampling_strategy = MIPT_LSS_RESTIR_DI");
+    if (!R.reservoirs[0].ptr)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "ReSTIR DI buffers are not allocated");
+}
+
+} // namespace
+
+bool restir_needs_prev_gbuffer(const RenderState& R)
+{
+    return R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI && R.restir.do_temporal_reuse_pass;
+}
+
+float4* restir_reservoir_buffer(Ctx& ctx, int32_t which)
+{
+    return ctx.render.reservoirs[which].as<float4>();
+}
+
+void allocate_restir_buffers(Ctx& ctx)
+{
+    RenderState& R = ctx.render;
+    const size_t n = R.pixels;
+    if (R.settings.direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI || n == 0)
+        return;
+    const int32_t K = R.restir.spatial_reuse_neighbor_count;
+    const size_t slots = (size_t)(K > 2 ? K : 2);
+    for (int i = 0; i < 3; i++)
+        R.reservoirs[i].reserve(32 * n);
+    if (restir_needs_prev_gbuffer(R))
+    {
+        R.pg_prim.reserve(4 * n);
+        R.pg_pos.reserve(16 * n);
+        R.pg_gn.reserve(4 * n);
+        R.pg_sn.reserve(4 * n);
+        R.pg_mat.reserve(4 * n);
+    }
+    R.rs_state.reserve(64 * n);
+    R.rs_rays.reserve(32 * n * slots);
+    R.rs_occ.reserve(n * slots);
+    R.rs_hits.reserve(16 * n);
+    R.rs_aux.reserve(8 * n * (size_t)(K > 0 ? K : 1));
+}
+
+void restir_reset(Ctx& ctx, hipStream_t s)
+{
+    RenderState& R = ctx.render;
+    if (R.settings.direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI || !R.reservoirs[0].ptr)
+        return;
+    for (int i = 0; i < 3; i++)
+        hipLaunchKernelGGL(k_reservoir_clear, dim3(blocks_for(R.pixels, 256)), dim3(256), 0, s,
+                           R.reservoirs[i].as<float4>(), R.pixels);
+    HIP_CHECK(hipGetLastError());
+    R.odd_frame = false;
+}
+
+// ReSTIR_DI_InitialCandidates: output = the initial-candidates buffer.
+void restir_initial_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+{
+    check_ready(ctx);
+    RenderState& R = ctx.render;
+    const mipt_restir_di_settings& st = R.restir;
+    if (ctx.emissive_count > 0)
+    {
+        PtData d = make_pt_data(ctx, sample_number, random_number);
+        RsData r = make_rs_data(ctx);
+        r.out = R.reservoirs[0].as<float4>();
+        const unsigned g = blocks_for(r.n, 256);
+        hipLaunchKernelGGL(k_ic_start, dim3(g), dim3(256), 0, s, d, r);
+        for (int32_t j = 0; j < st.number_of_initial_bsdf_candidates; j++)
+        {
+            trace_closest(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_hits.as<mipt_hit>(), nullptr, r.n, s);
+            hipLaunchKernelGGL(k_ic_bsdf, dim3(g), dim3(256), 0, s, d, r);
+        }
+        trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays + 2 * (size_t)r.n), R.rs_occ.as<uint8_t>() + r.n, r.n,
+                  s);
+        hipLaunchKernelGGL(k_ic_finish, dim3(g), dim3(256), 0, s, r);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (!st.do_temporal_reuse_pass && !st.do_spatial_reuse_pass)
+        end_frame(R, 0);
+}
+
+// ReSTIR_DI_TemporalReuse (ReSTIRDIRenderPass.cpp:466-503 for the buffers).
+void restir_temporal_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+{
+    check_ready(ctx);
+    RenderState& R = ctx.render;
+    const mipt_restir_di_settings& st = R.restir;
+    if (!st.do_temporal_reuse_pass)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "temporal reuse is disabled in the ReSTIR DI settings");
+    const int32_t out = st.do_spatial_reuse_pass ? 0 : (R.odd_frame ? 1 : 2);
+    PtData d = make_pt_data(ctx, sample_number, random_number);
+    RsData r = make_rs_data(ctx);
+    r.in = R.reservoirs[R.last_output].as<float4>();
+    r.init = R.reservoirs[0].as<float4>();
+    r.out = R.reservoirs[out].as<float4>();
+    const unsigned g = blocks_for(r.n, 256);
+    hipLaunchKernelGGL(k_tp_start, dim3(g), dim3(256), 0, s, d, r);
+    trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_occ.as<uint8_t>(), 2 * r.n, s);
+    hipLaunchKernelGGL(k_tp_finish, dim3(g), dim3(256), 0, s, d, r);
+    HIP_CHECK(hipGetLastError());
+    R.temporal_output = out;
+    if (!st.do_spatial_reuse_pass)
+        end_frame(R, out);
+}
+
+// ReSTIR_DI_SpatialReuse (ReSTIRDIRenderPass.cpp:530-571 for the buffers).
+void restir_spatial_pass(Ctx& ctx, int32_t pass_index, uint32_t sample_number, uint32_t random_number,
+                         hipStream_t s)
+{
+    check_ready(ctx);
+    RenderState& R = ctx.render;
+    const mipt_restir_di_settings& st = R.restir;
+    if (!st.do_spatial_reuse_pass)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "spatial reuse is disabled in the ReSTIR DI settings");
+    if (pass_index < 0 || pass_index >= st.spatial_number_of_passes)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "spatial pass index out of range");
+    int32_t in, out;
+    if (pass_index == 0)
+    {
+        in = st.do_temporal_reuse_pass ? R.temporal_output : 0;
+        out = 1;
+    }
+    else if ((pass_index & 1) == 0)
+    {
+        in = 2;
+        out = 1;
+    }
+    else
+    {
+        in = 1;
+        out = 2;
+    }
+    const bool last = pass_index == st.spatial_number_of_passes - 1;
+    PtData d = make_pt_data(ctx, sample_number, random_number);
+    RsData r = make_rs_data(ctx);
+    r.in = R.reservoirs[in].as<float4>();
+    r.out = R.reservoirs[out].as<float4>();
+    r.pass_index = pass_index;
+    // SpatialReuse.h:304-314: visibility reuse unless this is the final pass
+    // of a frame without temporal reuse.
+    r.kill_after = st.do_temporal_reuse_pass || !last;
+    const unsigned g = blocks_for(r.n, 256);
+    const int32_t K = st.spatial_reuse_neighbor_count;
+    hipLaunchKernelGGL(k_sp_start, dim3(g), dim3(256), 0, s, d, r);
+    if (K > 0)
+        trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_occ.as<uint8_t>(), (uint32_t)K * r.n, s);
+    hipLaunchKernelGGL(k_sp_center, dim3(g), dim3(256), 0, s, d, r);
+    trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_occ.as<uint8_t>(), r.n, s);
+    hipLaunchKernelGGL(k_sp_finish, dim3(g), dim3(256), 0, s, r);
+    HIP_CHECK(hipGetLastError());
+    if (last)
+        end_frame(R, out);
+}
+
+} // namespace mipt

@@ -89,6 +89,13 @@ enum LobeInfo : uint32_t {
     FROM_DIFFUSE_TRANSMISSION = 1u << 8,
 };
 
+// ReSTIRDISampleFlags (src/Device/includes/ReSTIR/DI/SampleFlags.h): an 8-bit
+// field holding the sampled lobe's LobeInfo bits and, in bit 7, UNOCCLUDED
+// (which FROM_DIFFUSE shares; FROM_DIFFUSE_TRANSMISSION does not fit).
+// flags_to_BSDF_incident_light_info keeps bits 1-6 (DI/Reservoir.h:45-48).
+constexpr uint32_t RESTIR_FLAG_UNOCCLUDED = 1u << 7;
+constexpr uint32_t RESTIR_FLAG_LOBE_MASK = 0x3fu << 1;
+
 enum RegMode : uint32_t { NO_REGULARIZATION = 0, REGULARIZATION_CLASSIC = 1, REGULARIZATION_MIS = 2 };
 
 // Per-pass constants of the BSDF (BSDFsData.h + MicrofacetRegularizationSettings.h).

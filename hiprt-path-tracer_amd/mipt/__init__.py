@@ -105,12 +105,44 @@ class RenderSettings(C.Structure):
         ("russian_roulette_throughput_clamp", C.c_float), ("freeze_random", C.c_int32),
         ("enable_direct", C.c_int32), ("microfacet_regularization_tau_0", C.c_float),
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
-        ("accumulate_fp64", C.c_int32),
+        ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
+        ("direct_light_sampling_base_strategy", C.c_int32),
+    ]
+
+
+class RestirDISettings(C.Structure):
+    """mipt_restir_di_settings (ReSTIRDISettings.h:17-163)."""
+
+    _fields_ = [
+        ("number_of_initial_light_candidates", C.c_int32), ("number_of_initial_bsdf_candidates", C.c_int32),
+        ("do_temporal_reuse_pass", C.c_int32), ("do_spatial_reuse_pass", C.c_int32),
+        ("spatial_number_of_passes", C.c_int32), ("spatial_reuse_radius", C.c_int32),
+        ("spatial_reuse_neighbor_count", C.c_int32), ("m_cap", C.c_int32), ("use_confidence_weights", C.c_int32),
+        ("use_normal_similarity_heuristic", C.c_int32), ("normal_similarity_angle_precomp", C.c_float),
+        ("use_plane_distance_heuristic", C.c_int32), ("plane_distance_threshold", C.c_float),
+        ("reject_using_geometric_normals", C.c_int32), ("do_final_shading_visibility", C.c_int32),
     ]
 
 
 MIPT_PASS_CAMERA_RAYS = 0
 MIPT_PASS_PATH_TRACE = 1
+MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES = 2
+MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE = 3
+
+
+def MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i: int) -> int:
+    return 16 + i
+
+
+MIPT_LSS_MIS_LIGHT_BSDF = 3
+MIPT_LSS_RESTIR_DI = 5
+MIPT_LSS_BASE_UNIFORM = 0
+MIPT_LSS_BASE_POWER = 1
+
+# ReSTIR DI reservoir record as read back by mipt_read_restir_reservoirs.
+RESERVOIR_DTYPE = np.dtype([("point", np.float32, 3), ("target", np.float32), ("M", np.int32),
+                            ("tri", np.int32), ("ucw", np.float32), ("flags", np.int32)])
+RESTIR_FLAG_UNOCCLUDED = 1 << 7
 
 
 class GeneratedScene(C.Structure):
@@ -148,6 +180,12 @@ _SIGNATURES = {
     "mipt_framebuffer_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "mipt_read_gbuffer_prims": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mipt_rejected_samples": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "mipt_restir_di_settings_default": (None, [C.POINTER(RestirDISettings)]),
+    "mipt_set_restir_di_settings": (C.c_int, [C.c_void_p, C.POINTER(RestirDISettings)]),
+    "mipt_draws_per_sample": (C.c_int32, [C.c_void_p]),
+    "mipt_get_power_alias_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "mipt_compute_alias_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p]),
+    "mipt_read_restir_reservoirs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
 }
 
 _lib = None
@@ -190,6 +228,30 @@ def default_render_settings(**overrides) -> RenderSettings:
     for k, v in overrides.items():
         setattr(rs, k, v)
     return rs
+
+
+def default_restir_di_settings(**overrides) -> RestirDISettings:
+    rs = RestirDISettings()
+    load_library().mipt_restir_di_settings_default(C.byref(rs))
+    for k, v in overrides.items():
+        setattr(rs, k, v)
+    return rs
+
+
+def compute_alias_table(weights, total_sum=None):
+    """Vose alias table of `weights` (FP64 normalisation), host code of libmipt."""
+    w = np.ascontiguousarray(weights, np.float32)
+    if total_sum is None:
+        total_sum = np.float32(0.0)
+        for x in w:  # float accumulation in order, like the reference's power sum
+            total_sum = np.float32(total_sum + x)
+    probas = np.zeros(w.size, np.float32)
+    alias = np.zeros(w.size, np.int32)
+    rc = load_library().mipt_compute_alias_table(w.ctypes.data_as(C.c_void_p), w.size, float(total_sum),
+                                                 probas.ctypes.data_as(C.c_void_p), alias.ctypes.data_as(C.c_void_p))
+    if rc != MIPT_OK:
+        raise MiptError(rc, "mipt_compute_alias_table")
+    return probas, alias
 
 
 def default_material() -> Material:
@@ -388,6 +450,29 @@ class Context:
         out = np.zeros(n, np.int32)
         self._check(self.lib.mipt_read_gbuffer_prims(self.h, out.ctypes.data_as(C.c_void_p)),
                     "mipt_read_gbuffer_prims")
+        return out
+
+    def set_restir_di_settings(self, settings: RestirDISettings):
+        self.restir = settings
+        self._check(self.lib.mipt_set_restir_di_settings(self.h, C.byref(settings)), "mipt_set_restir_di_settings")
+
+    def draws_per_sample(self) -> int:
+        return int(self.lib.mipt_draws_per_sample(self.h))
+
+    def power_alias_table(self, emissive_count: int):
+        probas = np.zeros(emissive_count, np.float32)
+        alias = np.zeros(emissive_count, np.int32)
+        total = C.c_float()
+        self._check(self.lib.mipt_get_power_alias_table(self.h, probas.ctypes.data_as(C.c_void_p),
+                                                        alias.ctypes.data_as(C.c_void_p), C.byref(total)),
+                    "mipt_get_power_alias_table")
+        return probas, alias, total.value
+
+    def read_reservoirs(self, which: int = 3) -> np.ndarray:
+        n = self.settings.width * self.settings.height
+        out = np.zeros(n, RESERVOIR_DTYPE)
+        self._check(self.lib.mipt_read_restir_reservoirs(self.h, which, out.ctypes.data_as(C.c_void_p)),
+                    "mipt_read_restir_reservoirs")
         return out
 
     def rejected_samples(self) -> int:

@@ -179,7 +179,29 @@ typedef struct {
     float microfacet_regularization_min_roughness;
     int32_t do_microfacet_regularization;
     int32_t accumulate_fp64;
+    int32_t direct_light_sampling_strategy;      /* 3 MIS light + BSDF, 5 ReSTIR DI */
+    int32_t direct_light_sampling_base_strategy; /* 0 uniform, 1 power */
 } orc_render_settings;
+
+/* ReSTIRDISettings subset (ReSTIRDISettings.h:17-163), same order as the
+ * library's settings struct. */
+typedef struct {
+    int32_t number_of_initial_light_candidates;
+    int32_t number_of_initial_bsdf_candidates;
+    int32_t do_temporal_reuse_pass;
+    int32_t do_spatial_reuse_pass;
+    int32_t spatial_number_of_passes;
+    int32_t spatial_reuse_radius;
+    int32_t spatial_reuse_neighbor_count;
+    int32_t m_cap;
+    int32_t use_confidence_weights;
+    int32_t use_normal_similarity_heuristic;
+    float normal_similarity_angle_precomp;
+    int32_t use_plane_distance_heuristic;
+    float plane_distance_threshold;
+    int32_t reject_using_geometric_normals;
+    int32_t do_final_shading_visibility;
+} orc_restir_settings;
 
 typedef struct {
     const float* vertices;          /* float[3V] */
@@ -203,6 +225,31 @@ int32_t orc_render_sample(const orc_render_scene* scene, const orc_octree* tree,
                           uint32_t rn_path, float* fb, double* fb64, int32_t* gb_prims, int32_t nthreads);
 
 float orc_consistent_tau(float tau_0, uint32_t sample_number);
+
+/* ---- Frame renderer with persistent state, pass by pass (CPURenderer.cpp:
+ * 534-1068 / the GPU renderer's pass order): G-buffer + previous-frame
+ * G-buffer, the ReSTIR DI reservoir buffers and their routing
+ * (ReSTIRDIRenderPass.cpp:442-677), the power alias table. */
+typedef struct orc_frame orc_frame;
+orc_frame* orc_frame_create(const orc_render_scene* scene, const orc_octree* tree, const orc_camera* cam,
+                            const orc_render_settings* s, const orc_restir_settings* rs);
+void orc_frame_free(orc_frame* f);
+void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
+void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
+void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
+void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_number, uint32_t random_number,
+                              int32_t nthreads);
+/* Path pass: accumulates into fb/fb64 (sum), returns the dropped samples. */
+int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t random_number, float* fb, double* fb64,
+                             int32_t nthreads);
+/* which: 0 initial, 1 / 2 spatial outputs, 3 the shaded reservoirs; 8 words
+ * per pixel as mipt_read_restir_reservoirs. */
+void orc_frame_read_reservoirs(const orc_frame* f, int32_t which, void* out);
+void orc_frame_gbuffer_prims(const orc_frame* f, int32_t* out);
+/* Power alias table the frame built (emissive_count entries). */
+void orc_frame_alias_table(const orc_frame* f, float* probas, int32_t* alias, float* sum);
+/* Utils::compute_alias_table (Utils.cpp:63-131). */
+void orc_alias_table(const float* weights, int32_t n, float total_sum, float* probas, int32_t* alias);
 
 /* Principled BSDF (metal + diffuse lobes) of one material, restating
  * principled_bsdf_eval / _sample (Principled.h:1620-1715, :1802-1939). Vectors

@@ -38,7 +38,20 @@ class RenderSettings(C.Structure):
         ("russian_roulette_throughput_clamp", C.c_float), ("freeze_random", C.c_int32),
         ("enable_direct", C.c_int32), ("microfacet_regularization_tau_0", C.c_float),
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
-        ("accumulate_fp64", C.c_int32),
+        ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
+        ("direct_light_sampling_base_strategy", C.c_int32),
+    ]
+
+
+class RestirSettings(C.Structure):
+    _fields_ = [
+        ("number_of_initial_light_candidates", C.c_int32), ("number_of_initial_bsdf_candidates", C.c_int32),
+        ("do_temporal_reuse_pass", C.c_int32), ("do_spatial_reuse_pass", C.c_int32),
+        ("spatial_number_of_passes", C.c_int32), ("spatial_reuse_radius", C.c_int32),
+        ("spatial_reuse_neighbor_count", C.c_int32), ("m_cap", C.c_int32), ("use_confidence_weights", C.c_int32),
+        ("use_normal_similarity_heuristic", C.c_int32), ("normal_similarity_angle_precomp", C.c_float),
+        ("use_plane_distance_heuristic", C.c_int32), ("plane_distance_threshold", C.c_float),
+        ("reject_using_geometric_normals", C.c_int32), ("do_final_shading_visibility", C.c_int32),
     ]
 
 
@@ -75,6 +88,17 @@ _SIG = {
     "orc_bsdf_eval": (None, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P, _P, _P]),
     "orc_bsdf_sample": (None, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
     "orc_fmath_eval": (None, [C.c_int32, _P, _P, _P, C.c_int32]),
+    "orc_frame_create": (C.c_void_p, [_P, C.c_void_p, _P, _P, _P]),
+    "orc_frame_free": (None, [C.c_void_p]),
+    "orc_frame_camera_pass": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_restir_initial": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_restir_temporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_restir_spatial": (None, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_path_trace": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, _P, _P, C.c_int32]),
+    "orc_frame_read_reservoirs": (None, [C.c_void_p, C.c_int32, _P]),
+    "orc_frame_gbuffer_prims": (None, [C.c_void_p, _P]),
+    "orc_frame_alias_table": (None, [C.c_void_p, _P, _P, _P]),
+    "orc_alias_table": (None, [_P, C.c_int32, C.c_float, _P, _P]),
 }
 
 _lib = None
@@ -225,13 +249,23 @@ def load_conductor_lut(path: str = LUT_PATH) -> np.ndarray:
     return np.fromfile(path, "<f4").reshape(128, 128)
 
 
-class Renderer:
-    """CameraRays + MegaKernel restated on the CPU, one sample per call."""
+RESERVOIR_DTYPE = np.dtype([("point", np.float32, 3), ("target", np.float32), ("M", np.int32),
+                            ("tri", np.int32), ("ucw", np.float32), ("flags", np.int32)])
 
-    def __init__(self, scene, settings, lut=None):
+
+class Renderer:
+    """The renderer restated on the CPU pass by pass, with persistent frame
+    state (G-buffers, ReSTIR DI reservoirs): the GPU renderer's passes run in
+    the order the caller gives, each with its random number."""
+
+    def __init__(self, scene, settings, restir=None, lut=None):
         self.scene = scene
         self.settings = RenderSettings()
         C.memmove(C.byref(self.settings), C.byref(settings), C.sizeof(RenderSettings))
+        self.restir = None
+        if restir is not None:
+            self.restir = RestirSettings()
+            C.memmove(C.byref(self.restir), C.byref(restir), C.sizeof(RestirSettings))
         self.verts = np.ascontiguousarray(scene.vertices, np.float32).reshape(-1)
         self.idx = np.ascontiguousarray(scene.indices, np.int32).reshape(-1)
         self.mat_idx = np.ascontiguousarray(scene.material_indices, np.int32)
@@ -250,17 +284,79 @@ class Renderer:
         rs.emissive_count = self.emissive.size
         rs.lut_conductor = _p(self.lut)
         self.rs = rs
-        n = settings.width * settings.height
-        self.fb = np.zeros(3 * n, np.float32)
-        self.fb64 = np.zeros(3 * n, np.float64)
-        self.gb_prims = np.zeros(n, np.int32)
+        self.cam = _cam(scene.camera)
+        self.n = settings.width * settings.height
+        self.fb = np.zeros(3 * self.n, np.float32)
+        self.fb64 = np.zeros(3 * self.n, np.float64)
+        self.gb_prims = np.zeros(self.n, np.int32)
         self.rejected = 0
+        self.h = lib().orc_frame_create(C.byref(self.rs), self.tree.h, C.byref(self.cam), C.byref(self.settings),
+                                        C.byref(self.restir) if self.restir is not None else None)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_frame_free(self.h)
+            self.h = None
+
+    def camera_pass(self, sample_number, rn, nthreads=0):
+        lib().orc_frame_camera_pass(self.h, sample_number, rn, nthreads)
+        lib().orc_frame_gbuffer_prims(self.h, _p(self.gb_prims))
+
+    def restir_initial(self, sample_number, rn, nthreads=0):
+        lib().orc_frame_restir_initial(self.h, sample_number, rn, nthreads)
+
+    def restir_temporal(self, sample_number, rn, nthreads=0):
+        lib().orc_frame_restir_temporal(self.h, sample_number, rn, nthreads)
+
+    def restir_spatial(self, pass_index, sample_number, rn, nthreads=0):
+        lib().orc_frame_restir_spatial(self.h, pass_index, sample_number, rn, nthreads)
+
+    def path_trace(self, sample_number, rn, nthreads=0):
+        self.rejected += lib().orc_frame_path_trace(self.h, sample_number, rn, _p(self.fb), _p(self.fb64), nthreads)
 
     def render_sample(self, sample_number, rn_camera, rn_path, nthreads=0):
-        cam = _cam(self.scene.camera)
-        self.rejected += lib().orc_render_sample(C.byref(self.rs), self.tree.h, C.byref(cam), C.byref(self.settings),
-                                                 sample_number, rn_camera, rn_path, _p(self.fb), _p(self.fb64),
-                                                 _p(self.gb_prims), nthreads)
+        """CAMERA_RAYS then PATH_TRACE."""
+        self.camera_pass(sample_number, rn_camera, nthreads)
+        self.path_trace(sample_number, rn_path, nthreads)
+
+    def render_frame(self, sample_number, random_numbers, nthreads=0):
+        """One sample in the GPU renderer's pass order, drawing the passes'
+        random numbers from `random_numbers` (an iterator over the host
+        stream): camera, [ReSTIR DI initial, temporal (two draws), spatial
+        passes], path."""
+        rn = iter(random_numbers)
+        self.camera_pass(sample_number, next(rn), nthreads)
+        if self.settings.direct_light_sampling_strategy == 5:
+            r = self.restir
+            self.restir_initial(sample_number, next(rn), nthreads)
+            if r.do_temporal_reuse_pass:
+                self.restir_temporal(sample_number, next(rn), nthreads)
+                next(rn)
+            if r.do_spatial_reuse_pass:
+                for i in range(r.spatial_number_of_passes):
+                    self.restir_spatial(i, sample_number, next(rn), nthreads)
+        self.path_trace(sample_number, next(rn), nthreads)
+
+    def read_reservoirs(self, which=3):
+        out = np.zeros(self.n, RESERVOIR_DTYPE)
+        lib().orc_frame_read_reservoirs(self.h, which, _p(out))
+        return out
+
+    def alias_table(self):
+        E = self.emissive.size
+        probas = np.zeros(max(E, 1), np.float32)
+        alias = np.zeros(max(E, 1), np.int32)
+        total = C.c_float()
+        lib().orc_frame_alias_table(self.h, _p(probas), _p(alias), C.byref(total))
+        return probas[:E], alias[:E], total.value
+
+
+def alias_table(weights, total_sum):
+    w = np.ascontiguousarray(weights, np.float32)
+    probas = np.zeros(max(w.size, 1), np.float32)
+    alias = np.zeros(max(w.size, 1), np.int32)
+    lib().orc_alias_table(_p(w), w.size, float(total_sum), _p(probas), _p(alias))
+    return probas[:w.size], alias[:w.size]
 
 
 def consistent_tau(tau_0, sample_number):

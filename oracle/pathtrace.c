@@ -120,6 +120,13 @@ typedef struct {
     orc_render_settings s;
     uint32_t sample_number;
     float consistent_tau;
+    /* LSS_BASE_POWER: alias table over the emissive list */
+    const float* alias_probas;
+    const int32_t* alias_alias;
+    float alias_sum;
+    /* LSS_RESTIR_DI: reservoirs shaded at bounce 0 (struct ores, restir part) */
+    const void* restir_out;
+    int32_t final_shading_visibility;
 } rctx;
 
 enum { NO_INFO = 0, FROM_COAT = 2, FROM_FIRST_METAL = 4, FROM_SECOND_METAL = 8, FROM_SPECULAR = 16,
@@ -546,29 +553,63 @@ static int trace_main(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out)
 }
 
 /* ------------------------------------------------------------------ */
-/* NEE with MIS (Lights.h:123-226, LightUtils.h:142-277,               */
+/* Emissive-triangle sampling (LightUtils.h:51-76, :142-309,            */
+/* AliasTable.h:13-22) and NEE with MIS (Lights.h:123-226,              */
 /* PDFConversion.h:19-38, Sampling.h:103-136)                           */
 /* ------------------------------------------------------------------ */
+static float lum(rgb c) { return 0.3086f * c.r + 0.6094f * c.g + 0.0820f * c.b; } /* Color.h:90 */
+
+typedef struct { int32_t tri; v3 normal; float area; rgb emission; v3 point; float area_pdf; } olight;
+
+static olight sample_emissive(const rctx* R, uint32_t* st)
+{
+    olight L;
+    int32_t e = orc_random_index(st, R->ne);
+    if (R->s.direct_light_sampling_base_strategy == 1)
+    {
+        float p = R->alias_probas[e];
+        if (orc_rand01(st) > p)
+            e = R->alias_alias[e];
+    }
+    L.tri = R->emissive[e];
+    v3 A, B, Cc;
+    tri(R, L.tri, &A, &B, &Cc);
+    float r1 = orc_rand01(st), r2 = orc_rand01(st);
+    if (r2 > r1) { r1 *= 0.5f; r2 -= r1; }
+    else { r2 *= 0.5f; r1 -= r2; }
+    v3 AB = vsub(B, A), AC = vsub(Cc, A);
+    v3 n = vcross(AB, AC);
+    float len = vlen(n);
+    L.point = vadd(vadd(A, vscale(AB, r1)), vscale(AC, r2));
+    L.normal = vdivs(n, len);
+    L.area = 0.5f * len;
+    L.emission = R->mats[R->mat_idx[L.tri]].emission;
+    L.area_pdf = 1.0f / L.area;
+    if (R->s.direct_light_sampling_base_strategy == 1)
+        L.area_pdf *= (lum(L.emission) * L.area) / R->alias_sum;
+    else
+        L.area_pdf /= (float)R->ne;
+    return L;
+}
+
+static float emissive_hit_area_pdf(const rctx* R, float area, rgb emission)
+{
+    float pdf = 1.0f / area;
+    if (R->s.direct_light_sampling_base_strategy == 1)
+        pdf *= (lum(emission) * area) / R->alias_sum;
+    else
+        pdf /= (float)R->ne;
+    return pdf;
+}
+
 static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi, v3 view, float path_r, uint32_t* st)
 {
     rgb light_mis = C1(0.0f);
     {
-        int32_t e = orc_random_index(st, R->ne);
-        int32_t lt = R->emissive[e];
-        v3 A, B, Cc;
-        tri(R, lt, &A, &B, &Cc);
-        float r1 = orc_rand01(st), r2 = orc_rand01(st);
-        if (r2 > r1) { r1 *= 0.5f; r2 -= r1; }
-        else { r2 *= 0.5f; r1 -= r2; }
-        v3 AB = vsub(B, A), AC = vsub(Cc, A);
-        v3 n = vcross(AB, AC);
-        float len = vlen(n);
-        v3 point = vadd(vadd(A, vscale(AB, r1)), vscale(AC, r2));
-        v3 ln = vdivs(n, len);
-        float area = 0.5f * len;
-        float apdf = 1.0f / area;
-        apdf /= (float)R->ne;
-        rgb emission = R->mats[R->mat_idx[lt]].emission;
+        olight L = sample_emissive(R, st);
+        v3 point = L.point, ln = L.normal;
+        float apdf = L.area_pdf;
+        rgb emission = L.emission;
         if (apdf > 0.0f)
         {
             v3 to = vsub(point, hi->p);
@@ -615,8 +656,7 @@ static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi,
                 float cl = hmax(0.0f, vdot(hgn, vneg(dir)));
                 if (!cblack(em) && cl > 0.0f)
                 {
-                    float pa = 1.0f / R->areas[h.prim];
-                    pa /= (float)R->ne;
+                    float pa = emissive_hit_area_pdf(R, R->areas[h.prim], em);
                     float lpdf = cl < 1.0e-8f ? 0.0f : pa * (h.t * h.t) / cl;
                     float w = bpdf == 0.0f ? 0.0f : bpdf / (1.0f * bpdf + 1.0f * lpdf);
                     float cosine = fabsf(vdot(hi->sn, dir));
@@ -626,6 +666,50 @@ static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi,
         }
     }
     return cadd(light_mis, bsdf_mis);
+}
+
+/* ------------------------------------------------------------------ */
+/* ReSTIR DI reservoir (DI/Reservoir.h:52-187) and its final shading    */
+/* (DI/FinalShading.h:16-125)                                           */
+/* ------------------------------------------------------------------ */
+typedef struct { v3 point; float target; int32_t M; int32_t tri; float ucw; uint32_t flags; float wsum; } ores;
+
+#define RESTIR_UNOCCLUDED (1u << 7)
+#define RESTIR_LOBE_MASK (0x3fu << 1) /* flags_to_BSDF_incident_light_info keeps bits 1-6 */
+#define FROM_GLASS_REFRACT_BIT (1u << 6)
+
+static rgb restir_final_shading(const rctx* R, const omat* m, const hitinfo* hi, v3 view, size_t pix)
+{
+    const ores* r = (const ores*)R->restir_out + pix;
+    if (r->ucw <= 0.0f || r->tri < 0) /* NaN weight sum with no selected sample */
+        return C1(0.0f);
+    v3 to = vsub(r->point, hi->p);
+    float dist = vlen(to);
+    v3 dir = vdivs(to, dist);
+    int in_shadow = 0;
+    if (r->flags & RESTIR_UNOCCLUDED)
+        in_shadow = 0;
+    else if (R->final_shading_visibility)
+    {
+        orc_hit h = trace1(R, hi->p, dir, hi->prim, INFINITY);
+        in_shadow = h.prim >= 0 && h.t < dist - 1.0e-4f;
+    }
+    if (in_shadow)
+        return C1(0.0f);
+    bctx c = {view, hi->sn, hi->gn, dir, (int)(r->flags & RESTIR_LOBE_MASK), REG_MIS, 0.0f};
+    float bpdf;
+    rgb f = principled_eval(R, m, &c, &bpdf);
+    float cosine = vdot(hi->sn, dir);
+    if (r->flags & FROM_GLASS_REFRACT_BIT)
+        cosine = fabsf(cosine);
+    if (!(cosine > 0.0f))
+        return C1(0.0f);
+    rgb em = R->mats[R->mat_idx[r->tri]].emission;
+    v3 a, b, cc;
+    tri(R, r->tri, &a, &b, &cc);
+    float conv = hmax(0.0f, vdot(vnorm(vcross(vsub(b, a), vsub(cc, a))), vneg(dir)));
+    conv /= dist * dist;
+    return cscale(cscale(cmul(cscale(f, r->ucw), em), cosine), conv);
 }
 
 /* ------------------------------------------------------------------ */
@@ -702,7 +786,9 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
         {
             /* estimate_direct_lighting (Lights.h:364-398) */
             rgb direct = C1(0.0f);
-            if (R->ne > 0)
+            if (R->ne > 0 && R->restir_out && bounce == 0)
+                direct = restir_final_shading(R, m, &hi, view, pix);
+            else if (R->ne > 0)
                 direct = cadd(C1(0.0f), sample_one_light_mis(R, m, &hi, view, path_r, &st));
             rgb total = C1(0.0f);
             if (bounce == 0)
@@ -753,82 +839,17 @@ float orc_consistent_tau(float tau_0, uint32_t sample_number)
     return 1.0f / (2.0f * M_PI_F * (1.0f - cosf(atanf(inner))));
 }
 
+/* forward declarations of the frame renderer below */
+static int32_t render_sample_via_frame(const orc_render_scene* sc, const orc_octree* tree, const orc_camera* cam,
+                                       const orc_render_settings* s, uint32_t sample_number, uint32_t rn_camera,
+                                       uint32_t rn_path, float* fb, double* fb64, int32_t* gb_prims, int32_t nthreads);
+
+/* One plain sample (CAMERA_RAYS then PATH_TRACE) on a fresh frame. */
 int32_t orc_render_sample(const orc_render_scene* sc, const orc_octree* tree, const orc_camera* cam,
                           const orc_render_settings* s, uint32_t sample_number, uint32_t rn_camera,
                           uint32_t rn_path, float* fb, double* fb64, int32_t* gb_prims, int32_t nthreads)
 {
-    omat* mats = (omat*)malloc(sizeof(omat) * (size_t)(sc->material_count > 0 ? sc->material_count : 1));
-    for (int32_t i = 0; i < sc->material_count; i++)
-        mats[i] = unpack_material(sc->materials + i);
-    float* areas = (float*)malloc(sizeof(float) * (size_t)(sc->triangle_count > 0 ? sc->triangle_count : 1));
-    for (int32_t t = 0; t < sc->triangle_count; t++)
-    {
-        /* ThreadFunctions.cpp:221-242 */
-        const float* a = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t];
-        const float* b = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t + 1];
-        const float* c = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t + 2];
-        v3 n = vcross(V(b[0] - a[0], b[1] - a[1], b[2] - a[2]), V(c[0] - a[0], c[1] - a[1], c[2] - a[2]));
-        areas[t] = vlen(n) * 0.5f;
-    }
-    rctx R = {sc->vertices, sc->indices, sc->material_indices, mats, sc->emissive, sc->emissive_count, areas,
-              sc->lut_conductor, tree, cam, *s, sample_number, 0.0f};
-    R.consistent_tau = orc_consistent_tau(s->microfacet_regularization_tau_0, sample_number);
-    int32_t rejected = 0;
-#ifdef _OPENMP
-    if (nthreads > 0)
-        omp_set_num_threads(nthreads);
-#endif
-#pragma omp parallel for schedule(dynamic) reduction(+ : rejected)
-    for (int y = 0; y < s->height; y++)
-    {
-        for (int x = 0; x < s->width; x++)
-        {
-            size_t pix = (size_t)x + (size_t)y * (size_t)s->width;
-            gbuf g;
-            camera_rays_pixel(&R, x, y, rn_camera, &g);
-            if (gb_prims)
-                gb_prims[pix] = g.prim;
-            rgb c = C1(0.0f);
-            if (g.prim != -1)
-                c = megakernel_pixel(&R, x, y, rn_path, &g);
-            /* SanityCheck.h:78-106: drop NaN/inf/negative samples */
-            int bad = 0;
-            float comp[3] = {c.r, c.g, c.b};
-            for (int k = 0; k < 3; k++)
-                bad |= comp[k] < 0.0f || isnan(comp[k]) || isinf(comp[k]);
-            if (bad)
-            {
-                rejected++;
-                c = C1(0.0f);
-                if (sample_number != 0)
-                    continue;
-            }
-            float* f = fb + 3 * pix;
-            if (sample_number == 0)
-            {
-                f[0] = c.r; f[1] = c.g; f[2] = c.b;
-            }
-            else
-            {
-                f[0] += c.r; f[1] += c.g; f[2] += c.b;
-            }
-            if (fb64)
-            {
-                double* d = fb64 + 3 * pix;
-                if (sample_number == 0)
-                {
-                    d[0] = c.r; d[1] = c.g; d[2] = c.b;
-                }
-                else
-                {
-                    d[0] += (double)c.r; d[1] += (double)c.g; d[2] += (double)c.b;
-                }
-            }
-        }
-    }
-    free(mats);
-    free(areas);
-    return rejected;
+    return render_sample_via_frame(sc, tree, cam, s, sample_number, rn_camera, rn_path, fb, fb64, gb_prims, nthreads);
 }
 
 /* ------------------------------------------------------------------ */
@@ -868,4 +889,781 @@ void orc_bsdf_sample(const orc_cpu_material* cm, const float* lut, float consist
     rgb f = principled_sample(&R, &m, &c, &d, out_pdf, rng_state);
     out_dir[0] = d.x; out_dir[1] = d.y; out_dir[2] = d.z;
     out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
+}
+
+/* ================================================================== */
+/* Frame renderer: persistent G-buffers, power alias table and the     */
+/* ReSTIR DI passes, one pixel at a time with inline ray queries.       */
+/* ================================================================== */
+
+/* Utils::compute_alias_table (src/Utils/Utils.cpp:63-131): Vose's method on
+ * the weights normalised in FP64 to mean 1; FIFO queues of small and large
+ * indices. */
+void orc_alias_table(const float* input, int32_t n, float total_sum, float* probas, int32_t* alias)
+{
+    if (n <= 0)
+        return;
+    double total = total_sum;
+    double* sc = (double*)malloc(sizeof(double) * (size_t)n);
+    int32_t* small_q = (int32_t*)malloc(sizeof(int32_t) * (size_t)(2 * n + 1));
+    int32_t* large_q = (int32_t*)malloc(sizeof(int32_t) * (size_t)(2 * n + 1));
+    size_t sh = 0, st = 0, lh = 0, lt = 0; /* head / tail of each FIFO */
+    for (int32_t i = 0; i < n; i++)
+    {
+        sc[i] = (double)input[i] / total;
+        sc[i] *= (double)n;
+        probas[i] = 0.0f;
+        alias[i] = 0;
+    }
+    for (int32_t i = 0; i < n; i++)
+    {
+        if (sc[i] < 1.0)
+            small_q[st++] = i;
+        else
+            large_q[lt++] = i;
+    }
+    while (sh < st && lh < lt)
+    {
+        int32_t s = small_q[sh++], l = large_q[lh++];
+        probas[s] = (float)sc[s];
+        alias[s] = l;
+        sc[l] = (sc[l] + sc[s]) - 1.0;
+        if (sc[l] > 1.0)
+            large_q[lt++] = l;
+        else
+            small_q[st++] = l;
+    }
+    while (lh < lt)
+        probas[large_q[lh++]] = 1.0f;
+    while (sh < st)
+        probas[small_q[sh++]] = 1.0f;
+    free(sc);
+    free(small_q);
+    free(large_q);
+}
+
+struct orc_frame {
+    rctx R;
+    omat* mats;
+    float* areas;
+    float* alias_probas;
+    int32_t* alias_alias;
+    orc_restir_settings rs;
+    gbuf* cur;
+    gbuf* prev;
+    ores* res[3];
+    int32_t last_output, shade_buffer, temporal_output, odd_frame;
+    size_t n;
+};
+
+static ores ores_empty(void)
+{
+    ores r;
+    r.point = V(0, 0, 0);
+    r.target = 0.0f;
+    r.M = 0;
+    r.tri = -1;
+    r.ucw = 0.0f;
+    r.flags = 0u;
+    r.wsum = 0.0f;
+    return r;
+}
+
+static void gbuf_reset(gbuf* g)
+{
+    g->prim = -1;
+    g->pos = V(0, 0, 0);
+    g->gn = 0u;
+    g->sn = 0u;
+    g->mat = -1;
+}
+
+static int restir_on(const orc_frame* f) { return f->R.s.direct_light_sampling_strategy == 5; }
+static int keep_prev(const orc_frame* f) { return restir_on(f) && f->rs.do_temporal_reuse_pass; }
+
+orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, const orc_camera* cam,
+                            const orc_render_settings* s, const orc_restir_settings* rs)
+{
+    orc_frame* f = (orc_frame*)calloc(1, sizeof(orc_frame));
+    f->mats = (omat*)malloc(sizeof(omat) * (size_t)(sc->material_count > 0 ? sc->material_count : 1));
+    for (int32_t i = 0; i < sc->material_count; i++)
+        f->mats[i] = unpack_material(sc->materials + i);
+    f->areas = (float*)malloc(sizeof(float) * (size_t)(sc->triangle_count > 0 ? sc->triangle_count : 1));
+    for (int32_t t = 0; t < sc->triangle_count; t++)
+    {
+        const float* a = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t];
+        const float* b = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t + 1];
+        const float* c = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t + 2];
+        v3 nrm = vcross(V(b[0] - a[0], b[1] - a[1], b[2] - a[2]), V(c[0] - a[0], c[1] - a[1], c[2] - a[2]));
+        f->areas[t] = vlen(nrm) * 0.5f;
+    }
+    /* Emissive power list (GPURenderer.cpp:221-295): area from the float
+     * cross product, luminance of the CPU material's emission times its
+     * strength and global factor, summed in order in binary32. */
+    int32_t E = sc->emissive_count;
+    f->alias_probas = (float*)calloc((size_t)(E > 0 ? E : 1), sizeof(float));
+    f->alias_alias = (int32_t*)calloc((size_t)(E > 0 ? E : 1), sizeof(int32_t));
+    float sum = 0.0f;
+    if (E > 0)
+    {
+        float* power = (float*)malloc(sizeof(float) * (size_t)E);
+        for (int32_t i = 0; i < E; i++)
+        {
+            int32_t t = sc->emissive[i];
+            const float* a = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t];
+            const float* b = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t + 1];
+            const float* c = sc->vertices + 3 * (int64_t)sc->indices[3 * (int64_t)t + 2];
+            v3 nrm = vcross(V(b[0] - a[0], b[1] - a[1], b[2] - a[2]), V(c[0] - a[0], c[1] - a[1], c[2] - a[2]));
+            float area = 0.5f * vlen(nrm);
+            const orc_cpu_material* m = sc->materials + sc->material_indices[t];
+            float l = 0.3086f * m->emission[0] + 0.6094f * m->emission[1] + 0.0820f * m->emission[2];
+            power[i] = l * m->emission_strength * m->global_emissive_factor * area;
+            sum += power[i];
+        }
+        orc_alias_table(power, E, sum, f->alias_probas, f->alias_alias);
+        free(power);
+    }
+    rctx R = {sc->vertices, sc->indices, sc->material_indices, f->mats, sc->emissive, sc->emissive_count, f->areas,
+              sc->lut_conductor, tree, cam, *s, 0, 0.0f, f->alias_probas, f->alias_alias, sum, NULL, 0};
+    f->R = R;
+    if (rs)
+        f->rs = *rs;
+    f->n = (size_t)s->width * (size_t)s->height;
+    f->cur = (gbuf*)malloc(sizeof(gbuf) * f->n);
+    f->prev = (gbuf*)malloc(sizeof(gbuf) * f->n);
+    for (size_t i = 0; i < f->n; i++)
+    {
+        gbuf_reset(f->cur + i);
+        gbuf_reset(f->prev + i);
+    }
+    for (int k = 0; k < 3; k++)
+    {
+        f->res[k] = (ores*)malloc(sizeof(ores) * f->n);
+        for (size_t i = 0; i < f->n; i++)
+            f->res[k][i] = ores_empty();
+    }
+    f->last_output = 1;
+    f->shade_buffer = 1;
+    f->temporal_output = 0;
+    f->odd_frame = 0;
+    return f;
+}
+
+void orc_frame_free(orc_frame* f)
+{
+    if (!f)
+        return;
+    free(f->mats);
+    free(f->areas);
+    free(f->alias_probas);
+    free(f->alias_alias);
+    free(f->cur);
+    free(f->prev);
+    for (int k = 0; k < 3; k++)
+        free(f->res[k]);
+    free(f);
+}
+
+static void set_threads(int32_t nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+/* CameraRays (CameraRays.h:83-213): reset at sample 0 (reset_render :20-59),
+ * previous-frame copy (:117-124), G-buffer write. */
+void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
+{
+    f->R.sample_number = sample_number;
+    f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    int kp = keep_prev(f);
+    if (sample_number == 0)
+    {
+        for (size_t i = 0; i < f->n; i++)
+        {
+            gbuf_reset(f->cur + i);
+            if (kp)
+                gbuf_reset(f->prev + i);
+        }
+        if (restir_on(f))
+        {
+            for (int k = 0; k < 3; k++)
+                for (size_t i = 0; i < f->n; i++)
+                    f->res[k][i] = ores_empty();
+            f->odd_frame = 0;
+        }
+    }
+    set_threads(nthreads);
+    const int w = f->R.s.width;
+#pragma omp parallel for schedule(dynamic)
+    for (int y = 0; y < f->R.s.height; y++)
+        for (int x = 0; x < w; x++)
+        {
+            size_t pix = (size_t)x + (size_t)y * (size_t)w;
+            if (kp)
+                f->prev[pix] = f->cur[pix];
+            camera_rays_pixel(&f->R, x, y, rn, f->cur + pix);
+        }
+}
+
+/* ---- ReSTIR DI helpers ---------------------------------------------- */
+typedef struct { int32_t mat, prim; v3 view, sn, gn, p; } osurf;
+
+static osurf surf_of(const orc_frame* f, const gbuf* g, size_t pix)
+{
+    osurf s;
+    v3 cam = V(f->R.cam->position[0], f->R.cam->position[1], f->R.cam->position[2]);
+    s.mat = g[pix].mat;
+    s.prim = g[pix].prim;
+    s.p = g[pix].pos;
+    /* the view direction comes from the current frame's hit position also for
+     * a previous-frame surface (Surface.h:62) */
+    s.view = vnorm(vsub(cam, f->cur[pix].pos));
+    s.sn = vnorm(oct_unpack(g[pix].sn));
+    s.gn = vnorm(oct_unpack(g[pix].gn));
+    return s;
+}
+
+/* ReSTIR_DI_evaluate_target_function (DI/TargetFunction.h:49-98); with_vis
+ * traces the shadow ray (Intersect.h:276-324) when the value is non-zero. */
+static float target_fn(const orc_frame* f, const osurf* s, v3 point, int32_t tri_i, uint32_t flags, int with_vis)
+{
+    const rctx* R = &f->R;
+    if (tri_i == -1 || s->mat < 0)
+        return 0.0f;
+    v3 to = vsub(point, s->p);
+    float dist = vlen(to);
+    v3 dir = vdivs(to, dist);
+    float cosine = vdot(s->sn, dir);
+    if (cosine <= 0.0f)
+        return 0.0f;
+    bctx c = {s->view, s->sn, s->gn, dir, (int)(flags & RESTIR_LOBE_MASK), NO_REG, 0.0f};
+    float bpdf;
+    rgb fc = principled_eval(R, &R->mats[s->mat], &c, &bpdf);
+    rgb em = R->mats[R->mat_idx[tri_i]].emission;
+    v3 a, b, cc;
+    tri(R, tri_i, &a, &b, &cc);
+    float geom = hmax(0.0f, vdot(vnorm(vcross(vsub(b, a), vsub(cc, a))), vneg(dir)));
+    geom /= dist * dist;
+    float t = lum(cscale(cscale(cmul(fc, em), cosine), geom));
+    if (t == 0.0f)
+        return 0.0f;
+    if (with_vis)
+    {
+        orc_hit h = trace1(R, s->p, dir, s->prim, INFINITY);
+        int occluded = h.prim >= 0 && h.t < dist - 1.0e-4f;
+        t = t * (occluded ? 0.0f : 1.0f);
+    }
+    return t;
+}
+
+static void add_candidate(ores* r, v3 point, int32_t tri_i, float target, uint32_t flags, float w, uint32_t* st)
+{
+    r->M++;
+    r->wsum += w;
+    if (orc_rand01(st) < w / r->wsum)
+    {
+        r->point = point;
+        r->tri = tri_i;
+        r->target = target;
+        r->flags = flags;
+    }
+}
+
+static int combine_with(ores* r, const ores* o, float mis, float target, float jac, uint32_t* st)
+{
+    if (o->ucw <= 0.0f)
+    {
+        r->M += o->M;
+        return 0;
+    }
+    float w = mis * target * o->ucw * jac;
+    r->M += o->M;
+    r->wsum += w;
+    if (orc_rand01(st) < w / r->wsum)
+    {
+        r->point = o->point;
+        r->tri = o->tri;
+        r->flags = o->flags;
+        r->target = target;
+        return 1;
+    }
+    return 0;
+}
+
+static void res_end(ores* r) { r->ucw = r->wsum == 0.0f ? 0.0f : 1.0f / r->target * r->wsum; }
+
+static void res_end_normalized(ores* r, float num, float den)
+{
+    if (r->wsum == 0.0f || r->wsum < 1.0e-10f || r->wsum > 1.0e10f || den == 0.0f || num == 0.0f)
+        r->ucw = 0.0f;
+    else
+        r->ucw = 1.0f / r->target * r->wsum * num / den;
+    if (r->M > 1000000)
+        r->M = 1000000;
+}
+
+/* ReSTIR_DI_visibility_test_kill_reservoir (ReSTIR/Utils.h:25-67) */
+static void visibility_kill(const orc_frame* f, ores* r, v3 p, int32_t prim)
+{
+    if (r->ucw <= 0.0f && r->wsum <= 0.0f)
+        return;
+    if (r->flags & RESTIR_UNOCCLUDED)
+        return;
+    v3 to = vsub(r->point, p);
+    float dist = vlen(to);
+    v3 dir = vdivs(to, dist);
+    orc_hit h = trace1(&f->R, p, dir, prim, INFINITY);
+    if (h.prim >= 0 && h.t < dist - 1.0e-4f)
+        r->ucw = -1.0f;
+    else
+        r->flags |= RESTIR_UNOCCLUDED;
+}
+
+static void end_frame(orc_frame* f, int32_t out)
+{
+    f->shade_buffer = out;
+    f->last_output = out;
+    f->odd_frame = !f->odd_frame;
+}
+
+/* ReSTIR_DI_InitialCandidates (DI/InitialCandidates.h:180-538), envmap
+ * candidate probability 0. */
+void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
+{
+    f->R.sample_number = sample_number;
+    f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    const rctx* R = &f->R;
+    const int32_t nl = f->rs.number_of_initial_light_candidates, nb = f->rs.number_of_initial_bsdf_candidates;
+    if (R->ne > 0)
+    {
+        set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t pi = 0; pi < (int64_t)f->n; pi++)
+        {
+            size_t pix = (size_t)pi;
+            uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, R->s.freeze_random);
+            if (f->cur[pix].prim == -1)
+                continue;
+            osurf s = surf_of(f, f->cur, pix);
+            const omat* m = &R->mats[s.mat];
+            ores res = ores_empty();
+            for (int32_t i = 0; i < nl; i++)
+            {
+                float cosine = 0.0f, pdf = 0.0f, dist = 0.0f;
+                v3 dir = V(0, 0, 0), point = V(0, 0, 0);
+                int32_t tri_i = -1;
+                if (orc_rand01(&st) > 0.0f)
+                {
+                    olight L = sample_emissive(R, &st);
+                    tri_i = L.tri;
+                    point = L.point;
+                    pdf = L.area_pdf;
+                    v3 to = vsub(point, s.p);
+                    dist = vlen(to);
+                    dir = vdivs(to, dist);
+                    if (pdf > 0.0f)
+                    {
+                        cosine = hmax(0.0f, vdot(s.sn, dir));
+                        pdf *= (1.0f - 0.0f);
+                    }
+                }
+                float w = 0.0f, sample_target = 0.0f;
+                if (cosine > 0.0f && pdf > 0.0f)
+                {
+                    bctx c = {s.view, s.sn, s.gn, dir, NO_INFO, NO_REG, 0.0f};
+                    float bsdf_pdf;
+                    principled_eval(R, m, &c, &bsdf_pdf);
+                    float target = target_fn(f, &s, point, tri_i, 0u, 0);
+                    if (!(bsdf_pdf <= 0.0f))
+                    {
+                        v3 a, b, cc;
+                        tri(R, tri_i, &a, &b, &cc);
+                        v3 ln = vcross(vsub(b, a), vsub(cc, a));
+                        float nl_len = vlen(ln);
+                        ln = vdivs(ln, nl_len);
+                        float cl = hmax(0.0f, vdot(ln, vneg(dir)));
+                        float lpdf = cl < 1.0e-8f ? 0.0f : (pdf / (1.0f - 0.0f)) * (dist * dist) / cl;
+                        lpdf *= (1.0f - 0.0f);
+                        float mis = lpdf == 0.0f ? 0.0f : lpdf / ((float)nl * lpdf + (float)nb * bsdf_pdf);
+                        w = mis * target / pdf;
+                        sample_target = target;
+                    }
+                }
+                add_candidate(&res, point, tri_i, sample_target, 0u, w, &st);
+            }
+            for (int32_t i = 0; i < nb; i++)
+            {
+                bctx c = {s.view, s.sn, s.gn, V(0, 0, 0), NO_INFO, NO_REG, 0.0f};
+                v3 dir;
+                float bpdf;
+                principled_sample(R, m, &c, &dir, &bpdf, &st);
+                if (!(bpdf > 0.0f))
+                    continue;
+                orc_hit h = trace1(R, s.p, dir, s.prim, INFINITY);
+                if (!(h.prim >= 0 && h.t < 1.0e35f - 1.0e-4f))
+                    continue;
+                rgb em = R->mats[R->mat_idx[h.prim]].emission;
+                if (cblack(em))
+                    continue;
+                v3 point = vadd(s.p, vscale(dir, h.t));
+                uint32_t flags = (RESTIR_UNOCCLUDED | (uint32_t)c.info) & 0xffu;
+                float target = target_fn(f, &s, point, h.prim, flags, 0);
+                v3 a, b, cc;
+                tri(R, h.prim, &a, &b, &cc);
+                v3 hgn = vnorm(vnorm(vcross(vsub(b, a), vsub(cc, a))));
+                float cl = hmax(0.0f, vdot(hgn, vneg(dir)));
+                float lpdf = 0.0f;
+                if (!(vdot(dir, s.sn) < 0.0f))
+                {
+                    float pa = emissive_hit_area_pdf(R, R->areas[h.prim], em);
+                    lpdf = cl < 1.0e-8f ? 0.0f : pa * (h.t * h.t) / cl;
+                }
+                lpdf *= (1.0f - 0.0f);
+                float mis = bpdf == 0.0f ? 0.0f : bpdf / ((float)nb * bpdf + (float)nl * lpdf);
+                float pdf_area = bpdf;
+                pdf_area /= (h.t * h.t);
+                pdf_area *= cl;
+                float w = mis * target / pdf_area;
+                add_candidate(&res, point, h.prim, target, flags, w, &st);
+            }
+            res_end(&res);
+            res.M = 1;
+            visibility_kill(f, &res, s.p, s.prim);
+            f->res[0][pix] = res;
+        }
+    }
+    if (!f->rs.do_temporal_reuse_pass && !f->rs.do_spatial_reuse_pass)
+        end_frame(f, 0);
+}
+
+/* ReSTIR_DI_TemporalReuse (DI/TemporalReuse.h:50-296), accumulation mode
+ * (the temporal neighbour is the pixel itself, UtilsTemporal.h:37-41), with
+ * the pairwise MIS weights (TemporalMISWeight.h:164-254). */
+void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
+{
+    f->R.sample_number = sample_number;
+    f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    const int32_t out_i = f->rs.do_spatial_reuse_pass ? 0 : (f->odd_frame ? 1 : 2);
+    ores* in = f->res[f->last_output];
+    ores* init = f->res[0];
+    ores* out = f->res[out_i];
+    const int cw = f->rs.use_confidence_weights != 0;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t pi = 0; pi < (int64_t)f->n; pi++)
+    {
+        size_t pix = (size_t)pi;
+        if (f->cur[pix].prim == -1)
+            continue;
+        uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, f->R.s.freeze_random);
+        ores tn = in[pix];
+        ores ini = init[pix];
+        if (f->R.s.freeze_random || tn.M == 0)
+        {
+            out[pix] = ini;
+            continue;
+        }
+        osurf prev = surf_of(f, f->prev, pix);
+        if (prev.mat >= 0)
+        {
+            rgb e = f->R.mats[prev.mat].emission;
+            const float nz = 1.0e-10f;
+            if (!(e.r < nz && e.r > -nz) || !(e.g < nz && e.g > -nz) || !(e.b < nz && e.b > -nz))
+            {
+                out[pix] = ini;
+                continue;
+            }
+        }
+        osurf center = surf_of(f, f->cur, pix);
+        ores o = ores_empty();
+        float tf_center = 0.0f;
+        if (tn.ucw > 0.0f)
+            tf_center = target_fn(f, &center, tn.point, tn.tri, tn.flags, 1);
+        float tn_M = cw ? (float)tn.M : 1.0f;
+        float center_M = cw ? (float)ini.M : 1.0f;
+        float conf_sum = cw ? (float)tn.M : 1.0f;
+        float nume = tn.target * tn_M;
+        float denom = tn.target * conf_sum + tf_center * center_M;
+        float mi = denom == 0.0f ? 0.0f : nume / denom;
+        float tf_csn = target_fn(f, &prev, ini.point, ini.tri, ini.flags, 1);
+        float tf_csc = ini.target;
+        float nume_mc = tf_csc * center_M;
+        float denom_mc = tf_csn * conf_sum + tf_csc * center_M;
+        float conf_mult = cw ? tn_M / conf_sum : 1.0f;
+        float mc = 0.0f;
+        if (denom_mc != 0.0f)
+            mc += nume_mc / denom_mc * conf_mult;
+        if (combine_with(&o, &tn, mi, tf_center, 1.0f, &st))
+            o.flags |= RESTIR_UNOCCLUDED;
+        float mis_init = mc == 0.0f ? 1.0f : mc;
+        if (combine_with(&o, &ini, mis_init, ini.target, 1.0f, &st))
+            o.flags |= RESTIR_UNOCCLUDED;
+        res_end_normalized(&o, 1.0f, 1.0f);
+        if (f->rs.m_cap > 0 && o.M > f->rs.m_cap)
+            o.M = f->rs.m_cap;
+        out[pix] = o;
+    }
+    f->temporal_output = out_i;
+    if (!f->rs.do_spatial_reuse_pass)
+        end_frame(f, out_i);
+}
+
+/* get_spatial_neighbor_pixel_index (UtilsSpatial.h:177-259) with
+ * sample_in_disk_uv (Sampling.h:50-57); the u draw is evaluated first. */
+static int64_t spatial_neighbor(const orc_frame* f, int cx, int cy, uint32_t* nst)
+{
+    float u = orc_rand01(nst);
+    float v = orc_rand01(nst);
+    float rad = (float)f->rs.spatial_reuse_radius * sqrtf(v);
+    float sn_, cs_;
+    orc_sincos(M_TWO_PI_F * u, &sn_, &cs_);
+    float ox = rad * cs_, oy = rad * sn_;
+    int nx = cx + (int)roundf(ox), ny = cy + (int)roundf(oy);
+    if (nx < 0 || nx >= f->R.s.width || ny < 0 || ny >= f->R.s.height)
+        return -1;
+    return (int64_t)nx + (int64_t)ny * f->R.s.width;
+}
+
+/* check_neighbor_similarity_heuristics (NeighborSimilarity.h:46-107) */
+static int similar(const orc_frame* f, size_t np, size_t cp, v3 cpoint, v3 cnormal)
+{
+    if (np == cp)
+        return 1;
+    if (f->cur[np].prim == -1)
+        return 0;
+    v3 npt = f->cur[np].pos;
+    v3 nn = f->rs.reject_using_geometric_normals ? vnorm(oct_unpack(f->cur[np].gn)) : vnorm(oct_unpack(f->cur[np].sn));
+    int plane = 1, normal = 1;
+    if (f->rs.use_plane_distance_heuristic)
+        plane = fabsf(vdot(vsub(npt, cpoint), cnormal)) < f->rs.plane_distance_threshold;
+    if (f->rs.use_normal_similarity_heuristic)
+        normal = vdot(cnormal, nn) > f->rs.normal_similarity_angle_precomp;
+    return plane && normal;
+}
+
+/* ReSTIR_DI_SpatialReuse (DI/SpatialReuse.h:43-322) with the pairwise MIS
+ * weights (SpatialMISWeight.h:118-234) and confidence weights. */
+void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_number, uint32_t rn, int32_t nthreads)
+{
+    f->R.sample_number = sample_number;
+    f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    int32_t in_i, out_i;
+    if (pass_index == 0)
+    {
+        in_i = f->rs.do_temporal_reuse_pass ? f->temporal_output : 0;
+        out_i = 1;
+    }
+    else if ((pass_index & 1) == 0)
+    {
+        in_i = 2;
+        out_i = 1;
+    }
+    else
+    {
+        in_i = 1;
+        out_i = 2;
+    }
+    int last = pass_index == f->rs.spatial_number_of_passes - 1;
+    int kill_after = f->rs.do_temporal_reuse_pass || !last;
+    ores* in = f->res[in_i];
+    ores* out = f->res[out_i];
+    const int32_t K = f->rs.spatial_reuse_neighbor_count;
+    const int cw = f->rs.use_confidence_weights != 0;
+    const int w = f->R.s.width;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t pi = 0; pi < (int64_t)f->n; pi++)
+    {
+        size_t pix = (size_t)pi;
+        if (f->cur[pix].prim == -1)
+            continue;
+        int cx = (int)(pix % (size_t)w), cy = (int)(pix / (size_t)w);
+        uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, f->R.s.freeze_random);
+        ores center = in[pix];
+        osurf cs = surf_of(f, f->cur, pix);
+        uint32_t nseed = orc_xorshift32(&st);
+        v3 cnormal = f->rs.reject_using_geometric_normals ? cs.gn : cs.sn;
+        int32_t valid_count = 0, valid_M = 0;
+        uint32_t cache = 0u;
+        uint32_t nst = nseed;
+        for (int32_t i = 0; i < K; i++)
+        {
+            int64_t np = spatial_neighbor(f, cx, cy, &nst);
+            if (np < 0)
+                continue;
+            if (!similar(f, (size_t)np, pix, cs.p, cnormal))
+                continue;
+            valid_M += in[np].M;
+            valid_count++;
+            cache |= 1u << i;
+        }
+        float center_M = cw ? (float)center.M : 1.0f;
+        float conf_sum = cw ? (float)valid_M : 1.0f;
+        float vdt = cw ? 1.0f : (float)valid_count;
+        int update_mc = center.M > 0 && center.ucw > 0.0f;
+        float tf_csc = center.target;
+        float nume_mc = tf_csc / vdt * center_M;
+        float mc = 0.0f;
+        ores o = ores_empty();
+        nst = nseed;
+        int32_t start = valid_M == 0 ? K : 0;
+        for (int32_t i = start; i < K + 1; i++)
+        {
+            if (i < K && !(cache & (1u << i)))
+            {
+                orc_xorshift32(&nst);
+                orc_xorshift32(&nst);
+                continue;
+            }
+            int64_t np = i == K ? (int64_t)pix : spatial_neighbor(f, cx, cy, &nst);
+            if (np < 0)
+                continue;
+            ores nr = in[np];
+            float tf_center = 0.0f;
+            if (nr.ucw > 0.0f)
+                tf_center = i == K ? nr.target : target_fn(f, &cs, nr.point, nr.tri, nr.flags, 0);
+            float mis;
+            if (i < K)
+            {
+                float res_M = cw ? (float)nr.M : 1.0f;
+                float nume = nr.target * res_M;
+                float denom = nr.target * conf_sum + tf_center / vdt * center_M;
+                float mi = denom == 0.0f ? 0.0f : nume / denom;
+                if (update_mc)
+                {
+                    osurf ns = surf_of(f, f->cur, (size_t)np);
+                    float tf_csn = target_fn(f, &ns, center.point, center.tri, center.flags, 1);
+                    float denom_mc = tf_csn * conf_sum + nume_mc;
+                    float conf_mult = cw ? (conf_sum == 0.0f ? 0.0f : res_M / conf_sum) : 1.0f;
+                    if (denom_mc != 0.0f)
+                        mc += nume_mc / denom_mc / vdt * conf_mult;
+                }
+                mis = mi / vdt;
+            }
+            else
+                mis = mc == 0.0f ? 1.0f : mc;
+            if (combine_with(&o, &nr, mis, tf_center, 1.0f, &st))
+            {
+                if (i == K)
+                    o.flags |= nr.flags & RESTIR_UNOCCLUDED;
+                else
+                    o.flags &= ~RESTIR_UNOCCLUDED;
+            }
+        }
+        res_end_normalized(&o, 1.0f, 1.0f);
+        if (kill_after)
+            visibility_kill(f, &o, cs.p, cs.prim);
+        if (f->rs.m_cap > 0 && o.M > f->rs.m_cap)
+            o.M = f->rs.m_cap;
+        out[pix] = o;
+    }
+    if (last)
+        end_frame(f, out_i);
+}
+
+int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t rn, float* fb, double* fb64,
+                             int32_t nthreads)
+{
+    f->R.sample_number = sample_number;
+    f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.restir_out = restir_on(f) ? f->res[f->shade_buffer] : NULL;
+    f->R.final_shading_visibility = f->rs.do_final_shading_visibility;
+    int32_t rejected = 0;
+    const int w = f->R.s.width;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic) reduction(+ : rejected)
+    for (int y = 0; y < f->R.s.height; y++)
+        for (int x = 0; x < w; x++)
+        {
+            size_t pix = (size_t)x + (size_t)y * (size_t)w;
+            const gbuf* g = f->cur + pix;
+            rgb c = C1(0.0f);
+            if (g->prim != -1)
+                c = megakernel_pixel(&f->R, x, y, rn, g);
+            int bad = 0;
+            float comp[3] = {c.r, c.g, c.b};
+            for (int k = 0; k < 3; k++)
+                bad |= comp[k] < 0.0f || isnan(comp[k]) || isinf(comp[k]);
+            if (bad)
+            {
+                rejected++;
+                c = C1(0.0f);
+                if (sample_number != 0)
+                    continue;
+            }
+            float* fp = fb + 3 * pix;
+            if (sample_number == 0)
+            {
+                fp[0] = c.r; fp[1] = c.g; fp[2] = c.b;
+            }
+            else
+            {
+                fp[0] += c.r; fp[1] += c.g; fp[2] += c.b;
+            }
+            if (fb64)
+            {
+                double* d = fb64 + 3 * pix;
+                if (sample_number == 0)
+                {
+                    d[0] = c.r; d[1] = c.g; d[2] = c.b;
+                }
+                else
+                {
+                    d[0] += (double)c.r; d[1] += (double)c.g; d[2] += (double)c.b;
+                }
+            }
+        }
+    return rejected;
+}
+
+void orc_frame_read_reservoirs(const orc_frame* f, int32_t which, void* out)
+{
+    const ores* r = f->res[which == 3 ? f->shade_buffer : which];
+    uint32_t* o = (uint32_t*)out;
+    for (size_t i = 0; i < f->n; i++)
+    {
+        float fl[4] = {r[i].point.x, r[i].point.y, r[i].point.z, r[i].target};
+        memcpy(o + 8 * i, fl, 16);
+        int32_t iv[2] = {r[i].M, r[i].tri};
+        memcpy(o + 8 * i + 4, iv, 8);
+        memcpy(o + 8 * i + 6, &r[i].ucw, 4);
+        memcpy(o + 8 * i + 7, &r[i].flags, 4);
+    }
+}
+
+void orc_frame_gbuffer_prims(const orc_frame* f, int32_t* out)
+{
+    for (size_t i = 0; i < f->n; i++)
+        out[i] = f->cur[i].prim;
+}
+
+void orc_frame_alias_table(const orc_frame* f, float* probas, int32_t* alias, float* sum)
+{
+    int32_t E = f->R.ne;
+    if (probas)
+        memcpy(probas, f->alias_probas, sizeof(float) * (size_t)E);
+    if (alias)
+        memcpy(alias, f->alias_alias, sizeof(int32_t) * (size_t)E);
+    if (sum)
+        *sum = f->R.alias_sum;
+}
+
+static int32_t render_sample_via_frame(const orc_render_scene* sc, const orc_octree* tree, const orc_camera* cam,
+                                       const orc_render_settings* s, uint32_t sample_number, uint32_t rn_camera,
+                                       uint32_t rn_path, float* fb, double* fb64, int32_t* gb_prims, int32_t nthreads)
+{
+    orc_render_settings plain = *s;
+    plain.direct_light_sampling_strategy = 3;
+    orc_frame* f = orc_frame_create(sc, tree, cam, &plain, NULL);
+    orc_frame_camera_pass(f, sample_number, rn_camera, nthreads);
+    if (gb_prims)
+        orc_frame_gbuffer_prims(f, gb_prims);
+    int32_t rejected = orc_frame_path_trace(f, sample_number, rn_path, fb, fb64, nthreads);
+    orc_frame_free(f);
+    return rejected;
 }
