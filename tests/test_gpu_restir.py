@@ -83,6 +83,8 @@ def render_both(mipt_mod, oracle_mod, scene, settings, restir, frames, check_eve
           f"{min([s[2] for s in stats]) * 100 if stats else 100:.4f} %; rejected gpu {rej} cpu {R.rejected}")
     assert frac >= MIN_PASS and frac64 >= MIN_PASS
     assert rej == R.rejected
+    # stronger than the bar: bit-identical frames (shared sin/cos/pow sequences)
+    assert identical == fb.size // 3
     ctx.close()
     return fb, R
 
