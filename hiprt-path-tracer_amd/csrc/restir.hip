@@ -438,7 +438,9 @@ __global__ void k_ic_bsdf(PtData d, RsData r)
                 float pdf_area = bpdf;
                 pdf_area /= (t * t);
                 pdf_area *= cos_l;
-                const float weight = mis * target / pdf_area;
+                float weight = mis * target / pdf_area;
+                if (r.s.guard_backfacing_bsdf_candidates && !(pdf_area > 0.0f))
+                    weight = 0.0f;
                 add_candidate(res, point, hp, target, flags, weight, rng);
             }
         }

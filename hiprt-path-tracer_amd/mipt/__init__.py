@@ -121,6 +121,7 @@ class RestirDISettings(C.Structure):
         ("use_normal_similarity_heuristic", C.c_int32), ("normal_similarity_angle_precomp", C.c_float),
         ("use_plane_distance_heuristic", C.c_int32), ("plane_distance_threshold", C.c_float),
         ("reject_using_geometric_normals", C.c_int32), ("do_final_shading_visibility", C.c_int32),
+        ("guard_backfacing_bsdf_candidates", C.c_int32),
     ]
 
 

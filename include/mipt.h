@@ -286,6 +286,12 @@ typedef struct mipt_restir_di_settings {
     float plane_distance_threshold;             /* 0.1 */
     int32_t reject_using_geometric_normals;     /* 1 */
     int32_t do_final_shading_visibility;        /* 1 */
+    /* Not a reference setting, default 0 (the reference's arithmetic): a BSDF
+     * candidate that hits the back of an emissive triangle has an area pdf of
+     * 0 and the reference's weight mis * target / pdf is 0/0 = NaN
+     * (InitialCandidates.h:402-410), which poisons the reservoir and then its
+     * spatial and temporal neighbours. 1 gives such a candidate weight 0. */
+    int32_t guard_backfacing_bsdf_candidates;
 } mipt_restir_di_settings;
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s);

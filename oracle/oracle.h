@@ -201,6 +201,7 @@ typedef struct {
     float plane_distance_threshold;
     int32_t reject_using_geometric_normals;
     int32_t do_final_shading_visibility;
+    int32_t guard_backfacing_bsdf_candidates; /* 0: the reference's 0/0 weight */
 } orc_restir_settings;
 
 typedef struct {

@@ -1329,6 +1329,8 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                 pdf_area /= (h.t * h.t);
                 pdf_area *= cl;
                 float w = mis * target / pdf_area;
+                if (f->rs.guard_backfacing_bsdf_candidates && !(pdf_area > 0.0f))
+                    w = 0.0f;
                 add_candidate(&res, point, h.prim, target, flags, w, &st);
             }
             res_end(&res);

@@ -139,16 +139,18 @@ def test_light_sampling_variants(mipt_mod, oracle_mod, torch_mod, variant):
     render_both(mipt_mod, oracle_mod, scene, s, mipt_mod.default_restir_di_settings(), 4)
 
 
-def test_restir_config2_geometry(mipt_mod, oracle_mod, torch_mod):
+@pytest.mark.parametrize("guard", [0, 1])
+def test_restir_config2_geometry(mipt_mod, oracle_mod, torch_mod, guard):
     """The 1M-triangle scene with 4096 power-sampled emitters and the
     ReSTIR DI parameters of the at-scale configuration (32 light + 1 BSDF
     candidates, temporal M-cap 20, 2 spatial passes of 5 neighbours in 16 px,
-    5 bounces), at 240x135."""
+    5 bounces), at 240x135; with the reference's back-facing candidate weight
+    (0/0) and with the guard."""
     scene = simplify_materials(mipt_mod.generate_scene(2, 240, 135))
     s = mipt_mod.default_render_settings(width=240, height=135, nb_bounces=5, direct_light_sampling_strategy=5,
                                          direct_light_sampling_base_strategy=1)
     rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
-                                             spatial_number_of_passes=2)
+                                             spatial_number_of_passes=2, guard_backfacing_bsdf_candidates=guard)
     fb, R = render_both(mipt_mod, oracle_mod, scene, s, rs, 3)
     assert fb.mean() > 0.0
 

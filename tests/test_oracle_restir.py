@@ -124,3 +124,30 @@ def test_restir_reservoir_invariants(mipt_mod, oracle_mod, scene48):
     sel = hit & (out["ucw"] > 0)
     assert sel.any()
     assert np.isin(out["tri"][sel], R.emissive).all()
+
+
+def test_backfacing_bsdf_candidate_weight(mipt_mod, oracle_mod):
+    """Pixels that see an emissive triangle can send a BSDF candidate into the
+    back of another emitter: the reference's weight is 0/0 = NaN
+    (InitialCandidates.h:402-410) and the reservoir's UCW becomes NaN. With
+    guard_backfacing_bsdf_candidates the candidate weighs 0 and every
+    reservoir stays finite."""
+    W2, H2 = 240, 135
+    scene = mipt_mod.generate_scene(2, W2, H2)
+    for m in scene.materials:
+        m.coat = m.sheen = m.specular_transmission = m.diffuse_transmission = m.thin_film = m.specular = 0.0
+    s = mipt_mod.default_render_settings(width=W2, height=H2, nb_bounces=1, direct_light_sampling_strategy=5,
+                                         direct_light_sampling_base_strategy=1)
+    counts = {}
+    for guard in (0, 1):
+        rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
+                                                 spatial_number_of_passes=2, guard_backfacing_bsdf_candidates=guard)
+        R = oracle_mod.Renderer(scene, s, rs)
+        bad = 0
+        for f in range(2):
+            R.render_frame(f, [mipt_mod.host_random_number(f * 7 + k) for k in range(7)])
+            for which in (0, 1, 2):
+                bad += int((~np.isfinite(R.read_reservoirs(which)["ucw"])).sum())
+        counts[guard] = (bad, R.rejected)
+    assert counts[0][0] > 0
+    assert counts[1] == (0, 0)
