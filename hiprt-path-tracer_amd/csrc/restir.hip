@@ -332,6 +332,8 @@ __global__ void k_ic_start(PtData d, RsData r)
         float cosine = 0.0f, pdf = 0.0f, dist = 0.0f;
         f3 dir = pt::mk(0.0f, 0.0f, 0.0f);
         f3 point = pt::mk(0.0f, 0.0f, 0.0f);
+        f3 ls_normal = pt::mk(0.0f, 0.0f, 0.0f);
+        Color ls_emission = pt::col(0.0f);
         int32_t tri = -1;
         // sample_fresh_light_candidate (:99-178): envmap candidate probability 0
         if (rng() > 0.0f)
@@ -340,6 +342,8 @@ __global__ void k_ic_start(PtData d, RsData r)
             tri = ls.tri;
             point = ls.point;
             pdf = ls.area_pdf;
+            ls_normal = ls.normal;
+            ls_emission = ls.emission;
             if (pdf > 0.0f)
             {
                 const f3 to = point - s.p;
@@ -360,15 +364,17 @@ __global__ void k_ic_start(PtData d, RsData r)
         {
             pt::BsdfCtx c{s.view, s.sn, s.gn, dir, pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f};
             float bsdf_pdf;
-            pt::principled_eval(d.g, m, c, bsdf_pdf);
-            f3 tdir;
-            float tdist;
-            const float target = target_function(d, s, point, tri, 0u, tdir, tdist);
+            const Color f = pt::principled_eval(d.g, m, c, bsdf_pdf);
+            // ReSTIR_DI_evaluate_target_function<false> of this light sample:
+            // its BSDF evaluation has the same arguments as the one above (the
+            // direction is recomputed from the same point by the same
+            // operations), and the normalised triangle normal equals the light
+            // sample's normal bit for bit, so both are reused.
+            const float geometry = cos_at_light(ls_normal, -1.0f * dir) / (dist * dist);
+            const float target = luminance(((f * ls_emission) * cosine) * geometry);
             if (!(bsdf_pdf <= 0.0f))
             {
-                f3 ln = tri_normal_unnormalized(d, tri);
-                const float nl_len = sqrtf(pt::dot(ln, ln));
-                ln = pt::vdiv(ln, nl_len);
+                const f3 ln = ls_normal;
                 float lpdf = area_to_solid_angle(pdf / (1.0f - 0.0f), dist, cos_at_light(ln, -1.0f * dir));
                 lpdf *= (1.0f - 0.0f);
                 const float mis = balance_heuristic(lpdf, (float)nl, bsdf_pdf, (float)nb);
