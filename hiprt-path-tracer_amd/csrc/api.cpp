@@ -8,6 +8,7 @@
 #include <string>
 
 #include "internal.h"
+#include "nrc.h"
 
 namespace mipt {
 
@@ -352,6 +353,18 @@ int mipt_set_scene(mipt_ctx* c, const mipt_scene_desc* s)
         ctx->host_indices.assign(s->triangles_vertex_indices,
                                  s->triangles_vertex_indices + 3 * (size_t)s->triangle_count);
         ctx->render.scene_ready = false;
+        for (int a = 0; a < 3; a++)
+        {
+            ctx->scene_bmin[a] = s->vertex_count > 0 ? s->vertices_positions[a] : 0.0f;
+            ctx->scene_bmax[a] = ctx->scene_bmin[a];
+        }
+        for (int32_t v = 0; v < s->vertex_count; v++)
+            for (int a = 0; a < 3; a++)
+            {
+                const float x = s->vertices_positions[3 * (size_t)v + a];
+                ctx->scene_bmin[a] = x < ctx->scene_bmin[a] ? x : ctx->scene_bmin[a];
+                ctx->scene_bmax[a] = x > ctx->scene_bmax[a] ? x : ctx->scene_bmax[a];
+            }
         ctx->triangle_count = s->triangle_count;
         ctx->vertex_count = s->vertex_count;
         ctx->material_count = s->material_count;
@@ -739,6 +752,148 @@ int mipt_rejected_samples(mipt_ctx* c, uint32_t* out_count)
         HIP_CHECK(hipMemcpy(out_count, ctx->render.counters.ptr, 4, hipMemcpyDeviceToHost));
     });
 }
+
+void mipt_radiance_cache_settings_default(mipt_radiance_cache_settings* s)
+{
+    if (!s)
+        return;
+    std::memset(s, 0, sizeof(*s));
+    s->grid_levels = 16;
+    s->grid_log2_table = 19;
+    s->grid_base_resolution = 16;
+    s->grid_per_level_scale = 1.3819f;
+    s->batch_size = 1 << 16;
+    s->learning_rate = 1.0e-2f;
+    s->adam_beta1 = 0.9f;
+    s->adam_beta2 = 0.99f;
+    s->adam_eps = 1.0e-8f;
+    s->seed = 1337u;
+    s->query_bounce = 2;
+    s->train_every = 16;
+    s->train_extension = 3;
+}
+
+int mipt_set_radiance_cache(mipt_ctx* c, const mipt_radiance_cache_settings* s)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !s || s->grid_levels < 1 || s->grid_levels > 16 || s->grid_log2_table < 4 ||
+        s->grid_log2_table > 24 || s->grid_base_resolution < 2 || !(s->grid_per_level_scale >= 1.0f) ||
+        s->batch_size < 16 || s->batch_size > (1 << 22) || !(s->learning_rate > 0.0f) ||
+        !(s->adam_beta1 >= 0.0f && s->adam_beta1 < 1.0f) || !(s->adam_beta2 >= 0.0f && s->adam_beta2 < 1.0f) ||
+        !(s->adam_eps >= 0.0f) || s->query_bounce < 1 || s->train_every < 1 || s->train_extension < 0)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->scene_set)
+            mipt::fail(MIPT_ERR_NO_SCENE, "no scene: the radiance cache spans the scene's bounds");
+        mipt::nrc_configure(*ctx, *s);
+    });
+}
+
+int mipt_radiance_cache_infer(mipt_ctx* c, const float* d_queries, uint32_t n, float* d_out, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_queries || !d_out)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::nrc_infer(*ctx, d_queries, nullptr, n, d_out, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_radiance_cache_encode(mipt_ctx* c, const float* d_queries, uint32_t n, uint16_t* d_out, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_queries || !d_out)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::nrc_encode(*ctx, d_queries, n, d_out, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_radiance_cache_train_step(mipt_ctx* c, const float* d_queries, const float* d_targets, uint32_t n,
+                                   float* out_loss, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_queries || !d_targets)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (n == 0)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_radiance_cache_train_step: empty batch");
+        mipt::nrc_gradients(*ctx, d_queries, d_targets, nullptr, n, s);
+        mipt::nrc_apply_adam(*ctx, s);
+        if (out_loss)
+        {
+            float sum = 0.0f;
+            HIP_CHECK(hipMemcpyAsync(&sum, ctx->nrc.loss.ptr, 4, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            const uint32_t rows = std::min<uint32_t>(n, (uint32_t)ctx->nrc.settings.batch_size);
+            *out_loss = sum / (3.0f * (float)rows);
+        }
+    });
+}
+
+int mipt_radiance_cache_gradients(mipt_ctx* c, const float* d_queries, const float* d_targets, uint32_t n,
+                                  float* host_mlp_grad, float* host_grid_grad, float* out_loss)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || n == 0 || !d_queries || !d_targets)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::Nrc& C = ctx->nrc;
+        mipt::nrc_gradients(*ctx, d_queries, d_targets, nullptr, n, nullptr);
+        HIP_CHECK(hipDeviceSynchronize());
+        if (host_mlp_grad)
+            HIP_CHECK(hipMemcpy(host_mlp_grad, C.w_grad.ptr, 4 * (size_t)mipt::NRC_WEIGHTS, hipMemcpyDeviceToHost));
+        if (host_grid_grad)
+            HIP_CHECK(hipMemcpy(host_grid_grad, C.grid_grad.ptr, 4 * C.grid_count, hipMemcpyDeviceToHost));
+        if (out_loss)
+        {
+            float sum = 0.0f;
+            HIP_CHECK(hipMemcpy(&sum, C.loss.ptr, 4, hipMemcpyDeviceToHost));
+            const uint32_t rows = std::min<uint32_t>(n, (uint32_t)C.settings.batch_size);
+            *out_loss = sum / (3.0f * (float)rows);
+        }
+        HIP_CHECK(hipMemset(C.w_grad.ptr, 0, 4 * (size_t)mipt::NRC_WEIGHTS));
+        HIP_CHECK(hipMemset(C.grid_grad.ptr, 0, 4 * C.grid_count));
+    });
+}
+
+int mipt_radiance_cache_get_params(mipt_ctx* c, float* mlp_master, uint16_t* mlp_bf16, float* grid_master,
+                                   uint16_t* grid_f16)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::Nrc& C = ctx->nrc;
+        if (!C.configured)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "radiance cache not configured: call mipt_set_radiance_cache first");
+        HIP_CHECK(hipDeviceSynchronize());
+        if (mlp_master)
+            HIP_CHECK(hipMemcpy(mlp_master, C.w_master.ptr, 4 * (size_t)mipt::NRC_WEIGHTS, hipMemcpyDeviceToHost));
+        if (mlp_bf16)
+            HIP_CHECK(hipMemcpy(mlp_bf16, C.wk.ptr, 2 * (size_t)mipt::NRC_WEIGHTS, hipMemcpyDeviceToHost));
+        if (grid_master)
+            HIP_CHECK(hipMemcpy(grid_master, C.grid_master.ptr, 4 * C.grid_count, hipMemcpyDeviceToHost));
+        if (grid_f16)
+            HIP_CHECK(hipMemcpy(grid_f16, C.grid_f16.ptr, 2 * C.grid_count, hipMemcpyDeviceToHost));
+    });
+}
+
+int mipt_radiance_cache_set_params(mipt_ctx* c, const float* mlp_master, const float* grid_master)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->nrc.configured)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "radiance cache not configured: call mipt_set_radiance_cache first");
+        HIP_CHECK(hipDeviceSynchronize());
+        mipt::nrc_set_params(*ctx, mlp_master, grid_master);
+    });
+}
+
+int32_t mipt_radiance_cache_weight_count(void) { return mipt::NRC_WEIGHTS; }
 
 uint32_t mipt_host_random_number(uint32_t pass_index)
 {

@@ -95,10 +95,27 @@ struct RenderState {
     DeviceBuffer rs_state, rs_rays, rs_occ, rs_hits, rs_aux;
 };
 
+// Learned radiance cache state (nrc.hip).
+struct Nrc {
+    bool configured = false;
+    mipt_radiance_cache_settings settings{};
+    size_t grid_count = 0;        // levels * table * 2 features
+    uint32_t step = 0;            // Adam steps taken
+    DeviceBuffer grid_f16, grid_master, grid_m, grid_v, grid_grad;
+    DeviceBuffer w_master, w_m, w_v, w_grad, wt, wk;
+    DeviceBuffer x0, xcm, dycm, dx0, loss, count;
+    // records of the path pass
+    DeviceBuffer queries, query_pix, query_out, counters, train_queries, train_targets, train_slot,
+        train_thr, train_rad;
+};
+
 struct Ctx {
     int device = 0;
     int cu_count = 0;
     std::string last_error;
+    float scene_bmin[3] = {0.0f, 0.0f, 0.0f};
+    float scene_bmax[3] = {1.0f, 1.0f, 1.0f};
+    Nrc nrc;
 
     // Scene (device copies of the upload arrays).
     int32_t triangle_count = 0;

@@ -106,7 +106,7 @@ class RenderSettings(C.Structure):
         ("enable_direct", C.c_int32), ("microfacet_regularization_tau_0", C.c_float),
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
-        ("direct_light_sampling_base_strategy", C.c_int32),
+        ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
     ]
 
 
@@ -125,10 +125,22 @@ class RestirDISettings(C.Structure):
     ]
 
 
+class RadianceCacheSettings(C.Structure):
+    """mipt_radiance_cache_settings (the learned radiance cache; no reference counterpart)."""
+
+    _fields_ = [
+        ("grid_levels", C.c_int32), ("grid_log2_table", C.c_int32), ("grid_base_resolution", C.c_int32),
+        ("grid_per_level_scale", C.c_float), ("batch_size", C.c_int32), ("learning_rate", C.c_float),
+        ("adam_beta1", C.c_float), ("adam_beta2", C.c_float), ("adam_eps", C.c_float), ("seed", C.c_uint32),
+        ("query_bounce", C.c_int32), ("train_every", C.c_int32), ("train_extension", C.c_int32),
+    ]
+
+
 MIPT_PASS_CAMERA_RAYS = 0
 MIPT_PASS_PATH_TRACE = 1
 MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES = 2
 MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE = 3
+MIPT_PASS_RADIANCE_CACHE_TRAIN = 4
 
 
 def MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i: int) -> int:
@@ -187,6 +199,17 @@ _SIGNATURES = {
     "mipt_get_power_alias_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "mipt_compute_alias_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p]),
     "mipt_read_restir_reservoirs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "mipt_radiance_cache_settings_default": (None, [C.POINTER(RadianceCacheSettings)]),
+    "mipt_set_radiance_cache": (C.c_int, [C.c_void_p, C.POINTER(RadianceCacheSettings)]),
+    "mipt_radiance_cache_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mipt_radiance_cache_encode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mipt_radiance_cache_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                 C.POINTER(C.c_float), C.c_void_p]),
+    "mipt_radiance_cache_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                                C.c_void_p, C.POINTER(C.c_float)]),
+    "mipt_radiance_cache_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mipt_radiance_cache_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mipt_radiance_cache_weight_count": (C.c_int32, []),
 }
 
 _lib = None
@@ -253,6 +276,14 @@ def compute_alias_table(weights, total_sum=None):
     if rc != MIPT_OK:
         raise MiptError(rc, "mipt_compute_alias_table")
     return probas, alias
+
+
+def default_radiance_cache_settings(**overrides) -> RadianceCacheSettings:
+    s = RadianceCacheSettings()
+    load_library().mipt_radiance_cache_settings_default(C.byref(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
 
 
 def default_material() -> Material:
@@ -475,6 +506,55 @@ class Context:
         self._check(self.lib.mipt_read_restir_reservoirs(self.h, which, out.ctypes.data_as(C.c_void_p)),
                     "mipt_read_restir_reservoirs")
         return out
+
+    # Learned radiance cache
+    def set_radiance_cache(self, settings: RadianceCacheSettings):
+        self.cache = settings
+        self._check(self.lib.mipt_set_radiance_cache(self.h, C.byref(settings)), "mipt_set_radiance_cache")
+
+    def cache_infer(self, d_queries, n: int, d_out, stream=None):
+        self._check(self.lib.mipt_radiance_cache_infer(self.h, ptr(d_queries), n, ptr(d_out), stream),
+                    "mipt_radiance_cache_infer")
+
+    def cache_encode(self, d_queries, n: int, d_out, stream=None):
+        self._check(self.lib.mipt_radiance_cache_encode(self.h, ptr(d_queries), n, ptr(d_out), stream),
+                    "mipt_radiance_cache_encode")
+
+    def cache_train_step(self, d_queries, d_targets, n: int, want_loss: bool = True, stream=None):
+        loss = C.c_float()
+        self._check(self.lib.mipt_radiance_cache_train_step(self.h, ptr(d_queries), ptr(d_targets), n,
+                                                            C.byref(loss) if want_loss else None, stream),
+                    "mipt_radiance_cache_train_step")
+        return loss.value if want_loss else None
+
+    def cache_gradients(self, d_queries, d_targets, n: int):
+        """(MLP gradient [K][N] per layer, grid gradient, loss) of one batch, without an update."""
+        wg = np.zeros(int(self.lib.mipt_radiance_cache_weight_count()), np.float32)
+        gg = np.zeros(self.cache.grid_levels * (1 << self.cache.grid_log2_table) * 2, np.float32)
+        loss = C.c_float()
+        self._check(self.lib.mipt_radiance_cache_gradients(self.h, ptr(d_queries), ptr(d_targets), n,
+                                                           wg.ctypes.data_as(C.c_void_p),
+                                                           gg.ctypes.data_as(C.c_void_p), C.byref(loss)),
+                    "mipt_radiance_cache_gradients")
+        return wg, gg, loss.value
+
+    def cache_params(self):
+        """(MLP FP32 master, MLP bf16 bits, grid FP32 master, grid FP16 bits)."""
+        nw = int(self.lib.mipt_radiance_cache_weight_count())
+        ng = self.cache.grid_levels * (1 << self.cache.grid_log2_table) * 2
+        wm, wb = np.zeros(nw, np.float32), np.zeros(nw, np.uint16)
+        gm, gh = np.zeros(ng, np.float32), np.zeros(ng, np.uint16)
+        self._check(self.lib.mipt_radiance_cache_get_params(self.h, *(a.ctypes.data_as(C.c_void_p)
+                                                                      for a in (wm, wb, gm, gh))),
+                    "mipt_radiance_cache_get_params")
+        return wm, wb, gm, gh
+
+    def set_cache_params(self, mlp_master=None, grid_master=None):
+        def p(a):
+            return None if a is None else np.ascontiguousarray(a, np.float32).ctypes.data_as(C.c_void_p)
+        keep = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (mlp_master, grid_master)]
+        self._check(self.lib.mipt_radiance_cache_set_params(self.h, p(keep[0]), p(keep[1])),
+                    "mipt_radiance_cache_set_params")
 
     def rejected_samples(self) -> int:
         v = C.c_uint32()

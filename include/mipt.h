@@ -253,6 +253,9 @@ typedef struct mipt_render_settings {
      * Defaults: MIS light + BSDF, uniform. */
     int32_t direct_light_sampling_strategy;
     int32_t direct_light_sampling_base_strategy;
+    /* 1: query the learned radiance cache (see mipt_set_radiance_cache); not
+     * a reference setting. Default 0. */
+    int32_t radiance_cache;
 } mipt_render_settings;
 
 #define MIPT_LSS_MIS_LIGHT_BSDF 3 /* LSS_MIS_LIGHT_BSDF: one light sample + one BSDF sample, balance heuristic */
@@ -345,6 +348,71 @@ int mipt_get_power_alias_table(mipt_ctx* ctx, float* probas, int32_t* alias, flo
 /* Vose alias table of n non-negative weights, normalised in FP64
  * (Utils::compute_alias_table, Utils.cpp:63-131). Host utility. */
 int mipt_compute_alias_table(const float* weights, int32_t n, float total_sum, float* out_probas, int32_t* out_alias);
+
+/* ------------------------------------------------------------------ */
+/* Learned radiance cache. No counterpart in the reference: a neural   */
+/* radiance cache trained online from the tracer's own paths. The      */
+/* position goes through a multiresolution hash grid (FP16 features,   */
+/* Instant-NGP hashing); direction and normal (octahedral) and         */
+/* roughness through one-blob encodings; albedo as is. A 6-matrix ReLU */
+/* MLP of width 64 with BF16 weights runs on the MFMA units (FP32      */
+/* accumulation); relative L2 loss, Adam.                               */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_radiance_cache_settings {
+    int32_t grid_levels;          /* 16 (at most 16) */
+    int32_t grid_log2_table;      /* 19: 2^19 entries x 2 features per level */
+    int32_t grid_base_resolution; /* 16 */
+    float grid_per_level_scale;   /* 1.3819: 2048 cells across the scene at level 15 */
+    int32_t batch_size;           /* 65536 training records per step at most */
+    float learning_rate;          /* 1e-2 */
+    float adam_beta1;             /* 0.9 */
+    float adam_beta2;             /* 0.99 */
+    float adam_eps;               /* 1e-8 */
+    uint32_t seed;                /* parameter initialisation stream */
+    int32_t query_bounce;         /* 2: paths stop at this bounce with the cache's radiance */
+    int32_t train_every;          /* 16: one path in train_every is a training path */
+    int32_t train_extension;      /* 3: training paths trace this many bounces past query_bounce */
+} mipt_radiance_cache_settings;
+
+void mipt_radiance_cache_settings_default(mipt_radiance_cache_settings* s);
+/* (Re)initialises the cache's parameters (Glorot-uniform weights, grid
+ * features U(-1e-4, 1e-4), from `seed`) and optimiser state. Needs a scene
+ * (the grid spans the scene's bounding cube). */
+int mipt_set_radiance_cache(mipt_ctx* ctx, const mipt_radiance_cache_settings* s);
+
+/* Query record: 16 floats = position[3], outgoing direction[3] (towards the
+ * viewer), shading normal[3], roughness, albedo[3], 3 unused. Output: 4
+ * floats per row (RGB radiance, 0). Device pointers. */
+int mipt_radiance_cache_infer(mipt_ctx* ctx, const float* d_queries, uint32_t n, float* d_out, void* stream);
+/* The network's input rows for n query records: 64 bfloat16 values per row
+ * (grid features level by level, then the one-blob and albedo inputs, a
+ * constant 1 and zero padding). Device pointers. */
+int mipt_radiance_cache_encode(mipt_ctx* ctx, const float* d_queries, uint32_t n, uint16_t* d_out, void* stream);
+/* One optimiser step on n records (targets: 4 floats per row, RGB + unused).
+ * *out_loss (host, may be NULL) receives the batch's relative L2 loss;
+ * synchronous when out_loss is given. */
+int mipt_radiance_cache_train_step(mipt_ctx* ctx, const float* d_queries, const float* d_targets, uint32_t n,
+                                   float* out_loss, void* stream);
+/* Gradients of the loss on n records, without an update (for gradient
+ * checks): host_mlp_grad float[weights] in the [K][N] layout of each layer,
+ * host_grid_grad float[levels * 2^log2_table * 2] (either may be NULL).
+ * Synchronous; the accumulated gradients are cleared afterwards. */
+int mipt_radiance_cache_gradients(mipt_ctx* ctx, const float* d_queries, const float* d_targets, uint32_t n,
+                                  float* host_mlp_grad, float* host_grid_grad, float* out_loss);
+/* Parameter readback / upload (host arrays; any may be NULL). MLP:
+ * float master and bf16 copy, [K][N] per layer, 5 x 64 x 64 + 64 x 16.
+ * Grid: float master and FP16 copy, [level][entry][feature]. */
+int mipt_radiance_cache_get_params(mipt_ctx* ctx, float* mlp_master, uint16_t* mlp_bf16, float* grid_master,
+                                   uint16_t* grid_f16);
+int mipt_radiance_cache_set_params(mipt_ctx* ctx, const float* mlp_master, const float* grid_master);
+int32_t mipt_radiance_cache_weight_count(void);
+
+/* The path pass with render_settings.radiance_cache = 1 stops paths at
+ * query_bounce and adds throughput x the cache's prediction; one path in
+ * train_every continues train_extension more bounces and records a training
+ * target. RADIANCE_CACHE_TRAIN then takes one optimiser step on the records
+ * of the last path pass. */
+#define MIPT_PASS_RADIANCE_CACHE_TRAIN 4
 
 /* Reservoir readback: which = 0 initial-candidates buffer, 1 / 2 the spatial
  * output buffers, 3 the reservoirs the path pass shades. Per pixel 8 words:
