@@ -13,9 +13,12 @@ on the launch stream) and the CPU baseline (the oracle's restatement of the
 reference CPU renderer's octree traversal, OpenMP over all host cores, on the
 same rays). A second object, "plain_render", times the plain path tracer on
 the 512x512 Cornell-box scene (16 x 1-spp passes, 4 bounces) beside the
-oracle's CPU megakernel restatement.
+oracle's CPU megakernel restatement. A third, "cache_vs_plain", is the
+second half of the metric: equal-time relMSE of the learned radiance cache
+against the plain tracer at 1920x1080 on the config-1 scene.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-render]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-render] [--no-cache]
+       [--cache-ref-spp 512] [--cache-plain-spp 32] [--cache-warmup 8]
 """
 import argparse
 import json
@@ -43,6 +46,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-render", action="store_true", help="skip the plain-render timing leg")
+    ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain leg")
+    ap.add_argument("--cache-ref-spp", type=int, default=512,
+                    help="samples of the FP64 plain reference of the cache leg")
+    ap.add_argument("--cache-plain-spp", type=int, default=32, help="plain samples whose time is the budget")
+    ap.add_argument("--cache-warmup", type=int, default=8,
+                    help="training-only frames before the cache form accumulates (their time counts)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -148,6 +157,8 @@ def main():
 
     render = None if args.no_render else plain_render_leg(mipt, torch, np, args.no_cpu)
     restir = None if args.no_render else restir_render_leg(mipt, torch, np)
+    cache = None if args.no_cache else cache_vs_plain_leg(mipt, torch, np, args.cache_ref_spp, args.cache_plain_spp,
+                                                          args.cache_warmup)
 
     out = {
         "metric": METRIC,
@@ -177,9 +188,127 @@ def main():
         "hit_fraction": round(float((hits["prim"] >= 0).mean()), 4),
         "plain_render": render,
         "restir_render": restir,
+        "cache_vs_plain": cache,
     }
     print(json.dumps(out))
     ctx.close()
+
+
+def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup):
+    """Equal-time relMSE, learned radiance cache vs plain, config-1 scene at
+    1920x1080, 4 bounces, uniform NEE + MIS. relMSE = mean over pixels and
+    channels of (x - r)^2 / (r^2 + 1e-2), reduced in float64, against a plain
+    render with FP64 accumulation at ref_spp samples. The plain form renders
+    plain_spp samples; its wall time is the budget. The cache form (queried at
+    bounce 2, trained every frame on 1/16 of the paths, batches of 2^16, Adam
+    lr 1e-2) first renders `warmup` training-only frames, whose time counts,
+    then accumulates until it matches the plain relMSE or runs 3x the budget.
+    Reported: both relMSE at equal time and the cache's time to reach the
+    plain relMSE as a fraction of the plain time (interpolated between
+    checkpoints in log-log). Wall times are host clock around synchronised
+    sample loops, excluding the relMSE evaluations."""
+    W, H = 1920, 1080
+    scene = mipt.generate_scene(1, W, H)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def make(cache):
+        ctx = mipt.Context(0)
+        ctx.set_scene(scene)
+        ctx.build_bvh()
+        if cache:
+            ctx.set_radiance_cache(mipt.default_radiance_cache_settings())
+        ctx.set_render_settings(mipt.default_render_settings(width=W, height=H, nb_bounces=4, accumulate_fp64=1,
+                                                             radiance_cache=1 if cache else 0))
+        ctx.set_camera(scene.camera)
+        return ctx
+
+    ctx = make(False)
+    t0 = time.perf_counter()
+    for s in range(ref_spp):
+        ctx.render_sample(s, sh)
+    stream.synchronize()
+    ref_s = time.perf_counter() - t0
+    ref = ctx.read_framebuffer(fp64=True)[1] / ref_spp
+    ctx.close()
+
+    def relmse(ctx, spp):
+        img = ctx.read_framebuffer(fp64=True)[1] / spp
+        return float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
+
+    checkpoints = sorted({1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, plain_spp})
+
+    def run(ctx, first_sample, stop):
+        """[(seconds, spp, relMSE)] at the checkpoints, until stop(curve)."""
+        t = 0.0
+        curve = []
+        s = 0
+        while True:
+            stream.synchronize()
+            t1 = time.perf_counter()
+            ctx.render_sample(first_sample + s if s else 0, sh)
+            stream.synchronize()
+            t += time.perf_counter() - t1
+            s += 1
+            if s in checkpoints or s % 256 == 0:
+                curve.append((t, s, relmse(ctx, s)))
+                if stop(curve):
+                    return curve
+
+    ctx = make(False)
+    plain = run(ctx, 0, lambda c: c[-1][1] >= plain_spp)
+    ctx.close()
+    budget, target = plain[-1][0], plain[-1][2]
+
+    ctx = make(True)
+    tw = time.perf_counter()
+    for i in range(warmup):
+        ctx.render_sample(1000000 + i, sh)
+    stream.synchronize()
+    warm_s = time.perf_counter() - tw
+    cached = run(ctx, 0, lambda c: (c[-1][0] + warm_s >= budget and c[-1][2] <= target)
+                 or c[-1][0] + warm_s >= 3.0 * budget)
+    queries, train, dropped, loss = ctx.cache_stats()
+    ctx.close()
+    cached = [(t + warm_s, s, e) for t, s, e in cached]
+
+    def at_time(curve, t):
+        # relMSE at time t (log-log interpolation; the last point beyond)
+        prev = None
+        for p in curve:
+            if p[0] >= t:
+                if prev is None:
+                    return p[2]
+                a = (np.log(t) - np.log(prev[0])) / (np.log(p[0]) - np.log(prev[0]))
+                return float(np.exp(np.log(prev[2]) + a * (np.log(p[2]) - np.log(prev[2]))))
+            prev = p
+        return curve[-1][2]
+
+    reach = None
+    prev = (warm_s, 0, None)
+    for p in cached:
+        if p[2] <= target:
+            if prev[2] is None:
+                reach = p[0]
+            else:
+                a = (np.log(target) - np.log(prev[2])) / (np.log(p[2]) - np.log(prev[2]))
+                reach = float(np.exp(np.log(prev[0]) + a * (np.log(p[0]) - np.log(prev[0]))))
+            break
+        prev = p
+    return {"workload": "config 1 scene (102,428 tris) 1920x1080, 4 bounces, uniform NEE + MIS; cache queried at "
+                        "bounce 2, trained on 1/16 of paths (batch 2^16, Adam lr 1e-2, relative L2)",
+            "reference": f"plain, FP64 accumulation, {ref_spp} spp ({ref_s:.1f} s)",
+            "plain_spp": plain_spp, "plain_seconds": round(budget, 4), "plain_relmse": target,
+            "plain_ms_per_spp": round(budget / plain_spp * 1e3, 3),
+            "cache_relmse_at_equal_time": at_time(cached, budget),
+            "cache_seconds_to_plain_relmse": None if reach is None else round(reach, 4),
+            "time_ratio_cache_over_plain": None if reach is None else round(reach / budget, 4),
+            "cache_warmup_frames": warmup, "cache_warmup_seconds": round(warm_s, 4),
+            "cache_ms_per_spp": round((cached[-1][0] - warm_s) / cached[-1][1] * 1e3, 3),
+            "cache_queries_last_frame": queries, "cache_training_records_last_frame": train,
+            "cache_dropped_targets": dropped, "cache_train_loss_last_frame": loss,
+            "curve_plain": [[round(t, 4), s, e] for t, s, e in plain],
+            "curve_cache": [[round(t, 4), s, e] for t, s, e in cached]}
 
 
 def restir_render_leg(mipt, torch, np):

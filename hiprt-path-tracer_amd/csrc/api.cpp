@@ -137,6 +137,10 @@ void allocate_render_buffers(Ctx& ctx)
     R.bsdf_hits.reserve(16 * n);
     R.cont_rays.reserve(32 * n);
     R.cont_hits.reserve(16 * n);
+    R.q_shadow.reserve(4 * n);
+    R.q_bsdf.reserve(4 * n);
+    R.q_cont.reserve(4 * n);
+    R.q_counts.reserve(64);
     R.fb.reserve(12 * n);
     HIP_CHECK(hipMemset(R.fb.ptr, 0, 12 * n));
     if (R.settings.accumulate_fp64)
@@ -547,7 +551,8 @@ int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
         (s->direct_light_sampling_strategy != MIPT_LSS_MIS_LIGHT_BSDF &&
          s->direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI) ||
         (s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_UNIFORM &&
-         s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_POWER))
+         s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_POWER) ||
+        (s->radiance_cache != 0 && s->radiance_cache != 1))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         ctx->render.settings = *s;
@@ -574,7 +579,7 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
     const bool spatial = pass >= MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0) && pass < MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(64);
     if (!ctx || (pass != MIPT_PASS_CAMERA_RAYS && pass != MIPT_PASS_PATH_TRACE &&
                  pass != MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES && pass != MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE &&
-                 !spatial))
+                 pass != MIPT_PASS_RADIANCE_CACHE_TRAIN && !spatial))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         if (!ctx->render.configured)
@@ -591,6 +596,8 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
             mipt::restir_initial_pass(*ctx, sample_number, random_number, s);
         else if (pass == MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE)
             mipt::restir_temporal_pass(*ctx, sample_number, random_number, s);
+        else if (pass == MIPT_PASS_RADIANCE_CACHE_TRAIN)
+            mipt::nrc_train_pass(*ctx, s);
         else
             mipt::restir_spatial_pass(*ctx, pass - MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0), sample_number, random_number,
                                       s);
@@ -642,7 +649,10 @@ int mipt_render_sample(mipt_ctx* c, uint32_t sample_number, void* stream)
     }
     if (rc != MIPT_OK)
         return rc;
-    return mipt_launch_pass(c, MIPT_PASS_PATH_TRACE, sample_number, mipt_host_random_number(k), stream);
+    rc = mipt_launch_pass(c, MIPT_PASS_PATH_TRACE, sample_number, mipt_host_random_number(k), stream);
+    if (rc == MIPT_OK && R.settings.radiance_cache)
+        rc = mipt_launch_pass(c, MIPT_PASS_RADIANCE_CACHE_TRAIN, sample_number, 0u, stream);
+    return rc;
 }
 
 int mipt_get_power_alias_table(mipt_ctx* c, float* probas, int32_t* alias, float* sum)
@@ -819,7 +829,7 @@ int mipt_radiance_cache_train_step(mipt_ctx* c, const float* d_queries, const fl
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
         if (n == 0)
             mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_radiance_cache_train_step: empty batch");
-        mipt::nrc_gradients(*ctx, d_queries, d_targets, nullptr, n, s);
+        mipt::nrc_gradients(*ctx, d_queries, d_targets, nullptr, n, 1u, 0u, true, s);
         mipt::nrc_apply_adam(*ctx, s);
         if (out_loss)
         {
@@ -840,7 +850,7 @@ int mipt_radiance_cache_gradients(mipt_ctx* c, const float* d_queries, const flo
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         mipt::Nrc& C = ctx->nrc;
-        mipt::nrc_gradients(*ctx, d_queries, d_targets, nullptr, n, nullptr);
+        mipt::nrc_gradients(*ctx, d_queries, d_targets, nullptr, n, 1u, 0u, true, nullptr);
         HIP_CHECK(hipDeviceSynchronize());
         if (host_mlp_grad)
             HIP_CHECK(hipMemcpy(host_mlp_grad, C.w_grad.ptr, 4 * (size_t)mipt::NRC_WEIGHTS, hipMemcpyDeviceToHost));
@@ -894,6 +904,29 @@ int mipt_radiance_cache_set_params(mipt_ctx* c, const float* mlp_master, const f
 }
 
 int32_t mipt_radiance_cache_weight_count(void) { return mipt::NRC_WEIGHTS; }
+
+int mipt_radiance_cache_stats(mipt_ctx* c, uint32_t* counts, float* train_loss)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::Nrc& C = ctx->nrc;
+        if (!C.configured || C.train_capacity == 0)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no path pass with the radiance cache has run");
+        HIP_CHECK(hipDeviceSynchronize());
+        uint32_t h[4] = {0, 0, 0, 0};
+        HIP_CHECK(hipMemcpy(h, C.counters.ptr, 12, hipMemcpyDeviceToHost));
+        if (counts)
+            std::memcpy(counts, h, 12);
+        if (train_loss)
+        {
+            float sum = 0.0f;
+            HIP_CHECK(hipMemcpy(&sum, C.loss.ptr, 4, hipMemcpyDeviceToHost));
+            *train_loss = h[1] ? sum / (3.0f * (float)h[1]) : 0.0f;
+        }
+    });
+}
 
 uint32_t mipt_host_random_number(uint32_t pass_index)
 {

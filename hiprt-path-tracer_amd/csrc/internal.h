@@ -74,8 +74,9 @@ struct RenderState {
     DeviceBuffer gb_prim, gb_pos, gb_gn, gb_sn, gb_mat;
     // path state
     DeviceBuffer throughput, thr_prev, radiance, vtx_pos, vtx_sn, vtx_gn, vtx_view, nee_light, nee_bsdf, rng, flags;
-    // rays of one bounce
+    // rays of one bounce, and the queues of live slots the traversal takes
     DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
+    DeviceBuffer q_shadow, q_bsdf, q_cont, q_counts;
     // framebuffer
     DeviceBuffer fb, fb64, counters;
     // power alias table of the emissive triangles (LSS_BASE_POWER)
@@ -101,12 +102,15 @@ struct Nrc {
     mipt_radiance_cache_settings settings{};
     size_t grid_count = 0;        // levels * table * 2 features
     uint32_t step = 0;            // Adam steps taken
+    // row selection of the last gradient batch (the Adam step skips an empty one)
+    const uint32_t* last_sel_count = nullptr;
+    uint32_t last_sel_stride = 1, last_sel_offset = 0, last_sel_limit = 0;
     DeviceBuffer grid_f16, grid_master, grid_m, grid_v, grid_grad;
     DeviceBuffer w_master, w_m, w_v, w_grad, wt, wk;
     DeviceBuffer x0, xcm, dycm, dx0, loss, count;
     // records of the path pass
-    DeviceBuffer queries, query_pix, query_out, counters, train_queries, train_targets, train_slot,
-        train_thr, train_rad;
+    DeviceBuffer queries, query_pix, query_out, counters, train_queries, train_targets, train_pix, train_rad;
+    uint32_t train_capacity = 0;  // training records one path pass can hold
 };
 
 struct Ctx {
@@ -175,6 +179,12 @@ void build_lbvh(Ctx& ctx);
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n,
                    hipStream_t stream);
 void trace_any(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, uint32_t n, hipStream_t stream);
+// Trace the rays whose slots are listed in queue[0, *queue_count) (count read
+// on the device, at most capacity); results go to the same slots.
+void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const uint32_t* queue,
+                         const uint32_t* queue_count, uint32_t capacity, hipStream_t stream);
+void trace_any_queue(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, const uint32_t* queue,
+                     const uint32_t* queue_count, uint32_t capacity, hipStream_t stream);
 
 // camera.hip
 void camera_rays(Ctx& ctx, const mipt_camera& cam, int32_t w, int32_t h, uint32_t sample_number,

@@ -104,23 +104,25 @@ __device__ __forceinline__ void normalized_position(const NrcDev& P, const float
 }
 
 // Query record (16 floats): position, direction, normal, roughness, albedo.
-// Encodes rows [0, n); rows at or past min(*count, rows_max) are zero.
-__global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, const uint32_t* count, uint32_t rows_max,
-                             uint32_t n, uint16_t* __restrict__ x0)
+// Encodes the selected rows into x0 rows [0, rows) and zero-fills the rows up
+// to the next multiple of pad (at most n rows in all).
+__global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSel sel, uint32_t n, uint32_t pad,
+                             uint16_t* __restrict__ x0)
 {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t rows = *count < rows_max ? *count : rows_max;
-    if (row >= n)
+    const uint32_t rows = sel.rows();
+    const uint32_t padded = (rows + pad - 1) / pad * pad;
+    if (row >= n || row >= padded)
         return;
     uint16_t* out = x0 + (size_t)row * NRC_WIDTH;
     if (row >= rows)
     {
-        for (int k = 0; k < NRC_WIDTH; k++)
-            out[k] = 0;
+        for (int k = 0; k < NRC_WIDTH; k += 8)
+            *reinterpret_cast<uint4*>(out + k) = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
-    const float4 a = queries[4 * (size_t)row], b = queries[4 * (size_t)row + 1], c = queries[4 * (size_t)row + 2],
-                 d = queries[4 * (size_t)row + 3];
+    const size_t rec = sel.record(row);
+    const float4 a = queries[4 * rec], b = queries[4 * rec + 1], c = queries[4 * rec + 2], d = queries[4 * rec + 3];
     const float q[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
     float pos[3];
     normalized_position(P, q, pos);
@@ -248,7 +250,7 @@ __global__ __launch_bounds__(256) void k_nrc_train_fwd_bwd(NrcTrainDev T)
     load_weights_lds(T.wk, wk, NRC_WEIGHTS);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t rows = *T.count < T.batch ? *T.count : T.batch;
+    const uint32_t rows = T.sel.rows();
     const uint32_t tiles_total = ((rows + 31) & ~31u) / 16;
     const uint32_t S = T.batch_stride;
     for (uint32_t t = blockIdx.x * 4 + wave; t < tiles_total; t += gridDim.x * 4)
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(256) void k_nrc_train_fwd_bwd(NrcTrainDev T)
                 if (col < 3 && row < rows)
                 {
                     const float p = pred[i];
-                    const float y = T.targets[4 * (size_t)row + col];
+                    const float y = T.targets[4 * T.sel.record(row) + col];
                     const float den = p * p + 0.01f;
                     const float d = p - y;
                     lsum += d * d / den;
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(256) void k_nrc_weight_grad(NrcTrainDev T, uint32_t
 {
     const int layer = blockIdx.y;
     const int lane = threadIdx.x & 63, kb = threadIdx.x >> 6;
-    const uint32_t rows = *T.count < T.batch ? *T.count : T.batch;
+    const uint32_t rows = T.sel.rows();
     const uint32_t rows_pad = (rows + 31) & ~31u;
     const uint32_t begin = blockIdx.x * chunk;
     if (begin >= rows_pad)
@@ -430,10 +432,10 @@ __global__ __launch_bounds__(256) void k_nrc_weight_grad(NrcTrainDev T, uint32_t
 __global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restrict__ queries)
 {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t rows = *T.count < T.batch ? *T.count : T.batch;
+    const uint32_t rows = T.sel.rows();
     if (row >= rows)
         return;
-    const float4 a = queries[4 * (size_t)row];
+    const float4 a = queries[4 * T.sel.record(row)];
     const float q[3] = {a.x, a.y, a.z};
     float pos[3];
     normalized_position(P, q, pos);
@@ -461,7 +463,7 @@ __global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restric
 __global__ void k_nrc_adam_mlp(NrcTrainDev T, float lr1, float lr_scale_b1, float lr_scale_b2)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NRC_WEIGHTS)
+    if (i >= NRC_WEIGHTS || T.sel.rows() == 0)
         return;
     const float g = T.wgrad[i];
     T.wgrad[i] = 0.0f;
@@ -483,7 +485,7 @@ __global__ void k_nrc_adam_mlp(NrcTrainDev T, float lr1, float lr_scale_b1, floa
 __global__ void k_nrc_adam_grid(NrcTrainDev T, size_t count, float lr1, float lr_scale_b1, float lr_scale_b2)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count)
+    if (i >= count || T.sel.rows() == 0)
         return;
     const float g = T.ggrad[i];
     T.ggrad[i] = 0.0f;
@@ -665,7 +667,8 @@ void nrc_infer(Ctx& ctx, const float* d_queries, const uint32_t* d_count, uint32
     }
     const NrcDev P = nrc_dev(ctx);
     hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n16, 128)), dim3(128), 0, s, P,
-                       reinterpret_cast<const float4*>(d_queries), d_count, n, n16, C.x0.as<uint16_t>());
+                       reinterpret_cast<const float4*>(d_queries), RowSel{d_count, 1u, 0u, n}, n16, 16u,
+                       C.x0.as<uint16_t>());
     const unsigned grid = (unsigned)std::min<size_t>(blocks_for((n + 15) / 16, 4), (size_t)ctx.cu_count * 4);
     hipLaunchKernelGGL(k_nrc_forward, dim3(grid), dim3(256), 0, s, C.wt.as<uint16_t>(), C.x0.as<uint16_t>(),
                        d_count, n, reinterpret_cast<float4*>(d_out));
@@ -682,17 +685,21 @@ void nrc_encode(Ctx& ctx, const float* d_queries, uint32_t n, uint16_t* d_out, h
     C.count.reserve(4);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, C.count.as<uint32_t>(), n);
     hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n, 128)), dim3(128), 0, s, nrc_dev(ctx),
-                       reinterpret_cast<const float4*>(d_queries), C.count.as<uint32_t>(), n, n, d_out);
+                       reinterpret_cast<const float4*>(d_queries), RowSel{C.count.as<uint32_t>(), 1u, 0u, n}, n, 1u,
+                       d_out);
     HIP_CHECK(hipGetLastError());
 }
 
 void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, const uint32_t* d_count, uint32_t n,
-                   hipStream_t s)
+                   uint32_t rec_stride, uint32_t rec_offset, bool clear_loss, hipStream_t s)
 {
     Nrc& C = ctx.nrc;
     if (!C.configured)
         fail(MIPT_ERR_INVALID_ARGUMENT, "radiance cache not configured: call mipt_set_radiance_cache first");
-    const uint32_t batch = std::min<uint32_t>(n, (uint32_t)C.settings.batch_size);
+    // rows of this batch at most (records below n)
+    const uint32_t avail = n > rec_offset ? (n - rec_offset + rec_stride - 1) / rec_stride : 0u;
+    const uint32_t batch = std::min<uint32_t>(avail, (uint32_t)C.settings.batch_size);
+    C.last_sel_count = nullptr;
     if (batch == 0)
         return;
     const uint32_t stride = (batch + 31) & ~31u;
@@ -704,21 +711,26 @@ void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, con
     const uint32_t* cnt = d_count;
     if (!cnt)
     {
-        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, C.count.as<uint32_t>(), batch);
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, C.count.as<uint32_t>(), n);
         cnt = C.count.as<uint32_t>();
     }
+    const RowSel sel{cnt, rec_stride, rec_offset, batch};
+    C.last_sel_count = cnt;
+    C.last_sel_stride = rec_stride;
+    C.last_sel_offset = rec_offset;
+    C.last_sel_limit = batch;
     const NrcDev P = nrc_dev(ctx);
-    // encode all stride rows (rows past the count are zero)
+    // encode the batch; rows past its count up to a multiple of 32 are zero
     hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(stride, 128)), dim3(128), 0, s, P,
-                       reinterpret_cast<const float4*>(d_queries), cnt, batch, stride, C.x0.as<uint16_t>());
-    hipLaunchKernelGGL(k_nrc_zero, dim3(1), dim3(64), 0, s, C.loss.as<float>(), (size_t)1);
+                       reinterpret_cast<const float4*>(d_queries), sel, stride, 32u, C.x0.as<uint16_t>());
+    if (clear_loss)
+        hipLaunchKernelGGL(k_nrc_zero, dim3(1), dim3(64), 0, s, C.loss.as<float>(), (size_t)1);
     NrcTrainDev T{};
     T.wt = C.wt.as<uint16_t>();
     T.wk = C.wk.as<uint16_t>();
     T.x0 = C.x0.as<uint16_t>();
     T.targets = d_targets;
-    T.count = cnt;
-    T.batch = batch;
+    T.sel = sel;
     T.batch_stride = stride;
     T.levels = C.settings.grid_levels;
     T.xcm = C.xcm.as<uint16_t>();
@@ -736,9 +748,29 @@ void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, con
     HIP_CHECK(hipGetLastError());
 }
 
+void nrc_train_pass(Ctx& ctx, hipStream_t s)
+{
+    Nrc& C = ctx.nrc;
+    if (!C.configured || C.train_capacity == 0)
+        fail(MIPT_ERR_INVALID_ARGUMENT,
+             "RADIANCE_CACHE_TRAIN: no training records; run a path-trace pass with radiance_cache on first");
+    const uint32_t cap = C.train_capacity;
+    const uint32_t batch = (uint32_t)C.settings.batch_size;
+    const uint32_t steps = (cap + batch - 1) / batch;
+    const uint32_t* count = C.counters.as<uint32_t>() + 1;
+    for (uint32_t i = 0; i < steps; i++)
+    {
+        nrc_gradients(ctx, C.train_queries.as<float>(), C.train_targets.as<float>(), count, cap, steps, i, i == 0,
+                      s);
+        nrc_apply_adam(ctx, s);
+    }
+}
+
 void nrc_apply_adam(Ctx& ctx, hipStream_t s)
 {
     Nrc& C = ctx.nrc;
+    if (!C.last_sel_count)
+        return; // no batch
     C.step++;
     const mipt_radiance_cache_settings& st = C.settings;
     const float b1t = 1.0f - (float)std::pow((double)st.adam_beta1, (double)C.step);
@@ -758,6 +790,7 @@ void nrc_apply_adam(Ctx& ctx, hipStream_t s)
     T.gv = C.grid_v.as<float>();
     T.gmaster = C.grid_master.as<float>();
     T.grid_rw = C.grid_f16.as<_Float16>();
+    T.sel = RowSel{C.last_sel_count, C.last_sel_stride, C.last_sel_offset, C.last_sel_limit};
     hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(blocks_for(NRC_WEIGHTS, 256)), dim3(256), 0, s, T, st.learning_rate, b1t,
                        b2t);
     hipLaunchKernelGGL(k_nrc_adam_grid, dim3(blocks_for(C.grid_count, 256)), dim3(256), 0, s, T, C.grid_count,

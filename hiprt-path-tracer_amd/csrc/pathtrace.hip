@@ -15,6 +15,14 @@
 //                     (Intersect.h:222-245)
 //       k_accumulate: framebuffer sum (PathTracing.h:176-217) + FP64 mirror
 //
+// With the learned radiance cache on, k_cache_query runs at the start of the
+// query bounce: a path that is not a training path stops there and records a
+// query; after the bounces the cache is evaluated on the queries and
+// k_cache_resolve adds throughput x prediction to their pixels. One path in
+// train_every (a hash of pixel and sample) is a training path: it records the
+// same query and its radiance so far, runs to the bounce limit, and
+// k_cache_targets turns its remaining radiance / throughput into the target.
+//
 // Each path carries its xorshift32 state between kernels, so the per-path
 // RNG sequence is the megakernel's. Rays of a bounce live in fixed per-path
 // slots; an inactive slot has tmax = -1 and leaves the traversal at the root.
@@ -23,6 +31,7 @@
 #include <cmath>
 #include <vector>
 
+#include "nrc.h"
 #include "pt_device.h"
 
 namespace mipt {
@@ -33,6 +42,32 @@ constexpr uint32_t FLAG_NEE_LIGHT = 2u;  // shadow ray pending with a contributi
 constexpr uint32_t FLAG_NEE_BSDF = 4u;   // BSDF-to-light ray pending
 constexpr uint32_t FLAG_CONTINUE = 8u;   // continuation ray traced this bounce
 constexpr uint32_t FLAG_DIRECT = 16u;    // direct lighting estimated this bounce
+
+// Slots of the live rays of one bounce, per ray kind; counts[k] is the length
+// of queue k (0 shadow, 1 BSDF-to-light, 2 continuation).
+struct RayQueues {
+    uint32_t* shadow;
+    uint32_t* bsdf;
+    uint32_t* cont;
+    uint32_t* counts;
+};
+
+// One slot per lane with pred from a shared counter: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0)
+        return 0u;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0u;
+    if (lane == leader)
+        base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader, 64);
+    return base + rank;
+}
+
 
 // ---------------------------------------------------------------------------
 // CAMERA_RAYS
@@ -163,19 +198,9 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
 
 // Lights.h:123-226 (NEE with MIS), Lights.h:261-294 (ReSTIR DI at bounce 0)
 // and PathTracing.h:81-101 (next bounce + Russian roulette).
-__global__ void k_shade(PtData d, int32_t bounce)
+__device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, uint32_t pix, uint32_t flags,
+                                             bool& shadow_active, bool& bsdf_active, bool& cont)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= (uint32_t)(d.width * d.height))
-        return;
-    uint32_t flags = d.flags[pix];
-    if (!(flags & FLAG_ALIVE))
-    {
-        write_inactive(d.shadow_rays, pix);
-        write_inactive(d.bsdf_rays, pix);
-        write_inactive(d.cont_rays, pix);
-        return;
-    }
     flags &= FLAG_ALIVE;
     const float4 vp = d.vtx_pos[pix], vs = d.vtx_sn[pix], vg = d.vtx_gn[pix];
     const f3 p = ld3(vp), sn = ld3(vs), gn = ld3(vg), view = ld3(d.vtx_view[pix]);
@@ -184,7 +209,6 @@ __global__ void k_shade(PtData d, int32_t bounce)
     float path_roughness = vp.w;
     pt::Xorshift32 rng{d.rng[pix]};
 
-    bool shadow_active = false, bsdf_active = false;
     if (bounce > 0 || d.enable_direct)
     {
         flags |= FLAG_DIRECT;
@@ -252,12 +276,9 @@ __global__ void k_shade(PtData d, int32_t bounce)
         }
     }
     if (!shadow_active)
-        write_inactive(d.shadow_rays, pix);
-    if (!bsdf_active)
-        write_inactive(d.bsdf_rays, pix);
+        d.shadow_occ[pix] = 0; // a contribution without a shadow ray is unoccluded
 
     // ---- continuation (PathTracing.h:26-101)
-    bool cont = false;
     {
         pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, path_roughness};
         f3 dir;
@@ -304,13 +325,37 @@ __global__ void k_shade(PtData d, int32_t bounce)
             }
         }
     }
-    if (!cont)
-        write_inactive(d.cont_rays, pix);
     // The direct-light estimate of this vertex uses the pre-update throughput,
     // snapshotted before this kernel.
     d.vtx_pos[pix] = st3(p, path_roughness);
     d.rng[pix] = rng.state;
     d.flags[pix] = flags;
+}
+
+// Shades the live paths and enqueues the slots of the rays they emitted.
+__global__ void k_shade(PtData d, int32_t bounce, RayQueues q)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pix < (uint32_t)(d.width * d.height);
+    const uint32_t flags = valid ? d.flags[pix] : 0u;
+    bool shadow_active = false, bsdf_active = false, cont = false;
+    if (flags & FLAG_ALIVE)
+        shade_vertex(d, bounce, pix, flags, shadow_active, bsdf_active, cont);
+    const uint32_t a = wave_append(&q.counts[0], shadow_active);
+    const uint32_t b = wave_append(&q.counts[1], bsdf_active);
+    const uint32_t c = wave_append(&q.counts[2], cont);
+    if (shadow_active)
+        q.shadow[a] = pix;
+    if (bsdf_active)
+        q.bsdf[b] = pix;
+    if (cont)
+        q.cont[c] = pix;
+}
+
+__global__ void k_reset_queues(uint32_t* counts)
+{
+    if (threadIdx.x < 3)
+        counts[threadIdx.x] = 0u;
 }
 
 __global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev)
@@ -442,7 +487,156 @@ __global__ void k_accumulate(PtData d)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Learned radiance cache records
+// ---------------------------------------------------------------------------
+struct CacheView {
+    float4* queries;          // 4 float4 per query: cache input + throughput in floats 13..15
+    uint32_t* query_pix;
+    const float4* query_out;  // cache prediction per query
+    float4* train_queries;    // same layout as queries
+    uint32_t* train_pix;
+    float4* train_rad;        // path radiance before the query vertex
+    float4* train_targets;
+    uint32_t* counters;       // [0] queries, [1] training records, [2] targets dropped as non-finite
+    uint32_t train_every;
+    uint32_t train_capacity;
+    uint32_t sample_number;
+};
+
+__device__ __forceinline__ bool is_training_path(uint32_t pix, uint32_t sample, uint32_t every)
+{
+    uint32_t h = pix * 0x9E3779B1u ^ (sample + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h % every == 0u;
+}
+
+__global__ void k_cache_counters_reset(uint32_t* c)
+{
+    if (threadIdx.x < 4)
+        c[threadIdx.x] = 0u;
+}
+
+// At the start of the query bounce: the vertex's cache input (position, view
+// direction, shading normal, roughness, base colour) and the path throughput.
+__global__ void k_cache_query(PtData d, CacheView cv)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pix < (uint32_t)(d.width * d.height);
+    const uint32_t flags = valid ? d.flags[pix] : 0u;
+    const bool alive = (flags & FLAG_ALIVE) != 0u;
+    const bool train = alive && is_training_path(pix, cv.sample_number, cv.train_every);
+    const uint32_t slot_q = wave_append(&cv.counters[0], alive && !train);
+    const uint32_t slot_t = wave_append(&cv.counters[1], train);
+    if (!alive || (train && slot_t >= cv.train_capacity))
+        return;
+    const float4 vp = d.vtx_pos[pix], vs = d.vtx_sn[pix], vg = d.vtx_gn[pix], vv = d.vtx_view[pix];
+    const pt::Material& m = d.mats[__float_as_int(vg.w)];
+    const float4 thr = d.throughput[pix];
+    const float4 r0 = make_float4(vp.x, vp.y, vp.z, vv.x);
+    const float4 r1 = make_float4(vv.y, vv.z, vs.x, vs.y);
+    const float4 r2 = make_float4(vs.z, m.roughness, m.base_color.r, m.base_color.g);
+    const float4 r3 = make_float4(m.base_color.b, thr.x, thr.y, thr.z);
+    float4* rec = (train ? cv.train_queries + 4 * (size_t)slot_t : cv.queries + 4 * (size_t)slot_q);
+    rec[0] = r0;
+    rec[1] = r1;
+    rec[2] = r2;
+    rec[3] = r3;
+    if (train)
+    {
+        cv.train_pix[slot_t] = pix;
+        cv.train_rad[slot_t] = d.radiance[pix];
+    }
+    else
+    {
+        cv.query_pix[slot_q] = pix;
+        d.flags[pix] = 0u; // the cache stands in for the rest of the path
+    }
+}
+
+__global__ void k_cache_clamp_counts(CacheView cv, uint32_t query_capacity)
+{
+    if (threadIdx.x == 0)
+    {
+        cv.counters[0] = min(cv.counters[0], query_capacity);
+        cv.counters[1] = min(cv.counters[1], cv.train_capacity);
+    }
+}
+
+// radiance += throughput * max(prediction, 0) for every query.
+__global__ void k_cache_resolve(PtData d, CacheView cv)
+{
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= cv.counters[0])
+        return;
+    const uint32_t pix = cv.query_pix[row];
+    const float4 thr = cv.queries[4 * (size_t)row + 3];
+    const float4 o = cv.query_out[row];
+    float4 r = d.radiance[pix];
+    r.x += thr.y * fmaxf(o.x, 0.0f);
+    r.y += thr.z * fmaxf(o.y, 0.0f);
+    r.z += thr.w * fmaxf(o.z, 0.0f);
+    d.radiance[pix] = r;
+}
+
+// Target of a training record: (radiance now - radiance before the query
+// vertex) / throughput at it. A non-finite or negative target (from a NaN
+// sample, which the framebuffer rejects) trains towards 0 and is counted.
+__global__ void k_cache_targets(PtData d, CacheView cv)
+{
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= cv.counters[1])
+        return;
+    const uint32_t pix = cv.train_pix[row];
+    const float4 thr = cv.train_queries[4 * (size_t)row + 3];
+    const float4 before = cv.train_rad[row], now = d.radiance[pix];
+    float t[3] = {(now.x - before.x) / thr.y, (now.y - before.y) / thr.z, (now.z - before.z) / thr.w};
+    bool bad = false;
+    for (int k = 0; k < 3; k++)
+        bad |= !(t[k] >= 0.0f) || isinf(t[k]);
+    if (bad)
+    {
+        atomicAdd(&cv.counters[2], 1u);
+        t[0] = t[1] = t[2] = 0.0f;
+    }
+    cv.train_targets[row] = make_float4(t[0], t[1], t[2], 0.0f);
+}
+
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+CacheView make_cache_view(Ctx& ctx, uint32_t sample_number)
+{
+    Nrc& C = ctx.nrc;
+    const uint32_t n = ctx.render.pixels;
+    const uint32_t every = (uint32_t)C.settings.train_every;
+    const uint32_t cap = n / every + n / (every * 8) + 1024; // expected n / every, binomial slack
+    C.queries.reserve(64 * (size_t)n);
+    C.query_pix.reserve(4 * (size_t)n);
+    C.query_out.reserve(16 * (size_t)n);
+    C.train_queries.reserve(64 * (size_t)cap);
+    C.train_pix.reserve(4 * (size_t)cap);
+    C.train_rad.reserve(16 * (size_t)cap);
+    C.train_targets.reserve(16 * (size_t)cap);
+    C.counters.reserve(16);
+    C.train_capacity = cap;
+    CacheView cv{};
+    cv.queries = C.queries.as<float4>();
+    cv.query_pix = C.query_pix.as<uint32_t>();
+    cv.query_out = C.query_out.as<float4>();
+    cv.train_queries = C.train_queries.as<float4>();
+    cv.train_pix = C.train_pix.as<uint32_t>();
+    cv.train_rad = C.train_rad.as<float4>();
+    cv.train_targets = C.train_targets.as<float4>();
+    cv.counters = C.counters.as<uint32_t>();
+    cv.train_every = every;
+    cv.train_capacity = cap;
+    cv.sample_number = sample_number;
+    return cv;
+}
 
 } // namespace
 
@@ -552,16 +746,39 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
     const uint32_t n = (uint32_t)R.settings.width * (uint32_t)R.settings.height;
     PtData d = make_pt_data(ctx, sample_number, random_number);
     const unsigned g = blocks_for(n, 256);
+    const bool cache = R.settings.radiance_cache != 0;
+    CacheView cv{};
+    if (cache)
+    {
+        if (!ctx.nrc.configured)
+            fail(MIPT_ERR_INVALID_ARGUMENT, "radiance_cache is on: call mipt_set_radiance_cache first");
+        cv = make_cache_view(ctx, sample_number);
+        hipLaunchKernelGGL(k_cache_counters_reset, dim3(1), dim3(64), 0, s, cv.counters);
+    }
+    const RayQueues q{R.q_shadow.as<uint32_t>(), R.q_bsdf.as<uint32_t>(), R.q_cont.as<uint32_t>(),
+                      R.q_counts.as<uint32_t>()};
     hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d);
     for (int32_t bounce = 0; bounce <= R.settings.nb_bounces; bounce++)
     {
+        if (cache && bounce == ctx.nrc.settings.query_bounce)
+            hipLaunchKernelGGL(k_cache_query, dim3(g), dim3(256), 0, s, d, cv);
         hipLaunchKernelGGL(k_snapshot_throughput, dim3(g), dim3(256), 0, s, d, R.thr_prev.as<float4>());
-        hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce);
-        trace_any(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), n, s);
-        trace_closest(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), nullptr, n, s);
+        hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, s, q.counts);
+        hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce, q);
+        trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, q.counts, n, s);
+        trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, q.counts + 1, n, s);
         if (bounce < R.settings.nb_bounces)
-            trace_closest(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), nullptr, n, s);
+            trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, q.counts + 2, n,
+                                s);
         hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)R.thr_prev.as<float4>());
+    }
+    if (cache)
+    {
+        hipLaunchKernelGGL(k_cache_clamp_counts, dim3(1), dim3(64), 0, s, cv, n);
+        nrc_infer(ctx, reinterpret_cast<const float*>(cv.queries), cv.counters, n,
+                  reinterpret_cast<float*>(ctx.nrc.query_out.ptr), s);
+        hipLaunchKernelGGL(k_cache_resolve, dim3(g), dim3(256), 0, s, d, cv);
+        hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cv.train_capacity, 256)), dim3(256), 0, s, d, cv);
     }
     hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, d);
     HIP_CHECK(hipGetLastError());

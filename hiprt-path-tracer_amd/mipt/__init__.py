@@ -210,6 +210,7 @@ _SIGNATURES = {
     "mipt_radiance_cache_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_radiance_cache_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_radiance_cache_weight_count": (C.c_int32, []),
+    "mipt_radiance_cache_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -555,6 +556,14 @@ class Context:
         keep = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (mlp_master, grid_master)]
         self._check(self.lib.mipt_radiance_cache_set_params(self.h, p(keep[0]), p(keep[1])),
                     "mipt_radiance_cache_set_params")
+
+    def cache_stats(self):
+        """(queries, training records, dropped targets, train loss) of the last path and train passes."""
+        counts = np.zeros(3, np.uint32)
+        loss = C.c_float()
+        self._check(self.lib.mipt_radiance_cache_stats(self.h, counts.ctypes.data_as(C.c_void_p), C.byref(loss)),
+                    "mipt_radiance_cache_stats")
+        return int(counts[0]), int(counts[1]), int(counts[2]), loss.value
 
     def rejected_samples(self) -> int:
         v = C.c_uint32()

@@ -406,6 +406,12 @@ int mipt_radiance_cache_get_params(mipt_ctx* ctx, float* mlp_master, uint16_t* m
                                    uint16_t* grid_f16);
 int mipt_radiance_cache_set_params(mipt_ctx* ctx, const float* mlp_master, const float* grid_master);
 int32_t mipt_radiance_cache_weight_count(void);
+/* After a path pass with the cache: counts[0] queries answered by the cache,
+ * counts[1] training records, counts[2] training targets dropped as non-finite
+ * or negative; *train_loss the mean relative L2 loss of the last
+ * RADIANCE_CACHE_TRAIN pass over its records (either may be NULL).
+ * Synchronous. */
+int mipt_radiance_cache_stats(mipt_ctx* ctx, uint32_t* counts, float* train_loss);
 
 /* The path pass with render_settings.radiance_cache = 1 stops paths at
  * query_bounce and adds throughput x the cache's prediction; one path in
