@@ -90,10 +90,10 @@ void prepare_scene_for_render(Ctx& ctx)
         fail(MIPT_ERR_NO_BVH, "no BVH: call mipt_build_bvh first");
     if (ctx.triangle_count > 0 && (ctx.materials_host.empty() || !ctx.material_indices.ptr))
         fail(MIPT_ERR_INVALID_ARGUMENT, "the path-tracing passes need materials and material indices");
-    std::vector<float> mats(32 * (ctx.materials_host.size() > 0 ? ctx.materials_host.size() : 1), 0.0f);
+    std::vector<float> mats(64 * (ctx.materials_host.size() > 0 ? ctx.materials_host.size() : 1), 0.0f);
     for (size_t i = 0; i < ctx.materials_host.size(); i++)
     {
-        std::string why = unpack_material(ctx.materials_host[i], mats.data() + 32 * i);
+        std::string why = unpack_material(ctx.materials_host[i], mats.data() + 64 * i);
         if (!why.empty())
             fail(MIPT_ERR_UNSUPPORTED, "material " + std::to_string(i) + " uses the " + why +
                                            ", which the path-tracing passes do not implement yet");
@@ -112,6 +112,10 @@ void prepare_scene_for_render(Ctx& ctx)
     }
     upload(R.areas, areas.data(), areas.size());
     upload(R.lut_conductor, conductor_lut_data(), 128 * 128);
+    upload(R.lut_glossy, glossy_dielectric_lut_data(), 128 * 64 * 128);
+    upload(R.lut_glass, glass_lut_data(), 256 * 16 * 128);
+    upload(R.lut_glass_inv, glass_inv_lut_data(), 256 * 16 * 128);
+    upload(R.lut_sheen, sheen_ltc_data(), 32 * 32 * 3);
     build_power_alias_table(ctx);
     R.scene_ready = true;
 }

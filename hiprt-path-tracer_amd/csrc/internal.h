@@ -69,6 +69,7 @@ struct RenderState {
     DeviceBuffer materials;     // pt::Material[material_count]
     DeviceBuffer areas;         // float[T]
     DeviceBuffer lut_conductor; // float[128*128]
+    DeviceBuffer lut_glossy, lut_glass, lut_glass_inv, lut_sheen;
     bool scene_ready = false;
     // G-buffer (CameraRays output)
     DeviceBuffer gb_prim, gb_pos, gb_gn, gb_sn, gb_mat;
@@ -147,10 +148,14 @@ struct Ctx {
 };
 
 // materials.cpp: CPUMaterial -> the 8-bit packed form -> unpacked floats
-// (pt::Material layout, 32 floats); returns an empty string or the reason a
+// (pt::Material layout, 64 floats); returns an empty string or the reason a
 // material uses a lobe this build does not implement.
-std::string unpack_material(const mipt_material& in, float out[32]);
+std::string unpack_material(const mipt_material& in, float out[64]);
 const float* conductor_lut_data(); // 128 x 128, luts.cpp
+const float* glossy_dielectric_lut_data(); // [128][64][128]
+const float* glass_lut_data();             // [128][16][256]
+const float* glass_inv_lut_data();         // [128][16][256]
+const float* sheen_ltc_data();             // [32][32][3]
 
 // pathtrace.hip
 void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);

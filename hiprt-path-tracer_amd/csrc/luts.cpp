@@ -1,17 +1,35 @@
-// luts.cpp -- the GGX conductor directional-albedo LUT (E_ss(cos_theta,
-// roughness), 128 x 128 binary32, height-uncorrelated Smith masking, the
-// reference's default flavour) embedded in the library. Source data:
-// hiprt-path-tracer_amd/data/ggx_conductor_uncorrelated_128x128.f32, converted
-// from the reference's baked .hdr by data/convert_luts.py.
-__asm__(".section .rodata\n"
-        ".balign 64\n"
-        ".global mipt_lut_ggx_conductor_uncorrelated\n"
-        "mipt_lut_ggx_conductor_uncorrelated:\n"
-        ".incbin \"" MIPT_DATA_DIR "/ggx_conductor_uncorrelated_128x128.f32\"\n"
-        ".previous\n");
+// luts.cpp -- the BSDF tables embedded in the library, converted from the
+// reference's baked data by data/convert_luts.py:
+//   - GGX conductor directional albedo E_ss(cos_theta, roughness), 128 x 128;
+//   - glossy dielectric (specular over diffuse) directional albedo, 128 layers
+//     (F0^(1/4)) of 64 (roughness) x 128 (cos_theta^(1/2.5));
+//   - GGX glass directional albedo and its inverse-IOR twin, 128 layers of
+//     16 x 256;
+//   - fitted sheen LTC parameters (Ai, Bi, Ri), 32 x 32 float3.
+// All binary32, height-uncorrelated Smith masking (the reference's default).
+#define MIPT_INCBIN(sym, file)                                                                                   \
+    __asm__(".section .rodata\n"                                                                                 \
+            ".balign 64\n"                                                                                       \
+            ".global " #sym "\n" #sym ":\n"                                                                      \
+            ".incbin \"" MIPT_DATA_DIR "/" file "\"\n"                                                           \
+            ".previous\n")
+
+MIPT_INCBIN(mipt_lut_ggx_conductor_uncorrelated, "ggx_conductor_uncorrelated_128x128.f32");
+MIPT_INCBIN(mipt_lut_glossy_dielectric_uncorrelated, "glossy_dielectric_uncorrelated_128x64x128.f32");
+MIPT_INCBIN(mipt_lut_ggx_glass_uncorrelated, "ggx_glass_uncorrelated_256x16x128.f32");
+MIPT_INCBIN(mipt_lut_ggx_glass_inv_uncorrelated, "ggx_glass_inv_uncorrelated_256x16x128.f32");
+MIPT_INCBIN(mipt_lut_sheen_ltc, "sheen_ltc_32x32.f32");
 
 extern "C" const float mipt_lut_ggx_conductor_uncorrelated[128 * 128];
+extern "C" const float mipt_lut_glossy_dielectric_uncorrelated[128 * 64 * 128];
+extern "C" const float mipt_lut_ggx_glass_uncorrelated[256 * 16 * 128];
+extern "C" const float mipt_lut_ggx_glass_inv_uncorrelated[256 * 16 * 128];
+extern "C" const float mipt_lut_sheen_ltc[32 * 32 * 3];
 
 namespace mipt {
 const float* conductor_lut_data() { return mipt_lut_ggx_conductor_uncorrelated; }
+const float* glossy_dielectric_lut_data() { return mipt_lut_glossy_dielectric_uncorrelated; }
+const float* glass_lut_data() { return mipt_lut_ggx_glass_uncorrelated; }
+const float* glass_inv_lut_data() { return mipt_lut_ggx_glass_inv_uncorrelated; }
+const float* sheen_ltc_data() { return mipt_lut_sheen_ltc; }
 } // namespace mipt

@@ -7,6 +7,8 @@
 // the same binary32 arithmetic: truncation u8(x * 255) on the way in;
 // u8 * (1/255) on the way out, except the green channel of a 3x8-bit colour,
 // which the reference unpacks as float(g << 8) * (1 / (255 << 8)).
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -31,10 +33,10 @@ void color3(const float in[3], float out[3])
 
 } // namespace
 
-// Output layout = pt::Material (csrc/shading.h), 32 floats.
-std::string unpack_material(const mipt_material& m, float o[32])
+// Output layout = pt::Material (csrc/shading.h), 64 floats.
+std::string unpack_material(const mipt_material& m, float o[64])
 {
-    std::memset(o, 0, 32 * sizeof(float));
+    std::memset(o, 0, 64 * sizeof(float));
     // emission is stored at full precision, pre-multiplied by its strength
     // and the global factor (MaterialCPU.h:64)
     for (int k = 0; k < 3; k++)
@@ -57,18 +59,55 @@ std::string unpack_material(const mipt_material& m, float o[32])
     o[24] = u8f(q8(m.thin_film));
     o[25] = m.ior;
     uint32_t flags = (m.do_metallic_energy_compensation ? 1u : 0u) | (m.do_specular_energy_compensation ? 2u : 0u) |
-                     (m.do_coat_energy_compensation ? 4u : 0u) | (m.do_glass_energy_compensation ? 8u : 0u);
+                     (m.do_coat_energy_compensation ? 4u : 0u) | (m.do_glass_energy_compensation ? 8u : 0u) |
+                     (m.thin_walled ? 16u : 0u);
     std::memcpy(&o[26], &flags, 4);
+    o[27] = u8f(q8(m.specular_tint));
+    color3(m.specular_color, o + 28);
+    o[31] = u8f(q8(m.specular_darkening));
+    color3(m.coat_medium_absorption, o + 32);
+    o[35] = m.coat_medium_thickness;
+    o[36] = u8f(q8(m.coat_roughness));
+    o[37] = u8f(q8(m.coat_roughening));
+    o[38] = u8f(q8(m.coat_darkening));
+    o[39] = u8f(q8(m.coat_anisotropy));
+    o[40] = u8f(q8(m.coat_anisotropy_rotation));
+    o[41] = m.coat_ior;
+    o[42] = u8f(q8(m.sheen_roughness));
+    color3(m.sheen_color, o + 43);
+    o[46] = m.absorption_at_distance;
+    color3(m.absorption_color, o + 47);
+    o[50] = u8f(q8(m.dispersion_scale));
+    o[51] = m.dispersion_abbe_number;
+    const int32_t priority = m.dielectric_priority;
+    std::memcpy(&o[52], &priority, 4);
 
-    // Lobes the passes implement: the two GGX metal lobes and the Lambertian
-    // diffuse lobe. Anything with weight on another lobe is refused.
-    const float metallic = o[7];
-    if (o[20] > 0.0f) return "coat lobe";
-    if (o[21] > 0.0f) return "sheen lobe";
+    // Coat roughening of the base and second metal roughness
+    // (get_intersection_material, src/Device/includes/Material.h:122-149).
+    const float coat = o[20], coat_roughening = o[37], coat_roughness = o[36];
+    if (coat > 0.0f && coat_roughening > 0.0f)
+    {
+        auto pow4 = [](float x) { float x2 = x * x; return x2 * x2; };
+        auto lerp = [](float a, float b, float t) { return (1.0f - t) * a + t * b; };
+        const float base = o[6];
+        float target = sqrtf(sqrtf(std::min(1.0f, pow4(base) + 2.0f * pow4(coat_roughness))));
+        float roughened = lerp(base, target, coat);
+        o[6] = lerp(base, roughened, coat_roughening);
+        if (o[17] > 0.0f)
+        {
+            const float second = o[18];
+            float target2 = sqrtf(sqrtf(std::min(1.0f, pow4(second) + 2.0f * pow4(coat_roughness))));
+            float roughened2 = lerp(second, target2, coat);
+            o[18] = lerp(second, roughened2, coat_roughening);
+        }
+    }
+
+    // Lobes the passes implement: coat, sheen, the two GGX metal lobes, the
+    // dielectric specular layer and the Lambertian diffuse lobe. Weight on
+    // another lobe is refused.
     if (o[22] > 0.0f) return "glass (specular transmission) lobe";
     if (o[23] > 0.0f) return "diffuse transmission lobe";
     if (o[24] > 0.0f) return "thin-film Fresnel";
-    if ((1.0f - metallic) * o[19] > 0.0f) return "dielectric specular lobe";
     return "";
 }
 

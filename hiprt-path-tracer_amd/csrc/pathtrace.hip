@@ -679,6 +679,8 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.sample_number = sample_number;
     d.random_number = random_number;
     d.g.ggx_conductor_ess = R.lut_conductor.as<float>();
+    d.g.glossy_dielectric_ess = R.lut_glossy.as<float>();
+    d.g.sheen_ltc = R.lut_sheen.as<float>();
     d.g.lut_size = 128;
     d.g.consistent_tau = consistent_tau(R.settings.microfacet_regularization_tau_0, sample_number);
     d.g.min_roughness = R.settings.microfacet_regularization_min_roughness;

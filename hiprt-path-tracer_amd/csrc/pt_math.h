@@ -127,6 +127,41 @@ PT_HD float pow_fixed(float x, float y)
     return exp2_fixed(y * log2_fixed(x));
 }
 
+// Natural logarithm and exponential through the base-2 sequences.
+PT_HD float log_fixed(float x) { return log2_fixed(x) * 0x1.62e43p-1f; }
+PT_HD float exp_fixed(float x) { return exp2_fixed(x * 0x1.715476p+0f); }
+
+// atan2 with IEEE zero and sign handling: the quotient of the smaller by the
+// larger magnitude is reduced to |t| <= tan(pi/8) with
+// atan(a) = pi/4 + atan((a - 1) / (a + 1)), then the odd Taylor series of
+// atan to t^15 (truncation error below 2e-8).
+PT_HD float atan2_fixed(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    float r;
+    if (ax == 0.0f && ay == 0.0f)
+        r = signbit(x) ? 0x1.921fb6p+1f : 0.0f;
+    else
+    {
+        const bool swap = ay > ax;
+        const float a = swap ? ax / ay : ay / ax;
+        float base = 0.0f, t = a;
+        if (a > 0x1.a8279ap-2f) // tan(pi/8)
+        {
+            t = (a - 1.0f) / (a + 1.0f);
+            base = 0x1.921fb6p-1f;
+        }
+        const float t2 = t * t;
+        const float p = t + t * (t2 * (-0x1.555556p-2f + t2 * (0x1.99999ap-3f + t2 * (-0x1.24924ap-3f + t2 * (0x1.c71c72p-4f + t2 * (-0x1.745d18p-4f + t2 * (0x1.3b13b2p-4f + t2 * -0x1.111112p-4f)))))));
+        r = base + p;
+        if (swap)
+            r = 0x1.921fb6p+0f - r;
+        if (signbit(x))
+            r = 0x1.921fb6p+1f - r;
+    }
+    return signbit(y) ? -r : r;
+}
+
 // ---------------------------------------------------------------------------
 // RNG: the reference's per-pixel streams (src/Device/includes/Hash.h:11-19,
 // src/HostDeviceCommon/Xorshift.h:18-66).

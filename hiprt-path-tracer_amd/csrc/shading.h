@@ -47,8 +47,9 @@ PT_HD float max_component(Color c) { return maxf_(c.r, maxf_(c.g, c.b)); }
 PT_HD bool is_black(Color c) { return !(c.r > 0.0f || c.g > 0.0f || c.b > 0.0f); }
 PT_HD Color lerpc(Color a, Color b, float t) { return (1.0f - t) * a + t * b; }
 
-// Unpacked material (after the reference's 8-bit packing round trip,
-// MaterialPacked.h). 128 bytes.
+// Unpacked effective material (after the reference's 8-bit packing round
+// trip, MaterialPacked.h, and the coat roughening of get_intersection_material,
+// Material.h:122-149). 256 bytes.
 struct Material {
     Color emission;
     Color base_color;
@@ -68,13 +69,34 @@ struct Material {
     float diffuse_transmission;
     float thin_film;
     float ior;
-    uint32_t flags; // bit 0: metallic energy compensation, bit 1: specular, bit 2: coat, bit 3: glass
-    float pad[5];
+    uint32_t flags; // MAT_* bits
+    float specular_tint;
+    Color specular_color;
+    float specular_darkening;
+    Color coat_medium_absorption;
+    float coat_medium_thickness;
+    float coat_roughness;
+    float coat_roughening;
+    float coat_darkening;
+    float coat_anisotropy;
+    float coat_anisotropy_rotation;
+    float coat_ior;
+    float sheen_roughness;
+    Color sheen_color;
+    float absorption_at_distance;
+    Color absorption_color;
+    float dispersion_scale;
+    float dispersion_abbe_number;
+    int32_t dielectric_priority;
+    float pad[11];
 };
-static_assert(sizeof(Material) == 128, "Material must be 128 bytes");
+static_assert(sizeof(Material) == 256, "Material must be 256 bytes");
 
 constexpr uint32_t MAT_METALLIC_EC = 1u;
 constexpr uint32_t MAT_SPECULAR_EC = 2u;
+constexpr uint32_t MAT_COAT_EC = 4u;
+constexpr uint32_t MAT_GLASS_EC = 8u;
+constexpr uint32_t MAT_THIN_WALLED = 16u;
 
 // Sampled-lobe information (src/Device/includes/BSDFs/BSDFIncidentLightInfo.h).
 enum LobeInfo : uint32_t {
@@ -101,6 +123,8 @@ enum RegMode : uint32_t { NO_REGULARIZATION = 0, REGULARIZATION_CLASSIC = 1, REG
 // Per-pass constants of the BSDF (BSDFsData.h + MicrofacetRegularizationSettings.h).
 struct BsdfGlobals {
     const float* ggx_conductor_ess; // 128 x 128, row 0 = roughness 1
+    const float* glossy_dielectric_ess; // [ior 128][roughness 64][cos 128], rows flipped like the 2D LUT
+    const float* sheen_ltc;         // 32 x 32 float3 (Ai, Bi, Ri)
     int32_t lut_size;
     float consistent_tau;           // MicrofacetRegularization::consistent_tau(tau_0, sample_number)
     float min_roughness;
@@ -171,15 +195,26 @@ PT_HD bool is_perfectly_smooth(float r) { return r <= PERFECTLY_SMOOTH_THRESHOLD
 
 enum DeltaSampled : int { NOT_SPECULAR = 0, SPECULAR_PEAK_SAMPLED = 1, SPECULAR_PEAK_NOT_SAMPLED = 2 };
 
-// MaterialUtils.h:143-175 (metal part; coat/glass lobes are not built yet).
+// MaterialUtils.h:143-175: whether a perfectly smooth lobe can evaluate to
+// anything but 0 for the incident direction `info` describes.
 PT_HD DeltaSampled delta_reflection_sampled(const Material& m, float roughness, float anisotropy, uint32_t info)
 {
     if (!is_perfectly_smooth(roughness))
         return NOT_SPECULAR;
     bool matching = fabsf(anisotropy - m.anisotropy) < 1.0e-3f;
+    bool glass = info == FROM_GLASS_REFLECT && is_perfectly_smooth(m.roughness) && matching;
+    if (glass)
+        return SPECULAR_PEAK_SAMPLED;
     bool first = info == FROM_FIRST_METAL && is_perfectly_smooth(m.roughness) && matching;
     bool second = info == FROM_SECOND_METAL && is_perfectly_smooth(m.second_roughness) && matching;
     if (first || second)
+        return SPECULAR_PEAK_SAMPLED;
+    bool coat_matching = fabsf(anisotropy - m.coat_anisotropy) < 1.0e-3f;
+    bool coat = info == FROM_COAT && coat_matching && is_perfectly_smooth(m.coat_roughness);
+    if (coat)
+        return SPECULAR_PEAK_SAMPLED;
+    bool specular = info == FROM_SPECULAR && is_perfectly_smooth(m.roughness) && matching;
+    if (specular)
         return SPECULAR_PEAK_SAMPLED;
     return SPECULAR_PEAK_NOT_SAMPLED;
 }
@@ -392,11 +427,303 @@ PT_HD float metallic_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx&
     return ggx_pdf_reflect(rr, anisotropy, lv, ll, lh, ds);
 }
 
+// ---------------------------------------------------------------------------
+// Dielectric Fresnel (src/Device/includes/Fresnel.h)
+// ---------------------------------------------------------------------------
+// Fresnel.h:11-26
+PT_HD float F0_from_eta(float eta_t, float eta_i)
+{
+    float nume = eta_t - eta_i;
+    float denom = eta_t + eta_i;
+    return (nume * nume) / (denom * denom);
+}
+PT_HD float F0_from_eta_t_and_relative_ior(float eta_t, float relative_eta)
+{
+    return F0_from_eta(eta_t, eta_t / relative_eta);
+}
+
+// Fresnel.h:41-62; relative_eta = eta_t / eta_i.
+PT_HD float fresnel_dielectric(float cos_theta_i, float relative_eta)
+{
+    if (fabsf(1.0f - relative_eta) < 1.0e-4f)
+        return 0.0f;
+    float sin_theta_i2 = 1.0f - cos_theta_i * cos_theta_i;
+    float sin_theta_t2 = sin_theta_i2 / (relative_eta * relative_eta);
+    if (sin_theta_t2 >= 1.0f)
+        return 1.0f;
+    float cos_theta_t = sqrtf(1.0f - sin_theta_t2);
+    float r_par = (relative_eta * cos_theta_i - cos_theta_t) / (relative_eta * cos_theta_i + cos_theta_t);
+    float r_perp = (cos_theta_i - relative_eta * cos_theta_t) / (cos_theta_i + relative_eta * cos_theta_t);
+    return (r_par * r_par + r_perp * r_perp) / 2.0f;
+}
+PT_HD float fresnel_dielectric(float cos_theta_i, float eta_i, float eta_t)
+{
+    return fresnel_dielectric(cos_theta_i, eta_t / eta_i);
+}
+
+// Fresnel.h:169-172 (d'Eon's fit of the hemispherical albedo of a smooth dielectric).
+PT_HD float fresnel_hemispherical_albedo_fit(float relative_eta)
+{
+    return log_fixed((10893.0f * relative_eta - 1438.2f) /
+                     (-774.4f * (relative_eta * relative_eta) + 10212.0f * relative_eta + 1.0f));
+}
+
+PT_HD Color cdiv(float a, Color b) { return col(a / b.r, a / b.g, a / b.b); }
+PT_HD bool is_white(Color c) { return c.r == 1.0f && c.g == 1.0f && c.b == 1.0f; }
+
+// Nearest-texel fetch of a stacked 3D LUT as the reference's CPU sampler does it
+// (Image32Bit3D::sample_rgba32f, src/Image/Image.cpp:761-803): wrap to [0, 1]
+// (1 stays 1), v flipped, texel (int)(u (W-1)), (int)(v (H-1)), layer (int)(w (D-1)).
+PT_HD float lut3d_fetch(const float* lut, int w, int h, int d, float u, float v, float t)
+{
+    if (u != 1.0f) u -= (float)(int)u;
+    if (v != 1.0f) v -= (float)(int)v;
+    if (t != 1.0f) t -= (float)(int)t;
+    u = u < 0.0f ? 1.0f + u : u;
+    v = v < 0.0f ? 1.0f + v : v;
+    t = t < 0.0f ? 1.0f + t : t;
+    v = 1.0f - v;
+    int x = (int)(u * (float)(w - 1));
+    int y = (int)(v * (float)(h - 1));
+    int z = (int)(t * (float)(d - 1));
+    return lut[(size_t)z * (size_t)(w * h) + (size_t)(x + y * w)];
+}
+
+// ---------------------------------------------------------------------------
+// Specular (dielectric) layer over the diffuse base: the "glossy base"
+// ---------------------------------------------------------------------------
+// Principled.h:249-285: IOR of the specular layer over the layer above it.
+PT_HD float specular_relative_ior(const Material& m, float incident_medium_ior)
+{
+    if (m.coat == 0.0f)
+        return m.ior;
+    float incident_layer_ior = lerpf_(incident_medium_ior, m.coat_ior, m.coat);
+    float relative_ior = m.ior / incident_layer_ior;
+    if (relative_ior < 1.0f)
+        relative_ior = 1.0f / relative_ior;
+    return relative_ior;
+}
+
+// Principled.h:215-241 with thin_film = 0 (thin-film materials are refused):
+// lerp(F, F_thin_film, 0) is F.
+PT_HD Color specular_fresnel(const Material& m, float relative_specular_ior, float cos_theta_i)
+{
+    return col(fresnel_dielectric(cos_theta_i, relative_specular_ior));
+}
+
+// PrincipledEnergyCompensation.h:15-60 (roughness threshold 0, no bounce limit).
+PT_HD float energy_compensation_glossy_base(const BsdfGlobals& g, const Material& m, float incident_medium_ior,
+                                            float NoV)
+{
+    bool disabled = !(m.flags & MAT_SPECULAR_EC);
+    bool roughness_low_enough = m.roughness < 0.0f;
+    bool no_specular_layer = m.specular < 1.0e-3f;
+    bool invalid_view_direction = NoV < 0.0f;
+    if (disabled || roughness_low_enough || no_specular_layer || invalid_view_direction)
+        return 1.0f;
+    float ior = m.ior;
+    float relative_ior = specular_relative_ior(m, incident_medium_ior);
+    if (fabsf(relative_ior - 1.0f) < 1.0e-3f)
+        relative_ior += 1.0e-3f;
+    float view_dir_remapped = pow_fixed(NoV, 1.0f / 2.5f);
+    float F0_remapped = sqrtf(sqrtf(F0_from_eta_t_and_relative_ior(ior, relative_ior)));
+    float ms = lut3d_fetch(g.glossy_dielectric_ess, 128, 64, 128, view_dir_remapped, m.roughness, F0_remapped);
+    float compensation = lerpf_(1.0f, ms, m.specular);
+    compensation = lerpf_(compensation, 1.0f, m.thin_film);
+    return compensation;
+}
+
+// PrincipledEnergyCompensation.h:76-125 (the whole BSDF under the coat).
+PT_HD float energy_compensation_clearcoat(const BsdfGlobals& g, const Material& m, float incident_medium_ior, float NoV)
+{
+    bool disabled = !(m.flags & MAT_SPECULAR_EC);
+    bool no_coat_layer = m.coat < 1.0e-3f;
+    bool invalid_view_direction = NoV < 0.0f;
+    if (disabled || no_coat_layer || invalid_view_direction)
+        return 1.0f;
+    if (fabsf(m.coat_ior / incident_medium_ior - 1.0f) < 1.0e-3f)
+        incident_medium_ior += 1.0e-3f;
+    float view_dir_remapped = pow_fixed(NoV, 1.0f / 2.5f);
+    float F0_remapped = sqrtf(sqrtf(F0_from_eta(m.coat_ior, incident_medium_ior)));
+    float ms = lut3d_fetch(g.glossy_dielectric_ess, 128, 64, 128, view_dir_remapped, m.coat_roughness, F0_remapped);
+    float compensation = lerpf_(1.0f, ms, m.coat * (1.0f - m.specular_transmission));
+    compensation = lerpf_(compensation, 1.0f, m.thin_film);
+    return compensation;
+}
+
+// Principled.h:292-313
+PT_HD Color specular_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, float relative_ior, f3 lv, f3 ll,
+                          f3 lh, float& pdf)
+{
+    float rr = regularize_reflection(g, c.reg, m.roughness, c.path_roughness);
+    DeltaSampled ds = delta_reflection_sampled(m, rr, m.anisotropy, c.info);
+    Color F = specular_fresnel(m, relative_ior, dot(ll, lh));
+    return ggx_eval_reflect_ss(rr, m.anisotropy, F, lv, ll, lh, pdf, ds);
+}
+
+PT_HD float specular_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, f3 lv, f3 ll, f3 lh)
+{
+    float rr = regularize_reflection(g, c.reg, m.roughness, c.path_roughness);
+    DeltaSampled ds = delta_reflection_sampled(m, rr, m.anisotropy, c.info);
+    return ggx_pdf_reflect(rr, m.anisotropy, lv, ll, lh, ds);
+}
+
+// Principled.h:1272-1303 (OpenPBR's TIR darkening with a diffuse base).
+PT_HD Color specular_darkening(const Material& m, float relative_eta, float view_dir_fresnel)
+{
+    if (m.specular_darkening == 0.0f)
+        return col(1.0f);
+    float Kr = 1.0f - (1.0f - fresnel_hemispherical_albedo_fit(relative_eta)) / (relative_eta * relative_eta);
+    float K = Kr;
+    Color base_albedo = m.base_color;
+    Color darkening = cdiv(1.0f - K, col(1.0f) - base_albedo * K);
+    return lerpc(col(1.0f), darkening, m.specular * m.specular_darkening * (1.0f - m.diffuse_transmission));
+}
+
+// ---------------------------------------------------------------------------
+// Coat layer
+// ---------------------------------------------------------------------------
+// Principled.h:49-73
+PT_HD Color coat_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, f3 lv, f3 ll, f3 lh,
+                      float incident_medium_ior, float& pdf)
+{
+    float HoL = clampf_(1.0e-8f, 1.0f, dot(lh, ll));
+    float rr = regularize_reflection(g, c.reg, m.coat_roughness, c.path_roughness);
+    DeltaSampled ds = delta_reflection_sampled(m, rr, m.coat_anisotropy, c.info);
+    Color F = col(fresnel_dielectric(HoL, incident_medium_ior, m.coat_ior));
+    return ggx_eval_reflect_ss(rr, m.coat_anisotropy, F, lv, ll, lh, pdf, ds);
+}
+
+PT_HD float coat_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, f3 lv, f3 ll, f3 lh)
+{
+    float rr = regularize_reflection(g, c.reg, m.coat_roughness, c.path_roughness);
+    DeltaSampled ds = delta_reflection_sampled(m, rr, m.coat_anisotropy, c.info);
+    return ggx_pdf_reflect(rr, m.coat_anisotropy, lv, ll, lh, ds);
+}
+
+// Principled.h:890-950 (OpenPBR coat darkening).
+PT_HD Color coat_darkening(const Material& m, float relative_eta, float view_dir_fresnel)
+{
+    if (m.coat_darkening == 0.0f)
+        return col(1.0f);
+    float Kr = 1.0f - (1.0f - fresnel_hemispherical_albedo_fit(relative_eta)) / (relative_eta * relative_eta);
+    float Ks = view_dir_fresnel;
+    float base_roughness = lerpf_(1.0f, m.roughness, maxf_(m.specular_transmission, maxf_(m.metallic, m.specular)));
+    float K = lerpf_(Ks, Kr, base_roughness);
+    float sheen = m.sheen;
+    Color base_albedo = (m.base_color + m.sheen_color * sheen) / (1.0f + sheen);
+    Color darkening = cdiv(1.0f - K, col(1.0f) - base_albedo * K);
+    return lerpc(col(1.0f), darkening, m.coat * m.coat_darkening * (1.0f - m.diffuse_transmission));
+}
+
+// ---------------------------------------------------------------------------
+// Sheen: LTC fit of the SGGX volumetric sheen (src/Device/includes/BSDFs/SheenLTC.h)
+// ---------------------------------------------------------------------------
+// read_LTC_parameters through the CPU 2D sampler (Image32Bit::sample_rgba32f,
+// src/Image/Image.cpp:564-600): uv = (cos_theta, clamp(roughness)), v flipped.
+PT_HD Color sheen_ltc_params(const BsdfGlobals& g, float roughness, float cos_theta)
+{
+    float u = cos_theta, v = clampf_(0.0f, 1.0f, roughness);
+    if (u != 1.0f) u -= (float)(int)u;
+    if (v != 1.0f) v -= (float)(int)v;
+    u = u < 0.0f ? 1.0f + u : u;
+    v = v < 0.0f ? 1.0f + v : v;
+    v = 1.0f - v;
+    int x = (int)(u * 31.0f);
+    int y = (int)(v * 31.0f);
+    const float* p = g.sheen_ltc + 3 * (x + y * 32);
+    return col(p[0], p[1], p[2]);
+}
+
+// SheenLTC.h:24-45
+PT_HD float eval_ltc(f3 l, Color AiBiRi)
+{
+    f3 o = mk(l.x * AiBiRi.r + l.z * AiBiRi.g, l.y * AiBiRi.r, l.z);
+    float len = sqrtf(dot(o, o));
+    o = vdiv(o, len);
+    float det = AiBiRi.r * AiBiRi.r;
+    float jacobian = det / (len * len * len);
+    return o.z * INV_PI_F * jacobian;
+}
+
+// SheenLTC.h:63-70
+PT_HD float get_phi(f3 d)
+{
+    float p = atan2_fixed(d.y, d.x);
+    if (p < 0.0f)
+        p += TWO_PI_F;
+    return p;
+}
+
+// SheenLTC.h:75-81: rotation of vec by angle around axis (Rodrigues).
+PT_HD f3 rotate_vector(f3 vec, f3 axis, float angle)
+{
+    float s, co;
+    sincos_fixed(angle, s, co);
+    return (vmul(vec, co) + vmul(vmul(axis, dot(vec, axis)), 1.0f - co)) + vmul(cross(axis, vec), s);
+}
+
+// SheenLTC.h:88-125; lv, ll in the (unrotated) local shading frame.
+PT_HD Color sheen_eval(const BsdfGlobals& g, const Material& m, f3 lv, f3 ll, float& pdf, float& reflectance)
+{
+    if (lv.z <= 0.0f || ll.z <= 0.0f)
+    {
+        pdf = 0.0f;
+        reflectance = lv.z > 0.0f ? sheen_ltc_params(g, m.sheen_roughness, lv.z).b : 0.0f;
+        return col(0.0f);
+    }
+    float phi = get_phi(lv);
+    f3 l_std = rotate_vector(ll, mk(0.0f, 0.0f, 1.0f), -phi);
+    Color AiBiRi = sheen_ltc_params(g, m.sheen_roughness, lv.z);
+    float Do = eval_ltc(l_std, AiBiRi);
+    pdf = Do;
+    reflectance = AiBiRi.b;
+    return m.sheen_color * AiBiRi.b * Do / ll.z;
+}
+
+PT_HD float sheen_pdf(const BsdfGlobals& g, const Material& m, f3 lv, f3 ll)
+{
+    if (lv.z <= 0.0f || ll.z <= 0.0f)
+        return 0.0f;
+    float phi = get_phi(lv);
+    f3 l_std = rotate_vector(ll, mk(0.0f, 0.0f, 1.0f), -phi);
+    Color AiBiRi = sheen_ltc_params(g, m.sheen_roughness, lv.z);
+    return eval_ltc(l_std, AiBiRi);
+}
+
+// Sampling.h:210-220
+PT_HD f3 cosine_sample_z_up(Xorshift32& rng)
+{
+    float r1 = rng();
+    float r2 = rng();
+    float phi = TWO_PI_F * r1;
+    float cos_theta = sqrtf(r2);
+    float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    float s, co;
+    sincos_fixed(phi, s, co);
+    return normalize(mk(co * sin_theta, s * sin_theta, cos_theta));
+}
+
+// SheenLTC.h:146-175: a cosine sample mapped through the LTC matrix M.
+PT_HD f3 sheen_sample(const BsdfGlobals& g, const Material& m, f3 lv, Xorshift32& rng)
+{
+    f3 cs = cosine_sample_z_up(rng);
+    Color AiBiRi = sheen_ltc_params(g, m.sheen_roughness, lv.z);
+    float Ai_inv = 1.0f / AiBiRi.r;
+    float Bi = AiBiRi.g;
+    f3 d = normalize(mk(cs.x * Ai_inv - cs.z * Bi * Ai_inv, cs.y * Ai_inv, cs.z));
+    return rotate_vector(d, mk(0.0f, 0.0f, 1.0f), get_phi(lv));
+}
+
+// ---------------------------------------------------------------------------
+// Lobe weights and sampling probabilities
+// ---------------------------------------------------------------------------
 struct LobeWeights {
     float coat, sheen, metal1, metal2, specular, diffuse, glass, dtrans;
 };
 
-// Principled.h:1515-1554 (outside the object: opaque scenes never enter one).
+// Principled.h:1515-1554; outside the object (the glass lobe, which lets a
+// path enter one, is refused at upload).
 PT_HD LobeWeights lobe_weights(const Material& m)
 {
     const float outside = 1.0f;
@@ -432,98 +759,248 @@ PT_HD LobeWeights lobe_probas(const LobeWeights& w)
     return p;
 }
 
-// Principled.h:1620-1715, for materials whose coat, sheen, specular, glass and
-// diffuse-transmission weights are zero (the lobes this build implements:
-// two GGX metal lobes and the Lambertian diffuse lobe).
+// The three local frames of one evaluation (Principled.h:1633-1645).
+struct LocalFrames {
+    f3 lv, ll, lh;    // unrotated ONB
+    f3 lvr, llr, lhr; // ONB rotated by anisotropy_rotation * pi
+};
+
+PT_HD LocalFrames local_frames(const Material& m, const BsdfCtx& c)
+{
+    LocalFrames f;
+    f3 T, B;
+    build_onb(c.sn, T, B);
+    f.lv = world_to_local(T, B, c.sn, c.view);
+    f.ll = world_to_local(T, B, c.sn, c.to_light);
+    f.lh = normalize(f.lv + f.ll);
+    f3 TR, BR;
+    build_rotated_onb(c.sn, TR, BR, m.anisotropy_rotation * PI_F);
+    f.lvr = world_to_local(TR, BR, c.sn, c.view);
+    f.llr = world_to_local(TR, BR, c.sn, c.to_light);
+    f.lhr = normalize(f.lvr + f.llr);
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// Layers (Principled.h:955-1512), evaluated top to bottom; `layers` is the
+// energy left for the layers below.
+// ---------------------------------------------------------------------------
+// Principled.h:955-1058
+PT_HD Color eval_coat_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
+                            float incident_ior, bool refracting, float weight, float proba, Color& layers,
+                            float& pdf)
+{
+    float coat_ior = m.coat_ior;
+    if (weight > 0.0f && ((f.lv.z > 0.0f && f.ll.z > 0.0f) || refracting))
+    {
+        float cpdf = 0.0f;
+        Color contribution = col(0.0f);
+        if (!refracting)
+        {
+            contribution = coat_eval(g, m, c, f.lv, f.ll, f.lh, incident_ior, cpdf);
+            contribution = contribution * weight;
+            contribution = contribution * layers;
+        }
+        pdf += cpdf * proba;
+        Color below = col(1.0f);
+        below = below * (1.0f - fresnel_dielectric(fabsf(f.ll.z), incident_ior, coat_ior));
+        float view_dir_fresnel = fresnel_dielectric(fabsf(f.lv.z), incident_ior, coat_ior);
+        below = below * (1.0f - view_dir_fresnel);
+        if (!is_white(m.coat_medium_absorption))
+        {
+            float in_angle = maxf_(1.0e-6f, sqrtf(1.0f - (1.0f - f.ll.z * f.ll.z) / (coat_ior * coat_ior)));
+            float out_angle = maxf_(1.0e-6f, sqrtf(1.0f - (1.0f - f.lv.z * f.lv.z) / (coat_ior * coat_ior)));
+            float distance = 1.0f / in_angle + 1.0f / out_angle;
+            const Color a = m.coat_medium_absorption;
+            Color p = col(pow_fixed(sqrtf(a.r), distance), pow_fixed(sqrtf(a.g), distance),
+                          pow_fixed(sqrtf(a.b), distance));
+            Color e = (col(0.0f) - (col(1.0f) - p)) * m.coat_medium_thickness;
+            below = below * col(exp_fixed(e.r), exp_fixed(e.g), exp_fixed(e.b));
+        }
+        below = below * coat_darkening(m, coat_ior / incident_ior, view_dir_fresnel);
+        below = lerpc(col(1.0f), below, m.coat);
+        layers = layers * below;
+        return contribution;
+    }
+    return col(0.0f);
+}
+
+PT_HD float pdf_coat_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
+                           bool refracting, float weight, float proba)
+{
+    if (weight > 0.0f && ((f.lv.z > 0.0f && f.ll.z > 0.0f) || refracting))
+    {
+        float cpdf = 0.0f;
+        if (!refracting)
+            cpdf = coat_pdf(g, m, c, f.lv, f.ll, f.lh);
+        return cpdf * proba;
+    }
+    return 0.0f;
+}
+
+// Principled.h:1088-1113
+PT_HD Color eval_sheen_layer(const BsdfGlobals& g, const Material& m, const LocalFrames& f, bool refracting,
+                             float weight, float proba, Color& layers, float& pdf)
+{
+    if ((weight > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f) || refracting)
+    {
+        float reflectance, spdf;
+        Color contribution = sheen_eval(g, m, f.lv, f.ll, spdf, reflectance);
+        contribution = contribution * weight;
+        contribution = contribution * layers;
+        pdf += spdf * proba;
+        layers = layers * (1.0f - m.sheen * reflectance);
+        return contribution;
+    }
+    return col(0.0f);
+}
+
+PT_HD float pdf_sheen_layer(const BsdfGlobals& g, const Material& m, const LocalFrames& f, bool refracting,
+                            float weight, float proba)
+{
+    if ((weight > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f) || refracting)
+        return sheen_pdf(g, m, f.lv, f.ll) * proba;
+    return 0.0f;
+}
+
+// Principled.h:1129-1157 (the metal lobe has nothing below it).
+PT_HD Color eval_metal_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
+                             float roughness, float weight, float proba, const Color& layers, float& pdf)
+{
+    if (weight > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
+    {
+        float mpdf = 0.0f;
+        Color contribution = metallic_eval(g, m, c, roughness, m.anisotropy, f.lvr, f.llr, f.lhr, mpdf);
+        contribution = contribution * weight;
+        contribution = contribution * layers;
+        pdf += mpdf * proba;
+        return contribution;
+    }
+    return col(0.0f);
+}
+
+PT_HD float pdf_metal_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
+                            float roughness, float weight, float proba)
+{
+    if (weight > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
+        return metallic_pdf(g, m, c, roughness, m.anisotropy, f.lvr, f.llr, f.lhr) * proba;
+    return 0.0f;
+}
+
+// Principled.h:1305-1405, in the rotated frame.
+PT_HD Color eval_specular_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
+                                float incident_ior, float weight, bool refracting, float proba, Color& layers,
+                                float& pdf)
+{
+    if (weight > 0.0f && ((f.lvr.z > 0.0f && f.llr.z > 0.0f) || (refracting && c.info == FROM_DIFFUSE_TRANSMISSION)))
+    {
+        float relative_ior = specular_relative_ior(m, incident_ior);
+        float spdf = 0.0f;
+        Color contribution = specular_eval(g, m, c, relative_ior, f.lvr, f.llr, f.lhr, spdf);
+        contribution = contribution * lerpc(col(1.0f), m.specular_tint * m.specular_color, m.specular);
+        contribution = contribution * weight;
+        contribution = contribution * layers;
+        Color below = col(1.0f);
+        Color light_dir_fresnel = specular_fresnel(m, relative_ior, fabsf(f.llr.z));
+        light_dir_fresnel = light_dir_fresnel * (1.0f - m.diffuse_transmission);
+        below = below * (col(1.0f) - light_dir_fresnel);
+        Color view_dir_fresnel = specular_fresnel(m, relative_ior, fabsf(f.lvr.z));
+        below = below * (col(1.0f) - view_dir_fresnel);
+        below = below * specular_darkening(m, relative_ior, luminance(view_dir_fresnel));
+        below = lerpc(col(1.0f), below, m.specular);
+        layers = layers * below;
+        pdf += spdf * proba;
+        return contribution;
+    }
+    return col(0.0f);
+}
+
+PT_HD float pdf_specular_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
+                               float weight, bool refracting, float proba)
+{
+    if (weight > 0.0f && ((f.lvr.z > 0.0f && f.llr.z > 0.0f) || (refracting && c.info == FROM_DIFFUSE_TRANSMISSION)))
+        return specular_pdf(g, m, c, f.lvr, f.llr, f.lhr) * proba;
+    return 0.0f;
+}
+
+// Principled.h:1436-1469 (Lambertian diffuse, Lambertian.h:14-38).
+PT_HD Color eval_diffuse_layer(const Material& m, const LocalFrames& f, float weight, float proba, const Color& layers,
+                               float& pdf)
+{
+    if (weight > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f)
+    {
+        float dpdf = 0.0f;
+        Color contribution = col(0.0f);
+        if (f.ll.z > 0.0f)
+        {
+            dpdf = f.ll.z * INV_PI_F;
+            contribution = m.base_color * INV_PI_F;
+        }
+        contribution = contribution * weight;
+        contribution = contribution * layers;
+        pdf += dpdf * proba;
+        return contribution;
+    }
+    return col(0.0f);
+}
+
+PT_HD float pdf_diffuse_layer(const LocalFrames& f, float weight, float proba)
+{
+    if (weight > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f)
+        return (f.ll.z > 0.0f ? f.ll.z * INV_PI_F : 0.0f) * proba;
+    return 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Principled BSDF (Principled.h:1620-1939). The glass and diffuse
+// transmission lobes are refused at upload, so their weights are 0 here and
+// their layers contribute nothing.
+// ---------------------------------------------------------------------------
 PT_HD Color principled_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, float& pdf)
 {
     pdf = 0.0f;
+    const float incident_ior = 1.0f; // air: no nested dielectrics without the glass lobe
     bool refracting = dot(c.sn, c.to_light) < 0.0f;
-    f3 T, B;
-    build_onb(c.sn, T, B);
-    f3 lv = world_to_local(T, B, c.sn, c.view);
-    f3 ll = world_to_local(T, B, c.sn, c.to_light);
-    f3 TR, BR;
-    build_rotated_onb(c.sn, TR, BR, m.anisotropy_rotation * PI_F);
-    f3 lvr = world_to_local(TR, BR, c.sn, c.view);
-    f3 llr = world_to_local(TR, BR, c.sn, c.to_light);
-    f3 lhr = normalize(lvr + llr);
+    LocalFrames f = local_frames(m, c);
     LobeWeights w = lobe_weights(m);
     LobeWeights p = lobe_probas(w);
     Color layers = col(1.0f);
     Color final_color = col(0.0f);
-    // metal lobes (Principled.h:1129-1157)
-    float wm[2] = {w.metal1 * (float)!refracting, w.metal2 * (float)!refracting};
-    float pm[2] = {p.metal1, p.metal2};
-    float rm[2] = {m.roughness, m.second_roughness};
-    for (int k = 0; k < 2; k++)
-    {
-        if (wm[k] > 0.0f && lvr.z > 0.0f && llr.z > 0.0f)
-        {
-            float mpdf = 0.0f;
-            Color contrib = metallic_eval(g, m, c, rm[k], m.anisotropy, lvr, llr, lhr, mpdf);
-            contrib = contrib * wm[k];
-            contrib = contrib * layers;
-            pdf += mpdf * pm[k];
-            final_color = final_color + contrib;
-        }
-        else
-            final_color = final_color + col(0.0f);
-    }
-    // glossy base = specular (weight 0 here) + Lambertian diffuse
-    // (Principled.h:1436-1492, Lambertian.h:14-23).
-    Color glossy = col(0.0f) + col(0.0f);
-    float wd = w.diffuse * (float)!refracting;
-    if (wd > 0.0f && lv.z > 0.0f && ll.z > 0.0f)
-    {
-        float dpdf = 0.0f;
-        Color contrib = col(0.0f);
-        if (ll.z > 0.0f)
-        {
-            dpdf = ll.z * INV_PI_F;
-            contrib = m.base_color * INV_PI_F;
-        }
-        contrib = contrib * wd;
-        contrib = contrib * layers;
-        pdf += dpdf * p.diffuse;
-        glossy = glossy + contrib;
-    }
-    // glossy-base energy compensation is 1 without a specular layer
-    // (PrincipledEnergyCompensation.h:21-25); clearcoat compensation is 1
-    // without a coat (:84-88).
-    final_color = final_color + glossy / 1.0f;
-    return final_color / 1.0f;
+    final_color = final_color + eval_coat_layer(g, m, c, f, incident_ior, refracting, w.coat, p.coat, layers, pdf);
+    final_color = final_color + eval_sheen_layer(g, m, f, refracting, w.sheen, p.sheen, layers, pdf);
+    final_color = final_color + eval_metal_layer(g, m, c, f, m.roughness, w.metal1 * (float)!refracting, p.metal1,
+                                                 layers, pdf);
+    final_color = final_color + eval_metal_layer(g, m, c, f, m.second_roughness, w.metal2 * (float)!refracting,
+                                                 p.metal2, layers, pdf);
+    // glossy base: specular layer over the diffuse layer (Principled.h:1474-1492)
+    Color glossy = col(0.0f);
+    glossy = glossy + eval_specular_layer(g, m, c, f, incident_ior, w.specular, refracting, p.specular, layers, pdf);
+    glossy = glossy + eval_diffuse_layer(m, f, w.diffuse * (float)!refracting, p.diffuse, layers, pdf);
+    final_color = final_color + glossy / energy_compensation_glossy_base(g, m, incident_ior, f.lv.z);
+    return final_color / energy_compensation_clearcoat(g, m, incident_ior, f.lv.z);
 }
 
 PT_HD float principled_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx& c)
 {
     float pdf = 0.0f;
     bool refracting = dot(c.sn, c.to_light) < 0.0f;
-    f3 T, B;
-    build_onb(c.sn, T, B);
-    f3 lv = world_to_local(T, B, c.sn, c.view);
-    f3 ll = world_to_local(T, B, c.sn, c.to_light);
-    f3 TR, BR;
-    build_rotated_onb(c.sn, TR, BR, m.anisotropy_rotation * PI_F);
-    f3 lvr = world_to_local(TR, BR, c.sn, c.view);
-    f3 llr = world_to_local(TR, BR, c.sn, c.to_light);
-    f3 lhr = normalize(lvr + llr);
+    LocalFrames f = local_frames(m, c);
     LobeWeights w = lobe_weights(m);
     LobeWeights p = lobe_probas(w);
-    float wm[2] = {w.metal1 * (float)!refracting, w.metal2 * (float)!refracting};
-    float pm[2] = {p.metal1, p.metal2};
-    float rm[2] = {m.roughness, m.second_roughness};
-    for (int k = 0; k < 2; k++)
-        if (wm[k] > 0.0f && lvr.z > 0.0f && llr.z > 0.0f)
-            pdf += metallic_pdf(g, m, c, rm[k], m.anisotropy, lvr, llr, lhr) * pm[k];
-    float wd = w.diffuse * (float)!refracting;
-    if (wd > 0.0f && lv.z > 0.0f && ll.z > 0.0f)
-        pdf += (ll.z > 0.0f ? ll.z * INV_PI_F : 0.0f) * p.diffuse;
+    pdf += pdf_coat_layer(g, m, c, f, refracting, w.coat, p.coat);
+    pdf += pdf_sheen_layer(g, m, f, refracting, w.sheen, p.sheen);
+    pdf += pdf_metal_layer(g, m, c, f, m.roughness, w.metal1 * (float)!refracting, p.metal1);
+    pdf += pdf_metal_layer(g, m, c, f, m.second_roughness, w.metal2 * (float)!refracting, p.metal2);
+    float glossy = 0.0f;
+    glossy += pdf_specular_layer(g, m, c, f, w.specular, refracting, p.specular);
+    glossy += pdf_diffuse_layer(f, w.diffuse * (float)!refracting, p.diffuse);
+    pdf += glossy;
     return pdf;
 }
 
 // Principled.h:1802-1939. Returns the BSDF value of the sampled direction;
-// pdf = 0 when the sample is invalid. `c.info` receives the sampled lobe.
+// pdf = 0 when the sample is invalid. `c.info` receives the sampled lobe
+// (the sheen lobe leaves it as it was).
 PT_HD Color principled_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& c, f3& out_dir, float& pdf,
                               Xorshift32& rng)
 {
@@ -543,13 +1020,22 @@ PT_HD Color principled_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& 
     f3 TR, BR;
     build_rotated_onb(c.sn, TR, BR, m.anisotropy_rotation * PI_F);
     f3 lvr = world_to_local(TR, BR, c.sn, c.view);
-    // Coat, sheen, specular, diffuse-transmission and glass lobes are not
-    // built; materials with those lobes are rejected at upload, so their
-    // branches carry zero probability.
     if (rand_1 < cdf0)
-        return col(0.0f);
+    {
+        f3 TC, BC;
+        build_rotated_onb(c.sn, TC, BC, m.coat_anisotropy_rotation * PI_F);
+        f3 lvc = world_to_local(TC, BC, c.sn, c.view);
+        c.info = FROM_COAT;
+        float rr = regularize_reflection(g, c.reg, m.coat_roughness, c.path_roughness);
+        out_dir = local_to_world(TC, BC, c.sn, ggx_sample_reflection(rr, m.coat_anisotropy, lvc, rng));
+    }
     else if (rand_1 < cdf1)
-        return col(0.0f);
+    {
+        f3 T, B;
+        build_onb(c.sn, T, B);
+        f3 lv = world_to_local(T, B, c.sn, c.view);
+        out_dir = local_to_world(T, B, c.sn, sheen_sample(g, m, lv, rng));
+    }
     else if (rand_1 < cdf2)
     {
         c.info = FROM_FIRST_METAL;
@@ -563,13 +1049,18 @@ PT_HD Color principled_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& 
         out_dir = local_to_world(TR, BR, c.sn, ggx_sample_reflection(rr, m.anisotropy, lvr, rng));
     }
     else if (rand_1 < cdf4)
-        return col(0.0f);
+    {
+        c.info = FROM_SPECULAR;
+        float rr = regularize_reflection(g, c.reg, m.roughness, c.path_roughness);
+        out_dir = local_to_world(TR, BR, c.sn, ggx_sample_reflection(rr, m.anisotropy, lvr, rng));
+    }
     else if (rand_1 < cdf5)
     {
         c.info = FROM_DIFFUSE;
         out_dir = cosine_sample_around_normal(c.sn, rng);
     }
     else
+        // diffuse transmission and glass carry zero probability (refused at upload)
         return col(0.0f);
     if (dot(out_dir, c.gn) < 0.0f && !sampling_glass && !sampling_dt)
         return col(0.0f);

@@ -744,6 +744,9 @@ void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int3
         case 1: out[i] = orc_cosf(x[i]); break;
         case 2: out[i] = orc_powf(x[i], y[i]); break;
         case 3: out[i] = orc_log2f(x[i]); break;
+        case 5: out[i] = orc_logf(x[i]); break;
+        case 6: out[i] = orc_expf(x[i]); break;
+        case 7: out[i] = orc_atan2f(x[i], y[i]); break;
         default: out[i] = orc_exp2f(x[i]); break;
         }
     }

@@ -204,6 +204,16 @@ typedef struct {
     int32_t guard_backfacing_bsdf_candidates; /* 0: the reference's 0/0 weight */
 } orc_restir_settings;
 
+/* The BSDF tables (the .f32 files of hiprt-path-tracer_amd/data, converted from the
+ * reference's baked data): layouts as the library embeds them. */
+typedef struct {
+    const float* conductor;         /* GGX conductor E_ss, 128 x 128, row 0 = roughness 1 */
+    const float* glossy_dielectric; /* [F0^(1/4) 128][roughness 64, flipped][cos^(1/2.5) 128] */
+    const float* sheen_ltc;         /* [32][32] x (Ai, Bi, Ri) */
+    const float* glass;             /* [128][16][256] */
+    const float* glass_inv;         /* [128][16][256] */
+} orc_luts;
+
 typedef struct {
     const float* vertices;          /* float[3V] */
     const int32_t* indices;         /* int[3T] */
@@ -213,7 +223,7 @@ typedef struct {
     int32_t material_count;
     const int32_t* emissive;        /* int[E] */
     int32_t emissive_count;
-    const float* lut_conductor;     /* 128 x 128 E_ss, row 0 = roughness 1 */
+    orc_luts luts;
 } orc_render_scene;
 
 /* One sample of every pixel: CameraRays (CameraRays.h:83-213) then MegaKernel
@@ -255,12 +265,15 @@ void orc_alias_table(const float* weights, int32_t n, float total_sum, float* pr
 /* Principled BSDF (metal + diffuse lobes) of one material, restating
  * principled_bsdf_eval / _sample (Principled.h:1620-1715, :1802-1939). Vectors
  * are float[3] in world space; reg_mode 0/1/2 = none/classic/MIS. */
-void orc_bsdf_eval(const orc_cpu_material* m, const float* lut, float consistent_tau, int32_t reg_mode,
+void orc_bsdf_eval(const orc_cpu_material* m, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                    float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light,
                    float* out_rgb, float* out_pdf);
-void orc_bsdf_sample(const orc_cpu_material* m, const float* lut, float consistent_tau, int32_t reg_mode,
+void orc_bsdf_sample(const orc_cpu_material* m, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                      float path_roughness, const float* view, const float* sn, const float* gn, uint32_t* rng_state,
                      float* out_dir, float* out_rgb, float* out_pdf);
+/* principled_bsdf_pdf (Principled.h:1717-1796) of the same configuration. */
+float orc_bsdf_pdf(const orc_cpu_material* m, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
+                   float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light);
 uint32_t orc_oct_pack(const float* n);
 /* fmath.h functions element-wise: fn 0 sin, 1 cos, 2 pow(x, y), 3 log2, 4 exp2. */
 void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n);

@@ -56,11 +56,18 @@ class RestirSettings(C.Structure):
     ]
 
 
+class Luts(C.Structure):
+    """orc_luts: the BSDF tables."""
+
+    _fields_ = [("conductor", C.c_void_p), ("glossy_dielectric", C.c_void_p), ("sheen_ltc", C.c_void_p),
+                ("glass", C.c_void_p), ("glass_inv", C.c_void_p)]
+
+
 class RenderScene(C.Structure):
     _fields_ = [
         ("vertices", C.c_void_p), ("indices", C.c_void_p), ("triangle_count", C.c_int32),
         ("material_indices", C.c_void_p), ("materials", C.c_void_p), ("material_count", C.c_int32),
-        ("emissive", C.c_void_p), ("emissive_count", C.c_int32), ("lut_conductor", C.c_void_p),
+        ("emissive", C.c_void_p), ("emissive_count", C.c_int32), ("luts", Luts),
     ]
 
 
@@ -88,6 +95,7 @@ _SIG = {
     "orc_oct_unpack": (None, [C.c_uint32, _P]),
     "orc_bsdf_eval": (None, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P, _P, _P]),
     "orc_bsdf_sample": (None, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_bsdf_pdf": (C.c_float, [_P, _P, C.c_float, C.c_int32, C.c_float, _P, _P, _P, _P]),
     "orc_fmath_eval": (None, [C.c_int32, _P, _P, _P, C.c_int32]),
     "orc_frame_create": (C.c_void_p, [_P, C.c_void_p, _P, _P, _P]),
     "orc_frame_free": (None, [C.c_void_p]),
@@ -243,11 +251,40 @@ def bounce_rays(vertices, indices, primary, hits, rng_state):
     return out, rng
 
 
-LUT_PATH = os.path.join(os.path.dirname(HERE), "hiprt-path-tracer_amd", "data", "ggx_conductor_uncorrelated_128x128.f32")
+DATA_DIR = os.path.join(os.path.dirname(HERE), "hiprt-path-tracer_amd", "data")
+LUT_PATH = os.path.join(DATA_DIR, "ggx_conductor_uncorrelated_128x128.f32")
+LUT_FILES = {
+    "conductor": ("ggx_conductor_uncorrelated_128x128.f32", (128, 128)),
+    "glossy_dielectric": ("glossy_dielectric_uncorrelated_128x64x128.f32", (128, 64, 128)),
+    "sheen_ltc": ("sheen_ltc_32x32.f32", (32, 32, 3)),
+    "glass": ("ggx_glass_uncorrelated_256x16x128.f32", (128, 16, 256)),
+    "glass_inv": ("ggx_glass_inv_uncorrelated_256x16x128.f32", (128, 16, 256)),
+}
 
 
 def load_conductor_lut(path: str = LUT_PATH) -> np.ndarray:
     return np.fromfile(path, "<f4").reshape(128, 128)
+
+
+_LUTS = {}
+
+
+def load_luts(conductor=None):
+    """{name: float32 array} of the BSDF tables (conductor replaceable)."""
+    if not _LUTS:
+        for k, (f, shape) in LUT_FILES.items():
+            _LUTS[k] = np.fromfile(os.path.join(DATA_DIR, f), "<f4").reshape(shape)
+    out = dict(_LUTS)
+    if conductor is not None:
+        out["conductor"] = np.ascontiguousarray(conductor, np.float32)
+    return out
+
+
+def make_luts(arrays):
+    s = Luts()
+    for k in LUT_FILES:
+        setattr(s, k, _p(arrays[k]))
+    return s
 
 
 RESERVOIR_DTYPE = np.dtype([("point", np.float32, 3), ("target", np.float32), ("M", np.int32),
@@ -272,7 +309,7 @@ class Renderer:
         self.mat_idx = np.ascontiguousarray(scene.material_indices, np.int32)
         self.mat_bytes = (C.c_char * C.sizeof(scene.materials)).from_buffer_copy(bytes(scene.materials))
         self.emissive = np.ascontiguousarray(scene.emissive, np.int32)
-        self.lut = np.ascontiguousarray(load_conductor_lut() if lut is None else lut, np.float32)
+        self.luts = load_luts(lut)
         self.tree = Octree(scene.vertices, scene.indices)
         rs = RenderScene()
         rs.vertices = _p(self.verts)
@@ -283,7 +320,7 @@ class Renderer:
         rs.material_count = len(scene.materials)
         rs.emissive = _p(self.emissive)
         rs.emissive_count = self.emissive.size
-        rs.lut_conductor = _p(self.lut)
+        rs.luts = make_luts(self.luts)
         self.rs = rs
         self.cam = _cam(scene.camera)
         self.n = settings.width * settings.height
@@ -364,11 +401,12 @@ def consistent_tau(tau_0, sample_number):
     return float(lib().orc_consistent_tau(tau_0, sample_number))
 
 
-FMATH = {"sin": 0, "cos": 1, "pow": 2, "log2": 3, "exp2": 4}
+FMATH = {"sin": 0, "cos": 1, "pow": 2, "log2": 3, "exp2": 4, "log": 5, "exp": 6, "atan2": 7}
 
 
 def fmath(name, x, y=None):
-    """The fixed-sequence sin/cos/pow/log2/exp2 of oracle/fmath.h, element-wise."""
+    """The fixed-sequence sin/cos/pow/log2/exp2/log/exp/atan2 of oracle/fmath.h,
+    element-wise (atan2 takes x = the y argument, y = the x argument)."""
     x = np.ascontiguousarray(x, np.float32).reshape(-1)
     y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float32)
     out = np.zeros_like(x)
@@ -392,7 +430,8 @@ class Bsdf:
 
     def __init__(self, material, lut=None, consistent_tau=1.0e6, reg_mode=0, path_roughness=0.0):
         self.mat = (C.c_char * C.sizeof(material)).from_buffer_copy(bytes(material))
-        self.lut = np.ascontiguousarray(load_conductor_lut() if lut is None else lut, np.float32)
+        self.arrays = load_luts(lut)
+        self.luts = make_luts(self.arrays)
         self.tau = consistent_tau
         self.reg = reg_mode
         self.path_r = path_roughness
@@ -401,9 +440,14 @@ class Bsdf:
         f = np.zeros(3, np.float32)
         pdf = C.c_float()
         a = [np.ascontiguousarray(x, np.float32) for x in (view, sn, gn, to_light)]
-        lib().orc_bsdf_eval(C.cast(self.mat, C.c_void_p), _p(self.lut), self.tau, self.reg, self.path_r,
+        lib().orc_bsdf_eval(C.cast(self.mat, C.c_void_p), C.byref(self.luts), self.tau, self.reg, self.path_r,
                             *[_p(x) for x in a], _p(f), C.byref(pdf))
         return f, pdf.value
+
+    def pdf(self, view, sn, gn, to_light):
+        a = [np.ascontiguousarray(x, np.float32) for x in (view, sn, gn, to_light)]
+        return float(lib().orc_bsdf_pdf(C.cast(self.mat, C.c_void_p), C.byref(self.luts), self.tau, self.reg,
+                                        self.path_r, *[_p(x) for x in a]))
 
     def sample(self, view, sn, gn, state):
         d = np.zeros(3, np.float32)
@@ -411,6 +455,6 @@ class Bsdf:
         pdf = C.c_float()
         st = C.c_uint32(state)
         a = [np.ascontiguousarray(x, np.float32) for x in (view, sn, gn)]
-        lib().orc_bsdf_sample(C.cast(self.mat, C.c_void_p), _p(self.lut), self.tau, self.reg, self.path_r,
+        lib().orc_bsdf_sample(C.cast(self.mat, C.c_void_p), C.byref(self.luts), self.tau, self.reg, self.path_r,
                               *[_p(x) for x in a], C.byref(st), _p(d), _p(f), C.byref(pdf))
         return d, f, pdf.value, int(st.value)

@@ -48,6 +48,7 @@ static inline float hclamp(float lo, float hi, float v) { return hmin(hi, hmax(l
 static inline float hlerp(float a, float b, float t) { return (1.0f - t) * a + t * b; }
 static inline float cmax(rgb c) { return hmax(c.r, hmax(c.g, c.b)); } /* Color.h:98 */
 static inline int cblack(rgb c) { return !(c.r > 0.0f || c.g > 0.0f || c.b > 0.0f); } /* Color.h:95 */
+static inline float lum(rgb c) { return 0.3086f * c.r + 0.6094f * c.g + 0.0820f * c.b; } /* Color.h:90 */
 
 #define M_PI_F 3.14159265358979323846f
 #define M_TWO_PI_F 6.28318530717958647693f
@@ -63,7 +64,19 @@ typedef struct {
     rgb F82, F90;
     float F90_exp, anisotropy, anisotropy_rotation, second_roughness_weight, second_roughness;
     float specular, coat, sheen, specular_transmission, diffuse_transmission, thin_film, ior;
-    int do_metallic_ec;
+    int do_metallic_ec, do_specular_ec, do_coat_ec, do_glass_ec, thin_walled;
+    float specular_tint;
+    rgb specular_color;
+    float specular_darkening;
+    rgb coat_absorption;
+    float coat_thickness, coat_roughness, coat_roughening, coat_darkening, coat_anisotropy, coat_anisotropy_rotation,
+        coat_ior;
+    float sheen_roughness;
+    rgb sheen_color;
+    float absorption_at_distance;
+    rgb absorption_color;
+    float dispersion_scale, dispersion_abbe_number;
+    int dielectric_priority;
 } omat;
 
 static unsigned char q8(float x) { return (unsigned char)(x * 255.0f); }
@@ -100,6 +113,47 @@ static omat unpack_material(const orc_cpu_material* m)
     o.thin_film = u8f(q8(m->thin_film));
     o.ior = m->ior;
     o.do_metallic_ec = m->do_metallic_energy_compensation != 0;
+    o.do_specular_ec = m->do_specular_energy_compensation != 0;
+    o.do_coat_ec = m->do_coat_energy_compensation != 0;
+    o.do_glass_ec = m->do_glass_energy_compensation != 0;
+    o.thin_walled = m->thin_walled != 0;
+    o.specular_tint = u8f(q8(m->specular_tint));
+    o.specular_color = color24(m->specular_color);
+    o.specular_darkening = u8f(q8(m->specular_darkening));
+    o.coat_absorption = color24(m->coat_medium_absorption);
+    o.coat_thickness = m->coat_medium_thickness;
+    o.coat_roughness = u8f(q8(m->coat_roughness));
+    o.coat_roughening = u8f(q8(m->coat_roughening));
+    o.coat_darkening = u8f(q8(m->coat_darkening));
+    o.coat_anisotropy = u8f(q8(m->coat_anisotropy));
+    o.coat_anisotropy_rotation = u8f(q8(m->coat_anisotropy_rotation));
+    o.coat_ior = m->coat_ior;
+    o.sheen_roughness = u8f(q8(m->sheen_roughness));
+    o.sheen_color = color24(m->sheen_color);
+    o.absorption_at_distance = m->absorption_at_distance;
+    o.absorption_color = color24(m->absorption_color);
+    o.dispersion_scale = u8f(q8(m->dispersion_scale));
+    o.dispersion_abbe_number = m->dispersion_abbe_number;
+    o.dielectric_priority = m->dielectric_priority;
+    /* get_intersection_material (Material.h:122-149): a coat roughens the
+     * base and second metal roughness */
+    if (o.coat > 0.0f && o.coat_roughening > 0.0f)
+    {
+        float r4 = o.roughness * o.roughness, c4 = o.coat_roughness * o.coat_roughness;
+        r4 = r4 * r4;
+        c4 = c4 * c4;
+        float target = sqrtf(sqrtf(hmin(1.0f, r4 + 2.0f * c4)));
+        float rough = hlerp(o.roughness, target, o.coat);
+        o.roughness = hlerp(o.roughness, rough, o.coat_roughening);
+        if (o.second_roughness_weight > 0.0f)
+        {
+            float s4 = o.second_roughness * o.second_roughness;
+            s4 = s4 * s4;
+            float target2 = sqrtf(sqrtf(hmin(1.0f, s4 + 2.0f * c4)));
+            float rough2 = hlerp(o.second_roughness, target2, o.coat);
+            o.second_roughness = hlerp(o.second_roughness, rough2, o.coat_roughening);
+        }
+    }
     return o;
 }
 
@@ -114,7 +168,9 @@ typedef struct {
     const int32_t* emissive;
     int32_t ne;
     const float* areas;
-    const float* lut;
+    const float* lut;        /* GGX conductor E_ss, 128 x 128 */
+    const float* lut_glossy; /* glossy dielectric E_ss, [128][64][128] */
+    const float* lut_sheen;  /* sheen LTC (Ai, Bi, Ri), 32 x 32 */
     const orc_octree* tree;
     const orc_camera* cam;
     orc_render_settings s;
@@ -130,7 +186,7 @@ typedef struct {
 } rctx;
 
 enum { NO_INFO = 0, FROM_COAT = 2, FROM_FIRST_METAL = 4, FROM_SECOND_METAL = 8, FROM_SPECULAR = 16,
-       FROM_DIFFUSE = 128, FROM_DIFFUSE_TRANSMISSION = 256 };
+       FROM_GLASS_REFLECT = 32, FROM_GLASS_REFRACT = 64, FROM_DIFFUSE = 128, FROM_DIFFUSE_TRANSMISSION = 256 };
 enum { NO_REG = 0, REG_CLASSIC = 1, REG_MIS = 2 };
 
 static void tri(const rctx* R, int32_t p, v3* a, v3* b, v3* c)
@@ -221,9 +277,17 @@ static int delta_sampled(const omat* m, float r, float aniso, int info)
     if (!smooth(r))
         return 0;
     int matching = fabsf(aniso - m->anisotropy) < 1.0e-3f;
+    if (info == FROM_GLASS_REFLECT && smooth(m->roughness) && matching)
+        return 1;
     int first = info == FROM_FIRST_METAL && smooth(m->roughness) && matching;
     int second = info == FROM_SECOND_METAL && smooth(m->second_roughness) && matching;
-    return (first || second) ? 1 : 2;
+    if (first || second)
+        return 1;
+    if (info == FROM_COAT && fabsf(aniso - m->coat_anisotropy) < 1.0e-3f && smooth(m->coat_roughness))
+        return 1;
+    if (info == FROM_SPECULAR && smooth(m->roughness) && matching)
+        return 1;
+    return 2;
 }
 
 static float regularize(const rctx* R, int mode, float r, float path_r)
@@ -394,47 +458,358 @@ static lobes probas(lobes w)
     return p;
 }
 
+/* Dielectric Fresnel (Fresnel.h:11-62, :169-172) */
+static float f0_eta(float eta_t, float eta_i)
+{
+    float n = eta_t - eta_i, d = eta_t + eta_i;
+    return (n * n) / (d * d);
+}
+static float fresnel_d(float cos_i, float rel)
+{
+    if (fabsf(1.0f - rel) < 1.0e-4f)
+        return 0.0f;
+    float si2 = 1.0f - cos_i * cos_i;
+    float st2 = si2 / (rel * rel);
+    if (st2 >= 1.0f)
+        return 1.0f;
+    float ct = sqrtf(1.0f - st2);
+    float rpa = (rel * cos_i - ct) / (rel * cos_i + ct);
+    float rpe = (cos_i - rel * ct) / (cos_i + rel * ct);
+    return (rpa * rpa + rpe * rpe) / 2.0f;
+}
+static float fresnel_d2(float cos_i, float eta_i, float eta_t) { return fresnel_d(cos_i, eta_t / eta_i); }
+static float hemi_albedo_fit(float rel)
+{
+    return orc_logf((10893.0f * rel - 1438.2f) / (-774.4f * (rel * rel) + 10212.0f * rel + 1.0f));
+}
+static rgb fdivc(float a, rgb b) { return C3(a / b.r, a / b.g, a / b.b); }
+static rgb clerp(rgb a, rgb b, float t) { return cadd(cscale(a, 1.0f - t), cscale(b, t)); }
+
+/* Stacked 3D LUT through the CPU nearest sampler (Image.cpp:761-803) */
+static float lut3d(const float* lut, int w, int h, int d, float u, float v, float t)
+{
+    if (u != 1.0f) u -= (float)(int)u;
+    if (v != 1.0f) v -= (float)(int)v;
+    if (t != 1.0f) t -= (float)(int)t;
+    u = u < 0 ? 1.0f + u : u;
+    v = v < 0 ? 1.0f + v : v;
+    t = t < 0 ? 1.0f + t : t;
+    v = 1.0f - v;
+    int x = (int)(u * (float)(w - 1)), y = (int)(v * (float)(h - 1)), z = (int)(t * (float)(d - 1));
+    return lut[(size_t)z * (size_t)(w * h) + (size_t)(x + y * w)];
+}
+
+/* Specular layer and glossy base (Principled.h:215-330, :1272-1512,
+ * PrincipledEnergyCompensation.h:15-125) */
+static float spec_rel_ior(const omat* m, float inc)
+{
+    if (m->coat == 0.0f)
+        return m->ior;
+    float layer = hlerp(inc, m->coat_ior, m->coat);
+    float rel = m->ior / layer;
+    return rel < 1.0f ? 1.0f / rel : rel;
+}
+
+static float ec_glossy(const rctx* R, const omat* m, float inc, float NoV)
+{
+    if (!m->do_specular_ec || m->roughness < 0.0f || m->specular < 1.0e-3f || NoV < 0.0f)
+        return 1.0f;
+    float rel = spec_rel_ior(m, inc);
+    if (fabsf(rel - 1.0f) < 1.0e-3f)
+        rel += 1.0e-3f;
+    float u = orc_powf(NoV, 1.0f / 2.5f);
+    float w = sqrtf(sqrtf(f0_eta(m->ior, m->ior / rel)));
+    float ms = lut3d(R->lut_glossy, 128, 64, 128, u, m->roughness, w);
+    float comp = hlerp(1.0f, ms, m->specular);
+    return hlerp(comp, 1.0f, m->thin_film);
+}
+
+static float ec_clearcoat(const rctx* R, const omat* m, float inc, float NoV)
+{
+    if (!m->do_specular_ec || m->coat < 1.0e-3f || NoV < 0.0f)
+        return 1.0f;
+    if (fabsf(m->coat_ior / inc - 1.0f) < 1.0e-3f)
+        inc += 1.0e-3f;
+    float u = orc_powf(NoV, 1.0f / 2.5f);
+    float w = sqrtf(sqrtf(f0_eta(m->coat_ior, inc)));
+    float ms = lut3d(R->lut_glossy, 128, 64, 128, u, m->coat_roughness, w);
+    float comp = hlerp(1.0f, ms, m->coat * (1.0f - m->specular_transmission));
+    return hlerp(comp, 1.0f, m->thin_film);
+}
+
+static rgb spec_darkening(const omat* m, float rel, float vf)
+{
+    (void)vf;
+    if (m->specular_darkening == 0.0f)
+        return C1(1.0f);
+    float K = 1.0f - (1.0f - hemi_albedo_fit(rel)) / (rel * rel);
+    rgb dk = fdivc(1.0f - K, csub(C1(1.0f), cscale(m->base_color, K)));
+    return clerp(C1(1.0f), dk, m->specular * m->specular_darkening * (1.0f - m->diffuse_transmission));
+}
+
+static rgb coat_darkening(const omat* m, float rel, float vf)
+{
+    if (m->coat_darkening == 0.0f)
+        return C1(1.0f);
+    float Kr = 1.0f - (1.0f - hemi_albedo_fit(rel)) / (rel * rel);
+    float base_r = hlerp(1.0f, m->roughness, hmax(m->specular_transmission, hmax(m->metallic, m->specular)));
+    float K = hlerp(vf, Kr, base_r);
+    rgb albedo = cdivs(cadd(m->base_color, cscale(m->sheen_color, m->sheen)), 1.0f + m->sheen);
+    rgb dk = fdivc(1.0f - K, csub(C1(1.0f), cscale(albedo, K)));
+    return clerp(C1(1.0f), dk, m->coat * m->coat_darkening * (1.0f - m->diffuse_transmission));
+}
+
+/* Sheen LTC (SheenLTC.h) */
+static rgb ltc_params(const rctx* R, float roughness, float cos_t)
+{
+    float u = cos_t, v = hclamp(0.0f, 1.0f, roughness);
+    if (u != 1.0f) u -= (float)(int)u;
+    if (v != 1.0f) v -= (float)(int)v;
+    u = u < 0 ? 1.0f + u : u;
+    v = v < 0 ? 1.0f + v : v;
+    v = 1.0f - v;
+    int x = (int)(u * 31.0f), y = (int)(v * 31.0f);
+    const float* p = R->lut_sheen + 3 * (x + y * 32);
+    return C3(p[0], p[1], p[2]);
+}
+static float ltc_eval(v3 l, rgb A)
+{
+    v3 o = V(l.x * A.r + l.z * A.g, l.y * A.r, l.z);
+    float len = sqrtf(vdot(o, o));
+    o = vdivs(o, len);
+    float jac = (A.r * A.r) / (len * len * len);
+    return o.z * M_INV_PI_F * jac;
+}
+static float phi_of(v3 d)
+{
+    float p = orc_atan2f(d.y, d.x);
+    return p < 0.0f ? p + M_TWO_PI_F : p;
+}
+static v3 rotate_v(v3 v, v3 axis, float angle)
+{
+    float s, c;
+    orc_sincos(angle, &s, &c);
+    return vadd(vadd(vscale(v, c), vscale(vscale(axis, vdot(v, axis)), 1.0f - c)), vscale(vcross(axis, v), s));
+}
+static rgb sheen_eval(const rctx* R, const omat* m, v3 lv, v3 ll, float* pdf, float* refl)
+{
+    if (lv.z <= 0.0f || ll.z <= 0.0f)
+    {
+        *pdf = 0.0f;
+        *refl = lv.z > 0.0f ? ltc_params(R, m->sheen_roughness, lv.z).b : 0.0f;
+        return C1(0.0f);
+    }
+    v3 ls = rotate_v(ll, V(0, 0, 1), -phi_of(lv));
+    rgb A = ltc_params(R, m->sheen_roughness, lv.z);
+    float Do = ltc_eval(ls, A);
+    *pdf = Do;
+    *refl = A.b;
+    return cdivs(cscale(cscale(m->sheen_color, A.b), Do), ll.z);
+}
+static float sheen_pdf(const rctx* R, const omat* m, v3 lv, v3 ll)
+{
+    if (lv.z <= 0.0f || ll.z <= 0.0f)
+        return 0.0f;
+    v3 ls = rotate_v(ll, V(0, 0, 1), -phi_of(lv));
+    return ltc_eval(ls, ltc_params(R, m->sheen_roughness, lv.z));
+}
+static v3 sheen_sample(const rctx* R, const omat* m, v3 lv, uint32_t* st)
+{
+    float r1 = orc_rand01(st), r2 = orc_rand01(st);
+    float phi = M_TWO_PI_F * r1, ct = sqrtf(r2), stt = sqrtf(1.0f - ct * ct);
+    float s, c;
+    orc_sincos(phi, &s, &c);
+    v3 cs = vnorm(V(c * stt, s * stt, ct));
+    rgb A = ltc_params(R, m->sheen_roughness, lv.z);
+    float ai = 1.0f / A.r, bi = A.g;
+    v3 d = vnorm(V(cs.x * ai - cs.z * bi * ai, cs.y * ai, cs.z));
+    return rotate_v(d, V(0, 0, 1), phi_of(lv));
+}
+
+typedef struct { v3 lv, ll, lh, lvr, llr, lhr; } frames;
+
+static frames make_frames(const omat* m, const bctx* c)
+{
+    frames f;
+    v3 T, B, TR, BR;
+    build_onb(c->sn, &T, &B);
+    f.lv = to_local(T, B, c->sn, c->view);
+    f.ll = to_local(T, B, c->sn, c->L);
+    f.lh = vnorm(vadd(f.lv, f.ll));
+    build_rotated_onb(c->sn, &TR, &BR, m->anisotropy_rotation * M_PI_F);
+    f.lvr = to_local(TR, BR, c->sn, c->view);
+    f.llr = to_local(TR, BR, c->sn, c->L);
+    f.lhr = vnorm(vadd(f.lvr, f.llr));
+    return f;
+}
+
+/* one GGX reflection lobe of a dielectric interface (coat or specular) */
+static rgb ggx_dielectric(const rctx* R, const omat* m, const bctx* c, float rough, float aniso, rgb F, v3 lv, v3 ll,
+                          v3 lh, float* pdf)
+{
+    float rr = regularize(R, c->reg, rough, c->path_r);
+    return ggx_reflect_ss(rr, aniso, F, lv, ll, lh, pdf, delta_sampled(m, rr, aniso, c->info));
+}
+static float ggx_dielectric_pdf(const rctx* R, const omat* m, const bctx* c, float rough, float aniso, v3 lv, v3 ll,
+                                v3 lh)
+{
+    float rr = regularize(R, c->reg, rough, c->path_r);
+    int ds = delta_sampled(m, rr, aniso, c->info);
+    float pdf;
+    if (smooth(rr))
+    {
+        if (ds == 2 || vdot(reflect(lv, V(0, 0, 1)), ll) < 0.999999f)
+            return 0.0f;
+        return 1.0e9f;
+    }
+    if (ll.z < 0.0f)
+        return 0.0f;
+    float ax, ay;
+    alphas(rr, aniso, &ax, &ay);
+    float D = ggx_D(ax, ay, lh);
+    float G1V = 1.0f / (1.0f + lambda_(ax, ay, lv));
+    pdf = vndf(D, G1V, lv, lh) / (4.0f * vdot(lv, lh));
+    return pdf;
+}
+
+static float metal_pdf(const rctx* R, const omat* m, const bctx* c, float r, v3 lv, v3 ll, v3 lh)
+{
+    float rr = regularize(R, c->reg, r, c->path_r);
+    int ds = delta_sampled(m, rr, m->anisotropy, c->info);
+    if (ds == 2)
+        return 0.0f;
+    return ggx_dielectric_pdf(R, m, c, r, m->anisotropy, lv, ll, lh);
+}
+
+/* The principled BSDF, layer by layer (Principled.h:955-1512, :1620-1800);
+ * glass and diffuse transmission are refused at upload (weight 0). */
 static rgb principled_eval(const rctx* R, const omat* m, const bctx* c, float* pdf)
 {
     *pdf = 0.0f;
-    int refracting = vdot(c->sn, c->L) < 0.0f;
-    v3 T, B, TR, BR;
-    build_onb(c->sn, &T, &B);
-    v3 lv = to_local(T, B, c->sn, c->view), ll = to_local(T, B, c->sn, c->L);
-    build_rotated_onb(c->sn, &TR, &BR, m->anisotropy_rotation * M_PI_F);
-    v3 lvr = to_local(TR, BR, c->sn, c->view), llr = to_local(TR, BR, c->sn, c->L);
-    v3 lhr = vnorm(vadd(lvr, llr));
+    const float inc = 1.0f;
+    int refr = vdot(c->sn, c->L) < 0.0f;
+    frames f = make_frames(m, c);
     lobes w = weights(m), p = probas(w);
     rgb layers = C1(1.0f), out = C1(0.0f);
-    float wm[2] = {w.m1 * (float)!refracting, w.m2 * (float)!refracting};
+    /* coat */
+    if (w.coat > 0.0f && ((f.lv.z > 0.0f && f.ll.z > 0.0f) || refr))
+    {
+        float cp = 0.0f;
+        rgb contrib = C1(0.0f);
+        if (!refr)
+        {
+            float HoL = hclamp(1.0e-8f, 1.0f, vdot(f.lh, f.ll));
+            rgb F = C1(fresnel_d2(HoL, inc, m->coat_ior));
+            contrib = ggx_dielectric(R, m, c, m->coat_roughness, m->coat_anisotropy, F, f.lv, f.ll, f.lh, &cp);
+            contrib = cmul(cscale(contrib, w.coat), layers);
+        }
+        *pdf += cp * p.coat;
+        rgb below = C1(1.0f);
+        below = cscale(below, 1.0f - fresnel_d2(fabsf(f.ll.z), inc, m->coat_ior));
+        float vf = fresnel_d2(fabsf(f.lv.z), inc, m->coat_ior);
+        below = cscale(below, 1.0f - vf);
+        if (!(m->coat_absorption.r == 1.0f && m->coat_absorption.g == 1.0f && m->coat_absorption.b == 1.0f))
+        {
+            float ci = m->coat_ior;
+            float ia = hmax(1.0e-6f, sqrtf(1.0f - (1.0f - f.ll.z * f.ll.z) / (ci * ci)));
+            float oa = hmax(1.0e-6f, sqrtf(1.0f - (1.0f - f.lv.z * f.lv.z) / (ci * ci)));
+            float dist = 1.0f / ia + 1.0f / oa;
+            rgb a = m->coat_absorption;
+            rgb pw = C3(orc_powf(sqrtf(a.r), dist), orc_powf(sqrtf(a.g), dist), orc_powf(sqrtf(a.b), dist));
+            rgb e = cscale(csub(C1(0.0f), csub(C1(1.0f), pw)), m->coat_thickness);
+            below = cmul(below, C3(orc_expf(e.r), orc_expf(e.g), orc_expf(e.b)));
+        }
+        below = cmul(below, coat_darkening(m, m->coat_ior / inc, vf));
+        below = clerp(C1(1.0f), below, m->coat);
+        layers = cmul(layers, below);
+        out = cadd(out, contrib);
+    }
+    else
+        out = cadd(out, C1(0.0f));
+    /* sheen */
+    if ((w.sheen > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f) || refr)
+    {
+        float sp, refl;
+        rgb contrib = sheen_eval(R, m, f.lv, f.ll, &sp, &refl);
+        contrib = cmul(cscale(contrib, w.sheen), layers);
+        *pdf += sp * p.sheen;
+        layers = cscale(layers, 1.0f - m->sheen * refl);
+        out = cadd(out, contrib);
+    }
+    else
+        out = cadd(out, C1(0.0f));
+    /* metal lobes */
+    float wm[2] = {w.m1 * (float)!refr, w.m2 * (float)!refr};
     float pm[2] = {p.m1, p.m2}, rm[2] = {m->roughness, m->second_roughness};
     for (int k = 0; k < 2; k++)
     {
         rgb contrib = C1(0.0f);
-        if (wm[k] > 0.0f && lvr.z > 0.0f && llr.z > 0.0f)
+        if (wm[k] > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
         {
             float mp = 0.0f;
-            contrib = metallic_eval(R, m, c, rm[k], lvr, llr, lhr, &mp);
-            contrib = cscale(contrib, wm[k]);
-            contrib = cmul(contrib, layers);
+            contrib = metallic_eval(R, m, c, rm[k], f.lvr, f.llr, f.lhr, &mp);
+            contrib = cmul(cscale(contrib, wm[k]), layers);
             *pdf += mp * pm[k];
         }
         out = cadd(out, contrib);
     }
-    rgb glossy = cadd(C1(0.0f), C1(0.0f));
-    float wd = w.diff * (float)!refracting;
-    if (wd > 0.0f && lv.z > 0.0f && ll.z > 0.0f)
+    /* glossy base: specular over diffuse */
+    rgb glossy = C1(0.0f);
+    if (w.spec > 0.0f && ((f.lvr.z > 0.0f && f.llr.z > 0.0f) || (refr && c->info == FROM_DIFFUSE_TRANSMISSION)))
     {
-        /* lambertian_brdf_eval (Lambertian.h:14-23) */
-        float dp = ll.z * M_INV_PI_F;
-        rgb contrib = cscale(m->base_color, M_INV_PI_F);
-        contrib = cscale(contrib, wd);
-        contrib = cmul(contrib, layers);
+        float rel = spec_rel_ior(m, inc);
+        float sp = 0.0f;
+        rgb F = C1(fresnel_d(vdot(f.llr, f.lhr), rel));
+        rgb contrib = ggx_dielectric(R, m, c, m->roughness, m->anisotropy, F, f.lvr, f.llr, f.lhr, &sp);
+        contrib = cmul(contrib, clerp(C1(1.0f), cscale(m->specular_color, m->specular_tint), m->specular));
+        contrib = cmul(cscale(contrib, w.spec), layers);
+        rgb below = C1(1.0f);
+        rgb lf = cscale(C1(fresnel_d(fabsf(f.llr.z), rel)), 1.0f - m->diffuse_transmission);
+        below = cmul(below, csub(C1(1.0f), lf));
+        rgb vfc = C1(fresnel_d(fabsf(f.lvr.z), rel));
+        below = cmul(below, csub(C1(1.0f), vfc));
+        below = cmul(below, spec_darkening(m, rel, lum(vfc)));
+        below = clerp(C1(1.0f), below, m->specular);
+        layers = cmul(layers, below);
+        *pdf += sp * p.spec;
+        glossy = cadd(glossy, contrib);
+    }
+    else
+        glossy = cadd(glossy, C1(0.0f));
+    float wd = w.diff * (float)!refr;
+    if (wd > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f)
+    {
+        float dp = f.ll.z * M_INV_PI_F;
+        rgb contrib = cmul(cscale(cscale(m->base_color, M_INV_PI_F), wd), layers);
         *pdf += dp * p.diff;
         glossy = cadd(glossy, contrib);
     }
-    out = cadd(out, cdivs(glossy, 1.0f));
-    return cdivs(out, 1.0f);
+    out = cadd(out, cdivs(glossy, ec_glossy(R, m, inc, f.lv.z)));
+    return cdivs(out, ec_clearcoat(R, m, inc, f.lv.z));
+}
+
+static float principled_pdf(const rctx* R, const omat* m, const bctx* c)
+{
+    float pdf = 0.0f;
+    int refr = vdot(c->sn, c->L) < 0.0f;
+    frames f = make_frames(m, c);
+    lobes w = weights(m), p = probas(w);
+    if (w.coat > 0.0f && ((f.lv.z > 0.0f && f.ll.z > 0.0f) || refr))
+        pdf += (refr ? 0.0f : ggx_dielectric_pdf(R, m, c, m->coat_roughness, m->coat_anisotropy, f.lv, f.ll, f.lh)) *
+               p.coat;
+    if ((w.sheen > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f) || refr)
+        pdf += sheen_pdf(R, m, f.lv, f.ll) * p.sheen;
+    if (w.m1 * (float)!refr > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
+        pdf += metal_pdf(R, m, c, m->roughness, f.lvr, f.llr, f.lhr) * p.m1;
+    if (w.m2 * (float)!refr > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
+        pdf += metal_pdf(R, m, c, m->second_roughness, f.lvr, f.llr, f.lhr) * p.m2;
+    float glossy = 0.0f;
+    if (w.spec > 0.0f && ((f.lvr.z > 0.0f && f.llr.z > 0.0f) || (refr && c->info == FROM_DIFFUSE_TRANSMISSION)))
+        glossy += ggx_dielectric_pdf(R, m, c, m->roughness, m->anisotropy, f.lvr, f.llr, f.lhr) * p.spec;
+    if (w.diff * (float)!refr > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f)
+        glossy += (f.ll.z > 0.0f ? f.ll.z * M_INV_PI_F : 0.0f) * p.diff;
+    pdf += glossy;
+    return pdf;
 }
 
 static rgb principled_sample(const rctx* R, const omat* m, bctx* c, v3* dir, float* pdf, uint32_t* st)
@@ -448,8 +823,21 @@ static rgb principled_sample(const rctx* R, const omat* m, bctx* c, v3* dir, flo
     v3 TR, BR;
     build_rotated_onb(c->sn, &TR, &BR, m->anisotropy_rotation * M_PI_F);
     v3 lvr = to_local(TR, BR, c->sn, c->view);
-    if (r1 < cdf0) return C1(0.0f);
-    else if (r1 < cdf1) return C1(0.0f);
+    if (r1 < cdf0)
+    {
+        v3 TC, BC;
+        build_rotated_onb(c->sn, &TC, &BC, m->coat_anisotropy_rotation * M_PI_F);
+        v3 lvc = to_local(TC, BC, c->sn, c->view);
+        c->info = FROM_COAT;
+        *dir = to_world(TC, BC, c->sn, ggx_sample(regularize(R, c->reg, m->coat_roughness, c->path_r), m->coat_anisotropy, lvc, st));
+    }
+    else if (r1 < cdf1)
+    {
+        v3 T, B;
+        build_onb(c->sn, &T, &B);
+        v3 lv = to_local(T, B, c->sn, c->view);
+        *dir = to_world(T, B, c->sn, sheen_sample(R, m, lv, st));
+    }
     else if (r1 < cdf2)
     {
         c->info = FROM_FIRST_METAL;
@@ -460,7 +848,11 @@ static rgb principled_sample(const rctx* R, const omat* m, bctx* c, v3* dir, flo
         c->info = FROM_SECOND_METAL;
         *dir = to_world(TR, BR, c->sn, ggx_sample(regularize(R, c->reg, m->second_roughness, c->path_r), m->anisotropy, lvr, st));
     }
-    else if (r1 < cdf4) return C1(0.0f);
+    else if (r1 < cdf4)
+    {
+        c->info = FROM_SPECULAR;
+        *dir = to_world(TR, BR, c->sn, ggx_sample(regularize(R, c->reg, m->roughness, c->path_r), m->anisotropy, lvr, st));
+    }
     else if (r1 < cdf5)
     {
         c->info = FROM_DIFFUSE;
@@ -557,7 +949,6 @@ static int trace_main(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out)
 /* AliasTable.h:13-22) and NEE with MIS (Lights.h:123-226,              */
 /* PDFConversion.h:19-38, Sampling.h:103-136)                           */
 /* ------------------------------------------------------------------ */
-static float lum(rgb c) { return 0.3086f * c.r + 0.6094f * c.g + 0.0820f * c.b; } /* Color.h:90 */
 
 typedef struct { int32_t tri; v3 normal; float area; rgb emission; v3 point; float area_pdf; } olight;
 
@@ -855,21 +1246,23 @@ int32_t orc_render_sample(const orc_render_scene* sc, const orc_octree* tree, co
 /* ------------------------------------------------------------------ */
 /* BSDF entry points for the oracle's own property tests.               */
 /* ------------------------------------------------------------------ */
-static rctx bsdf_ctx(const float* lut, float consistent_tau, int32_t regularize)
+static rctx bsdf_ctx(const orc_luts* luts, float consistent_tau, int32_t regularize)
 {
     rctx R;
     memset(&R, 0, sizeof(R));
-    R.lut = lut;
+    R.lut = luts->conductor;
+    R.lut_glossy = luts->glossy_dielectric;
+    R.lut_sheen = luts->sheen_ltc;
     R.consistent_tau = consistent_tau;
     R.s.do_microfacet_regularization = regularize;
     return R;
 }
 
-void orc_bsdf_eval(const orc_cpu_material* cm, const float* lut, float consistent_tau, int32_t reg_mode,
+void orc_bsdf_eval(const orc_cpu_material* cm, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                    float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light,
                    float* out_rgb, float* out_pdf)
 {
-    rctx R = bsdf_ctx(lut, consistent_tau, 1);
+    rctx R = bsdf_ctx(luts, consistent_tau, 1);
     omat m = unpack_material(cm);
     bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]),
               V(to_light[0], to_light[1], to_light[2]), NO_INFO, reg_mode, path_roughness};
@@ -877,11 +1270,11 @@ void orc_bsdf_eval(const orc_cpu_material* cm, const float* lut, float consisten
     out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
 }
 
-void orc_bsdf_sample(const orc_cpu_material* cm, const float* lut, float consistent_tau, int32_t reg_mode,
+void orc_bsdf_sample(const orc_cpu_material* cm, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                      float path_roughness, const float* view, const float* sn, const float* gn, uint32_t* rng_state,
                      float* out_dir, float* out_rgb, float* out_pdf)
 {
-    rctx R = bsdf_ctx(lut, consistent_tau, 1);
+    rctx R = bsdf_ctx(luts, consistent_tau, 1);
     omat m = unpack_material(cm);
     bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]), V(0, 0, 0), NO_INFO,
               reg_mode, path_roughness};
@@ -889,6 +1282,16 @@ void orc_bsdf_sample(const orc_cpu_material* cm, const float* lut, float consist
     rgb f = principled_sample(&R, &m, &c, &d, out_pdf, rng_state);
     out_dir[0] = d.x; out_dir[1] = d.y; out_dir[2] = d.z;
     out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
+}
+
+float orc_bsdf_pdf(const orc_cpu_material* cm, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
+                   float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light)
+{
+    rctx R = bsdf_ctx(luts, consistent_tau, 1);
+    omat m = unpack_material(cm);
+    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]),
+              V(to_light[0], to_light[1], to_light[2]), NO_INFO, reg_mode, path_roughness};
+    return principled_pdf(&R, &m, &c);
 }
 
 /* ================================================================== */
@@ -1024,7 +1427,8 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
         free(power);
     }
     rctx R = {sc->vertices, sc->indices, sc->material_indices, f->mats, sc->emissive, sc->emissive_count, f->areas,
-              sc->lut_conductor, tree, cam, *s, 0, 0.0f, f->alias_probas, f->alias_alias, sum, NULL, 0};
+              sc->luts.conductor, sc->luts.glossy_dielectric, sc->luts.sheen_ltc, tree, cam, *s, 0, 0.0f,
+              f->alias_probas, f->alias_alias, sum, NULL, 0};
     f->R = R;
     if (rs)
         f->rs = *rs;

@@ -155,6 +155,21 @@ def test_restir_config2_geometry(mipt_mod, oracle_mod, torch_mod, guard):
     assert fb.mean() > 0.0
 
 
+def test_restir_config2_layered_materials(mipt_mod, oracle_mod, torch_mod):
+    """ReSTIR DI at the at-scale settings on the 1M-triangle scene with its
+    layered principled materials (coat, sheen, dielectric specular,
+    anisotropy), the glass lobe off."""
+    scene = mipt_mod.generate_scene(2, 160, 90)
+    for m in scene.materials:
+        m.specular_transmission = 0.0
+    s = mipt_mod.default_render_settings(width=160, height=90, nb_bounces=5, direct_light_sampling_strategy=5,
+                                         direct_light_sampling_base_strategy=1)
+    rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
+                                             spatial_number_of_passes=2)
+    fb, R = render_both(mipt_mod, oracle_mod, scene, s, rs, 2)
+    assert fb.mean() > 0.0
+
+
 def test_power_alias_table(mipt_mod, oracle_mod, torch_mod):
     scene = simplify_materials(mipt_mod.generate_scene(2, 64, 36))
     s = mipt_mod.default_render_settings(width=64, height=36, direct_light_sampling_base_strategy=1)

@@ -1,4 +1,4 @@
-"""Accuracy of the fixed-sequence sin/cos/pow (oracle/fmath.h; the kernels
+"""Accuracy of the fixed-sequence sin/cos/pow/log/exp/atan2 (oracle/fmath.h; the kernels
 evaluate the same sequences, so renders agree bit for bit).
 
 Each function is compared with the correctly rounded binary32 value of the
@@ -72,3 +72,34 @@ def test_pow_ulp(oracle_mod, y):
         assert (got == 1.0).all()
     if y == 1.0:
         assert np.array_equal(got, x)
+
+
+def test_log_exp_ulp(oracle_mod):
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(1e-6, 1, 100000), rng.uniform(1, 1e4, 100000)]).astype(np.float32)
+    got = oracle_mod.fmath("log", x)
+    exact = np.log(x.astype(np.float64))
+    ok = (ulp_err(got, exact) <= 6) | (np.abs(got - exact) <= 3e-7)
+    assert ok.all(), (x[~ok][:3], got[~ok][:3], exact[~ok][:3])
+    v = rng.uniform(-60, 60, 100000).astype(np.float32)
+    got = oracle_mod.fmath("exp", v)
+    # the product x * log2(e) in binary32 scales the error by |x log2 e|
+    amp = 1.0 + np.abs(v.astype(np.float64)) * 1.4427
+    assert (ulp_err(got, np.exp(v.astype(np.float64))) <= 4 * amp).all()
+    assert oracle_mod.fmath("exp", [0.0])[0] == 1.0 and oracle_mod.fmath("log", [1.0])[0] == 0.0
+
+
+def test_atan2(oracle_mod):
+    rng = np.random.default_rng(4)
+    y = np.concatenate([rng.normal(size=200000), rng.uniform(-1e-3, 1e-3, 1000)]).astype(np.float32)
+    x = np.concatenate([rng.normal(size=200000), rng.uniform(-1e-3, 1e-3, 1000)]).astype(np.float32)
+    got = oracle_mod.fmath("atan2", y, x)
+    exact = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    ok = (ulp_err(got, exact) <= 4) | (np.abs(got - exact) <= 2.5e-7)
+    assert ok.all(), (y[~ok][:3], x[~ok][:3], got[~ok][:3], exact[~ok][:3])
+    # IEEE special cases of the signed zeros and the axes
+    zs = np.float32([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 0.0, 0.0])
+    xs = np.float32([0.0, 0.0, -0.0, -0.0, 0.0, 0.0, 1.0, -1.0])
+    want = np.arctan2(zs.astype(np.float64), xs.astype(np.float64)).astype(np.float32)
+    got = oracle_mod.fmath("atan2", zs, xs)
+    assert np.array_equal(np.signbit(got), np.signbit(want)) and np.allclose(got, want, atol=0, rtol=1e-7)
