@@ -141,10 +141,11 @@ void allocate_render_buffers(Ctx& ctx)
     R.bsdf_hits.reserve(16 * n);
     R.cont_rays.reserve(32 * n);
     R.cont_hits.reserve(16 * n);
-    R.q_shadow.reserve(4 * n);
-    R.q_bsdf.reserve(4 * n);
-    R.q_cont.reserve(4 * n);
-    R.q_counts.reserve(64);
+    const size_t qslots = (size_t)SEG_COUNT * seg_capacity((uint32_t)n);
+    R.q_shadow.reserve(4 * qslots);
+    R.q_bsdf.reserve(4 * qslots);
+    R.q_cont.reserve(4 * qslots);
+    R.q_counts.reserve(4 * 3 * (size_t)SEG_COUNTER_WORDS);
     R.fb.reserve(12 * n);
     HIP_CHECK(hipMemset(R.fb.ptr, 0, 12 * n));
     if (R.settings.accumulate_fp64)

@@ -9,6 +9,7 @@
 
 #include "../../include/mipt.h"
 #include "layout.h"
+#include "seglist.h"
 
 namespace mipt {
 
@@ -111,6 +112,7 @@ struct Nrc {
     DeviceBuffer x0, xcm, dycm, dx0, loss, count;
     // records of the path pass
     DeviceBuffer queries, query_pix, query_out, counters, train_queries, train_targets, train_pix, train_rad;
+    DeviceBuffer query_list, train_list, list_counts; // segmented pixel lists of the query kernel
     uint32_t train_capacity = 0;  // training records one path pass can hold
 };
 
@@ -184,12 +186,12 @@ void build_lbvh(Ctx& ctx);
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n,
                    hipStream_t stream);
 void trace_any(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, uint32_t n, hipStream_t stream);
-// Trace the rays whose slots are listed in queue[0, *queue_count) (count read
-// on the device, at most capacity); results go to the same slots.
-void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const uint32_t* queue,
-                         const uint32_t* queue_count, uint32_t capacity, hipStream_t stream);
-void trace_any_queue(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, const uint32_t* queue,
-                     const uint32_t* queue_count, uint32_t capacity, hipStream_t stream);
+// Trace the rays whose slots a segmented queue lists (seglist.h; length read
+// on the device, at most capacity rays); results go to the same slots.
+void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const SegList& queue, uint32_t capacity,
+                         hipStream_t stream);
+void trace_any_queue(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, const SegList& queue, uint32_t capacity,
+                     hipStream_t stream);
 
 // camera.hip
 void camera_rays(Ctx& ctx, const mipt_camera& cam, int32_t w, int32_t h, uint32_t sample_number,

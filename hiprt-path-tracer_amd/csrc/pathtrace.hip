@@ -43,31 +43,20 @@ constexpr uint32_t FLAG_NEE_BSDF = 4u;   // BSDF-to-light ray pending
 constexpr uint32_t FLAG_CONTINUE = 8u;   // continuation ray traced this bounce
 constexpr uint32_t FLAG_DIRECT = 16u;    // direct lighting estimated this bounce
 
-// Slots of the live rays of one bounce, per ray kind; counts[k] is the length
-// of queue k (0 shadow, 1 BSDF-to-light, 2 continuation).
+// Slots of the live rays of one bounce, per ray kind (segmented lists).
 struct RayQueues {
-    uint32_t* shadow;
-    uint32_t* bsdf;
-    uint32_t* cont;
-    uint32_t* counts;
+    SegList shadow;
+    SegList bsdf;
+    SegList cont;
 };
 
-// One slot per lane with pred from a shared counter: one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+// Zeroes the counters of `lists` segmented lists stored back to back.
+__global__ void k_seg_reset(uint32_t* counts, uint32_t lists)
 {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0)
-        return 0u;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0u;
-    if (lane == leader)
-        base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader, 64);
-    return base + rank;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < lists * SEG_COUNT)
+        counts[i * SEG_STRIDE] = 0u;
 }
-
 
 // ---------------------------------------------------------------------------
 // CAMERA_RAYS
@@ -341,21 +330,9 @@ __global__ void k_shade(PtData d, int32_t bounce, RayQueues q)
     bool shadow_active = false, bsdf_active = false, cont = false;
     if (flags & FLAG_ALIVE)
         shade_vertex(d, bounce, pix, flags, shadow_active, bsdf_active, cont);
-    const uint32_t a = wave_append(&q.counts[0], shadow_active);
-    const uint32_t b = wave_append(&q.counts[1], bsdf_active);
-    const uint32_t c = wave_append(&q.counts[2], cont);
-    if (shadow_active)
-        q.shadow[a] = pix;
-    if (bsdf_active)
-        q.bsdf[b] = pix;
-    if (cont)
-        q.cont[c] = pix;
-}
-
-__global__ void k_reset_queues(uint32_t* counts)
-{
-    if (threadIdx.x < 3)
-        counts[threadIdx.x] = 0u;
+    seg_append(q.shadow, shadow_active, pix);
+    seg_append(q.bsdf, bsdf_active, pix);
+    seg_append(q.cont, cont, pix);
 }
 
 __global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev)
@@ -499,6 +476,8 @@ struct CacheView {
     float4* train_rad;        // path radiance before the query vertex
     float4* train_targets;
     uint32_t* counters;       // [0] queries, [1] training records, [2] targets dropped as non-finite
+    SegList query_list;       // pixels of the queries, as the query kernel appends them
+    SegList train_list;       // pixels of the training records
     uint32_t train_every;
     uint32_t train_capacity;
     uint32_t sample_number;
@@ -521,8 +500,8 @@ __global__ void k_cache_counters_reset(uint32_t* c)
         c[threadIdx.x] = 0u;
 }
 
-// At the start of the query bounce: the vertex's cache input (position, view
-// direction, shading normal, roughness, base colour) and the path throughput.
+// At the start of the query bounce: lists the live paths as queries (their
+// path stops here) or training records.
 __global__ void k_cache_query(PtData d, CacheView cv)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -530,40 +509,54 @@ __global__ void k_cache_query(PtData d, CacheView cv)
     const uint32_t flags = valid ? d.flags[pix] : 0u;
     const bool alive = (flags & FLAG_ALIVE) != 0u;
     const bool train = alive && is_training_path(pix, cv.sample_number, cv.train_every);
-    const uint32_t slot_q = wave_append(&cv.counters[0], alive && !train);
-    const uint32_t slot_t = wave_append(&cv.counters[1], train);
-    if (!alive || (train && slot_t >= cv.train_capacity))
-        return;
+    seg_append(cv.query_list, alive && !train, pix);
+    seg_append(cv.train_list, train, pix);
+    if (alive && !train)
+        d.flags[pix] = 0u; // the cache stands in for the rest of the path
+}
+
+// The vertex's cache input (position, view direction, shading normal,
+// roughness, base colour) and the path throughput, as a 16-float record.
+__device__ __forceinline__ void write_cache_record(const PtData& d, uint32_t pix, float4* rec)
+{
     const float4 vp = d.vtx_pos[pix], vs = d.vtx_sn[pix], vg = d.vtx_gn[pix], vv = d.vtx_view[pix];
     const pt::Material& m = d.mats[__float_as_int(vg.w)];
     const float4 thr = d.throughput[pix];
-    const float4 r0 = make_float4(vp.x, vp.y, vp.z, vv.x);
-    const float4 r1 = make_float4(vv.y, vv.z, vs.x, vs.y);
-    const float4 r2 = make_float4(vs.z, m.roughness, m.base_color.r, m.base_color.g);
-    const float4 r3 = make_float4(m.base_color.b, thr.x, thr.y, thr.z);
-    float4* rec = (train ? cv.train_queries + 4 * (size_t)slot_t : cv.queries + 4 * (size_t)slot_q);
-    rec[0] = r0;
-    rec[1] = r1;
-    rec[2] = r2;
-    rec[3] = r3;
-    if (train)
-    {
-        cv.train_pix[slot_t] = pix;
-        cv.train_rad[slot_t] = d.radiance[pix];
-    }
-    else
-    {
-        cv.query_pix[slot_q] = pix;
-        d.flags[pix] = 0u; // the cache stands in for the rest of the path
-    }
+    rec[0] = make_float4(vp.x, vp.y, vp.z, vv.x);
+    rec[1] = make_float4(vv.y, vv.z, vs.x, vs.y);
+    rec[2] = make_float4(vs.z, m.roughness, m.base_color.r, m.base_color.g);
+    rec[3] = make_float4(m.base_color.b, thr.x, thr.y, thr.z);
 }
 
-__global__ void k_cache_clamp_counts(CacheView cv, uint32_t query_capacity)
+// Dense, contiguous query and training records from the two lists (right
+// after k_cache_query, while the path state is that of the query vertex);
+// also stores the record counts the cache kernels read.
+__global__ void k_cache_compact(PtData d, CacheView cv)
 {
-    if (threadIdx.x == 0)
+    SegMap mq, mt;
+    mq.init(cv.query_list.counts, cv.query_list.seg_cap);
+    mt.init(cv.train_list.counts, cv.train_list.seg_cap);
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tq = mq.total, tt = min(mt.total, cv.train_capacity);
+    if (w == 0)
     {
-        cv.counters[0] = min(cv.counters[0], query_capacity);
-        cv.counters[1] = min(cv.counters[1], cv.train_capacity);
+        cv.counters[0] = tq;
+        cv.counters[1] = tt;
+    }
+    const size_t sq = tq ? mq.slot(w < tq ? w : tq - 1u) : 0;
+    const size_t st = tt ? mt.slot(w < tt ? w : tt - 1u) : 0;
+    if (w < tq)
+    {
+        const uint32_t pix = cv.query_list.slots[sq];
+        write_cache_record(d, pix, cv.queries + 4 * (size_t)w);
+        cv.query_pix[w] = pix;
+    }
+    if (w < tt)
+    {
+        const uint32_t pix = cv.train_list.slots[st];
+        write_cache_record(d, pix, cv.train_queries + 4 * (size_t)w);
+        cv.train_pix[w] = pix;
+        cv.train_rad[w] = d.radiance[pix];
     }
 }
 
@@ -622,6 +615,10 @@ CacheView make_cache_view(Ctx& ctx, uint32_t sample_number)
     C.train_rad.reserve(16 * (size_t)cap);
     C.train_targets.reserve(16 * (size_t)cap);
     C.counters.reserve(16);
+    const uint32_t seg_cap = seg_capacity(n);
+    C.query_list.reserve(4 * (size_t)SEG_COUNT * seg_cap);
+    C.train_list.reserve(4 * (size_t)SEG_COUNT * seg_cap);
+    C.list_counts.reserve(4 * 2 * (size_t)SEG_COUNTER_WORDS);
     C.train_capacity = cap;
     CacheView cv{};
     cv.queries = C.queries.as<float4>();
@@ -632,6 +629,8 @@ CacheView make_cache_view(Ctx& ctx, uint32_t sample_number)
     cv.train_rad = C.train_rad.as<float4>();
     cv.train_targets = C.train_targets.as<float4>();
     cv.counters = C.counters.as<uint32_t>();
+    cv.query_list = SegList{C.query_list.as<uint32_t>(), C.list_counts.as<uint32_t>(), seg_cap};
+    cv.train_list = SegList{C.train_list.as<uint32_t>(), C.list_counts.as<uint32_t>() + SEG_COUNTER_WORDS, seg_cap};
     cv.train_every = every;
     cv.train_capacity = cap;
     cv.sample_number = sample_number;
@@ -757,26 +756,31 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
         cv = make_cache_view(ctx, sample_number);
         hipLaunchKernelGGL(k_cache_counters_reset, dim3(1), dim3(64), 0, s, cv.counters);
     }
-    const RayQueues q{R.q_shadow.as<uint32_t>(), R.q_bsdf.as<uint32_t>(), R.q_cont.as<uint32_t>(),
-                      R.q_counts.as<uint32_t>()};
+    const uint32_t seg_cap = seg_capacity(n);
+    uint32_t* qc = R.q_counts.as<uint32_t>();
+    const RayQueues q{SegList{R.q_shadow.as<uint32_t>(), qc, seg_cap},
+                      SegList{R.q_bsdf.as<uint32_t>(), qc + SEG_COUNTER_WORDS, seg_cap},
+                      SegList{R.q_cont.as<uint32_t>(), qc + 2 * SEG_COUNTER_WORDS, seg_cap}};
     hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d);
     for (int32_t bounce = 0; bounce <= R.settings.nb_bounces; bounce++)
     {
         if (cache && bounce == ctx.nrc.settings.query_bounce)
+        {
+            hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(2 * SEG_COUNT), 0, s, cv.query_list.counts, 2u);
             hipLaunchKernelGGL(k_cache_query, dim3(g), dim3(256), 0, s, d, cv);
+            hipLaunchKernelGGL(k_cache_compact, dim3(g), dim3(256), 0, s, d, cv);
+        }
         hipLaunchKernelGGL(k_snapshot_throughput, dim3(g), dim3(256), 0, s, d, R.thr_prev.as<float4>());
-        hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, s, q.counts);
+        hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(3 * SEG_COUNT), 0, s, qc, 3u);
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce, q);
-        trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, q.counts, n, s);
-        trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, q.counts + 1, n, s);
+        trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, n, s);
+        trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, n, s);
         if (bounce < R.settings.nb_bounces)
-            trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, q.counts + 2, n,
-                                s);
+            trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, n, s);
         hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)R.thr_prev.as<float4>());
     }
     if (cache)
     {
-        hipLaunchKernelGGL(k_cache_clamp_counts, dim3(1), dim3(64), 0, s, cv, n);
         nrc_infer(ctx, reinterpret_cast<const float*>(cv.queries), cv.counters, n,
                   reinterpret_cast<float*>(ctx.nrc.query_out.ptr), s);
         hipLaunchKernelGGL(k_cache_resolve, dim3(g), dim3(256), 0, s, d, cv);

@@ -14,9 +14,9 @@
 //     deeper than the LDS stack spill to a per-thread global array;
 //   * a persistent grid sized from occupancy; each wave pulls batches of 64
 //     rays from a global counter so that long rays do not stall a whole block;
-//   * optionally the rays to trace come from a queue of slot indices whose
-//     length is read on the device (the path passes enqueue only live rays);
-//     results go to the rays' own slots.
+//   * optionally the rays to trace come from a segmented queue of slot
+//     indices (seglist.h) whose length is read on the device (the path passes
+//     enqueue only live rays); results go to the rays' own slots.
 //
 // Results are exact: the box test is widened (boxes padded at build time and
 // the slab interval relatively widened by 2^-16) so it never rejects a node
@@ -30,6 +30,7 @@
 
 #include "internal.h"
 #include "pt_math.h"
+#include "seglist.h"
 
 namespace mipt {
 namespace {
@@ -66,13 +67,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
                                                  int32_t* __restrict__ spill, uint32_t spill_stride,
                                                  uint32_t* __restrict__ work_counter, uint32_t max_steps,
                                                  const uint32_t* __restrict__ queue,
-                                                 const uint32_t* __restrict__ queue_count)
+                                                 const uint32_t* __restrict__ queue_counts, uint32_t queue_seg_cap)
 {
     __shared__ int32_t stack[STACK * BLOCK];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    SegMap map{};
     if (queue)
-        n = min(n, *queue_count);
+    {
+        map.init(queue_counts, queue_seg_cap);
+        n = min(n, map.total);
+    }
 
     for (;;)
     {
@@ -82,9 +87,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
         base = __shfl(base, 0, 64);
         if (base >= n)
             break;
-        if (base + lane >= n)
+        const uint32_t w = base + lane;
+        uint32_t r = w;
+        if (queue)
+            r = queue[map.slot(w < n ? w : n - 1u)]; // every lane takes part in the search
+        if (w >= n)
             continue;
-        const uint32_t r = queue ? queue[base + lane] : base + lane;
 
         const float4 ra = rays[2 * (size_t)r];
         const float4 rb = rays[2 * (size_t)r + 1];
@@ -214,7 +222,7 @@ __global__ void k_all_miss(int4* __restrict__ hits, uint8_t* __restrict__ occlud
 }
 
 using TraceFn = void (*)(const BvhNode*, const TriRecord*, int32_t, const float4*, int4*, uint8_t*, uint2*,
-                         uint32_t, int32_t*, uint32_t, uint32_t*, uint32_t, const uint32_t*, const uint32_t*);
+                         uint32_t, int32_t*, uint32_t, uint32_t*, uint32_t, const uint32_t*, const uint32_t*, uint32_t);
 
 template <int BLOCK, int STACK>
 TraceFn pick_mode(int mode)
@@ -284,7 +292,7 @@ Launch plan(Ctx& ctx, int mode, uint32_t n)
 
 // n: rays (or, with a queue, the queue's capacity: slots [0, n) of rays).
 void launch(Ctx& ctx, int mode, const mipt_ray* rays, mipt_hit* hits, uint8_t* occluded, uint32_t* counts, uint32_t n,
-            const uint32_t* queue, const uint32_t* queue_count, hipStream_t stream)
+            const SegList* queue, hipStream_t stream)
 {
     if (n == 0)
         return;
@@ -315,7 +323,8 @@ void launch(Ctx& ctx, int mode, const mipt_ray* rays, mipt_hit* hits, uint8_t* o
     hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(block), 0, stream, ctx.bvh.nodes.as<BvhNode>(),
                        ctx.bvh.tris.as<TriRecord>(), ctx.bvh.root, reinterpret_cast<const float4*>(rays),
                        reinterpret_cast<int4*>(hits), occluded, reinterpret_cast<uint2*>(counts), n, spill, threads,
-                       counter, (uint32_t)ctx.bvh.node_count + 1u, queue, queue_count);
+                       counter, (uint32_t)ctx.bvh.node_count + 1u, queue ? queue->slots : nullptr,
+                       queue ? queue->counts : nullptr, queue ? queue->seg_cap : 0u);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -323,24 +332,24 @@ void launch(Ctx& ctx, int mode, const mipt_ray* rays, mipt_hit* hits, uint8_t* o
 
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n, hipStream_t stream)
 {
-    launch(ctx, counts ? CLOSEST_COUNTED : CLOSEST, rays, hits, nullptr, counts, n, nullptr, nullptr, stream);
+    launch(ctx, counts ? CLOSEST_COUNTED : CLOSEST, rays, hits, nullptr, counts, n, nullptr, stream);
 }
 
 void trace_any(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, uint32_t n, hipStream_t stream)
 {
-    launch(ctx, ANY, rays, nullptr, occluded, nullptr, n, nullptr, nullptr, stream);
+    launch(ctx, ANY, rays, nullptr, occluded, nullptr, n, nullptr, stream);
 }
 
-void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const uint32_t* queue,
-                         const uint32_t* queue_count, uint32_t capacity, hipStream_t stream)
+void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const SegList& queue, uint32_t capacity,
+                         hipStream_t stream)
 {
-    launch(ctx, CLOSEST, rays, hits, nullptr, nullptr, capacity, queue, queue_count, stream);
+    launch(ctx, CLOSEST, rays, hits, nullptr, nullptr, capacity, &queue, stream);
 }
 
-void trace_any_queue(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, const uint32_t* queue,
-                     const uint32_t* queue_count, uint32_t capacity, hipStream_t stream)
+void trace_any_queue(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, const SegList& queue, uint32_t capacity,
+                     hipStream_t stream)
 {
-    launch(ctx, ANY, rays, nullptr, occluded, nullptr, capacity, queue, queue_count, stream);
+    launch(ctx, ANY, rays, nullptr, occluded, nullptr, capacity, &queue, stream);
 }
 
 } // namespace mipt
