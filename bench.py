@@ -52,11 +52,16 @@ def main():
     ap.add_argument("--cache-plain-spp", type=int, default=32, help="plain samples whose time is the budget")
     ap.add_argument("--cache-warmup", type=int, default=8,
                     help="training-only frames before the cache form accumulates (their time counts)")
+    ap.add_argument("--frames", choices=["plain", "cache"], default=None,
+                    help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain or with the "
+                         "radiance cache, and print ms per sample (a kernel-profiling workload)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
+    if args.frames:
+        return frames_only(args.frames, args.steps)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -190,6 +195,40 @@ def main():
         "restir_render": restir,
         "cache_vs_plain": cache,
     }
+    print(json.dumps(out))
+    ctx.close()
+
+
+def frames_only(mode, samples):
+    """Config-1 scene at 1920x1080, 4 bounces, uniform NEE + MIS: 3 warm-up and
+    `samples` timed calls of mipt_render_sample, plain or with the radiance
+    cache (default settings). Prints one JSON line."""
+    import torch
+
+    import mipt
+
+    W, H = 1920, 1080
+    scene = mipt.generate_scene(1, W, H)
+    ctx = mipt.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    cache = mode == "cache"
+    if cache:
+        ctx.set_radiance_cache(mipt.default_radiance_cache_settings())
+    ctx.set_render_settings(mipt.default_render_settings(width=W, height=H, nb_bounces=4, accumulate_fp64=1,
+                                                         radiance_cache=int(cache)))
+    ctx.set_camera(scene.camera)
+    for i in range(3):
+        ctx.render_sample(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(samples):
+        ctx.render_sample(3 + i)
+    torch.cuda.synchronize()
+    out = {"frames": mode, "samples": samples, "ms_per_sample": (time.perf_counter() - t0) / samples * 1e3}
+    if cache:
+        q, t, dropped, loss = ctx.cache_stats()
+        out.update({"queries": q, "training_records": t, "dropped_targets": dropped, "train_loss": loss})
     print(json.dumps(out))
     ctx.close()
 

@@ -105,11 +105,14 @@ __device__ __forceinline__ void normalized_position(const NrcDev& P, const float
 
 // Query record (16 floats): position, direction, normal, roughness, albedo.
 // Encodes the selected rows into x0 rows [0, rows) and zero-fills the rows up
-// to the next multiple of pad (at most n rows in all).
+// to the next multiple of pad (at most n rows in all). Level-major: block row
+// y < levels interpolates grid level y (so one level's table is what the
+// blocks in flight read), block row y == levels writes the other inputs.
 __global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSel sel, uint32_t n, uint32_t pad,
                              uint16_t* __restrict__ x0)
 {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)blockIdx.y;
     const uint32_t rows = sel.rows();
     const uint32_t padded = (rows + pad - 1) / pad * pad;
     if (row >= n || row >= padded)
@@ -117,18 +120,19 @@ __global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSe
     uint16_t* out = x0 + (size_t)row * NRC_WIDTH;
     if (row >= rows)
     {
-        for (int k = 0; k < NRC_WIDTH; k += 8)
-            *reinterpret_cast<uint4*>(out + k) = make_uint4(0u, 0u, 0u, 0u);
+        if (l == P.levels)
+            for (int k = 0; k < NRC_WIDTH; k += 8)
+                *reinterpret_cast<uint4*>(out + k) = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
     const size_t rec = sel.record(row);
-    const float4 a = queries[4 * rec], b = queries[4 * rec + 1], c = queries[4 * rec + 2], d = queries[4 * rec + 3];
-    const float q[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
-    float pos[3];
-    normalized_position(P, q, pos);
-    const uint32_t tmask = (1u << P.log2_table) - 1u;
-    for (int l = 0; l < P.levels; l++)
+    if (l < P.levels)
     {
+        const float4 a = queries[4 * rec];
+        const float q[3] = {a.x, a.y, a.z};
+        float pos[3];
+        normalized_position(P, q, pos);
+        const uint32_t tmask = (1u << P.log2_table) - 1u;
         const NrcGridLevel& L = P.level[l];
         float f0 = 0.0f, f1 = 0.0f;
         for (int cc = 0; cc < 8; cc++)
@@ -137,12 +141,15 @@ __global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSe
             float w;
             corner(L, pos, cc, tmask, idx, w);
             const size_t e = ((size_t)l << P.log2_table) + idx;
-            f0 += w * (float)P.grid_f16[2 * e];
-            f1 += w * (float)P.grid_f16[2 * e + 1];
+            const uint32_t pair = reinterpret_cast<const uint32_t*>(P.grid_f16)[e];
+            f0 += w * (float)__builtin_bit_cast(_Float16, (uint16_t)(pair & 0xffffu));
+            f1 += w * (float)__builtin_bit_cast(_Float16, (uint16_t)(pair >> 16));
         }
-        out[2 * l] = to_bf16(f0);
-        out[2 * l + 1] = to_bf16(f1);
+        *reinterpret_cast<uint32_t*>(out + 2 * l) = (uint32_t)to_bf16(f0) | ((uint32_t)to_bf16(f1) << 16);
+        return;
     }
+    const float4 a = queries[4 * rec], b = queries[4 * rec + 1], c = queries[4 * rec + 2], d = queries[4 * rec + 3];
+    const float q[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
     int k = 2 * P.levels;
     float u, v;
     oct_uv(q[3], q[4], q[5], u, v);
@@ -428,7 +435,6 @@ __global__ __launch_bounds__(256) void k_nrc_weight_grad(NrcTrainDev T, uint32_t
         }
 }
 
-// Grid gradient: scatter the feature gradients to the 8 corners of each level.
 __global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restrict__ queries)
 {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -666,7 +672,7 @@ void nrc_infer(Ctx& ctx, const float* d_queries, const uint32_t* d_count, uint32
         d_count = C.count.as<uint32_t>();
     }
     const NrcDev P = nrc_dev(ctx);
-    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n16, 128)), dim3(128), 0, s, P,
+    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n16, 128), (unsigned)P.levels + 1u), dim3(128), 0, s, P,
                        reinterpret_cast<const float4*>(d_queries), RowSel{d_count, 1u, 0u, n}, n16, 16u,
                        C.x0.as<uint16_t>());
     const unsigned grid = (unsigned)std::min<size_t>(blocks_for((n + 15) / 16, 4), (size_t)ctx.cu_count * 4);
@@ -684,7 +690,7 @@ void nrc_encode(Ctx& ctx, const float* d_queries, uint32_t n, uint16_t* d_out, h
         return;
     C.count.reserve(4);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, C.count.as<uint32_t>(), n);
-    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n, 128)), dim3(128), 0, s, nrc_dev(ctx),
+    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n, 128), (unsigned)C.settings.grid_levels + 1u), dim3(128), 0, s, nrc_dev(ctx),
                        reinterpret_cast<const float4*>(d_queries), RowSel{C.count.as<uint32_t>(), 1u, 0u, n}, n, 1u,
                        d_out);
     HIP_CHECK(hipGetLastError());
@@ -721,7 +727,7 @@ void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, con
     C.last_sel_limit = batch;
     const NrcDev P = nrc_dev(ctx);
     // encode the batch; rows past its count up to a multiple of 32 are zero
-    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(stride, 128)), dim3(128), 0, s, P,
+    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(stride, 128), (unsigned)P.levels + 1u), dim3(128), 0, s, P,
                        reinterpret_cast<const float4*>(d_queries), sel, stride, 32u, C.x0.as<uint16_t>());
     if (clear_loss)
         hipLaunchKernelGGL(k_nrc_zero, dim3(1), dim3(64), 0, s, C.loss.as<float>(), (size_t)1);
