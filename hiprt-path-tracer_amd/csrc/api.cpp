@@ -99,6 +99,13 @@ void prepare_scene_for_render(Ctx& ctx)
                                            ", which the path-tracing passes do not implement yet");
     }
     upload(R.materials, mats.data(), mats.size());
+    R.scene_volumes = R.scene_dispersive = false;
+    for (size_t i = 0; i < ctx.materials_host.size(); i++)
+    {
+        const float* o = mats.data() + 64 * i; // pt::Material: 22 specular / 23 diffuse transmission, 50 dispersion
+        R.scene_volumes |= o[22] > 0.0f || o[23] > 0.0f;
+        R.scene_dispersive |= o[50] > 0.0f && o[22] > 0.0f;
+    }
     std::vector<float> areas(ctx.host_vertices.size() ? ctx.triangle_count : 0);
     for (int32_t t = 0; t < (int32_t)areas.size(); t++)
     {
@@ -115,6 +122,8 @@ void prepare_scene_for_render(Ctx& ctx)
     upload(R.lut_glossy, glossy_dielectric_lut_data(), 128 * 64 * 128);
     upload(R.lut_glass, glass_lut_data(), 256 * 16 * 128);
     upload(R.lut_glass_inv, glass_inv_lut_data(), 256 * 16 * 128);
+    upload(R.lut_thin_glass, thin_glass_lut_data(), 32 * 32 * 96);
+    upload(R.lut_glass_exponent, glass_exponent_data(), 2 * 10 * 11 * 4);
     upload(R.lut_sheen, sheen_ltc_data(), 32 * 32 * 3);
     build_power_alias_table(ctx);
     R.scene_ready = true;
@@ -135,6 +144,8 @@ void allocate_render_buffers(Ctx& ctx)
         b->reserve(16 * n);
     R.rng.reserve(4 * n);
     R.flags.reserve(4 * n);
+    R.vol_a.reserve(16 * n);
+    R.vol_b.reserve(16 * n);
     R.shadow_rays.reserve(32 * n);
     R.shadow_occ.reserve(n);
     R.bsdf_rays.reserve(32 * n);
@@ -145,7 +156,8 @@ void allocate_render_buffers(Ctx& ctx)
     R.q_shadow.reserve(4 * qslots);
     R.q_bsdf.reserve(4 * qslots);
     R.q_cont.reserve(4 * qslots);
-    R.q_counts.reserve(4 * 3 * (size_t)SEG_COUNTER_WORDS);
+    R.q_skip.reserve(4 * 2 * qslots);
+    R.q_counts.reserve(4 * 5 * (size_t)SEG_COUNTER_WORDS);
     R.fb.reserve(12 * n);
     HIP_CHECK(hipMemset(R.fb.ptr, 0, 12 * n));
     if (R.settings.accumulate_fp64)
@@ -592,6 +604,13 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         if (!ctx->render.camera_set)
             mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_launch_pass: call mipt_set_camera first");
         mipt::prepare_scene_for_render(*ctx);
+        // The ReSTIR DI passes rebuild each surface's volume state, which for
+        // a dispersive glass draws a wavelength from the pass's stream at
+        // every surface fetch (ReSTIR/Surface.h:25-43); those draws are not
+        // implemented.
+        if (ctx->render.scene_dispersive && ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
+            mipt::fail(MIPT_ERR_UNSUPPORTED, "ReSTIR DI with a dispersive glass material (dispersion_scale > 0 and "
+                                             "specular_transmission > 0) is not implemented yet");
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
         if (pass == MIPT_PASS_CAMERA_RAYS)
             mipt::camera_rays_pass(*ctx, sample_number, random_number, s);

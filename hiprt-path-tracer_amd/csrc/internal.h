@@ -70,15 +70,18 @@ struct RenderState {
     DeviceBuffer materials;     // pt::Material[material_count]
     DeviceBuffer areas;         // float[T]
     DeviceBuffer lut_conductor; // float[128*128]
-    DeviceBuffer lut_glossy, lut_glass, lut_glass_inv, lut_sheen;
+    DeviceBuffer lut_glossy, lut_glass, lut_glass_inv, lut_thin_glass, lut_glass_exponent, lut_sheen;
     bool scene_ready = false;
+    bool scene_volumes = false;    // some material transmits light
+    bool scene_dispersive = false; // some material is dispersive glass
     // G-buffer (CameraRays output)
     DeviceBuffer gb_prim, gb_pos, gb_gn, gb_sn, gb_mat;
     // path state
     DeviceBuffer throughput, thr_prev, radiance, vtx_pos, vtx_sn, vtx_gn, vtx_view, nee_light, nee_bsdf, rng, flags;
+    DeviceBuffer vol_a, vol_b;    // volume state of the paths that keep one (uint4 each)
     // rays of one bounce, and the queues of live slots the traversal takes
     DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
-    DeviceBuffer q_shadow, q_bsdf, q_cont, q_counts;
+    DeviceBuffer q_shadow, q_bsdf, q_cont, q_skip, q_counts; // q_skip: two lists back to back
     // framebuffer
     DeviceBuffer fb, fb64, counters;
     // power alias table of the emissive triangles (LSS_BASE_POWER)
@@ -157,6 +160,8 @@ const float* conductor_lut_data(); // 128 x 128, luts.cpp
 const float* glossy_dielectric_lut_data(); // [128][64][128]
 const float* glass_lut_data();             // [128][16][256]
 const float* glass_inv_lut_data();         // [128][16][256]
+const float* thin_glass_lut_data();        // [96][32][32]
+const float* glass_exponent_data();        // [2][10][11][4]
 const float* sheen_ltc_data();             // [32][32][3]
 
 // pathtrace.hip

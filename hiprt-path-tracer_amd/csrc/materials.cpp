@@ -31,11 +31,66 @@ void color3(const float in[3], float out[3])
     out[2] = static_cast<float>(b) * INV_255;
 }
 
+float clampf(float lo, float hi, float v) { return std::min(hi, std::max(lo, v)); }
+void clamp3(float c[3], float lo, float hi)
+{
+    for (int k = 0; k < 3; k++)
+        c[k] = clampf(lo, hi, c[k]);
+}
+
+// CPUMaterial::make_safe (src/HostDeviceCommon/Material/MaterialCPU.h:156-212),
+// which the scene parser applies to every material (SceneParser.cpp:437).
+mipt_material make_safe(mipt_material m)
+{
+    clamp3(m.base_color, 0.0f, 1.0f);
+    m.metallic = clampf(0.0f, 1.0f, m.metallic);
+    clamp3(m.metallic_F82, 0.0f, 1.0f);
+    clamp3(m.metallic_F90, 0.0f, 1.0f);
+    m.anisotropy = clampf(0.0f, 1.0f, m.anisotropy);
+    m.anisotropy_rotation = clampf(0.0f, 1.0f, m.anisotropy_rotation);
+    m.second_roughness_weight = clampf(0.0f, 1.0f, m.second_roughness_weight);
+    m.second_roughness = clampf(0.0f, 1.0f, m.second_roughness);
+    m.specular = clampf(0.0f, 1.0f, m.specular);
+    m.specular_tint = clampf(0.0f, 1.0f, m.specular_tint);
+    clamp3(m.specular_color, 0.0f, 1.0f);
+    m.specular_darkening = clampf(0.0f, 1.0f, m.specular_darkening);
+    m.coat = clampf(0.0f, 1.0f, m.coat);
+    clamp3(m.coat_medium_absorption, 0.0f, 1.0f);
+    m.coat_roughness = clampf(1.0e-4f, 1.0f, m.coat_roughness);
+    m.coat_roughening = clampf(0.0f, 1.0f, m.coat_roughening);
+    m.coat_darkening = clampf(0.0f, 1.0f, m.coat_darkening);
+    m.coat_anisotropy = clampf(0.0f, 1.0f, m.coat_anisotropy);
+    m.coat_anisotropy_rotation = clampf(0.0f, 1.0f, m.coat_anisotropy_rotation);
+    m.sheen = clampf(0.0f, 1.0f, m.sheen);
+    m.sheen_roughness = clampf(1.0e-4f, 1.0f, m.sheen_roughness);
+    clamp3(m.sheen_color, 0.0f, 1.0f);
+    m.specular_transmission = clampf(0.0f, 1.0f, m.specular_transmission);
+    m.absorption_at_distance = std::max(m.absorption_at_distance, 1.0e-4f);
+    for (int k = 0; k < 3; k++)
+        m.absorption_color[k] = std::max(m.absorption_color[k], 1.0f / 512.0f);
+    clamp3(m.absorption_color, 0.0f, 1.0f);
+    m.dispersion_abbe_number = std::max(1.0e-5f, m.dispersion_abbe_number);
+    m.dispersion_scale = clampf(0.0f, 1.0f, m.dispersion_scale);
+    m.thin_film = clampf(0.0f, 1.0f, m.thin_film);
+    m.thin_film_hue_shift_degrees = clampf(0.0f, 360.0f, m.thin_film_hue_shift_degrees);
+    m.thin_film_ior = std::max(1.0005f, m.thin_film_ior);
+    m.alpha_opacity = clampf(0.0f, 1.0f, m.alpha_opacity);
+    m.dielectric_priority = std::min(15, std::max(0, m.dielectric_priority));
+    for (int k = 0; k < 3; k++)
+        m.emission[k] = std::max(0.0f, m.emission[k]);
+    // an opaque boundary is never skipped: maximum priority, (1 << 15) - 1,
+    // which the packed material keeps in 8 bits (MaterialPacked.h)
+    if (m.specular_transmission == 0.0f && m.diffuse_transmission == 0.0f)
+        m.dielectric_priority = (1 << 15) - 1;
+    return m;
+}
+
 } // namespace
 
 // Output layout = pt::Material (csrc/shading.h), 64 floats.
-std::string unpack_material(const mipt_material& m, float o[64])
+std::string unpack_material(const mipt_material& in, float o[64])
 {
+    const mipt_material m = make_safe(in);
     std::memset(o, 0, 64 * sizeof(float));
     // emission is stored at full precision, pre-multiplied by its strength
     // and the global factor (MaterialCPU.h:64)
@@ -79,7 +134,7 @@ std::string unpack_material(const mipt_material& m, float o[64])
     color3(m.absorption_color, o + 47);
     o[50] = u8f(q8(m.dispersion_scale));
     o[51] = m.dispersion_abbe_number;
-    const int32_t priority = m.dielectric_priority;
+    const int32_t priority = (uint8_t)m.dielectric_priority;
     std::memcpy(&o[52], &priority, 4);
 
     // Coat roughening of the base and second metal roughness
@@ -102,11 +157,7 @@ std::string unpack_material(const mipt_material& m, float o[64])
         }
     }
 
-    // Lobes the passes implement: coat, sheen, the two GGX metal lobes, the
-    // dielectric specular layer and the Lambertian diffuse lobe. Weight on
-    // another lobe is refused.
-    if (o[22] > 0.0f) return "glass (specular transmission) lobe";
-    if (o[23] > 0.0f) return "diffuse transmission lobe";
+    // Every lobe but the thin-film interference Fresnel is implemented.
     if (o[24] > 0.0f) return "thin-film Fresnel";
     return "";
 }

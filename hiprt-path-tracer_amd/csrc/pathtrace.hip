@@ -42,12 +42,20 @@ constexpr uint32_t FLAG_NEE_LIGHT = 2u;  // shadow ray pending with a contributi
 constexpr uint32_t FLAG_NEE_BSDF = 4u;   // BSDF-to-light ray pending
 constexpr uint32_t FLAG_CONTINUE = 8u;   // continuation ray traced this bounce
 constexpr uint32_t FLAG_DIRECT = 16u;    // direct lighting estimated this bounce
+constexpr uint32_t FLAG_VOL = 32u;       // volume state stored in vol_a / vol_b
+constexpr uint32_t FLAG_SKIP = 64u;      // continuation ray goes on past a skipped boundary
+// The light sample's BSDF evaluation floors the distance in the medium at
+// 1e-6 (Beer-Lambert); the reference evaluates it only when the shadow ray is
+// unoccluded, so the floor is applied after the shadow ray, at the next vertex.
+constexpr uint32_t FLAG_NEE_FLOOR = 128u;
 
-// Slots of the live rays of one bounce, per ray kind (segmented lists).
+// Slots of the live rays of one bounce, per ray kind (segmented lists), and
+// the two lists the boundary-skip rounds alternate between.
 struct RayQueues {
     SegList shadow;
     SegList bsdf;
     SegList cont;
+    SegList skip[2];
 };
 
 // Zeroes the counters of `lists` segmented lists stored back to back.
@@ -147,13 +155,34 @@ __global__ void k_path_init(PtData d)
     d.vtx_sn[pix] = st3(sn, __int_as_float(prim));
     d.vtx_gn[pix] = st3(gn, __int_as_float(d.gb_mat[pix]));
     d.vtx_view[pix] = st3(-1.0f * ray_dir, 0.0f);
-    d.flags[pix] = FLAG_ALIVE;
+    uint32_t flags = FLAG_ALIVE;
+    // RayVolumeState::reconstruct_first_hit (RayVolumeState.h:35-59): the
+    // camera hit's material is pushed on an empty stack (implicit: a path
+    // without FLAG_VOL has exactly that state); a dispersive glass samples
+    // the wavelength with the path's first draw.
+    if (d.scene_dispersive)
+    {
+        const int32_t mi = d.mat_idx[prim];
+        const pt::Material& m = d.mats[mi];
+        if (dispersive(m))
+        {
+            pt::VolumeState vs;
+            vol_canonical_at(vs, mi, m.dielectric_priority);
+            pt::Xorshift32 rng{d.rng[pix]};
+            vs.wavelength = -pt::wavelength_from_draw(rng());
+            d.rng[pix] = rng.state;
+            vol_store(d, pix, vs);
+            flags |= FLAG_VOL;
+        }
+    }
+    d.flags[pix] = flags;
 }
 
 // evaluate_ReSTIR_DI_reservoir (ReSTIR/DI/FinalShading.h:16-101) without the
 // shadow ray: returns the contribution and whether a shadow ray must decide it.
 __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t pix, const pt::Material& m, f3 p, f3 sn,
-                                                      f3 gn, f3 view, bool& need_shadow, f3& dir_out, float& dist_out)
+                                                      f3 gn, f3 view, pt::VolumeState& vs, bool& need_shadow,
+                                                      f3& dir_out, float& dist_out)
 {
     need_shadow = false;
     const float4 ra = d.restir_out[2 * (size_t)pix], rb = d.restir_out[2 * (size_t)pix + 1];
@@ -171,7 +200,7 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
     dist_out = dist;
     if (!(flags & pt::RESTIR_FLAG_UNOCCLUDED) && d.final_shading_visibility)
         need_shadow = true;
-    pt::BsdfCtx c{view, sn, gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::REGULARIZATION_MIS, 0.0f};
+    pt::BsdfCtx c{view, sn, gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::REGULARIZATION_MIS, 0.0f, &vs, 0};
     float bpdf;
     const Color f = pt::principled_eval(d.g, m, c, bpdf);
     float cosine = pt::dot(sn, dir);
@@ -190,13 +219,22 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
 __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, uint32_t pix, uint32_t flags,
                                              bool& shadow_active, bool& bsdf_active, bool& cont)
 {
+    const bool stored_volume = (flags & FLAG_VOL) != 0u;
     flags &= FLAG_ALIVE;
-    const float4 vp = d.vtx_pos[pix], vs = d.vtx_sn[pix], vg = d.vtx_gn[pix];
-    const f3 p = ld3(vp), sn = ld3(vs), gn = ld3(vg), view = ld3(d.vtx_view[pix]);
-    const int32_t prim = __float_as_int(vs.w);
-    const pt::Material m = d.mats[__float_as_int(vg.w)];
+    const float4 vp = d.vtx_pos[pix], vsn = d.vtx_sn[pix], vg = d.vtx_gn[pix];
+    const f3 p = ld3(vp), sn = ld3(vsn), gn = ld3(vg), view = ld3(d.vtx_view[pix]);
+    const int32_t prim = __float_as_int(vsn.w);
+    const int32_t mat_index = __float_as_int(vg.w);
+    const pt::Material m = d.mats[mat_index];
     float path_roughness = vp.w;
     pt::Xorshift32 rng{d.rng[pix]};
+    bool light_floor = false;
+    // the path's volume state after the push of this vertex's material
+    pt::VolumeState vs;
+    if (stored_volume)
+        vol_load(d, pix, vs);
+    else
+        vol_canonical_at(vs, mat_index, m.dielectric_priority);
 
     if (bounce > 0 || d.enable_direct)
     {
@@ -210,7 +248,7 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
         bool need_shadow;
         f3 dir;
         float dist;
-        const Color contrib = restir_final_shading(d, pix, m, p, sn, gn, view, need_shadow, dir, dist);
+        const Color contrib = restir_final_shading(d, pix, m, p, sn, gn, view, vs, need_shadow, dir, dist);
         d.nee_light[pix] = stc(contrib, 0.0f);
         flags |= FLAG_NEE_LIGHT;
         if (need_shadow)
@@ -230,9 +268,16 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
                 const f3 to = ls.point - p;
                 const float dist = sqrtf(pt::dot(to, to));
                 const f3 dirn = pt::vdiv(to, dist);
-                pt::BsdfCtx c{view, sn, gn, dirn, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness};
+                pt::VolumeState vs_light = vs;
+                pt::BsdfCtx c{view, sn, gn, dirn, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness, &vs_light, 0};
                 float bsdf_pdf;
                 const Color f = pt::principled_eval(d.g, m, c, bsdf_pdf);
+                if (vs_light.distance != vs.distance)
+                {
+                    light_floor = true;
+                    write_ray(d.shadow_rays, pix, p, prim, dirn, dist - 1.0e-4f);
+                    shadow_active = true;
+                }
                 if (bsdf_pdf > 0.0f)
                 {
                     const float cos_l = pt::maxf_(0.0f, pt::dot(ls.normal, -1.0f * dirn));
@@ -251,7 +296,8 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
         }
         d.nee_light[pix] = stc(light_mis, 0.0f);
         // ---- BSDF sample for MIS (Lights.h:174-223)
-        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness};
+        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness, &vs,
+                      0};
         f3 dir;
         float bpdf;
         const Color f = pt::principled_sample(d.g, m, c, dir, bpdf, rng);
@@ -269,14 +315,18 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
 
     // ---- continuation (PathTracing.h:26-101)
     {
-        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, path_roughness};
+        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, path_roughness, &vs,
+                      1};
         f3 dir;
         float bpdf;
         const Color f = pt::principled_sample(d.g, m, c, dir, bpdf, rng);
-        // RayPayload::accumulate_roughness (RayPayload.h)
+        // RayPayload::accumulate_roughness (RayPayload.h:52-89)
         if (c.info == pt::FROM_DIFFUSE || c.info == pt::FROM_DIFFUSE_TRANSMISSION)
             path_roughness = 1.0f;
-        else if (c.info == pt::FROM_FIRST_METAL || c.info == pt::FROM_SPECULAR)
+        else if (c.info == pt::FROM_COAT)
+            path_roughness = pt::maxf_(m.coat_roughness, path_roughness);
+        else if (c.info == pt::FROM_FIRST_METAL || c.info == pt::FROM_SPECULAR || c.info == pt::FROM_GLASS_REFLECT ||
+                 c.info == pt::FROM_GLASS_REFRACT)
             path_roughness = pt::maxf_(m.roughness, path_roughness);
         else if (c.info == pt::FROM_SECOND_METAL)
             path_roughness = pt::maxf_(m.second_roughness, path_roughness);
@@ -301,7 +351,7 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
             }
             if (survive)
             {
-                thr = thr * pt::col(1.0f); // dispersion filter (no dispersive material)
+                thr = thr * dispersion_filter(vs.wavelength, m.dispersion_scale);
                 thr = thr * att;
                 thr = pt::col(pt::maxf_(thr.r, 1.0e-5f), pt::maxf_(thr.g, 1.0e-5f), pt::maxf_(thr.b, 1.0e-5f));
                 if (!pt::is_black(thr) && bounce < d.nb_bounces)
@@ -318,6 +368,15 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
     // snapshotted before this kernel.
     d.vtx_pos[pix] = st3(p, path_roughness);
     d.rng[pix] = rng.state;
+    // A reflection pops the vertex's material again: most paths are back to
+    // the canonical state and keep nothing in memory.
+    if (light_floor && !vs.reset)
+        flags |= FLAG_NEE_FLOOR;
+    if (!pt::vol_canonical(vs) || (flags & FLAG_NEE_FLOOR))
+    {
+        vol_store(d, pix, vs);
+        flags |= FLAG_VOL;
+    }
     d.flags[pix] = flags;
 }
 
@@ -335,14 +394,76 @@ __global__ void k_shade(PtData d, int32_t bounce, RayQueues q)
     seg_append(q.cont, cont, pix);
 }
 
-__global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev)
+// The continuation ray's closest hit (trace_main_path_ray, Intersect.h:200-275):
+// the boundary's material is pushed on the interior stack (paths without a
+// stored state rebuild theirs at the vertex); a boundary of lower priority
+// than the medium the ray is in is skipped and the ray goes on from the hit
+// point with the same self-intersection filter (returns true). Hitting a
+// dispersive glass samples the path's wavelength.
+__device__ __forceinline__ bool next_vertex(const PtData& d, uint32_t pix, uint32_t flags)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= (uint32_t)(d.width * d.height))
-        return;
+    const int4 h = d.cont_hits[pix];
+    if (h.x < 0)
+    {
+        d.flags[pix] = 0u; // miss: no envmap contribution
+        return false;
+    }
+    const float4 ra = d.cont_rays[2 * (size_t)pix], rb = d.cont_rays[2 * (size_t)pix + 1];
+    const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
+    const float t = __int_as_float(h.y);
+    f3 p, gn, sn;
+    hit_attributes(d, o, dir, h.x, t, p, gn, sn);
+    const int32_t mi = d.mat_idx[h.x];
+    bool stored = (flags & FLAG_VOL) != 0u;
+    pt::VolumeState vs;
+    if (stored)
+    {
+        vol_load(d, pix, vs);
+        if ((flags & FLAG_NEE_FLOOR) && !d.shadow_occ[pix])
+            vs.distance = pt::maxf_(vs.distance, 1.0e-6f);
+        const bool skip = pt::vol_push(vs, mi, d.mats[mi].dielectric_priority);
+        if (vs.inside)
+            vs.distance += t;
+        if (skip)
+        {
+            vs.distance += t;
+            vol_store(d, pix, vs);
+            write_ray(d.cont_rays, pix, p, __float_as_int(ra.w), dir, INFINITY);
+            d.flags[pix] = FLAG_VOL | FLAG_SKIP;
+            return true;
+        }
+    }
+    if (d.scene_dispersive)
+    {
+        const pt::Material& m = d.mats[mi];
+        if (!stored)
+            vol_canonical_at(vs, mi, m.dielectric_priority);
+        if (dispersive(m) && vs.wavelength == 0.0f)
+        {
+            pt::Xorshift32 rng{d.rng[pix]};
+            vs.wavelength = -pt::wavelength_from_draw(rng());
+            d.rng[pix] = rng.state;
+            stored = true;
+        }
+    }
+    if (stored)
+        vol_store(d, pix, vs);
+    d.vtx_pos[pix] = st3(p, d.vtx_pos[pix].w);
+    d.vtx_sn[pix] = st3(sn, __int_as_float(h.x));
+    d.vtx_gn[pix] = st3(gn, __int_as_float(mi));
+    d.vtx_view[pix] = st3(-1.0f * dir, 0.0f);
+    d.flags[pix] = FLAG_ALIVE | (stored ? FLAG_VOL : 0u);
+    return false;
+}
+
+// Direct lighting of the bounce (Lights.h:364-398) and the next vertex;
+// returns whether the continuation ray skips a boundary.
+__device__ __forceinline__ bool resolve_path(const PtData& d, int32_t bounce, const float4* __restrict__ thr_prev,
+                                             uint32_t pix)
+{
     const uint32_t flags = d.flags[pix];
     if (!(flags & FLAG_ALIVE))
-        return;
+        return false;
     Color radiance = ldc(d.radiance[pix]);
     if (flags & FLAG_DIRECT)
     {
@@ -387,24 +508,39 @@ __global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ t
     if (!(flags & FLAG_CONTINUE))
     {
         d.flags[pix] = 0u;
-        return;
+        return false;
     }
-    // next vertex (trace_main_path_ray, Intersect.h:222-245)
-    const int4 h = d.cont_hits[pix];
-    if (h.x < 0)
+    return next_vertex(d, pix, flags);
+}
+
+__global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev, SegList skip)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pix < (uint32_t)(d.width * d.height);
+    const bool skipping = valid && resolve_path(d, bounce, thr_prev, pix);
+    seg_append(skip, skipping, pix);
+}
+
+// One round of boundary skipping: the paths of `in` have traced their ray on
+// from the skipped boundary; those that skip again go to `out`. The grid
+// covers every pixel; waves past the list's length leave at once.
+__global__ void k_resolve_skip(PtData d, SegList in, SegList out)
+{
+    SegMap map;
+    map.init(in.counts, in.seg_cap);
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = map.total;
+    if ((w & ~63u) >= total)
+        return; // the whole wave: no lane has an entry
+    const size_t slot = map.slot(w < total ? w : total - 1u);
+    uint32_t pix = 0u;
+    bool skipping = false;
+    if (w < total)
     {
-        d.flags[pix] = 0u; // miss: no envmap contribution
-        return;
+        pix = in.slots[slot];
+        skipping = next_vertex(d, pix, d.flags[pix]);
     }
-    const float4 ra = d.cont_rays[2 * (size_t)pix], rb = d.cont_rays[2 * (size_t)pix + 1];
-    const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
-    f3 p, gn, sn;
-    hit_attributes(d, o, dir, h.x, __int_as_float(h.y), p, gn, sn);
-    d.vtx_pos[pix] = st3(p, d.vtx_pos[pix].w);
-    d.vtx_sn[pix] = st3(sn, __int_as_float(h.x));
-    d.vtx_gn[pix] = st3(gn, __int_as_float(d.mat_idx[h.x]));
-    d.vtx_view[pix] = st3(-1.0f * dir, 0.0f);
-    d.flags[pix] = FLAG_ALIVE;
+    seg_append(out, skipping, pix);
 }
 
 __global__ void k_snapshot_throughput(PtData d, float4* __restrict__ out)
@@ -675,13 +811,21 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.rr_clamp = R.settings.russian_roulette_throughput_clamp;
     d.freeze = R.settings.freeze_random;
     d.enable_direct = R.settings.enable_direct;
+    d.scene_volumes = R.scene_volumes;
+    d.scene_dispersive = R.scene_dispersive;
     d.sample_number = sample_number;
     d.random_number = random_number;
     d.g.ggx_conductor_ess = R.lut_conductor.as<float>();
     d.g.glossy_dielectric_ess = R.lut_glossy.as<float>();
     d.g.sheen_ltc = R.lut_sheen.as<float>();
+    d.g.glass_ess = R.lut_glass.as<float>();
+    d.g.glass_inv_ess = R.lut_glass_inv.as<float>();
+    d.g.thin_glass_ess = R.lut_thin_glass.as<float>();
+    d.g.glass_exponent = R.lut_glass_exponent.as<float>();
+    d.g.mats = d.mats;
     d.g.lut_size = 128;
     d.g.consistent_tau = consistent_tau(R.settings.microfacet_regularization_tau_0, sample_number);
+    d.g.consistent_tau_refract = consistent_tau(R.settings.microfacet_regularization_tau_0, sample_number + 1);
     d.g.min_roughness = R.settings.microfacet_regularization_min_roughness;
     d.g.regularize = R.settings.do_microfacet_regularization;
     d.gb_prim = R.gb_prim.as<int32_t>();
@@ -699,6 +843,8 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.nee_bsdf = R.nee_bsdf.as<float4>();
     d.rng = R.rng.as<uint32_t>();
     d.flags = R.flags.as<uint32_t>();
+    d.vol_a = R.vol_a.as<uint4>();
+    d.vol_b = R.vol_b.as<uint4>();
     d.shadow_rays = R.shadow_rays.as<float4>();
     d.shadow_occ = R.shadow_occ.as<uint8_t>();
     d.bsdf_rays = R.bsdf_rays.as<float4>();
@@ -758,9 +904,18 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
     }
     const uint32_t seg_cap = seg_capacity(n);
     uint32_t* qc = R.q_counts.as<uint32_t>();
+    const size_t qn = (size_t)SEG_COUNT * seg_cap;
+    uint32_t* qskip = R.q_skip.as<uint32_t>();
     const RayQueues q{SegList{R.q_shadow.as<uint32_t>(), qc, seg_cap},
                       SegList{R.q_bsdf.as<uint32_t>(), qc + SEG_COUNTER_WORDS, seg_cap},
-                      SegList{R.q_cont.as<uint32_t>(), qc + 2 * SEG_COUNTER_WORDS, seg_cap}};
+                      SegList{R.q_cont.as<uint32_t>(), qc + 2 * SEG_COUNTER_WORDS, seg_cap},
+                      {SegList{qskip, qc + 3 * SEG_COUNTER_WORDS, seg_cap},
+                       SegList{qskip + qn, qc + 4 * SEG_COUNTER_WORDS, seg_cap}}};
+    // A skip needs a medium of higher priority than the boundary hit; only
+    // transmissive materials take priorities below the maximum. Each skip
+    // adds a stack entry without a pop, so at most 2 follow one another on a
+    // 4-entry stack (entry 0 is the air; the first medium sits at 1).
+    const int32_t skip_rounds = d.scene_volumes ? 2 : 0;
     hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d);
     for (int32_t bounce = 0; bounce <= R.settings.nb_bounces; bounce++)
     {
@@ -771,13 +926,22 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
             hipLaunchKernelGGL(k_cache_compact, dim3(g), dim3(256), 0, s, d, cv);
         }
         hipLaunchKernelGGL(k_snapshot_throughput, dim3(g), dim3(256), 0, s, d, R.thr_prev.as<float4>());
-        hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(3 * SEG_COUNT), 0, s, qc, 3u);
+        hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(4 * SEG_COUNT), 0, s, qc, 4u);
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce, q);
         trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, n, s);
         trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, n, s);
         if (bounce < R.settings.nb_bounces)
             trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, n, s);
-        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)R.thr_prev.as<float4>());
+        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)R.thr_prev.as<float4>(),
+                           q.skip[0]);
+        for (int32_t k = 0; k < skip_rounds && bounce < R.settings.nb_bounces; k++)
+        {
+            const SegList& in = q.skip[k & 1];
+            const SegList& out = q.skip[(k + 1) & 1];
+            hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(SEG_COUNT), 0, s, out.counts, 1u);
+            trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), in, n, s);
+            hipLaunchKernelGGL(k_resolve_skip, dim3(g), dim3(256), 0, s, d, in, out);
+        }
     }
     if (cache)
     {

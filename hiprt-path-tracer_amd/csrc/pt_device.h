@@ -37,6 +37,8 @@ struct PtData {
     int32_t do_rr, rr_min_depth;
     float rr_clamp;
     int32_t freeze, enable_direct;
+    int32_t scene_volumes;     // some material transmits light (glass or diffuse transmission)
+    int32_t scene_dispersive;  // some material is dispersive glass
     uint32_t sample_number;
     uint32_t random_number;
     pt::BsdfGlobals g;
@@ -57,6 +59,11 @@ struct PtData {
     float4* nee_bsdf;     // xyz = bsdf_color * cos, w = bsdf pdf
     uint32_t* rng;
     uint32_t* flags;
+    // volume state of the paths whose state is not canonical (FLAG_VOL):
+    // vol_a = stack entries 1-3 + wavelength bits, vol_b = pos | inside << 8,
+    // incident, outgoing, distance bits. Entry 0 is always the air.
+    uint4* vol_a;
+    uint4* vol_b;
     // rays of the current bounce
     float4* shadow_rays;
     uint8_t* shadow_occ;
@@ -143,6 +150,58 @@ __device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, in
     gn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
     sn = gn;
     fix_backfacing(-1.0f * dir, gn, sn);
+}
+
+// Per-path volume state in memory (see PtData::vol_a / vol_b).
+__device__ __forceinline__ void vol_load(const PtData& d, uint32_t pix, pt::VolumeState& v)
+{
+    const uint4 a = d.vol_a[pix], b = d.vol_b[pix];
+    v.entry[0] = pt::vol_entry(pt::VOL_AIR, 0u, true, true);
+    v.entry[1] = a.x;
+    v.entry[2] = a.y;
+    v.entry[3] = a.z;
+    v.wavelength = __uint_as_float(a.w);
+    v.pos = (int32_t)(b.x & 0xffu);
+    v.inside = (int32_t)(b.x >> 8);
+    v.incident = (int32_t)b.y;
+    v.outgoing = (int32_t)b.z;
+    v.distance = __uint_as_float(b.w);
+    v.reset = 0;
+}
+
+__device__ __forceinline__ void vol_store(const PtData& d, uint32_t pix, const pt::VolumeState& v)
+{
+    d.vol_a[pix] = make_uint4(v.entry[1], v.entry[2], v.entry[3], __float_as_uint(v.wavelength));
+    d.vol_b[pix] = make_uint4((uint32_t)v.pos | ((uint32_t)v.inside << 8), (uint32_t)v.incident, (uint32_t)v.outgoing,
+                              __float_as_uint(v.distance));
+}
+
+// The state right after the first push of `mat` on an empty stack: what a
+// path without a stored state has at a vertex of that material.
+__device__ __forceinline__ void vol_canonical_at(pt::VolumeState& v, int32_t mat, int32_t priority)
+{
+    pt::vol_init(v);
+    pt::vol_push(v, mat, priority);
+}
+
+// Dispersive glass: hitting it samples the path's wavelength
+// (Intersect.h:264-272, RayVolumeState.h:50-58).
+__device__ __forceinline__ bool dispersive(const pt::Material& m)
+{
+    return m.dispersion_scale > 0.0f && m.specular_transmission > 0.0f;
+}
+
+// get_dispersion_ray_color (Dispersion.h:512-535): the colour filter of the
+// sampled wavelength, applied once (the wavelength turns positive).
+__device__ __forceinline__ Color dispersion_filter(float& wavelength, float dispersion_scale)
+{
+    if (dispersion_scale == 0.0f)
+        return pt::col(1.0f);
+    if (wavelength >= 0.0f)
+        return pt::col(1.0f);
+    wavelength *= -1.0f;
+    const f3 c = pt::wavelength_to_rgb(wavelength);
+    return pt::col(c.x, c.y, c.z);
 }
 
 // Octahedral24BitNormal::unpack (Packing.h:237-360): decode then normalize.

@@ -74,9 +74,12 @@ PT_HD float cos_fixed(float x)
 }
 
 // log2 of a positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
-// ln m = 2 atanh(t), t = (m - 1) / (m + 1), |t| < 0.172, by its series to t^9.
+// ln m = 2 atanh(t), t = (m - 1) / (m + 1), |t| < 0.172, by its series to t^9;
+// log2(0) = -inf.
 PT_HD float log2_fixed(float x)
 {
+    if (x == 0.0f)
+        return -INFINITY;
     int e;
     float m = frexpf(x, &e);
     if (m < 0x1.6a09e6p-1f)

@@ -192,6 +192,17 @@ struct Surface {
     f3 view, sn, gn, p;
 };
 
+// The surface's volume state (RayVolumeState::reconstruct_first_hit,
+// RayVolumeState.h:35-59): its material pushed on an empty stack. Dispersive
+// glass, which would also draw a wavelength, is refused with ReSTIR DI.
+__device__ __forceinline__ pt::VolumeState surface_volume(const PtData& d, const Surface& s)
+{
+    pt::VolumeState v;
+    pt::vol_init(v);
+    pt::vol_push(v, d.mat_idx[s.prim], d.mats[s.mat].dielectric_priority);
+    return v;
+}
+
 __device__ __forceinline__ Surface surface_current(const PtData& d, uint32_t pix)
 {
     Surface s;
@@ -237,7 +248,8 @@ __device__ __forceinline__ float target_function(const PtData& d, const Surface&
     if (cosine <= 0.0f)
         return 0.0f;
     const pt::Material m = d.mats[s.mat];
-    pt::BsdfCtx c{s.view, s.sn, s.gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::NO_REGULARIZATION, 0.0f};
+    pt::VolumeState vs = surface_volume(d, s);
+    pt::BsdfCtx c{s.view, s.sn, s.gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::NO_REGULARIZATION, 0.0f, &vs, 0};
     float bpdf;
     const Color f = pt::principled_eval(d.g, m, c, bpdf);
     const Color emission = d.mats[d.mat_idx[tri]].emission;
@@ -272,7 +284,8 @@ __device__ __forceinline__ bool next_bsdf_candidate(const PtData& d, const RsDat
     const int32_t nb = r.s.number_of_initial_bsdf_candidates;
     while (cand < nb)
     {
-        pt::BsdfCtx c{s.view, s.sn, s.gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f};
+        pt::VolumeState vs = surface_volume(d, s);
+        pt::BsdfCtx c{s.view, s.sn, s.gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f, &vs, 0};
         f3 dir;
         float bpdf;
         pt::principled_sample(d.g, m, c, dir, bpdf, rng);
@@ -362,7 +375,8 @@ __global__ void k_ic_start(PtData d, RsData r)
         float weight = 0.0f, sample_target = 0.0f;
         if (cosine > 0.0f && pdf > 0.0f)
         {
-            pt::BsdfCtx c{s.view, s.sn, s.gn, dir, pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f};
+            pt::VolumeState vs = surface_volume(d, s);
+            pt::BsdfCtx c{s.view, s.sn, s.gn, dir, pt::NO_INFO, pt::NO_REGULARIZATION, 0.0f, &vs, 0};
             float bsdf_pdf;
             const Color f = pt::principled_eval(d.g, m, c, bsdf_pdf);
             // ReSTIR_DI_evaluate_target_function<false> of this light sample:

@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "pt_math.h"
+#include "volume.h"
 
 namespace pt {
 
@@ -125,8 +126,14 @@ struct BsdfGlobals {
     const float* ggx_conductor_ess; // 128 x 128, row 0 = roughness 1
     const float* glossy_dielectric_ess; // [ior 128][roughness 64][cos 128], rows flipped like the 2D LUT
     const float* sheen_ltc;         // 32 x 32 float3 (Ai, Bi, Ri)
+    const float* glass_ess;         // [F0^(1/4) 128][roughness 16][cos 256], entering the medium
+    const float* glass_inv_ess;     // same, leaving the medium
+    const float* thin_glass_ess;    // [F0^(1/4) 96][roughness 32][cos 32]
+    const float* glass_exponent;    // [lower/higher][IOR knot 10][roughness segment 11][r_max, kind, a, b]
+    const Material* mats;           // media of the nested-dielectrics stack (IOR, absorption)
     int32_t lut_size;
     float consistent_tau;           // MicrofacetRegularization::consistent_tau(tau_0, sample_number)
+    float consistent_tau_refract;   // consistent_tau(tau_0, sample_number + 1)
     float min_roughness;
     int32_t regularize;             // PrincipledBSDFDoMicrofacetRegularization
 };
@@ -395,6 +402,8 @@ struct BsdfCtx {
     uint32_t info;    // LobeInfo of to_light
     uint32_t reg;     // RegMode
     float path_roughness;
+    VolumeState* vs;  // volume state of the path at this vertex (after the push of its material)
+    int32_t update;   // sampling the continuation: pops the stack, resets the distance
 };
 
 // Principled.h:125-153
@@ -717,16 +726,464 @@ PT_HD f3 sheen_sample(const BsdfGlobals& g, const Material& m, f3 lv, Xorshift32
 
 // ---------------------------------------------------------------------------
 // Lobe weights and sampling probabilities
+
+// ---------------------------------------------------------------------------
+// Glass (specular transmission) and diffuse transmission lobes
+// ---------------------------------------------------------------------------
+// Sampling.h:160-171 (no total-internal-reflection check: NaN then).
+PT_HD f3 refract_ray(f3 d, f3 n, float relative_eta)
+{
+    float NoI = dot(d, n);
+    float sin_theta_i2 = 1.0f - NoI * NoI;
+    float root = 1.0f - sin_theta_i2 / (relative_eta * relative_eta);
+    float cos_theta_t = sqrtf(root);
+    return vdiv(-1.0f * d, relative_eta) + (NoI / relative_eta - cos_theta_t) * n;
+}
+
+// MaterialUtils.h:32-57 (Kulla & Conty 2017 remapping of a thin interface).
+PT_HD float thin_walled_roughness(bool thin_walled, float roughness, float relative_eta)
+{
+    if (!thin_walled)
+        return roughness;
+    float sq = (relative_eta - 0.5f) * (relative_eta - 0.5f);
+    float cube = relative_eta * relative_eta * relative_eta;
+    float remapped = roughness * sqrtf(3.7f * (relative_eta - 1.0f) * sq / cube);
+    return clampf_(0.0f, 1.0f, remapped / 1.39f);
+}
+
+// MicrofacetRegularization.h:60-96: tau of the refraction lobe uses the
+// consistent parameterisation of sample_number + 1.
+PT_HD float regularize_refraction(const BsdfGlobals& g, uint32_t mode, float roughness, float path_roughness,
+                                  float eta_i, float eta_t)
+{
+    if (!g.regularize || mode == NO_REGULARIZATION)
+        return roughness;
+    float path_tau = g.consistent_tau_refract / maxf_(path_roughness * path_roughness, 1.0e-8f);
+    float d = eta_i - eta_t, mx = maxf_(eta_i, eta_t);
+    float reg = sqrtf(sqrtf(1.0f / (path_tau * PI_F * (d * d) / (4.0f * (mx * mx)))));
+    return maxf_(g.min_roughness, maxf_(roughness, reg));
+}
+
+// MicrofacetRegularization.h:98-146: 3/4 refraction + 1/4 reflection.
+PT_HD float regularize_mix(const BsdfGlobals& g, uint32_t mode, float roughness, float path_roughness, float eta_i,
+                           float eta_t)
+{
+    if (!g.regularize || mode == NO_REGULARIZATION)
+        return roughness;
+    float path_tau = g.consistent_tau / maxf_(path_roughness * path_roughness, 1.0e-8f);
+    float reflection = sqrtf(sqrtf(1.0f / (path_tau * PI_F)));
+    if (eta_i == eta_t)
+        return reflection;
+    float d = eta_i - eta_t, mx = maxf_(eta_i, eta_t);
+    float refraction = sqrtf(sqrtf(1.0f / (path_tau * PI_F * (d * d) / (4.0f * (mx * mx)))));
+    return maxf_(g.min_roughness, maxf_(roughness, refraction * 0.75f + reflection * 0.25f));
+}
+
+// One IOR knot of the correction exponent: the first roughness segment with
+// roughness <= r_max gives a constant or lerp(a, b, (roughness - r0) / 0.1)
+// with r0 the bound of the segment before.
+PT_HD float glass_exponent_knot(const float* seg, float roughness)
+{
+    for (int k = 0; k < 11; k++)
+    {
+        if (roughness <= seg[4 * k])
+        {
+            if (seg[4 * k + 1] == 0.0f)
+                return seg[4 * k + 2];
+            float r0 = seg[4 * (k - 1)];
+            return lerpf_(seg[4 * k + 2], seg[4 * k + 3], (roughness - r0) / 0.1f);
+        }
+    }
+    return 2.5f;
+}
+
+// GGX_glass_energy_compensation_get_correction_exponent
+// (MicrofacetEnergyCompensation.h:101-631): the exponent of the cos_theta
+// axis of the glass LUTs, interpolated between the two IOR knots around
+// relative_eta.
+PT_HD float glass_correction_exponent(const float* tab, float roughness, float relative_eta)
+{
+    if ((roughness < 1.0e-10f && roughness > -1.0e-10f) || fabsf(1.0f - relative_eta) < 1.0e-3f)
+        return 2.5f;
+    const float knots[10] = {1.01f, 1.02f, 1.03f, 1.1f, 1.2f, 1.4f, 1.5f, 2.0f, 2.4f, 3.0f};
+    float lower_bound = 1.01f, lower = 2.5f;
+    for (int k = 0; k < 9; k++)
+    {
+        if (relative_eta > knots[k] && relative_eta <= knots[k + 1])
+        {
+            lower_bound = knots[k];
+            lower = glass_exponent_knot(tab + 44 * k, roughness);
+            break;
+        }
+    }
+    int hk = 9;
+    for (int k = 0; k < 9; k++)
+    {
+        if (relative_eta <= knots[k])
+        {
+            hk = k;
+            break;
+        }
+    }
+    float higher_bound = knots[hk];
+    float higher = glass_exponent_knot(tab + 44 * (10 + hk), roughness);
+    if (higher_bound == lower_bound)
+        return lower;
+    return lerpf_(lower, higher, (relative_eta - lower_bound) / (higher_bound - lower_bound));
+}
+
+// get_GGX_energy_compensation_dielectrics (MicrofacetEnergyCompensation.h:633-703),
+// roughness threshold 0 and no bounce limit (BSDFsData.h:63,71); thin-film
+// materials are refused, so the thin-film lerp is the identity.
+PT_HD float glass_energy_compensation(const BsdfGlobals& g, const Material& m, float roughness, bool inside,
+                                      float eta_t, float eta_i, float relative_eta, float NoV)
+{
+    bool smooth_enough = roughness <= 0.0f;
+    if (!(m.flags & MAT_GLASS_EC) || smooth_enough)
+        return 1.0f;
+    float compensation = 1.0f;
+    if (m.thin_film < 1.0f)
+    {
+        const bool thin = (m.flags & MAT_THIN_WALLED) != 0;
+        float relative_eta_for_correction = inside ? 1.0f / relative_eta : relative_eta;
+        float exponent = 2.5f;
+        if (!thin)
+            exponent = glass_correction_exponent(g.glass_exponent, roughness, relative_eta_for_correction);
+        float view = pow_fixed(maxf_(1.0e-3f, NoV), 1.0f / exponent);
+        float F0 = F0_from_eta(eta_t, eta_i);
+        float F0_remapped = sqrtf(sqrtf(F0));
+        if (thin)
+            compensation = lut3d_fetch(g.thin_glass_ess, 32, 32, 96, view, roughness, F0_remapped);
+        else
+            compensation = lut3d_fetch(inside ? g.glass_inv_ess : g.glass_ess, 256, 16, 128, view, roughness,
+                                       F0_remapped);
+        compensation = lerpf_(compensation, 1.0f, m.thin_film * roughness);
+    }
+    return compensation;
+}
+
+// principled_beer_absorption (Principled.h:332-357): absorption of the medium
+// the ray travelled in (the incident one), with the distance floored at 1e-6
+// in the volume state itself.
+PT_HD Color beer_absorption(const BsdfGlobals& g, VolumeState& v)
+{
+    const Material& medium = g.mats[v.incident];
+    const Color a = medium.absorption_color;
+    if (!is_white(a))
+    {
+        v.distance = maxf_(v.distance, 1.0e-6f);
+        Color k = col(log_fixed(a.r), log_fixed(a.g), log_fixed(a.b)) / medium.absorption_at_distance;
+        Color e = k * v.distance;
+        return col(exp_fixed(e.r), exp_fixed(e.g), exp_fixed(e.b));
+    }
+    return col(1.0f);
+}
+
+// Walter et al. 2007 refraction lobe (Microfacet.h:288-352).
+PT_HD Color ggx_eval_refract(const Material& m, float roughness, float relative_eta, Color F, f3 lv, f3 ll, f3 lh,
+                             float& pdf, uint32_t info)
+{
+    float NoL = ll.z, NoV = lv.z;
+    float HoL = dot(ll, lh), HoV = dot(lv, lh);
+    Color color;
+    if (is_perfectly_smooth(roughness))
+    {
+        bool perfect = dot(refract_ray(lv, mk(0.0f, 0.0f, 1.0f), relative_eta), ll) > DELTA_ALIGN_THRESHOLD;
+        if (info == FROM_GLASS_REFRACT && perfect)
+        {
+            color = col(DELTA_HIGH_VALUE) * (col(1.0f) - F) * m.base_color;
+            color = color / fabsf(NoL);
+            pdf = DELTA_HIGH_VALUE;
+        }
+        else
+        {
+            color = col(0.0f);
+            pdf = 0.0f;
+        }
+    }
+    else
+    {
+        float dot_prod = HoL + HoV / relative_eta;
+        float dot_prod2 = dot_prod * dot_prod;
+        float denom = dot_prod2 * NoL * NoV;
+        float ax, ay;
+        get_alphas(roughness, m.anisotropy, ax, ay);
+        float D = ggx_d(ax, ay, lh);
+        float G1_V = 1.0f / (1.0f + smith_lambda(ax, ay, lv));
+        float G1_L = 1.0f / (1.0f + smith_lambda(ax, ay, ll));
+        float G2 = G1_V * G1_L;
+        float dwm_dwi = fabsf(HoL) / dot_prod2;
+        float D_pdf = G1_V / fabsf(NoV) * D * fabsf(HoV);
+        pdf = dwm_dwi * D_pdf;
+        color = m.base_color * D * (col(1.0f) - F) * G2 * fabsf(HoL * HoV / denom);
+    }
+    color = color / (relative_eta * relative_eta);
+    return color;
+}
+
+PT_HD float ggx_pdf_refract(const Material& m, float roughness, float relative_eta, f3 lv, f3 ll, f3 lh, uint32_t info)
+{
+    float NoV = lv.z;
+    float HoL = dot(ll, lh), HoV = dot(lv, lh);
+    if (is_perfectly_smooth(roughness))
+    {
+        bool perfect = dot(refract_ray(lv, mk(0.0f, 0.0f, 1.0f), relative_eta), ll) > DELTA_ALIGN_THRESHOLD;
+        return info == FROM_GLASS_REFRACT && perfect ? DELTA_HIGH_VALUE : 0.0f;
+    }
+    float dot_prod = HoL + HoV / relative_eta;
+    float dot_prod2 = dot_prod * dot_prod;
+    float ax, ay;
+    get_alphas(roughness, m.anisotropy, ax, ay);
+    float D = ggx_d(ax, ay, lh);
+    float G1_V = 1.0f / (1.0f + smith_lambda(ax, ay, lv));
+    float dwm_dwi = fabsf(HoL) / dot_prod2;
+    float D_pdf = G1_V / fabsf(NoV) * D * fabsf(HoV);
+    return dwm_dwi * D_pdf;
+}
+
+// IORs on both sides of the interface (Principled.h:374-380): the materials
+// of the incident / outgoing media (1 for the air) at the sampled wavelength.
+PT_HD float medium_ior(const BsdfGlobals& g, int32_t medium)
+{
+    return (uint32_t)medium == VOL_AIR ? 1.0f : g.mats[medium].ior;
+}
+
+PT_HD void glass_etas(const BsdfGlobals& g, const Material& m, const VolumeState& v, float& eta_i, float& eta_t,
+                      float& relative_eta)
+{
+    eta_i = medium_ior(g, v.incident);
+    eta_t = medium_ior(g, v.outgoing);
+    eta_i = dispersion_ior(m.dispersion_abbe_number, m.dispersion_scale, eta_i, fabsf(v.wavelength));
+    eta_t = dispersion_ior(m.dispersion_abbe_number, m.dispersion_scale, eta_t, fabsf(v.wavelength));
+    relative_eta = eta_t / eta_i;
+    // a relative IOR of exactly 1 with opposite view and light gives a null half vector
+    if (fabsf(relative_eta - 1.0f) < 1.0e-5f)
+        relative_eta = 1.0f + 1.0e-5f;
+}
+
+// Fresnel reflectance of the glass interface and the probability of sampling
+// the reflection (its luminance). Thin-film materials are refused, so
+// lerp(F, F_thin_film, 0) is F. A thin interface adds the inter-reflections
+// between its two sides (PBR book 4th ed., dielectric BSDF).
+PT_HD float glass_fresnel(bool thin_walled, float scaled_roughness, float relative_eta, float HoV, float& reflect_proba)
+{
+    float F = fresnel_dielectric(HoV, relative_eta);
+    if (thin_walled && F < 1.0f && scaled_roughness < 0.1f)
+        F = F + (1.0f - F) * (1.0f - F) * F / (1.0f - F * F);
+    reflect_proba = luminance(col(F));
+    return F;
+}
+
+// Generalised half vector of the glass lobe (Principled.h:410-443), flipped
+// into the upper hemisphere.
+PT_HD f3 glass_half_vector(const BsdfGlobals& g, bool reflecting, bool thin_walled, float scaled_roughness,
+                           float relative_eta, f3 lv, f3 ll)
+{
+    f3 lh;
+    if (scaled_roughness <= ROUGHNESS_CLAMP && !g.regularize)
+        lh = mk(0.0f, 0.0f, 1.0f);
+    else
+    {
+        if (reflecting)
+            lh = ll + lv;
+        else if (thin_walled)
+            lh = mk(ll.x, ll.y, -ll.z) + lv;
+        else
+            lh = mk(ll.x * relative_eta, ll.y * relative_eta, ll.z * relative_eta) + lv;
+        lh = normalize(lh);
+    }
+    if (lh.z < 0.0f)
+        lh = -1.0f * lh;
+    return lh;
+}
+
+// principled_glass_eval (Principled.h:359-558), in the rotated frame.
+PT_HD Color glass_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, f3 lv, f3 ll, float& pdf)
+{
+    pdf = 0.0f;
+    float NoV = lv.z, NoL = ll.z;
+    if (fabsf(NoL) < 1.0e-8f)
+        return col(0.0f);
+    bool reflecting = NoL * NoV > 0.0f;
+    VolumeState& v = *c.vs;
+    float eta_i, eta_t, relative_eta;
+    glass_etas(g, m, v, eta_i, eta_t, relative_eta);
+    const bool thin = (m.flags & MAT_THIN_WALLED) != 0;
+    float scaled = thin_walled_roughness(thin, m.roughness, relative_eta);
+    f3 lh = glass_half_vector(g, reflecting, thin, scaled, relative_eta, lv, ll);
+    float HoL = dot(ll, lh), HoV = dot(lv, lh);
+    if (HoL * NoL < 0.0f || HoV * NoV < 0.0f)
+        return col(0.0f);
+    float reflect_proba;
+    float F = glass_fresnel(thin, scaled, relative_eta, HoV, reflect_proba);
+    Color color;
+    if (reflecting)
+    {
+        float rr = scaled;
+        if (g.regularize && c.reg == REGULARIZATION_MIS)
+            rr = regularize_mix(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+        else if (g.regularize && c.reg == REGULARIZATION_CLASSIC)
+            rr = regularize_reflection(g, c.reg, scaled, c.path_roughness);
+        DeltaSampled ds = delta_reflection_sampled(m, scaled, m.anisotropy, c.info);
+        color = ggx_eval_reflect_ss(rr, m.anisotropy, col(F), lv, ll, lh, pdf, ds);
+        float compensation = glass_energy_compensation(g, m, m.roughness, v.inside != 0, eta_t, eta_i, relative_eta,
+                                                       lv.z);
+        color = color / compensation;
+        pdf *= reflect_proba;
+    }
+    else
+    {
+        float rr = scaled;
+        if (g.regularize && c.reg == REGULARIZATION_MIS)
+            rr = regularize_mix(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+        else if (g.regularize && c.reg == REGULARIZATION_CLASSIC)
+            rr = regularize_refraction(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+        color = ggx_eval_refract(m, rr, relative_eta, col(F), lv, ll, lh, pdf, c.info);
+        pdf *= 1.0f - reflect_proba;
+        float compensation = glass_energy_compensation(g, m, rr, v.inside != 0, eta_t, eta_i, relative_eta, lv.z);
+        color = color / compensation;
+        if (thin)
+            color = color * m.base_color; // both sides of the thin interface at once
+        if (thin && c.update)
+            vol_pop(v, v.inside != 0);
+        else if ((uint32_t)v.incident != VOL_AIR)
+        {
+            color = color * beer_absorption(g, v);
+            if (c.update)
+            {
+                v.distance = 0.0f;
+                v.reset = 1;
+                if (v.inside)
+                    vol_pop(v, true);
+            }
+        }
+    }
+    return color;
+}
+
+// principled_glass_pdf (Principled.h:560-724)
+PT_HD float glass_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, f3 lv, f3 ll)
+{
+    float NoV = lv.z, NoL = ll.z;
+    if (fabsf(NoL) < 1.0e-8f)
+        return 0.0f;
+    bool reflecting = NoL * NoV > 0.0f;
+    const VolumeState& v = *c.vs;
+    float eta_i, eta_t, relative_eta;
+    glass_etas(g, m, v, eta_i, eta_t, relative_eta);
+    const bool thin = (m.flags & MAT_THIN_WALLED) != 0;
+    float scaled = thin_walled_roughness(thin, m.roughness, relative_eta);
+    f3 lh = glass_half_vector(g, reflecting, thin, scaled, relative_eta, lv, ll);
+    float HoL = dot(ll, lh), HoV = dot(lv, lh);
+    if (HoL * NoL < 0.0f || HoV * NoV < 0.0f)
+        return 0.0f;
+    float reflect_proba;
+    glass_fresnel(thin, scaled, relative_eta, HoV, reflect_proba);
+    float pdf;
+    if (reflecting)
+    {
+        float rr = scaled;
+        if (g.regularize && c.reg == REGULARIZATION_MIS)
+            rr = regularize_mix(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+        else if (g.regularize && c.reg == REGULARIZATION_CLASSIC)
+            rr = regularize_reflection(g, c.reg, scaled, c.path_roughness);
+        DeltaSampled ds = delta_reflection_sampled(m, scaled, m.anisotropy, c.info);
+        pdf = ggx_pdf_reflect(rr, m.anisotropy, lv, ll, lh, ds);
+        pdf *= reflect_proba;
+    }
+    else
+    {
+        float rr = scaled;
+        if (g.regularize && c.reg == REGULARIZATION_MIS)
+            rr = regularize_mix(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+        else if (g.regularize && c.reg == REGULARIZATION_CLASSIC)
+            rr = regularize_refraction(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+        pdf = ggx_pdf_refract(m, rr, relative_eta, lv, ll, lh, c.info);
+        pdf *= 1.0f - reflect_proba;
+    }
+    return pdf;
+}
+
+// principled_glass_sample (Principled.h:726-830): a microfacet normal from the
+// visible normals, then reflection with the Fresnel probability, else
+// refraction (straight through a thin interface).
+PT_HD f3 glass_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& c, f3 lv, Xorshift32& rng)
+{
+    VolumeState& v = *c.vs;
+    float eta_i, eta_t, relative_eta;
+    glass_etas(g, m, v, eta_i, eta_t, relative_eta);
+    const bool thin = (m.flags & MAT_THIN_WALLED) != 0;
+    float scaled = thin_walled_roughness(thin, m.roughness, relative_eta);
+    if (g.regularize && c.reg == REGULARIZATION_MIS)
+        scaled = regularize_mix(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
+    float ax, ay;
+    get_alphas(scaled, m.anisotropy, ax, ay);
+    f3 mn = (ax <= ROUGHNESS_CLAMP && ay <= ROUGHNESS_CLAMP) ? mk(0.0f, 0.0f, 1.0f) : ggx_vndf_sample(lv, ax, ay, rng);
+    float HoV = dot(lv, mn);
+    float reflect_proba;
+    glass_fresnel(thin, scaled, relative_eta, HoV, reflect_proba);
+    float rand_1 = rng();
+    if (rand_1 < reflect_proba)
+    {
+        c.info = FROM_GLASS_REFLECT;
+        if (c.update)
+            vol_pop(v, false);
+        return reflect_ray(lv, mn);
+    }
+    c.info = FROM_GLASS_REFRACT;
+    if (dot(mn, lv) < 0.0f)
+        mn = -1.0f * mn;
+    if (thin)
+    {
+        f3 reflected = reflect_ray(lv, mn);
+        reflected.z *= -1.0f;
+        if (c.update)
+            vol_pop(v, false);
+        return reflected;
+    }
+    return refract_ray(lv, mn, relative_eta);
+}
+
+// principled_diffuse_transmission_eval / _pdf (Principled.h:832-873).
+PT_HD Color dtrans_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, f3 lv, f3 ll, float& pdf)
+{
+    pdf = 0.0f;
+    if (lv.z * ll.z > 0.0f)
+        return col(0.0f);
+    Color color = m.base_color * INV_PI_F;
+    VolumeState& v = *c.vs;
+    if ((uint32_t)v.incident != VOL_AIR)
+    {
+        color = color * beer_absorption(g, v);
+        if (c.update)
+        {
+            v.distance = 0.0f;
+            v.reset = 1;
+            if (v.inside)
+                vol_pop(v, true);
+        }
+    }
+    pdf = fabsf(ll.z * INV_PI_F);
+    return color;
+}
+
+PT_HD float dtrans_pdf(f3 lv, f3 ll)
+{
+    if (lv.z * ll.z > 0.0f)
+        return 0.0f;
+    return fabsf(ll.z * INV_PI_F);
+}
+
 // ---------------------------------------------------------------------------
 struct LobeWeights {
     float coat, sheen, metal1, metal2, specular, diffuse, glass, dtrans;
 };
 
-// Principled.h:1515-1554; outside the object (the glass lobe, which lets a
-// path enter one, is refused at upload).
-PT_HD LobeWeights lobe_weights(const Material& m)
+// Principled.h:1515-1554. Inside an object only the two transmission lobes
+// remain (the path is leaving the medium through its boundary).
+PT_HD LobeWeights lobe_weights(const Material& m, bool outside_object)
 {
-    const float outside = 1.0f;
+    const float outside = outside_object ? 1.0f : 0.0f;
     LobeWeights w;
     w.coat = m.coat * outside;
     w.sheen = m.sheen * outside;
@@ -735,11 +1192,11 @@ PT_HD LobeWeights lobe_weights(const Material& m)
     w.metal2 = metallic * outside;
     w.metal1 = lerpf_(w.metal1, 0.0f, m.second_roughness_weight);
     w.metal2 = lerpf_(0.0f, w.metal2, m.second_roughness_weight);
-    w.glass = (1.0f - metallic) * (1.0f - m.diffuse_transmission) * m.specular_transmission;
-    w.dtrans = (1.0f - metallic) * m.diffuse_transmission;
-    w.specular = (1.0f - metallic) * (1.0f - m.specular_transmission * (1.0f - m.diffuse_transmission)) * m.specular *
-                 outside;
-    w.diffuse = (1.0f - metallic) * (1.0f - m.specular_transmission) * (1.0f - m.diffuse_transmission) * outside;
+    float st = m.specular_transmission, dt = m.diffuse_transmission;
+    w.glass = !outside_object ? (1.0f - dt) : (1.0f - metallic) * (1.0f - dt) * st;
+    w.dtrans = !outside_object ? dt : (1.0f - metallic) * dt;
+    w.specular = (1.0f - metallic) * (1.0f - st * (1.0f - dt)) * m.specular * outside;
+    w.diffuse = (1.0f - metallic) * (1.0f - st) * (1.0f - dt) * outside;
     return w;
 }
 
@@ -952,17 +1409,18 @@ PT_HD float pdf_diffuse_layer(const LocalFrames& f, float weight, float proba)
 }
 
 // ---------------------------------------------------------------------------
-// Principled BSDF (Principled.h:1620-1939). The glass and diffuse
-// transmission lobes are refused at upload, so their weights are 0 here and
-// their layers contribute nothing.
+// Principled BSDF (Principled.h:1620-1939)
 // ---------------------------------------------------------------------------
+// Principled.h:1556-1618 and :1620-1715
 PT_HD Color principled_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, float& pdf)
 {
     pdf = 0.0f;
-    const float incident_ior = 1.0f; // air: no nested dielectrics without the glass lobe
-    bool refracting = dot(c.sn, c.to_light) < 0.0f;
+    const VolumeState& v = *c.vs;
+    bool outside = !v.inside;
+    bool refracting = dot(c.sn, c.to_light) < 0.0f && outside;
     LocalFrames f = local_frames(m, c);
-    LobeWeights w = lobe_weights(m);
+    const float incident_ior = medium_ior(g, v.incident);
+    LobeWeights w = lobe_weights(m, outside);
     LobeWeights p = lobe_probas(w);
     Color layers = col(1.0f);
     Color final_color = col(0.0f);
@@ -972,40 +1430,70 @@ PT_HD Color principled_eval(const BsdfGlobals& g, const Material& m, const BsdfC
                                                  layers, pdf);
     final_color = final_color + eval_metal_layer(g, m, c, f, m.second_roughness, w.metal2 * (float)!refracting,
                                                  p.metal2, layers, pdf);
+    // glass layer (Principled.h:1177-1200), rotated frame
+    if (w.glass > 0.0f)
+    {
+        float gpdf = 0.0f;
+        Color contribution = glass_eval(g, m, c, f.lvr, f.llr, gpdf);
+        contribution = contribution * w.glass;
+        contribution = contribution * layers;
+        pdf += gpdf * p.glass;
+        final_color = final_color + contribution;
+    }
     // glossy base: specular layer over the diffuse layer (Principled.h:1474-1492)
     Color glossy = col(0.0f);
     glossy = glossy + eval_specular_layer(g, m, c, f, incident_ior, w.specular, refracting, p.specular, layers, pdf);
     glossy = glossy + eval_diffuse_layer(m, f, w.diffuse * (float)!refracting, p.diffuse, layers, pdf);
     final_color = final_color + glossy / energy_compensation_glossy_base(g, m, incident_ior, f.lv.z);
+    // diffuse transmission layer (Principled.h:1219-1241)
+    if (w.dtrans > 0.0f && f.ll.z < 0.0f)
+    {
+        float dpdf = 0.0f;
+        Color contribution = dtrans_eval(g, m, c, f.lv, f.ll, dpdf);
+        contribution = contribution * w.dtrans;
+        contribution = contribution * layers;
+        pdf += dpdf * p.dtrans;
+        final_color = final_color + contribution;
+    }
     return final_color / energy_compensation_clearcoat(g, m, incident_ior, f.lv.z);
 }
 
+// Principled.h:1717-1800
 PT_HD float principled_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx& c)
 {
     float pdf = 0.0f;
-    bool refracting = dot(c.sn, c.to_light) < 0.0f;
+    const VolumeState& v = *c.vs;
+    bool outside = !v.inside;
+    bool refracting = dot(c.sn, c.to_light) < 0.0f && outside;
     LocalFrames f = local_frames(m, c);
-    LobeWeights w = lobe_weights(m);
+    LobeWeights w = lobe_weights(m, outside);
     LobeWeights p = lobe_probas(w);
     pdf += pdf_coat_layer(g, m, c, f, refracting, w.coat, p.coat);
     pdf += pdf_sheen_layer(g, m, f, refracting, w.sheen, p.sheen);
     pdf += pdf_metal_layer(g, m, c, f, m.roughness, w.metal1 * (float)!refracting, p.metal1);
     pdf += pdf_metal_layer(g, m, c, f, m.second_roughness, w.metal2 * (float)!refracting, p.metal2);
+    if (w.glass > 0.0f)
+        pdf += glass_pdf(g, m, c, f.lvr, f.llr) * p.glass;
     float glossy = 0.0f;
     glossy += pdf_specular_layer(g, m, c, f, w.specular, refracting, p.specular);
     glossy += pdf_diffuse_layer(f, w.diffuse * (float)!refracting, p.diffuse);
     pdf += glossy;
+    if (w.dtrans > 0.0f && f.ll.z < 0.0f)
+        pdf += dtrans_pdf(f.lv, f.ll) * p.dtrans;
     return pdf;
 }
 
 // Principled.h:1802-1939. Returns the BSDF value of the sampled direction;
 // pdf = 0 when the sample is invalid. `c.info` receives the sampled lobe
-// (the sheen lobe leaves it as it was).
+// (the sheen lobe leaves it as it was). With c.update, a reflection pops the
+// material of the vertex from the interior stack (the path stays outside).
 PT_HD Color principled_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& c, f3& out_dir, float& pdf,
                               Xorshift32& rng)
 {
     pdf = 0.0f;
-    LobeWeights w = lobe_weights(m);
+    VolumeState& v = *c.vs;
+    bool outside = !v.inside;
+    LobeWeights w = lobe_weights(m, outside);
     LobeWeights p = lobe_probas(w);
     float cdf0 = p.coat;
     float cdf1 = cdf0 + p.sheen;
@@ -1017,6 +1505,8 @@ PT_HD Color principled_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& 
     float rand_1 = rng();
     bool sampling_dt = rand_1 > cdf5 && rand_1 < cdf6;
     bool sampling_glass = rand_1 > cdf6;
+    if (c.update && !sampling_glass && !sampling_dt)
+        vol_pop(v, false);
     f3 TR, BR;
     build_rotated_onb(c.sn, TR, BR, m.anisotropy_rotation * PI_F);
     f3 lvr = world_to_local(TR, BR, c.sn, c.view);
@@ -1059,9 +1549,13 @@ PT_HD Color principled_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& 
         c.info = FROM_DIFFUSE;
         out_dir = cosine_sample_around_normal(c.sn, rng);
     }
+    else if (rand_1 < cdf6)
+    {
+        c.info = FROM_DIFFUSE_TRANSMISSION;
+        out_dir = cosine_sample_around_normal(-1.0f * c.sn, rng);
+    }
     else
-        // diffuse transmission and glass carry zero probability (refused at upload)
-        return col(0.0f);
+        out_dir = local_to_world(TR, BR, c.sn, glass_sample(g, m, c, lvr, rng));
     if (dot(out_dir, c.gn) < 0.0f && !sampling_glass && !sampling_dt)
         return col(0.0f);
     c.to_light = out_dir;

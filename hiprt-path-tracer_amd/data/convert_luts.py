@@ -42,9 +42,19 @@ LUTS_3D = [
      "GGX_Glass_Ess_Uncorrelated_256x16x128.hdr", 128),
     ("ggx_glass_inv_uncorrelated_256x16x128.f32", "data/BRDFsData/GGX/Glass",
      "inv_GGX_Glass_Ess_Uncorrelated_256x16x128.hdr", 128),
+    ("ggx_thin_glass_uncorrelated_32x32x96.f32", "data/BRDFsData/GGX/Glass",
+     "GGX_Thin_Glass_Ess_Uncorrelated_32x32x96.hdr", 96),
 ]
 
 SHEEN_TABLE = ("sheen_ltc_32x32.f32", "src/Device/includes/BSDFs/SheenLTCFittedParameters.h")
+
+# Glass energy-compensation correction exponent: a piecewise-linear function of
+# the roughness, tabulated by hand at ten relative IORs
+# (GGX_glass_energy_compensation_get_correction_exponent,
+# src/Device/includes/BSDFs/MicrofacetEnergyCompensation.h:101-631).
+EXPONENT_SOURCE = "src/Device/includes/BSDFs/MicrofacetEnergyCompensation.h"
+EXPONENT_TABLE = "glass_correction_exponent_2x10x11x4.f32"
+EXPONENT_KNOTS = ["1.01", "1.02", "1.03", "1.1", "1.2", "1.4", "1.5", "2.0", "2.4", "3.0"]
 
 
 def decimal_to_f32(text):
@@ -73,6 +83,54 @@ def sheen_table(ref_root):
     triples = re.findall(r"make_float3\(\s*([-0-9.eE]+)f?\s*,\s*([-0-9.eE]+)f?\s*,\s*([-0-9.eE]+)f?\s*\)", body)
     assert len(triples) == 32 * 32, len(triples)
     return np.array([[decimal_to_f32(v) for v in t] for t in triples], np.float32)
+
+
+def _segments(block, which):
+    """The roughness segments of one IOR band: 'if (roughness <= R) x = C;' or
+    'x = hippt::lerp(A, B, (roughness - R0) / 0.1f);' for R = 0.0, 0.1, ..., 1.0."""
+    pat = re.compile(r"roughness <= ([0-9.]+)f\)\s*" + which + r"_correction = "
+                     r"(?:hippt::lerp\(([0-9.]+)f, ([0-9.]+)f, \(roughness - ([0-9.]+)f\) / 0\.1f\)|([0-9.]+)f);")
+    segs = pat.findall(block)
+    assert len(segs) == 11, (which, len(segs))
+    rows = []
+    for k, (rmax, a, b, r0, c) in enumerate(segs):
+        assert abs(float(rmax) - 0.1 * k) < 1e-6, rmax
+        if c:
+            rows.append([decimal_to_f32(rmax), 0.0, decimal_to_f32(c), 0.0])
+        else:
+            # r0 is the previous segment's bound, written the same way
+            assert k > 0 and r0 == segs[k - 1][0], (r0, k)
+            rows.append([decimal_to_f32(rmax), 1.0, decimal_to_f32(a), decimal_to_f32(b)])
+    return rows
+
+
+def exponent_table(ref_root):
+    """[section lower/higher][IOR knot][roughness segment][r_max, kind, a, b] float32.
+    kind 0: constant a; kind 1: lerp(a, b, (roughness - r0) / 0.1f) with r0 the
+    previous segment's r_max (0 for the first segment). The lower section has no
+    band at the last knot (its row is 2.5 everywhere, never read)."""
+    text = open(os.path.join(ref_root, EXPONENT_SOURCE)).read()
+    start = text.index("GGX_glass_energy_compensation_get_correction_exponent(float roughness")
+    end = text.index("return hippt::lerp(lower_correction", start)
+    body = text[start:end]
+    split = body.index("float higher_relative_eta_bound")
+    out = np.full((2, 10, 11, 4), 2.5, np.float32)
+    out[..., 1] = 0.0
+    for sec, (part, which) in enumerate(((body[:split], "lower"), (body[split:], "higher"))):
+        bands = re.split(which + r"_relative_eta_bound = ", part)[2:]  # [0] = text before, [1] = the default
+        for band in bands:
+            knot = band[:band.index("f;")]
+            k = EXPONENT_KNOTS.index(knot)
+            out[sec, k] = np.array(_segments(band, which), np.float32)
+        expected = 9 if sec == 0 else 10
+        assert len(bands) == expected, (which, len(bands))
+    # the band conditions: lower band k is (knots[k], knots[k+1]], higher band k is
+    # 'relative_eta <= knots[k]' (else the last knot)
+    low_conds = re.findall(r"relative_eta > ([0-9.]+)f && relative_eta <= ([0-9.]+)f", body[:split])
+    assert low_conds == list(zip(EXPONENT_KNOTS[:-1], EXPONENT_KNOTS[1:])), low_conds
+    high_conds = re.findall(r"if \(relative_eta <= ([0-9.]+)f\)", body[split:])
+    assert high_conds == EXPONENT_KNOTS[:-1], high_conds
+    return out
 
 
 def read_rgbe(path):
@@ -132,6 +190,9 @@ def main(ref_root):
                         for z in range(layers)])
         vol.astype("<f4").tofile(os.path.join(HERE, out_name))
         print(out_name, vol.shape, float(vol.min()), float(vol.max()))
+    exp_tab = exponent_table(ref_root)
+    exp_tab.astype("<f4").tofile(os.path.join(HERE, EXPONENT_TABLE))
+    print(EXPONENT_TABLE, exp_tab.shape)
     tab = sheen_table(ref_root)
     tab.astype("<f4").tofile(os.path.join(HERE, SHEEN_TABLE[0]))
     print(SHEEN_TABLE[0], tab.shape, float(tab.min()), float(tab.max()))

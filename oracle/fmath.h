@@ -46,6 +46,8 @@ static inline float orc_cosf(float x)
 
 static inline float orc_log2f(float x)
 {
+    if (x == 0.0f)
+        return -INFINITY;
     int e;
     float m = frexpf(x, &e);
     if (m < 0x1.6a09e6p-1f) /* sqrt(1/2) */

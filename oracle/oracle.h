@@ -212,6 +212,8 @@ typedef struct {
     const float* sheen_ltc;         /* [32][32] x (Ai, Bi, Ri) */
     const float* glass;             /* [128][16][256] */
     const float* glass_inv;         /* [128][16][256] */
+    const float* thin_glass;        /* [96][32][32] */
+    const float* glass_exponent;    /* [lower/higher][IOR knot 10][roughness segment 11][r_max, kind, a, b] */
 } orc_luts;
 
 typedef struct {
@@ -274,6 +276,37 @@ void orc_bsdf_sample(const orc_cpu_material* m, const orc_luts* luts, float cons
 /* principled_bsdf_pdf (Principled.h:1717-1796) of the same configuration. */
 float orc_bsdf_pdf(const orc_cpu_material* m, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                    float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light);
+/* One BSDF query with a material table and a volume state: the material
+ * pushes of the path so far (the last one is the evaluated surface's), then
+ * the distance travelled in the medium and the sampled wavelength. */
+typedef struct {
+    const orc_cpu_material* mats; /* at most 16 */
+    int32_t nmats;
+    int32_t mat;                  /* material of the evaluated surface */
+    const int32_t* pushes;
+    int32_t npush;
+    float distance, wavelength;
+    float consistent_tau, consistent_tau_refract;
+    int32_t regularize, reg_mode;
+    float path_roughness;
+    int32_t info;                 /* lobe the direction comes from (eval / pdf) */
+    int32_t update;               /* the continuation sample: pops, resets */
+} orc_bsdf_query;
+/* out_state (may be null): pos, inside, incident, outgoing, distance, wavelength after the call */
+int32_t orc_bsdf_eval_q(const orc_bsdf_query* q, const orc_luts* luts, const float* view, const float* sn,
+                        const float* gn, const float* to_light, float* out_rgb, float* out_pdf, float* out_state);
+int32_t orc_bsdf_sample_q(const orc_bsdf_query* q, const orc_luts* luts, const float* view, const float* sn,
+                          const float* gn, uint32_t* rng_state, float* out_dir, float* out_rgb, float* out_pdf,
+                          int32_t* out_info, float* out_state);
+float orc_bsdf_pdf_q(const orc_bsdf_query* q, const orc_luts* luts, const float* view, const float* sn,
+                     const float* gn, const float* to_light);
+/* the nested-dielectrics stack alone: step i pushes mats[i] (priority
+ * prios[i]) or, for mats[i] < 0, pops with inside = -1 - mats[i]; out holds
+ * 9 ints per step: pos, inside, incident, outgoing, skip, entries 0-3 */
+void orc_vol_script(const int32_t* mats, const int32_t* prios, int32_t n, int32_t* out);
+float orc_glass_correction_exponent(const float* table, float roughness, float relative_eta);
+void orc_wavelength_rgb(float w, float* out);
+float orc_dispersion_ior(float abbe, float scale, float ior, float wavelength);
 uint32_t orc_oct_pack(const float* n);
 /* fmath.h functions element-wise: fn 0 sin, 1 cos, 2 pow(x, y), 3 log2, 4 exp2. */
 void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n);

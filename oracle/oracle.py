@@ -60,7 +60,8 @@ class Luts(C.Structure):
     """orc_luts: the BSDF tables."""
 
     _fields_ = [("conductor", C.c_void_p), ("glossy_dielectric", C.c_void_p), ("sheen_ltc", C.c_void_p),
-                ("glass", C.c_void_p), ("glass_inv", C.c_void_p)]
+                ("glass", C.c_void_p), ("glass_inv", C.c_void_p), ("thin_glass", C.c_void_p),
+                ("glass_exponent", C.c_void_p)]
 
 
 class RenderScene(C.Structure):
@@ -108,6 +109,13 @@ _SIG = {
     "orc_frame_gbuffer_prims": (None, [C.c_void_p, _P]),
     "orc_frame_alias_table": (None, [C.c_void_p, _P, _P, _P]),
     "orc_alias_table": (None, [_P, C.c_int32, C.c_float, _P, _P]),
+    "orc_bsdf_eval_q": (C.c_int32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_bsdf_sample_q": (C.c_int32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_bsdf_pdf_q": (C.c_float, [_P, _P, _P, _P, _P, _P]),
+    "orc_vol_script": (None, [_P, _P, C.c_int32, _P]),
+    "orc_glass_correction_exponent": (C.c_float, [_P, C.c_float, C.c_float]),
+    "orc_wavelength_rgb": (None, [C.c_float, _P]),
+    "orc_dispersion_ior": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
 }
 
 _lib = None
@@ -259,6 +267,8 @@ LUT_FILES = {
     "sheen_ltc": ("sheen_ltc_32x32.f32", (32, 32, 3)),
     "glass": ("ggx_glass_uncorrelated_256x16x128.f32", (128, 16, 256)),
     "glass_inv": ("ggx_glass_inv_uncorrelated_256x16x128.f32", (128, 16, 256)),
+    "thin_glass": ("ggx_thin_glass_uncorrelated_32x32x96.f32", (96, 32, 32)),
+    "glass_exponent": ("glass_correction_exponent_2x10x11x4.f32", (2, 10, 11, 4)),
 }
 
 
@@ -399,6 +409,104 @@ def alias_table(weights, total_sum):
 
 def consistent_tau(tau_0, sample_number):
     return float(lib().orc_consistent_tau(tau_0, sample_number))
+
+
+class BsdfQuery(C.Structure):
+    _fields_ = [("mats", C.c_void_p), ("nmats", C.c_int32), ("mat", C.c_int32), ("pushes", C.c_void_p),
+                ("npush", C.c_int32), ("distance", C.c_float), ("wavelength", C.c_float),
+                ("consistent_tau", C.c_float), ("consistent_tau_refract", C.c_float), ("regularize", C.c_int32),
+                ("reg_mode", C.c_int32), ("path_roughness", C.c_float), ("info", C.c_int32), ("update", C.c_int32)]
+
+
+class MediumBsdf:
+    """The oracle's Principled BSDF at a surface of material `mat` of a small
+    material table, with the path's volume state: the material indices pushed
+    on the interior stack so far (the last push is this surface's), the
+    distance travelled in the current medium and the sampled wavelength.
+    pushes=[m] enters m from the air; [m, m] is inside m, leaving it."""
+
+    STATE = ("pos", "inside", "incident", "outgoing", "distance", "wavelength")
+
+    def __init__(self, materials, mat, pushes, distance=0.0, wavelength=0.0, consistent_tau=1.0e6,
+                 consistent_tau_refract=1.0e6, reg_mode=0, path_roughness=0.0, regularize=1):
+        self.buf = b"".join(bytes(m) for m in materials)
+        self.mats = (C.c_char * len(self.buf)).from_buffer_copy(self.buf)
+        self.pushes = np.ascontiguousarray(pushes, np.int32)
+        self.arrays = load_luts()
+        self.luts = make_luts(self.arrays)
+        q = BsdfQuery()
+        q.mats = C.cast(self.mats, C.c_void_p)
+        q.nmats, q.mat = len(materials), mat
+        q.pushes = _p(self.pushes)
+        q.npush = self.pushes.size
+        q.distance, q.wavelength = distance, wavelength
+        q.consistent_tau, q.consistent_tau_refract = consistent_tau, consistent_tau_refract
+        q.regularize, q.reg_mode, q.path_roughness = regularize, reg_mode, path_roughness
+        self.q = q
+
+    def eval(self, view, sn, gn, to_light, info=0, update=0):
+        f = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        st = np.zeros(6, np.float32)
+        self.q.info, self.q.update = info, update
+        a = [np.ascontiguousarray(x, np.float32) for x in (view, sn, gn, to_light)]
+        rc = lib().orc_bsdf_eval_q(C.addressof(self.q), C.addressof(self.luts), *[_p(x) for x in a], _p(f), _p(pdf),
+                                   _p(st))
+        assert rc == 0
+        return f, float(pdf[0]), dict(zip(self.STATE, st.tolist()))
+
+    def pdf(self, view, sn, gn, to_light, info=0):
+        self.q.info, self.q.update = info, 0
+        a = [np.ascontiguousarray(x, np.float32) for x in (view, sn, gn, to_light)]
+        return float(lib().orc_bsdf_pdf_q(C.addressof(self.q), C.addressof(self.luts), *[_p(x) for x in a]))
+
+    def sample(self, view, sn, gn, state, update=0):
+        d = np.zeros(3, np.float32)
+        f = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        info = np.zeros(1, np.int32)
+        st = np.zeros(6, np.float32)
+        rs = np.array([state], np.uint32)
+        self.q.info, self.q.update = 0, update
+        a = [np.ascontiguousarray(x, np.float32) for x in (view, sn, gn)]
+        rc = lib().orc_bsdf_sample_q(C.addressof(self.q), C.addressof(self.luts), *[_p(x) for x in a], _p(rs), _p(d),
+                                     _p(f), _p(pdf), _p(info), _p(st))
+        assert rc == 0
+        return d, f, float(pdf[0]), int(rs[0]), int(info[0]), dict(zip(self.STATE, st.tolist()))
+
+
+def vol_script(steps):
+    """Runs pushes (mat, priority) and pops ('pop', inside) on an empty
+    nested-dielectrics stack; returns per step (pos, inside, incident, outgoing,
+    skip, [(material, priority, odd, topmost) of entries 0-3])."""
+    mats = np.array([(-1 - int(b)) if a == "pop" else a for a, b in steps], np.int32)
+    prios = np.array([0 if a == "pop" else b for a, b in steps], np.int32)
+    out = np.zeros((len(steps), 9), np.int32)
+    lib().orc_vol_script(_p(mats), _p(prios), len(steps), _p(out))
+    res = []
+    for o in out:
+        e = [int(x) & 0xffffffff for x in o[5:]]
+        entries = [(x >> 6, x & 15, (x >> 5) & 1, (x >> 4) & 1) for x in e]
+        res.append((int(o[0]), int(o[1]), int(o[2]), int(o[3]), int(o[4]), entries))
+    return res
+
+
+VOL_AIR = 0xffffffff >> 6
+
+
+def glass_correction_exponent(roughness, relative_eta):
+    tab = load_luts()["glass_exponent"]
+    return float(lib().orc_glass_correction_exponent(_p(tab), roughness, relative_eta))
+
+
+def dispersion_ior(abbe, scale, ior, wavelength):
+    return float(lib().orc_dispersion_ior(abbe, scale, ior, wavelength))
+
+
+def wavelength_rgb(w):
+    out = np.zeros(3, np.float32)
+    lib().orc_wavelength_rgb(w, _p(out))
+    return out
 
 
 FMATH = {"sin": 0, "cos": 1, "pow": 2, "log2": 3, "exp2": 4, "log": 5, "exp": 6, "atan2": 7}

@@ -89,8 +89,58 @@ static rgb color24(const float* c)
     return C3((float)r * (1.0f / 255.0f), (float)(g << 8) * (1.0f / (float)(255 << 8)), (float)b * (1.0f / 255.0f));
 }
 
-static omat unpack_material(const orc_cpu_material* m)
+/* CPUMaterial::make_safe (MaterialCPU.h:156-212), applied by the scene
+ * parser to every material (SceneParser.cpp:437) */
+static void clamp_c(float* c, float lo, float hi)
 {
+    for (int k = 0; k < 3; k++)
+        c[k] = hclamp(lo, hi, c[k]);
+}
+
+static orc_cpu_material make_safe(orc_cpu_material m)
+{
+    clamp_c(m.base_color, 0.0f, 1.0f);
+    m.metallic = hclamp(0.0f, 1.0f, m.metallic);
+    clamp_c(m.metallic_F82, 0.0f, 1.0f);
+    clamp_c(m.metallic_F90, 0.0f, 1.0f);
+    m.anisotropy = hclamp(0.0f, 1.0f, m.anisotropy);
+    m.anisotropy_rotation = hclamp(0.0f, 1.0f, m.anisotropy_rotation);
+    m.second_roughness_weight = hclamp(0.0f, 1.0f, m.second_roughness_weight);
+    m.second_roughness = hclamp(0.0f, 1.0f, m.second_roughness);
+    m.specular = hclamp(0.0f, 1.0f, m.specular);
+    m.specular_tint = hclamp(0.0f, 1.0f, m.specular_tint);
+    clamp_c(m.specular_color, 0.0f, 1.0f);
+    m.specular_darkening = hclamp(0.0f, 1.0f, m.specular_darkening);
+    m.coat = hclamp(0.0f, 1.0f, m.coat);
+    clamp_c(m.coat_medium_absorption, 0.0f, 1.0f);
+    m.coat_roughness = hclamp(1.0e-4f, 1.0f, m.coat_roughness);
+    m.coat_roughening = hclamp(0.0f, 1.0f, m.coat_roughening);
+    m.coat_darkening = hclamp(0.0f, 1.0f, m.coat_darkening);
+    m.coat_anisotropy = hclamp(0.0f, 1.0f, m.coat_anisotropy);
+    m.coat_anisotropy_rotation = hclamp(0.0f, 1.0f, m.coat_anisotropy_rotation);
+    m.sheen = hclamp(0.0f, 1.0f, m.sheen);
+    m.sheen_roughness = hclamp(1.0e-4f, 1.0f, m.sheen_roughness);
+    clamp_c(m.sheen_color, 0.0f, 1.0f);
+    m.specular_transmission = hclamp(0.0f, 1.0f, m.specular_transmission);
+    m.absorption_at_distance = hmax(m.absorption_at_distance, 1.0e-4f);
+    for (int k = 0; k < 3; k++)
+        m.absorption_color[k] = hmax(m.absorption_color[k], 1.0f / 512.0f);
+    clamp_c(m.absorption_color, 0.0f, 1.0f);
+    m.dispersion_abbe_number = hmax(1.0e-5f, m.dispersion_abbe_number);
+    m.dispersion_scale = hclamp(0.0f, 1.0f, m.dispersion_scale);
+    m.thin_film = hclamp(0.0f, 1.0f, m.thin_film);
+    m.dielectric_priority = m.dielectric_priority < 0 ? 0 : m.dielectric_priority > 15 ? 15 : m.dielectric_priority;
+    for (int k = 0; k < 3; k++)
+        m.emission[k] = hmax(0.0f, m.emission[k]);
+    if (m.specular_transmission == 0.0f && m.diffuse_transmission == 0.0f)
+        m.dielectric_priority = (1 << 15) - 1; /* stored in 8 bits: 255 */
+    return m;
+}
+
+static omat unpack_material(const orc_cpu_material* in)
+{
+    const orc_cpu_material safe = make_safe(*in);
+    const orc_cpu_material* m = &safe;
     omat o;
     o.emission = C3(m->emission[0] * m->emission_strength * m->global_emissive_factor,
                     m->emission[1] * m->emission_strength * m->global_emissive_factor,
@@ -134,7 +184,7 @@ static omat unpack_material(const orc_cpu_material* m)
     o.absorption_color = color24(m->absorption_color);
     o.dispersion_scale = u8f(q8(m->dispersion_scale));
     o.dispersion_abbe_number = m->dispersion_abbe_number;
-    o.dielectric_priority = m->dielectric_priority;
+    o.dielectric_priority = (unsigned char)m->dielectric_priority;
     /* get_intersection_material (Material.h:122-149): a coat roughens the
      * base and second metal roughness */
     if (o.coat > 0.0f && o.coat_roughening > 0.0f)
@@ -171,11 +221,16 @@ typedef struct {
     const float* lut;        /* GGX conductor E_ss, 128 x 128 */
     const float* lut_glossy; /* glossy dielectric E_ss, [128][64][128] */
     const float* lut_sheen;  /* sheen LTC (Ai, Bi, Ri), 32 x 32 */
+    const float* lut_glass;      /* GGX glass E, [128][16][256], entering */
+    const float* lut_glass_inv;  /* same, leaving */
+    const float* lut_thin_glass; /* thin-walled glass E, [96][32][32] */
+    const float* glass_exp;      /* correction exponent, [2][10][11][4] */
     const orc_octree* tree;
     const orc_camera* cam;
     orc_render_settings s;
     uint32_t sample_number;
     float consistent_tau;
+    float consistent_tau_refract; /* consistent_tau(tau_0, sample_number + 1) */
     /* LSS_BASE_POWER: alias table over the emissive list */
     const float* alias_probas;
     const int32_t* alias_alias;
@@ -414,7 +469,99 @@ static v3 ggx_sample(float r, float aniso, v3 lv, uint32_t* st)
 /* Principled BSDF, metal + diffuse lobes (Principled.h:125-215,        */
 /* :1129-1157, :1436-1492, :1515-1618, :1620-1715, :1802-1939)           */
 /* ------------------------------------------------------------------ */
-typedef struct { v3 view, sn, gn, L; int info, reg; float path_r; } bctx;
+/* Nested-dielectrics interior stack and the medium state of a path
+ * (NestedDielectrics.h:125-255, RayVolumeState.h:14-80): 4 entries, entry 0
+ * is the air. Entry word: priority bits 0-3, topmost bit 4, odd parity bit 5,
+ * material index from bit 6 (all ones: the air). */
+#define VOL_N 4
+#define VOL_AIR (0xffffffffu >> 6)
+typedef struct {
+    uint32_t e[VOL_N];
+    int pos;
+    int incident, outgoing, inside;
+    float distance, wavelength;
+} ovol;
+
+static uint32_t ve_mat(uint32_t e) { return e >> 6; }
+static uint32_t ve_make(uint32_t mat, uint32_t prio, int odd, int top)
+{
+    return ((mat & VOL_AIR) << 6) | (odd ? 32u : 0u) | (top ? 16u : 0u) | (prio & 15u);
+}
+
+static void vol_init(ovol* v)
+{
+    for (int i = 0; i < VOL_N; i++)
+        v->e[i] = ve_make(VOL_AIR, 0, 1, 1);
+    v->pos = 0;
+    v->incident = v->outgoing = -1;
+    v->inside = 0;
+    v->distance = 0.0f;
+    v->wavelength = 0.0f;
+}
+
+/* returns 1 when the boundary is skipped (inside a medium of higher priority) */
+static int vol_push(ovol* v, int mat, int prio)
+{
+    if (v->pos == VOL_N - 1)
+        return 0;
+    uint32_t m = (uint32_t)mat & VOL_AIR;
+    int last;
+    for (last = v->pos; last >= 0; last--)
+        if (ve_mat(v->e[last]) != m && (v->e[last] & 16u) && (v->e[last] & 32u))
+            break;
+    int odd = 1;
+    for (int i = v->pos; i >= 0; i--)
+        if (ve_mat(v->e[i]) == m)
+        {
+            v->e[i] &= ~16u;
+            odd = !(v->e[i] & 32u);
+            break;
+        }
+    v->inside = !odd;
+    if (v->pos < VOL_N - 1)
+        v->pos++;
+    v->e[v->pos] = ve_make(m, (uint32_t)prio, odd, 1);
+    if (prio < (int)(v->e[last] & 15u))
+        return 1;
+    if (odd)
+    {
+        v->incident = (int)ve_mat(v->e[last]);
+        v->outgoing = (int)m;
+    }
+    else
+    {
+        v->incident = (int)m;
+        v->outgoing = (int)ve_mat(v->e[last]);
+    }
+    return 0;
+}
+
+static void vol_pop(ovol* v, int inside)
+{
+    uint32_t top = ve_mat(v->e[v->pos]);
+    if (v->pos > 0)
+        v->pos--;
+    if (inside)
+    {
+        int prev;
+        for (prev = v->pos; prev >= 0; prev--)
+            if (ve_mat(v->e[prev]) == top)
+                break;
+        if (prev >= 0)
+            for (int i = prev + 1; i <= v->pos; i++)
+                v->e[i - 1] = v->e[i];
+        if (v->pos > 0)
+            v->pos--;
+    }
+    for (int i = v->pos; i >= 0; i--)
+        if (ve_mat(v->e[i]) == top)
+        {
+            v->e[i] |= 16u;
+            break;
+        }
+}
+
+typedef struct { v3 view, sn, gn, L; int info, reg; float path_r; ovol* vs; int update; } bctx;
 
 static rgb metallic_eval(const rctx* R, const omat* m, const bctx* c, float r, v3 lv, v3 ll, v3 lh, float* pdf)
 {
@@ -434,18 +581,19 @@ static rgb metallic_eval(const rctx* R, const omat* m, const bctx* c, float r, v
 
 typedef struct { float coat, sheen, m1, m2, spec, diff, glass, dt; } lobes;
 
-static lobes weights(const omat* m)
+static lobes weights(const omat* m, int outside_object)
 {
     lobes w;
-    const float outside = 1.0f;
+    const float outside = outside_object ? 1.0f : 0.0f;
     w.coat = m->coat * outside;
     w.sheen = m->sheen * outside;
     w.m1 = m->metallic * outside;
     w.m2 = m->metallic * outside;
     w.m1 = hlerp(w.m1, 0.0f, m->second_roughness_weight);
     w.m2 = hlerp(0.0f, w.m2, m->second_roughness_weight);
-    w.glass = (1.0f - m->metallic) * (1.0f - m->diffuse_transmission) * m->specular_transmission;
-    w.dt = (1.0f - m->metallic) * m->diffuse_transmission;
+    w.glass = !outside_object ? (1.0f - m->diffuse_transmission)
+                              : (1.0f - m->metallic) * (1.0f - m->diffuse_transmission) * m->specular_transmission;
+    w.dt = !outside_object ? m->diffuse_transmission : (1.0f - m->metallic) * m->diffuse_transmission;
     w.spec = (1.0f - m->metallic) * (1.0f - m->specular_transmission * (1.0f - m->diffuse_transmission)) * m->specular * outside;
     w.diff = (1.0f - m->metallic) * (1.0f - m->specular_transmission) * (1.0f - m->diffuse_transmission) * outside;
     return w;
@@ -681,15 +829,357 @@ static float metal_pdf(const rctx* R, const omat* m, const bctx* c, float r, v3 
     return ggx_dielectric_pdf(R, m, c, r, m->anisotropy, lv, ll, lh);
 }
 
-/* The principled BSDF, layer by layer (Principled.h:955-1512, :1620-1800);
- * glass and diffuse transmission are refused at upload (weight 0). */
+/* ------------------------------------------------------------------ */
+/* Glass and diffuse transmission (Principled.h:332-873, Microfacet.h:   */
+/* 288-392, MicrofacetEnergyCompensation.h:101-703, MicrofacetRegular-   */
+/* ization.h:60-146, MaterialUtils.h:32-57, Dispersion.h:433-535)        */
+/* ------------------------------------------------------------------ */
+static v3 refract_dir(v3 d, v3 n, float eta)
+{
+    float c = vdot(d, n);
+    float s2 = 1.0f - c * c;
+    float ct = sqrtf(1.0f - s2 / (eta * eta));
+    return vadd(vdivs(vneg(d), eta), vscale(n, c / eta - ct));
+}
+
+static float thin_rough(int thin, float r, float eta)
+{
+    if (!thin)
+        return r;
+    float a = (eta - 0.5f) * (eta - 0.5f), b = eta * eta * eta;
+    return hclamp(0.0f, 1.0f, r * sqrtf(3.7f * (eta - 1.0f) * a / b) / 1.39f);
+}
+
+static float reg_refract(const rctx* R, int mode, float r, float path_r, float ei, float et)
+{
+    if (!R->s.do_microfacet_regularization || mode == NO_REG)
+        return r;
+    float tau = R->consistent_tau_refract / hmax(path_r * path_r, 1.0e-8f);
+    float d = ei - et, mx = hmax(ei, et);
+    float reg = sqrtf(sqrtf(1.0f / (tau * M_PI_F * (d * d) / (4.0f * (mx * mx)))));
+    return hmax(R->s.microfacet_regularization_min_roughness, hmax(r, reg));
+}
+
+static float reg_mix(const rctx* R, int mode, float r, float path_r, float ei, float et)
+{
+    if (!R->s.do_microfacet_regularization || mode == NO_REG)
+        return r;
+    float tau = R->consistent_tau / hmax(path_r * path_r, 1.0e-8f);
+    float refl = sqrtf(sqrtf(1.0f / (tau * M_PI_F)));
+    if (ei == et)
+        return refl;
+    float d = ei - et, mx = hmax(ei, et);
+    float refr = sqrtf(sqrtf(1.0f / (tau * M_PI_F * (d * d) / (4.0f * (mx * mx)))));
+    return hmax(R->s.microfacet_regularization_min_roughness, hmax(r, refr * 0.75f + refl * 0.25f));
+}
+
+/* one IOR knot of the hand-tabulated correction exponent */
+static float exp_knot(const float* seg, float r)
+{
+    for (int k = 0; k < 11; k++)
+        if (r <= seg[4 * k])
+            return seg[4 * k + 1] == 0.0f ? seg[4 * k + 2]
+                                          : hlerp(seg[4 * k + 2], seg[4 * k + 3], (r - seg[4 * (k - 1)]) / 0.1f);
+    return 2.5f;
+}
+
+static float correction_exponent(const float* tab, float r, float eta)
+{
+    static const float knot[10] = {1.01f, 1.02f, 1.03f, 1.1f, 1.2f, 1.4f, 1.5f, 2.0f, 2.4f, 3.0f};
+    if ((r < 1.0e-10f && r > -1.0e-10f) || fabsf(1.0f - eta) < 1.0e-3f)
+        return 2.5f;
+    float lo_b = 1.01f, lo = 2.5f;
+    for (int k = 0; k < 9; k++)
+        if (eta > knot[k] && eta <= knot[k + 1])
+        {
+            lo_b = knot[k];
+            lo = exp_knot(tab + 44 * k, r);
+            break;
+        }
+    int h = 9;
+    for (int k = 0; k < 9; k++)
+        if (eta <= knot[k])
+        {
+            h = k;
+            break;
+        }
+    float hi = exp_knot(tab + 44 * (10 + h), r);
+    if (knot[h] == lo_b)
+        return lo;
+    return hlerp(lo, hi, (eta - lo_b) / (knot[h] - lo_b));
+}
+
+static float ec_glass(const rctx* R, const omat* m, float r, int inside, float et, float ei, float eta, float NoV)
+{
+    if (!m->do_glass_ec || r <= 0.0f)
+        return 1.0f;
+    float comp = 1.0f;
+    if (m->thin_film < 1.0f)
+    {
+        float e_corr = inside ? 1.0f / eta : eta;
+        float ex = m->thin_walled ? 2.5f : correction_exponent(R->glass_exp, r, e_corr);
+        float u = orc_powf(hmax(1.0e-3f, NoV), 1.0f / ex);
+        float f0 = sqrtf(sqrtf(f0_eta(et, ei)));
+        if (m->thin_walled)
+            comp = lut3d(R->lut_thin_glass, 32, 32, 96, u, r, f0);
+        else
+            comp = lut3d(inside ? R->lut_glass_inv : R->lut_glass, 256, 16, 128, u, r, f0);
+        comp = hlerp(comp, 1.0f, m->thin_film * r);
+    }
+    return comp;
+}
+
+static rgb beer(const rctx* R, ovol* v)
+{
+    const omat* md = &R->mats[v->incident];
+    rgb a = md->absorption_color;
+    if (a.r == 1.0f && a.g == 1.0f && a.b == 1.0f)
+        return C1(1.0f);
+    v->distance = hmax(v->distance, 1.0e-6f);
+    rgb k = cdivs(C3(orc_logf(a.r), orc_logf(a.g), orc_logf(a.b)), md->absorption_at_distance);
+    rgb e = cscale(k, v->distance);
+    return C3(orc_expf(e.r), orc_expf(e.g), orc_expf(e.b));
+}
+
+static rgb ggx_refract(const omat* m, float r, float eta, rgb F, v3 lv, v3 ll, v3 lh, float* pdf, int info)
+{
+    float NoL = ll.z, NoV = lv.z, HoL = vdot(ll, lh), HoV = vdot(lv, lh);
+    rgb col;
+    if (smooth(r))
+    {
+        int perfect = vdot(refract_dir(lv, V(0, 0, 1), eta), ll) > 0.999999f;
+        if (info == FROM_GLASS_REFRACT && perfect)
+        {
+            col = cmul(cmul(C1(1.0e9f), csub(C1(1.0f), F)), m->base_color);
+            col = cdivs(col, fabsf(NoL));
+            *pdf = 1.0e9f;
+        }
+        else
+        {
+            col = C1(0.0f);
+            *pdf = 0.0f;
+        }
+    }
+    else
+    {
+        float dp = HoL + HoV / eta, dp2 = dp * dp;
+        float denom = dp2 * NoL * NoV;
+        float ax, ay;
+        alphas(r, m->anisotropy, &ax, &ay);
+        float D = ggx_D(ax, ay, lh);
+        float g1v = 1.0f / (1.0f + lambda_(ax, ay, lv)), g1l = 1.0f / (1.0f + lambda_(ax, ay, ll));
+        float G2 = g1v * g1l;
+        *pdf = (fabsf(HoL) / dp2) * (g1v / fabsf(NoV) * D * fabsf(HoV));
+        col = cscale(cscale(cmul(cscale(m->base_color, D), csub(C1(1.0f), F)), G2), fabsf(HoL * HoV / denom));
+    }
+    return cdivs(col, eta * eta);
+}
+
+static float ggx_refract_pdf(const omat* m, float r, float eta, v3 lv, v3 ll, v3 lh, int info)
+{
+    float NoV = lv.z, HoL = vdot(ll, lh), HoV = vdot(lv, lh);
+    if (smooth(r))
+        return info == FROM_GLASS_REFRACT && vdot(refract_dir(lv, V(0, 0, 1), eta), ll) > 0.999999f ? 1.0e9f : 0.0f;
+    float dp = HoL + HoV / eta, dp2 = dp * dp;
+    float ax, ay;
+    alphas(r, m->anisotropy, &ax, &ay);
+    float D = ggx_D(ax, ay, lh);
+    float g1v = 1.0f / (1.0f + lambda_(ax, ay, lv));
+    return (fabsf(HoL) / dp2) * (g1v / fabsf(NoV) * D * fabsf(HoV));
+}
+
+/* Cauchy dispersion fitted to the Abbe number (OpenPBR), Dispersion.h:485-501 */
+static float disp_ior(float abbe, float scale, float ior, float w)
+{
+    if (scale == 0.0f)
+        return ior;
+    float ab = abbe / scale;
+    float B = (ior - 1.0f) / (ab * 0.00000191038851931481f);
+    float A = ior - B / 334777.96f;
+    return A + B / (w * w);
+}
+
+static float medium_ior(const rctx* R, int medium) { return (uint32_t)medium == VOL_AIR ? 1.0f : R->mats[medium].ior; }
+
+static void glass_etas(const rctx* R, const omat* m, const ovol* v, float* ei, float* et, float* eta)
+{
+    *ei = disp_ior(m->dispersion_abbe_number, m->dispersion_scale, medium_ior(R, v->incident), fabsf(v->wavelength));
+    *et = disp_ior(m->dispersion_abbe_number, m->dispersion_scale, medium_ior(R, v->outgoing), fabsf(v->wavelength));
+    *eta = *et / *ei;
+    if (fabsf(*eta - 1.0f) < 1.0e-5f)
+        *eta = 1.0f + 1.0e-5f;
+}
+
+static float glass_F(int thin, float r, float eta, float HoV, float* prob)
+{
+    float F = fresnel_d(HoV, eta);
+    if (thin && F < 1.0f && r < 0.1f)
+        F = F + (1.0f - F) * (1.0f - F) * F / (1.0f - F * F);
+    *prob = lum(C1(F));
+    return F;
+}
+
+static v3 glass_h(const rctx* R, int refl, int thin, float r, float eta, v3 lv, v3 ll)
+{
+    v3 h;
+    if (r <= 1.0e-4f && !R->s.do_microfacet_regularization)
+        h = V(0, 0, 1);
+    else
+        h = vnorm(refl ? vadd(ll, lv) : thin ? vadd(V(ll.x, ll.y, -ll.z), lv) : vadd(vscale(ll, eta), lv));
+    return h.z < 0.0f ? vneg(h) : h;
+}
+
+static rgb glass_eval(const rctx* R, const omat* m, const bctx* c, v3 lv, v3 ll, float* pdf)
+{
+    *pdf = 0.0f;
+    if (fabsf(ll.z) < 1.0e-8f)
+        return C1(0.0f);
+    int refl = ll.z * lv.z > 0.0f;
+    ovol* v = c->vs;
+    float ei, et, eta;
+    glass_etas(R, m, v, &ei, &et, &eta);
+    float sr = thin_rough(m->thin_walled, m->roughness, eta);
+    v3 h = glass_h(R, refl, m->thin_walled, sr, eta, lv, ll);
+    float HoL = vdot(ll, h), HoV = vdot(lv, h);
+    if (HoL * ll.z < 0.0f || HoV * lv.z < 0.0f)
+        return C1(0.0f);
+    float prob, F = glass_F(m->thin_walled, sr, eta, HoV, &prob);
+    int reg = R->s.do_microfacet_regularization;
+    if (refl)
+    {
+        float rr = reg && c->reg == REG_MIS ? reg_mix(R, c->reg, sr, c->path_r, ei, et)
+                 : reg && c->reg == REG_CLASSIC ? regularize(R, c->reg, sr, c->path_r) : sr;
+        rgb col = ggx_reflect_ss(rr, m->anisotropy, C1(F), lv, ll, h, pdf, delta_sampled(m, sr, m->anisotropy, c->info));
+        col = cdivs(col, ec_glass(R, m, m->roughness, v->inside, et, ei, eta, lv.z));
+        *pdf *= prob;
+        return col;
+    }
+    float rr = reg && c->reg == REG_MIS ? reg_mix(R, c->reg, sr, c->path_r, ei, et)
+             : reg && c->reg == REG_CLASSIC ? reg_refract(R, c->reg, sr, c->path_r, ei, et) : sr;
+    rgb col = ggx_refract(m, rr, eta, C1(F), lv, ll, h, pdf, c->info);
+    *pdf *= 1.0f - prob;
+    col = cdivs(col, ec_glass(R, m, rr, v->inside, et, ei, eta, lv.z));
+    if (m->thin_walled)
+        col = cmul(col, m->base_color);
+    if (m->thin_walled && c->update)
+        vol_pop(v, v->inside);
+    else if ((uint32_t)v->incident != VOL_AIR)
+    {
+        col = cmul(col, beer(R, v));
+        if (c->update)
+        {
+            v->distance = 0.0f;
+            if (v->inside)
+                vol_pop(v, 1);
+        }
+    }
+    return col;
+}
+
+static float glass_pdf(const rctx* R, const omat* m, const bctx* c, v3 lv, v3 ll)
+{
+    if (fabsf(ll.z) < 1.0e-8f)
+        return 0.0f;
+    int refl = ll.z * lv.z > 0.0f;
+    float ei, et, eta;
+    glass_etas(R, m, c->vs, &ei, &et, &eta);
+    float sr = thin_rough(m->thin_walled, m->roughness, eta);
+    v3 h = glass_h(R, refl, m->thin_walled, sr, eta, lv, ll);
+    float HoL = vdot(ll, h), HoV = vdot(lv, h);
+    if (HoL * ll.z < 0.0f || HoV * lv.z < 0.0f)
+        return 0.0f;
+    float prob;
+    glass_F(m->thin_walled, sr, eta, HoV, &prob);
+    int reg = R->s.do_microfacet_regularization;
+    if (refl)
+    {
+        float rr = reg && c->reg == REG_MIS ? reg_mix(R, c->reg, sr, c->path_r, ei, et)
+                 : reg && c->reg == REG_CLASSIC ? regularize(R, c->reg, sr, c->path_r) : sr;
+        int ds = delta_sampled(m, sr, m->anisotropy, c->info);
+        float pdf;
+        if (smooth(rr))
+            pdf = (ds == 2 || vdot(reflect(lv, V(0, 0, 1)), ll) < 0.999999f) ? 0.0f : 1.0e9f;
+        else if (ll.z < 0.0f)
+            pdf = 0.0f;
+        else
+        {
+            float ax, ay;
+            alphas(rr, m->anisotropy, &ax, &ay);
+            float D = ggx_D(ax, ay, h);
+            float G1V = 1.0f / (1.0f + lambda_(ax, ay, lv));
+            pdf = vndf(D, G1V, lv, h) / (4.0f * vdot(lv, h));
+        }
+        return pdf * prob;
+    }
+    float rr = reg && c->reg == REG_MIS ? reg_mix(R, c->reg, sr, c->path_r, ei, et)
+             : reg && c->reg == REG_CLASSIC ? reg_refract(R, c->reg, sr, c->path_r, ei, et) : sr;
+    return ggx_refract_pdf(m, rr, eta, lv, ll, h, c->info) * (1.0f - prob);
+}
+
+static v3 glass_sample(const rctx* R, const omat* m, bctx* c, v3 lv, uint32_t* st)
+{
+    ovol* v = c->vs;
+    float ei, et, eta;
+    glass_etas(R, m, v, &ei, &et, &eta);
+    float sr = thin_rough(m->thin_walled, m->roughness, eta);
+    if (R->s.do_microfacet_regularization && c->reg == REG_MIS)
+        sr = reg_mix(R, c->reg, sr, c->path_r, ei, et);
+    float ax, ay;
+    alphas(sr, m->anisotropy, &ax, &ay);
+    v3 mn = (ax <= 1.0e-4f && ay <= 1.0e-4f) ? V(0, 0, 1) : vndf_sample(lv, ax, ay, st);
+    float prob;
+    glass_F(m->thin_walled, sr, eta, vdot(lv, mn), &prob);
+    if (orc_rand01(st) < prob)
+    {
+        c->info = FROM_GLASS_REFLECT;
+        if (c->update)
+            vol_pop(v, 0);
+        return reflect(lv, mn);
+    }
+    c->info = FROM_GLASS_REFRACT;
+    if (vdot(mn, lv) < 0.0f)
+        mn = vneg(mn);
+    if (m->thin_walled)
+    {
+        v3 r = reflect(lv, mn);
+        r.z = r.z * -1.0f;
+        if (c->update)
+            vol_pop(v, 0);
+        return r;
+    }
+    return refract_dir(lv, mn, eta);
+}
+
+static rgb dt_eval(const rctx* R, const omat* m, const bctx* c, v3 lv, v3 ll, float* pdf)
+{
+    *pdf = 0.0f;
+    if (lv.z * ll.z > 0.0f)
+        return C1(0.0f);
+    rgb col = cscale(m->base_color, M_INV_PI_F);
+    ovol* v = c->vs;
+    if ((uint32_t)v->incident != VOL_AIR)
+    {
+        col = cmul(col, beer(R, v));
+        if (c->update)
+        {
+            v->distance = 0.0f;
+            if (v->inside)
+                vol_pop(v, 1);
+        }
+    }
+    *pdf = fabsf(ll.z * M_INV_PI_F);
+    return col;
+}
+
+/* The principled BSDF, layer by layer (Principled.h:955-1512, :1620-1800). */
 static rgb principled_eval(const rctx* R, const omat* m, const bctx* c, float* pdf)
 {
     *pdf = 0.0f;
-    const float inc = 1.0f;
-    int refr = vdot(c->sn, c->L) < 0.0f;
+    int outside = !c->vs->inside;
+    int refr = vdot(c->sn, c->L) < 0.0f && outside;
     frames f = make_frames(m, c);
-    lobes w = weights(m), p = probas(w);
+    const float inc = medium_ior(R, c->vs->incident);
+    lobes w = weights(m, outside), p = probas(w);
     rgb layers = C1(1.0f), out = C1(0.0f);
     /* coat */
     if (w.coat > 0.0f && ((f.lv.z > 0.0f && f.ll.z > 0.0f) || refr))
@@ -753,6 +1243,14 @@ static rgb principled_eval(const rctx* R, const omat* m, const bctx* c, float* p
         }
         out = cadd(out, contrib);
     }
+    /* glass (rotated frame) */
+    if (w.glass > 0.0f)
+    {
+        float gp = 0.0f;
+        rgb contrib = cmul(cscale(glass_eval(R, m, c, f.lvr, f.llr, &gp), w.glass), layers);
+        *pdf += gp * p.glass;
+        out = cadd(out, contrib);
+    }
     /* glossy base: specular over diffuse */
     rgb glossy = C1(0.0f);
     if (w.spec > 0.0f && ((f.lvr.z > 0.0f && f.llr.z > 0.0f) || (refr && c->info == FROM_DIFFUSE_TRANSMISSION)))
@@ -785,15 +1283,24 @@ static rgb principled_eval(const rctx* R, const omat* m, const bctx* c, float* p
         glossy = cadd(glossy, contrib);
     }
     out = cadd(out, cdivs(glossy, ec_glossy(R, m, inc, f.lv.z)));
+    /* diffuse transmission */
+    if (w.dt > 0.0f && f.ll.z < 0.0f)
+    {
+        float dp = 0.0f;
+        rgb contrib = cmul(cscale(dt_eval(R, m, c, f.lv, f.ll, &dp), w.dt), layers);
+        *pdf += dp * p.dt;
+        out = cadd(out, contrib);
+    }
     return cdivs(out, ec_clearcoat(R, m, inc, f.lv.z));
 }
 
 static float principled_pdf(const rctx* R, const omat* m, const bctx* c)
 {
     float pdf = 0.0f;
-    int refr = vdot(c->sn, c->L) < 0.0f;
+    int outside = !c->vs->inside;
+    int refr = vdot(c->sn, c->L) < 0.0f && outside;
     frames f = make_frames(m, c);
-    lobes w = weights(m), p = probas(w);
+    lobes w = weights(m, outside), p = probas(w);
     if (w.coat > 0.0f && ((f.lv.z > 0.0f && f.ll.z > 0.0f) || refr))
         pdf += (refr ? 0.0f : ggx_dielectric_pdf(R, m, c, m->coat_roughness, m->coat_anisotropy, f.lv, f.ll, f.lh)) *
                p.coat;
@@ -803,23 +1310,29 @@ static float principled_pdf(const rctx* R, const omat* m, const bctx* c)
         pdf += metal_pdf(R, m, c, m->roughness, f.lvr, f.llr, f.lhr) * p.m1;
     if (w.m2 * (float)!refr > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
         pdf += metal_pdf(R, m, c, m->second_roughness, f.lvr, f.llr, f.lhr) * p.m2;
+    if (w.glass > 0.0f)
+        pdf += glass_pdf(R, m, c, f.lvr, f.llr) * p.glass;
     float glossy = 0.0f;
     if (w.spec > 0.0f && ((f.lvr.z > 0.0f && f.llr.z > 0.0f) || (refr && c->info == FROM_DIFFUSE_TRANSMISSION)))
         glossy += ggx_dielectric_pdf(R, m, c, m->roughness, m->anisotropy, f.lvr, f.llr, f.lhr) * p.spec;
     if (w.diff * (float)!refr > 0.0f && f.lv.z > 0.0f && f.ll.z > 0.0f)
         glossy += (f.ll.z > 0.0f ? f.ll.z * M_INV_PI_F : 0.0f) * p.diff;
     pdf += glossy;
+    if (w.dt > 0.0f && f.ll.z < 0.0f)
+        pdf += (f.lv.z * f.ll.z > 0.0f ? 0.0f : fabsf(f.ll.z * M_INV_PI_F)) * p.dt;
     return pdf;
 }
 
 static rgb principled_sample(const rctx* R, const omat* m, bctx* c, v3* dir, float* pdf, uint32_t* st)
 {
     *pdf = 0.0f;
-    lobes w = weights(m), p = probas(w);
+    lobes w = weights(m, !c->vs->inside), p = probas(w);
     float cdf0 = p.coat, cdf1 = cdf0 + p.sheen, cdf2 = cdf1 + p.m1, cdf3 = cdf2 + p.m2;
     float cdf4 = cdf3 + p.spec, cdf5 = cdf4 + p.diff, cdf6 = cdf5 + p.dt;
     float r1 = orc_rand01(st);
     int s_dt = r1 > cdf5 && r1 < cdf6, s_glass = r1 > cdf6;
+    if (c->update && !s_glass && !s_dt)
+        vol_pop(c->vs, 0);
     v3 TR, BR;
     build_rotated_onb(c->sn, &TR, &BR, m->anisotropy_rotation * M_PI_F);
     v3 lvr = to_local(TR, BR, c->sn, c->view);
@@ -858,7 +1371,13 @@ static rgb principled_sample(const rctx* R, const omat* m, bctx* c, v3* dir, flo
         c->info = FROM_DIFFUSE;
         *dir = cosine_around_normal(c->sn, st);
     }
-    else return C1(0.0f);
+    else if (r1 < cdf6)
+    {
+        c->info = FROM_DIFFUSE_TRANSMISSION;
+        *dir = cosine_around_normal(vneg(c->sn), st);
+    }
+    else
+        *dir = to_world(TR, BR, c->sn, glass_sample(R, m, c, lvr, st));
     if (vdot(*dir, c->gn) < 0.0f && !s_glass && !s_dt)
         return C1(0.0f);
     c->L = *dir;
@@ -944,6 +1463,72 @@ static int trace_main(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out)
     return 1;
 }
 
+/* trace_main_path_ray (Intersect.h:200-275): every boundary hit is pushed
+ * on the interior stack; one of lower priority than the medium the ray is
+ * in is skipped and the ray goes on from the hit point (same filter); the
+ * distance inside a medium is accumulated (twice over a skipped boundary,
+ * as there). Reaching a dispersive glass samples the wavelength. */
+static int trace_path(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out, ovol* v, uint32_t* st)
+{
+    int skip;
+    do
+    {
+        orc_hit h = trace1(R, o, d, last, INFINITY);
+        if (h.prim < 0)
+            return 0;
+        v3 a, b, c;
+        tri(R, h.prim, &a, &b, &c);
+        out->p = vadd(o, vscale(d, h.t));
+        out->gn = vnorm(vnorm(vcross(vsub(b, a), vsub(c, a))));
+        out->sn = out->gn;
+        out->prim = h.prim;
+        int mat = R->mat_idx[h.prim];
+        skip = vol_push(v, mat, R->mats[mat].dielectric_priority);
+        if (v->inside)
+            v->distance += h.t;
+        fix_backfacing(vneg(d), &out->gn, &out->sn);
+        if (skip)
+        {
+            o = out->p;
+            v->distance += h.t;
+        }
+    } while (skip);
+    const omat* m = &R->mats[R->mat_idx[out->prim]];
+    if (m->dispersion_scale > 0.0f && m->specular_transmission > 0.0f && v->wavelength == 0.0f)
+        v->wavelength = -(orc_rand01(st) * (830.0f - 360.0f) + 360.0f);
+    return 1;
+}
+
+/* wavelength_to_RGB_fit (Dispersion.h:433-458) */
+static rgb wavelength_rgb(float w)
+{
+    rgb c;
+    if (w < 463.0f)
+        c.r = -1.2776028240727566e-01f / (1.0f + orc_expf((w - 4.2680623367293401e+02f) / 8.2197460736637176e+00f)) +
+              -1.3925673552505122e-11f * orc_expf((w - 45.0f) / 1.8175459086411596e+01f) + 1.2898689750552100e-01f;
+    else if (w > 553.0f)
+    {
+        float t = (w - 6.0625724092824566e+02f) * (1.0f / 2.6577826611702449e+01f);
+        c.r = 1.7963649137825513e+01f * (1.0f / 2.6577826611702449e+01f) * orc_expf(-0.5f * (t * t)) +
+              2.5574660155104657e-03f;
+    }
+    else
+        c.r = 0.0f;
+    float tg = (w - 5.4209217455705152e+02f) / -2.9598170255834638e+01f;
+    c.g = 3.4962267376163049e+02f * orc_expf(-0.5f * (tg * tg)) / w;
+    c.b = orc_expf(3.2987659944421112e+03f + (-2.0975839709372405e+05f / w) - 4.6368268395094020e+02f * orc_logf(w));
+    return cscale(c, 10.0f);
+}
+
+/* get_dispersion_ray_color (Dispersion.h:512-535) */
+static rgb dispersion_filter(float* w, float scale)
+{
+    if (scale == 0.0f || *w >= 0.0f)
+        return C1(1.0f);
+    *w = -*w;
+    return wavelength_rgb(*w);
+}
+
 /* ------------------------------------------------------------------ */
 /* Emissive-triangle sampling (LightUtils.h:51-76, :142-309,            */
 /* AliasTable.h:13-22) and NEE with MIS (Lights.h:123-226,              */
@@ -993,7 +1578,8 @@ static float emissive_hit_area_pdf(const rctx* R, float area, rgb emission)
     return pdf;
 }
 
-static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi, v3 view, float path_r, uint32_t* st)
+static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi, v3 view, float path_r, uint32_t* st,
+                                ovol* vs)
 {
     rgb light_mis = C1(0.0f);
     {
@@ -1011,7 +1597,7 @@ static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi,
             int in_shadow = sh.prim >= 0 && sh.t < dist - 1.0e-4f;
             if (!in_shadow)
             {
-                bctx c = {view, hi->sn, hi->gn, dn, NO_INFO, REG_MIS, path_r};
+                bctx c = {view, hi->sn, hi->gn, dn, NO_INFO, REG_MIS, path_r, vs, 0};
                 float bpdf;
                 rgb f = principled_eval(R, m, &c, &bpdf);
                 if (bpdf > 0.0f)
@@ -1030,7 +1616,7 @@ static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi,
     }
     rgb bsdf_mis = C1(0.0f);
     {
-        bctx c = {view, hi->sn, hi->gn, V(0, 0, 0), NO_INFO, REG_MIS, path_r};
+        bctx c = {view, hi->sn, hi->gn, V(0, 0, 0), NO_INFO, REG_MIS, path_r, vs, 0};
         v3 dir;
         float bpdf;
         rgb f = principled_sample(R, m, &c, &dir, &bpdf, st);
@@ -1069,7 +1655,7 @@ typedef struct { v3 point; float target; int32_t M; int32_t tri; float ucw; uint
 #define RESTIR_LOBE_MASK (0x3fu << 1) /* flags_to_BSDF_incident_light_info keeps bits 1-6 */
 #define FROM_GLASS_REFRACT_BIT (1u << 6)
 
-static rgb restir_final_shading(const rctx* R, const omat* m, const hitinfo* hi, v3 view, size_t pix)
+static rgb restir_final_shading(const rctx* R, const omat* m, const hitinfo* hi, v3 view, size_t pix, ovol* vs)
 {
     const ores* r = (const ores*)R->restir_out + pix;
     if (r->ucw <= 0.0f || r->tri < 0) /* NaN weight sum with no selected sample */
@@ -1087,7 +1673,7 @@ static rgb restir_final_shading(const rctx* R, const omat* m, const hitinfo* hi,
     }
     if (in_shadow)
         return C1(0.0f);
-    bctx c = {view, hi->sn, hi->gn, dir, (int)(r->flags & RESTIR_LOBE_MASK), REG_MIS, 0.0f};
+    bctx c = {view, hi->sn, hi->gn, dir, (int)(r->flags & RESTIR_LOBE_MASK), REG_MIS, 0.0f, vs, 0};
     float bpdf;
     rgb f = principled_eval(R, m, &c, &bpdf);
     float cosine = vdot(hi->sn, dir);
@@ -1154,6 +1740,16 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
     float path_r = 0.0f;
     int missed = 0;
     int found = g->prim != -1;
+    /* RayVolumeState::reconstruct_first_hit (RayVolumeState.h:35-59) */
+    ovol vs;
+    vol_init(&vs);
+    if (found)
+    {
+        const omat* m0 = &R->mats[mat];
+        vol_push(&vs, R->mat_idx[g->prim], m0->dielectric_priority);
+        if (m0->dispersion_scale > 0.0f && m0->specular_transmission > 0.0f && vs.wavelength == 0.0f)
+            vs.wavelength = -(orc_rand01(&st) * (830.0f - 360.0f) + 360.0f);
+    }
     for (int bounce = 0; bounce < R->s.nb_bounces + 1; bounce++)
     {
         if (missed)
@@ -1162,7 +1758,7 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
         {
             v3 o = hi.p;
             int32_t last = hi.prim;
-            found = trace_main(R, o, dir, last, &hi);
+            found = trace_path(R, o, dir, last, &hi, &vs, &st);
             if (found)
                 mat = R->mat_idx[hi.prim];
         }
@@ -1178,9 +1774,9 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
             /* estimate_direct_lighting (Lights.h:364-398) */
             rgb direct = C1(0.0f);
             if (R->ne > 0 && R->restir_out && bounce == 0)
-                direct = restir_final_shading(R, m, &hi, view, pix);
+                direct = restir_final_shading(R, m, &hi, view, pix, &vs);
             else if (R->ne > 0)
-                direct = cadd(C1(0.0f), sample_one_light_mis(R, m, &hi, view, path_r, &st));
+                direct = cadd(C1(0.0f), sample_one_light_mis(R, m, &hi, view, path_r, &st, &vs));
             rgb total = C1(0.0f);
             if (bounce == 0)
                 total = cadd(total, m->emission);
@@ -1188,13 +1784,17 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
             color = cadd(color, total);
         }
         /* path_tracing_compute_next_indirect_bounce (PathTracing.h:26-101) */
-        bctx c = {view, hi.sn, hi.gn, V(0, 0, 0), NO_INFO, NO_REG, path_r};
+        bctx c = {view, hi.sn, hi.gn, V(0, 0, 0), NO_INFO, NO_REG, path_r, &vs, 1};
         v3 nd;
         float bpdf;
         rgb f = principled_sample(R, m, &c, &nd, &bpdf, &st);
+        /* RayPayload::accumulate_roughness (RayPayload.h:52-89) */
         if (c.info == FROM_DIFFUSE || c.info == FROM_DIFFUSE_TRANSMISSION)
             path_r = 1.0f;
-        else if (c.info == FROM_FIRST_METAL || c.info == FROM_SPECULAR)
+        else if (c.info == FROM_COAT)
+            path_r = hmax(m->coat_roughness, path_r);
+        else if (c.info == FROM_FIRST_METAL || c.info == FROM_SPECULAR || c.info == FROM_GLASS_REFLECT ||
+                 c.info == FROM_GLASS_REFRACT)
             path_r = hmax(m->roughness, path_r);
         else if (c.info == FROM_SECOND_METAL)
             path_r = hmax(m->second_roughness, path_r);
@@ -1212,7 +1812,7 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
                 sc = hmin(sc, R->s.russian_roulette_throughput_clamp);
             thr = cscale(thr, sc);
         }
-        thr = cmul(thr, C1(1.0f));
+        thr = cmul(thr, dispersion_filter(&vs.wavelength, m->dispersion_scale));
         thr = cmul(thr, att);
         thr = C3(hmax(thr.r, 1.0e-5f), hmax(thr.g, 1.0e-5f), hmax(thr.b, 1.0e-5f));
         if (cblack(thr))
@@ -1253,45 +1853,184 @@ static rctx bsdf_ctx(const orc_luts* luts, float consistent_tau, int32_t regular
     R.lut = luts->conductor;
     R.lut_glossy = luts->glossy_dielectric;
     R.lut_sheen = luts->sheen_ltc;
+    R.lut_glass = luts->glass;
+    R.lut_glass_inv = luts->glass_inv;
+    R.lut_thin_glass = luts->thin_glass;
+    R.glass_exp = luts->glass_exponent;
     R.consistent_tau = consistent_tau;
     R.s.do_microfacet_regularization = regularize;
     return R;
+}
+
+/* A material table (at most 16 materials), the evaluated material and the
+ * path's volume state, built by pushing q->pushes (material indices) on an
+ * empty stack, then setting the distance and wavelength. */
+typedef struct {
+    omat mats[16];
+    ovol vs;
+} bsdf_setup;
+
+static int setup_query(const orc_bsdf_query* q, rctx* R, bsdf_setup* b)
+{
+    if (q->nmats < 1 || q->nmats > 16 || q->mat < 0 || q->mat >= q->nmats)
+        return -1;
+    for (int i = 0; i < q->nmats; i++)
+        b->mats[i] = unpack_material(q->mats + i);
+    R->mats = b->mats;
+    R->consistent_tau_refract = q->consistent_tau_refract;
+    vol_init(&b->vs);
+    for (int i = 0; i < q->npush; i++)
+    {
+        if (q->pushes[i] < 0 || q->pushes[i] >= q->nmats)
+            return -1;
+        vol_push(&b->vs, q->pushes[i], b->mats[q->pushes[i]].dielectric_priority);
+    }
+    b->vs.distance = q->distance;
+    b->vs.wavelength = q->wavelength;
+    return 0;
+}
+
+static void export_state(const ovol* v, float* out)
+{
+    if (!out)
+        return;
+    out[0] = (float)v->pos;
+    out[1] = (float)v->inside;
+    out[2] = (float)v->incident;
+    out[3] = (float)v->outgoing;
+    out[4] = v->distance;
+    out[5] = v->wavelength;
+}
+
+int32_t orc_bsdf_eval_q(const orc_bsdf_query* q, const orc_luts* luts, const float* view, const float* sn,
+                        const float* gn, const float* to_light, float* out_rgb, float* out_pdf, float* out_state)
+{
+    rctx R = bsdf_ctx(luts, q->consistent_tau, q->regularize);
+    bsdf_setup b;
+    if (setup_query(q, &R, &b))
+        return -1;
+    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]),
+              V(to_light[0], to_light[1], to_light[2]), q->info, q->reg_mode, q->path_roughness, &b.vs, q->update};
+    rgb f = principled_eval(&R, &b.mats[q->mat], &c, out_pdf);
+    out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
+    export_state(&b.vs, out_state);
+    return 0;
+}
+
+int32_t orc_bsdf_sample_q(const orc_bsdf_query* q, const orc_luts* luts, const float* view, const float* sn,
+                          const float* gn, uint32_t* rng_state, float* out_dir, float* out_rgb, float* out_pdf,
+                          int32_t* out_info, float* out_state)
+{
+    rctx R = bsdf_ctx(luts, q->consistent_tau, q->regularize);
+    bsdf_setup b;
+    if (setup_query(q, &R, &b))
+        return -1;
+    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]), V(0, 0, 0), NO_INFO,
+              q->reg_mode, q->path_roughness, &b.vs, q->update};
+    v3 d = V(0, 0, 0);
+    rgb f = principled_sample(&R, &b.mats[q->mat], &c, &d, out_pdf, rng_state);
+    out_dir[0] = d.x; out_dir[1] = d.y; out_dir[2] = d.z;
+    out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
+    if (out_info)
+        *out_info = c.info;
+    export_state(&b.vs, out_state);
+    return 0;
+}
+
+float orc_bsdf_pdf_q(const orc_bsdf_query* q, const orc_luts* luts, const float* view, const float* sn,
+                     const float* gn, const float* to_light)
+{
+    rctx R = bsdf_ctx(luts, q->consistent_tau, q->regularize);
+    bsdf_setup b;
+    if (setup_query(q, &R, &b))
+        return -1.0f;
+    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]),
+              V(to_light[0], to_light[1], to_light[2]), q->info, q->reg_mode, q->path_roughness, &b.vs, 0};
+    return principled_pdf(&R, &b.mats[q->mat], &c);
+}
+
+/* the single-material forms: the material alone, entered from the air */
+static orc_bsdf_query single(const orc_cpu_material* cm, float consistent_tau, int32_t reg_mode, float path_roughness)
+{
+    static const int32_t push0[1] = {0};
+    orc_bsdf_query q;
+    memset(&q, 0, sizeof(q));
+    q.mats = cm;
+    q.nmats = 1;
+    q.mat = 0;
+    q.pushes = push0;
+    q.npush = 1;
+    q.consistent_tau = consistent_tau;
+    q.consistent_tau_refract = consistent_tau;
+    q.regularize = 1;
+    q.reg_mode = reg_mode;
+    q.path_roughness = path_roughness;
+    return q;
 }
 
 void orc_bsdf_eval(const orc_cpu_material* cm, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                    float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light,
                    float* out_rgb, float* out_pdf)
 {
-    rctx R = bsdf_ctx(luts, consistent_tau, 1);
-    omat m = unpack_material(cm);
-    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]),
-              V(to_light[0], to_light[1], to_light[2]), NO_INFO, reg_mode, path_roughness};
-    rgb f = principled_eval(&R, &m, &c, out_pdf);
-    out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
+    orc_bsdf_query q = single(cm, consistent_tau, reg_mode, path_roughness);
+    orc_bsdf_eval_q(&q, luts, view, sn, gn, to_light, out_rgb, out_pdf, NULL);
 }
 
 void orc_bsdf_sample(const orc_cpu_material* cm, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                      float path_roughness, const float* view, const float* sn, const float* gn, uint32_t* rng_state,
                      float* out_dir, float* out_rgb, float* out_pdf)
 {
-    rctx R = bsdf_ctx(luts, consistent_tau, 1);
-    omat m = unpack_material(cm);
-    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]), V(0, 0, 0), NO_INFO,
-              reg_mode, path_roughness};
-    v3 d = V(0, 0, 0);
-    rgb f = principled_sample(&R, &m, &c, &d, out_pdf, rng_state);
-    out_dir[0] = d.x; out_dir[1] = d.y; out_dir[2] = d.z;
-    out_rgb[0] = f.r; out_rgb[1] = f.g; out_rgb[2] = f.b;
+    orc_bsdf_query q = single(cm, consistent_tau, reg_mode, path_roughness);
+    orc_bsdf_sample_q(&q, luts, view, sn, gn, rng_state, out_dir, out_rgb, out_pdf, NULL, NULL);
 }
 
 float orc_bsdf_pdf(const orc_cpu_material* cm, const orc_luts* luts, float consistent_tau, int32_t reg_mode,
                    float path_roughness, const float* view, const float* sn, const float* gn, const float* to_light)
 {
-    rctx R = bsdf_ctx(luts, consistent_tau, 1);
-    omat m = unpack_material(cm);
-    bctx c = {V(view[0], view[1], view[2]), V(sn[0], sn[1], sn[2]), V(gn[0], gn[1], gn[2]),
-              V(to_light[0], to_light[1], to_light[2]), NO_INFO, reg_mode, path_roughness};
-    return principled_pdf(&R, &m, &c);
+    orc_bsdf_query q = single(cm, consistent_tau, reg_mode, path_roughness);
+    return orc_bsdf_pdf_q(&q, luts, view, sn, gn, to_light);
+}
+
+/* The nested-dielectrics stack on its own: pushes (mat >= 0, its priority)
+ * and pops (mat = -1 - inside flag) applied in order; out[i] receives the
+ * state after step i: pos, inside, incident, outgoing, skip, entries 0-3. */
+void orc_vol_script(const int32_t* mats, const int32_t* prios, int32_t n, int32_t* out)
+{
+    ovol v;
+    vol_init(&v);
+    for (int i = 0; i < n; i++)
+    {
+        int skip = 0;
+        if (mats[i] >= 0)
+            skip = vol_push(&v, mats[i], prios[i]);
+        else
+            vol_pop(&v, -1 - mats[i]);
+        int32_t* o = out + 9 * i;
+        o[0] = v.pos;
+        o[1] = v.inside;
+        o[2] = v.incident;
+        o[3] = v.outgoing;
+        o[4] = skip;
+        for (int k = 0; k < 4; k++)
+            o[5 + k] = (int32_t)v.e[k];
+    }
+}
+
+/* glass correction exponent and dispersion helpers (property tests) */
+float orc_glass_correction_exponent(const float* table, float roughness, float relative_eta)
+{
+    return correction_exponent(table, roughness, relative_eta);
+}
+
+float orc_dispersion_ior(float abbe, float scale, float ior, float wavelength)
+{
+    return disp_ior(abbe, scale, ior, wavelength);
+}
+
+void orc_wavelength_rgb(float w, float* out)
+{
+    rgb c = wavelength_rgb(w);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
 }
 
 /* ================================================================== */
@@ -1426,9 +2165,28 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
         orc_alias_table(power, E, sum, f->alias_probas, f->alias_alias);
         free(power);
     }
-    rctx R = {sc->vertices, sc->indices, sc->material_indices, f->mats, sc->emissive, sc->emissive_count, f->areas,
-              sc->luts.conductor, sc->luts.glossy_dielectric, sc->luts.sheen_ltc, tree, cam, *s, 0, 0.0f,
-              f->alias_probas, f->alias_alias, sum, NULL, 0};
+    rctx R;
+    memset(&R, 0, sizeof(R));
+    R.verts = sc->vertices;
+    R.idx = sc->indices;
+    R.mat_idx = sc->material_indices;
+    R.mats = f->mats;
+    R.emissive = sc->emissive;
+    R.ne = sc->emissive_count;
+    R.areas = f->areas;
+    R.lut = sc->luts.conductor;
+    R.lut_glossy = sc->luts.glossy_dielectric;
+    R.lut_sheen = sc->luts.sheen_ltc;
+    R.lut_glass = sc->luts.glass;
+    R.lut_glass_inv = sc->luts.glass_inv;
+    R.lut_thin_glass = sc->luts.thin_glass;
+    R.glass_exp = sc->luts.glass_exponent;
+    R.tree = tree;
+    R.cam = cam;
+    R.s = *s;
+    R.alias_probas = f->alias_probas;
+    R.alias_alias = f->alias_alias;
+    R.alias_sum = sum;
     f->R = R;
     if (rs)
         f->rs = *rs;
@@ -1484,6 +2242,7 @@ void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t rn, in
 {
     f->R.sample_number = sample_number;
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     int kp = keep_prev(f);
     if (sample_number == 0)
     {
@@ -1517,6 +2276,19 @@ void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t rn, in
 /* ---- ReSTIR DI helpers ---------------------------------------------- */
 typedef struct { int32_t mat, prim; v3 view, sn, gn, p; } osurf;
 
+/* the surface's volume state: its material pushed on an empty stack
+ * (RayVolumeState::reconstruct_first_hit); dispersive glass is refused with
+ * ReSTIR DI */
+static ovol surf_vol(const orc_frame* f, const osurf* s);
+
+static ovol surf_vol(const orc_frame* f, const osurf* s)
+{
+    ovol v;
+    vol_init(&v);
+    vol_push(&v, f->R.mat_idx[s->prim], f->R.mats[s->mat].dielectric_priority);
+    return v;
+}
+
 static osurf surf_of(const orc_frame* f, const gbuf* g, size_t pix)
 {
     osurf s;
@@ -1545,7 +2317,8 @@ static float target_fn(const orc_frame* f, const osurf* s, v3 point, int32_t tri
     float cosine = vdot(s->sn, dir);
     if (cosine <= 0.0f)
         return 0.0f;
-    bctx c = {s->view, s->sn, s->gn, dir, (int)(flags & RESTIR_LOBE_MASK), NO_REG, 0.0f};
+    ovol vs = surf_vol(f, s);
+    bctx c = {s->view, s->sn, s->gn, dir, (int)(flags & RESTIR_LOBE_MASK), NO_REG, 0.0f, &vs, 0};
     float bpdf;
     rgb fc = principled_eval(R, &R->mats[s->mat], &c, &bpdf);
     rgb em = R->mats[R->mat_idx[tri_i]].emission;
@@ -1641,6 +2414,7 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
 {
     f->R.sample_number = sample_number;
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     const rctx* R = &f->R;
     const int32_t nl = f->rs.number_of_initial_light_candidates, nb = f->rs.number_of_initial_bsdf_candidates;
     if (R->ne > 0)
@@ -1679,7 +2453,8 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                 float w = 0.0f, sample_target = 0.0f;
                 if (cosine > 0.0f && pdf > 0.0f)
                 {
-                    bctx c = {s.view, s.sn, s.gn, dir, NO_INFO, NO_REG, 0.0f};
+                    ovol vs = surf_vol(f, &s);
+                    bctx c = {s.view, s.sn, s.gn, dir, NO_INFO, NO_REG, 0.0f, &vs, 0};
                     float bsdf_pdf;
                     principled_eval(R, m, &c, &bsdf_pdf);
                     float target = target_fn(f, &s, point, tri_i, 0u, 0);
@@ -1702,7 +2477,8 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
             }
             for (int32_t i = 0; i < nb; i++)
             {
-                bctx c = {s.view, s.sn, s.gn, V(0, 0, 0), NO_INFO, NO_REG, 0.0f};
+                ovol vs = surf_vol(f, &s);
+                bctx c = {s.view, s.sn, s.gn, V(0, 0, 0), NO_INFO, NO_REG, 0.0f, &vs, 0};
                 v3 dir;
                 float bpdf;
                 principled_sample(R, m, &c, &dir, &bpdf, &st);
@@ -1754,6 +2530,7 @@ void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t rn
 {
     f->R.sample_number = sample_number;
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     const int32_t out_i = f->rs.do_spatial_reuse_pass ? 0 : (f->odd_frame ? 1 : 2);
     ores* in = f->res[f->last_output];
     ores* init = f->res[0];
@@ -1858,6 +2635,7 @@ void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_
 {
     f->R.sample_number = sample_number;
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     int32_t in_i, out_i;
     if (pass_index == 0)
     {
@@ -1977,6 +2755,7 @@ int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t rn, 
 {
     f->R.sample_number = sample_number;
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     f->R.restir_out = restir_on(f) ? f->res[f->shade_buffer] : NULL;
     f->R.final_shading_visibility = f->rs.do_final_shading_visibility;
     int32_t rejected = 0;

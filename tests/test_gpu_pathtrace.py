@@ -183,7 +183,8 @@ def test_scene_without_emitters_is_black(mipt_mod, oracle_mod, torch_mod):
 
 
 def test_unsupported_material_is_refused(mipt_mod, torch_mod):
-    scene = mipt_mod.generate_scene(2, 64, 36)  # principled materials with glass
+    scene = mipt_mod.generate_scene(2, 64, 36)
+    scene.materials[3].thin_film = 0.5  # thin-film interference is not implemented
     ctx = mipt_mod.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
