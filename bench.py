@@ -52,6 +52,10 @@ def main():
     ap.add_argument("--cache-plain-spp", type=int, default=32, help="plain samples whose time is the budget")
     ap.add_argument("--cache-warmup", type=int, default=8,
                     help="training-only frames before the cache form accumulates (their time counts)")
+    ap.add_argument("--cache-query-bounce", type=int, default=None,
+                    help="vertex at which the cache answers (default: the library's, 2)")
+    ap.add_argument("--cache-train-every", type=int, default=None,
+                    help="one training path in N (default: the library's, 16)")
     ap.add_argument("--frames", choices=["plain", "cache"], default=None,
                     help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain or with the "
                          "radiance cache, and print ms per sample (a kernel-profiling workload)")
@@ -162,8 +166,10 @@ def main():
 
     render = None if args.no_render else plain_render_leg(mipt, torch, np, args.no_cpu)
     restir = None if args.no_render else restir_render_leg(mipt, torch, np)
+    cache_over = {k: v for k, v in (("query_bounce", args.cache_query_bounce),
+                                    ("train_every", args.cache_train_every)) if v is not None}
     cache = None if args.no_cache else cache_vs_plain_leg(mipt, torch, np, args.cache_ref_spp, args.cache_plain_spp,
-                                                          args.cache_warmup)
+                                                          args.cache_warmup, cache_over)
 
     out = {
         "metric": METRIC,
@@ -233,14 +239,14 @@ def frames_only(mode, samples):
     ctx.close()
 
 
-def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup):
+def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup, cache_over=None):
     """Equal-time relMSE, learned radiance cache vs plain, config-1 scene at
     1920x1080, 4 bounces, uniform NEE + MIS. relMSE = mean over pixels and
     channels of (x - r)^2 / (r^2 + 1e-2), reduced in float64, against a plain
     render with FP64 accumulation at ref_spp samples. The plain form renders
-    plain_spp samples; its wall time is the budget. The cache form (queried at
-    bounce 2, trained every frame on 1/16 of the paths, batches of 2^16, Adam
-    lr 1e-2) first renders `warmup` training-only frames, whose time counts,
+    plain_spp samples; its wall time is the budget. The cache form (by default
+    queried at bounce 2, trained every frame on 1/16 of the paths, batches of
+    2^16, Adam lr 1e-2; cache_over replaces settings) first renders `warmup` training-only frames, whose time counts,
     then accumulates until it matches the plain relMSE or runs 3x the budget.
     Reported: both relMSE at equal time and the cache's time to reach the
     plain relMSE as a fraction of the plain time (interpolated between
@@ -256,7 +262,7 @@ def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup):
         ctx.set_scene(scene)
         ctx.build_bvh()
         if cache:
-            ctx.set_radiance_cache(mipt.default_radiance_cache_settings())
+            ctx.set_radiance_cache(mipt.default_radiance_cache_settings(**(cache_over or {})))
         ctx.set_render_settings(mipt.default_render_settings(width=W, height=H, nb_bounces=4, accumulate_fp64=1,
                                                              radiance_cache=1 if cache else 0))
         ctx.set_camera(scene.camera)
@@ -342,6 +348,7 @@ def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup):
             "cache_relmse_at_equal_time": at_time(cached, budget),
             "cache_seconds_to_plain_relmse": None if reach is None else round(reach, 4),
             "time_ratio_cache_over_plain": None if reach is None else round(reach / budget, 4),
+            "cache_settings_overrides": dict(cache_over or {}),
             "cache_warmup_frames": warmup, "cache_warmup_seconds": round(warm_s, 4),
             "cache_ms_per_spp": round((cached[-1][0] - warm_s) / cached[-1][1] * 1e3, 3),
             "cache_queries_last_frame": queries, "cache_training_records_last_frame": train,

@@ -115,7 +115,7 @@ std::string unpack_material(const mipt_material& in, float o[64])
     o[25] = m.ior;
     uint32_t flags = (m.do_metallic_energy_compensation ? 1u : 0u) | (m.do_specular_energy_compensation ? 2u : 0u) |
                      (m.do_coat_energy_compensation ? 4u : 0u) | (m.do_glass_energy_compensation ? 8u : 0u) |
-                     (m.thin_walled ? 16u : 0u);
+                     (m.thin_walled ? 16u : 0u) | (m.thin_film_do_ior_override ? 32u : 0u);
     std::memcpy(&o[26], &flags, 4);
     o[27] = u8f(q8(m.specular_tint));
     color3(m.specular_color, o + 28);
@@ -136,6 +136,13 @@ std::string unpack_material(const mipt_material& in, float o[64])
     o[51] = m.dispersion_abbe_number;
     const int32_t priority = (uint8_t)m.dielectric_priority;
     std::memcpy(&o[52], &priority, 4);
+    // thin film: IOR, thickness, kappa and override IOR at full precision, the
+    // hue shift as 8 bits of degrees / 360 (MaterialCPU.h:125-132)
+    o[53] = m.thin_film_ior;
+    o[54] = m.thin_film_thickness;
+    o[55] = m.thin_film_kappa_3;
+    o[56] = u8f(q8(m.thin_film_hue_shift_degrees / 360.0f));
+    o[57] = m.thin_film_base_ior_override;
 
     // Coat roughening of the base and second metal roughness
     // (get_intersection_material, src/Device/includes/Material.h:122-149).
@@ -157,8 +164,6 @@ std::string unpack_material(const mipt_material& in, float o[64])
         }
     }
 
-    // Every lobe but the thin-film interference Fresnel is implemented.
-    if (o[24] > 0.0f) return "thin-film Fresnel";
     return "";
 }
 

@@ -435,9 +435,13 @@ __global__ __launch_bounds__(256) void k_nrc_weight_grad(NrcTrainDev T, uint32_t
         }
 }
 
+// Grid gradient: scatters the feature gradients to the 8 corners of a level.
 __global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restrict__ queries)
 {
+    // one thread per (row, level): a batch has few rows, and a row's 16 levels
+    // x 8 corners x 2 atomics in one thread left the GPU latency-bound
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)blockIdx.y;
     const uint32_t rows = T.sel.rows();
     if (row >= rows)
         return;
@@ -447,20 +451,17 @@ __global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restric
     normalized_position(P, q, pos);
     const uint32_t tmask = (1u << P.log2_table) - 1u;
     const float scale = 1.0f / (3.0f * (float)rows);
-    for (int l = 0; l < P.levels; l++)
+    const NrcGridLevel& L = P.level[l];
+    const float g0 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l] * scale;
+    const float g1 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l + 1] * scale;
+    for (int cc = 0; cc < 8; cc++)
     {
-        const NrcGridLevel& L = P.level[l];
-        const float g0 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l] * scale;
-        const float g1 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l + 1] * scale;
-        for (int cc = 0; cc < 8; cc++)
-        {
-            uint32_t idx;
-            float w;
-            corner(L, pos, cc, tmask, idx, w);
-            const size_t e = ((size_t)l << P.log2_table) + idx;
-            atomicAdd(T.ggrad + 2 * e, w * g0);
-            atomicAdd(T.ggrad + 2 * e + 1, w * g1);
-        }
+        uint32_t idx;
+        float w;
+        corner(L, pos, cc, tmask, idx, w);
+        const size_t e = ((size_t)l << P.log2_table) + idx;
+        atomicAdd(T.ggrad + 2 * e, w * g0);
+        atomicAdd(T.ggrad + 2 * e + 1, w * g1);
     }
 }
 
@@ -749,7 +750,7 @@ void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, con
     hipLaunchKernelGGL(k_nrc_train_fwd_bwd, dim3(grid), dim3(256), 0, s, T);
     const uint32_t chunk = 1024;
     hipLaunchKernelGGL(k_nrc_weight_grad, dim3(blocks_for(stride, chunk), NRC_LAYERS), dim3(256), 0, s, T, chunk);
-    hipLaunchKernelGGL(k_nrc_grid_grad, dim3(blocks_for(stride, 128)), dim3(128), 0, s, P, T,
+    hipLaunchKernelGGL(k_nrc_grid_grad, dim3(blocks_for(stride, 128), (unsigned)P.levels), dim3(128), 0, s, P, T,
                        reinterpret_cast<const float4*>(d_queries));
     HIP_CHECK(hipGetLastError());
 }

@@ -89,7 +89,12 @@ struct Material {
     float dispersion_scale;
     float dispersion_abbe_number;
     int32_t dielectric_priority;
-    float pad[11];
+    float thin_film_ior;
+    float thin_film_thickness;       // nm
+    float thin_film_kappa_3;         // extinction of the base under the film (IOR override only)
+    float thin_film_hue_shift;       // hue shift / 360 degrees, 8-bit
+    float thin_film_base_ior_override;
+    float pad[6];
 };
 static_assert(sizeof(Material) == 256, "Material must be 256 bytes");
 
@@ -98,6 +103,7 @@ constexpr uint32_t MAT_SPECULAR_EC = 2u;
 constexpr uint32_t MAT_COAT_EC = 4u;
 constexpr uint32_t MAT_GLASS_EC = 8u;
 constexpr uint32_t MAT_THIN_WALLED = 16u;
+constexpr uint32_t MAT_THIN_FILM_IOR_OVERRIDE = 32u;
 
 // Sampled-lobe information (src/Device/includes/BSDFs/BSDFIncidentLightInfo.h).
 enum LobeInfo : uint32_t {
@@ -407,8 +413,10 @@ struct BsdfCtx {
 };
 
 // Principled.h:125-153
+PT_HD Color thin_film_fresnel(const Material& m, float ambient_ior, float HoL);
+
 PT_HD Color metallic_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, float roughness, float anisotropy,
-                          f3 lv, f3 ll, f3 lh, float& pdf)
+                          float incident_ior, f3 lv, f3 ll, f3 lh, float& pdf)
 {
     float rr = regularize_reflection(g, c.reg, roughness, c.path_roughness);
     DeltaSampled ds = delta_reflection_sampled(m, rr, anisotropy, c.info);
@@ -418,9 +426,8 @@ PT_HD Color metallic_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx
         return col(0.0f);
     }
     float HoL = clampf_(1.0e-8f, 1.0f, dot(lh, ll));
-    Color F = f82_tint_fresnel(m.base_color, m.metallic_F82, m.metallic_F90, m.metallic_F90_falloff_exponent, HoL);
-    // Thin-film Fresnel is lerped in with weight material.thin_film (0 for
-    // every material this build accepts).
+    Color F_metal = f82_tint_fresnel(m.base_color, m.metallic_F82, m.metallic_F90, m.metallic_F90_falloff_exponent, HoL);
+    Color F = lerpc(F_metal, thin_film_fresnel(m, incident_ior, HoL), m.thin_film);
     Color ms = ggx_energy_compensation_conductor(g, F, rr, (m.flags & MAT_METALLIC_EC) != 0, lv);
     Color ss = ggx_eval_reflect_ss(rr, anisotropy, F, lv, ll, lh, pdf, ds);
     return ss * ms;
@@ -477,6 +484,153 @@ PT_HD float fresnel_hemispherical_albedo_fit(float relative_eta)
                      (-774.4f * (relative_eta * relative_eta) + 10212.0f * relative_eta + 1.0f));
 }
 
+
+// ---------------------------------------------------------------------------
+// Thin-film interference Fresnel (Belcour & Barla 2017, "A Practical Extension
+// to Microfacet Theory for the Modeling of Varying Iridescence", and its
+// supplemental document), src/Device/includes/BSDFs/ThinFilm.h
+// ---------------------------------------------------------------------------
+// eval_sensitivity (ThinFilm.h:13-30): the Gaussian fit of the XYZ colour
+// matching functions in the Fourier domain, for an optical path difference in um.
+PT_HD Color thin_film_sensitivity(float opd, float shift)
+{
+    const float phase = 2.0f * PI_F * opd * 1.0e-6f;
+    const float val[3] = {5.4856e-13f, 4.4201e-13f, 5.2481e-13f};
+    const float pos[3] = {1.6810e+06f, 1.7953e+06f, 2.2084e+06f};
+    const float var[3] = {4.3278e+09f, 9.3046e+09f, 6.6121e+09f};
+    float xyz[3];
+    for (int k = 0; k < 3; k++)
+        xyz[k] = val[k] * sqrtf(2.0f * PI_F * var[k]) * cos_fixed(pos[k] * phase + shift) *
+                 exp_fixed(-1.0f * var[k] * phase * phase);
+    xyz[0] += 9.7470e-14f * sqrtf(2.0f * PI_F * 4.5282e+09f) * cos_fixed(2.2399e+06f * phase + shift) *
+              exp_fixed(-4.5282e+09f * phase * phase);
+    return col(xyz[0] / 1.0685e-7f, xyz[1] / 1.0685e-7f, xyz[2] / 1.0685e-7f);
+}
+
+// fresnel_phase (ThinFilm.h:37-62): phase shifts of the parallel and
+// perpendicular polarisations at an interface eta1 | eta2 + i kappa2.
+PT_HD void thin_film_phase(float cos_theta_i, float eta1, float eta2, float kappa2, float& phi_par, float& phi_perp)
+{
+    float sin_theta2 = 1.0f - cos_theta_i * cos_theta_i;
+    float A = (eta2 * eta2) * (1.0f - kappa2 * kappa2) - (eta1 * eta1) * sin_theta2;
+    float t = 2.0f * (eta2 * eta2) * kappa2;
+    float B = sqrtf(A * A + t * t);
+    float U = (float)sqrt((double)(A + B) * 0.5); // the reference's 0.5 is a double
+    float V = sqrtf((B - A) * 0.5f);
+    float ec = eta1 * cos_theta_i;
+    float phi_perp_y = 2.0f * eta1 * V * cos_theta_i;
+    float phi_perp_x = U * U + V * V - ec * ec;
+    phi_perp = atan2_fixed(phi_perp_y, phi_perp_x);
+    float phi_par_y = 2.0f * eta1 * (eta2 * eta2) * cos_theta_i * (2.0f * kappa2 * U - (1.0f - kappa2 * kappa2) * V);
+    float px = (eta2 * eta2) * (1.0f + kappa2 * kappa2) * cos_theta_i;
+    float phi_par_x = px * px - (eta1 * eta1) * (U * U + V * V);
+    phi_par = atan2_fixed(phi_par_y, phi_par_x);
+}
+
+// fresnel_conductor (ThinFilm.h:64-88): reflectances of both polarisations.
+PT_HD void thin_film_conductor(float cos_theta_i, float eta, float k, float& Rp2, float& Rs2)
+{
+    float c2 = cos_theta_i * cos_theta_i;
+    float s2 = 1.0f - c2;
+    float temp1 = eta * eta - k * k - s2;
+    float a2pb2 = sqrtf(temp1 * temp1 + 4.0f * k * k * eta * eta);
+    float a = sqrtf(0.5f * (a2pb2 + temp1));
+    float term1 = a2pb2 + c2;
+    float term2 = 2.0f * a * cos_theta_i;
+    Rs2 = clampf_(0.0f, 1.0f, (term1 - term2) / (term1 + term2));
+    float term3 = a2pb2 * c2 + s2 * s2;
+    float term4 = term2 * s2;
+    Rp2 = clampf_(0.0f, 1.0f, Rs2 * (term3 - term4) / (term3 + term4));
+}
+
+// RGB_hue_shift (ThinFilm.h:93-124): rotation about the grey axis.
+PT_HD Color rgb_hue_shift(Color c, float degrees)
+{
+    if (degrees == 0.0f)
+        return c;
+    float cosA = cos_fixed(degrees / 180.0f * PI_F);
+    float sinA = sin_fixed(degrees / 180.0f * PI_F);
+    const float sqrt_1_3 = 0.57735026918962576451f;
+    const float one_over_3 = 1.0f / 3.0f;
+    float m00 = cosA + (1.0f - cosA) / 3.0f;
+    float m01 = one_over_3 * (1.0f - cosA) - sqrt_1_3 * sinA;
+    float m02 = one_over_3 * (1.0f - cosA) + sqrt_1_3 * sinA;
+    float m10 = one_over_3 * (1.0f - cosA) + sqrt_1_3 * sinA;
+    float m11 = cosA + one_over_3 * (1.0f - cosA);
+    float m12 = one_over_3 * (1.0f - cosA) - sqrt_1_3 * sinA;
+    float m20 = one_over_3 * (1.0f - cosA) - sqrt_1_3 * sinA;
+    float m21 = one_over_3 * (1.0f - cosA) + sqrt_1_3 * sinA;
+    float m22 = cosA + one_over_3 * (1.0f - cosA);
+    Color h = col(c.r * m00 + c.g * m01 + c.b * m02, c.r * m10 + c.g * m11 + c.b * m12,
+                  c.r * m20 + c.g * m21 + c.b * m22);
+    return col(clampf_(0.0f, 1.0f, h.r), clampf_(0.0f, 1.0f, h.g), clampf_(0.0f, 1.0f, h.b));
+}
+
+// thin_film_fresnel (ThinFilm.h:131-226): an airy summation of the two
+// interfaces ambient | film | base, both polarisations, two orders of the
+// spectral terms, then XYZ -> sRGB.
+PT_HD Color thin_film_fresnel(const Material& m, float ambient_ior, float HoL)
+{
+    if (m.thin_film == 0.0f)
+        return col(0.0f);
+    const bool over = (m.flags & MAT_THIN_FILM_IOR_OVERRIDE) != 0;
+    float eta1 = ambient_ior;
+    float eta2 = m.thin_film_ior;
+    float eta3 = over ? m.thin_film_base_ior_override : m.ior;
+    float kappa3 = over ? m.thin_film_kappa_3 : 0.0f;
+    float R12p = 0.0f, R12s = 0.0f, T121p = 0.0f, T121s = 0.0f, R23p = 0.0f, R23s = 0.0f;
+    float cos_theta_2 = 0.0f;
+    float e12 = eta1 / eta2;
+    float ct2 = 1.0f - (1.0f - HoL * HoL) * (e12 * e12);
+    if (ct2 <= 0.0f)
+    {
+        R12s = 1.0f;
+        R12p = 1.0f;
+        T121p = 0.0f;
+        T121s = 0.0f;
+    }
+    else
+    {
+        cos_theta_2 = sqrtf(ct2);
+        thin_film_conductor(HoL, eta2 / eta1, 0.0f, R12p, R12s);
+        thin_film_conductor(cos_theta_2, eta3 / eta2, kappa3, R23p, R23s);
+        T121p = 1.0f - R12p;
+        T121s = 1.0f - R12s;
+    }
+    float D = m.thin_film_thickness / 1000.0f * cos_theta_2;
+    float phi21p, phi21s, phi23p, phi23s;
+    thin_film_phase(HoL, eta1, eta2, 0.0f, phi21p, phi21s);
+    thin_film_phase(cos_theta_2, eta2, eta3, kappa3, phi23p, phi23s);
+    phi21p = PI_F - phi21p;
+    phi21s = PI_F - phi21s;
+    float r123p = sqrtf(R12p * R23p);
+    float r123s = sqrtf(R12s * R23s);
+    float Rs = (T121p * T121p * R23p) / (1.0f - R12p * R23p);
+    Color I = col(R12p + Rs);
+    float Cm = Rs - T121p;
+    for (int k = 1; k <= 2; k++)
+    {
+        Cm *= r123p;
+        Color Sm = 2.0f * thin_film_sensitivity((float)k * D, (float)k * (phi23p + phi21p));
+        I = I + Cm * Sm;
+    }
+    Rs = (T121s * T121s * R23s) / (1.0f - R12s * R23s);
+    I = I + col(R12s + Rs);
+    Cm = Rs - T121s;
+    for (int k = 1; k <= 2; k++)
+    {
+        Cm *= r123s;
+        Color Sm = 2.0f * thin_film_sensitivity((float)k * D, (float)k * (phi23s + phi21s));
+        I = I + Cm * Sm;
+    }
+    I = I * 0.5f;
+    float r = 2.3646381f * I.r - 0.8965361f * I.g - 0.4680737f * I.b;
+    float gg = -0.5151664f * I.r + 1.4264000f * I.g + 0.0887608f * I.b;
+    float b = 0.0052037f * I.r - 0.0144081f * I.g + 1.0092106f * I.b;
+    I = col(clampf_(0.0f, 1.0f, r), clampf_(0.0f, 1.0f, gg), clampf_(0.0f, 1.0f, b));
+    return rgb_hue_shift(I, m.thin_film_hue_shift * 360.0f);
+}
+
 PT_HD Color cdiv(float a, Color b) { return col(a / b.r, a / b.g, a / b.b); }
 PT_HD bool is_white(Color c) { return c.r == 1.0f && c.g == 1.0f && c.b == 1.0f; }
 
@@ -513,11 +667,16 @@ PT_HD float specular_relative_ior(const Material& m, float incident_medium_ior)
     return relative_ior;
 }
 
-// Principled.h:215-241 with thin_film = 0 (thin-film materials are refused):
-// lerp(F, F_thin_film, 0) is F.
+// Principled.h:215-241: the dielectric Fresnel lerped with the thin-film one
+// (seen from the layer above the specular layer).
 PT_HD Color specular_fresnel(const Material& m, float relative_specular_ior, float cos_theta_i)
 {
-    return col(fresnel_dielectric(cos_theta_i, relative_specular_ior));
+    float layer_above_ior = m.ior / relative_specular_ior;
+    Color F_specular = col(0.0f);
+    if (m.thin_film < 1.0f)
+        F_specular = col(fresnel_dielectric(cos_theta_i, relative_specular_ior));
+    Color F_thin_film = thin_film_fresnel(m, layer_above_ior, cos_theta_i);
+    return lerpc(F_specular, F_thin_film, m.thin_film);
 }
 
 // PrincipledEnergyCompensation.h:15-60 (roughness threshold 0, no bounce limit).
@@ -961,16 +1120,21 @@ PT_HD void glass_etas(const BsdfGlobals& g, const Material& m, const VolumeState
         relative_eta = 1.0f + 1.0e-5f;
 }
 
-// Fresnel reflectance of the glass interface and the probability of sampling
-// the reflection (its luminance). Thin-film materials are refused, so
-// lerp(F, F_thin_film, 0) is F. A thin interface adds the inter-reflections
-// between its two sides (PBR book 4th ed., dielectric BSDF).
-PT_HD float glass_fresnel(bool thin_walled, float scaled_roughness, float relative_eta, float HoV, float& reflect_proba)
+// Fresnel reflectance of the glass interface (lerped with the thin-film one)
+// and the probability of sampling the reflection (its luminance). A thin
+// interface without film adds the inter-reflections between its two sides
+// (PBR book 4th ed., dielectric BSDF).
+PT_HD Color glass_fresnel(const Material& m, bool thin_walled, float scaled_roughness, float eta_i, float relative_eta,
+                          float HoV, float& reflect_proba)
 {
-    float F = fresnel_dielectric(HoV, relative_eta);
-    if (thin_walled && F < 1.0f && scaled_roughness < 0.1f)
-        F = F + (1.0f - F) * (1.0f - F) * F / (1.0f - F * F);
-    reflect_proba = luminance(col(F));
+    Color F_thin_film = thin_film_fresnel(m, eta_i, HoV);
+    Color F_plain = col(0.0f);
+    if (m.thin_film < 1.0f)
+        F_plain = col(fresnel_dielectric(HoV, relative_eta));
+    Color F = lerpc(F_plain, F_thin_film, m.thin_film);
+    if (thin_walled && F.r < 1.0f && m.thin_film == 0.0f && scaled_roughness < 0.1f)
+        F = F + col((1.0f - F.r) * (1.0f - F.r) * F.r / (1.0f - F.r * F.r));
+    reflect_proba = luminance(F);
     return F;
 }
 
@@ -1015,7 +1179,7 @@ PT_HD Color glass_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c
     if (HoL * NoL < 0.0f || HoV * NoV < 0.0f)
         return col(0.0f);
     float reflect_proba;
-    float F = glass_fresnel(thin, scaled, relative_eta, HoV, reflect_proba);
+    Color F = glass_fresnel(m, thin, scaled, eta_i, relative_eta, HoV, reflect_proba);
     Color color;
     if (reflecting)
     {
@@ -1025,7 +1189,7 @@ PT_HD Color glass_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c
         else if (g.regularize && c.reg == REGULARIZATION_CLASSIC)
             rr = regularize_reflection(g, c.reg, scaled, c.path_roughness);
         DeltaSampled ds = delta_reflection_sampled(m, scaled, m.anisotropy, c.info);
-        color = ggx_eval_reflect_ss(rr, m.anisotropy, col(F), lv, ll, lh, pdf, ds);
+        color = ggx_eval_reflect_ss(rr, m.anisotropy, F, lv, ll, lh, pdf, ds);
         float compensation = glass_energy_compensation(g, m, m.roughness, v.inside != 0, eta_t, eta_i, relative_eta,
                                                        lv.z);
         color = color / compensation;
@@ -1038,7 +1202,7 @@ PT_HD Color glass_eval(const BsdfGlobals& g, const Material& m, const BsdfCtx& c
             rr = regularize_mix(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
         else if (g.regularize && c.reg == REGULARIZATION_CLASSIC)
             rr = regularize_refraction(g, c.reg, scaled, c.path_roughness, eta_i, eta_t);
-        color = ggx_eval_refract(m, rr, relative_eta, col(F), lv, ll, lh, pdf, c.info);
+        color = ggx_eval_refract(m, rr, relative_eta, F, lv, ll, lh, pdf, c.info);
         pdf *= 1.0f - reflect_proba;
         float compensation = glass_energy_compensation(g, m, rr, v.inside != 0, eta_t, eta_i, relative_eta, lv.z);
         color = color / compensation;
@@ -1078,7 +1242,7 @@ PT_HD float glass_pdf(const BsdfGlobals& g, const Material& m, const BsdfCtx& c,
     if (HoL * NoL < 0.0f || HoV * NoV < 0.0f)
         return 0.0f;
     float reflect_proba;
-    glass_fresnel(thin, scaled, relative_eta, HoV, reflect_proba);
+    glass_fresnel(m, thin, scaled, eta_i, relative_eta, HoV, reflect_proba);
     float pdf;
     if (reflecting)
     {
@@ -1121,7 +1285,7 @@ PT_HD f3 glass_sample(const BsdfGlobals& g, const Material& m, BsdfCtx& c, f3 lv
     f3 mn = (ax <= ROUGHNESS_CLAMP && ay <= ROUGHNESS_CLAMP) ? mk(0.0f, 0.0f, 1.0f) : ggx_vndf_sample(lv, ax, ay, rng);
     float HoV = dot(lv, mn);
     float reflect_proba;
-    glass_fresnel(thin, scaled, relative_eta, HoV, reflect_proba);
+    glass_fresnel(m, thin, scaled, eta_i, relative_eta, HoV, reflect_proba);
     float rand_1 = rng();
     if (rand_1 < reflect_proba)
     {
@@ -1322,12 +1486,13 @@ PT_HD float pdf_sheen_layer(const BsdfGlobals& g, const Material& m, const Local
 
 // Principled.h:1129-1157 (the metal lobe has nothing below it).
 PT_HD Color eval_metal_layer(const BsdfGlobals& g, const Material& m, const BsdfCtx& c, const LocalFrames& f,
-                             float roughness, float weight, float proba, const Color& layers, float& pdf)
+                             float roughness, float incident_ior, float weight, float proba, const Color& layers,
+                             float& pdf)
 {
     if (weight > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
     {
         float mpdf = 0.0f;
-        Color contribution = metallic_eval(g, m, c, roughness, m.anisotropy, f.lvr, f.llr, f.lhr, mpdf);
+        Color contribution = metallic_eval(g, m, c, roughness, m.anisotropy, incident_ior, f.lvr, f.llr, f.lhr, mpdf);
         contribution = contribution * weight;
         contribution = contribution * layers;
         pdf += mpdf * proba;
@@ -1426,10 +1591,10 @@ PT_HD Color principled_eval(const BsdfGlobals& g, const Material& m, const BsdfC
     Color final_color = col(0.0f);
     final_color = final_color + eval_coat_layer(g, m, c, f, incident_ior, refracting, w.coat, p.coat, layers, pdf);
     final_color = final_color + eval_sheen_layer(g, m, f, refracting, w.sheen, p.sheen, layers, pdf);
-    final_color = final_color + eval_metal_layer(g, m, c, f, m.roughness, w.metal1 * (float)!refracting, p.metal1,
-                                                 layers, pdf);
-    final_color = final_color + eval_metal_layer(g, m, c, f, m.second_roughness, w.metal2 * (float)!refracting,
-                                                 p.metal2, layers, pdf);
+    final_color = final_color + eval_metal_layer(g, m, c, f, m.roughness, incident_ior, w.metal1 * (float)!refracting,
+                                                 p.metal1, layers, pdf);
+    final_color = final_color + eval_metal_layer(g, m, c, f, m.second_roughness, incident_ior,
+                                                 w.metal2 * (float)!refracting, p.metal2, layers, pdf);
     // glass layer (Principled.h:1177-1200), rotated frame
     if (w.glass > 0.0f)
     {

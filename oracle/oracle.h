@@ -307,6 +307,9 @@ void orc_vol_script(const int32_t* mats, const int32_t* prios, int32_t n, int32_
 float orc_glass_correction_exponent(const float* table, float roughness, float relative_eta);
 void orc_wavelength_rgb(float w, float* out);
 float orc_dispersion_ior(float abbe, float scale, float ior, float wavelength);
+/* thin_film_fresnel (ThinFilm.h:131-226) of a material, RGB */
+void orc_thin_film_fresnel(const orc_cpu_material* m, float ambient_ior, float HoL, float* out);
+void orc_hue_shift(const float* in, float degrees, float* out);
 uint32_t orc_oct_pack(const float* n);
 /* fmath.h functions element-wise: fn 0 sin, 1 cos, 2 pow(x, y), 3 log2, 4 exp2. */
 void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n);

@@ -116,6 +116,8 @@ _SIG = {
     "orc_glass_correction_exponent": (C.c_float, [_P, C.c_float, C.c_float]),
     "orc_wavelength_rgb": (None, [C.c_float, _P]),
     "orc_dispersion_ior": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
+    "orc_thin_film_fresnel": (None, [_P, C.c_float, C.c_float, _P]),
+    "orc_hue_shift": (None, [_P, C.c_float, _P]),
 }
 
 _lib = None
@@ -497,6 +499,20 @@ VOL_AIR = 0xffffffff >> 6
 def glass_correction_exponent(roughness, relative_eta):
     tab = load_luts()["glass_exponent"]
     return float(lib().orc_glass_correction_exponent(_p(tab), roughness, relative_eta))
+
+
+def thin_film_fresnel(material, ambient_ior, cos_theta):
+    out = np.zeros(3, np.float32)
+    buf = (C.c_char * C.sizeof(material)).from_buffer_copy(bytes(material))
+    lib().orc_thin_film_fresnel(C.cast(buf, C.c_void_p), ambient_ior, cos_theta, _p(out))
+    return out
+
+
+def hue_shift(rgb, degrees):
+    a = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_hue_shift(_p(a), degrees, _p(out))
+    return out
 
 
 def dispersion_ior(abbe, scale, ior, wavelength):

@@ -49,6 +49,7 @@ static inline float hlerp(float a, float b, float t) { return (1.0f - t) * a + t
 static inline float cmax(rgb c) { return hmax(c.r, hmax(c.g, c.b)); } /* Color.h:98 */
 static inline int cblack(rgb c) { return !(c.r > 0.0f || c.g > 0.0f || c.b > 0.0f); } /* Color.h:95 */
 static inline float lum(rgb c) { return 0.3086f * c.r + 0.6094f * c.g + 0.0820f * c.b; } /* Color.h:90 */
+static inline rgb clerp(rgb a, rgb b, float t) { return cadd(cscale(a, 1.0f - t), cscale(b, t)); } /* Math.h lerp */
 
 #define M_PI_F 3.14159265358979323846f
 #define M_TWO_PI_F 6.28318530717958647693f
@@ -77,6 +78,8 @@ typedef struct {
     rgb absorption_color;
     float dispersion_scale, dispersion_abbe_number;
     int dielectric_priority;
+    float tf_ior, tf_thickness, tf_kappa3, tf_hue, tf_base_ior; /* thin film; hue = 8-bit degrees / 360 */
+    int tf_override;
 } omat;
 
 static unsigned char q8(float x) { return (unsigned char)(x * 255.0f); }
@@ -129,6 +132,8 @@ static orc_cpu_material make_safe(orc_cpu_material m)
     m.dispersion_abbe_number = hmax(1.0e-5f, m.dispersion_abbe_number);
     m.dispersion_scale = hclamp(0.0f, 1.0f, m.dispersion_scale);
     m.thin_film = hclamp(0.0f, 1.0f, m.thin_film);
+    m.thin_film_hue_shift_degrees = hclamp(0.0f, 360.0f, m.thin_film_hue_shift_degrees);
+    m.thin_film_ior = hmax(1.0005f, m.thin_film_ior);
     m.dielectric_priority = m.dielectric_priority < 0 ? 0 : m.dielectric_priority > 15 ? 15 : m.dielectric_priority;
     for (int k = 0; k < 3; k++)
         m.emission[k] = hmax(0.0f, m.emission[k]);
@@ -185,6 +190,12 @@ static omat unpack_material(const orc_cpu_material* in)
     o.dispersion_scale = u8f(q8(m->dispersion_scale));
     o.dispersion_abbe_number = m->dispersion_abbe_number;
     o.dielectric_priority = (unsigned char)m->dielectric_priority;
+    o.tf_ior = m->thin_film_ior;
+    o.tf_thickness = m->thin_film_thickness;
+    o.tf_kappa3 = m->thin_film_kappa_3;
+    o.tf_hue = u8f(q8(m->thin_film_hue_shift_degrees / 360.0f));
+    o.tf_base_ior = m->thin_film_base_ior_override;
+    o.tf_override = m->thin_film_do_ior_override != 0;
     /* get_intersection_material (Material.h:122-149): a coat roughens the
      * base and second metal roughness */
     if (o.coat > 0.0f && o.coat_roughening > 0.0f)
@@ -563,7 +574,118 @@ static void vol_pop(ovol* v, int inside)
 
 typedef struct { v3 view, sn, gn, L; int info, reg; float path_r; ovol* vs; int update; } bctx;
 
-static rgb metallic_eval(const rctx* R, const omat* m, const bctx* c, float r, v3 lv, v3 ll, v3 lh, float* pdf)
+/* ------------------------------------------------------------------ */
+/* Thin-film interference (ThinFilm.h:13-226; Belcour & Barla 2017)      */
+/* ------------------------------------------------------------------ */
+static rgb tf_sensitivity(float opd, float shift)
+{
+    float phase = 2.0f * M_PI_F * opd * 1.0e-6f;
+    static const float val[3] = {5.4856e-13f, 4.4201e-13f, 5.2481e-13f};
+    static const float pos[3] = {1.6810e+06f, 1.7953e+06f, 2.2084e+06f};
+    static const float var[3] = {4.3278e+09f, 9.3046e+09f, 6.6121e+09f};
+    float x[3];
+    for (int k = 0; k < 3; k++)
+        x[k] = val[k] * sqrtf(2.0f * M_PI_F * var[k]) * orc_cosf(pos[k] * phase + shift) *
+               orc_expf(-1.0f * var[k] * phase * phase);
+    x[0] += 9.7470e-14f * sqrtf(2.0f * M_PI_F * 4.5282e+09f) * orc_cosf(2.2399e+06f * phase + shift) *
+            orc_expf(-4.5282e+09f * phase * phase);
+    return C3(x[0] / 1.0685e-7f, x[1] / 1.0685e-7f, x[2] / 1.0685e-7f);
+}
+
+static void tf_phase(float ci, float e1, float e2, float k2, float* par, float* perp)
+{
+    float s2 = 1.0f - ci * ci;
+    float A = (e2 * e2) * (1.0f - k2 * k2) - (e1 * e1) * s2;
+    float t = 2.0f * (e2 * e2) * k2;
+    float B = sqrtf(A * A + t * t);
+    float U = (float)sqrt((double)(A + B) * 0.5);
+    float V = sqrtf((B - A) * 0.5f);
+    float ec = e1 * ci;
+    *perp = orc_atan2f(2.0f * e1 * V * ci, U * U + V * V - ec * ec);
+    float px = (e2 * e2) * (1.0f + k2 * k2) * ci;
+    *par = orc_atan2f(2.0f * e1 * (e2 * e2) * ci * (2.0f * k2 * U - (1.0f - k2 * k2) * V),
+                      px * px - (e1 * e1) * (U * U + V * V));
+}
+
+static void tf_conductor(float ci, float eta, float k, float* Rp, float* Rs)
+{
+    float c2 = ci * ci, s2 = 1.0f - c2;
+    float t1 = eta * eta - k * k - s2;
+    float ab = sqrtf(t1 * t1 + 4.0f * k * k * eta * eta);
+    float a = sqrtf(0.5f * (ab + t1));
+    float u1 = ab + c2, u2 = 2.0f * a * ci;
+    *Rs = hclamp(0.0f, 1.0f, (u1 - u2) / (u1 + u2));
+    float u3 = ab * c2 + s2 * s2, u4 = u2 * s2;
+    *Rp = hclamp(0.0f, 1.0f, *Rs * (u3 - u4) / (u3 + u4));
+}
+
+static rgb hue_shift(rgb c, float deg)
+{
+    if (deg == 0.0f)
+        return c;
+    float ca = orc_cosf(deg / 180.0f * M_PI_F), sa = orc_sinf(deg / 180.0f * M_PI_F);
+    const float s13 = 0.57735026918962576451f, o3 = 1.0f / 3.0f;
+    float a00 = ca + (1.0f - ca) / 3.0f, a01 = o3 * (1.0f - ca) - s13 * sa, a02 = o3 * (1.0f - ca) + s13 * sa;
+    float a10 = o3 * (1.0f - ca) + s13 * sa, a11 = ca + o3 * (1.0f - ca), a12 = o3 * (1.0f - ca) - s13 * sa;
+    float a20 = o3 * (1.0f - ca) - s13 * sa, a21 = o3 * (1.0f - ca) + s13 * sa, a22 = ca + o3 * (1.0f - ca);
+    return C3(hclamp(0.0f, 1.0f, c.r * a00 + c.g * a01 + c.b * a02), hclamp(0.0f, 1.0f, c.r * a10 + c.g * a11 + c.b * a12),
+              hclamp(0.0f, 1.0f, c.r * a20 + c.g * a21 + c.b * a22));
+}
+
+static rgb thin_film(const omat* m, float amb, float HoL)
+{
+    if (m->thin_film == 0.0f)
+        return C1(0.0f);
+    float e1 = amb, e2 = m->tf_ior, e3 = m->tf_override ? m->tf_base_ior : m->ior;
+    float k3 = m->tf_override ? m->tf_kappa3 : 0.0f;
+    float R12p = 0, R12s = 0, T121p = 0, T121s = 0, R23p = 0, R23s = 0, c2 = 0.0f;
+    float r = e1 / e2;
+    float ct2 = 1.0f - (1.0f - HoL * HoL) * (r * r);
+    if (ct2 <= 0.0f)
+    {
+        R12s = R12p = 1.0f;
+    }
+    else
+    {
+        c2 = sqrtf(ct2);
+        tf_conductor(HoL, e2 / e1, 0.0f, &R12p, &R12s);
+        tf_conductor(c2, e3 / e2, k3, &R23p, &R23s);
+        T121p = 1.0f - R12p;
+        T121s = 1.0f - R12s;
+    }
+    float D = m->tf_thickness / 1000.0f * c2;
+    float p21p, p21s, p23p, p23s;
+    tf_phase(HoL, e1, e2, 0.0f, &p21p, &p21s);
+    tf_phase(c2, e2, e3, k3, &p23p, &p23s);
+    p21p = M_PI_F - p21p;
+    p21s = M_PI_F - p21s;
+    float rp = sqrtf(R12p * R23p), rs = sqrtf(R12s * R23s);
+    float Rsum = (T121p * T121p * R23p) / (1.0f - R12p * R23p);
+    rgb I = C1(R12p + Rsum);
+    float Cm = Rsum - T121p;
+    for (int k = 1; k <= 2; k++)
+    {
+        Cm *= rp;
+        I = cadd(I, cscale(cscale(tf_sensitivity((float)k * D, (float)k * (p23p + p21p)), 2.0f), Cm));
+    }
+    Rsum = (T121s * T121s * R23s) / (1.0f - R12s * R23s);
+    I = cadd(I, C1(R12s + Rsum));
+    Cm = Rsum - T121s;
+    for (int k = 1; k <= 2; k++)
+    {
+        Cm *= rs;
+        I = cadd(I, cscale(cscale(tf_sensitivity((float)k * D, (float)k * (p23s + p21s)), 2.0f), Cm));
+    }
+    I = cscale(I, 0.5f);
+    float rr = 2.3646381f * I.r - 0.8965361f * I.g - 0.4680737f * I.b;
+    float gg = -0.5151664f * I.r + 1.4264000f * I.g + 0.0887608f * I.b;
+    float bb = 0.0052037f * I.r - 0.0144081f * I.g + 1.0092106f * I.b;
+    I = C3(hclamp(0.0f, 1.0f, rr), hclamp(0.0f, 1.0f, gg), hclamp(0.0f, 1.0f, bb));
+    return hue_shift(I, m->tf_hue * 360.0f);
+}
+
+static rgb metallic_eval(const rctx* R, const omat* m, const bctx* c, float r, float inc, v3 lv, v3 ll, v3 lh,
+                         float* pdf)
 {
     float rr = regularize(R, c->reg, r, c->path_r);
     int ds = delta_sampled(m, rr, m->anisotropy, c->info);
@@ -573,7 +695,7 @@ static rgb metallic_eval(const rctx* R, const omat* m, const bctx* c, float r, v
         return C1(0.0f);
     }
     float HoL = hclamp(1.0e-8f, 1.0f, vdot(lh, ll));
-    rgb F = f82(m->base_color, m->F82, m->F90, m->F90_exp, HoL);
+    rgb F = clerp(f82(m->base_color, m->F82, m->F90, m->F90_exp, HoL), thin_film(m, inc, HoL), m->thin_film);
     rgb ms = ec_conductor(R, F, rr, m->do_metallic_ec, lv);
     rgb ss = ggx_reflect_ss(rr, m->anisotropy, F, lv, ll, lh, pdf, ds);
     return cmul(ss, ms);
@@ -631,7 +753,6 @@ static float hemi_albedo_fit(float rel)
     return orc_logf((10893.0f * rel - 1438.2f) / (-774.4f * (rel * rel) + 10212.0f * rel + 1.0f));
 }
 static rgb fdivc(float a, rgb b) { return C3(a / b.r, a / b.g, a / b.b); }
-static rgb clerp(rgb a, rgb b, float t) { return cadd(cscale(a, 1.0f - t), cscale(b, t)); }
 
 /* Stacked 3D LUT through the CPU nearest sampler (Image.cpp:761-803) */
 static float lut3d(const float* lut, int w, int h, int d, float u, float v, float t)
@@ -656,6 +777,13 @@ static float spec_rel_ior(const omat* m, float inc)
     float layer = hlerp(inc, m->coat_ior, m->coat);
     float rel = m->ior / layer;
     return rel < 1.0f ? 1.0f / rel : rel;
+}
+
+/* principled_specular_fresnel (Principled.h:215-241) */
+static rgb spec_fresnel(const omat* m, float rel, float cos_i)
+{
+    rgb Fs = m->thin_film < 1.0f ? C1(fresnel_d(cos_i, rel)) : C1(0.0f);
+    return clerp(Fs, thin_film(m, m->ior / rel, cos_i), m->thin_film);
 }
 
 static float ec_glossy(const rctx* R, const omat* m, float inc, float NoV)
@@ -1010,12 +1138,13 @@ static void glass_etas(const rctx* R, const omat* m, const ovol* v, float* ei, f
         *eta = 1.0f + 1.0e-5f;
 }
 
-static float glass_F(int thin, float r, float eta, float HoV, float* prob)
+static rgb glass_F(const omat* m, float r, float ei, float eta, float HoV, float* prob)
 {
-    float F = fresnel_d(HoV, eta);
-    if (thin && F < 1.0f && r < 0.1f)
-        F = F + (1.0f - F) * (1.0f - F) * F / (1.0f - F * F);
-    *prob = lum(C1(F));
+    rgb tf = thin_film(m, ei, HoV);
+    rgb F = clerp(m->thin_film < 1.0f ? C1(fresnel_d(HoV, eta)) : C1(0.0f), tf, m->thin_film);
+    if (m->thin_walled && F.r < 1.0f && m->thin_film == 0.0f && r < 0.1f)
+        F = cadd(F, C1((1.0f - F.r) * (1.0f - F.r) * F.r / (1.0f - F.r * F.r)));
+    *prob = lum(F);
     return F;
 }
 
@@ -1043,20 +1172,21 @@ static rgb glass_eval(const rctx* R, const omat* m, const bctx* c, v3 lv, v3 ll,
     float HoL = vdot(ll, h), HoV = vdot(lv, h);
     if (HoL * ll.z < 0.0f || HoV * lv.z < 0.0f)
         return C1(0.0f);
-    float prob, F = glass_F(m->thin_walled, sr, eta, HoV, &prob);
+    float prob;
+    rgb F = glass_F(m, sr, ei, eta, HoV, &prob);
     int reg = R->s.do_microfacet_regularization;
     if (refl)
     {
         float rr = reg && c->reg == REG_MIS ? reg_mix(R, c->reg, sr, c->path_r, ei, et)
                  : reg && c->reg == REG_CLASSIC ? regularize(R, c->reg, sr, c->path_r) : sr;
-        rgb col = ggx_reflect_ss(rr, m->anisotropy, C1(F), lv, ll, h, pdf, delta_sampled(m, sr, m->anisotropy, c->info));
+        rgb col = ggx_reflect_ss(rr, m->anisotropy, F, lv, ll, h, pdf, delta_sampled(m, sr, m->anisotropy, c->info));
         col = cdivs(col, ec_glass(R, m, m->roughness, v->inside, et, ei, eta, lv.z));
         *pdf *= prob;
         return col;
     }
     float rr = reg && c->reg == REG_MIS ? reg_mix(R, c->reg, sr, c->path_r, ei, et)
              : reg && c->reg == REG_CLASSIC ? reg_refract(R, c->reg, sr, c->path_r, ei, et) : sr;
-    rgb col = ggx_refract(m, rr, eta, C1(F), lv, ll, h, pdf, c->info);
+    rgb col = ggx_refract(m, rr, eta, F, lv, ll, h, pdf, c->info);
     *pdf *= 1.0f - prob;
     col = cdivs(col, ec_glass(R, m, rr, v->inside, et, ei, eta, lv.z));
     if (m->thin_walled)
@@ -1089,7 +1219,7 @@ static float glass_pdf(const rctx* R, const omat* m, const bctx* c, v3 lv, v3 ll
     if (HoL * ll.z < 0.0f || HoV * lv.z < 0.0f)
         return 0.0f;
     float prob;
-    glass_F(m->thin_walled, sr, eta, HoV, &prob);
+    glass_F(m, sr, ei, eta, HoV, &prob);
     int reg = R->s.do_microfacet_regularization;
     if (refl)
     {
@@ -1128,7 +1258,7 @@ static v3 glass_sample(const rctx* R, const omat* m, bctx* c, v3 lv, uint32_t* s
     alphas(sr, m->anisotropy, &ax, &ay);
     v3 mn = (ax <= 1.0e-4f && ay <= 1.0e-4f) ? V(0, 0, 1) : vndf_sample(lv, ax, ay, st);
     float prob;
-    glass_F(m->thin_walled, sr, eta, vdot(lv, mn), &prob);
+    glass_F(m, sr, ei, eta, vdot(lv, mn), &prob);
     if (orc_rand01(st) < prob)
     {
         c->info = FROM_GLASS_REFLECT;
@@ -1237,7 +1367,7 @@ static rgb principled_eval(const rctx* R, const omat* m, const bctx* c, float* p
         if (wm[k] > 0.0f && f.lvr.z > 0.0f && f.llr.z > 0.0f)
         {
             float mp = 0.0f;
-            contrib = metallic_eval(R, m, c, rm[k], f.lvr, f.llr, f.lhr, &mp);
+            contrib = metallic_eval(R, m, c, rm[k], inc, f.lvr, f.llr, f.lhr, &mp);
             contrib = cmul(cscale(contrib, wm[k]), layers);
             *pdf += mp * pm[k];
         }
@@ -1257,14 +1387,14 @@ static rgb principled_eval(const rctx* R, const omat* m, const bctx* c, float* p
     {
         float rel = spec_rel_ior(m, inc);
         float sp = 0.0f;
-        rgb F = C1(fresnel_d(vdot(f.llr, f.lhr), rel));
+        rgb F = spec_fresnel(m, rel, vdot(f.llr, f.lhr));
         rgb contrib = ggx_dielectric(R, m, c, m->roughness, m->anisotropy, F, f.lvr, f.llr, f.lhr, &sp);
         contrib = cmul(contrib, clerp(C1(1.0f), cscale(m->specular_color, m->specular_tint), m->specular));
         contrib = cmul(cscale(contrib, w.spec), layers);
         rgb below = C1(1.0f);
-        rgb lf = cscale(C1(fresnel_d(fabsf(f.llr.z), rel)), 1.0f - m->diffuse_transmission);
+        rgb lf = cscale(spec_fresnel(m, rel, fabsf(f.llr.z)), 1.0f - m->diffuse_transmission);
         below = cmul(below, csub(C1(1.0f), lf));
-        rgb vfc = C1(fresnel_d(fabsf(f.lvr.z), rel));
+        rgb vfc = spec_fresnel(m, rel, fabsf(f.lvr.z));
         below = cmul(below, csub(C1(1.0f), vfc));
         below = cmul(below, spec_darkening(m, rel, lum(vfc)));
         below = clerp(C1(1.0f), below, m->specular);
@@ -2020,6 +2150,19 @@ void orc_vol_script(const int32_t* mats, const int32_t* prios, int32_t n, int32_
 float orc_glass_correction_exponent(const float* table, float roughness, float relative_eta)
 {
     return correction_exponent(table, roughness, relative_eta);
+}
+
+void orc_thin_film_fresnel(const orc_cpu_material* cm, float ambient_ior, float HoL, float* out)
+{
+    omat m = unpack_material(cm);
+    rgb F = thin_film(&m, ambient_ior, HoL);
+    out[0] = F.r; out[1] = F.g; out[2] = F.b;
+}
+
+void orc_hue_shift(const float* in, float degrees, float* out)
+{
+    rgb c = hue_shift(C3(in[0], in[1], in[2]), degrees);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
 }
 
 float orc_dispersion_ior(float abbe, float scale, float ior, float wavelength)

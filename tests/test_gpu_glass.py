@@ -138,6 +138,29 @@ def test_restir_with_dispersion_is_refused(mipt_mod, torch_mod):
     ctx.close()
 
 
+def test_config1_thin_film(mipt_mod, oracle_mod, torch_mod):
+    """Thin-film interference on a metal, a dielectric specular layer, a
+    coated base and a glass sphere (hue shift, IOR override with kappa, partial
+    film weight), 6 bounces."""
+    scene = nested_glass_scene(mipt_mod, 96, 96, dispersive=False)
+    mats = scene.materials
+    mi = scene.material_indices
+    sphere = [k for k in range(len(mats)) if (mi == k).sum() == 5120]
+    films = [(sphere[6], dict(metallic=1.0, roughness=0.3, thin_film=1.0, thin_film_thickness=420.0,
+                              thin_film_ior=1.4, thin_film_hue_shift_degrees=40.0)),
+             (sphere[7], dict(metallic=0.0, specular=1.0, roughness=0.2, thin_film=0.7, thin_film_thickness=250.0,
+                              thin_film_ior=1.6)),
+             (sphere[8], dict(metallic=0.0, specular=0.8, coat=1.0, thin_film=1.0, thin_film_thickness=600.0,
+                              thin_film_ior=1.3, thin_film_do_ior_override=1, thin_film_base_ior_override=0.2,
+                              thin_film_kappa_3=3.0)),
+             (sphere[2], dict(thin_film=0.5, thin_film_thickness=500.0, thin_film_ior=1.35))]  # thin glass
+    for k, params in films:
+        for a, v in params.items():
+            setattr(mats[k], a, v)
+    s = mipt_mod.default_render_settings(width=96, height=96, nb_bounces=6)
+    _check(mipt_mod, oracle_mod, scene, s, 3)
+
+
 def test_glass_changes_the_image(mipt_mod, torch_mod):
     """The glass spheres transmit: the nested-glass scene differs from the same
     geometry with the glass made opaque, and both stay finite."""

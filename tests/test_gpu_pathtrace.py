@@ -182,18 +182,6 @@ def test_scene_without_emitters_is_black(mipt_mod, oracle_mod, torch_mod):
     assert np.array_equal(prims, cprims) and cfb.max() == 0.0
 
 
-def test_unsupported_material_is_refused(mipt_mod, torch_mod):
-    scene = mipt_mod.generate_scene(2, 64, 36)
-    scene.materials[3].thin_film = 0.5  # thin-film interference is not implemented
-    ctx = mipt_mod.Context(0)
-    ctx.set_scene(scene)
-    ctx.build_bvh()
-    with pytest.raises(mipt_mod.MiptError) as e:
-        ctx.set_render_settings(mipt_mod.default_render_settings(width=64, height=36))
-    assert "do not implement" in str(e.value)
-    ctx.close()
-
-
 def test_framebuffer_is_a_sum(mipt_mod, torch_mod):
     """PathTracing.h:176-217: sample 0 overwrites, later samples add."""
     scene = mipt_mod.generate_scene(1, 64, 64)
