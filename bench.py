@@ -65,7 +65,9 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.frames:
-        return frames_only(args.frames, args.steps)
+        over = {k: v for k, v in (("query_bounce", args.cache_query_bounce),
+                                  ("train_every", args.cache_train_every)) if v is not None}
+        return frames_only(args.frames, args.steps, over)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -205,10 +207,10 @@ def main():
     ctx.close()
 
 
-def frames_only(mode, samples):
+def frames_only(mode, samples, cache_over=None):
     """Config-1 scene at 1920x1080, 4 bounces, uniform NEE + MIS: 3 warm-up and
     `samples` timed calls of mipt_render_sample, plain or with the radiance
-    cache (default settings). Prints one JSON line."""
+    cache (default settings, cache_over replacing some). Prints one JSON line."""
     import torch
 
     import mipt
@@ -220,7 +222,7 @@ def frames_only(mode, samples):
     ctx.build_bvh()
     cache = mode == "cache"
     if cache:
-        ctx.set_radiance_cache(mipt.default_radiance_cache_settings())
+        ctx.set_radiance_cache(mipt.default_radiance_cache_settings(**(cache_over or {})))
     ctx.set_render_settings(mipt.default_render_settings(width=W, height=H, nb_bounces=4, accumulate_fp64=1,
                                                          radiance_cache=int(cache)))
     ctx.set_camera(scene.camera)

@@ -156,10 +156,10 @@ __device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, in
 __device__ __forceinline__ void vol_load(const PtData& d, uint32_t pix, pt::VolumeState& v)
 {
     const uint4 a = d.vol_a[pix], b = d.vol_b[pix];
-    v.entry[0] = pt::vol_entry(pt::VOL_AIR, 0u, true, true);
-    v.entry[1] = a.x;
-    v.entry[2] = a.y;
-    v.entry[3] = a.z;
+    v.e0 = pt::vol_entry(pt::VOL_AIR, 0u, true, true);
+    v.e1 = a.x;
+    v.e2 = a.y;
+    v.e3 = a.z;
     v.wavelength = __uint_as_float(a.w);
     v.pos = (int32_t)(b.x & 0xffu);
     v.inside = (int32_t)(b.x >> 8);
@@ -171,7 +171,7 @@ __device__ __forceinline__ void vol_load(const PtData& d, uint32_t pix, pt::Volu
 
 __device__ __forceinline__ void vol_store(const PtData& d, uint32_t pix, const pt::VolumeState& v)
 {
-    d.vol_a[pix] = make_uint4(v.entry[1], v.entry[2], v.entry[3], __float_as_uint(v.wavelength));
+    d.vol_a[pix] = make_uint4(v.e1, v.e2, v.e3, __float_as_uint(v.wavelength));
     d.vol_b[pix] = make_uint4((uint32_t)v.pos | ((uint32_t)v.inside << 8), (uint32_t)v.incident, (uint32_t)v.outgoing,
                               __float_as_uint(v.distance));
 }

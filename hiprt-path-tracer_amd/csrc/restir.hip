@@ -247,7 +247,7 @@ __device__ __forceinline__ float target_function(const PtData& d, const Surface&
     const float cosine = pt::dot(s.sn, dir);
     if (cosine <= 0.0f)
         return 0.0f;
-    const pt::Material m = d.mats[s.mat];
+    const pt::Material& m = d.mats[s.mat];
     pt::VolumeState vs = surface_volume(d, s);
     pt::BsdfCtx c{s.view, s.sn, s.gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::NO_REGULARIZATION, 0.0f, &vs, 0};
     float bpdf;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void finish_initial(const RsData& r, uint32_t pix, co
     need_vis = 1u;
 }
 
-__global__ void k_ic_start(PtData d, RsData r)
+__global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= r.n)
@@ -335,7 +335,7 @@ __global__ void k_ic_start(PtData d, RsData r)
         return;
     }
     const Surface s = surface_current(d, pix);
-    const pt::Material m = d.mats[s.mat];
+    const pt::Material& m = d.mats[s.mat];
     Res res = res_empty();
     const int32_t nl = r.s.number_of_initial_light_candidates;
     const int32_t nb = r.s.number_of_initial_bsdf_candidates;
@@ -413,7 +413,7 @@ __global__ void k_ic_start(PtData d, RsData r)
 }
 
 // sample_bsdf_candidates (InitialCandidates.h:300-440) after the closest hit.
-__global__ void k_ic_bsdf(PtData d, RsData r)
+__global__ __launch_bounds__(256, 2) void k_ic_bsdf(PtData d, RsData r)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= r.n)
@@ -428,7 +428,7 @@ __global__ void k_ic_bsdf(PtData d, RsData r)
     int32_t cand = __float_as_int(candf);
     pt::Xorshift32 rng{rng_state};
     const Surface s = surface_current(d, pix);
-    const pt::Material m = d.mats[s.mat];
+    const pt::Material& m = d.mats[s.mat];
     const int32_t nl = r.s.number_of_initial_light_candidates;
     const int32_t nb = r.s.number_of_initial_bsdf_candidates;
     const float4 ra = r.rays[2 * (size_t)pix], rb = r.rays[2 * (size_t)pix + 1];
@@ -647,7 +647,7 @@ __device__ __forceinline__ bool similar(const PtData& d, const RsData& r, int32_
     return plane && normal;
 }
 
-__global__ void k_sp_start(PtData d, RsData r)
+__global__ __launch_bounds__(256, 2) void k_sp_start(PtData d, RsData r)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= r.n)

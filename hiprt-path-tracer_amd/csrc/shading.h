@@ -960,17 +960,35 @@ PT_HD float glass_exponent_knot(const float* seg, float roughness)
 // (MicrofacetEnergyCompensation.h:101-631): the exponent of the cos_theta
 // axis of the glass LUTs, interpolated between the two IOR knots around
 // relative_eta.
+// The ten relative IORs the exponent is tabulated at (a switch, not an array:
+// a dynamically indexed local array would live in scratch memory).
+PT_HD float glass_exponent_ior_knot(int k)
+{
+    switch (k)
+    {
+    case 0: return 1.01f;
+    case 1: return 1.02f;
+    case 2: return 1.03f;
+    case 3: return 1.1f;
+    case 4: return 1.2f;
+    case 5: return 1.4f;
+    case 6: return 1.5f;
+    case 7: return 2.0f;
+    case 8: return 2.4f;
+    default: return 3.0f;
+    }
+}
+
 PT_HD float glass_correction_exponent(const float* tab, float roughness, float relative_eta)
 {
     if ((roughness < 1.0e-10f && roughness > -1.0e-10f) || fabsf(1.0f - relative_eta) < 1.0e-3f)
         return 2.5f;
-    const float knots[10] = {1.01f, 1.02f, 1.03f, 1.1f, 1.2f, 1.4f, 1.5f, 2.0f, 2.4f, 3.0f};
     float lower_bound = 1.01f, lower = 2.5f;
     for (int k = 0; k < 9; k++)
     {
-        if (relative_eta > knots[k] && relative_eta <= knots[k + 1])
+        if (relative_eta > glass_exponent_ior_knot(k) && relative_eta <= glass_exponent_ior_knot(k + 1))
         {
-            lower_bound = knots[k];
+            lower_bound = glass_exponent_ior_knot(k);
             lower = glass_exponent_knot(tab + 44 * k, roughness);
             break;
         }
@@ -978,13 +996,13 @@ PT_HD float glass_correction_exponent(const float* tab, float roughness, float r
     int hk = 9;
     for (int k = 0; k < 9; k++)
     {
-        if (relative_eta <= knots[k])
+        if (relative_eta <= glass_exponent_ior_knot(k))
         {
             hk = k;
             break;
         }
     }
-    float higher_bound = knots[hk];
+    float higher_bound = glass_exponent_ior_knot(hk);
     float higher = glass_exponent_knot(tab + 44 * (10 + hk), roughness);
     if (higher_bound == lower_bound)
         return lower;

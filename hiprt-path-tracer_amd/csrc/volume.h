@@ -33,8 +33,10 @@ PT_HD uint32_t vol_entry(uint32_t mat, uint32_t prio, bool odd, bool topmost)
            (prio & VOL_PRIO_MASK);
 }
 
+// The four entries are separate fields read and written through selects: a
+// dynamically indexed array would be placed in scratch memory.
 struct VolumeState {
-    uint32_t entry[VOL_STACK_SIZE];
+    uint32_t e0, e1, e2, e3;
     int32_t pos;       // last valid entry
     int32_t incident;  // medium the ray arrives from: material index or VOL_AIR (-1 before any push)
     int32_t outgoing;  // medium on the other side of the surface
@@ -44,10 +46,18 @@ struct VolumeState {
     int32_t reset;     // set when a refraction or transmission reset the distance (not stored)
 };
 
+PT_HD uint32_t vol_get(const VolumeState& v, int i) { return i == 0 ? v.e0 : i == 1 ? v.e1 : i == 2 ? v.e2 : v.e3; }
+PT_HD void vol_set(VolumeState& v, int i, uint32_t x)
+{
+    v.e0 = i == 0 ? x : v.e0;
+    v.e1 = i == 1 ? x : v.e1;
+    v.e2 = i == 2 ? x : v.e2;
+    v.e3 = i == 3 ? x : v.e3;
+}
+
 PT_HD void vol_init(VolumeState& v)
 {
-    for (int i = 0; i < VOL_STACK_SIZE; i++)
-        v.entry[i] = vol_entry(VOL_AIR, 0u, true, true);
+    v.e0 = v.e1 = v.e2 = v.e3 = vol_entry(VOL_AIR, 0u, true, true);
     v.pos = 0;
     v.incident = -1;
     v.outgoing = -1;
@@ -68,25 +78,26 @@ PT_HD bool vol_push(VolumeState& v, int32_t mat, int32_t priority)
     int last = v.pos;
     for (; last >= 0; last--)
     {
-        const uint32_t e = v.entry[last];
+        const uint32_t e = vol_get(v, last);
         if (vol_mat(e) != m && vol_topmost(e) && vol_odd(e))
             break;
     }
     bool odd = true;
     for (int i = v.pos; i >= 0; i--)
     {
-        if (vol_mat(v.entry[i]) == m)
+        const uint32_t e = vol_get(v, i);
+        if (vol_mat(e) == m)
         {
-            v.entry[i] &= ~VOL_TOPMOST;
-            odd = !vol_odd(v.entry[i]);
+            vol_set(v, i, e & ~VOL_TOPMOST);
+            odd = !vol_odd(e);
             break;
         }
     }
     v.inside = odd ? 0 : 1;
     if (v.pos < VOL_STACK_SIZE - 1)
         v.pos++;
-    v.entry[v.pos] = vol_entry(m, (uint32_t)priority, odd, true);
-    const uint32_t last_entry = v.entry[last];
+    vol_set(v, v.pos, vol_entry(m, (uint32_t)priority, odd, true));
+    const uint32_t last_entry = vol_get(v, last);
     if (priority < (int32_t)vol_prio(last_entry))
         return true;
     if (odd)
@@ -104,26 +115,27 @@ PT_HD bool vol_push(VolumeState& v, int32_t mat, int32_t priority)
 
 PT_HD void vol_pop(VolumeState& v, bool inside)
 {
-    const uint32_t top = vol_mat(v.entry[v.pos]);
+    const uint32_t top = vol_mat(vol_get(v, v.pos));
     if (v.pos > 0)
         v.pos--;
     if (inside)
     {
         int prev = v.pos;
         for (; prev >= 0; prev--)
-            if (vol_mat(v.entry[prev]) == top)
+            if (vol_mat(vol_get(v, prev)) == top)
                 break;
         if (prev >= 0)
             for (int i = prev + 1; i <= v.pos; i++)
-                v.entry[i - 1] = v.entry[i];
+                vol_set(v, i - 1, vol_get(v, i));
         if (v.pos > 0)
             v.pos--;
     }
     for (int i = v.pos; i >= 0; i--)
     {
-        if (vol_mat(v.entry[i]) == top)
+        const uint32_t e = vol_get(v, i);
+        if (vol_mat(e) == top)
         {
-            v.entry[i] |= VOL_TOPMOST;
+            vol_set(v, i, e | VOL_TOPMOST);
             break;
         }
     }
