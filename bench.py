@@ -51,23 +51,28 @@ def main():
                     help="samples of the FP64 plain reference of the cache leg")
     ap.add_argument("--cache-plain-spp", type=int, default=32, help="plain samples whose time is the budget")
     ap.add_argument("--cache-warmup", type=int, default=8,
-                    help="training-only frames before the cache form accumulates (their time counts)")
+                    help="warm-up frames (training paths only) before the cache form accumulates (their time "
+                         "counts)")
     ap.add_argument("--cache-query-bounce", type=int, default=None,
-                    help="vertex at which the cache answers (default: the library's, 2)")
+                    help="vertex at which the cache answers (default: the library's, 1)")
     ap.add_argument("--cache-train-every", type=int, default=None,
-                    help="one training path in N (default: the library's, 16)")
-    ap.add_argument("--frames", choices=["plain", "cache"], default=None,
-                    help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain or with the "
-                         "radiance cache, and print ms per sample (a kernel-profiling workload)")
+                    help="one training path in N (default: the library's, 64)")
+    ap.add_argument("--cache-warmup-train-every", type=int, default=None,
+                    help="one path in N runs in a warm-up frame (default: the library's, 16)")
+    ap.add_argument("--frames", choices=["plain", "cache", "warmup"], default=None,
+                    help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain, with the "
+                         "radiance cache or as cache warm-up frames, and print ms per sample (a kernel-profiling "
+                         "workload)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
+    cache_over = {k: v for k, v in (("query_bounce", args.cache_query_bounce),
+                                    ("train_every", args.cache_train_every),
+                                    ("warmup_train_every", args.cache_warmup_train_every)) if v is not None}
     if args.frames:
-        over = {k: v for k, v in (("query_bounce", args.cache_query_bounce),
-                                  ("train_every", args.cache_train_every)) if v is not None}
-        return frames_only(args.frames, args.steps, over)
+        return frames_only(args.frames, args.steps, cache_over)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -168,8 +173,6 @@ def main():
 
     render = None if args.no_render else plain_render_leg(mipt, torch, np, args.no_cpu)
     restir = None if args.no_render else restir_render_leg(mipt, torch, np)
-    cache_over = {k: v for k, v in (("query_bounce", args.cache_query_bounce),
-                                    ("train_every", args.cache_train_every)) if v is not None}
     cache = None if args.no_cache else cache_vs_plain_leg(mipt, torch, np, args.cache_ref_spp, args.cache_plain_spp,
                                                           args.cache_warmup, cache_over)
 
@@ -210,7 +213,8 @@ def main():
 def frames_only(mode, samples, cache_over=None):
     """Config-1 scene at 1920x1080, 4 bounces, uniform NEE + MIS: 3 warm-up and
     `samples` timed calls of mipt_render_sample, plain or with the radiance
-    cache (default settings, cache_over replacing some). Prints one JSON line."""
+    cache (default settings, cache_over replacing some), or of
+    mipt_radiance_cache_warmup. Prints one JSON line."""
     import torch
 
     import mipt
@@ -220,18 +224,19 @@ def frames_only(mode, samples, cache_over=None):
     ctx = mipt.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
-    cache = mode == "cache"
+    cache = mode != "plain"
+    frame = ctx.cache_warmup if mode == "warmup" else ctx.render_sample
     if cache:
         ctx.set_radiance_cache(mipt.default_radiance_cache_settings(**(cache_over or {})))
     ctx.set_render_settings(mipt.default_render_settings(width=W, height=H, nb_bounces=4, accumulate_fp64=1,
                                                          radiance_cache=int(cache)))
     ctx.set_camera(scene.camera)
     for i in range(3):
-        ctx.render_sample(i)
+        frame(i)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(samples):
-        ctx.render_sample(3 + i)
+        frame(3 + i)
     torch.cuda.synchronize()
     out = {"frames": mode, "samples": samples, "ms_per_sample": (time.perf_counter() - t0) / samples * 1e3}
     if cache:
@@ -247,9 +252,11 @@ def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup, cache_over=N
     channels of (x - r)^2 / (r^2 + 1e-2), reduced in float64, against a plain
     render with FP64 accumulation at ref_spp samples. The plain form renders
     plain_spp samples; its wall time is the budget. The cache form (by default
-    queried at bounce 2, trained every frame on 1/16 of the paths, batches of
-    2^16, Adam lr 1e-2; cache_over replaces settings) first renders `warmup` training-only frames, whose time counts,
-    then accumulates until it matches the plain relMSE or runs 3x the budget.
+    queried at bounce 1, trained every frame on 1/64 of the paths, batches of
+    2^16, Adam lr 1e-2; cache_over replaces settings) first runs `warmup`
+    warm-up frames (mipt_radiance_cache_warmup: 1/16 of the paths, all of them
+    training paths, no image), whose time counts, then accumulates until it
+    matches the plain relMSE or runs 3x the budget.
     Reported: both relMSE at equal time and the cache's time to reach the
     plain relMSE as a fraction of the plain time (interpolated between
     checkpoints in log-log). Wall times are host clock around synchronised
@@ -308,9 +315,10 @@ def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup, cache_over=N
     budget, target = plain[-1][0], plain[-1][2]
 
     ctx = make(True)
+    settings = mipt.default_radiance_cache_settings(**(cache_over or {}))
     tw = time.perf_counter()
     for i in range(warmup):
-        ctx.render_sample(1000000 + i, sh)
+        ctx.cache_warmup(1000000 + i, sh)
     stream.synchronize()
     warm_s = time.perf_counter() - tw
     cached = run(ctx, 0, lambda c: (c[-1][0] + warm_s >= budget and c[-1][2] <= target)
@@ -342,8 +350,10 @@ def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup, cache_over=N
                 reach = float(np.exp(np.log(prev[0]) + a * (np.log(p[0]) - np.log(prev[0]))))
             break
         prev = p
-    return {"workload": "config 1 scene (102,428 tris) 1920x1080, 4 bounces, uniform NEE + MIS; cache queried at "
-                        "bounce 2, trained on 1/16 of paths (batch 2^16, Adam lr 1e-2, relative L2)",
+    return {"workload": f"config 1 scene (102,428 tris) 1920x1080, 4 bounces, uniform NEE + MIS; cache queried at "
+                        f"bounce {settings.query_bounce}, trained on 1/{settings.train_every} of paths (batch "
+                        f"2^16, Adam lr 1e-2, relative L2); {warmup} warm-up frames of 1/{settings.warmup_train_every}"
+                        f" of paths",
             "reference": f"plain, FP64 accumulation, {ref_spp} spp ({ref_s:.1f} s)",
             "plain_spp": plain_spp, "plain_seconds": round(budget, 4), "plain_relmse": target,
             "plain_ms_per_spp": round(budget / plain_spp * 1e3, 3),

@@ -157,7 +157,8 @@ void allocate_render_buffers(Ctx& ctx)
     R.q_bsdf.reserve(4 * qslots);
     R.q_cont.reserve(4 * qslots);
     R.q_skip.reserve(4 * 2 * qslots);
-    R.q_counts.reserve(4 * 5 * (size_t)SEG_COUNTER_WORDS);
+    R.q_live.reserve(4 * 2 * LIVE_APPENDERS * qslots);
+    R.q_counts.reserve(4 * 7 * (size_t)SEG_COUNTER_WORDS);
     R.fb.reserve(12 * n);
     HIP_CHECK(hipMemset(R.fb.ptr, 0, 12 * n));
     if (R.settings.accumulate_fp64)
@@ -596,7 +597,7 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
     const bool spatial = pass >= MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0) && pass < MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(64);
     if (!ctx || (pass != MIPT_PASS_CAMERA_RAYS && pass != MIPT_PASS_PATH_TRACE &&
                  pass != MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES && pass != MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE &&
-                 pass != MIPT_PASS_RADIANCE_CACHE_TRAIN && !spatial))
+                 pass != MIPT_PASS_RADIANCE_CACHE_TRAIN && pass != MIPT_PASS_RADIANCE_CACHE_WARMUP && !spatial))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         if (!ctx->render.configured)
@@ -622,6 +623,8 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
             mipt::restir_temporal_pass(*ctx, sample_number, random_number, s);
         else if (pass == MIPT_PASS_RADIANCE_CACHE_TRAIN)
             mipt::nrc_train_pass(*ctx, s);
+        else if (pass == MIPT_PASS_RADIANCE_CACHE_WARMUP)
+            mipt::path_trace_pass(*ctx, sample_number, random_number, s, true);
         else
             mipt::restir_spatial_pass(*ctx, pass - MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0), sample_number, random_number,
                                       s);
@@ -646,10 +649,15 @@ int32_t mipt_draws_per_sample(const mipt_ctx* c)
     return draws;
 }
 
-int mipt_render_sample(mipt_ctx* c, uint32_t sample_number, void* stream)
+namespace {
+// The passes of one sample; a warm-up frame runs RADIANCE_CACHE_WARMUP in
+// place of PATH_TRACE.
+int render_sample_passes(mipt_ctx* c, uint32_t sample_number, void* stream, bool warmup)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
     if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (warmup && !ctx->render.settings.radiance_cache)
         return MIPT_ERR_INVALID_ARGUMENT;
     const uint32_t draws = (uint32_t)mipt_draws_per_sample(c);
     uint32_t k = sample_number * draws;
@@ -673,10 +681,22 @@ int mipt_render_sample(mipt_ctx* c, uint32_t sample_number, void* stream)
     }
     if (rc != MIPT_OK)
         return rc;
-    rc = mipt_launch_pass(c, MIPT_PASS_PATH_TRACE, sample_number, mipt_host_random_number(k), stream);
+    rc = mipt_launch_pass(c, warmup ? MIPT_PASS_RADIANCE_CACHE_WARMUP : MIPT_PASS_PATH_TRACE, sample_number,
+                          mipt_host_random_number(k), stream);
     if (rc == MIPT_OK && R.settings.radiance_cache)
         rc = mipt_launch_pass(c, MIPT_PASS_RADIANCE_CACHE_TRAIN, sample_number, 0u, stream);
     return rc;
+}
+} // namespace
+
+int mipt_render_sample(mipt_ctx* c, uint32_t sample_number, void* stream)
+{
+    return render_sample_passes(c, sample_number, stream, false);
+}
+
+int mipt_radiance_cache_warmup(mipt_ctx* c, uint32_t sample_number, void* stream)
+{
+    return render_sample_passes(c, sample_number, stream, true);
 }
 
 int mipt_get_power_alias_table(mipt_ctx* c, float* probas, int32_t* alias, float* sum)
@@ -802,9 +822,10 @@ void mipt_radiance_cache_settings_default(mipt_radiance_cache_settings* s)
     s->adam_beta2 = 0.99f;
     s->adam_eps = 1.0e-8f;
     s->seed = 1337u;
-    s->query_bounce = 2;
-    s->train_every = 16;
+    s->query_bounce = 1;
+    s->train_every = 64;
     s->train_extension = 3;
+    s->warmup_train_every = 16;
 }
 
 int mipt_set_radiance_cache(mipt_ctx* c, const mipt_radiance_cache_settings* s)
@@ -814,7 +835,8 @@ int mipt_set_radiance_cache(mipt_ctx* c, const mipt_radiance_cache_settings* s)
         s->grid_log2_table > 24 || s->grid_base_resolution < 2 || !(s->grid_per_level_scale >= 1.0f) ||
         s->batch_size < 16 || s->batch_size > (1 << 22) || !(s->learning_rate > 0.0f) ||
         !(s->adam_beta1 >= 0.0f && s->adam_beta1 < 1.0f) || !(s->adam_beta2 >= 0.0f && s->adam_beta2 < 1.0f) ||
-        !(s->adam_eps >= 0.0f) || s->query_bounce < 1 || s->train_every < 1 || s->train_extension < 0)
+        !(s->adam_eps >= 0.0f) || s->query_bounce < 1 || s->train_every < 1 || s->train_extension < 0 ||
+        s->warmup_train_every < 1)
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         if (!ctx->scene_set)

@@ -25,6 +25,13 @@ void hip_check(hipError_t e, const char* what);
 
 #define HIP_CHECK(expr) ::mipt::hip_check((expr), #expr)
 
+// Boundary-skip rounds of a bounce at most (nested dielectrics: each skip adds
+// a stack entry without a pop, so at most 2 follow one another on a 4-entry
+// stack whose entry 0 is the air), and the kernels that append to a bounce's
+// list of live paths: its resolve kernel and the skip rounds.
+constexpr int32_t MAX_SKIP_ROUNDS = 2;
+constexpr uint32_t LIVE_APPENDERS = 1u + (uint32_t)MAX_SKIP_ROUNDS;
+
 // Owning device allocation (hipMalloc), sized in bytes.
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -81,7 +88,7 @@ struct RenderState {
     DeviceBuffer vol_a, vol_b;    // volume state of the paths that keep one (uint4 each)
     // rays of one bounce, and the queues of live slots the traversal takes
     DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
-    DeviceBuffer q_shadow, q_bsdf, q_cont, q_skip, q_counts; // q_skip: two lists back to back
+    DeviceBuffer q_shadow, q_bsdf, q_cont, q_skip, q_live, q_counts; // q_skip, q_live: two lists back to back
     // framebuffer
     DeviceBuffer fb, fb64, counters;
     // power alias table of the emissive triangles (LSS_BASE_POWER)
@@ -166,7 +173,8 @@ const float* sheen_ltc_data();             // [32][32][3]
 
 // pathtrace.hip
 void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
-void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream,
+                     bool warmup = false);
 float consistent_tau(float tau_0, uint32_t sample_number);
 
 // lights.cpp: Utils::compute_alias_table (src/Utils/Utils.cpp:63-131) and the

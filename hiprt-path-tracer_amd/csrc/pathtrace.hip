@@ -129,12 +129,27 @@ __global__ void k_gbuffer_write(PtData d, const float4* __restrict__ rays, const
 // PATH_TRACE
 // ---------------------------------------------------------------------------
 
-// Megakernel.h:52-86: seed, first vertex from the G-buffer.
-__global__ void k_path_init(PtData d)
+// One path in `every` of a sample is a training path of the radiance cache
+// (a hash of pixel and sample number).
+__device__ __forceinline__ bool is_training_path(uint32_t pix, uint32_t sample, uint32_t every)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= (uint32_t)(d.width * d.height))
-        return;
+    uint32_t h = pix * 0x9E3779B1u ^ (sample + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h % every == 0u;
+}
+
+// Megakernel.h:52-86: seed, first vertex from the G-buffer.
+__device__ __forceinline__ bool init_path(const PtData& d, uint32_t pix)
+{
+    if (d.train_only_every && !is_training_path(pix, d.sample_number, d.train_only_every))
+    {
+        d.flags[pix] = 0u; // warm-up pass: only the training paths run
+        return false;
+    }
     d.rng[pix] = pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0);
     d.throughput[pix] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     d.radiance[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -142,7 +157,7 @@ __global__ void k_path_init(PtData d)
     if (prim < 0)
     {
         d.flags[pix] = 0u; // camera ray missed: no envmap in this build's scenes
-        return;
+        return false;
     }
     const f3 p = ld3(d.gb_pos[pix]);
     const f3 gn = gb_normal(d.gb_gn[pix]);
@@ -176,6 +191,42 @@ __global__ void k_path_init(PtData d)
         }
     }
     d.flags[pix] = flags;
+    return true;
+}
+
+// Starts every pixel's path and lists the live ones: the later kernels of
+// the pass run over that list, not over the image.
+__global__ void k_path_init(PtData d, SegList live)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool alive = pix < (uint32_t)(d.width * d.height) && init_path(d, pix);
+    seg_append(live, alive, pix);
+}
+
+// Pixel of dense index w of a list of live paths; false past its end. Every
+// lane of the wave must call it; a wave past the end leaves at once.
+struct LiveIndex {
+    uint32_t pix;
+    bool valid;
+    bool wave_empty;
+};
+
+__device__ __forceinline__ LiveIndex live_index(const SegList& l)
+{
+    SegMap map;
+    map.init(l.counts, l.seg_cap);
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = map.total;
+    LiveIndex r{0u, false, (w & ~63u) >= total};
+    if (r.wave_empty)
+        return r;
+    const size_t slot = map.slot(w < total ? w : total - 1u);
+    if (w < total)
+    {
+        r.pix = l.slots[slot];
+        r.valid = true;
+    }
+    return r;
 }
 
 // evaluate_ReSTIR_DI_reservoir (ReSTIR/DI/FinalShading.h:16-101) without the
@@ -365,7 +416,7 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
         }
     }
     // The direct-light estimate of this vertex uses the pre-update throughput,
-    // snapshotted before this kernel.
+    // which k_shade keeps in thr_prev before calling this.
     d.vtx_pos[pix] = st3(p, path_roughness);
     d.rng[pix] = rng.state;
     // A reflection pops the vertex's material again: most paths are back to
@@ -380,18 +431,26 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
     d.flags[pix] = flags;
 }
 
-// Shades the live paths and enqueues the slots of the rays they emitted.
+// Shades the live paths of the list (dense, so that no wave idles on dead
+// or stopped paths) and enqueues the slots of the rays they emitted; keeps
+// the throughput before the vertex for the direct-light estimate.
 // The layered BSDF with every lobe needs ~140 VGPRs: allowing them (3 waves per
 // SIMD) beats spilling to scratch at 4 waves (11.08 vs 11.74 ms per 1080p
 // config-1 sample).
-__global__ __launch_bounds__(256, 2) void k_shade(PtData d, int32_t bounce, RayQueues q)
+__global__ __launch_bounds__(256, 2) void k_shade(PtData d, int32_t bounce, RayQueues q, SegList live,
+                                                  float4* __restrict__ thr_prev)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = pix < (uint32_t)(d.width * d.height);
-    const uint32_t flags = valid ? d.flags[pix] : 0u;
+    const LiveIndex li = live_index(live);
+    if (li.wave_empty)
+        return;
+    const uint32_t pix = li.pix;
+    const uint32_t flags = li.valid ? d.flags[pix] : 0u;
     bool shadow_active = false, bsdf_active = false, cont = false;
     if (flags & FLAG_ALIVE)
+    {
+        thr_prev[pix] = d.throughput[pix];
         shade_vertex(d, bounce, pix, flags, shadow_active, bsdf_active, cont);
+    }
     seg_append(q.shadow, shadow_active, pix);
     seg_append(q.bsdf, bsdf_active, pix);
     seg_append(q.cont, cont, pix);
@@ -403,13 +462,15 @@ __global__ __launch_bounds__(256, 2) void k_shade(PtData d, int32_t bounce, RayQ
 // than the medium the ray is in is skipped and the ray goes on from the hit
 // point with the same self-intersection filter (returns true). Hitting a
 // dispersive glass samples the path's wavelength.
-__device__ __forceinline__ bool next_vertex(const PtData& d, uint32_t pix, uint32_t flags)
+enum : int { PATH_ENDED = 0, PATH_AT_VERTEX = 1, PATH_SKIPPING = 2 };
+
+__device__ __forceinline__ int next_vertex(const PtData& d, uint32_t pix, uint32_t flags)
 {
     const int4 h = d.cont_hits[pix];
     if (h.x < 0)
     {
         d.flags[pix] = 0u; // miss: no envmap contribution
-        return false;
+        return PATH_ENDED;
     }
     const float4 ra = d.cont_rays[2 * (size_t)pix], rb = d.cont_rays[2 * (size_t)pix + 1];
     const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
@@ -433,7 +494,7 @@ __device__ __forceinline__ bool next_vertex(const PtData& d, uint32_t pix, uint3
             vol_store(d, pix, vs);
             write_ray(d.cont_rays, pix, p, __float_as_int(ra.w), dir, INFINITY);
             d.flags[pix] = FLAG_VOL | FLAG_SKIP;
-            return true;
+            return PATH_SKIPPING;
         }
     }
     if (d.scene_dispersive)
@@ -456,17 +517,17 @@ __device__ __forceinline__ bool next_vertex(const PtData& d, uint32_t pix, uint3
     d.vtx_gn[pix] = st3(gn, __int_as_float(mi));
     d.vtx_view[pix] = st3(-1.0f * dir, 0.0f);
     d.flags[pix] = FLAG_ALIVE | (stored ? FLAG_VOL : 0u);
-    return false;
+    return PATH_AT_VERTEX;
 }
 
 // Direct lighting of the bounce (Lights.h:364-398) and the next vertex;
-// returns whether the continuation ray skips a boundary.
-__device__ __forceinline__ bool resolve_path(const PtData& d, int32_t bounce, const float4* __restrict__ thr_prev,
-                                             uint32_t pix)
+// returns the path's state after the continuation ray (PATH_*).
+__device__ __forceinline__ int resolve_path(const PtData& d, int32_t bounce, const float4* __restrict__ thr_prev,
+                                            uint32_t pix)
 {
     const uint32_t flags = d.flags[pix];
     if (!(flags & FLAG_ALIVE))
-        return false;
+        return PATH_ENDED;
     Color radiance = ldc(d.radiance[pix]);
     if (flags & FLAG_DIRECT)
     {
@@ -511,46 +572,36 @@ __device__ __forceinline__ bool resolve_path(const PtData& d, int32_t bounce, co
     if (!(flags & FLAG_CONTINUE))
     {
         d.flags[pix] = 0u;
-        return false;
+        return PATH_ENDED;
     }
     return next_vertex(d, pix, flags);
 }
 
-__global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev, SegList skip)
+// Over the paths shaded this bounce: those at a new vertex go to `next`, those
+// whose continuation ray skipped a boundary to `skip`.
+__global__ void k_resolve(PtData d, int32_t bounce, const float4* __restrict__ thr_prev, SegList live, SegList skip,
+                          SegList next)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = pix < (uint32_t)(d.width * d.height);
-    const bool skipping = valid && resolve_path(d, bounce, thr_prev, pix);
-    seg_append(skip, skipping, pix);
+    const LiveIndex li = live_index(live);
+    if (li.wave_empty)
+        return;
+    const int st = li.valid ? resolve_path(d, bounce, thr_prev, li.pix) : PATH_ENDED;
+    seg_append(skip, st == PATH_SKIPPING, li.pix);
+    seg_append(next, st == PATH_AT_VERTEX, li.pix);
 }
 
 // One round of boundary skipping: the paths of `in` have traced their ray on
-// from the skipped boundary; those that skip again go to `out`. The grid
-// covers every pixel; waves past the list's length leave at once.
-__global__ void k_resolve_skip(PtData d, SegList in, SegList out)
+// from the skipped boundary; those that skip again go to `out`, those at a
+// vertex to `next`. The grid covers every pixel; waves past the list's length
+// leave at once.
+__global__ void k_resolve_skip(PtData d, SegList in, SegList out, SegList next)
 {
-    SegMap map;
-    map.init(in.counts, in.seg_cap);
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = map.total;
-    if ((w & ~63u) >= total)
-        return; // the whole wave: no lane has an entry
-    const size_t slot = map.slot(w < total ? w : total - 1u);
-    uint32_t pix = 0u;
-    bool skipping = false;
-    if (w < total)
-    {
-        pix = in.slots[slot];
-        skipping = next_vertex(d, pix, d.flags[pix]);
-    }
-    seg_append(out, skipping, pix);
-}
-
-__global__ void k_snapshot_throughput(PtData d, float4* __restrict__ out)
-{
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix < (uint32_t)(d.width * d.height))
-        out[pix] = d.throughput[pix];
+    const LiveIndex li = live_index(in);
+    if (li.wave_empty)
+        return;
+    const int st = li.valid ? next_vertex(d, li.pix, d.flags[li.pix]) : PATH_ENDED;
+    seg_append(out, st == PATH_SKIPPING, li.pix);
+    seg_append(next, st == PATH_AT_VERTEX, li.pix);
 }
 
 // SanityCheck.h:78-106 + PathTracing.h:176-217: a NaN/inf/negative sample is
@@ -622,17 +673,6 @@ struct CacheView {
     uint32_t sample_number;
 };
 
-__device__ __forceinline__ bool is_training_path(uint32_t pix, uint32_t sample, uint32_t every)
-{
-    uint32_t h = pix * 0x9E3779B1u ^ (sample + 0x7F4A7C15u) * 0x85EBCA77u;
-    h ^= h >> 15;
-    h *= 0x2C1B3C6Du;
-    h ^= h >> 12;
-    h *= 0x297A2D39u;
-    h ^= h >> 15;
-    return h % every == 0u;
-}
-
 __global__ void k_cache_counters_reset(uint32_t* c)
 {
     if (threadIdx.x < 4)
@@ -640,12 +680,15 @@ __global__ void k_cache_counters_reset(uint32_t* c)
 }
 
 // At the start of the query bounce: lists the live paths as queries (their
-// path stops here) or training records.
-__global__ void k_cache_query(PtData d, CacheView cv)
+// path stops here) or training records. The training list is the bounce's
+// list of live paths from here on.
+__global__ void k_cache_query(PtData d, CacheView cv, SegList live)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = pix < (uint32_t)(d.width * d.height);
-    const uint32_t flags = valid ? d.flags[pix] : 0u;
+    const LiveIndex li = live_index(live);
+    if (li.wave_empty)
+        return;
+    const uint32_t pix = li.pix;
+    const uint32_t flags = li.valid ? d.flags[pix] : 0u;
     const bool alive = (flags & FLAG_ALIVE) != 0u;
     const bool train = alive && is_training_path(pix, cv.sample_number, cv.train_every);
     seg_append(cv.query_list, alive && !train, pix);
@@ -740,11 +783,10 @@ __global__ void k_cache_targets(PtData d, CacheView cv)
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-CacheView make_cache_view(Ctx& ctx, uint32_t sample_number)
+CacheView make_cache_view(Ctx& ctx, uint32_t sample_number, uint32_t every)
 {
     Nrc& C = ctx.nrc;
     const uint32_t n = ctx.render.pixels;
-    const uint32_t every = (uint32_t)C.settings.train_every;
     const uint32_t cap = n / every + n / (every * 8) + 1024; // expected n / every, binomial slack
     C.queries.reserve(64 * (size_t)n);
     C.query_pix.reserve(4 * (size_t)n);
@@ -890,19 +932,26 @@ void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, 
     HIP_CHECK(hipGetLastError());
 }
 
-void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s, bool warmup)
 {
     RenderState& R = ctx.render;
     const uint32_t n = (uint32_t)R.settings.width * (uint32_t)R.settings.height;
     PtData d = make_pt_data(ctx, sample_number, random_number);
     const unsigned g = blocks_for(n, 256);
     const bool cache = R.settings.radiance_cache != 0;
+    if (warmup && !cache)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "RADIANCE_CACHE_WARMUP needs render_settings.radiance_cache = 1");
     CacheView cv{};
     if (cache)
     {
         if (!ctx.nrc.configured)
             fail(MIPT_ERR_INVALID_ARGUMENT, "radiance_cache is on: call mipt_set_radiance_cache first");
-        cv = make_cache_view(ctx, sample_number);
+        // A warm-up pass traces only its training paths, one in
+        // warmup_train_every, and leaves the framebuffer alone.
+        const uint32_t every =
+            (uint32_t)(warmup ? ctx.nrc.settings.warmup_train_every : ctx.nrc.settings.train_every);
+        d.train_only_every = warmup ? every : 0u;
+        cv = make_cache_view(ctx, sample_number, every);
         hipLaunchKernelGGL(k_cache_counters_reset, dim3(1), dim3(64), 0, s, cv.counters);
     }
     const uint32_t seg_cap = seg_capacity(n);
@@ -914,46 +963,62 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
                       SegList{R.q_cont.as<uint32_t>(), qc + 2 * SEG_COUNTER_WORDS, seg_cap},
                       {SegList{qskip, qc + 3 * SEG_COUNTER_WORDS, seg_cap},
                        SegList{qskip + qn, qc + 4 * SEG_COUNTER_WORDS, seg_cap}}};
+    // Live paths of a bounce, ping-ponged: the list a bounce runs over and
+    // the one its resolve kernels fill for the next. k_resolve and each skip
+    // round append to it, each at most once per thread, so a segment holds up
+    // to LIVE_APPENDERS times a one-kernel list's share.
+    uint32_t* qlive = R.q_live.as<uint32_t>();
+    const uint32_t live_cap = LIVE_APPENDERS * seg_cap;
+    const SegList live[2] = {SegList{qlive, qc + 5 * SEG_COUNTER_WORDS, live_cap},
+                             SegList{qlive + (size_t)SEG_COUNT * live_cap, qc + 6 * SEG_COUNTER_WORDS, live_cap}};
     // A skip needs a medium of higher priority than the boundary hit; only
-    // transmissive materials take priorities below the maximum. Each skip
-    // adds a stack entry without a pop, so at most 2 follow one another on a
-    // 4-entry stack (entry 0 is the air; the first medium sits at 1).
-    const int32_t skip_rounds = d.scene_volumes ? 2 : 0;
-    hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d);
+    // transmissive materials take priorities below the maximum.
+    const int32_t skip_rounds = d.scene_volumes ? MAX_SKIP_ROUNDS : 0;
+    float4* thr_prev = R.thr_prev.as<float4>();
+    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(SEG_COUNT), 0, s, live[0].counts, 1u);
+    hipLaunchKernelGGL(k_path_init, dim3(g), dim3(256), 0, s, d, live[0]);
+    int cur = 0;
+    SegList cur_list = live[0];
     for (int32_t bounce = 0; bounce <= R.settings.nb_bounces; bounce++)
     {
         if (cache && bounce == ctx.nrc.settings.query_bounce)
         {
             hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(2 * SEG_COUNT), 0, s, cv.query_list.counts, 2u);
-            hipLaunchKernelGGL(k_cache_query, dim3(g), dim3(256), 0, s, d, cv);
+            hipLaunchKernelGGL(k_cache_query, dim3(g), dim3(256), 0, s, d, cv, cur_list);
             hipLaunchKernelGGL(k_cache_compact, dim3(g), dim3(256), 0, s, d, cv);
+            cur_list = cv.train_list; // the paths that go on
         }
-        hipLaunchKernelGGL(k_snapshot_throughput, dim3(g), dim3(256), 0, s, d, R.thr_prev.as<float4>());
+        const SegList next = live[cur ^ 1];
         hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(4 * SEG_COUNT), 0, s, qc, 4u);
-        hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce, q);
+        hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(SEG_COUNT), 0, s, next.counts, 1u);
+        hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce, q, cur_list, thr_prev);
         trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, n, s);
         trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, n, s);
         if (bounce < R.settings.nb_bounces)
             trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, n, s);
-        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)R.thr_prev.as<float4>(),
-                           q.skip[0]);
+        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)thr_prev, cur_list, q.skip[0],
+                           next);
         for (int32_t k = 0; k < skip_rounds && bounce < R.settings.nb_bounces; k++)
         {
             const SegList& in = q.skip[k & 1];
             const SegList& out = q.skip[(k + 1) & 1];
             hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(SEG_COUNT), 0, s, out.counts, 1u);
             trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), in, n, s);
-            hipLaunchKernelGGL(k_resolve_skip, dim3(g), dim3(256), 0, s, d, in, out);
+            hipLaunchKernelGGL(k_resolve_skip, dim3(g), dim3(256), 0, s, d, in, out, next);
         }
+        cur ^= 1;
+        cur_list = next;
     }
-    if (cache)
+    if (cache && !warmup)
     {
         nrc_infer(ctx, reinterpret_cast<const float*>(cv.queries), cv.counters, n,
                   reinterpret_cast<float*>(ctx.nrc.query_out.ptr), s);
         hipLaunchKernelGGL(k_cache_resolve, dim3(g), dim3(256), 0, s, d, cv);
-        hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cv.train_capacity, 256)), dim3(256), 0, s, d, cv);
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, d);
+    if (cache)
+        hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cv.train_capacity, 256)), dim3(256), 0, s, d, cv);
+    if (!warmup)
+        hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, d);
     HIP_CHECK(hipGetLastError());
 }
 

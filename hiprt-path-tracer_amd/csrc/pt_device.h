@@ -41,6 +41,7 @@ struct PtData {
     int32_t scene_dispersive;  // some material is dispersive glass
     uint32_t sample_number;
     uint32_t random_number;
+    uint32_t train_only_every; // > 0: a cache warm-up pass traces only its training paths
     pt::BsdfGlobals g;
     // G-buffer
     int32_t* gb_prim;

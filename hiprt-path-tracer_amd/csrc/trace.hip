@@ -79,12 +79,24 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
         n = min(n, map.total);
     }
 
+    // Each wave's first batch of 64 rays is its own by index; later batches
+    // come from the work counter, past the statically assigned ones. A
+    // launch on fewer rays than the grid's lanes then touches the counter
+    // only from the waves that had work: a same-address atomic from every
+    // wave costs tens of microseconds per launch.
+    const uint32_t waves = gridDim.x * (BLOCK / 64u);
+    uint32_t next = (blockIdx.x * (BLOCK / 64u) + threadIdx.x / 64u) * 64u;
+    bool first = true;
     for (;;)
     {
-        uint32_t base = 0;
-        if (lane == 0)
-            base = atomicAdd(work_counter, 64u);
-        base = __shfl(base, 0, 64);
+        uint32_t base = next;
+        if (!first)
+        {
+            if (lane == 0)
+                base = atomicAdd(work_counter, 64u) + waves * 64u;
+            base = __shfl(base, 0, 64);
+        }
+        first = false;
         if (base >= n)
             break;
         const uint32_t w = base + lane;

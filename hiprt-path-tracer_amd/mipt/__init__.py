@@ -133,6 +133,7 @@ class RadianceCacheSettings(C.Structure):
         ("grid_per_level_scale", C.c_float), ("batch_size", C.c_int32), ("learning_rate", C.c_float),
         ("adam_beta1", C.c_float), ("adam_beta2", C.c_float), ("adam_eps", C.c_float), ("seed", C.c_uint32),
         ("query_bounce", C.c_int32), ("train_every", C.c_int32), ("train_extension", C.c_int32),
+        ("warmup_train_every", C.c_int32),
     ]
 
 
@@ -141,6 +142,7 @@ MIPT_PASS_PATH_TRACE = 1
 MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES = 2
 MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE = 3
 MIPT_PASS_RADIANCE_CACHE_TRAIN = 4
+MIPT_PASS_RADIANCE_CACHE_WARMUP = 5
 
 
 def MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i: int) -> int:
@@ -189,6 +191,7 @@ _SIGNATURES = {
     "mipt_set_camera": (C.c_int, [C.c_void_p, C.POINTER(Camera)]),
     "mipt_launch_pass": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "mipt_render_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mipt_radiance_cache_warmup": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "mipt_read_framebuffer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_framebuffer_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "mipt_read_gbuffer_prims": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -468,6 +471,11 @@ class Context:
 
     def render_sample(self, sample_number: int, stream=None):
         self._check(self.lib.mipt_render_sample(self.h, sample_number, stream), "mipt_render_sample")
+
+    def cache_warmup(self, sample_number: int, stream=None):
+        """One radiance-cache warm-up frame: training paths only, then an
+        optimiser step; the framebuffer is left alone."""
+        self._check(self.lib.mipt_radiance_cache_warmup(self.h, sample_number, stream), "mipt_radiance_cache_warmup")
 
     def read_framebuffer(self, fp64: bool = False):
         n = self.settings.width * self.settings.height

@@ -369,9 +369,10 @@ typedef struct mipt_radiance_cache_settings {
     float adam_beta2;             /* 0.99 */
     float adam_eps;               /* 1e-8 */
     uint32_t seed;                /* parameter initialisation stream */
-    int32_t query_bounce;         /* 2: paths stop at this bounce with the cache's radiance */
-    int32_t train_every;          /* 16: one path in train_every is a training path */
+    int32_t query_bounce;         /* 1: paths stop at this bounce with the cache's radiance */
+    int32_t train_every;          /* 64: one path in train_every is a training path */
     int32_t train_extension;      /* 3: training paths trace this many bounces past query_bounce */
+    int32_t warmup_train_every;   /* 16: one path in N runs (as a training path) in a warm-up frame */
 } mipt_radiance_cache_settings;
 
 void mipt_radiance_cache_settings_default(mipt_radiance_cache_settings* s);
@@ -416,9 +417,17 @@ int mipt_radiance_cache_stats(mipt_ctx* ctx, uint32_t* counts, float* train_loss
 /* The path pass with render_settings.radiance_cache = 1 stops paths at
  * query_bounce and adds throughput x the cache's prediction; one path in
  * train_every continues train_extension more bounces and records a training
- * target. RADIANCE_CACHE_TRAIN then takes one optimiser step on the records
- * of the last path pass. */
+ * target. RADIANCE_CACHE_TRAIN then takes one optimiser step per batch_size
+ * records of the last path pass. */
 #define MIPT_PASS_RADIANCE_CACHE_TRAIN 4
+/* A path pass that traces only training paths (one in warmup_train_every)
+ * and leaves the framebuffer and its sample count alone: training frames
+ * that cost a fraction of a rendered frame, for a cache that starts cold. */
+#define MIPT_PASS_RADIANCE_CACHE_WARMUP 5
+/* One warm-up frame: the passes of mipt_render_sample with
+ * RADIANCE_CACHE_WARMUP in place of PATH_TRACE, then RADIANCE_CACHE_TRAIN.
+ * Needs render_settings.radiance_cache = 1. */
+int mipt_radiance_cache_warmup(mipt_ctx* ctx, uint32_t sample_number, void* stream);
 
 /* Reservoir readback: which = 0 initial-candidates buffer, 1 / 2 the spatial
  * output buffers, 3 the reservoirs the path pass shades. Per pixel 8 words:

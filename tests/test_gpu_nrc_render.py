@@ -2,8 +2,10 @@
 
 - Bookkeeping: every path alive at the query bounce is either a cache query
   or a training record; about 1 in train_every paths trains; no target is
-  dropped.
-- Gain: after warm-up training, the cache form's relMSE against an
+  dropped. A warm-up frame (mipt_radiance_cache_warmup) runs only the paths
+  that train, about 1 in warmup_train_every, answers no query and leaves the
+  framebuffer as it was.
+- Gain: after warm-up frames, the cache form's relMSE against an
   FP64-accumulated high-spp plain render is below 0.8x the plain form's at
   the same sample count (measured 0.71x: the cache removes the variance of
   bounces 2 and later, not that of the first two vertices). relMSE = mean
@@ -79,8 +81,36 @@ def test_cache_records(mipt_mod, torch_mod, scene_small):
     assert dropped == 0
     assert 0.3 * n < queries + train <= n
     frac = train / (queries + train)
-    assert abs(frac - 1.0 / 16) < 0.01
+    every = mipt_mod.default_radiance_cache_settings().train_every
+    assert abs(frac * every - 1.0) < 0.25
     assert np.isfinite(loss) and loss > 0
+    ctx.close()
+
+
+def test_warmup_frame(mipt_mod, torch_mod, scene_small):
+    ctx = _ctx(mipt_mod, scene_small, cache=True)
+    ctx.render_sample(0)
+    torch_mod.cuda.synchronize()
+    before = ctx.read_framebuffer(fp64=True)
+    params_before = ctx.cache_params()
+    for i in range(3):
+        ctx.cache_warmup(100 + i)
+    torch_mod.cuda.synchronize()
+    queries, train, dropped, loss = ctx.cache_stats()
+    after = ctx.read_framebuffer(fp64=True)
+    params_after = ctx.cache_params()
+    ctx.close()
+    n = W * H
+    every = mipt_mod.default_radiance_cache_settings().warmup_train_every
+    print(f"warm-up frame: queries {queries}, training records {train}, loss {loss:.4g}")
+    assert queries == 0 and dropped == 0
+    # the training paths that reach the query vertex (most of those started)
+    assert 0.3 * n / every < train < 1.25 * n / every
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert np.isfinite(loss) and loss > 0
+    # the optimiser stepped: the parameters moved
+    assert not np.array_equal(params_before[0], params_after[0])
+    assert not np.array_equal(params_before[2], params_after[2])
 
 
 def test_cache_gain_at_equal_spp(mipt_mod, torch_mod, scene_small, reference):
@@ -90,8 +120,8 @@ def test_cache_gain_at_equal_spp(mipt_mod, torch_mod, scene_small, reference):
     img_plain = _render(plain, torch_mod, spp)
     t_plain = time.perf_counter() - t0
     cache = _ctx(mipt_mod, scene_small, cache=True)
-    for i in range(64):  # warm-up training frames, outside the accumulated image
-        cache.render_sample(10000 + i)
+    for i in range(64):  # warm-up frames, outside the accumulated image
+        cache.cache_warmup(10000 + i)
     torch_mod.cuda.synchronize()
     t0 = time.perf_counter()
     img_cache = _render(cache, torch_mod, spp)
