@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--cache-ref-spp", type=int, default=512,
                     help="samples of the FP64 plain reference of the cache leg")
     ap.add_argument("--cache-plain-spp", type=int, default=32, help="plain samples whose time is the budget")
-    ap.add_argument("--cache-warmup", type=int, default=8,
+    ap.add_argument("--cache-warmup", type=int, default=2,
                     help="warm-up frames (training paths only) before the cache form accumulates (their time "
                          "counts)")
     ap.add_argument("--cache-query-bounce", type=int, default=None,

@@ -205,6 +205,15 @@ void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const S
                          hipStream_t stream);
 void trace_any_queue(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, const SegList& queue, uint32_t capacity,
                      hipStream_t stream);
+// Up to three queued ray sets in one launch: a set with `occluded` is traced
+// any-hit (shadow rays), the others closest-hit into `hits`.
+struct TraceQueue {
+    const mipt_ray* rays;
+    mipt_hit* hits;
+    uint8_t* occluded;
+    const SegList* queue;
+};
+void trace_queues(Ctx& ctx, const TraceQueue* queues, int count, uint32_t capacity, hipStream_t stream);
 
 // camera.hip
 void camera_rays(Ctx& ctx, const mipt_camera& cam, int32_t w, int32_t h, uint32_t sample_number,

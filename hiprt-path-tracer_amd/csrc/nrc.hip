@@ -105,34 +105,44 @@ __device__ __forceinline__ void normalized_position(const NrcDev& P, const float
 
 // Query record (16 floats): position, direction, normal, roughness, albedo.
 // Encodes the selected rows into x0 rows [0, rows) and zero-fills the rows up
-// to the next multiple of pad (at most n rows in all). Level-major: block row
-// y < levels interpolates grid level y (so one level's table is what the
-// blocks in flight read), block row y == levels writes the other inputs.
-__global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSel sel, uint32_t n, uint32_t pad,
-                             uint16_t* __restrict__ x0)
+// to the next multiple of pad (at most n rows in all).
+//
+// Launch: NRC_XCDS x chunks blocks of 128 threads (encode_launch). Workgroups
+// go to the 8 XCDs round-robin, so block b runs on XCD b % 8 and interpolates
+// the levels l with l % 8 == b % 8 for rows [128 (b / 8), +128): each XCD's
+// L2 (4 MB) then holds the tables of two levels (2 MB of FP16 pairs each at
+// 2^19 entries) instead of missing on all sixteen. The block whose chunk
+// index matches its XCD (chunk % 8 == b % 8) also writes the other inputs
+// and the zero padding of its rows.
+constexpr uint32_t NRC_XCDS = 8;
+
+__global__ __launch_bounds__(128) void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSel sel,
+                                                    uint32_t n, uint32_t pad, uint16_t* __restrict__ x0)
 {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = (int)blockIdx.y;
+    const uint32_t xcd = blockIdx.x % NRC_XCDS;
+    const uint32_t chunk = blockIdx.x / NRC_XCDS;
+    const uint32_t row = chunk * blockDim.x + threadIdx.x;
     const uint32_t rows = sel.rows();
     const uint32_t padded = (rows + pad - 1) / pad * pad;
     if (row >= n || row >= padded)
         return;
     uint16_t* out = x0 + (size_t)row * NRC_WIDTH;
+    const bool misc = chunk % NRC_XCDS == xcd;
     if (row >= rows)
     {
-        if (l == P.levels)
+        if (misc)
             for (int k = 0; k < NRC_WIDTH; k += 8)
                 *reinterpret_cast<uint4*>(out + k) = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
     const size_t rec = sel.record(row);
-    if (l < P.levels)
+    const float4 a = queries[4 * rec];
+    const float q3[3] = {a.x, a.y, a.z};
+    float pos[3];
+    normalized_position(P, q3, pos);
+    const uint32_t tmask = (1u << P.log2_table) - 1u;
+    for (int l = (int)xcd; l < P.levels; l += (int)NRC_XCDS)
     {
-        const float4 a = queries[4 * rec];
-        const float q[3] = {a.x, a.y, a.z};
-        float pos[3];
-        normalized_position(P, q, pos);
-        const uint32_t tmask = (1u << P.log2_table) - 1u;
         const NrcGridLevel& L = P.level[l];
         float f0 = 0.0f, f1 = 0.0f;
         for (int cc = 0; cc < 8; cc++)
@@ -146,9 +156,10 @@ __global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSe
             f1 += w * (float)__builtin_bit_cast(_Float16, (uint16_t)(pair >> 16));
         }
         *reinterpret_cast<uint32_t*>(out + 2 * l) = (uint32_t)to_bf16(f0) | ((uint32_t)to_bf16(f1) << 16);
-        return;
     }
-    const float4 a = queries[4 * rec], b = queries[4 * rec + 1], c = queries[4 * rec + 2], d = queries[4 * rec + 3];
+    if (!misc)
+        return;
+    const float4 b = queries[4 * rec + 1], c = queries[4 * rec + 2], d = queries[4 * rec + 3];
     const float q[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
     int k = 2 * P.levels;
     float u, v;
@@ -168,6 +179,14 @@ __global__ void k_nrc_encode(NrcDev P, const float4* __restrict__ queries, RowSe
     out[k++] = to_bf16(1.0f);
     for (; k < NRC_WIDTH; k++)
         out[k] = 0;
+}
+
+void encode_launch(const NrcDev& P, const float* d_queries, const RowSel& sel, uint32_t n, uint32_t pad,
+                   uint16_t* x0, hipStream_t s)
+{
+    const unsigned chunks = (unsigned)((n + 127u) / 128u);
+    hipLaunchKernelGGL(k_nrc_encode, dim3(NRC_XCDS * chunks), dim3(128), 0, s, P,
+                       reinterpret_cast<const float4*>(d_queries), sel, n, pad, x0);
 }
 
 // ---------------------------------------------------------------------------
@@ -436,12 +455,14 @@ __global__ __launch_bounds__(256) void k_nrc_weight_grad(NrcTrainDev T, uint32_t
 }
 
 // Grid gradient: scatters the feature gradients to the 8 corners of a level.
-__global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restrict__ queries)
+__global__ __launch_bounds__(128) void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restrict__ queries)
 {
-    // one thread per (row, level): a batch has few rows, and a row's 16 levels
-    // x 8 corners x 2 atomics in one thread left the GPU latency-bound
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = (int)blockIdx.y;
+    // One thread per (row, level): a batch has few rows, and a row's 16 levels
+    // x 8 corners x 2 atomics in one thread left the GPU latency-bound. Blocks
+    // map to XCDs as in k_nrc_encode (levels l % 8 == block % 8), so each
+    // XCD's atomics stay within the tables of two levels.
+    const uint32_t xcd = blockIdx.x % NRC_XCDS;
+    const uint32_t row = (blockIdx.x / NRC_XCDS) * blockDim.x + threadIdx.x;
     const uint32_t rows = T.sel.rows();
     if (row >= rows)
         return;
@@ -451,17 +472,20 @@ __global__ void k_nrc_grid_grad(NrcDev P, NrcTrainDev T, const float4* __restric
     normalized_position(P, q, pos);
     const uint32_t tmask = (1u << P.log2_table) - 1u;
     const float scale = 1.0f / (3.0f * (float)rows);
-    const NrcGridLevel& L = P.level[l];
-    const float g0 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l] * scale;
-    const float g1 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l + 1] * scale;
-    for (int cc = 0; cc < 8; cc++)
+    for (int l = (int)xcd; l < P.levels; l += (int)NRC_XCDS)
     {
-        uint32_t idx;
-        float w;
-        corner(L, pos, cc, tmask, idx, w);
-        const size_t e = ((size_t)l << P.log2_table) + idx;
-        atomicAdd(T.ggrad + 2 * e, w * g0);
-        atomicAdd(T.ggrad + 2 * e + 1, w * g1);
+        const NrcGridLevel& L = P.level[l];
+        const float g0 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l] * scale;
+        const float g1 = T.dx0[(size_t)row * NRC_GRID_MAX + 2 * l + 1] * scale;
+        for (int cc = 0; cc < 8; cc++)
+        {
+            uint32_t idx;
+            float w;
+            corner(L, pos, cc, tmask, idx, w);
+            const size_t e = ((size_t)l << P.log2_table) + idx;
+            atomicAdd(T.ggrad + 2 * e, w * g0);
+            atomicAdd(T.ggrad + 2 * e + 1, w * g1);
+        }
     }
 }
 
@@ -673,9 +697,7 @@ void nrc_infer(Ctx& ctx, const float* d_queries, const uint32_t* d_count, uint32
         d_count = C.count.as<uint32_t>();
     }
     const NrcDev P = nrc_dev(ctx);
-    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n16, 128), (unsigned)P.levels + 1u), dim3(128), 0, s, P,
-                       reinterpret_cast<const float4*>(d_queries), RowSel{d_count, 1u, 0u, n}, n16, 16u,
-                       C.x0.as<uint16_t>());
+    encode_launch(P, d_queries, RowSel{d_count, 1u, 0u, n}, n16, 16u, C.x0.as<uint16_t>(), s);
     const unsigned grid = (unsigned)std::min<size_t>(blocks_for((n + 15) / 16, 4), (size_t)ctx.cu_count * 4);
     hipLaunchKernelGGL(k_nrc_forward, dim3(grid), dim3(256), 0, s, C.wt.as<uint16_t>(), C.x0.as<uint16_t>(),
                        d_count, n, reinterpret_cast<float4*>(d_out));
@@ -691,9 +713,7 @@ void nrc_encode(Ctx& ctx, const float* d_queries, uint32_t n, uint16_t* d_out, h
         return;
     C.count.reserve(4);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, C.count.as<uint32_t>(), n);
-    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(n, 128), (unsigned)C.settings.grid_levels + 1u), dim3(128), 0, s, nrc_dev(ctx),
-                       reinterpret_cast<const float4*>(d_queries), RowSel{C.count.as<uint32_t>(), 1u, 0u, n}, n, 1u,
-                       d_out);
+    encode_launch(nrc_dev(ctx), d_queries, RowSel{C.count.as<uint32_t>(), 1u, 0u, n}, n, 1u, d_out, s);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -728,8 +748,7 @@ void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, con
     C.last_sel_limit = batch;
     const NrcDev P = nrc_dev(ctx);
     // encode the batch; rows past its count up to a multiple of 32 are zero
-    hipLaunchKernelGGL(k_nrc_encode, dim3(blocks_for(stride, 128), (unsigned)P.levels + 1u), dim3(128), 0, s, P,
-                       reinterpret_cast<const float4*>(d_queries), sel, stride, 32u, C.x0.as<uint16_t>());
+    encode_launch(P, d_queries, sel, stride, 32u, C.x0.as<uint16_t>(), s);
     if (clear_loss)
         hipLaunchKernelGGL(k_nrc_zero, dim3(1), dim3(64), 0, s, C.loss.as<float>(), (size_t)1);
     NrcTrainDev T{};
@@ -750,7 +769,7 @@ void nrc_gradients(Ctx& ctx, const float* d_queries, const float* d_targets, con
     hipLaunchKernelGGL(k_nrc_train_fwd_bwd, dim3(grid), dim3(256), 0, s, T);
     const uint32_t chunk = 1024;
     hipLaunchKernelGGL(k_nrc_weight_grad, dim3(blocks_for(stride, chunk), NRC_LAYERS), dim3(256), 0, s, T, chunk);
-    hipLaunchKernelGGL(k_nrc_grid_grad, dim3(blocks_for(stride, 128), (unsigned)P.levels), dim3(128), 0, s, P, T,
+    hipLaunchKernelGGL(k_nrc_grid_grad, dim3(NRC_XCDS * blocks_for(stride, 128)), dim3(128), 0, s, P, T,
                        reinterpret_cast<const float4*>(d_queries));
     HIP_CHECK(hipGetLastError());
 }

@@ -992,10 +992,26 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
         hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(4 * SEG_COUNT), 0, s, qc, 4u);
         hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(SEG_COUNT), 0, s, next.counts, 1u);
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, s, d, bounce, q, cur_list, thr_prev);
-        trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, n, s);
-        trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, n, s);
-        if (bounce < R.settings.nb_bounces)
-            trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, n, s);
+        // Shadow (any hit), BSDF-to-light and continuation rays. At bounce 0
+        // every pixel has its rays: one launch per kind, at the single-kind
+        // kernel's higher occupancy. Later bounces carry fewer paths, where a
+        // launch costs about its longest ray's latency: one launch for all.
+        const bool cont_rays = bounce < R.settings.nb_bounces;
+        if (bounce == 0)
+        {
+            trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, n, s);
+            trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, n, s);
+            if (cont_rays)
+                trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, n, s);
+        }
+        else
+        {
+            const TraceQueue tq[3] = {
+                {R.shadow_rays.as<mipt_ray>(), nullptr, R.shadow_occ.as<uint8_t>(), &q.shadow},
+                {R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), nullptr, &q.bsdf},
+                {R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), nullptr, &q.cont}};
+            trace_queues(ctx, tq, cont_rays ? 3 : 2, n, s);
+        }
         hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, s, d, bounce, (const float4*)thr_prev, cur_list, q.skip[0],
                            next);
         for (int32_t k = 0; k < skip_rounds && bounce < R.settings.nb_bounces; k++)
