@@ -476,12 +476,16 @@ struct Scratch {
 
 Scratch trace_scratch(Ctx& ctx, uint32_t threads)
 {
-    // Spill entries: the stack never holds more than max_depth entries.
+    // Spill entries: the stack never holds more than max_depth entries. The
+    // columns are sized for every thread the device can hold (32 waves per
+    // CU), not for this launch's grid, so that launches of different grids
+    // on one stream never reallocate the buffer under queued kernels.
     const int spill_depth = ctx.bvh.max_depth - ctx.trace_cfg.lds_stack;
     const size_t spill_off = 256; // work counter lives at the front
+    const size_t columns = std::max<size_t>(threads, (size_t)ctx.cu_count * 32u * 64u);
     size_t spill_bytes = spill_off;
     if (spill_depth > 0)
-        spill_bytes = spill_off + sizeof(int32_t) * (size_t)spill_depth * threads;
+        spill_bytes = spill_off + sizeof(int32_t) * (size_t)spill_depth * columns;
     ctx.spill.reserve(spill_bytes);
     return Scratch{ctx.spill.as<uint32_t>(),
                    reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ctx.spill.ptr) + spill_off)};

@@ -209,3 +209,20 @@ def test_restir_pass_errors(mipt_mod, torch_mod):
     with pytest.raises(mipt_mod.MiptError):
         ctx.set_render_settings(bad)
     ctx.close()
+
+
+def test_restir_above_the_verified_size_is_refused(mipt_mod, torch_mod):
+    """ReSTIR DI is refused above MIPT_RESTIR_MAX_PIXELS (32,400): a
+    1920x1080 render of the 1M-triangle scene made an illegal memory access
+    whose cause is not found. The refusal comes before any pass kernel."""
+    scene = mipt_mod.generate_scene(1, 192, 192)
+    ctx = mipt_mod.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    ctx.set_render_settings(mipt_mod.default_render_settings(width=192, height=192, direct_light_sampling_strategy=5))
+    ctx.set_camera(scene.camera)
+    with pytest.raises(mipt_mod.MiptError) as e:
+        ctx.render_sample(0)
+    assert "32,400" in str(e.value)
+    ctx.close()
+
