@@ -123,6 +123,8 @@ struct Nrc {
     // records of the path pass
     DeviceBuffer queries, query_pix, query_out, counters, train_queries, train_targets, train_pix, train_rad;
     DeviceBuffer query_list, train_list, list_counts; // segmented pixel lists of the query kernel
+    // query records in spatial (Morton) order, and the sort's buffers
+    DeviceBuffer queries_sorted, query_pix_sorted, sort_keys, sort_keys_out, sort_vals, sort_vals_out, sort_tmp;
     uint32_t train_capacity = 0;  // training records one path pass can hold
 };
 

@@ -82,6 +82,12 @@ NrcDev nrc_dev(const Ctx& ctx);
 // null) of d_queries (16 floats per row) into d_out (4 floats per row).
 void nrc_infer(Ctx& ctx, const float* d_queries, const uint32_t* d_count, uint32_t n, float* d_out,
                hipStream_t stream);
+// Copies the query records [0, *d_count) (16 floats each) and their pixel
+// indices into q_out / pix_out sorted by the Morton code of the position in
+// the grid's cube: rows next to each other then read the same hash-grid
+// entries. n bounds *d_count.
+void nrc_spatial_order(Ctx& ctx, const float4* d_queries, const uint32_t* d_pix, const uint32_t* d_count,
+                       uint32_t n, float4* q_out, uint32_t* pix_out, hipStream_t stream);
 // Writes the encoded input rows of [0, n) (64 bf16 per row) to d_out.
 void nrc_encode(Ctx& ctx, const float* d_queries, uint32_t n, uint16_t* d_out, hipStream_t stream);
 // Adds the loss gradients of one batch (records offset, offset + stride, ...

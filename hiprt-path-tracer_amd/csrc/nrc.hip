@@ -10,6 +10,10 @@
 // layers through a padded LDS tile (D layout -> row-major -> A layout).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -678,6 +682,80 @@ NrcDev nrc_dev(const Ctx& ctx)
     }
     P.grid_f16 = C.grid_f16.as<_Float16>();
     return P;
+}
+
+// Spreads the 10 low bits of v to every third bit.
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Morton key of each valid row's position (10 bits per axis of the grid's
+// cube); rows past the count get the largest key and sort last.
+__global__ void k_nrc_order_keys(NrcDev P, const float4* __restrict__ queries, const uint32_t* count, uint32_t n,
+                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n)
+        return;
+    vals[row] = row;
+    if (row >= *count)
+    {
+        keys[row] = 0xffffffffu;
+        return;
+    }
+    const float4 a = queries[4 * (size_t)row];
+    const float q[3] = {a.x, a.y, a.z};
+    float pos[3];
+    normalized_position(P, q, pos);
+    uint32_t c[3];
+    for (int k = 0; k < 3; k++)
+        c[k] = (uint32_t)fminf(pos[k] * 1024.0f, 1023.0f);
+    keys[row] = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+}
+
+__global__ void k_nrc_order_gather(const float4* __restrict__ queries, const uint32_t* __restrict__ pix,
+                                   const uint32_t* __restrict__ perm, const uint32_t* count, float4* __restrict__ q_out,
+                                   uint32_t* __restrict__ pix_out)
+{
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= *count)
+        return;
+    const uint32_t src = perm[row];
+    for (int k = 0; k < 4; k++)
+        q_out[4 * (size_t)row + k] = queries[4 * (size_t)src + k];
+    pix_out[row] = pix[src];
+}
+
+void nrc_spatial_order(Ctx& ctx, const float4* d_queries, const uint32_t* d_pix, const uint32_t* d_count, uint32_t n,
+                       float4* q_out, uint32_t* pix_out, hipStream_t s)
+{
+    Nrc& C = ctx.nrc;
+    if (n == 0)
+        return;
+    C.sort_keys.reserve(4 * (size_t)n);
+    C.sort_keys_out.reserve(4 * (size_t)n);
+    C.sort_vals.reserve(4 * (size_t)n);
+    C.sort_vals_out.reserve(4 * (size_t)n);
+    const NrcDev P = nrc_dev(ctx);
+    hipLaunchKernelGGL(k_nrc_order_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, d_queries, d_count, n,
+                       C.sort_keys.as<uint32_t>(), C.sort_vals.as<uint32_t>());
+    size_t tmp_bytes = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, C.sort_keys.as<uint32_t>(), C.sort_keys_out.as<uint32_t>(),
+                                        C.sort_vals.as<uint32_t>(), C.sort_vals_out.as<uint32_t>(), (size_t)n, 0u, 32u,
+                                        s));
+    C.sort_tmp.reserve(tmp_bytes > 0 ? tmp_bytes : 1);
+    HIP_CHECK(rocprim::radix_sort_pairs(C.sort_tmp.ptr, tmp_bytes, C.sort_keys.as<uint32_t>(),
+                                        C.sort_keys_out.as<uint32_t>(), C.sort_vals.as<uint32_t>(),
+                                        C.sort_vals_out.as<uint32_t>(), (size_t)n, 0u, 32u, s));
+    hipLaunchKernelGGL(k_nrc_order_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_queries, d_pix,
+                       C.sort_vals_out.as<uint32_t>(), d_count, q_out, pix_out);
+    HIP_CHECK(hipGetLastError());
 }
 
 void nrc_infer(Ctx& ctx, const float* d_queries, const uint32_t* d_count, uint32_t n, float* d_out, hipStream_t s)

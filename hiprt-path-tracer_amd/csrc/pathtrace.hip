@@ -790,6 +790,8 @@ CacheView make_cache_view(Ctx& ctx, uint32_t sample_number, uint32_t every)
     const uint32_t cap = n / every + n / (every * 8) + 1024; // expected n / every, binomial slack
     C.queries.reserve(64 * (size_t)n);
     C.query_pix.reserve(4 * (size_t)n);
+    C.queries_sorted.reserve(64 * (size_t)n);
+    C.query_pix_sorted.reserve(4 * (size_t)n);
     C.query_out.reserve(16 * (size_t)n);
     C.train_queries.reserve(64 * (size_t)cap);
     C.train_pix.reserve(4 * (size_t)cap);
@@ -1027,9 +1029,15 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
     }
     if (cache && !warmup)
     {
-        nrc_infer(ctx, reinterpret_cast<const float*>(cv.queries), cv.counters, n,
+        // Queries in spatial order: a wave's rows then share hash-grid
+        // entries, and the encode's gathers hit the caches.
+        CacheView sorted = cv;
+        sorted.queries = ctx.nrc.queries_sorted.as<float4>();
+        sorted.query_pix = ctx.nrc.query_pix_sorted.as<uint32_t>();
+        nrc_spatial_order(ctx, cv.queries, cv.query_pix, cv.counters, n, sorted.queries, sorted.query_pix, s);
+        nrc_infer(ctx, reinterpret_cast<const float*>(sorted.queries), cv.counters, n,
                   reinterpret_cast<float*>(ctx.nrc.query_out.ptr), s);
-        hipLaunchKernelGGL(k_cache_resolve, dim3(g), dim3(256), 0, s, d, cv);
+        hipLaunchKernelGGL(k_cache_resolve, dim3(g), dim3(256), 0, s, d, sorted);
     }
     if (cache)
         hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cv.train_capacity, 256)), dim3(256), 0, s, d, cv);
