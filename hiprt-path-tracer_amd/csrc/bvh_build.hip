@@ -535,6 +535,38 @@ void build_lbvh(Ctx& ctx)
     for (int32_t q = 0; q < n; q++)
         if (!tri_seen[q])
             fail(MIPT_ERR_HIP, "BVH validation failed: triangle in no leaf");
+    // Depth-first layout of the reachable interior nodes: a node's first
+    // child follows it, so a traversal step into it often reads the 128-byte
+    // line its parent was in. Hits do not depend on the node order.
+    {
+        std::vector<int32_t> remap(ninner, -1);
+        std::vector<int32_t> order;
+        order.reserve(inner);
+        std::vector<int32_t> dfs{0};
+        while (!dfs.empty())
+        {
+            const int32_t nd = dfs.back();
+            dfs.pop_back();
+            remap[nd] = (int32_t)order.size();
+            order.push_back(nd);
+            const int32_t c1 = hn[nd].d.y, c0 = hn[nd].d.x;
+            if (n > 1 && c1 >= 0)
+                dfs.push_back(c1);
+            if (c0 >= 0)
+                dfs.push_back(c0);
+        }
+        std::vector<BvhNode> laid(order.size());
+        for (size_t i = 0; i < order.size(); i++)
+        {
+            BvhNode nd = hn[order[i]];
+            if (nd.d.x >= 0)
+                nd.d.x = remap[nd.d.x];
+            if (n > 1 && nd.d.y >= 0)
+                nd.d.y = remap[nd.d.y];
+            laid[i] = nd;
+        }
+        HIP_CHECK(hipMemcpy(bvh.nodes.ptr, laid.data(), sizeof(BvhNode) * laid.size(), hipMemcpyHostToDevice));
+    }
     bvh.node_count = inner;
     bvh.leaf_count = leaves;
     bvh.max_depth = maxd;
