@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <utility>
 
 #include "internal.h"
@@ -440,19 +441,26 @@ QueuesFn pick_queues(int block, int stack)
 unsigned persistent_grid(Ctx& ctx, const void* fn, uint32_t n)
 {
     const int block = ctx.trace_cfg.block_size;
+    // Blocks per CU of each kernel on each device, computed once; contexts on
+    // several host threads share the table.
     static std::map<std::pair<const void*, int>, int> occupancy;
-    auto key = std::make_pair(fn, ctx.device);
-    auto it = occupancy.find(key);
+    static std::mutex occupancy_lock;
+    const auto key = std::make_pair(fn, ctx.device);
     int per_cu = 0;
-    if (it == occupancy.end())
+    {
+        std::lock_guard<std::mutex> guard(occupancy_lock);
+        auto it = occupancy.find(key);
+        if (it != occupancy.end())
+            per_cu = it->second;
+    }
+    if (per_cu == 0)
     {
         HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
         if (per_cu < 1)
             per_cu = 1;
+        std::lock_guard<std::mutex> guard(occupancy_lock);
         occupancy[key] = per_cu;
     }
-    else
-        per_cu = it->second;
     const uint64_t max_grid = (uint64_t)per_cu * (uint64_t)ctx.cu_count;
     const uint64_t need = ((uint64_t)n + block - 1) / block;
     const uint64_t g = need < max_grid ? need : max_grid;
