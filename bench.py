@@ -156,7 +156,7 @@ def main():
 
         rays_host = mipt.from_device(d_rays, mipt.RAY_DTYPE)[:n]
         # The host cores this process may use (OMP_NUM_THREADS caps it on a
-        # shared GPU box), not every core the machine reports.
+        # shared GPU host), not every core the machine reports.
         cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         tree = oracle.Octree(scene.vertices, scene.indices)
         sample = rays_host[::2]

@@ -156,7 +156,7 @@ __device__ __forceinline__ bool init_path(const PtData& d, uint32_t pix)
     const int32_t prim = d.gb_prim[pix];
     if (prim < 0)
     {
-        d.flags[pix] = 0u; // camera ray missed: no envmap in this build's scenes
+        d.flags[pix] = 0u; // camera ray missed: the scenes have no envmap
         return false;
     }
     const f3 p = ld3(d.gb_pos[pix]);

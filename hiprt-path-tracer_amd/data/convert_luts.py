@@ -15,7 +15,7 @@ ltc_parameters_table_approximation (src/Device/includes/BSDFs/SheenLTCFittedPara
 the published fit of Zeltner et al. 2022), each decimal literal rounded to the
 nearest binary32 as a C compiler does.
 
-Run once in the build container (the reference is not present on the GPU box):
+Run once where the reference sources are present (the GPU hosts do not have them):
     python hiprt-path-tracer_amd/data/convert_luts.py /path/to/reference
 """
 import os

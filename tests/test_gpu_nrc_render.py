@@ -13,7 +13,7 @@
   The equal-time comparison runs at 1920x1080 in bench.py, where the GPU
   work, not the launch rate of a 256x256 frame, sets the time.
 - Cache off leaves the plain passes as they are (their bit-exact parity is
-  covered by test_gpu_pathtrace.py; here the plain form is only the judge).
+  covered by test_gpu_pathtrace.py; here the plain form is only the yardstick).
 """
 import time
 
