@@ -68,6 +68,8 @@ def main():
                     help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain, with the "
                          "radiance cache or as cache warm-up frames, and print ms per sample (a kernel-profiling "
                          "workload)")
+    ap.add_argument("--restir-layered", action="store_true",
+                    help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -107,6 +109,7 @@ def main():
     ctx = mipt.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
+    bvh_name = "LBVH"
     if args.block or args.stack:
         ctx.set_trace_config(args.block, args.stack)
     info = ctx.bvh_info()
@@ -173,20 +176,40 @@ def main():
     # (scaled by a plain copy), recorded in profiles/ for this kernel and
     # launch configuration; null when the configuration differs.
     traffic = None
+    l2_hit_rate = None
     prof = os.path.join(ROOT, "profiles", "trace_closest_config2_rinc.json")
     if os.path.exists(prof) and not (args.block or args.stack):
         with open(prof) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            pj = json.load(f)
+        if pj.get("bvh", "LBVH") == bvh_name:
+            traffic = pj.get("traffic_bytes_per_launch")
+            l2_hit_rate = pj.get("l2_hit_rate")
+    # What bounds the kernel, from the counters: the algorithmic bytes are an
+    # effective bandwidth (they grow with node visits, most of which L2 and
+    # the Infinity Cache serve); the measured HBM bytes say whether HBM is
+    # the limit.
+    measured = None
+    limiter = None
+    if traffic:
+        hbm_gbs = traffic / mean_launch_s / 1e9
+        measured = {"hbm_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
+                    "traffic_over_algorithmic": round(traffic / (n * b_ray), 4), "l2_hit_rate": l2_hit_rate}
+        limiter = ("hbm bandwidth" if hbm_gbs / HBM_PEAK_GBS >= 0.5 else
+                   f"dependent node-fetch latency and L2 (HBM traffic is {traffic / (n * b_ray) * 100:.1f} % of the "
+                   f"algorithmic bytes and {hbm_gbs / HBM_PEAK_GBS * 100:.1f} % of HBM peak; L2 hit rate "
+                   f"{(l2_hit_rate or 0) * 100:.1f} %)")
 
     if not args.no_render:
         step("plain render leg: config 1, 512x512, 16 x 1-spp passes")
     render = None if args.no_render else plain_render_leg(mipt, torch, np, args.no_cpu)
-    # The ReSTIR DI leg (config 2 scene at 1920x1080) made an illegal memory
-    # access whose cause is not found; the library refuses ReSTIR DI above
-    # MIPT_RESTIR_MAX_PIXELS until it is.
-    restir = None if args.no_render else {
-        "skipped": "ReSTIR DI at 1920x1080 is refused by the library (MIPT_RESTIR_MAX_PIXELS = 32,400) until an "
-                   "illegal memory access seen there is explained"}
+    if not args.no_render:
+        step("ReSTIR DI leg: config 2 scene, 1920x1080, 5 bounces, diffuse + metal lobes")
+    restir = None if args.no_render else restir_render_leg(mipt, torch, np, layered=False)
+    if not args.no_render and args.restir_layered:
+        step("ReSTIR DI leg: config 2 scene, 1920x1080, 5 bounces, full layered materials")
+        restir_layered = restir_render_leg(mipt, torch, np, layered=True)
+    else:
+        restir_layered = None
     if not args.no_cache:
         step("radiance cache vs plain leg: config 1, 1920x1080")
     cache = None if args.no_cache else cache_vs_plain_leg(mipt, torch, np, args.cache_ref_spp, args.cache_plain_spp,
@@ -206,11 +229,13 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded procedural scene, GPU-generated ray set)",
         "config": {"workload": "config 2 scene (1,040,384 tris) at 1920x1080, R_inc incoherent bounce rays, "
-                               "closest hit", "rays_per_step": n, "bvh": "LBVH (SAH-collapsed leaves)",
+                               "closest hit", "rays_per_step": n, "bvh": bvh_name,
                    "block": args.block or 128,
                    "lds_stack": args.stack or 16},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "achieved_kind": "algorithmic bytes (B_ray x rays) / mean launch time: an effective bandwidth",
+                     "measured": measured, "limiter": limiter,
                      "traffic_source": "profiles/trace_closest_config2_rinc.md" if traffic else None,
                      "bytes_per_ray": round(b_ray, 1), "mean_nodes_per_ray": round(mean_nodes, 2),
                      "mean_tris_per_ray": round(mean_tris, 2), "mean_launch_ms": round(mean_launch_s * 1e3, 4)},
@@ -220,6 +245,7 @@ def main():
         "hit_fraction": round(float((hits["prim"] >= 0).mean()), 4),
         "plain_render": render,
         "restir_render": restir,
+        "restir_render_layered": restir_layered,
         "cache_vs_plain": cache,
     }
     print(json.dumps(out))
@@ -383,6 +409,78 @@ def cache_vs_plain_leg(mipt, torch, np, ref_spp, plain_spp, warmup, cache_over=N
             "cache_dropped_targets": dropped, "cache_train_loss_last_frame": loss,
             "curve_plain": [[round(t, 4), s, e] for t, s, e in plain],
             "curve_cache": [[round(t, 4), s, e] for t, s, e in cached]}
+
+
+def restir_render_leg(mipt, torch, np, layered):
+    """ReSTIR DI at scale: the 1,040,384-triangle scene at 1920x1080 with 4096
+    power-sampled emitters; 32 light + 1 BSDF initial candidates, temporal
+    reuse (M-cap 20), 2 spatial passes of 5 neighbours in 16 px, pairwise
+    MIS, 5 bounces. With layered=False the materials keep their diffuse and
+    metal lobes only (coat, sheen, specular, transmission and thin film set
+    to 0); with layered=True they are the scene's full layered Principled
+    materials. Times whole frames (camera + ReSTIR DI passes + path pass)
+    and each part, with HIP events on the launch stream, after 2 warm-up
+    frames."""
+    W, H = 1920, 1080
+    scene = mipt.generate_scene(2, W, H)
+    if not layered:
+        for m in scene.materials:
+            m.coat = m.sheen = m.specular_transmission = m.diffuse_transmission = m.thin_film = m.specular = 0.0
+    s = mipt.default_render_settings(width=W, height=H, nb_bounces=5,
+                                     direct_light_sampling_strategy=mipt.MIPT_LSS_RESTIR_DI,
+                                     direct_light_sampling_base_strategy=mipt.MIPT_LSS_BASE_POWER)
+    rs = mipt.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20, spatial_number_of_passes=2)
+    ctx = mipt.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    ctx.set_restir_di_settings(rs)
+    ctx.set_render_settings(s)
+    ctx.set_camera(scene.camera)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    draws = ctx.draws_per_sample()
+    for f in range(2):
+        ctx.render_sample(f, sh)
+    stream.synchronize()
+    frames = 5
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(frames)]
+    t0 = time.perf_counter()
+    for i in range(frames):
+        f = 2 + i
+        k = f * draws
+        e = ev[i]
+        e[0].record(stream)
+        ctx.launch_pass(mipt.MIPT_PASS_CAMERA_RAYS, f, mipt.host_random_number(k), sh)
+        e[1].record(stream)
+        ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, f, mipt.host_random_number(k + 1), sh)
+        ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, f, mipt.host_random_number(k + 2), sh)
+        for p in range(rs.spatial_number_of_passes):
+            ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(p), f, mipt.host_random_number(k + 4 + p), sh)
+        e[2].record(stream)
+        ctx.launch_pass(mipt.MIPT_PASS_PATH_TRACE, f, mipt.host_random_number(k + draws - 1), sh)
+        e[3].record(stream)
+    stream.synchronize()
+    frame_ms = (time.perf_counter() - t0) / frames * 1e3
+    cam_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    restir_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    path_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
+    res = ctx.read_reservoirs(3)
+    hit = ctx.read_gbuffer_prims() >= 0
+    rej = ctx.rejected_samples()
+    fb = ctx.read_framebuffer()
+    ctx.close()
+    return {"workload": "config 2 scene (1,040,384 tris, 4096 emitters) 1920x1080, ReSTIR DI 32+1 candidates, "
+                        "temporal M-cap 20, 2 spatial passes x 5 neighbours r=16, pairwise MIS, 5 bounces; "
+                        + ("full layered Principled materials" if layered else "diffuse + metal lobes"),
+            "frames": 2 + frames, "ms_per_frame": round(frame_ms, 3), "camera_pass_ms": round(cam_ms, 3),
+            "restir_di_passes_ms": round(restir_ms, 3), "path_pass_ms": round(path_ms, 3),
+            "rejected_samples": rej,
+            "shaded_reservoirs": {"hit_pixels": int(hit.sum()),
+                                  "M_min_on_hits": int(res["M"][hit].min()) if hit.any() else None,
+                                  "M_max": int(res["M"].max()),
+                                  "ucw_positive": int((res["ucw"] > 0).sum()),
+                                  "ucw_nonfinite": int((~np.isfinite(res["ucw"])).sum())},
+            "mean_radiance": [round(float(x), 5) for x in (fb.reshape(-1, 3) / (2 + frames)).mean(axis=0)]}
 
 
 def plain_render_leg(mipt, torch, np, no_cpu):

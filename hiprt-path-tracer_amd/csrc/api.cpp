@@ -617,13 +617,6 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         if (ctx->render.scene_dispersive && ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
             mipt::fail(MIPT_ERR_UNSUPPORTED, "ReSTIR DI with a dispersive glass material (dispersion_scale > 0 and "
                                              "specular_transmission > 0) is not implemented yet");
-        // ReSTIR DI on the 1,040,384-triangle scene at 1920x1080 made an
-        // illegal memory access whose cause is not found yet; images up to
-        // 32,400 pixels (240x135) are verified bit-exact. Larger ones are
-        // refused until the cause is fixed.
-        if (ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI &&
-            ctx->render.pixels > MIPT_RESTIR_MAX_PIXELS)
-            mipt::fail(MIPT_ERR_UNSUPPORTED, "ReSTIR DI is limited to images of at most 32,400 pixels in this build");
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
         if (pass == MIPT_PASS_CAMERA_RAYS)
             mipt::camera_rays_pass(*ctx, sample_number, random_number, s);

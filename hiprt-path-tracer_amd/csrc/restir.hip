@@ -193,13 +193,18 @@ struct Surface {
 };
 
 // The surface's volume state (RayVolumeState::reconstruct_first_hit,
-// RayVolumeState.h:35-59): its material pushed on an empty stack. Dispersive
-// glass, which would also draw a wavelength, is refused with ReSTIR DI.
+// RayVolumeState.h:35-59): its material pushed on an empty stack. A surface
+// without a primary hit keeps the empty stack (the early return at
+// RayVolumeState.h:37-39): a previous-frame pixel whose camera ray missed has
+// prim -1 and the material of an older hit, and reaches the temporal target
+// function. Dispersive glass, which would also draw a wavelength, is refused
+// with ReSTIR DI.
 __device__ __forceinline__ pt::VolumeState surface_volume(const PtData& d, const Surface& s)
 {
     pt::VolumeState v;
     pt::vol_init(v);
-    pt::vol_push(v, d.mat_idx[s.prim], d.mats[s.mat].dielectric_priority);
+    if (s.prim >= 0)
+        pt::vol_push(v, d.mat_idx[s.prim], d.mats[s.mat].dielectric_priority);
     return v;
 }
 

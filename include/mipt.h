@@ -258,10 +258,6 @@ typedef struct mipt_render_settings {
     int32_t radiance_cache;
 } mipt_render_settings;
 
-/* ReSTIR DI passes (and the path pass under ReSTIR DI) refuse images with
- * more pixels than this (MIPT_ERR_UNSUPPORTED): a 1920x1080 render of the
- * 1M-triangle scene made an illegal memory access not yet explained. */
-#define MIPT_RESTIR_MAX_PIXELS 32400
 #define MIPT_LSS_MIS_LIGHT_BSDF 3 /* LSS_MIS_LIGHT_BSDF: one light sample + one BSDF sample, balance heuristic */
 #define MIPT_LSS_RESTIR_DI 5      /* LSS_RESTIR_DI: ReSTIR DI reservoir at bounce 0, MIS light + BSDF after */
 #define MIPT_LSS_BASE_UNIFORM 0   /* LSS_BASE_UNIFORM: emissive triangle picked uniformly */

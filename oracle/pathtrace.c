@@ -2420,15 +2420,17 @@ void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t rn, in
 typedef struct { int32_t mat, prim; v3 view, sn, gn, p; } osurf;
 
 /* the surface's volume state: its material pushed on an empty stack
- * (RayVolumeState::reconstruct_first_hit); dispersive glass is refused with
- * ReSTIR DI */
-static ovol surf_vol(const orc_frame* f, const osurf* s);
-
+ * (RayVolumeState::reconstruct_first_hit, RayVolumeState.h:35-59). A surface
+ * without a primary hit (prim -1: a previous-frame pixel whose camera ray
+ * missed, with the material left from an older hit) keeps the empty stack,
+ * as the reference's early return does (RayVolumeState.h:37-39). Dispersive
+ * glass is refused with ReSTIR DI. */
 static ovol surf_vol(const orc_frame* f, const osurf* s)
 {
     ovol v;
     vol_init(&v);
-    vol_push(&v, f->R.mat_idx[s->prim], f->R.mats[s->mat].dielectric_priority);
+    if (s->prim >= 0)
+        vol_push(&v, f->R.mat_idx[s->prim], f->R.mats[s->mat].dielectric_priority);
     return v;
 }
 

@@ -39,3 +39,24 @@ def scene1(mipt_mod):
 @pytest.fixture(scope="session")
 def scene2(mipt_mod):
     return mipt_mod.generate_scene(2, 1920, 1080)
+
+
+def open_back_box(mipt, width, height):
+    """The config-1 scene without the Cornell box's back wall (triangles
+    8-11, the quads at z = -1): the camera looks through the box into empty
+    space, so the far edge of the lit floor and the sphere outlines lie
+    against the background. Under camera jitter those pixels alternate
+    between a hit and a miss from one frame to the next."""
+    import numpy as np
+
+    full = mipt.generate_scene(1, width, height)
+    keep = np.ones(full.triangle_count, bool)
+    keep[8:12] = False
+    assert (full.emissive >= 12).all()
+    return mipt.Scene(full.vertices, full.indices[keep], full.material_indices[keep], full.materials,
+                      full.emissive - 4, full.camera, width, height)
+
+
+@pytest.fixture(scope="session")
+def open_back_box_scene(mipt_mod):
+    return open_back_box

@@ -211,18 +211,14 @@ def test_restir_pass_errors(mipt_mod, torch_mod):
     ctx.close()
 
 
-def test_restir_above_the_verified_size_is_refused(mipt_mod, torch_mod):
-    """ReSTIR DI is refused above MIPT_RESTIR_MAX_PIXELS (32,400): a
-    1920x1080 render of the 1M-triangle scene made an illegal memory access
-    whose cause is not found. The refusal comes before any pass kernel."""
-    scene = mipt_mod.generate_scene(1, 192, 192)
-    ctx = mipt_mod.Context(0)
-    ctx.set_scene(scene)
-    ctx.build_bvh()
-    ctx.set_render_settings(mipt_mod.default_render_settings(width=192, height=192, direct_light_sampling_strategy=5))
-    ctx.set_camera(scene.camera)
-    with pytest.raises(mipt_mod.MiptError) as e:
-        ctx.render_sample(0)
-    assert "32,400" in str(e.value)
-    ctx.close()
+def test_restir_temporal_surface_that_missed(mipt_mod, oracle_mod, torch_mod, open_back_box_scene):
+    """Regression: pixels of the open-back box go hit -> miss -> hit under
+    camera jitter, so the temporal pass evaluates a previous-frame surface
+    without a primary hit (prim -1, material of an older hit). Reading the
+    material index of primitive -1 there made the illegal memory access of
+    ReSTIR DI at 1920x1080; the surface now keeps the empty volume stack
+    (RayVolumeState.h:37-39). Every frame bit-exact against the oracle."""
+    scene = open_back_box_scene(mipt_mod, 32, 32)
+    s = mipt_mod.default_render_settings(width=32, height=32, nb_bounces=1, direct_light_sampling_strategy=5)
+    render_both(mipt_mod, oracle_mod, scene, s, mipt_mod.default_restir_di_settings(), 5, check_every_frame=True)
 

@@ -151,3 +151,39 @@ def test_backfacing_bsdf_candidate_weight(mipt_mod, oracle_mod):
         counts[guard] = (bad, R.rejected)
     assert counts[0][0] > 0
     assert counts[1] == (0, 0)
+
+
+def test_temporal_reuse_of_a_surface_that_missed(mipt_mod, oracle_mod, open_back_box_scene):
+    """A pixel whose camera ray hits in frame f-2, misses in f-1 and hits in
+    f: the temporal pass of frame f evaluates the initial candidate at the
+    previous-frame surface, which has no primary hit (prim -1) but keeps the
+    material of the older hit. Its volume state is the empty stack
+    (RayVolumeState.h:37-39); reading the material index of primitive -1
+    was an out-of-bounds access. This pins that the open-back box reaches
+    the case within 4 frames: such pixels exist with a temporal reservoir
+    (M > 0) and an initial candidate on a light, and the oracle's temporal
+    output for them is a finite reservoir."""
+    Wt = Ht = 24
+    scene = open_back_box_scene(mipt_mod, Wt, Ht)
+    s = mipt_mod.default_render_settings(width=Wt, height=Ht, nb_bounces=1, direct_light_sampling_strategy=5)
+    rs = mipt_mod.default_restir_di_settings()
+    R = oracle_mod.Renderer(scene, s, rs)
+    hist = []
+    reached = 0
+    for f in range(4):
+        rn = [mipt_mod.host_random_number(f * 6 + k) for k in range(6)]
+        R.camera_pass(f, rn[0])
+        hist.append(R.gb_prims.copy())
+        tn = R.read_reservoirs(1)  # the last frame's output: this frame's temporal input
+        R.restir_initial(f, rn[1])
+        if f >= 2:
+            sel = (hist[-3] >= 0) & (hist[-2] < 0) & (hist[-1] >= 0)
+            sel &= (tn["M"] > 0) & (R.read_reservoirs(0)["tri"] >= 0)
+            reached += int(sel.sum())
+        R.restir_temporal(f, rn[2])
+        if f >= 2:
+            out = R.read_reservoirs(0)
+            assert np.isfinite(out["ucw"][sel]).all() and (out["M"][sel] >= 1).all()
+        R.restir_spatial(0, f, rn[4])
+        R.path_trace(f, rn[5])
+    assert reached >= 4
