@@ -68,6 +68,7 @@ def main():
                     help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain, with the "
                          "radiance cache or as cache warm-up frames, and print ms per sample (a kernel-profiling "
                          "workload)")
+    ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
     ap.add_argument("--block", type=int, default=0)
@@ -202,10 +203,11 @@ def main():
     if not args.no_render:
         step("plain render leg: config 1, 512x512, 16 x 1-spp passes")
     render = None if args.no_render else plain_render_leg(mipt, torch, np, args.no_cpu)
-    if not args.no_render:
+    with_restir = not (args.no_render or args.no_restir)
+    if with_restir:
         step("ReSTIR DI leg: config 2 scene, 1920x1080, 5 bounces, diffuse + metal lobes")
-    restir = None if args.no_render else restir_render_leg(mipt, torch, np, layered=False)
-    if not args.no_render and args.restir_layered:
+    restir = restir_render_leg(mipt, torch, np, layered=False) if with_restir else None
+    if with_restir and args.restir_layered:
         step("ReSTIR DI leg: config 2 scene, 1920x1080, 5 bounces, full layered materials")
         restir_layered = restir_render_leg(mipt, torch, np, layered=True)
     else:

@@ -408,9 +408,12 @@ int mipt_build_bvh(mipt_ctx* c, int32_t builder)
     return mipt::guarded(ctx, [&] {
         if (!ctx->scene_set)
             mipt::fail(MIPT_ERR_NO_SCENE, "mipt_build_bvh: no scene");
-        if (builder != MIPT_BVH_LBVH)
+        if (builder == MIPT_BVH_LBVH)
+            mipt::build_lbvh(*ctx);
+        else if (builder == MIPT_BVH_SAH_BINNED)
+            mipt::build_sah_binned(*ctx);
+        else
             mipt::fail(MIPT_ERR_UNSUPPORTED, "mipt_build_bvh: unknown builder " + std::to_string(builder));
-        mipt::build_lbvh(*ctx);
     });
 }
 
@@ -432,6 +435,25 @@ int mipt_get_bvh_info(const mipt_ctx* c, mipt_bvh_info* out)
     out->triangle_bytes = (int64_t)b.triangle_count * (int64_t)sizeof(mipt::TriRecord);
     out->build_ms = b.build_ms;
     out->sah_cost = b.sah_cost;
+    return MIPT_OK;
+}
+
+int mipt_read_bvh(const mipt_ctx* c, void* host_nodes, void* host_triangles)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (!ctx->bvh.valid)
+        return MIPT_ERR_NO_BVH;
+    const mipt::Bvh& b = ctx->bvh;
+    if (host_nodes && b.node_count > 0)
+        if (hipMemcpy(host_nodes, b.nodes.ptr, sizeof(mipt::BvhNode) * (size_t)b.node_count,
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return MIPT_ERR_HIP;
+    if (host_triangles && b.triangle_count > 0)
+        if (hipMemcpy(host_triangles, b.tris.ptr, sizeof(mipt::TriRecord) * (size_t)b.triangle_count,
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return MIPT_ERR_HIP;
     return MIPT_OK;
 }
 

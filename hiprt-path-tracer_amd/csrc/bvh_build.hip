@@ -3,7 +3,8 @@
 // absent HIPRT library).
 //
 // Pipeline (all on the device, one stream):
-//   1. per-triangle boxes + scene/centroid bounds (wave reduction + atomics)
+//   1. per-triangle boxes + scene/centroid bounds (block partials, folded
+//      without atomics)
 //   2. 63-bit Morton code of each box centre (21 bits per axis)
 //   3. radix sort (Morton, primitive) pairs
 //   4. Karras 2012 hierarchy: node i of n-1 interior nodes from the key prefixes
@@ -11,10 +12,8 @@
 //      and a collapse flag (subtree of <= MAX_LEAF_SIZE triangles becomes one
 //      leaf when that is cheaper)
 //   6. emit 64-byte BvhNodes and 48-byte TriRecords in Morton order.
-//
-// Boxes are padded outwards (relative 2^-20 of the coordinate plus 2^-22 of
-// the scene extent) so that box tests never cull a triangle that the exact
-// Moller-Trumbore test accepts.
+// Boxes are padded (bvh_common.h). finalize_bvh (host) validates the tree,
+// computes its statistics and lays it out depth first for both builders.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,80 +23,18 @@
 
 #include <rocprim/rocprim.hpp>
 
-#include "internal.h"
-#include "pt_math.h"
+#include "bvh_common.h"
 
 namespace mipt {
 namespace {
 
-constexpr float SAH_C_NODE = 1.0f;
-constexpr float SAH_C_TRI = 1.0f;
-
-__device__ __forceinline__ uint32_t f2o(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__host__ __device__ __forceinline__ float o2f(uint32_t u)
-{
-    uint32_t v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
-    float f;
-    memcpy(&f, &v, 4);
-    return f;
-}
-
-__device__ __forceinline__ float wave_min(float v)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        v = fminf(v, __shfl_xor(v, off, 64));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// Bounds layout in `bounds` (uint32 ordered floats): [0..2] scene lo, [3..5]
-// scene hi, [6..8] centroid lo, [9..11] centroid hi.
-__global__ void k_tri_bounds(const float* __restrict__ verts, const int32_t* __restrict__ idx, int32_t n,
-                             float4* __restrict__ lo_out, float4* __restrict__ hi_out,
-                             uint32_t* __restrict__ bounds)
-{
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float v[12];
-    if (i < n)
-    {
-        pt::f3 p[3];
-        for (int k = 0; k < 3; k++)
-        {
-            int32_t vi = idx[3 * (int64_t)i + k];
-            p[k] = pt::mk(verts[3 * (int64_t)vi], verts[3 * (int64_t)vi + 1], verts[3 * (int64_t)vi + 2]);
-        }
-        pt::f3 lo = pt::fmin3(p[0], pt::fmin3(p[1], p[2]));
-        pt::f3 hi = pt::fmax3(p[0], pt::fmax3(p[1], p[2]));
-        lo_out[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
-        hi_out[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
-        float cx = 0.5f * (lo.x + hi.x), cy = 0.5f * (lo.y + hi.y), cz = 0.5f * (lo.z + hi.z);
-        float t[12] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, cx, cy, cz, cx, cy, cz};
-        for (int k = 0; k < 12; k++) v[k] = t[k];
-    }
-    else
-    {
-        for (int k = 0; k < 12; k++) v[k] = (k % 6) < 3 ? INFINITY : -INFINITY;
-    }
-    for (int k = 0; k < 12; k++)
-        v[k] = ((k % 6) < 3) ? wave_min(v[k]) : wave_max(v[k]);
-    if ((threadIdx.x & 63) == 0)
-    {
-        for (int k = 0; k < 12; k++)
-        {
-            if ((k % 6) < 3) atomicMin(&bounds[k], f2o(v[k]));
-            else atomicMax(&bounds[k], f2o(v[k]));
-        }
-    }
-}
+using bvhc::f2o;
+using bvhc::f4max;
+using bvhc::f4min;
+using bvhc::half_area;
+using bvhc::o2f;
+using bvhc::SAH_C_NODE;
+using bvhc::SAH_C_TRI;
 
 __device__ __forceinline__ uint64_t expand21(uint64_t v)
 {
@@ -191,33 +128,13 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, int32_t n, int32_t* 
         node_parent[0] = -1;
 }
 
-__device__ __forceinline__ float half_area(float4 lo, float4 hi)
-{
-    float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
-    return dx * dy + dy * dz + dz * dx;
-}
-
-// Padded box of sorted leaf k. Pad = 2^-20 |coord| + 2^-22 * scene extent.
+// Padded box of sorted leaf k.
 __device__ __forceinline__ void leaf_box(const float4* __restrict__ plo, const float4* __restrict__ phi,
                                          const uint32_t* __restrict__ vals, int32_t k, float abs_pad,
                                          float4& lo, float4& hi)
 {
-    uint32_t p = vals[k];
-    float4 a = plo[p], b = phi[p];
-    const float rel = 9.5367431640625e-07f; // 2^-20
-    lo = make_float4(a.x - (fabsf(a.x) * rel + abs_pad), a.y - (fabsf(a.y) * rel + abs_pad),
-                     a.z - (fabsf(a.z) * rel + abs_pad), 0.0f);
-    hi = make_float4(b.x + (fabsf(b.x) * rel + abs_pad), b.y + (fabsf(b.y) * rel + abs_pad),
-                     b.z + (fabsf(b.z) * rel + abs_pad), 0.0f);
-}
-
-__device__ __forceinline__ float4 f4min(float4 a, float4 b)
-{
-    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
-}
-__device__ __forceinline__ float4 f4max(float4 a, float4 b)
-{
-    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.0f);
+    const uint32_t p = vals[k];
+    bvhc::pad_box(plo[p], phi[p], abs_pad, lo, hi);
 }
 
 __global__ void k_refit(int32_t n, const float4* __restrict__ plo, const float4* __restrict__ phi,
@@ -306,170 +223,32 @@ __global__ void k_emit_nodes(int32_t n, const float4* __restrict__ plo, const fl
     nodes[i] = nd;
 }
 
-__global__ void k_emit_tris(const float* __restrict__ verts, const int32_t* __restrict__ idx, int32_t n,
-                            const uint32_t* __restrict__ vals, TriRecord* __restrict__ tris)
-{
-    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n)
-        return;
-    int32_t prim = (int32_t)vals[k];
-    pt::f3 p[3];
-    for (int j = 0; j < 3; j++)
-    {
-        int32_t vi = idx[3 * (int64_t)prim + j];
-        p[j] = pt::mk(verts[3 * (int64_t)vi], verts[3 * (int64_t)vi + 1], verts[3 * (int64_t)vi + 2]);
-    }
-    pt::f3 e1 = p[1] - p[0];
-    pt::f3 e2 = p[2] - p[0];
-    TriRecord r;
-    r.v0 = make_float4(p[0].x, p[0].y, p[0].z, __int_as_float(prim));
-    r.e1 = make_float4(e1.x, e1.y, e1.z, 0.0f);
-    r.e2 = make_float4(e2.x, e2.y, e2.z, 0.0f);
-    tris[k] = r;
-}
-
-__global__ void k_pad(const uint32_t* __restrict__ bounds, float* __restrict__ pad)
-{
-    float ext = 0.0f;
-    for (int k = 0; k < 3; k++)
-    {
-        float e = o2f(bounds[3 + k]) - o2f(bounds[k]);
-        ext = fmaxf(ext, e);
-    }
-    pad[0] = ext * 2.384185791015625e-07f; // 2^-22
-}
-
-inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+using bvhc::blocks;
 
 } // namespace
 
-void build_lbvh(Ctx& ctx)
+void write_single_triangle_root(Ctx& ctx, const float4* d_lo, const float4* d_hi)
 {
-    const int32_t n = ctx.triangle_count;
+    float4 lo, hi;
+    HIP_CHECK(hipMemcpy(&lo, d_lo, sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(&hi, d_hi, sizeof(float4), hipMemcpyDeviceToHost));
+    const float ext = std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z});
+    float4 plo, phi;
+    bvhc::pad_box(lo, hi, ext * bvhc::PAD_ABS, plo, phi);
+    BvhNode nd;
+    nd.a = make_float4(plo.x, plo.y, plo.z, phi.x);
+    // Empty second child: every slab of an all-(+inf) box is at +inf or -inf,
+    // so the box test always fails.
+    nd.b = make_float4(phi.y, phi.z, INFINITY, INFINITY);
+    nd.c = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+    nd.d = make_int4(make_leaf(0, 1), make_leaf(0, 1), 0, 0);
+    HIP_CHECK(hipMemcpy(ctx.bvh.nodes.ptr, &nd, sizeof(nd), hipMemcpyHostToDevice));
+}
+
+void finalize_bvh(Ctx& ctx, int32_t ninner)
+{
     Bvh& bvh = ctx.bvh;
-    bvh.valid = false;
-    bvh.builder = MIPT_BVH_LBVH;
-    bvh.triangle_count = n;
-    if (n <= 0)
-    {
-        bvh.node_count = bvh.leaf_count = bvh.max_depth = bvh.max_leaf_size = 0;
-        bvh.valid = true;
-        return;
-    }
-    const float* verts = ctx.vertices.as<float>();
-    const int32_t* idx = ctx.indices.as<int32_t>();
-
-    hipEvent_t ev0, ev1;
-    HIP_CHECK(hipEventCreate(&ev0));
-    HIP_CHECK(hipEventCreate(&ev1));
-
-    DeviceBuffer plo, phi, bounds, pad, keys, keys_sorted, vals, vals_sorted, sort_tmp;
-    plo.reserve(sizeof(float4) * n);
-    phi.reserve(sizeof(float4) * n);
-    bounds.reserve(sizeof(uint32_t) * 12);
-    pad.reserve(sizeof(float));
-    keys.reserve(sizeof(uint64_t) * n);
-    keys_sorted.reserve(sizeof(uint64_t) * n);
-    vals.reserve(sizeof(uint32_t) * n);
-    vals_sorted.reserve(sizeof(uint32_t) * n);
-
-    std::vector<uint32_t> init(12);
-    for (int k = 0; k < 12; k++)
-    {
-        float f = (k % 6) < 3 ? INFINITY : -INFINITY;
-        uint32_t u;
-        memcpy(&u, &f, 4);
-        init[k] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-    }
-    HIP_CHECK(hipMemcpy(bounds.ptr, init.data(), 12 * sizeof(uint32_t), hipMemcpyHostToDevice));
-
-    const hipStream_t s = nullptr;
-    HIP_CHECK(hipEventRecord(ev0, s));
-    const int B = 256;
-    hipLaunchKernelGGL(k_tri_bounds, dim3(blocks(n, B)), dim3(B), 0, s, verts, idx, n, plo.as<float4>(),
-                       phi.as<float4>(), bounds.as<uint32_t>());
-    hipLaunchKernelGGL(k_pad, dim3(1), dim3(1), 0, s, bounds.as<uint32_t>(), pad.as<float>());
-    hipLaunchKernelGGL(k_morton, dim3(blocks(n, B)), dim3(B), 0, s, plo.as<float4>(), phi.as<float4>(), n,
-                       bounds.as<uint32_t>(), keys.as<uint64_t>(), vals.as<uint32_t>());
-    HIP_CHECK(hipGetLastError());
-
-    size_t tmp_bytes = 0;
-    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
-                                        vals.as<uint32_t>(), vals_sorted.as<uint32_t>(), (size_t)n, 0u, 63u, s));
-    sort_tmp.reserve(tmp_bytes > 0 ? tmp_bytes : 1);
-    HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
-                                        vals.as<uint32_t>(), vals_sorted.as<uint32_t>(), (size_t)n, 0u, 63u, s));
-
-    const int32_t ninner = n > 1 ? n - 1 : 1;
-    DeviceBuffer child, node_parent, leaf_parent, range_lo, range_hi, visits, nlo, nhi, cost, collapse;
-    child.reserve(sizeof(int32_t) * 2 * ninner);
-    node_parent.reserve(sizeof(int32_t) * ninner);
-    leaf_parent.reserve(sizeof(int32_t) * n);
-    range_lo.reserve(sizeof(int32_t) * ninner);
-    range_hi.reserve(sizeof(int32_t) * ninner);
-    visits.reserve(sizeof(uint32_t) * ninner);
-    nlo.reserve(sizeof(float4) * ninner);
-    nhi.reserve(sizeof(float4) * ninner);
-    cost.reserve(sizeof(float) * ninner);
-    collapse.reserve(sizeof(uint8_t) * ninner);
-    HIP_CHECK(hipMemsetAsync(visits.ptr, 0, sizeof(uint32_t) * ninner, s));
-    HIP_CHECK(hipMemsetAsync(node_parent.ptr, 0xff, sizeof(int32_t) * ninner, s));
-    HIP_CHECK(hipMemsetAsync(leaf_parent.ptr, 0xff, sizeof(int32_t) * n, s));
-    HIP_CHECK(hipMemsetAsync(collapse.ptr, 0, sizeof(uint8_t) * ninner, s));
-
-    bvh.nodes.reserve(sizeof(BvhNode) * ninner);
-    bvh.tris.reserve(sizeof(TriRecord) * n);
-
-    if (n == 1)
-    {
-        // One triangle: a root whose first child is the leaf and whose second
-        // child has an empty box.
-        float4 lo, hi;
-        std::vector<float4> hlo(1), hhi(1);
-        HIP_CHECK(hipMemcpy(hlo.data(), plo.ptr, sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(hhi.data(), phi.ptr, sizeof(float4), hipMemcpyDeviceToHost));
-        lo = hlo[0];
-        hi = hhi[0];
-        float ext = std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z});
-        float ap = ext * 2.384185791015625e-07f;
-        auto padf = [&](float v, float sgn) { return v + sgn * (std::fabs(v) * 9.5367431640625e-07f + ap); };
-        BvhNode nd;
-        nd.a = make_float4(padf(lo.x, -1), padf(lo.y, -1), padf(lo.z, -1), padf(hi.x, 1));
-        // Empty second child: every slab of an all-(+inf) box is at +inf or
-        // -inf, so the box test always fails.
-        nd.b = make_float4(padf(hi.y, 1), padf(hi.z, 1), INFINITY, INFINITY);
-        nd.c = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
-        nd.d = make_int4(make_leaf(0, 1), make_leaf(0, 1), 0, 0);
-        HIP_CHECK(hipMemcpy(bvh.nodes.ptr, &nd, sizeof(nd), hipMemcpyHostToDevice));
-    }
-    else
-    {
-        hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, B)), dim3(B), 0, s, keys_sorted.as<uint64_t>(), n,
-                           child.as<int32_t>(), node_parent.as<int32_t>(), leaf_parent.as<int32_t>(),
-                           range_lo.as<int32_t>(), range_hi.as<int32_t>());
-        hipLaunchKernelGGL(k_refit, dim3(blocks(n, B)), dim3(B), 0, s, n, plo.as<float4>(), phi.as<float4>(),
-                           vals_sorted.as<uint32_t>(), child.as<int32_t>(), node_parent.as<int32_t>(),
-                           leaf_parent.as<int32_t>(), range_lo.as<int32_t>(), range_hi.as<int32_t>(),
-                           visits.as<uint32_t>(), nlo.as<float4>(), nhi.as<float4>(), cost.as<float>(),
-                           collapse.as<uint8_t>(), pad.as<float>());
-        hipLaunchKernelGGL(k_emit_nodes, dim3(blocks(n - 1, B)), dim3(B), 0, s, n, plo.as<float4>(),
-                           phi.as<float4>(), vals_sorted.as<uint32_t>(), child.as<int32_t>(), range_lo.as<int32_t>(),
-                           range_hi.as<int32_t>(), nlo.as<float4>(), nhi.as<float4>(), collapse.as<uint8_t>(),
-                           pad.as<float>(), bvh.nodes.as<BvhNode>());
-    }
-    hipLaunchKernelGGL(k_emit_tris, dim3(blocks(n, B)), dim3(B), 0, s, verts, idx, n, vals_sorted.as<uint32_t>(),
-                       bvh.tris.as<TriRecord>());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(ev1, s));
-    HIP_CHECK(hipEventSynchronize(ev1));
-    float ms = 0.0f;
-    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-    bvh.build_ms = ms;
-    HIP_CHECK(hipEventDestroy(ev0));
-    HIP_CHECK(hipEventDestroy(ev1));
-
-    // Host-side statistics over the reachable tree: interior nodes, leaves,
-    // depth, leaf sizes and SAH cost. Not part of the build time.
+    const int32_t n = bvh.triangle_count;
     bvh.root = 0;
     std::vector<BvhNode> hn(ninner);
     HIP_CHECK(hipMemcpy(hn.data(), bvh.nodes.ptr, sizeof(BvhNode) * ninner, hipMemcpyDeviceToHost));
@@ -573,6 +352,108 @@ void build_lbvh(Ctx& ctx)
     bvh.max_leaf_size = maxleaf;
     bvh.sah_cost = root_area > 0 ? sah / root_area : 0.0;
     bvh.valid = true;
+}
+
+void build_lbvh(Ctx& ctx)
+{
+    const int32_t n = ctx.triangle_count;
+    Bvh& bvh = ctx.bvh;
+    bvh.valid = false;
+    bvh.builder = MIPT_BVH_LBVH;
+    bvh.triangle_count = n;
+    if (n <= 0)
+    {
+        bvh.node_count = bvh.leaf_count = bvh.max_depth = bvh.max_leaf_size = 0;
+        bvh.valid = true;
+        return;
+    }
+    const float* verts = ctx.vertices.as<float>();
+    const int32_t* idx = ctx.indices.as<int32_t>();
+
+    hipEvent_t ev0, ev1;
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+
+    DeviceBuffer plo, phi, bounds, pad, partial, keys, keys_sorted, vals, vals_sorted, sort_tmp;
+    const unsigned bgrid = bvhc::bounds_grid(n);
+    plo.reserve(sizeof(float4) * n);
+    phi.reserve(sizeof(float4) * n);
+    bounds.reserve(sizeof(uint32_t) * 12);
+    pad.reserve(sizeof(float));
+    partial.reserve(sizeof(float) * 12 * bgrid);
+    keys.reserve(sizeof(uint64_t) * n);
+    keys_sorted.reserve(sizeof(uint64_t) * n);
+    vals.reserve(sizeof(uint32_t) * n);
+    vals_sorted.reserve(sizeof(uint32_t) * n);
+
+    const hipStream_t s = nullptr;
+    HIP_CHECK(hipEventRecord(ev0, s));
+    const int B = 256;
+    hipLaunchKernelGGL(bvhc::k_tri_boxes, dim3(bgrid), dim3(bvhc::BOUNDS_BLOCK), 0, s, verts, idx, n,
+                       plo.as<float4>(), phi.as<float4>(), partial.as<float>());
+    hipLaunchKernelGGL(bvhc::k_bounds_final, dim3(1), dim3(64), 0, s, partial.as<float>(), (int32_t)bgrid,
+                       bounds.as<uint32_t>(), pad.as<float>());
+    hipLaunchKernelGGL(k_morton, dim3(blocks(n, B)), dim3(B), 0, s, plo.as<float4>(), phi.as<float4>(), n,
+                       bounds.as<uint32_t>(), keys.as<uint64_t>(), vals.as<uint32_t>());
+    HIP_CHECK(hipGetLastError());
+
+    size_t tmp_bytes = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
+                                        vals.as<uint32_t>(), vals_sorted.as<uint32_t>(), (size_t)n, 0u, 63u, s));
+    sort_tmp.reserve(tmp_bytes > 0 ? tmp_bytes : 1);
+    HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
+                                        vals.as<uint32_t>(), vals_sorted.as<uint32_t>(), (size_t)n, 0u, 63u, s));
+
+    const int32_t ninner = n > 1 ? n - 1 : 1;
+    DeviceBuffer child, node_parent, leaf_parent, range_lo, range_hi, visits, nlo, nhi, cost, collapse;
+    child.reserve(sizeof(int32_t) * 2 * ninner);
+    node_parent.reserve(sizeof(int32_t) * ninner);
+    leaf_parent.reserve(sizeof(int32_t) * n);
+    range_lo.reserve(sizeof(int32_t) * ninner);
+    range_hi.reserve(sizeof(int32_t) * ninner);
+    visits.reserve(sizeof(uint32_t) * ninner);
+    nlo.reserve(sizeof(float4) * ninner);
+    nhi.reserve(sizeof(float4) * ninner);
+    cost.reserve(sizeof(float) * ninner);
+    collapse.reserve(sizeof(uint8_t) * ninner);
+    HIP_CHECK(hipMemsetAsync(visits.ptr, 0, sizeof(uint32_t) * ninner, s));
+    HIP_CHECK(hipMemsetAsync(node_parent.ptr, 0xff, sizeof(int32_t) * ninner, s));
+    HIP_CHECK(hipMemsetAsync(leaf_parent.ptr, 0xff, sizeof(int32_t) * n, s));
+    HIP_CHECK(hipMemsetAsync(collapse.ptr, 0, sizeof(uint8_t) * ninner, s));
+
+    bvh.nodes.reserve(sizeof(BvhNode) * ninner);
+    bvh.tris.reserve(sizeof(TriRecord) * n);
+
+    if (n == 1)
+        write_single_triangle_root(ctx, plo.as<float4>(), phi.as<float4>());
+    else
+    {
+        hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, B)), dim3(B), 0, s, keys_sorted.as<uint64_t>(), n,
+                           child.as<int32_t>(), node_parent.as<int32_t>(), leaf_parent.as<int32_t>(),
+                           range_lo.as<int32_t>(), range_hi.as<int32_t>());
+        hipLaunchKernelGGL(k_refit, dim3(blocks(n, B)), dim3(B), 0, s, n, plo.as<float4>(), phi.as<float4>(),
+                           vals_sorted.as<uint32_t>(), child.as<int32_t>(), node_parent.as<int32_t>(),
+                           leaf_parent.as<int32_t>(), range_lo.as<int32_t>(), range_hi.as<int32_t>(),
+                           visits.as<uint32_t>(), nlo.as<float4>(), nhi.as<float4>(), cost.as<float>(),
+                           collapse.as<uint8_t>(), pad.as<float>());
+        hipLaunchKernelGGL(k_emit_nodes, dim3(blocks(n - 1, B)), dim3(B), 0, s, n, plo.as<float4>(),
+                           phi.as<float4>(), vals_sorted.as<uint32_t>(), child.as<int32_t>(), range_lo.as<int32_t>(),
+                           range_hi.as<int32_t>(), nlo.as<float4>(), nhi.as<float4>(), collapse.as<uint8_t>(),
+                           pad.as<float>(), bvh.nodes.as<BvhNode>());
+    }
+    hipLaunchKernelGGL(bvhc::k_emit_tris, dim3(blocks(n, B)), dim3(B), 0, s, verts, idx, n,
+                       vals_sorted.as<uint32_t>(), bvh.tris.as<TriRecord>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ev1, s));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    bvh.build_ms = ms;
+    HIP_CHECK(hipEventDestroy(ev0));
+    HIP_CHECK(hipEventDestroy(ev1));
+    // Statistics, validation and the depth-first layout: not part of the
+    // build time.
+    finalize_bvh(ctx, ninner);
 }
 
 } // namespace mipt

@@ -196,6 +196,7 @@ void restir_spatial_pass(Ctx& ctx, int32_t pass_index, uint32_t sample_number, u
 
 // bvh_build.hip
 void build_lbvh(Ctx& ctx);
+void build_sah_binned(Ctx& ctx);
 
 // trace.hip
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n,

@@ -197,14 +197,18 @@ struct Surface {
 // without a primary hit keeps the empty stack (the early return at
 // RayVolumeState.h:37-39): a previous-frame pixel whose camera ray missed has
 // prim -1 and the material of an older hit, and reaches the temporal target
-// function. Dispersive glass, which would also draw a wavelength, is refused
-// with ReSTIR DI.
+// function. Its incident and outgoing media are the air: the reference leaves
+// them at -1 (RayVolumeState.h:65) and its BSDF evaluation then reads the IOR
+// of material -1 (Principled.h:1646), an out-of-bounds read. Dispersive
+// glass, which would also draw a wavelength, is refused with ReSTIR DI.
 __device__ __forceinline__ pt::VolumeState surface_volume(const PtData& d, const Surface& s)
 {
     pt::VolumeState v;
     pt::vol_init(v);
     if (s.prim >= 0)
         pt::vol_push(v, d.mat_idx[s.prim], d.mats[s.mat].dielectric_priority);
+    else
+        v.incident = v.outgoing = (int32_t)pt::VOL_AIR;
     return v;
 }
 

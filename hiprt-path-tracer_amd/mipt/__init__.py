@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PACKAGE_DIR, "libmipt.so")
 
 MIPT_OK = 0
 MIPT_BVH_LBVH = 0
+MIPT_BVH_SAH_BINNED = 1
 
 RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("last_prim", "<i4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
@@ -174,6 +175,7 @@ _SIGNATURES = {
     "mipt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     "mipt_build_bvh": (C.c_int, [C.c_void_p, C.c_int32]),
     "mipt_get_bvh_info": (C.c_int, [C.c_void_p, C.POINTER(BvhInfo)]),
+    "mipt_read_bvh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mipt_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mipt_trace_closest_counted": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -432,6 +434,17 @@ class Context:
         info = BvhInfo()
         self._check(self.lib.mipt_get_bvh_info(self.h, C.byref(info)), "mipt_get_bvh_info")
         return info
+
+    def read_bvh(self):
+        """(nodes, triangles): the BVH's 64-byte nodes as (n, 16) float32
+        rows (boxes; the child refs are the int32 view of columns 12-13) and
+        its 48-byte triangle records as (T, 12) float32 rows."""
+        info = self.bvh_info()
+        nodes = np.zeros((max(info.node_count, 0), 16), np.float32)
+        tris = np.zeros((max(info.triangle_count, 0), 12), np.float32)
+        self._check(self.lib.mipt_read_bvh(self.h, nodes.ctypes.data_as(C.c_void_p),
+                                           tris.ctypes.data_as(C.c_void_p)), "mipt_read_bvh")
+        return nodes, tris
 
     def set_trace_config(self, block_size: int = 0, lds_stack: int = 0):
         self._check(self.lib.mipt_set_trace_config(self.h, block_size, lds_stack), "mipt_set_trace_config")

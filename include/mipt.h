@@ -151,8 +151,13 @@ typedef struct mipt_scene_desc {
     int32_t emissive_triangle_count;
 } mipt_scene_desc;
 
-/* BVH builders (replace hiprtBuildGeometry, src/HIPRT-Orochi/HIPRTScene.h:87-150). */
+/* BVH builders (replace hiprtBuildGeometry, src/HIPRT-Orochi/HIPRTScene.h:87-150).
+ * LBVH: Morton-ordered Karras hierarchy with SAH-collapsed leaves (fast
+ * build, the hiprtBuildFlagBitPreferFastBuild role). SAH_BINNED: top-down
+ * binned SAH, 32 bins per axis (the hiprtBuildFlagBitPreferHighQualityBuild
+ * role the reference requests). Both give identical hits. */
 #define MIPT_BVH_LBVH 0
+#define MIPT_BVH_SAH_BINNED 1
 
 typedef struct mipt_bvh_info {
     int32_t builder;
@@ -188,6 +193,11 @@ int mipt_set_scene(mipt_ctx* ctx, const mipt_scene_desc* scene);
 /* Replaces HIPRTGeometry::build_bvh (HIPRTScene.h:87-150). Synchronous. */
 int mipt_build_bvh(mipt_ctx* ctx, int32_t builder);
 int mipt_get_bvh_info(const mipt_ctx* ctx, mipt_bvh_info* out);
+/* Copies the BVH to the host: node_count 64-byte nodes (root = node 0) and
+ * triangle_count 48-byte triangle records in leaf order (the layouts of
+ * csrc/layout.h); either pointer may be NULL. For inspection and tests: the
+ * reference's HIPRT keeps its BVH opaque, so this has no counterpart there. */
+int mipt_read_bvh(const mipt_ctx* ctx, void* host_nodes, void* host_triangles);
 
 /* Closest hit over the scene BVH: replaces hiprtGeomTraversalClosest
  * (used by trace_main_path_ray, src/Device/includes/Intersect.h:200-270, and

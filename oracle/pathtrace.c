@@ -2423,14 +2423,19 @@ typedef struct { int32_t mat, prim; v3 view, sn, gn, p; } osurf;
  * (RayVolumeState::reconstruct_first_hit, RayVolumeState.h:35-59). A surface
  * without a primary hit (prim -1: a previous-frame pixel whose camera ray
  * missed, with the material left from an older hit) keeps the empty stack,
- * as the reference's early return does (RayVolumeState.h:37-39). Dispersive
- * glass is refused with ReSTIR DI. */
+ * as the reference's early return does (RayVolumeState.h:37-39). Its incident
+ * and outgoing media are the air: the reference leaves them at -1
+ * (RayVolumeState.h:65) and its BSDF evaluation then reads the IOR of
+ * material -1 (Principled.h:1646), an out-of-bounds read. Dispersive glass is
+ * refused with ReSTIR DI. */
 static ovol surf_vol(const orc_frame* f, const osurf* s)
 {
     ovol v;
     vol_init(&v);
     if (s->prim >= 0)
         vol_push(&v, f->R.mat_idx[s->prim], f->R.mats[s->mat].dielectric_priority);
+    else
+        v.incident = v.outgoing = (int)VOL_AIR;
     return v;
 }
 
