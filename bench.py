@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
+    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="lbvh",
+                    help="BVH builder of the traversal leg: LBVH or top-down binned SAH")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -114,8 +116,9 @@ def main():
     scene = mipt.generate_scene(2, W, H)
     ctx = mipt.Context(0)
     ctx.set_scene(scene)
-    ctx.build_bvh()
-    bvh_name = "LBVH"
+    builder = mipt.MIPT_BVH_SAH_BINNED if args.bvh == "sah" else mipt.MIPT_BVH_LBVH
+    bvh_name = "SAH_BINNED" if args.bvh == "sah" else "LBVH"
+    ctx.build_bvh(builder)
     if args.block or args.stack:
         ctx.set_trace_config(args.block, args.stack)
     info = ctx.bvh_info()

@@ -247,6 +247,12 @@ typedef struct orc_frame orc_frame;
 orc_frame* orc_frame_create(const orc_render_scene* scene, const orc_octree* tree, const orc_camera* cam,
                             const orc_render_settings* s, const orc_restir_settings* rs);
 void orc_frame_free(orc_frame* f);
+/* Restricts the passes to the pixel window [x0, x1) x [y0, y1) (clipped to
+ * the image); pixels outside keep their state. A pass at a pixel reads the
+ * previous pass's output within the spatial reuse radius, so a window that
+ * shrinks by that radius from pass to pass reproduces the full-frame result
+ * inside the last window. */
+void orc_frame_set_window(orc_frame* f, int32_t x0, int32_t x1, int32_t y0, int32_t y1);
 void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
 void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
 void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);

@@ -100,6 +100,7 @@ _SIG = {
     "orc_fmath_eval": (None, [C.c_int32, _P, _P, _P, C.c_int32]),
     "orc_frame_create": (C.c_void_p, [_P, C.c_void_p, _P, _P, _P]),
     "orc_frame_free": (None, [C.c_void_p]),
+    "orc_frame_set_window": (None, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "orc_frame_camera_pass": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_initial": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_temporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
@@ -348,6 +349,10 @@ class Renderer:
             lib().orc_frame_free(self.h)
             self.h = None
 
+    def set_window(self, x0, x1, y0, y1):
+        """Later passes compute only the pixels of [x0, x1) x [y0, y1)."""
+        lib().orc_frame_set_window(self.h, int(x0), int(x1), int(y0), int(y1))
+
     def camera_pass(self, sample_number, rn, nthreads=0):
         lib().orc_frame_camera_pass(self.h, sample_number, rn, nthreads)
         lib().orc_frame_gbuffer_prims(self.h, _p(self.gb_prims))
@@ -386,6 +391,45 @@ class Renderer:
                 for i in range(r.spatial_number_of_passes):
                     self.restir_spatial(i, sample_number, next(rn), nthreads)
         self.path_trace(sample_number, next(rn), nthreads)
+
+    def render_frames_in_window(self, frames, tile, random_numbers_of_frame, halo=None, nthreads=0):
+        """Frames 0 .. frames-1 in the GPU renderer's pass order (as
+        render_frame), each pass computing only the pixels that the final
+        reservoirs and framebuffer inside `tile` = (x0, x1, y0, y1) depend on.
+        With a static camera a pixel's temporal pass reads the pixel itself
+        and a spatial pass reads neighbours at most spatial_reuse_radius
+        pixels away (rounded disk offsets), so the last spatial pass of the
+        last frame runs over the tile, each earlier spatial pass over a
+        window one halo (radius + 1) wider, and a frame's camera, initial
+        and temporal passes over the window of its first spatial pass plus
+        one halo; the path pass runs over the tile. `halo` overrides the
+        per-pass margin. Restores the whole-image window at the end."""
+        r = self.restir
+        restir_on = self.settings.direct_light_sampling_strategy == 5
+        P = r.spatial_number_of_passes if (restir_on and r.do_spatial_reuse_pass) else 0
+        h = r.spatial_reuse_radius + 1 if halo is None else halo
+        x0, x1, y0, y1 = tile
+
+        def win(e):
+            self.set_window(x0 - e, x1 + e, y0 - e, y1 + e)
+
+        for f in range(frames):
+            last = (frames - 1 - f) * P * h
+            rn = iter(random_numbers_of_frame(f))
+            win(last + P * h)
+            self.camera_pass(f, next(rn), nthreads)
+            if restir_on:
+                self.restir_initial(f, next(rn), nthreads)
+                if r.do_temporal_reuse_pass:
+                    self.restir_temporal(f, next(rn), nthreads)
+                    next(rn)
+                if r.do_spatial_reuse_pass:
+                    for i in range(P):
+                        win(last + (P - 1 - i) * h)
+                        self.restir_spatial(i, f, next(rn), nthreads)
+            win(0)
+            self.path_trace(f, next(rn), nthreads)
+        self.set_window(0, self.settings.width, 0, self.settings.height)
 
     def read_reservoirs(self, which=3):
         out = np.zeros(self.n, RESERVOIR_DTYPE)

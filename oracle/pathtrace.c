@@ -2239,7 +2239,25 @@ struct orc_frame {
     ores* res[3];
     int32_t last_output, shade_buffer, temporal_output, odd_frame;
     size_t n;
+    /* pixel window [x0, x1) x [y0, y1) that the passes compute (the whole
+     * image by default); pixels outside it keep their previous state */
+    int32_t win[4];
 };
+
+static int in_window(const orc_frame* f, size_t pix)
+{
+    const size_t w = (size_t)f->R.s.width;
+    const int32_t x = (int32_t)(pix % w), y = (int32_t)(pix / w);
+    return x >= f->win[0] && x < f->win[1] && y >= f->win[2] && y < f->win[3];
+}
+
+void orc_frame_set_window(orc_frame* f, int32_t x0, int32_t x1, int32_t y0, int32_t y1)
+{
+    f->win[0] = x0 < 0 ? 0 : x0;
+    f->win[1] = x1 > f->R.s.width ? f->R.s.width : x1;
+    f->win[2] = y0 < 0 ? 0 : y0;
+    f->win[3] = y1 > f->R.s.height ? f->R.s.height : y1;
+}
 
 static ores ores_empty(void)
 {
@@ -2351,6 +2369,7 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
     f->shade_buffer = 1;
     f->temporal_output = 0;
     f->odd_frame = 0;
+    orc_frame_set_window(f, 0, s->width, 0, s->height);
     return f;
 }
 
@@ -2410,6 +2429,8 @@ void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t rn, in
         for (int x = 0; x < w; x++)
         {
             size_t pix = (size_t)x + (size_t)y * (size_t)w;
+            if (!in_window(f, pix))
+                continue;
             if (kp)
                 f->prev[pix] = f->cur[pix];
             camera_rays_pixel(&f->R, x, y, rn, f->cur + pix);
@@ -2575,7 +2596,7 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
         {
             size_t pix = (size_t)pi;
             uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, R->s.freeze_random);
-            if (f->cur[pix].prim == -1)
+            if (f->cur[pix].prim == -1 || !in_window(f, pix))
                 continue;
             osurf s = surf_of(f, f->cur, pix);
             const omat* m = &R->mats[s.mat];
@@ -2691,7 +2712,7 @@ void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t rn
     for (int64_t pi = 0; pi < (int64_t)f->n; pi++)
     {
         size_t pix = (size_t)pi;
-        if (f->cur[pix].prim == -1)
+        if (f->cur[pix].prim == -1 || !in_window(f, pix))
             continue;
         uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, f->R.s.freeze_random);
         ores tn = in[pix];
@@ -2814,7 +2835,7 @@ void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_
     for (int64_t pi = 0; pi < (int64_t)f->n; pi++)
     {
         size_t pix = (size_t)pi;
-        if (f->cur[pix].prim == -1)
+        if (f->cur[pix].prim == -1 || !in_window(f, pix))
             continue;
         int cx = (int)(pix % (size_t)w), cy = (int)(pix / (size_t)w);
         uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, f->R.s.freeze_random);
@@ -2916,6 +2937,8 @@ int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t rn, 
         for (int x = 0; x < w; x++)
         {
             size_t pix = (size_t)x + (size_t)y * (size_t)w;
+            if (!in_window(f, pix))
+                continue;
             const gbuf* g = f->cur + pix;
             rgb c = C1(0.0f);
             if (g->prim != -1)

@@ -187,3 +187,50 @@ def test_temporal_reuse_of_a_surface_that_missed(mipt_mod, oracle_mod, open_back
         R.restir_spatial(0, f, rn[4])
         R.path_trace(f, rn[5])
     assert reached >= 4
+
+
+def _frame_numbers(mipt_mod, draws):
+    return lambda f: [mipt_mod.host_random_number(f * draws + k) for k in range(draws)]
+
+
+def _draws(s, rs):
+    return 2 + (1 + 2 * rs.do_temporal_reuse_pass + rs.spatial_number_of_passes * rs.do_spatial_reuse_pass
+                if s.direct_light_sampling_strategy == 5 else 0)
+
+
+def test_windowed_frames_equal_full_frames_inside_the_tile(mipt_mod, oracle_mod):
+    """The windowed frame renderer (each pass over the pixels the tile's
+    final values depend on) gives, inside the tile, the full-frame
+    renderer's reservoirs and framebuffer bit for bit, over 3 frames of
+    temporal + 2 spatial passes. A halo one pixel narrower than the reuse
+    radius changes some of them: the window does leave pixels out, and the
+    margin is the radius."""
+    Wt = Ht = 128
+    scene = mipt_mod.generate_scene(1, Wt, Ht)
+    s = mipt_mod.default_render_settings(width=Wt, height=Ht, nb_bounces=2, direct_light_sampling_strategy=5)
+    rs = mipt_mod.default_restir_di_settings(spatial_reuse_radius=6, spatial_number_of_passes=2, m_cap=20)
+    draws = _draws(s, rs)
+    frames = 3
+    tile = (52, 76, 40, 64)
+    full = oracle_mod.Renderer(scene, s, rs)
+    for f in range(frames):
+        full.render_frame(f, _frame_numbers(mipt_mod, draws)(f))
+    sel = np.zeros((Ht, Wt), bool)
+    sel[tile[2]:tile[3], tile[0]:tile[1]] = True
+    sel = sel.reshape(-1)
+    assert (full.gb_prims[sel] >= 0).all()
+    diffs = {}
+    for halo in (None, rs.spatial_reuse_radius - 1):
+        win = oracle_mod.Renderer(scene, s, rs)
+        win.render_frames_in_window(frames, tile, _frame_numbers(mipt_mod, draws), halo=halo)
+        # the first window is the tile grown by frames x passes halos and
+        # stays inside the image: the pixels outside it were never computed
+        assert (win.read_reservoirs(0)["M"][~sel] == 0).any()
+        n = 0
+        for which in (0, 1, 2, 3):
+            a, b = win.read_reservoirs(which)[sel], full.read_reservoirs(which)[sel]
+            n += int((a.view(np.uint32).reshape(len(a), -1) != b.view(np.uint32).reshape(len(b), -1)).any(axis=1).sum())
+        n += int((win.fb.reshape(-1, 3)[sel] != full.fb.reshape(-1, 3)[sel]).any(axis=1).sum())
+        diffs[halo] = n
+    assert diffs[None] == 0
+    assert diffs[rs.spatial_reuse_radius - 1] > 0
