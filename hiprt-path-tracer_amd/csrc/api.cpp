@@ -309,6 +309,7 @@ int mipt_create(int hip_device, mipt_ctx** out)
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hip_device)) != hipSuccess)
         return create_fail(ctx, "hipDeviceGetAttribute(MultiprocessorCount)", e);
     ctx->cu_count = cus;
+    mipt::noalloc_scratch_init(hip_device);
     mipt_render_settings_default(&ctx->render.settings);
     mipt_restir_di_settings_default(&ctx->render.restir);
     *out = reinterpret_cast<mipt_ctx*>(ctx);
@@ -480,6 +481,16 @@ int mipt_trace_closest(mipt_ctx* c, const mipt_ray* d_rays, mipt_hit* d_hits, ui
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         mipt::trace_closest(*ctx, d_rays, d_hits, nullptr, n, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int mipt_trace_closest_noalloc(mipt_ctx* c, const mipt_ray* d_rays, mipt_hit* d_hits, uint32_t n, void* stream)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (n > 0 && (!d_rays || !d_hits)))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::trace_closest_noalloc(*ctx, d_rays, d_hits, n, reinterpret_cast<hipStream_t>(stream));
     });
 }
 

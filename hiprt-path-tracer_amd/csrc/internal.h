@@ -202,6 +202,11 @@ void build_sah_binned(Ctx& ctx);
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n,
                    hipStream_t stream);
 void trace_any(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, uint32_t n, hipStream_t stream);
+// mipt_trace_closest_noalloc: the closest-hit kernel with its work counter
+// and spill stack in the library's static device scratch, whose per-device
+// address noalloc_scratch_init looks up (once per device, device current).
+void noalloc_scratch_init(int device);
+void trace_closest_noalloc(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t n, hipStream_t stream);
 // Trace the rays whose slots a segmented queue lists (seglist.h; length read
 // on the device, at most capacity rays); results go to the same slots.
 void trace_closest_queue(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, const SegList& queue, uint32_t capacity,

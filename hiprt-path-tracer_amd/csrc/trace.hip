@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 #include <utility>
 
 #include "internal.h"
@@ -531,7 +532,89 @@ void launch(Ctx& ctx, int mode, const mipt_ray* rays, mipt_hit* hits, uint8_t* o
     HIP_CHECK(hipGetLastError());
 }
 
+// ---- closest hit without allocation ---------------------------------
+// Work counter and spill columns of mipt_trace_closest_noalloc, in a
+// file-scope device array: each device holds its own copy from module load
+// until the process ends. Host code reaches it only through the per-device
+// address from hipGetSymbolAddress (the name itself is a host stand-in).
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+static_assert(MIPT_TRACE_NOALLOC_SCRATCH_BYTES > NOALLOC_COUNTER_BYTES, "scratch too small");
+static __device__ __align__(256) uint8_t scratch[MIPT_TRACE_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex noalloc_lock;
+void* noalloc_ptr[MAX_DEVICES] = {};
+hipError_t noalloc_err[MAX_DEVICES] = {};
+bool noalloc_done[MAX_DEVICES] = {};
+
 } // namespace
+
+void noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return;
+    std::lock_guard<std::mutex> guard(noalloc_lock);
+    if (noalloc_done[device])
+        return;
+    // the device is current (mipt_create set it)
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    noalloc_err[device] = e;
+    noalloc_ptr[device] = e == hipSuccess ? p : nullptr;
+    noalloc_done[device] = true;
+}
+
+void trace_closest_noalloc(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t n, hipStream_t stream)
+{
+    if (n == 0)
+        return;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> guard(noalloc_lock);
+        if (ctx.device < 0 || ctx.device >= MAX_DEVICES || !noalloc_done[ctx.device])
+            fail(MIPT_ERR_HIP, "mipt_trace_closest_noalloc: no scratch address for this device");
+        if (noalloc_err[ctx.device] != hipSuccess)
+            fail(MIPT_ERR_HIP, std::string("mipt_trace_closest_noalloc: hipGetSymbolAddress(scratch): ") +
+                                   hipGetErrorString(noalloc_err[ctx.device]));
+        base = noalloc_ptr[ctx.device];
+    }
+    if (!trace_ready(ctx))
+    {
+        hipLaunchKernelGGL(k_all_miss, dim3((n + 255) / 256), dim3(256), 0, stream, reinterpret_cast<int4*>(hits),
+                           nullptr, n);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const Launch l = plan(ctx, CLOSEST, n);
+    const int block = ctx.trace_cfg.block_size;
+    // Spill columns of the persistent grid: when the tree is deep enough
+    // that the whole grid's columns do not fit, the grid shrinks to the
+    // blocks whose columns do (every wave takes rays from the work counter,
+    // so fewer blocks trace the same rays).
+    const int spill_depth = std::max(0, ctx.bvh.max_depth - ctx.trace_cfg.lds_stack);
+    const size_t room = MIPT_TRACE_NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES;
+    unsigned grid = l.grid;
+    if (spill_depth > 0)
+    {
+        const size_t per_block = sizeof(int32_t) * (size_t)spill_depth * (size_t)block;
+        const size_t fit = room / per_block;
+        if (fit == 0)
+            fail(MIPT_ERR_INVALID_ARGUMENT,
+                 "mipt_trace_closest_noalloc: a BVH of depth " + std::to_string(ctx.bvh.max_depth) +
+                     " needs more than the " + std::to_string(MIPT_TRACE_NOALLOC_SCRATCH_BYTES) +
+                     "-byte scratch for one block's spill stack");
+        grid = (unsigned)std::min<size_t>(grid, fit);
+    }
+    const uint32_t threads = grid * (uint32_t)block;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(base);
+    int32_t* spill = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(base) + NOALLOC_COUNTER_BYTES);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, counter);
+    hipLaunchKernelGGL(l.fn, dim3(grid), dim3(block), 0, stream, ctx.bvh.nodes.as<BvhNode>(),
+                       ctx.bvh.tris.as<TriRecord>(), ctx.bvh.root, reinterpret_cast<const float4*>(rays),
+                       reinterpret_cast<int4*>(hits), nullptr, nullptr, n, spill, threads, counter,
+                       (uint32_t)ctx.bvh.node_count + 1u, nullptr, nullptr, 0u);
+    HIP_CHECK(hipGetLastError());
+}
 
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n, hipStream_t stream)
 {

@@ -178,6 +178,7 @@ _SIGNATURES = {
     "mipt_read_bvh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mipt_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mipt_trace_closest_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mipt_trace_closest_counted": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                              C.c_void_p]),
     "mipt_set_trace_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -452,6 +453,12 @@ class Context:
     def trace_closest(self, d_rays, d_hits, n: int, stream=None):
         self._check(self.lib.mipt_trace_closest(self.h, ptr(d_rays), ptr(d_hits), n, stream),
                     "mipt_trace_closest")
+
+    def trace_closest_noalloc(self, d_rays, d_hits, n: int, stream=None):
+        """mipt_trace_closest through the library's static scratch: no device
+        allocation (calls overlapping on one GPU race)."""
+        self._check(self.lib.mipt_trace_closest_noalloc(self.h, ptr(d_rays), ptr(d_hits), n, stream),
+                    "mipt_trace_closest_noalloc")
 
     def trace_closest_counted(self, d_rays, d_hits, d_counts, n: int, stream=None):
         self._check(self.lib.mipt_trace_closest_counted(self.h, ptr(d_rays), ptr(d_hits), ptr(d_counts), n, stream),

@@ -211,6 +211,23 @@ int mipt_trace_closest(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* d_hits, 
 int mipt_trace_any(mipt_ctx* ctx, const mipt_ray* d_rays, uint8_t* d_occluded, uint32_t n,
                    void* stream);
 
+/* Closest hit that allocates nothing: the result of mipt_trace_closest,
+ * with the traversal's work counter and spill stack in a static device
+ * scratch array of MIPT_TRACE_NOALLOC_SCRATCH_BYTES per GPU (held from the
+ * library's load until the process ends) instead of memory the context
+ * allocates. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). It calls no hipMalloc/hipFree.
+ * When the tree's spill columns for the whole persistent grid exceed the
+ * scratch, fewer blocks trace the rays; a BVH too deep for one block's
+ * spill stack is refused (MIPT_ERR_INVALID_ARGUMENT). Every call on one GPU
+ * shares that GPU's scratch: calls that overlap in time on one GPU (two
+ * streams, two host threads) race and must be serialised by the caller.
+ * The BVH must be built (mipt_build_bvh). */
+#define MIPT_TRACE_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int mipt_trace_closest_noalloc(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* d_hits, uint32_t n,
+                               void* stream);
+
 /* Instrumented closest hit: same result as mipt_trace_closest and, per ray,
  * d_counts[2*i] = 64-byte nodes fetched, d_counts[2*i+1] = triangles tested.
  * Used for the bytes-per-ray accounting; slower than the plain kernel. */
