@@ -5,7 +5,8 @@ configuration 2 -- 1,040,384 triangles (512x512-quad fBm terrain, 400
 icospheres, 4096 emissive triangles) -- at 1920x1080. The ray set is R_inc:
 one cosine-weighted bounce ray from each primary hit (2,073,600 rays),
 generated on the GPU by the camera and bounce-ray kernels. One step = one
-mipt_trace_closest launch over all 2,073,600 rays, inputs resident in HBM.
+mipt_trace_closest launch over all 2,073,600 rays, inputs resident in HBM,
+over the binned-SAH tree (--bvh lbvh: the LBVH).
 
 Prints ONE JSON line: Mrays/s over the timed steps, the roofline of the
 traversal kernel (algorithmic bytes per launch / mean launch time, HIP events
@@ -22,7 +23,7 @@ bounces), "cache_vs_plain_tuned" with the library's defaults (bounce 1,
 1/64 of the paths). "restir_render" times ReSTIR DI frames of the
 1M-triangle scene at 1920x1080.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-render] [--no-cache]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--bvh sah|lbvh] [--no-cpu] [--no-render] [--no-cache]
        [--cache-ref-spp 4096] [--cache-plain-spp 32] [--cache-warmup 2] [--no-restir] [--restir-layered]
 """
 import argparse
@@ -76,7 +77,7 @@ def main():
     ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
-    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="lbvh",
+    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="sah",
                     help="BVH builder of the traversal leg: LBVH or top-down binned SAH")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
@@ -186,13 +187,17 @@ def main():
     # launch configuration; null when the configuration differs.
     traffic = None
     l2_hit_rate = None
-    prof = os.path.join(ROOT, "profiles", "trace_closest_config2_rinc.json")
-    if os.path.exists(prof) and not (args.block or args.stack):
+    prof_source = None
+    for name in ("trace_closest_config2_rinc.json", "trace_closest_config2_rinc_sah.json"):
+        prof = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(prof) or args.block or args.stack:
+            continue
         with open(prof) as f:
             pj = json.load(f)
         if pj.get("bvh", "LBVH") == bvh_name:
             traffic = pj.get("traffic_bytes_per_launch")
             l2_hit_rate = pj.get("l2_hit_rate")
+            prof_source = pj.get("summary")
     # What bounds the kernel, from the counters: the algorithmic bytes are an
     # effective bandwidth (they grow with node visits, most of which L2 and
     # the Infinity Cache serve); the measured HBM bytes say whether HBM is
@@ -257,7 +262,7 @@ def main():
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "achieved_kind": "algorithmic bytes (B_ray x rays) / mean launch time: an effective bandwidth",
                      "measured": measured, "limiter": limiter,
-                     "traffic_source": "profiles/trace_closest_config2_rinc.md" if traffic else None,
+                     "traffic_source": prof_source if traffic else None,
                      "bytes_per_ray": round(b_ray, 1), "mean_nodes_per_ray": round(mean_nodes, 2),
                      "mean_tris_per_ray": round(mean_tris, 2), "mean_launch_ms": round(mean_launch_s * 1e3, 4)},
         "cpu_baseline": cpu,

@@ -428,7 +428,10 @@ class Context:
         d = scene.desc()
         self._check(self.lib.mipt_set_scene(self.h, C.byref(d)), "mipt_set_scene")
 
-    def build_bvh(self, builder: int = MIPT_BVH_LBVH):
+    def build_bvh(self, builder: int = MIPT_BVH_SAH_BINNED):
+        """Builds the scene BVH; by default with the top-down binned-SAH
+        builder (the reference builds with HIPRT's high-quality flag,
+        HIPRTScene.h:87-150), MIPT_BVH_LBVH for the faster LBVH build."""
         self._check(self.lib.mipt_build_bvh(self.h, builder), "mipt_build_bvh")
 
     def bvh_info(self) -> BvhInfo:
