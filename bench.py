@@ -74,6 +74,8 @@ def main():
                     help="only render 3 + --steps samples of the config-1 scene at 1920x1080, plain, with the "
                          "radiance cache or as cache warm-up frames, and print ms per sample (a kernel-profiling "
                          "workload)")
+    ap.add_argument("--shade-order", choices=["live", "material", "auto"], default=None,
+                    help="with --frames: the path pass's shading order (default: the library's, auto)")
     ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
@@ -88,7 +90,7 @@ def main():
                                     ("train_every", args.cache_train_every),
                                     ("warmup_train_every", args.cache_warmup_train_every)) if v is not None}
     if args.frames:
-        return frames_only(args.frames, args.steps, cache_over)
+        return frames_only(args.frames, args.steps, cache_over, args.shade_order)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -279,7 +281,7 @@ def main():
     ctx.close()
 
 
-def frames_only(mode, samples, cache_over=None):
+def frames_only(mode, samples, cache_over=None, shade_order=None):
     """Config-1 scene at 1920x1080, 4 bounces, uniform NEE + MIS: 3 warm-up and
     `samples` timed calls of mipt_render_sample, plain or with the radiance
     cache (default settings, cache_over replacing some), or of
@@ -293,6 +295,9 @@ def frames_only(mode, samples, cache_over=None):
     ctx = mipt.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
+    if shade_order:
+        ctx.set_shading_order({"live": mipt.MIPT_SHADE_ORDER_LIVE, "material": mipt.MIPT_SHADE_ORDER_MATERIAL,
+                               "auto": mipt.MIPT_SHADE_ORDER_AUTO}[shade_order])
     cache = mode != "plain"
     frame = ctx.cache_warmup if mode == "warmup" else ctx.render_sample
     if cache:
@@ -307,7 +312,8 @@ def frames_only(mode, samples, cache_over=None):
     for i in range(samples):
         frame(3 + i)
     torch.cuda.synchronize()
-    out = {"frames": mode, "samples": samples, "ms_per_sample": (time.perf_counter() - t0) / samples * 1e3}
+    out = {"frames": mode, "samples": samples, "shade_order": shade_order or "auto",
+           "ms_per_sample": (time.perf_counter() - t0) / samples * 1e3}
     if cache:
         q, t, dropped, loss = ctx.cache_stats()
         out.update({"queries": q, "training_records": t, "dropped_targets": dropped, "train_loss": loss})

@@ -3,6 +3,7 @@
 // failure into a negative status plus a message for mipt_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -111,6 +112,38 @@ void prepare_scene_for_render(Ctx& ctx)
         R.scene_volumes |= o[22] > 0.0f || o[23] > 0.0f;
         R.scene_dispersive |= o[50] > 0.0f && o[22] > 0.0f;
     }
+    // Lobe class of each material: the set of Principled lobes with a
+    // non-zero weight outside the object (Principled.h:1515-1554) and thin
+    // film; materials of one class take the same branches in the BSDF.
+    // Classes are numbered in order of first appearance; the 32nd and later
+    // share class 31.
+    {
+        std::vector<int32_t> cls(ctx.materials_host.size() > 0 ? ctx.materials_host.size() : 1, 0);
+        std::vector<uint32_t> seen;
+        for (size_t i = 0; i < ctx.materials_host.size(); i++)
+        {
+            const float* o = mats.data() + 64 * i; // pt::Material floats
+            const float metallic = o[7], srw = o[17], spec = o[19], st = o[22], dt = o[23];
+            uint32_t mask = 0u;
+            mask |= (o[20] > 0.0f) ? 1u : 0u;                                      // coat
+            mask |= (o[21] > 0.0f) ? 2u : 0u;                                      // sheen
+            mask |= (metallic > 0.0f && srw < 1.0f) ? 4u : 0u;                      // first metal
+            mask |= (metallic > 0.0f && srw > 0.0f) ? 8u : 0u;                      // second metal
+            mask |= ((1.0f - metallic) * (1.0f - st * (1.0f - dt)) * spec > 0.0f) ? 16u : 0u; // specular
+            mask |= ((1.0f - metallic) * (1.0f - st) * (1.0f - dt) > 0.0f) ? 32u : 0u;    // diffuse
+            mask |= ((1.0f - metallic) * (1.0f - dt) * st > 0.0f) ? 64u : 0u;             // glass
+            mask |= ((1.0f - metallic) * dt > 0.0f) ? 128u : 0u;                            // diffuse transmission
+            mask |= (o[24] > 0.0f) ? 256u : 0u;                                     // thin film
+            size_t k = 0;
+            while (k < seen.size() && seen[k] != mask)
+                k++;
+            if (k == seen.size() && seen.size() < 31)
+                seen.push_back(mask);
+            cls[i] = (int32_t)std::min<size_t>(k, 31);
+        }
+        R.lobe_classes = (int32_t)std::max<size_t>(seen.size(), 1);
+        upload(R.mat_class, cls.data(), cls.size());
+    }
     std::vector<float> areas(ctx.host_vertices.size() ? ctx.triangle_count : 0);
     for (int32_t t = 0; t < (int32_t)areas.size(); t++)
     {
@@ -164,6 +197,8 @@ void allocate_render_buffers(Ctx& ctx)
     R.q_skip.reserve(4 * 2 * qslots);
     R.q_live.reserve(4 * 2 * LIVE_APPENDERS * qslots);
     R.q_counts.reserve(4 * 7 * (size_t)SEG_COUNTER_WORDS);
+    R.shade_sorted.reserve(4 * n);
+    R.sort_counts.reserve(4 * 2 * (size_t)SHADE_SORT_COUNTERS * SEG_STRIDE);
     R.fb.reserve(12 * n);
     HIP_CHECK(hipMemset(R.fb.ptr, 0, 12 * n));
     if (R.settings.accumulate_fp64)
@@ -471,6 +506,15 @@ int mipt_set_trace_config(mipt_ctx* c, int32_t block_size, int32_t lds_stack)
         ctx->trace_cfg.block_size = block_size;
     if (lds_stack)
         ctx->trace_cfg.lds_stack = lds_stack;
+    return MIPT_OK;
+}
+
+int mipt_set_shading_order(mipt_ctx* c, int32_t order)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || order < MIPT_SHADE_ORDER_LIVE || order > MIPT_SHADE_ORDER_AUTO)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    ctx->render.shade_order = order;
     return MIPT_OK;
 }
 

@@ -32,6 +32,13 @@ void hip_check(hipError_t e, const char* what);
 constexpr int32_t MAX_SKIP_ROUNDS = 2;
 constexpr uint32_t LIVE_APPENDERS = 1u + (uint32_t)MAX_SKIP_ROUNDS;
 
+// Material-sorted shading order: 64 sort classes (32 lobe classes x the
+// path's volume state stored or not), each counted on 16 sub-counters
+// (a block b adds to sub-counter b % 16) to spread the atomics.
+constexpr uint32_t SHADE_SORT_CLASSES = 64;
+constexpr uint32_t SHADE_SORT_SUBS = 16;
+constexpr uint32_t SHADE_SORT_COUNTERS = SHADE_SORT_CLASSES * SHADE_SORT_SUBS;
+
 // Owning device allocation (hipMalloc), sized in bytes.
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -75,6 +82,10 @@ struct RenderState {
     uint32_t pixels = 0;
     // scene-derived
     DeviceBuffer materials;     // pt::Material[material_count]
+    DeviceBuffer mat_class;     // int32[material_count]: lobe class (0..30) of each material
+    int32_t lobe_classes = 0;   // distinct lobe classes in the scene
+    int32_t shade_order = MIPT_SHADE_ORDER_AUTO;
+    DeviceBuffer shade_sorted, sort_counts; // live paths in shading order; class counters and cursors
     DeviceBuffer areas;         // float[T]
     DeviceBuffer lut_conductor; // float[128*128]
     DeviceBuffer lut_glossy, lut_glass, lut_glass_inv, lut_thin_glass, lut_glass_exponent, lut_sheen;

@@ -21,6 +21,7 @@ struct PtData {
     const int32_t* idx;
     const int32_t* mat_idx;
     const pt::Material* mats;
+    const int32_t* mat_class; // lobe class of each material (the shading order's sort key)
     const int32_t* emissive;
     const float* areas;
     int32_t emissive_count;

@@ -18,6 +18,9 @@ LIB_PATH = os.path.join(PACKAGE_DIR, "libmipt.so")
 
 MIPT_OK = 0
 MIPT_BVH_LBVH = 0
+MIPT_SHADE_ORDER_LIVE = 0
+MIPT_SHADE_ORDER_MATERIAL = 1
+MIPT_SHADE_ORDER_AUTO = 2
 MIPT_BVH_SAH_BINNED = 1
 
 RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("last_prim", "<i4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
@@ -182,6 +185,7 @@ _SIGNATURES = {
     "mipt_trace_closest_counted": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                              C.c_void_p]),
     "mipt_set_trace_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "mipt_set_shading_order": (C.c_int, [C.c_void_p, C.c_int32]),
     "mipt_camera_rays": (C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                    C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_bounce_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -433,6 +437,10 @@ class Context:
         builder (the reference builds with HIPRT's high-quality flag,
         HIPRTScene.h:87-150), MIPT_BVH_LBVH for the faster LBVH build."""
         self._check(self.lib.mipt_build_bvh(self.h, builder), "mipt_build_bvh")
+
+    def set_shading_order(self, order: int):
+        """MIPT_SHADE_ORDER_LIVE / _MATERIAL / _AUTO (the default)."""
+        self._check(self.lib.mipt_set_shading_order(self.h, order), "mipt_set_shading_order")
 
     def bvh_info(self) -> BvhInfo:
         info = BvhInfo()

@@ -234,6 +234,20 @@ int mipt_trace_closest_noalloc(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* 
 int mipt_trace_closest_counted(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* d_hits,
                                uint32_t* d_counts, uint32_t n, void* stream);
 
+/* Order in which the path pass shades the live paths of a bounce. The
+ * reference's megakernel shades one pixel per thread (Megakernel.h:23-144);
+ * the wavefront passes shade a list of live paths, which is either kept in
+ * the order the paths reached their vertex (LIVE) or sorted by the lobe
+ * class of the material at the vertex and by whether the path carries a
+ * volume state (MATERIAL: the paths of a wave then take the same BSDF
+ * branches). AUTO (the default) sorts when the scene has transmissive
+ * materials (glass, diffuse transmission), where paths inside media and at
+ * opaque surfaces would share waves. Results do not depend on the order. */
+#define MIPT_SHADE_ORDER_LIVE 0
+#define MIPT_SHADE_ORDER_MATERIAL 1
+#define MIPT_SHADE_ORDER_AUTO 2
+int mipt_set_shading_order(mipt_ctx* ctx, int32_t order);
+
 /* Traversal launch configuration (the autotuned knobs). block_size in
  * {64,128,256}; lds_stack in {8,16,24,32}. 0 keeps the current value. */
 int mipt_set_trace_config(mipt_ctx* ctx, int32_t block_size, int32_t lds_stack);

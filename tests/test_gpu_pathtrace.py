@@ -200,3 +200,45 @@ def test_framebuffer_is_a_sum(mipt_mod, torch_mod):
     assert np.array_equal(a, c)
     assert (b >= a - 1e-6).all() and b.sum() > a.sum()
     ctx.close()
+
+
+@pytest.mark.parametrize("scene_kind", ["config1", "nested_glass", "cache"])
+def test_shading_order_does_not_change_the_image(mipt_mod, torch_mod, scene_kind):
+    """The path pass shades the live paths in list order or sorted by
+    material lobe class (and stored volume state); each path's computation
+    depends only on its own state, so the framebuffer (FP32 and FP64 sums)
+    is bit-identical in every order: the config-1 scene (Lambertian and
+    metal classes), the nested-glass scene (glass, diffuse transmission,
+    coat, paths inside media) and the config-1 scene with the radiance
+    cache (the sorted list is then the training paths' list past the query
+    bounce)."""
+    from test_gpu_glass import nested_glass_scene
+
+    w = h = 128
+    scene = nested_glass_scene(mipt_mod, w, h) if scene_kind == "nested_glass" else \
+        mipt_mod.generate_scene(1, w, h)
+    cache = scene_kind == "cache"
+    out = {}
+    for order in (mipt_mod.MIPT_SHADE_ORDER_LIVE, mipt_mod.MIPT_SHADE_ORDER_MATERIAL,
+                  mipt_mod.MIPT_SHADE_ORDER_AUTO):
+        ctx = mipt_mod.Context(0)
+        ctx.set_scene(scene)
+        ctx.build_bvh()
+        ctx.set_shading_order(order)
+        if cache:
+            ctx.set_radiance_cache(mipt_mod.default_radiance_cache_settings())
+        ctx.set_render_settings(mipt_mod.default_render_settings(width=w, height=h, nb_bounces=6,
+                                                                 accumulate_fp64=1, radiance_cache=int(cache)))
+        ctx.set_camera(scene.camera)
+        for s in range(3):
+            ctx.render_sample(s)
+        fb, fb64 = ctx.read_framebuffer(fp64=True)
+        out[order] = (fb.tobytes(), fb64.tobytes())
+        ctx.close()
+    assert out[0] == out[1] == out[2]
+    with pytest.raises(mipt_mod.MiptError):
+        ctx = mipt_mod.Context(0)
+        try:
+            ctx.set_shading_order(3)
+        finally:
+            ctx.close()
