@@ -82,20 +82,13 @@ template <class T> void upload(DeviceBuffer& buf, const T* host, size_t count)
         HIP_CHECK(hipMemcpy(buf.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice));
 }
 
-// Scene data the passes need beyond the traversal: unpacked materials,
-// triangle areas (ThreadFunctions.cpp:221-242: |cross(AB, AC)| * 0.5) and the
-// energy-compensation LUT.
-void prepare_scene_for_render(Ctx& ctx)
+// Unpacks the CPU materials (CPUMaterial::pack_to_GPU, then the device
+// record), uploads them and derives what the passes read from them: the
+// lobe class of each material and whether some material transmits light or
+// is dispersive.
+void upload_materials(Ctx& ctx)
 {
     RenderState& R = ctx.render;
-    if (R.scene_ready)
-        return;
-    if (!ctx.scene_set)
-        fail(MIPT_ERR_NO_SCENE, "no scene: call mipt_set_scene first");
-    if (!ctx.bvh.valid)
-        fail(MIPT_ERR_NO_BVH, "no BVH: call mipt_build_bvh first");
-    if (ctx.triangle_count > 0 && (ctx.materials_host.empty() || !ctx.material_indices.ptr))
-        fail(MIPT_ERR_INVALID_ARGUMENT, "the path-tracing passes need materials and material indices");
     std::vector<float> mats(64 * (ctx.materials_host.size() > 0 ? ctx.materials_host.size() : 1), 0.0f);
     for (size_t i = 0; i < ctx.materials_host.size(); i++)
     {
@@ -144,6 +137,23 @@ void prepare_scene_for_render(Ctx& ctx)
         R.lobe_classes = (int32_t)std::max<size_t>(seen.size(), 1);
         upload(R.mat_class, cls.data(), cls.size());
     }
+}
+
+// Scene data the passes need beyond the traversal: unpacked materials,
+// triangle areas (ThreadFunctions.cpp:221-242: |cross(AB, AC)| * 0.5) and the
+// energy-compensation LUT.
+void prepare_scene_for_render(Ctx& ctx)
+{
+    RenderState& R = ctx.render;
+    if (R.scene_ready)
+        return;
+    if (!ctx.scene_set)
+        fail(MIPT_ERR_NO_SCENE, "no scene: call mipt_set_scene first");
+    if (!ctx.bvh.valid)
+        fail(MIPT_ERR_NO_BVH, "no BVH: call mipt_build_bvh first");
+    if (ctx.triangle_count > 0 && (ctx.materials_host.empty() || !ctx.material_indices.ptr))
+        fail(MIPT_ERR_INVALID_ARGUMENT, "the path-tracing passes need materials and material indices");
+    upload_materials(ctx);
     std::vector<float> areas(ctx.host_vertices.size() ? ctx.triangle_count : 0);
     for (int32_t t = 0; t < (int32_t)areas.size(); t++)
     {
@@ -509,6 +519,53 @@ int mipt_set_trace_config(mipt_ctx* c, int32_t block_size, int32_t lds_stack)
     return MIPT_OK;
 }
 
+int mipt_update_materials(mipt_ctx* c, const mipt_material* materials, int32_t count)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !materials)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (!ctx->scene_set)
+            mipt::fail(MIPT_ERR_NO_SCENE, "mipt_update_materials: no scene");
+        if (count != (int32_t)ctx->materials_host.size())
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_update_materials: " + std::to_string(count) +
+                                                      " materials for a scene of " +
+                                                      std::to_string(ctx->materials_host.size()));
+        const std::vector<mipt_material> old = ctx->materials_host;
+        ctx->materials_host.assign(materials, materials + count);
+        if (ctx->render.scene_ready)
+        {
+            // kernels still queued on any stream may read the material records
+            HIP_CHECK(hipDeviceSynchronize());
+            try
+            {
+                mipt::upload_materials(*ctx);
+            }
+            catch (...)
+            {
+                ctx->materials_host = old; // an unsupported material leaves the scene as it was
+                mipt::upload_materials(*ctx);
+                throw;
+            }
+        }
+    });
+}
+
+int mipt_update_material(mipt_ctx* c, int32_t index, const mipt_material* material)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !material)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (index < 0 || index >= (int32_t)ctx->materials_host.size())
+    {
+        ctx->last_error = "mipt_update_material: index " + std::to_string(index) + " out of range";
+        return MIPT_ERR_INVALID_ARGUMENT;
+    }
+    std::vector<mipt_material> all = ctx->materials_host;
+    all[(size_t)index] = *material;
+    return mipt_update_materials(c, all.data(), (int32_t)all.size());
+}
+
 int mipt_set_shading_order(mipt_ctx* c, int32_t order)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
@@ -604,6 +661,7 @@ void mipt_render_settings_default(mipt_render_settings* s)
     s->accumulate_fp64 = 1;
     s->direct_light_sampling_strategy = MIPT_LSS_MIS_LIGHT_BSDF;
     s->direct_light_sampling_base_strategy = MIPT_LSS_BASE_UNIFORM;
+    s->do_normal_mapping = 1;
 }
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s)

@@ -114,7 +114,7 @@ __global__ void k_gbuffer_write(PtData d, const float4* __restrict__ rays, const
     if (h.x >= 0)
     {
         f3 p, gn, sn;
-        hit_attributes(d, o, dir, h.x, __int_as_float(h.y), p, gn, sn);
+        hit_attributes(d, o, dir, h.x, __int_as_float(h.y), __int_as_float(h.z), __int_as_float(h.w), p, gn, sn);
         d.gb_gn[pix] = pt::oct_pack(gn);
         d.gb_sn[pix] = pt::oct_pack(sn);
         d.gb_mat[pix] = d.mat_idx[h.x];
@@ -585,7 +585,7 @@ __device__ __forceinline__ int next_vertex(const PtData& d, uint32_t pix, uint32
     const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
     const float t = __int_as_float(h.y);
     f3 p, gn, sn;
-    hit_attributes(d, o, dir, h.x, t, p, gn, sn);
+    hit_attributes(d, o, dir, h.x, t, __int_as_float(h.z), __int_as_float(h.w), p, gn, sn);
     const int32_t mi = d.mat_idx[h.x];
     bool stored = (flags & FLAG_VOL) != 0u;
     pt::VolumeState vs;
@@ -952,6 +952,9 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.mat_idx = ctx.material_indices.as<int32_t>();
     d.mats = R.materials.as<pt::Material>();
     d.mat_class = R.mat_class.as<int32_t>();
+    const bool smooth = R.settings.do_normal_mapping && ctx.has_normals.ptr && ctx.vertex_normals.ptr;
+    d.has_normals = smooth ? ctx.has_normals.as<uint8_t>() : nullptr;
+    d.vnormals = smooth ? ctx.vertex_normals.as<float>() : nullptr;
     d.emissive = ctx.emissive_prims.as<int32_t>();
     d.areas = R.areas.as<float>();
     d.emissive_count = ctx.emissive_count;

@@ -24,6 +24,10 @@ struct PtData {
     const int32_t* mat_class; // lobe class of each material (the shading order's sort key)
     const int32_t* emissive;
     const float* areas;
+    // smooth shading (get_shading_normal): per-vertex flag and normals, null
+    // when the scene has none or do_normal_mapping is 0
+    const uint8_t* has_normals;
+    const float* vnormals;
     int32_t emissive_count;
     // light sampling (LSS_BASE_UNIFORM / LSS_BASE_POWER)
     int32_t light_base;
@@ -141,16 +145,35 @@ __device__ __forceinline__ void fix_backfacing(f3 view, f3& gn, f3& sn)
     }
 }
 
-// Hit attributes of trace_main_path_ray for meshes without vertex normals
-// (Intersect.h:222-245).
-__device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, int32_t prim, float t, f3& p, f3& gn,
-                                               f3& sn)
+// Shading normal (get_shading_normal, Intersect.h:114-133): when the
+// triangle's first vertex has a vertex normal, the normalised barycentric
+// interpolation n_B u + n_C v + n_A (1 - u - v) (uv_interpolate,
+// Texture.h:248-251), else the geometric normal.
+__device__ __forceinline__ f3 shading_normal(const PtData& d, int32_t prim, float u, float v, f3 gn)
+{
+    if (!d.has_normals)
+        return gn;
+    const int32_t* t = d.idx + 3 * (int64_t)prim;
+    if (!d.has_normals[t[0]])
+        return gn;
+    const float* na = d.vnormals + 3 * (int64_t)t[0];
+    const float* nb = d.vnormals + 3 * (int64_t)t[1];
+    const float* nc = d.vnormals + 3 * (int64_t)t[2];
+    const float w = (1.0f - u) - v;
+    const f3 n = (u * pt::mk(nb[0], nb[1], nb[2]) + v * pt::mk(nc[0], nc[1], nc[2])) + w * pt::mk(na[0], na[1], na[2]);
+    return pt::normalize(n);
+}
+
+// Hit attributes of trace_main_path_ray (Intersect.h:222-245): geometric
+// normal, shading normal, back-face fix.
+__device__ __forceinline__ void hit_attributes(const PtData& d, f3 o, f3 dir, int32_t prim, float t, float u, float v,
+                                               f3& p, f3& gn, f3& sn)
 {
     f3 a, b, c;
     tri_verts(d, prim, a, b, c);
     p = o + t * dir;
     gn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
-    sn = gn;
+    sn = shading_normal(d, prim, u, v, gn);
     fix_backfacing(-1.0f * dir, gn, sn);
 }
 

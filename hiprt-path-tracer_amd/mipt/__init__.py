@@ -111,6 +111,7 @@ class RenderSettings(C.Structure):
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
         ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
+        ("do_normal_mapping", C.c_int32),
     ]
 
 
@@ -186,6 +187,8 @@ _SIGNATURES = {
                                              C.c_void_p]),
     "mipt_set_trace_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "mipt_set_shading_order": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mipt_update_materials": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "mipt_update_material": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mipt_camera_rays": (C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                    C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mipt_bounce_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -316,8 +319,15 @@ class Scene:
     """Host arrays of a scene (numpy), in the layout of struct Scene."""
 
     def __init__(self, vertices, indices, material_indices=None, materials=None, emissive=None,
-                 camera=None, width=0, height=0):
+                 camera=None, width=0, height=0, vertex_normals=None, has_vertex_normals=None):
         self.vertices = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        # smooth shading: float32 (V, 3) normals and a uint8 (V,) flag per vertex
+        self.vertex_normals = None if vertex_normals is None else \
+            np.ascontiguousarray(vertex_normals, dtype=np.float32).reshape(-1, 3)
+        self.has_vertex_normals = None
+        if self.vertex_normals is not None:
+            self.has_vertex_normals = np.ones(self.vertices.shape[0], np.uint8) if has_vertex_normals is None else \
+                np.ascontiguousarray(has_vertex_normals, dtype=np.uint8)
         self.indices = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
         self.material_indices = None if material_indices is None else np.ascontiguousarray(material_indices,
                                                                                               dtype=np.int32)
@@ -339,6 +349,9 @@ class Scene:
         d.vertex_count = self.vertices.shape[0]
         d.has_vertex_normals = None
         d.vertex_normals = None
+        if self.vertex_normals is not None:
+            d.has_vertex_normals = self.has_vertex_normals.ctypes.data_as(C.POINTER(C.c_uint8))
+            d.vertex_normals = self.vertex_normals.ctypes.data_as(C.POINTER(C.c_float))
         if self.material_indices is not None and self.materials is not None:
             d.material_indices = self.material_indices.ctypes.data_as(C.POINTER(C.c_int32))
             d.materials = C.cast(self.materials, C.POINTER(Material))
@@ -437,6 +450,15 @@ class Context:
         builder (the reference builds with HIPRT's high-quality flag,
         HIPRTScene.h:87-150), MIPT_BVH_LBVH for the faster LBVH build."""
         self._check(self.lib.mipt_build_bvh(self.h, builder), "mipt_build_bvh")
+
+    def update_materials(self, materials):
+        """GPURenderer::update_all_materials: a ctypes array of Material."""
+        self._check(self.lib.mipt_update_materials(self.h, C.cast(materials, C.c_void_p), len(materials)),
+                    "mipt_update_materials")
+
+    def update_material(self, index: int, material):
+        """GPURenderer::update_one_material."""
+        self._check(self.lib.mipt_update_material(self.h, index, C.byref(material)), "mipt_update_material")
 
     def set_shading_order(self, order: int):
         """MIPT_SHADE_ORDER_LIVE / _MATERIAL / _AUTO (the default)."""

@@ -190,6 +190,15 @@ const char* mipt_version(void);
  * (packed), emissive list. Does not build the BVH. */
 int mipt_set_scene(mipt_ctx* ctx, const mipt_scene_desc* scene);
 
+/* Replace GPURenderer::update_all_materials / update_one_material
+ * (src/Renderer/GPURenderer.cpp:1022-1054): new CPU materials for the scene
+ * (count = the scene's material count), packed and uploaded without a BVH
+ * rebuild. As in the reference, the emissive triangle list and the power
+ * alias table built from it stay as they were. Waits for the device before
+ * overwriting the material records. */
+int mipt_update_materials(mipt_ctx* ctx, const mipt_material* materials, int32_t count);
+int mipt_update_material(mipt_ctx* ctx, int32_t index, const mipt_material* material);
+
 /* Replaces HIPRTGeometry::build_bvh (HIPRTScene.h:87-150). Synchronous. */
 int mipt_build_bvh(mipt_ctx* ctx, int32_t builder);
 int mipt_get_bvh_info(const mipt_ctx* ctx, mipt_bvh_info* out);
@@ -297,6 +306,12 @@ typedef struct mipt_render_settings {
     /* 1: query the learned radiance cache (see mipt_set_radiance_cache); not
      * a reference setting. Default 0. */
     int32_t radiance_cache;
+    /* RenderSettings.h:203 (default 1): the shading normal of a hit on a
+     * triangle whose first vertex has a vertex normal is the normalised
+     * barycentric interpolation of its three vertex normals
+     * (get_shading_normal, Intersect.h:114-133; normal maps need textures,
+     * which this build does not upload); 0 keeps the geometric normal. */
+    int32_t do_normal_mapping;
 } mipt_render_settings;
 
 #define MIPT_LSS_MIS_LIGHT_BSDF 3 /* LSS_MIS_LIGHT_BSDF: one light sample + one BSDF sample, balance heuristic */
@@ -345,9 +360,11 @@ int mipt_set_restir_di_settings(mipt_ctx* ctx, const mipt_restir_di_settings* s)
 
 /* Allocates the G-buffer, path state and framebuffer for width x height and
  * packs the scene's materials (replaces GPURenderer::resize + the render-data
- * wiring of update_render_data, GPURenderer.cpp:441, :769-820). Fails with
- * MIPT_ERR_UNSUPPORTED if a material uses a BSDF lobe the passes do not
- * implement (coat, sheen, specular, glass, diffuse transmission, thin film). */
+ * wiring of update_render_data, GPURenderer.cpp:441, :769-820). Every lobe
+ * of the layered Principled BSDF is implemented (coat, sheen, the two metal
+ * lobes, dielectric specular, diffuse, glass with dispersion, diffuse
+ * transmission, thin film); a dispersive glass is refused
+ * (MIPT_ERR_UNSUPPORTED) only under ReSTIR DI, at the pass. */
 int mipt_set_render_settings(mipt_ctx* ctx, const mipt_render_settings* s);
 
 /* Replaces GPURenderer::set_camera (GPURenderer.cpp:1102). */

@@ -181,6 +181,8 @@ typedef struct {
     int32_t accumulate_fp64;
     int32_t direct_light_sampling_strategy;      /* 3 MIS light + BSDF, 5 ReSTIR DI */
     int32_t direct_light_sampling_base_strategy; /* 0 uniform, 1 power */
+    int32_t radiance_cache;                      /* not read by the oracle */
+    int32_t do_normal_mapping;                   /* RenderSettings.h:203: smooth vertex normals */
 } orc_render_settings;
 
 /* ReSTIRDISettings subset (ReSTIRDISettings.h:17-163), same order as the
@@ -226,6 +228,8 @@ typedef struct {
     const int32_t* emissive;        /* int[E] */
     int32_t emissive_count;
     orc_luts luts;
+    const uint8_t* has_vertex_normals; /* uint8[V] or NULL */
+    const float* vertex_normals;       /* float[3V] or NULL */
 } orc_render_scene;
 
 /* One sample of every pixel: CameraRays (CameraRays.h:83-213) then MegaKernel
@@ -253,6 +257,10 @@ void orc_frame_free(orc_frame* f);
  * shrinks by that radius from pass to pass reproduces the full-frame result
  * inside the last window. */
 void orc_frame_set_window(orc_frame* f, int32_t x0, int32_t x1, int32_t y0, int32_t y1);
+/* get_shading_normal (Intersect.h:114-133) of one hit: the interpolated
+ * vertex normal when vertex 0 of the triangle has one, else geometric_normal. */
+void orc_shading_normal(const int32_t* idx, const float* vertex_normals, const uint8_t* has_vertex_normals,
+                        int32_t prim, float u, float v, const float* geometric_normal, float* out);
 void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
 void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
 void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);

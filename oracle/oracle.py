@@ -39,7 +39,8 @@ class RenderSettings(C.Structure):
         ("enable_direct", C.c_int32), ("microfacet_regularization_tau_0", C.c_float),
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
-        ("direct_light_sampling_base_strategy", C.c_int32),
+        ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
+        ("do_normal_mapping", C.c_int32),
     ]
 
 
@@ -69,6 +70,7 @@ class RenderScene(C.Structure):
         ("vertices", C.c_void_p), ("indices", C.c_void_p), ("triangle_count", C.c_int32),
         ("material_indices", C.c_void_p), ("materials", C.c_void_p), ("material_count", C.c_int32),
         ("emissive", C.c_void_p), ("emissive_count", C.c_int32), ("luts", Luts),
+        ("has_vertex_normals", C.c_void_p), ("vertex_normals", C.c_void_p),
     ]
 
 
@@ -101,6 +103,7 @@ _SIG = {
     "orc_frame_create": (C.c_void_p, [_P, C.c_void_p, _P, _P, _P]),
     "orc_frame_free": (None, [C.c_void_p]),
     "orc_frame_set_window": (None, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "orc_shading_normal": (None, [_P, _P, _P, C.c_int32, C.c_float, C.c_float, _P, _P]),
     "orc_frame_camera_pass": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_initial": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_temporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
@@ -334,6 +337,12 @@ class Renderer:
         rs.emissive = _p(self.emissive)
         rs.emissive_count = self.emissive.size
         rs.luts = make_luts(self.luts)
+        self.vnormals = getattr(scene, "vertex_normals", None)
+        if self.vnormals is not None:
+            self.vnormals = np.ascontiguousarray(self.vnormals, np.float32).reshape(-1)
+            self.has_vnormals = np.ascontiguousarray(scene.has_vertex_normals, np.uint8)
+            rs.has_vertex_normals = _p(self.has_vnormals)
+            rs.vertex_normals = _p(self.vnormals)
         self.rs = rs
         self.cam = _cam(scene.camera)
         self.n = settings.width * settings.height
@@ -443,6 +452,17 @@ class Renderer:
         total = C.c_float()
         lib().orc_frame_alias_table(self.h, _p(probas), _p(alias), C.byref(total))
         return probas[:E], alias[:E], total.value
+
+
+def shading_normal(indices, vertex_normals, has_vertex_normals, prim, u, v, geometric_normal):
+    """The oracle's get_shading_normal for one hit (prim, barycentrics u, v)."""
+    idx = np.ascontiguousarray(indices, np.int32).reshape(-1)
+    vn = np.ascontiguousarray(vertex_normals, np.float32).reshape(-1)
+    has = np.ascontiguousarray(has_vertex_normals, np.uint8)
+    gn = np.ascontiguousarray(geometric_normal, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_shading_normal(_p(idx), _p(vn), _p(has), int(prim), float(u), float(v), _p(gn), _p(out))
+    return out
 
 
 def alias_table(weights, total_sum):

@@ -229,6 +229,8 @@ typedef struct {
     const int32_t* emissive;
     int32_t ne;
     const float* areas;
+    const uint8_t* has_normals; /* smooth shading; NULL: geometric normals */
+    const float* vnormals;
     const float* lut;        /* GGX conductor E_ss, 128 x 128 */
     const float* lut_glossy; /* glossy dielectric E_ss, [128][64][128] */
     const float* lut_sheen;  /* sheen LTC (Ai, Bi, Ri), 32 x 32 */
@@ -1578,6 +1580,39 @@ static void fix_backfacing(v3 view, v3* gn, v3* sn)
     }
 }
 
+/* get_shading_normal (Intersect.h:114-133): with a vertex normal at the
+ * triangle's first vertex, normalize(n_B u + n_C v + n_A (1 - u - v))
+ * (uv_interpolate, Texture.h:248-251); otherwise the geometric normal. */
+static v3 shading_normal(const rctx* R, int32_t prim, float u, float v, v3 gn)
+{
+    if (!R->has_normals)
+        return gn;
+    const int32_t* t = R->idx + 3 * (int64_t)prim;
+    if (!R->has_normals[t[0]])
+        return gn;
+    const float* na = R->vnormals + 3 * (int64_t)t[0];
+    const float* nb = R->vnormals + 3 * (int64_t)t[1];
+    const float* nc = R->vnormals + 3 * (int64_t)t[2];
+    float w = (1.0f - u) - v;
+    v3 n = vadd(vadd(vscale(V(nb[0], nb[1], nb[2]), u), vscale(V(nc[0], nc[1], nc[2]), v)),
+                vscale(V(na[0], na[1], na[2]), w));
+    return vnorm(n);
+}
+
+void orc_shading_normal(const int32_t* idx, const float* vertex_normals, const uint8_t* has_vertex_normals,
+                        int32_t prim, float u, float v, const float* geometric_normal, float* out)
+{
+    rctx R;
+    memset(&R, 0, sizeof(R));
+    R.idx = idx;
+    R.has_normals = has_vertex_normals;
+    R.vnormals = vertex_normals;
+    v3 n = shading_normal(&R, prim, u, v, V(geometric_normal[0], geometric_normal[1], geometric_normal[2]));
+    out[0] = n.x;
+    out[1] = n.y;
+    out[2] = n.z;
+}
+
 static int trace_main(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out)
 {
     orc_hit h = trace1(R, o, d, last, INFINITY);
@@ -1587,7 +1622,7 @@ static int trace_main(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out)
     tri(R, h.prim, &a, &b, &c);
     out->p = vadd(o, vscale(d, h.t));
     out->gn = vnorm(vnorm(vcross(vsub(b, a), vsub(c, a))));
-    out->sn = out->gn;
+    out->sn = shading_normal(R, h.prim, h.u, h.v, out->gn);
     out->prim = h.prim;
     fix_backfacing(vneg(d), &out->gn, &out->sn);
     return 1;
@@ -1610,7 +1645,7 @@ static int trace_path(const rctx* R, v3 o, v3 d, int32_t last, hitinfo* out, ovo
         tri(R, h.prim, &a, &b, &c);
         out->p = vadd(o, vscale(d, h.t));
         out->gn = vnorm(vnorm(vcross(vsub(b, a), vsub(c, a))));
-        out->sn = out->gn;
+        out->sn = shading_normal(R, h.prim, h.u, h.v, out->gn);
         out->prim = h.prim;
         int mat = R->mat_idx[h.prim];
         skip = vol_push(v, mat, R->mats[mat].dielectric_priority);
@@ -2335,6 +2370,11 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
     R.emissive = sc->emissive;
     R.ne = sc->emissive_count;
     R.areas = f->areas;
+    if (s->do_normal_mapping && sc->has_vertex_normals && sc->vertex_normals)
+    {
+        R.has_normals = sc->has_vertex_normals;
+        R.vnormals = sc->vertex_normals;
+    }
     R.lut = sc->luts.conductor;
     R.lut_glossy = sc->luts.glossy_dielectric;
     R.lut_sheen = sc->luts.sheen_ltc;

@@ -208,10 +208,12 @@ def test_shading_order_does_not_change_the_image(mipt_mod, torch_mod, scene_kind
     material lobe class (and stored volume state); each path's computation
     depends only on its own state, so the framebuffer (FP32 and FP64 sums)
     is bit-identical in every order: the config-1 scene (Lambertian and
-    metal classes), the nested-glass scene (glass, diffuse transmission,
-    coat, paths inside media) and the config-1 scene with the radiance
-    cache (the sorted list is then the training paths' list past the query
-    bounce)."""
+    metal classes) and the nested-glass scene (glass, diffuse transmission,
+    coat, paths inside media). With the radiance cache (the sorted list is
+    then the training paths' list past the query bounce) the paths are the
+    same, but the network's gradients are FP32 atomic sums whose order
+    follows the records' order, so the trained weights and the images agree
+    to rounding only: mean relative difference below 1e-3."""
     from test_gpu_glass import nested_glass_scene
 
     w = h = 128
@@ -233,12 +235,126 @@ def test_shading_order_does_not_change_the_image(mipt_mod, torch_mod, scene_kind
         for s in range(3):
             ctx.render_sample(s)
         fb, fb64 = ctx.read_framebuffer(fp64=True)
-        out[order] = (fb.tobytes(), fb64.tobytes())
+        out[order] = (fb.tobytes(), fb64.tobytes()) if not cache else fb64.astype(np.float64)
         ctx.close()
-    assert out[0] == out[1] == out[2]
+    if cache:
+        for o in (1, 2):
+            rel = np.abs(out[o] - out[0]) / (np.abs(out[0]) + 1e-3)
+            assert rel.mean() < 1e-3, (o, rel.mean())
+    else:
+        assert out[0] == out[1] == out[2]
     with pytest.raises(mipt_mod.MiptError):
         ctx = mipt_mod.Context(0)
         try:
             ctx.set_shading_order(3)
         finally:
             ctx.close()
+
+
+def smooth_sphere_scene(mipt_mod, w, h, config=1):
+    """The config-1 (or config-2) scene with vertex normals on every
+    icosphere: normalize(vertex - sphere centre); the walls, the terrain and
+    the lights keep geometric normals (no flag on their vertices)."""
+    scene = mipt_mod.generate_scene(config, w, h)
+    mi = scene.material_indices
+    vn = np.zeros_like(scene.vertices)
+    has = np.zeros(scene.vertices.shape[0], np.uint8)
+    per_sphere = 5120 if config == 1 else 1280
+    for k in range(len(scene.materials)):
+        sel = mi == k
+        if sel.sum() != per_sphere:
+            continue
+        used = np.unique(scene.indices[sel])
+        d = scene.vertices[used].astype(np.float64) - scene.vertices[used].astype(np.float64).mean(axis=0)
+        vn[used] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        has[used] = 1
+    assert has.sum() > 0
+    return mipt_mod.Scene(scene.vertices, scene.indices, scene.material_indices, scene.materials, scene.emissive,
+                          scene.camera, w, h, vertex_normals=vn, has_vertex_normals=has)
+
+
+def test_config1_smooth_normals(mipt_mod, oracle_mod, torch_mod):
+    """Smooth shading normals (get_shading_normal, Intersect.h:114-133) on
+    the spheres: bit-exact against the oracle; the image differs from the
+    flat-shaded one, and do_normal_mapping = 0 gives the flat-shaded image."""
+    w = h = 128
+    scene = smooth_sphere_scene(mipt_mod, w, h)
+    s = mipt_mod.default_render_settings(width=w, height=h)
+    fb, _ = _check(mipt_mod, oracle_mod, scene, s, 4)
+    flat = mipt_mod.Scene(scene.vertices, scene.indices, scene.material_indices, scene.materials, scene.emissive,
+                          scene.camera, w, h)
+    fb_flat = _render_gpu(mipt_mod, flat, s, 4)[0]
+    off = _render_gpu(mipt_mod, scene, mipt_mod.default_render_settings(width=w, height=h, do_normal_mapping=0), 4)[0]
+    assert off.tobytes() == fb_flat.tobytes()
+    changed = (fb.reshape(-1, 3) != fb_flat.reshape(-1, 3)).any(axis=1).mean()
+    assert changed > 0.05
+
+
+def test_restir_smooth_normals(mipt_mod, oracle_mod, torch_mod):
+    """ReSTIR DI surfaces read the G-buffer's shading normal: the
+    config-2 geometry with smooth spheres at 160x90, at-scale ReSTIR
+    settings, bit-exact against the oracle over 3 frames."""
+    from test_gpu_restir import render_both, simplify_materials
+
+    scene = simplify_materials(smooth_sphere_scene(mipt_mod, 160, 90, config=2))
+    s = mipt_mod.default_render_settings(width=160, height=90, nb_bounces=3, direct_light_sampling_strategy=5,
+                                         direct_light_sampling_base_strategy=1)
+    rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
+                                             spatial_number_of_passes=2)
+    fb, R = render_both(mipt_mod, oracle_mod, scene, s, rs, 3, check_every_frame=True)
+    assert fb.mean() > 0.0
+
+
+def test_update_materials_without_rebuild(mipt_mod, torch_mod):
+    """GPURenderer::update_all_materials / update_one_material: new materials
+    on a context that has rendered, without a BVH rebuild, render exactly
+    what a fresh context built with them renders; a glass material switches
+    the shading order's volume handling on; a wrong count or index is
+    refused and leaves the materials as they were."""
+    import ctypes as C
+
+    w = h = 96
+    scene = mipt_mod.generate_scene(1, w, h)
+    s = mipt_mod.default_render_settings(width=w, height=h)
+    changed = (type(scene.materials[0]) * len(scene.materials))(*scene.materials)
+    sphere = [k for k in range(len(changed)) if (scene.material_indices == k).sum() == 5120]
+    changed[sphere[0]].metallic = 1.0
+    changed[sphere[0]].roughness = 0.2
+    changed[sphere[1]].specular_transmission = 1.0
+    changed[sphere[1]].metallic = 0.0
+    changed[sphere[1]].ior = 1.5
+    changed[sphere[2]].base_color[0] = 0.05
+
+    def fresh(mats):
+        ctx = mipt_mod.Context(0)
+        ctx.set_scene(mipt_mod.Scene(scene.vertices, scene.indices, scene.material_indices, mats, scene.emissive,
+                                     scene.camera, w, h))
+        ctx.build_bvh()
+        ctx.set_render_settings(s)
+        ctx.set_camera(scene.camera)
+        return ctx
+
+    def render(ctx):
+        for sn in range(3):
+            ctx.render_sample(sn)
+        return ctx.read_framebuffer().tobytes()
+
+    ref = fresh(changed)
+    want = render(ref)
+    ref.close()
+    ctx = fresh(scene.materials)
+    before = render(ctx)
+    ctx.update_materials(changed)
+    assert render(ctx) == want
+    ctx.update_materials(scene.materials)
+    assert render(ctx) == before
+    for k in (sphere[0], sphere[1], sphere[2]):
+        ctx.update_material(k, changed[k])
+    assert render(ctx) == want
+    with pytest.raises(mipt_mod.MiptError):
+        ctx.update_material(len(changed), changed[0])
+    short = (type(scene.materials[0]) * 3)(*list(scene.materials)[:3])
+    with pytest.raises(mipt_mod.MiptError):
+        ctx.update_materials(short)
+    assert render(ctx) == want
+    ctx.close()
