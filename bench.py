@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--shade-order", choices=["live", "material", "auto"], default=None,
                     help="with --frames: the path pass's shading order (default: the library's, auto)")
     ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI leg")
+    ap.add_argument("--no-views", action="store_true", help="skip the batched-views leg (configuration 3)")
+    ap.add_argument("--views", type=int, default=96, help="views of the batched-views leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
     ap.add_argument("--bvh", choices=["lbvh", "sah"], default="sah",
@@ -227,6 +229,10 @@ def main():
         restir_layered = restir_render_leg(mipt, torch, np, layered=True)
     else:
         restir_layered = None
+    views = None
+    if not (args.no_render or args.no_views):
+        step(f"batched views leg: {args.views} views of the config-1 scene, 4 spp, 5 bounces")
+        views = views_render_leg(mipt, torch, np, args.views)
     cache = cache_tuned = None
     if not args.no_cache:
         step(f"radiance cache vs plain legs: config 1, 1920x1080; FP64 reference of {args.cache_ref_spp} spp")
@@ -274,6 +280,7 @@ def main():
         "plain_render": render,
         "restir_render": restir,
         "restir_render_layered": restir_layered,
+        "views_render": views,
         "cache_vs_plain": cache,
         "cache_vs_plain_tuned": cache_tuned,
     }
@@ -534,6 +541,84 @@ def restir_render_leg(mipt, torch, np, layered):
                                   "ucw_positive": int((res["ucw"] > 0).sum()),
                                   "ucw_nonfinite": int((~np.isfinite(res["ucw"])).sum())},
             "mean_radiance": [round(float(x), 5) for x in (fb.reshape(-1, 3) / (2 + frames)).mean(axis=0)]}
+
+
+def views_render_leg(mipt, torch, np, count):
+    """Configuration 3: `count` views of the config-1 scene (heights
+    log-uniform in [128, 1024] px, aspect in [1, 16/9]) rendered as one
+    batch at 4 spp, 5 bounces, uniform NEE + MIS: every pass kernel and
+    traversal launch covers the pixels of every view, the traversal launch
+    configuration is chosen by the autotuner on incoherent bounce rays of the
+    largest view, and the samples replay a captured hipGraph. Timed: 4-spp
+    frames of the batch (graph replays, and eager launches), and the same
+    views rendered one after the other (one view per launch sequence)."""
+    scene = mipt.generate_scene(1, 64, 64)
+    views = mipt.generate_views(count)
+    sizes = [(v.width, v.height) for v in views]
+    total = sum(w * h for w, h in sizes)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ctx = mipt.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    ctx.set_render_settings(mipt.default_render_settings(nb_bounces=5, accumulate_fp64=1))
+    # autotuner: incoherent bounce rays of the largest view
+    big = max(range(count), key=lambda i: sizes[i][0] * sizes[i][1])
+    w, h = sizes[big]
+    n = w * h
+    d_cam = mipt.empty_device(32 * n)
+    d_rng = mipt.empty_device(4 * n)
+    d_hits = mipt.empty_device(16 * n)
+    d_rays = mipt.empty_device(32 * n)
+    ctx.camera_rays(views[big].camera, w, h, 0, mipt.host_random_number(0), 0, d_cam, d_rng, sh)
+    ctx.trace_closest(d_cam, d_hits, n, sh)
+    ctx.bounce_rays(d_cam, d_hits, d_rng, d_rays, n, sh)
+    stream.synchronize()
+    tuning, table = ctx.autotune_trace(d_rays, n, 3, sh)
+    tuned_on = f"{n:,} incoherent bounce rays of the largest view ({w}x{h})"
+    del d_cam, d_rng, d_hits, d_rays
+    ctx.set_views(views)
+
+    def frame(first, spp):
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for s in range(first, first + spp):
+            ctx.render_sample(s, sh)
+        stream.synchronize()
+        return time.perf_counter() - t0
+
+    frame(0, 1)  # sample 0 runs eagerly (it resets the accumulation)
+    ctx.set_graph_capture(True)
+    frame(1, 2)
+    graph_s = min(frame(3, 4), frame(7, 4))
+    replays = ctx.graph_replays()
+    ctx.set_graph_capture(False)
+    eager_s = frame(11, 4)
+    fb = ctx.read_framebuffer()
+    mean = [round(float(x), 5) for x in (fb.reshape(-1, 3) / 15).mean(axis=0)]
+    # the same views one at a time
+    ctx.set_views([])
+    seq_s = 0.0
+    for v, (w, h) in zip(views, sizes):
+        ctx.set_render_settings(mipt.default_render_settings(width=w, height=h, nb_bounces=5, accumulate_fp64=1))
+        ctx.set_camera(v.camera)
+        seq_s += frame(0, 4)
+    ctx.close()
+    return {"workload": f"{count} views of the config-1 scene (102,428 tris), {total:,} pixels, heights log-uniform "
+                        f"[128, 1024], aspect [1, 16/9]; 4 spp, 5 bounces, uniform NEE + MIS; one wavefront over "
+                        f"every view",
+            "views": count, "pixels": total,
+            "batch_graph_ms_per_4spp": round(graph_s * 1e3, 2),
+            "batch_eager_ms_per_4spp": round(eager_s * 1e3, 2),
+            "one_view_at_a_time_ms_per_4spp": round(seq_s * 1e3, 2),
+            "batch_speedup_over_one_at_a_time": round(seq_s / graph_s, 3),
+            "Mpaths_per_s": round(total * 4 / graph_s / 1e6, 1),
+            "graph_replays": replays,
+            "autotune": {"block": tuning.block_size, "lds_stack": tuning.lds_stack, "queue_tile": tuning.queue_tile,
+                         "best_us": round(tuning.best_ms * 1e3, 1), "default_us": round(tuning.default_ms * 1e3, 1),
+                         "configurations": tuning.configurations,
+                         "rays": tuned_on},
+            "mean_radiance": mean}
 
 
 def plain_render_leg(mipt, torch, np, no_cpu):

@@ -240,6 +240,81 @@ void allocate_render_buffers(Ctx& ctx)
     HIP_CHECK(hipDeviceSynchronize());
 }
 
+// Batched views render with the plain path tracer only: ReSTIR DI reuses
+// neighbours in one image, and the radiance cache trains on one view's paths.
+void check_views_settings(const mipt_render_settings& s)
+{
+    if (s.direct_light_sampling_strategy != MIPT_LSS_MIS_LIGHT_BSDF || s.radiance_cache)
+        fail(MIPT_ERR_UNSUPPORTED, "batched views render with MIS light + BSDF sampling and no radiance cache");
+}
+
+uint64_t views_total(const std::vector<mipt_view>& views)
+{
+    uint64_t t = 0;
+    for (const mipt_view& v : views)
+        t += (uint64_t)v.width * (uint64_t)v.height;
+    return t;
+}
+
+void destroy_graph(RenderState& R)
+{
+    if (R.graph_exec)
+        (void)hipGraphExecDestroy(R.graph_exec);
+    if (R.graph)
+        (void)hipGraphDestroy(R.graph);
+    R.graph_exec = nullptr;
+    R.graph = nullptr;
+}
+
+// One plain sample (camera pass + path pass) as a replay of a captured
+// hipGraph. The capture records the kernels with the device record of the
+// per-sample values as their source; nothing is allocated while capturing
+// (the traversal's spill stack is sized first; every other buffer is
+// allocated with the render settings).
+void render_sample_graph(Ctx& ctx, uint32_t sample_number, hipStream_t stream)
+{
+    RenderState& R = ctx.render;
+    prepare_scene_for_render(ctx);
+    const uint32_t k = sample_number * 2u;
+    FrameParams p;
+    p.sample_number = sample_number;
+    p.rn_camera = mipt_host_random_number(k);
+    p.rn_path = mipt_host_random_number(k + 1u);
+    p.consistent_tau = consistent_tau(R.settings.microfacet_regularization_tau_0, sample_number);
+    p.consistent_tau_refract = consistent_tau(R.settings.microfacet_regularization_tau_0, sample_number + 1);
+    if (R.graph_dirty || !R.graph_exec)
+    {
+        destroy_graph(R);
+        if (!R.capture_stream)
+            HIP_CHECK(hipStreamCreateWithFlags(&R.capture_stream, hipStreamNonBlocking));
+        R.frame_params.reserve(sizeof(FrameParams));
+        trace_reserve(ctx);
+        HIP_CHECK(hipDeviceSynchronize());
+        const FrameParams* fp = R.frame_params.as<FrameParams>();
+        HIP_CHECK(hipStreamBeginCapture(R.capture_stream, hipStreamCaptureModeRelaxed));
+        try
+        {
+            camera_rays_pass(ctx, sample_number, p.rn_camera, R.capture_stream, fp);
+            path_trace_pass(ctx, sample_number, p.rn_path, R.capture_stream, false, fp);
+        }
+        catch (...)
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(R.capture_stream, &g);
+            if (g)
+                (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(R.capture_stream, &R.graph));
+        HIP_CHECK(hipGraphInstantiate(&R.graph_exec, R.graph, nullptr, nullptr, 0));
+        R.graph_dirty = false;
+        R.graph_replays = 0;
+    }
+    write_frame_params(R.frame_params.as<FrameParams>(), p, stream);
+    HIP_CHECK(hipGraphLaunch(R.graph_exec, stream));
+    R.graph_replays++;
+}
+
 } // namespace
 } // namespace mipt
 
@@ -368,6 +443,9 @@ void mipt_destroy(mipt_ctx* c)
         return;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
+    mipt::destroy_graph(ctx->render);
+    if (ctx->render.capture_stream)
+        (void)hipStreamDestroy(ctx->render.capture_stream);
     delete ctx;
 }
 
@@ -380,6 +458,8 @@ const char* mipt_last_error(const mipt_ctx* c)
 int mipt_set_scene(mipt_ctx* c, const mipt_scene_desc* s)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx)
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
@@ -449,6 +529,8 @@ int mipt_set_scene(mipt_ctx* c, const mipt_scene_desc* s)
 int mipt_build_bvh(mipt_ctx* c, int32_t builder)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx)
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
@@ -506,6 +588,8 @@ int mipt_read_bvh(const mipt_ctx* c, void* host_nodes, void* host_triangles)
 int mipt_set_trace_config(mipt_ctx* c, int32_t block_size, int32_t lds_stack)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx)
         return MIPT_ERR_INVALID_ARGUMENT;
     if (block_size != 0 && block_size != 64 && block_size != 128 && block_size != 256)
@@ -522,6 +606,8 @@ int mipt_set_trace_config(mipt_ctx* c, int32_t block_size, int32_t lds_stack)
 int mipt_update_materials(mipt_ctx* c, const mipt_material* materials, int32_t count)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || !materials)
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
@@ -569,6 +655,8 @@ int mipt_update_material(mipt_ctx* c, int32_t index, const mipt_material* materi
 int mipt_set_shading_order(mipt_ctx* c, int32_t order)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || order < MIPT_SHADE_ORDER_LIVE || order > MIPT_SHADE_ORDER_AUTO)
         return MIPT_ERR_INVALID_ARGUMENT;
     ctx->render.shade_order = order;
@@ -690,6 +778,8 @@ void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
 int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || !s || s->number_of_initial_light_candidates < 0 || s->number_of_initial_bsdf_candidates < 0 ||
         s->spatial_number_of_passes < 1 || s->spatial_reuse_radius < 0 || s->spatial_reuse_neighbor_count < 0 ||
         s->spatial_reuse_neighbor_count > 32 || s->m_cap < 0)
@@ -704,6 +794,8 @@ int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
 int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || !s || s->width <= 0 || s->height <= 0 || s->nb_bounces < 0 ||
         (int64_t)s->width * (int64_t)s->height > (1ll << 28) ||
         (s->direct_light_sampling_strategy != MIPT_LSS_MIS_LIGHT_BSDF &&
@@ -713,7 +805,14 @@ int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
         (s->radiance_cache != 0 && s->radiance_cache != 1))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
-        ctx->render.settings = *s;
+        mipt_render_settings settings = *s;
+        if (!ctx->render.views.empty())
+        {
+            mipt::check_views_settings(settings);
+            settings.width = (int32_t)mipt::views_total(ctx->render.views);
+            settings.height = 1;
+        }
+        ctx->render.settings = settings;
         ctx->render.configured = false;
         mipt::prepare_scene_for_render(*ctx);
         mipt::allocate_render_buffers(*ctx);
@@ -721,9 +820,106 @@ int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
     });
 }
 
+int mipt_set_views(mipt_ctx* c, const mipt_view* views, int32_t count)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || count < 0 || (count > 0 && !views))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        mipt::RenderState& R = ctx->render;
+        R.graph_dirty = true;
+        if (count == 0)
+        {
+            R.views.clear();
+            R.configured = false; // mipt_set_render_settings sets the single view's size again
+            return;
+        }
+        uint64_t total = 0;
+        for (int32_t v = 0; v < count; v++)
+        {
+            if (views[v].width <= 0 || views[v].height <= 0)
+                mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_views: view " + std::to_string(v) + " has no pixels");
+            total += (uint64_t)views[v].width * (uint64_t)views[v].height;
+        }
+        if (total > (1ull << 28))
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_views: more than 2^28 pixels in all");
+        mipt_render_settings settings = R.settings;
+        mipt::check_views_settings(settings);
+        std::vector<mipt::ViewDev> table((size_t)count);
+        uint32_t start = 0;
+        for (int32_t v = 0; v < count; v++)
+        {
+            mipt::ViewDev& d = table[(size_t)v];
+            std::memcpy(d.iv, views[v].camera.inverse_view, sizeof(d.iv));
+            std::memcpy(d.ip, views[v].camera.inverse_projection, sizeof(d.ip));
+            std::memcpy(d.position, views[v].camera.position, sizeof(d.position));
+            d.jitter = views[v].camera.do_jittering;
+            d.width = views[v].width;
+            d.height = views[v].height;
+            d.start = start;
+            start += (uint32_t)views[v].width * (uint32_t)views[v].height;
+        }
+        HIP_CHECK(hipDeviceSynchronize()); // queued kernels may read the old table
+        mipt::upload(R.view_table, table.data(), table.size());
+        R.views.assign(views, views + count);
+        settings.width = (int32_t)total;
+        settings.height = 1;
+        R.settings = settings;
+        R.configured = false;
+        mipt::prepare_scene_for_render(*ctx);
+        mipt::allocate_render_buffers(*ctx);
+        R.configured = true;
+        R.camera = views[0].camera; // the first view's camera (ReSTIR DI and the cache are off)
+        R.camera_set = true;
+    });
+}
+
+int mipt_set_graph_capture(mipt_ctx* c, int32_t enable)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (enable != 0 && enable != 1))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    ctx->render.graph_enabled = enable != 0;
+    ctx->render.graph_dirty = true;
+    return MIPT_OK;
+}
+
+int mipt_graph_replays(const mipt_ctx* c, uint32_t* out)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    if (!ctx || !out)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    *out = ctx->render.graph_replays;
+    return MIPT_OK;
+}
+
+int mipt_set_queue_tile(mipt_ctx* c, int32_t tile)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || (tile != 1 && tile != 2 && tile != 4))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    ctx->trace_cfg.queue_tile = tile;
+    ctx->render.graph_dirty = true;
+    return MIPT_OK;
+}
+
+int mipt_autotune_trace(mipt_ctx* c, const mipt_ray* d_rays, uint32_t n, int32_t repeats, void* stream,
+                        mipt_trace_tuning* out, double* table_ms)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !d_rays || n == 0 || repeats < 1)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        ctx->render.graph_dirty = true;
+        mipt::autotune_trace(*ctx, d_rays, n, repeats, reinterpret_cast<hipStream_t>(stream), out, table_ms);
+    });
+}
+
 int mipt_set_camera(mipt_ctx* c, const mipt_camera* cam)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || !cam)
         return MIPT_ERR_INVALID_ARGUMENT;
     ctx->render.camera = *cam;
@@ -799,6 +995,16 @@ int render_sample_passes(mipt_ctx* c, uint32_t sample_number, void* stream, bool
         return MIPT_ERR_INVALID_ARGUMENT;
     if (warmup && !ctx->render.settings.radiance_cache)
         return MIPT_ERR_INVALID_ARGUMENT;
+    {
+        mipt::RenderState& R = ctx->render;
+        const bool plain = R.settings.direct_light_sampling_strategy == MIPT_LSS_MIS_LIGHT_BSDF &&
+                           !R.settings.radiance_cache && !warmup;
+        if (R.graph_enabled && plain && sample_number > 0 && R.configured && R.camera_set)
+            return mipt::guarded(ctx, [&] {
+                mipt::render_sample_graph(*ctx, sample_number, reinterpret_cast<hipStream_t>(stream));
+            });
+        R.graph_replays = 0;
+    }
     const uint32_t draws = (uint32_t)mipt_draws_per_sample(c);
     uint32_t k = sample_number * draws;
     int rc = mipt_launch_pass(c, MIPT_PASS_CAMERA_RAYS, sample_number, mipt_host_random_number(k++), stream);
@@ -971,6 +1177,8 @@ void mipt_radiance_cache_settings_default(mipt_radiance_cache_settings* s)
 int mipt_set_radiance_cache(mipt_ctx* c, const mipt_radiance_cache_settings* s)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || !s || s->grid_levels < 1 || s->grid_levels > 16 || s->grid_log2_table < 4 ||
         s->grid_log2_table > 24 || s->grid_base_resolution < 2 || !(s->grid_per_level_scale >= 1.0f) ||
         s->batch_size < 16 || s->batch_size > (1 << 22) || !(s->learning_rate > 0.0f) ||
@@ -1079,6 +1287,8 @@ int mipt_radiance_cache_get_params(mipt_ctx* c, float* mlp_master, uint16_t* mlp
 int mipt_radiance_cache_set_params(mipt_ctx* c, const float* mlp_master, const float* grid_master)
 {
     Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (ctx)
+        ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx)
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {

@@ -38,32 +38,80 @@ __device__ __forceinline__ pt::f3 matrix_x_point(const float (*m)[4], pt::f3 p)
     return pt::mk(xt * inv_w, yt * inv_w, zt * inv_w);
 }
 
-__global__ void k_camera_rays(CamDev cam, int32_t w, int32_t h, uint32_t sample_number, uint32_t random_number,
-                              int32_t freeze, float4* __restrict__ rays, uint32_t* __restrict__ rng_out)
+// Ray of pixel (x, y) of a w x h view: seed of its index in the view, two
+// jitter draws, get_camera_ray. Writes slot `out` (the pixel's index in the
+// buffer: the same in a single view, offset in a batch of views).
+__device__ __forceinline__ void camera_ray(const float (*iv)[4], const float (*ip)[4], int32_t jitter, int32_t x,
+                                           int32_t y, int32_t w, int32_t h, uint32_t sample_number,
+                                           uint32_t random_number, int32_t freeze, float4* __restrict__ rays,
+                                           uint32_t* __restrict__ rng_out, uint32_t out)
 {
-    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= w || y >= h)
-        return;
     const uint32_t pix = (uint32_t)x + (uint32_t)y * (uint32_t)w;
     pt::Xorshift32 rng{pt::pixel_seed(pix, sample_number, random_number, freeze != 0)};
     float px = (float)x + 0.5f;
     float py = (float)y + 0.5f;
-    if (cam.jitter)
+    if (jitter)
     {
         px += rng() - 0.5f;
         py += rng() - 0.5f;
     }
     const float xn = px / (float)w * 2.0f - 1.0f;
     const float yn = py / (float)h * 2.0f - 1.0f;
-    const pt::f3 origin = matrix_x_point(cam.iv, pt::mk(0.0f, 0.0f, 0.0f));
-    const pt::f3 p_vs = matrix_x_point(cam.ip, pt::mk(xn, yn, -1.0f));
-    const pt::f3 p_ws = matrix_x_point(cam.iv, p_vs);
+    const pt::f3 origin = matrix_x_point(iv, pt::mk(0.0f, 0.0f, 0.0f));
+    const pt::f3 p_vs = matrix_x_point(ip, pt::mk(xn, yn, -1.0f));
+    const pt::f3 p_ws = matrix_x_point(iv, p_vs);
     const pt::f3 dir = pt::normalize(p_ws - origin);
-    rays[2 * (size_t)pix] = make_float4(origin.x, origin.y, origin.z, __int_as_float(-1));
-    rays[2 * (size_t)pix + 1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
+    rays[2 * (size_t)out] = make_float4(origin.x, origin.y, origin.z, __int_as_float(-1));
+    rays[2 * (size_t)out + 1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
     if (rng_out)
-        rng_out[pix] = rng.state;
+        rng_out[out] = rng.state;
+}
+
+__global__ void k_camera_rays(CamDev cam, int32_t w, int32_t h, uint32_t sample_number, uint32_t random_number,
+                              int32_t freeze, float4* __restrict__ rays, uint32_t* __restrict__ rng_out,
+                              const FrameParams* __restrict__ frame)
+{
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h)
+        return;
+    if (frame)
+    {
+        sample_number = frame->sample_number;
+        random_number = frame->rn_camera;
+    }
+    camera_ray(cam.iv, cam.ip, cam.jitter, x, y, w, h, sample_number, random_number, freeze, rays, rng_out,
+               (uint32_t)x + (uint32_t)y * (uint32_t)w);
+}
+
+// The rays of a batch of views, one thread per pixel of the batch; the view
+// of a pixel by a binary search over the views' first pixels.
+__global__ void k_camera_rays_views(const ViewDev* __restrict__ views, int32_t view_count, uint32_t total,
+                                    uint32_t sample_number, uint32_t random_number, int32_t freeze,
+                                    float4* __restrict__ rays, const FrameParams* __restrict__ frame)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= total)
+        return;
+    if (frame)
+    {
+        sample_number = frame->sample_number;
+        random_number = frame->rn_camera;
+    }
+    int32_t lo = 0, hi = view_count - 1;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (views[mid].start <= pix)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const ViewDev& v = views[lo];
+    const uint32_t local = pix - v.start;
+    const int32_t x = (int32_t)(local % (uint32_t)v.width), y = (int32_t)(local / (uint32_t)v.width);
+    camera_ray(v.iv, v.ip, v.jitter, x, y, v.width, v.height, sample_number, random_number, freeze, rays, nullptr,
+               pix);
 }
 
 __device__ __forceinline__ pt::f3 reflect_ray(pt::f3 d, pt::f3 n)
@@ -139,7 +187,8 @@ __global__ void k_bounce_rays(const float* __restrict__ verts, const int32_t* __
 } // namespace
 
 void camera_rays(Ctx& ctx, const mipt_camera& cam, int32_t w, int32_t h, uint32_t sample_number,
-                 uint32_t random_number, int32_t freeze, mipt_ray* rays, uint32_t* rng, hipStream_t stream)
+                 uint32_t random_number, int32_t freeze, mipt_ray* rays, uint32_t* rng, hipStream_t stream,
+                 const FrameParams* frame)
 {
     (void)ctx;
     CamDev c;
@@ -149,7 +198,18 @@ void camera_rays(Ctx& ctx, const mipt_camera& cam, int32_t w, int32_t h, uint32_
     dim3 block(8, 8);
     dim3 grid((w + 7) / 8, (h + 7) / 8);
     hipLaunchKernelGGL(k_camera_rays, grid, block, 0, stream, c, w, h, sample_number, random_number, freeze,
-                       reinterpret_cast<float4*>(rays), rng);
+                       reinterpret_cast<float4*>(rays), rng, frame);
+    HIP_CHECK(hipGetLastError());
+}
+
+void camera_rays_views(Ctx& ctx, uint32_t sample_number, uint32_t random_number, int32_t freeze, mipt_ray* rays,
+                       hipStream_t stream, const FrameParams* frame)
+{
+    const RenderState& R = ctx.render;
+    const uint32_t total = R.pixels;
+    hipLaunchKernelGGL(k_camera_rays_views, dim3((total + 255) / 256), dim3(256), 0, stream,
+                       R.view_table.as<ViewDev>(), (int32_t)R.views.size(), total, sample_number, random_number,
+                       freeze, reinterpret_cast<float4*>(rays), frame);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -56,6 +56,29 @@ struct DeviceBuffer {
 struct TraceConfig {
     int block_size = 128;
     int lds_stack = 16;
+    int queue_tile = 1; // rays a wave takes per work-counter atomic, in units of 64
+};
+
+// Per-sample values the pass kernels read from device memory when they run
+// as a replayed hipGraph (written before each replay by a one-thread kernel);
+// eager launches pass them by value instead.
+struct FrameParams {
+    uint32_t sample_number;
+    uint32_t rn_camera;   // host random number of the camera pass
+    uint32_t rn_path;     // of the path-tracing pass
+    float consistent_tau; // consistent_tau(tau_0, sample_number)
+    float consistent_tau_refract;
+};
+
+// One camera view of a batch (device copy): the HIPRTCamera matrices the
+// ray generation reads, its resolution and its first pixel in the batch.
+struct ViewDev {
+    float iv[4][4];
+    float ip[4][4];
+    float position[3]; // the camera position (the first vertex's view direction)
+    int32_t jitter;
+    int32_t width, height;
+    uint32_t start;
 };
 
 struct Bvh {
@@ -86,6 +109,18 @@ struct RenderState {
     int32_t lobe_classes = 0;   // distinct lobe classes in the scene
     int32_t shade_order = MIPT_SHADE_ORDER_AUTO;
     DeviceBuffer shade_sorted, sort_counts; // live paths in shading order; class counters and cursors
+    // batched camera views (mipt_set_views): the pixels of every view back to
+    // back (settings.width = total, height = 1)
+    std::vector<mipt_view> views;
+    DeviceBuffer view_table;    // ViewDev[views]
+    // graph replay of a plain sample (mipt_set_graph_capture)
+    bool graph_enabled = false;
+    bool graph_dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    DeviceBuffer frame_params;  // FrameParams
+    uint32_t graph_replays = 0;
     DeviceBuffer areas;         // float[T]
     DeviceBuffer lut_conductor; // float[128*128]
     DeviceBuffer lut_glossy, lut_glass, lut_glass_inv, lut_thin_glass, lut_glass_exponent, lut_sheen;
@@ -168,6 +203,7 @@ struct Ctx {
 
     // Traversal spill stack (entries beyond the LDS stack), sized on demand.
     DeviceBuffer spill;
+    DeviceBuffer autotune_hits; // hit records of the autotuner's timed launches
 
     RenderState render;
 };
@@ -185,10 +221,13 @@ const float* glass_exponent_data();        // [2][10][11][4]
 const float* sheen_ltc_data();             // [32][32][3]
 
 // pathtrace.hip
-void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+// With `frame` (graph capture), the kernels read the sample's values from it.
+void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream,
+                      const FrameParams* frame = nullptr);
 void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream,
-                     bool warmup = false);
+                     bool warmup = false, const FrameParams* frame = nullptr);
 float consistent_tau(float tau_0, uint32_t sample_number);
+void write_frame_params(FrameParams* frame, const FrameParams& values, hipStream_t stream);
 
 // lights.cpp: Utils::compute_alias_table (src/Utils/Utils.cpp:63-131) and the
 // emissive power list (GPURenderer.cpp:221-295).
@@ -213,6 +252,15 @@ void build_sah_binned(Ctx& ctx);
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n,
                    hipStream_t stream);
 void trace_any(Ctx& ctx, const mipt_ray* rays, uint8_t* occluded, uint32_t n, hipStream_t stream);
+// Sizes the traversal's spill stack for the current tree and launch
+// configuration (it is then never reallocated by a launch: graph capture).
+void trace_reserve(Ctx& ctx);
+// Times closest-hit launches over `rays` for every block size (64, 128,
+// 256), LDS stack depth (8, 16, 24, 32) and queue tile (1, 2, 4 batches of
+// 64 rays) and keeps the fastest; table (36 entries, [block][stack][tile])
+// receives every mean launch time in ms.
+void autotune_trace(Ctx& ctx, const mipt_ray* rays, uint32_t n, int32_t repeats, hipStream_t stream,
+                    mipt_trace_tuning* out, double* table);
 // mipt_trace_closest_noalloc: the closest-hit kernel with its work counter
 // and spill stack in the library's static device scratch, whose per-device
 // address noalloc_scratch_init looks up (once per device, device current).
@@ -237,7 +285,11 @@ void trace_queues(Ctx& ctx, const TraceQueue* queues, int count, uint32_t capaci
 // camera.hip
 void camera_rays(Ctx& ctx, const mipt_camera& cam, int32_t w, int32_t h, uint32_t sample_number,
                  uint32_t random_number, int32_t freeze, mipt_ray* rays, uint32_t* rng,
-                 hipStream_t stream);
+                 hipStream_t stream, const FrameParams* frame = nullptr);
+// The camera rays of every view of a batch (RenderState::views), written to
+// the views' pixels back to back.
+void camera_rays_views(Ctx& ctx, uint32_t sample_number, uint32_t random_number, int32_t freeze, mipt_ray* rays,
+                       hipStream_t stream, const FrameParams* frame);
 void bounce_rays(Ctx& ctx, const mipt_ray* primary, const mipt_hit* hits, uint32_t* rng, mipt_ray* out,
                  uint32_t n, hipStream_t stream);
 

@@ -145,12 +145,12 @@ __device__ __forceinline__ bool is_training_path(uint32_t pix, uint32_t sample, 
 // Megakernel.h:52-86: seed, first vertex from the G-buffer.
 __device__ __forceinline__ bool init_path(const PtData& d, uint32_t pix)
 {
-    if (d.train_only_every && !is_training_path(pix, d.sample_number, d.train_only_every))
+    if (d.train_only_every && !is_training_path(pix, frame_sample(d), d.train_only_every))
     {
         d.flags[pix] = 0u; // warm-up pass: only the training paths run
         return false;
     }
-    d.rng[pix] = pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0);
+    d.rng[pix] = pt::pixel_seed(view_pixel(d, pix), frame_sample(d), frame_random(d), d.freeze != 0);
     d.throughput[pix] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     d.radiance[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const int32_t prim = d.gb_prim[pix];
@@ -162,7 +162,12 @@ __device__ __forceinline__ bool init_path(const PtData& d, uint32_t pix)
     const f3 p = ld3(d.gb_pos[pix]);
     const f3 gn = gb_normal(d.gb_gn[pix]);
     const f3 sn = gb_normal(d.gb_sn[pix]);
-    const f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
+    f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
+    if (d.views)
+    {
+        const ViewDev& v = d.views[view_of(d.views, d.view_count, pix)];
+        cam = pt::mk(v.position[0], v.position[1], v.position[2]);
+    }
     // GBufferDevice.h:19-22: view = normalize(camera - hit); the path ray is
     // normalize(-view) and the BSDF view direction is -ray.direction.
     const f3 ray_dir = pt::normalize(-1.0f * pt::normalize(cam - p));
@@ -253,7 +258,7 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
         need_shadow = true;
     pt::BsdfCtx c{view, sn, gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::REGULARIZATION_MIS, 0.0f, &vs, 0};
     float bpdf;
-    const Color f = pt::principled_eval(d.g, m, c, bpdf);
+    const Color f = pt::principled_eval(frame_globals(d), m, c, bpdf);
     float cosine = pt::dot(sn, dir);
     if (flags & pt::FROM_GLASS_REFRACT)
         cosine = fabsf(cosine);
@@ -322,7 +327,7 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
                 pt::VolumeState vs_light = vs;
                 pt::BsdfCtx c{view, sn, gn, dirn, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness, &vs_light, 0};
                 float bsdf_pdf;
-                const Color f = pt::principled_eval(d.g, m, c, bsdf_pdf);
+                const Color f = pt::principled_eval(frame_globals(d), m, c, bsdf_pdf);
                 if (vs_light.distance != vs.distance)
                 {
                     light_floor = true;
@@ -351,7 +356,7 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
                       0};
         f3 dir;
         float bpdf;
-        const Color f = pt::principled_sample(d.g, m, c, dir, bpdf, rng);
+        const Color f = pt::principled_sample(frame_globals(d), m, c, dir, bpdf, rng);
         if (bpdf > 0.0f)
         {
             const float cosine = fabsf(pt::dot(sn, dir));
@@ -370,7 +375,7 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
                       1};
         f3 dir;
         float bpdf;
-        const Color f = pt::principled_sample(d.g, m, c, dir, bpdf, rng);
+        const Color f = pt::principled_sample(frame_globals(d), m, c, dir, bpdf, rng);
         // RayPayload::accumulate_roughness (RayPayload.h:52-89)
         if (c.info == pt::FROM_DIFFUSE || c.info == pt::FROM_DIFFUSE_TRANSMISSION)
             path_roughness = 1.0f;
@@ -729,11 +734,11 @@ __global__ void k_accumulate(PtData d)
     {
         atomicAdd(d.rejected, 1u);
         r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (d.sample_number != 0)
+        if (frame_sample(d) != 0)
             return;
     }
     float* fb = d.fb + 3 * (size_t)pix;
-    if (d.sample_number == 0)
+    if (frame_sample(d) == 0)
     {
         fb[0] = r.x;
         fb[1] = r.y;
@@ -748,7 +753,7 @@ __global__ void k_accumulate(PtData d)
     if (d.fb64)
     {
         double* f64 = d.fb64 + 3 * (size_t)pix;
-        if (d.sample_number == 0)
+        if (frame_sample(d) == 0)
         {
             f64[0] = r.x;
             f64[1] = r.y;
@@ -952,6 +957,9 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.mat_idx = ctx.material_indices.as<int32_t>();
     d.mats = R.materials.as<pt::Material>();
     d.mat_class = R.mat_class.as<int32_t>();
+    d.views = R.views.empty() ? nullptr : R.view_table.as<ViewDev>();
+    d.view_count = (int32_t)R.views.size();
+    d.frame = nullptr;
     const bool smooth = R.settings.do_normal_mapping && ctx.has_normals.ptr && ctx.vertex_normals.ptr;
     d.has_normals = smooth ? ctx.has_normals.as<uint8_t>() : nullptr;
     d.vnormals = smooth ? ctx.vertex_normals.as<float>() : nullptr;
@@ -1020,7 +1028,19 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     return d;
 }
 
-void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+// The per-sample record of a graph replay, written by one thread: a
+// kernel's arguments are copied at launch, so the host may fill the next
+// sample's values at once (no pinned buffer to keep alive).
+__global__ void k_set_frame(FrameParams* __restrict__ frame, FrameParams values) { *frame = values; }
+
+void write_frame_params(FrameParams* frame, const FrameParams& values, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, frame, values);
+    HIP_CHECK(hipGetLastError());
+}
+
+void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s,
+                      const FrameParams* frame)
 {
     RenderState& R = ctx.render;
     const int32_t W = R.settings.width, H = R.settings.height;
@@ -1030,7 +1050,7 @@ void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, 
                     R.gb_mat.as<int32_t>()};
     GBufferView prev{R.pg_prim.as<int32_t>(), R.pg_pos.as<float4>(), R.pg_gn.as<uint32_t>(), R.pg_sn.as<uint32_t>(),
                      R.pg_mat.as<int32_t>()};
-    if (sample_number == 0)
+    if (sample_number == 0 && !frame)
     {
         // need_to_reset: G-buffers and ReSTIR DI reservoirs start over.
         hipLaunchKernelGGL(k_gbuffer_reset, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur, n);
@@ -1038,20 +1058,27 @@ void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, 
             hipLaunchKernelGGL(k_gbuffer_reset, dim3(blocks_for(n, 256)), dim3(256), 0, s, prev, n);
         restir_reset(ctx, s);
     }
-    camera_rays(ctx, R.camera, W, H, sample_number, random_number, R.settings.freeze_random,
-                R.cont_rays.as<mipt_ray>(), nullptr, s);
+    if (!R.views.empty())
+        camera_rays_views(ctx, sample_number, random_number, R.settings.freeze_random, R.cont_rays.as<mipt_ray>(), s,
+                          frame);
+    else
+        camera_rays(ctx, R.camera, W, H, sample_number, random_number, R.settings.freeze_random,
+                    R.cont_rays.as<mipt_ray>(), nullptr, s, frame);
     trace_closest(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), nullptr, n, s);
     PtData d = make_pt_data(ctx, sample_number, random_number);
+    d.frame = frame;
     hipLaunchKernelGGL(k_gbuffer_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, d.cont_rays, d.cont_hits, prev,
                        (int32_t)keep_prev);
     HIP_CHECK(hipGetLastError());
 }
 
-void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s, bool warmup)
+void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s, bool warmup,
+                     const FrameParams* frame)
 {
     RenderState& R = ctx.render;
     const uint32_t n = (uint32_t)R.settings.width * (uint32_t)R.settings.height;
     PtData d = make_pt_data(ctx, sample_number, random_number);
+    d.frame = frame;
     const unsigned g = blocks_for(n, 256);
     const bool cache = R.settings.radiance_cache != 0;
     if (warmup && !cache)

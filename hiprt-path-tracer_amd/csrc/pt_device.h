@@ -22,6 +22,11 @@ struct PtData {
     const int32_t* mat_idx;
     const pt::Material* mats;
     const int32_t* mat_class; // lobe class of each material (the shading order's sort key)
+    // batched views: first pixel of each view (null with one view)
+    const ViewDev* views;
+    int32_t view_count;
+    // graph replay: per-sample values in device memory (null: the fields below)
+    const FrameParams* frame;
     const int32_t* emissive;
     const float* areas;
     // smooth shading (get_shading_normal): per-vertex flag and normals, null
@@ -88,6 +93,50 @@ struct PtData {
 };
 
 PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number);
+
+// The sample's values: from the graph's device record when replaying.
+__device__ __forceinline__ uint32_t frame_sample(const PtData& d)
+{
+    return d.frame ? d.frame->sample_number : d.sample_number;
+}
+__device__ __forceinline__ uint32_t frame_random(const PtData& d)
+{
+    return d.frame ? d.frame->rn_path : d.random_number;
+}
+__device__ __forceinline__ pt::BsdfGlobals frame_globals(const PtData& d)
+{
+    pt::BsdfGlobals g = d.g;
+    if (d.frame)
+    {
+        g.consistent_tau = d.frame->consistent_tau;
+        g.consistent_tau_refract = d.frame->consistent_tau_refract;
+    }
+    return g;
+}
+
+// View of pixel `pix` of a batch (last view whose first pixel is <= pix).
+__device__ __forceinline__ int32_t view_of(const ViewDev* views, int32_t count, uint32_t pix)
+{
+    int32_t lo = 0, hi = count - 1;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (views[mid].start <= pix)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// The pixel's index within its own view: what the per-pixel random seed of
+// a single-view render uses (Megakernel.h:52-57).
+__device__ __forceinline__ uint32_t view_pixel(const PtData& d, uint32_t pix)
+{
+    if (!d.views)
+        return pix;
+    return pix - d.views[view_of(d.views, d.view_count, pix)].start;
+}
 
 __device__ __forceinline__ f3 ld3(float4 v) { return pt::mk(v.x, v.y, v.z); }
 __device__ __forceinline__ float4 st3(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }

@@ -450,6 +450,29 @@ extern "C" int mipt_scene_generate(int32_t config_id, int32_t width, int32_t hei
     }
 }
 
+extern "C" int mipt_views_generate(int32_t count, mipt_view* out)
+{
+    if (count < 0 || (count > 0 && !out))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    mipt::Pcg32 rng(0x2026120200000000ULL + 3, 3);
+    for (int32_t v = 0; v < count; v++)
+    {
+        // height log-uniform in [128, 1024], aspect uniform in [1, 16/9]
+        const double h = 128.0 * std::exp(rng.uniform() * std::log(8.0));
+        const double aspect = 1.0 + rng.uniform() * (16.0 / 9.0 - 1.0);
+        const int32_t H = (int32_t)std::lround(h);
+        const int32_t W = (int32_t)std::lround(h * aspect);
+        // inside the box, in front of the spheres, looking at the back wall
+        const mipt::V3 eye{rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6), rng.uniform(0.5, 0.98)};
+        const mipt::V3 target{rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4), -1.0};
+        const double vfov = rng.uniform(40.0, 70.0) * M_PI / 180.0;
+        out[v].camera = mipt::make_camera(eye, target, {0, 1, 0}, vfov, W, H);
+        out[v].width = W;
+        out[v].height = H;
+    }
+    return MIPT_OK;
+}
+
 extern "C" void mipt_scene_free(mipt_generated_scene* g)
 {
     if (!g)

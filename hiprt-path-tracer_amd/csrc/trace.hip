@@ -183,29 +183,31 @@ __device__ __forceinline__ int4 hit_record(const RayHit& h)
                        : make_int4(-1, __float_as_int(-1.0f), 0, 0);
 }
 
-// Next batch of 64 rays of a wave. Each wave's first batch is its own by
-// index; later batches come from the work counter, past the statically
-// assigned ones. A launch on fewer rays than the grid's lanes then touches
-// the counter only from the waves that had work: a same-address atomic from
-// every wave costs tens of microseconds per launch.
+// Next tile of a wave: `tile` consecutive batches of 64 rays (the queue
+// tile). Each wave's first tile is its own by index; later tiles come from
+// the work counter, past the statically assigned ones. A launch on fewer
+// rays than the grid's lanes then touches the counter only from the waves
+// that had work: a same-address atomic from every wave costs tens of
+// microseconds per launch. Larger tiles take fewer atomics and balance
+// the waves' uneven ray lengths less finely.
 struct BatchCursor {
     uint32_t next;
     bool first;
 
-    __device__ __forceinline__ void init(int block)
+    __device__ __forceinline__ void init(int block, uint32_t tile)
     {
-        next = (blockIdx.x * ((uint32_t)block / 64u) + threadIdx.x / 64u) * 64u;
+        next = (blockIdx.x * ((uint32_t)block / 64u) + threadIdx.x / 64u) * 64u * tile;
         first = true;
     }
 
-    __device__ __forceinline__ uint32_t take(uint32_t* work_counter, int block)
+    __device__ __forceinline__ uint32_t take(uint32_t* work_counter, int block, uint32_t tile)
     {
         uint32_t base = next;
         if (!first)
         {
             const uint32_t waves = gridDim.x * ((uint32_t)block / 64u);
             if ((threadIdx.x & 63u) == 0)
-                base = atomicAdd(work_counter, 64u) + waves * 64u;
+                base = atomicAdd(work_counter, 64u * tile) + waves * 64u * tile;
             base = __shfl(base, 0, 64);
         }
         first = false;
@@ -220,7 +222,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
                                                  int32_t* __restrict__ spill, uint32_t spill_stride,
                                                  uint32_t* __restrict__ work_counter, uint32_t max_steps,
                                                  const uint32_t* __restrict__ queue,
-                                                 const uint32_t* __restrict__ queue_counts, uint32_t queue_seg_cap)
+                                                 const uint32_t* __restrict__ queue_counts, uint32_t queue_seg_cap,
+                                                 uint32_t tile)
 {
     __shared__ int32_t stack[STACK * BLOCK];
     const uint32_t lane = threadIdx.x & 63u;
@@ -232,28 +235,34 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
         n = min(n, map.total);
     }
     BatchCursor cur;
-    cur.init(BLOCK);
+    cur.init(BLOCK, tile);
     for (;;)
     {
-        const uint32_t base = cur.take(work_counter, BLOCK);
-        if (base >= n)
+        const uint32_t first = cur.take(work_counter, BLOCK, tile);
+        if (first >= n)
             break;
-        const uint32_t w = base + lane;
-        uint32_t r = w;
-        if (queue)
-            r = queue[map.slot(w < n ? w : n - 1u)]; // every lane takes part in the search
-        if (w >= n)
-            continue;
-        const RayHit h = traverse<BLOCK, STACK, MODE == CLOSEST_COUNTED>(
-            nodes, tris, root, rays[2 * (size_t)r], rays[2 * (size_t)r + 1], MODE == ANY, stack, spill, spill_stride,
-            gtid, max_steps);
-        if (MODE == ANY)
-            occluded[r] = h.prim >= 0 ? 1 : 0;
-        else
+        for (uint32_t k = 0; k < tile; k++)
         {
-            hits[r] = hit_record(h);
-            if (MODE == CLOSEST_COUNTED)
-                counts[r] = make_uint2(h.nodes, h.tris);
+            const uint32_t base = first + 64u * k;
+            if (base >= n)
+                break;
+            const uint32_t w = base + lane;
+            uint32_t r = w;
+            if (queue)
+                r = queue[map.slot(w < n ? w : n - 1u)]; // every lane takes part in the search
+            if (w >= n)
+                continue;
+            const RayHit h = traverse<BLOCK, STACK, MODE == CLOSEST_COUNTED>(
+                nodes, tris, root, rays[2 * (size_t)r], rays[2 * (size_t)r + 1], MODE == ANY, stack, spill,
+                spill_stride, gtid, max_steps);
+            if (MODE == ANY)
+                occluded[r] = h.prim >= 0 ? 1 : 0;
+            else
+            {
+                hits[r] = hit_record(h);
+                if (MODE == CLOSEST_COUNTED)
+                    counts[r] = make_uint2(h.nodes, h.tris);
+            }
         }
     }
 }
@@ -281,7 +290,8 @@ template <int BLOCK, int STACK>
 __global__ __launch_bounds__(BLOCK) void k_trace_queues(const BvhNode* __restrict__ nodes,
                                                         const TriRecord* __restrict__ tris, int32_t root, QueueJobs q,
                                                         int32_t* __restrict__ spill, uint32_t spill_stride,
-                                                        uint32_t* __restrict__ work_counter, uint32_t max_steps)
+                                                        uint32_t* __restrict__ work_counter, uint32_t max_steps,
+                                                        uint32_t tile)
 {
     __shared__ int32_t stack[STACK * BLOCK];
     // Segment starts of the three queues (the same for every wave): read
@@ -315,48 +325,54 @@ __global__ __launch_bounds__(BLOCK) void k_trace_queues(const BvhNode* __restric
     const uint32_t start2 = start1 + ((n1 + 63u) & ~63u);
     const uint32_t total = start2 + n2;
     BatchCursor cur;
-    cur.init(BLOCK);
+    cur.init(BLOCK, tile);
     for (;;)
     {
-        const uint32_t base = cur.take(work_counter, BLOCK);
-        if (base >= total)
+        const uint32_t first = cur.take(work_counter, BLOCK, tile);
+        if (first >= total)
             break;
-        // the wave's queue (uniform over the wave)
-        const int k = base < start1 ? 0 : (base < start2 ? 1 : 2);
-        const uint32_t local = base + lane - (k == 0 ? 0u : (k == 1 ? start1 : start2));
-        const uint32_t count = k == 0 ? n0 : (k == 1 ? n1 : n2);
-        if (local >= count)
-            continue;
-        // fields picked by selects: indexing the argument struct by a
-        // run-time value would copy it to scratch memory
-        const float4* rays = k == 0 ? q.job[0].rays : (k == 1 ? q.job[1].rays : q.job[2].rays);
-        const uint32_t* slots = k == 0 ? q.job[0].slots : (k == 1 ? q.job[1].slots : q.job[2].slots);
-        const uint32_t seg_cap = k == 0 ? q.job[0].seg_cap : (k == 1 ? q.job[1].seg_cap : q.job[2].seg_cap);
-        // last segment whose start is <= local
-        int lo = 0, hi = (int)SEG_COUNT - 1;
-        for (int step = 0; step < 6; step++)
+        for (uint32_t kt = 0; kt < tile; kt++)
         {
-            const int mid = (lo + hi + 1) >> 1;
-            if (seg_start[k][mid] <= local)
-                lo = mid;
+            const uint32_t base = first + 64u * kt;
+            if (base >= total)
+                break;
+            // the wave's queue (uniform over the wave)
+            const int k = base < start1 ? 0 : (base < start2 ? 1 : 2);
+            const uint32_t local = base + lane - (k == 0 ? 0u : (k == 1 ? start1 : start2));
+            const uint32_t count = k == 0 ? n0 : (k == 1 ? n1 : n2);
+            if (local >= count)
+                continue;
+            // fields picked by selects: indexing the argument struct by a
+            // run-time value would copy it to scratch memory
+            const float4* rays = k == 0 ? q.job[0].rays : (k == 1 ? q.job[1].rays : q.job[2].rays);
+            const uint32_t* slots = k == 0 ? q.job[0].slots : (k == 1 ? q.job[1].slots : q.job[2].slots);
+            const uint32_t seg_cap = k == 0 ? q.job[0].seg_cap : (k == 1 ? q.job[1].seg_cap : q.job[2].seg_cap);
+            // last segment whose start is <= local
+            int lo = 0, hi = (int)SEG_COUNT - 1;
+            for (int step = 0; step < 6; step++)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (seg_start[k][mid] <= local)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const uint32_t r = slots[(size_t)lo * seg_cap + (local - seg_start[k][lo])];
+            const float4 ra = rays[2 * (size_t)r], rb = rays[2 * (size_t)r + 1];
+            uint8_t* occluded = k == 0 ? q.job[0].occluded : (k == 1 ? q.job[1].occluded : q.job[2].occluded);
+            if (occluded)
+            {
+                const RayHit h = traverse<BLOCK, STACK, false>(nodes, tris, root, ra, rb, true, stack, spill,
+                                                              spill_stride, gtid, max_steps);
+                occluded[r] = h.prim >= 0 ? 1 : 0;
+            }
             else
-                hi = mid - 1;
-        }
-        const uint32_t r = slots[(size_t)lo * seg_cap + (local - seg_start[k][lo])];
-        const float4 ra = rays[2 * (size_t)r], rb = rays[2 * (size_t)r + 1];
-        uint8_t* occluded = k == 0 ? q.job[0].occluded : (k == 1 ? q.job[1].occluded : q.job[2].occluded);
-        if (occluded)
-        {
-            const RayHit h = traverse<BLOCK, STACK, false>(nodes, tris, root, ra, rb, true, stack, spill,
-                                                          spill_stride, gtid, max_steps);
-            occluded[r] = h.prim >= 0 ? 1 : 0;
-        }
-        else
-        {
-            const RayHit h = traverse<BLOCK, STACK, false>(nodes, tris, root, ra, rb, false, stack, spill,
-                                                          spill_stride, gtid, max_steps);
-            int4* hits = k == 0 ? q.job[0].hits : (k == 1 ? q.job[1].hits : q.job[2].hits);
-            hits[r] = hit_record(h);
+            {
+                const RayHit h = traverse<BLOCK, STACK, false>(nodes, tris, root, ra, rb, false, stack, spill,
+                                                              spill_stride, gtid, max_steps);
+                int4* hits = k == 0 ? q.job[0].hits : (k == 1 ? q.job[1].hits : q.job[2].hits);
+                hits[r] = hit_record(h);
+            }
         }
     }
 }
@@ -373,7 +389,8 @@ __global__ void k_all_miss(int4* __restrict__ hits, uint8_t* __restrict__ occlud
 }
 
 using TraceFn = void (*)(const BvhNode*, const TriRecord*, int32_t, const float4*, int4*, uint8_t*, uint2*,
-                         uint32_t, int32_t*, uint32_t, uint32_t*, uint32_t, const uint32_t*, const uint32_t*, uint32_t);
+                         uint32_t, int32_t*, uint32_t, uint32_t*, uint32_t, const uint32_t*, const uint32_t*, uint32_t,
+                         uint32_t);
 
 template <int BLOCK, int STACK>
 TraceFn pick_mode(int mode)
@@ -413,7 +430,8 @@ struct Launch {
     unsigned grid;
 };
 
-using QueuesFn = void (*)(const BvhNode*, const TriRecord*, int32_t, QueueJobs, int32_t*, uint32_t, uint32_t*, uint32_t);
+using QueuesFn = void (*)(const BvhNode*, const TriRecord*, int32_t, QueueJobs, int32_t*, uint32_t, uint32_t*, uint32_t,
+                          uint32_t);
 
 template <int BLOCK>
 QueuesFn pick_queues_stack(int stack)
@@ -528,7 +546,8 @@ void launch(Ctx& ctx, int mode, const mipt_ray* rays, mipt_hit* hits, uint8_t* o
                        ctx.bvh.tris.as<TriRecord>(), ctx.bvh.root, reinterpret_cast<const float4*>(rays),
                        reinterpret_cast<int4*>(hits), occluded, reinterpret_cast<uint2*>(counts), n, sc.spill, threads,
                        sc.counter, (uint32_t)ctx.bvh.node_count + 1u, queue ? queue->slots : nullptr,
-                       queue ? queue->counts : nullptr, queue ? queue->seg_cap : 0u);
+                       queue ? queue->counts : nullptr, queue ? queue->seg_cap : 0u,
+                       (uint32_t)ctx.trace_cfg.queue_tile);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -612,8 +631,83 @@ void trace_closest_noalloc(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint3
     hipLaunchKernelGGL(l.fn, dim3(grid), dim3(block), 0, stream, ctx.bvh.nodes.as<BvhNode>(),
                        ctx.bvh.tris.as<TriRecord>(), ctx.bvh.root, reinterpret_cast<const float4*>(rays),
                        reinterpret_cast<int4*>(hits), nullptr, nullptr, n, spill, threads, counter,
-                       (uint32_t)ctx.bvh.node_count + 1u, nullptr, nullptr, 0u);
+                       (uint32_t)ctx.bvh.node_count + 1u, nullptr, nullptr, 0u, (uint32_t)ctx.trace_cfg.queue_tile);
     HIP_CHECK(hipGetLastError());
+}
+
+void trace_reserve(Ctx& ctx)
+{
+    if (ctx.bvh.valid && ctx.bvh.triangle_count > 0)
+        (void)trace_scratch(ctx, 0u);
+}
+
+void autotune_trace(Ctx& ctx, const mipt_ray* rays, uint32_t n, int32_t repeats, hipStream_t stream,
+                    mipt_trace_tuning* out, double* table)
+{
+    static const int blocks[3] = {64, 128, 256};
+    static const int stacks[4] = {8, 16, 24, 32};
+    static const int tiles[3] = {1, 2, 4};
+    if (!trace_ready(ctx))
+        fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_autotune_trace: the scene has no triangles");
+    const TraceConfig keep = ctx.trace_cfg;
+    ctx.autotune_hits.reserve(sizeof(mipt_hit) * (size_t)n);
+    mipt_hit* hits = ctx.autotune_hits.as<mipt_hit>();
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    double best = 0.0, at_default = 0.0;
+    TraceConfig best_cfg = keep;
+    int tried = 0;
+    try
+    {
+        for (int bi = 0; bi < 3; bi++)
+            for (int si = 0; si < 4; si++)
+                for (int ti = 0; ti < 3; ti++)
+                {
+                    ctx.trace_cfg.block_size = blocks[bi];
+                    ctx.trace_cfg.lds_stack = stacks[si];
+                    ctx.trace_cfg.queue_tile = tiles[ti];
+                    // one untimed launch: occupancy query, spill columns
+                    launch(ctx, CLOSEST, rays, hits, nullptr, nullptr, n, nullptr, stream);
+                    HIP_CHECK(hipEventRecord(e0, stream));
+                    for (int r = 0; r < repeats; r++)
+                        launch(ctx, CLOSEST, rays, hits, nullptr, nullptr, n, nullptr, stream);
+                    HIP_CHECK(hipEventRecord(e1, stream));
+                    HIP_CHECK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    const double mean = (double)ms / repeats;
+                    if (table)
+                        table[(bi * 4 + si) * 3 + ti] = mean;
+                    if (blocks[bi] == 128 && stacks[si] == 16 && tiles[ti] == 1)
+                        at_default = mean;
+                    if (tried == 0 || mean < best)
+                    {
+                        best = mean;
+                        best_cfg = ctx.trace_cfg;
+                    }
+                    tried++;
+                }
+    }
+    catch (...)
+    {
+        ctx.trace_cfg = keep;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    ctx.trace_cfg = best_cfg;
+    if (out)
+    {
+        out->block_size = best_cfg.block_size;
+        out->lds_stack = best_cfg.lds_stack;
+        out->queue_tile = best_cfg.queue_tile;
+        out->configurations = tried;
+        out->best_ms = best;
+        out->default_ms = at_default;
+    }
 }
 
 void trace_closest(Ctx& ctx, const mipt_ray* rays, mipt_hit* hits, uint32_t* counts, uint32_t n, hipStream_t stream)
@@ -673,7 +767,7 @@ void trace_queues(Ctx& ctx, const TraceQueue* q, int count, uint32_t capacity, h
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, sc.counter);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, stream, ctx.bvh.nodes.as<BvhNode>(),
                        ctx.bvh.tris.as<TriRecord>(), ctx.bvh.root, jobs, sc.spill, threads, sc.counter,
-                       (uint32_t)ctx.bvh.node_count + 1u);
+                       (uint32_t)ctx.bvh.node_count + 1u, (uint32_t)ctx.trace_cfg.queue_tile);
     HIP_CHECK(hipGetLastError());
 }
 

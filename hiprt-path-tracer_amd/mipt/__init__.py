@@ -91,6 +91,19 @@ class SceneDesc(C.Structure):
     ]
 
 
+class View(C.Structure):
+    """mipt_view: one camera view of a batch."""
+
+    _fields_ = [("camera", Camera), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class TraceTuning(C.Structure):
+    """mipt_trace_tuning: the autotuner's choice."""
+
+    _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("queue_tile", C.c_int32),
+                ("configurations", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
 class BvhInfo(C.Structure):
     _fields_ = [
         ("builder", C.c_int32), ("triangle_count", C.c_int32), ("node_count", C.c_int32),
@@ -187,6 +200,13 @@ _SIGNATURES = {
                                              C.c_void_p]),
     "mipt_set_trace_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "mipt_set_shading_order": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mipt_set_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "mipt_views_generate": (C.c_int, [C.c_int32, C.c_void_p]),
+    "mipt_set_graph_capture": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mipt_graph_replays": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "mipt_set_queue_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mipt_autotune_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     "mipt_update_materials": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "mipt_update_material": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mipt_camera_rays": (C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
@@ -361,6 +381,15 @@ class Scene:
         return d
 
 
+def generate_views(count: int):
+    """The seeded view batch of configuration 3 (mipt_views_generate)."""
+    views = (View * count)()
+    rc = load_library().mipt_views_generate(count, C.cast(views, C.c_void_p))
+    if rc != MIPT_OK:
+        raise MiptError(rc, "mipt_views_generate")
+    return views
+
+
 def generate_scene(config_id: int, width: int, height: int) -> Scene:
     """Procedural scene of configuration 1 or 2, generated by libmipt."""
     lib = load_library()
@@ -460,6 +489,38 @@ class Context:
         """GPURenderer::update_one_material."""
         self._check(self.lib.mipt_update_material(self.h, index, C.byref(material)), "mipt_update_material")
 
+    def set_views(self, views):
+        """mipt_set_views: a ctypes array of View (or an empty list for one view)."""
+        if len(views) == 0:
+            self._check(self.lib.mipt_set_views(self.h, None, 0), "mipt_set_views")
+            self._views_total = 0
+        else:
+            self._check(self.lib.mipt_set_views(self.h, C.cast(views, C.c_void_p), len(views)), "mipt_set_views")
+            self._views_total = sum(v.width * v.height for v in views)
+            if getattr(self, "settings", None) is None:
+                self.settings = RenderSettings()
+                self.lib.mipt_render_settings_default(C.byref(self.settings))
+            self.settings.width, self.settings.height = self._views_total, 1
+
+    def set_graph_capture(self, enable: bool):
+        self._check(self.lib.mipt_set_graph_capture(self.h, int(enable)), "mipt_set_graph_capture")
+
+    def graph_replays(self) -> int:
+        out = C.c_uint32()
+        self._check(self.lib.mipt_graph_replays(self.h, C.byref(out)), "mipt_graph_replays")
+        return int(out.value)
+
+    def set_queue_tile(self, tile: int):
+        self._check(self.lib.mipt_set_queue_tile(self.h, tile), "mipt_set_queue_tile")
+
+    def autotune_trace(self, d_rays, n: int, repeats: int = 3, stream=None):
+        """(TraceTuning, table[3 blocks][4 stacks][3 tiles] of mean ms)."""
+        out = TraceTuning()
+        table = np.zeros((3, 4, 3), np.float64)
+        self._check(self.lib.mipt_autotune_trace(self.h, ptr(d_rays), n, repeats, stream, C.byref(out),
+                                                 table.ctypes.data_as(C.c_void_p)), "mipt_autotune_trace")
+        return out, table
+
     def set_shading_order(self, order: int):
         """MIPT_SHADE_ORDER_LIVE / _MATERIAL / _AUTO (the default)."""
         self._check(self.lib.mipt_set_shading_order(self.h, order), "mipt_set_shading_order")
@@ -512,8 +573,12 @@ class Context:
 
     # Path-tracing passes
     def set_render_settings(self, settings: RenderSettings):
-        self.settings = settings
         self._check(self.lib.mipt_set_render_settings(self.h, C.byref(settings)), "mipt_set_render_settings")
+        self.settings = RenderSettings()
+        C.memmove(C.byref(self.settings), C.byref(settings), C.sizeof(RenderSettings))
+        if getattr(self, "_views_total", 0):
+            # a batch of views: every view's pixels back to back
+            self.settings.width, self.settings.height = self._views_total, 1
 
     def set_camera(self, cam: Camera):
         self._check(self.lib.mipt_set_camera(self.h, C.byref(cam)), "mipt_set_camera")

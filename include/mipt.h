@@ -257,9 +257,69 @@ int mipt_trace_closest_counted(mipt_ctx* ctx, const mipt_ray* d_rays, mipt_hit* 
 #define MIPT_SHADE_ORDER_AUTO 2
 int mipt_set_shading_order(mipt_ctx* ctx, int32_t order);
 
+/* ------------------------------------------------------------------ */
+/* Batched camera views (many views of uneven resolution rendered as    */
+/* one wavefront: each pass kernel and each traversal launch covers the */
+/* pixels of every view). No counterpart in the reference, which        */
+/* renders one camera.                                                  */
+/* ------------------------------------------------------------------ */
+typedef struct mipt_view {
+    mipt_camera camera; /* HIPRTCamera of the view (sensor size unused) */
+    int32_t width, height;
+} mipt_view;
+
+/* Switches the context to `count` views: the G-buffer, path state and
+ * framebuffer hold the pixels of every view back to back (view v's pixel
+ * x + y * width_v at the offset of the views before it), so
+ * mipt_read_framebuffer returns them in that order. Each pixel draws the
+ * random numbers a render of its view alone would (the seed uses the
+ * pixel's index in its view), so a batch equals its views rendered one by
+ * one, bit for bit. The render settings' width and height are replaced by
+ * (total pixels, 1); the plain path tracer only (MIS light + BSDF, no
+ * ReSTIR DI, no radiance cache). count = 0 returns to one view (call
+ * mipt_set_render_settings again). */
+int mipt_set_views(mipt_ctx* ctx, const mipt_view* views, int32_t count);
+
+/* Seeded view batch (configuration 3): `count` views of the config-1 scene
+ * with heights log-uniform in [128, 1024] px, aspect (width / height)
+ * uniform in [1, 16/9], cameras inside the box looking at its back wall
+ * (PCG32, state 0x2026120200000000 + 3, stream 3). */
+int mipt_views_generate(int32_t count, mipt_view* out);
+
+/* Graph replay of plain samples: with enable = 1, the first
+ * mipt_render_sample with sample_number > 0 after any change of scene,
+ * settings, camera, views or launch configuration records the sample's
+ * kernels (camera pass + path pass) into a hipGraph; later samples replay it,
+ * the per-sample values (sample number, host random numbers, consistent
+ * tau) written to device memory by one small kernel before each replay.
+ * Results equal the eager launches bit for bit. Sample 0 always runs
+ * eagerly (it resets the accumulation). Plain path tracing only. */
+int mipt_set_graph_capture(mipt_ctx* ctx, int32_t enable);
+/* Replays since the last capture (0 when the last sample ran eagerly). */
+int mipt_graph_replays(const mipt_ctx* ctx, uint32_t* out);
+
 /* Traversal launch configuration (the autotuned knobs). block_size in
  * {64,128,256}; lds_stack in {8,16,24,32}. 0 keeps the current value. */
 int mipt_set_trace_config(mipt_ctx* ctx, int32_t block_size, int32_t lds_stack);
+/* Queue tile: rays a wave of the persistent traversal takes per
+ * work-counter atomic, in batches of 64 (1, 2 or 4). */
+int mipt_set_queue_tile(mipt_ctx* ctx, int32_t tile);
+
+/* Autotuner of the traversal launch (no counterpart in the reference,
+ * whose HIPRT kernels are compiled with fixed block and stack sizes,
+ * MegaKernelRenderPass.cpp:22-23): times `repeats` closest-hit launches
+ * over the caller's n device rays for every block size (64, 128, 256), LDS
+ * stack depth (8, 16, 24, 32) and queue tile (1, 2, 4), and sets the
+ * fastest on the context. table_ms (36 doubles, [block][stack][tile], may be
+ * NULL) receives every mean launch time. Synchronous. */
+typedef struct mipt_trace_tuning {
+    int32_t block_size, lds_stack, queue_tile;
+    int32_t configurations; /* timed */
+    double best_ms;         /* mean launch time of the chosen configuration */
+    double default_ms;      /* of 128 threads, 16 LDS entries, tile 1 */
+} mipt_trace_tuning;
+int mipt_autotune_trace(mipt_ctx* ctx, const mipt_ray* d_rays, uint32_t n, int32_t repeats, void* stream,
+                        mipt_trace_tuning* out, double* table_ms);
 
 /* ------------------------------------------------------------------ */
 /* Camera rays (the ray-generation half of CameraRays,                  */
