@@ -224,6 +224,10 @@ def main():
     if with_restir:
         step("ReSTIR DI leg: config 2 scene, 1920x1080, 5 bounces, diffuse + metal lobes")
     restir = restir_render_leg(mipt, torch, np, layered=False) if with_restir else None
+    restir_fused = None
+    if with_restir:
+        step("ReSTIR DI leg: the same with light presampling and the fused spatiotemporal pass")
+        restir_fused = restir_render_leg(mipt, torch, np, layered=False, fused=True)
     if with_restir and args.restir_layered:
         step("ReSTIR DI leg: config 2 scene, 1920x1080, 5 bounces, full layered materials")
         restir_layered = restir_render_leg(mipt, torch, np, layered=True)
@@ -280,6 +284,7 @@ def main():
         "plain_render": render,
         "restir_render": restir,
         "restir_render_layered": restir_layered,
+        "restir_render_fused": restir_fused,
         "views_render": views,
         "cache_vs_plain": cache,
         "cache_vs_plain_tuned": cache_tuned,
@@ -471,16 +476,18 @@ def cache_vs_plain_leg(mipt, torch, np, ref, ref_desc, plain_spp, warmup, cache_
             "curve_cache": [[round(t, 4), s, e] for t, s, e in cached]}
 
 
-def restir_render_leg(mipt, torch, np, layered):
+def restir_render_leg(mipt, torch, np, layered, fused=False):
     """ReSTIR DI at scale: the 1,040,384-triangle scene at 1920x1080 with 4096
     power-sampled emitters; 32 light + 1 BSDF initial candidates, temporal
     reuse (M-cap 20), 2 spatial passes of 5 neighbours in 16 px, pairwise
     MIS, 5 bounces. With layered=False the materials keep their diffuse and
     metal lobes only (coat, sheen, specular, transmission and thin film set
     to 0); with layered=True they are the scene's full layered Principled
-    materials. Times whole frames (camera + ReSTIR DI passes + path pass)
-    and each part, with HIP events on the launch stream, after 2 warm-up
-    frames."""
+    materials. With fused=True the temporal pass and the first spatial pass
+    are the one fused spatiotemporal pass, fed by the default light
+    presampling (128 subsets of 1024 lights, 8x8 pixel tiles per subset).
+    Times whole frames (camera + ReSTIR DI passes + path pass) and each
+    part, with HIP events on the launch stream, after 2 warm-up frames."""
     W, H = 1920, 1080
     scene = mipt.generate_scene(2, W, H)
     if not layered:
@@ -489,7 +496,8 @@ def restir_render_leg(mipt, torch, np, layered):
     s = mipt.default_render_settings(width=W, height=H, nb_bounces=5,
                                      direct_light_sampling_strategy=mipt.MIPT_LSS_RESTIR_DI,
                                      direct_light_sampling_base_strategy=mipt.MIPT_LSS_BASE_POWER)
-    rs = mipt.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20, spatial_number_of_passes=2)
+    rs = mipt.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20, spatial_number_of_passes=2,
+                                         do_fused_spatiotemporal=int(fused), do_light_presampling=int(fused))
     ctx = mipt.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
@@ -512,10 +520,19 @@ def restir_render_leg(mipt, torch, np, layered):
         e[0].record(stream)
         ctx.launch_pass(mipt.MIPT_PASS_CAMERA_RAYS, f, mipt.host_random_number(k), sh)
         e[1].record(stream)
+        # the order and random numbers of mipt_render_sample
+        if fused:
+            ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING, f, mipt.host_random_number(k), sh)
         ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, f, mipt.host_random_number(k + 1), sh)
-        ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, f, mipt.host_random_number(k + 2), sh)
-        for p in range(rs.spatial_number_of_passes):
-            ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(p), f, mipt.host_random_number(k + 4 + p), sh)
+        if fused:
+            ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE, f, mipt.host_random_number(k + 4), sh)
+            first_spatial = 1
+        else:
+            ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, f, mipt.host_random_number(k + 2), sh)
+            first_spatial = 0
+        for p in range(first_spatial, rs.spatial_number_of_passes):
+            ctx.launch_pass(mipt.MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(p), f,
+                            mipt.host_random_number(k + 4 + p - first_spatial + (1 if fused else 0)), sh)
         e[2].record(stream)
         ctx.launch_pass(mipt.MIPT_PASS_PATH_TRACE, f, mipt.host_random_number(k + draws - 1), sh)
         e[3].record(stream)
@@ -530,7 +547,9 @@ def restir_render_leg(mipt, torch, np, layered):
     fb = ctx.read_framebuffer()
     ctx.close()
     return {"workload": "config 2 scene (1,040,384 tris, 4096 emitters) 1920x1080, ReSTIR DI 32+1 candidates, "
-                        "temporal M-cap 20, 2 spatial passes x 5 neighbours r=16, pairwise MIS, 5 bounces; "
+                        + ("light presampling 128 x 1024, fused spatiotemporal pass + 1 spatial pass"
+                           if fused else "temporal M-cap 20, 2 spatial passes")
+                        + " x 5 neighbours r=16, pairwise MIS, 5 bounces; "
                         + ("full layered Principled materials" if layered else "diffuse + metal lobes"),
             "frames": 2 + frames, "ms_per_frame": round(frame_ms, 3), "camera_pass_ms": round(cam_ms, 3),
             "restir_di_passes_ms": round(restir_ms, 3), "path_pass_ms": round(path_ms, 3),

@@ -237,6 +237,7 @@ void allocate_render_buffers(Ctx& ctx)
     R.last_output = 1;
     R.restir_shade_buffer = 1;
     R.temporal_output = 0;
+    R.fused_output = 1;
     HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -773,6 +774,12 @@ void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
     s->plane_distance_threshold = 0.1f;
     s->reject_using_geometric_normals = 1;
     s->do_final_shading_visibility = 1;
+    s->do_fused_spatiotemporal = 0;
+    s->do_light_presampling = 0;
+    // ReSTIRDILightPresamplingSettings (ReSTIRDISettings.h:45-56)
+    s->light_presampling_number_of_subsets = 128;
+    s->light_presampling_subset_size = 1024;
+    s->light_presampling_tile_size = 8;
 }
 
 int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
@@ -782,8 +789,18 @@ int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
         ctx->render.graph_dirty = true; // a captured sample would replay stale state
     if (!ctx || !s || s->number_of_initial_light_candidates < 0 || s->number_of_initial_bsdf_candidates < 0 ||
         s->spatial_number_of_passes < 1 || s->spatial_reuse_radius < 0 || s->spatial_reuse_neighbor_count < 0 ||
-        s->spatial_reuse_neighbor_count > 32 || s->m_cap < 0)
+        s->spatial_reuse_neighbor_count > 32 || s->m_cap < 0 ||
+        (s->do_light_presampling &&
+         (s->light_presampling_number_of_subsets < 1 || s->light_presampling_subset_size < 1 ||
+          s->light_presampling_tile_size < 1 ||
+          (int64_t)s->light_presampling_number_of_subsets * s->light_presampling_subset_size > (1ll << 24))))
         return MIPT_ERR_INVALID_ARGUMENT;
+    if (s->do_fused_spatiotemporal && !s->do_temporal_reuse_pass)
+    {
+        ctx->last_error = "the fused spatiotemporal pass needs do_temporal_reuse_pass (it reads last frame's "
+                          "reservoirs and G-buffer)";
+        return MIPT_ERR_UNSUPPORTED;
+    }
     return mipt::guarded(ctx, [&] {
         ctx->render.restir = *s;
         if (ctx->render.configured)
@@ -933,7 +950,9 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
     const bool spatial = pass >= MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0) && pass < MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(64);
     if (!ctx || (pass != MIPT_PASS_CAMERA_RAYS && pass != MIPT_PASS_PATH_TRACE &&
                  pass != MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES && pass != MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE &&
-                 pass != MIPT_PASS_RADIANCE_CACHE_TRAIN && pass != MIPT_PASS_RADIANCE_CACHE_WARMUP && !spatial))
+                 pass != MIPT_PASS_RADIANCE_CACHE_TRAIN && pass != MIPT_PASS_RADIANCE_CACHE_WARMUP &&
+                 pass != MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING && pass != MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE &&
+                 !spatial))
         return MIPT_ERR_INVALID_ARGUMENT;
     return mipt::guarded(ctx, [&] {
         if (!ctx->render.configured)
@@ -957,6 +976,10 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
             mipt::restir_initial_pass(*ctx, sample_number, random_number, s);
         else if (pass == MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE)
             mipt::restir_temporal_pass(*ctx, sample_number, random_number, s);
+        else if (pass == MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING)
+            mipt::restir_presampling_pass(*ctx, sample_number, random_number, s);
+        else if (pass == MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE)
+            mipt::restir_spatiotemporal_pass(*ctx, sample_number, random_number, s);
         else if (pass == MIPT_PASS_RADIANCE_CACHE_TRAIN)
             mipt::nrc_train_pass(*ctx, s);
         else if (pass == MIPT_PASS_RADIANCE_CACHE_WARMUP)
@@ -977,10 +1000,15 @@ int32_t mipt_draws_per_sample(const mipt_ctx* c)
     if (R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
     {
         draws += 1;
-        if (R.restir.do_temporal_reuse_pass)
-            draws += 2;
-        if (R.restir.do_spatial_reuse_pass)
-            draws += R.restir.spatial_number_of_passes;
+        if (R.restir.do_fused_spatiotemporal)
+            draws += 3 + (R.restir.spatial_number_of_passes - 1);
+        else
+        {
+            if (R.restir.do_temporal_reuse_pass)
+                draws += 2;
+            if (R.restir.do_spatial_reuse_pass)
+                draws += R.restir.spatial_number_of_passes;
+        }
     }
     return draws;
 }
@@ -1011,19 +1039,39 @@ int render_sample_passes(mipt_ctx* c, uint32_t sample_number, void* stream, bool
     const mipt::RenderState& R = ctx->render;
     if (rc == MIPT_OK && R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
     {
-        rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, sample_number, mipt_host_random_number(k++),
-                              stream);
-        if (rc == MIPT_OK && R.restir.do_temporal_reuse_pass)
-        {
-            // the second draw is the permutation-sampling bits, unused here
-            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, sample_number, mipt_host_random_number(k),
-                                  stream);
-            k += 2;
-        }
-        for (int32_t i = 0; rc == MIPT_OK && R.restir.do_spatial_reuse_pass && i < R.restir.spatial_number_of_passes;
-             i++)
-            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i), sample_number,
+        // ReSTIRDIRenderPass::launch_async (ReSTIRDIRenderPass.cpp:343-381):
+        // the presampling kernel reads the camera pass's random number
+        if (R.restir.do_light_presampling)
+            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING, sample_number,
+                                  mipt_host_random_number(k - 1), stream);
+        if (rc == MIPT_OK)
+            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, sample_number,
                                   mipt_host_random_number(k++), stream);
+        if (rc == MIPT_OK && R.restir.do_fused_spatiotemporal)
+        {
+            // the temporal pass's number and permutation bits are drawn, then
+            // the fused kernel's (configure_spatiotemporal_pass)
+            k += 2;
+            rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE, sample_number,
+                                  mipt_host_random_number(k++), stream);
+            for (int32_t i = 1; rc == MIPT_OK && i < R.restir.spatial_number_of_passes; i++)
+                rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i), sample_number,
+                                      mipt_host_random_number(k++), stream);
+        }
+        else
+        {
+            if (rc == MIPT_OK && R.restir.do_temporal_reuse_pass)
+            {
+                // the second draw is the permutation-sampling bits, unused here
+                rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, sample_number,
+                                      mipt_host_random_number(k), stream);
+                k += 2;
+            }
+            for (int32_t i = 0;
+                 rc == MIPT_OK && R.restir.do_spatial_reuse_pass && i < R.restir.spatial_number_of_passes; i++)
+                rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i), sample_number,
+                                      mipt_host_random_number(k++), stream);
+        }
     }
     if (rc != MIPT_OK)
         return rc;

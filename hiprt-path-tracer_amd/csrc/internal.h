@@ -149,9 +149,12 @@ struct RenderState {
     int32_t last_output = 1;      // m_last_restir_output_reservoirs (ReSTIRDIRenderPass.cpp:251)
     int32_t restir_shade_buffer = 1; // restir_output_reservoirs of the current frame
     int32_t temporal_output = 0;  // buffer the temporal pass of this frame wrote
+    int32_t fused_output = 1;     // buffer the fused pass (or the spatial pass after it) wrote
     bool odd_frame = false;
     // ReSTIR DI pass scratch
     DeviceBuffer rs_state, rs_rays, rs_occ, rs_hits, rs_aux;
+    DeviceBuffer presampled;      // presampled lights (light presampling)
+    bool presampled_ready = false; // the presampling pass has filled `presampled` since it was allocated
 };
 
 // Learned radiance cache state (nrc.hip).
@@ -243,6 +246,8 @@ void restir_initial_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_numbe
 void restir_temporal_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
 void restir_spatial_pass(Ctx& ctx, int32_t pass_index, uint32_t sample_number, uint32_t random_number,
                          hipStream_t stream);
+void restir_presampling_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
+void restir_spatiotemporal_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t stream);
 
 // bvh_build.hip
 void build_lbvh(Ctx& ctx);

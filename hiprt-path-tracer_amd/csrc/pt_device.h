@@ -317,10 +317,10 @@ struct LightSample {
 // sample_point_on_generic_triangle (:163-199, Heitz 2019 square-to-triangle
 // :142-156) and DeviceAliasTable::sample (AliasTable.h:13-22). Requires
 // emissive_count > 0.
-__device__ __forceinline__ LightSample sample_emissive(const PtData& d, pt::Xorshift32& rng)
+__device__ __forceinline__ LightSample sample_emissive_with(const PtData& d, int32_t strategy, pt::Xorshift32& rng)
 {
     int32_t e = rng.random_index(d.emissive_count);
-    if (d.light_base == MIPT_LSS_BASE_POWER)
+    if (strategy == MIPT_LSS_BASE_POWER)
     {
         const float p = d.alias_probas[e];
         if (rng() > p)
@@ -350,11 +350,17 @@ __device__ __forceinline__ LightSample sample_emissive(const PtData& d, pt::Xors
     s.area = 0.5f * len;
     s.emission = d.mats[d.mat_idx[s.tri]].emission;
     s.area_pdf = 1.0f / s.area;
-    if (d.light_base == MIPT_LSS_BASE_POWER)
+    if (strategy == MIPT_LSS_BASE_POWER)
         s.area_pdf *= (luminance(s.emission) * s.area) / d.alias_sum;
     else
         s.area_pdf /= (float)d.emissive_count;
     return s;
+}
+
+// With the render settings' base strategy.
+__device__ __forceinline__ LightSample sample_emissive(const PtData& d, pt::Xorshift32& rng)
+{
+    return sample_emissive_with(d, d.light_base, rng);
 }
 
 // pdf_of_emissive_triangle_hit_area_measure (LightUtils.h:51-76)

@@ -28,6 +28,8 @@
 // float4 (M bits, emissive_triangle_index bits, UCW, flags bits).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pt_device.h"
 
 namespace mipt {
@@ -83,6 +85,7 @@ __device__ __forceinline__ void res_store(float4* buf, size_t pix, const Res& r)
 
 // Scratch state of a pixel between kernels: 4 float4.
 struct RsData {
+    float4* presampled; // presampled lights: 3 float4 each (point, tri), (normal, pdf), (radiance, flags)
     float4* state;      // 4 per pixel
     float4* rays;       // slots of 2 float4, slot k of pixel p at k * n + p
     const uint8_t* occ; // per slot
@@ -281,6 +284,54 @@ __device__ __forceinline__ bool is_emissive(const PtData& d, int32_t mat)
 }
 
 // ---------------------------------------------------------------------------
+// LIGHT PRESAMPLING (LightsPresampling.h:74-112) and its use by the initial
+// candidates
+// ---------------------------------------------------------------------------
+// cantor_pairing_function (InitialCandidates.h:28-31)
+__device__ __forceinline__ int32_t cantor_pair(int32_t x, int32_t y) { return (x + y + 1) * (x + y) / 2 + y; }
+
+// Index of the presampled light that light candidate draws for pixel `pix`:
+// subset from the tile's stream (seeded random_number * (tile + 1)), light of
+// the subset from the pixel's stream (one draw).
+__device__ __forceinline__ uint32_t presampled_index(const PtData& d, const RsData& r, uint32_t pix, pt::Xorshift32& rng)
+{
+    const int32_t ts = r.s.light_presampling_tile_size;
+    const int32_t x = (int32_t)(pix % (uint32_t)d.width), y = (int32_t)(pix / (uint32_t)d.width);
+    const int32_t tile = cantor_pair(x / ts, y / ts);
+    pt::Xorshift32 subset_rng{d.random_number * (uint32_t)(tile + 1)};
+    const int32_t subset = subset_rng.random_index(r.s.light_presampling_number_of_subsets);
+    const int32_t in_subset = rng.random_index(r.s.light_presampling_subset_size);
+    return (uint32_t)subset * (uint32_t)r.s.light_presampling_subset_size + (uint32_t)in_subset;
+}
+
+// One presampled light per thread: an envmap draw (probability 0 here), then
+// an emissive triangle by power (ReSTIR_DI_LightPresamplingStrategy,
+// ReSTIRDIOptions.h:162). A zero-pdf sample keeps the empty record.
+__global__ void k_presample_lights(PtData d, float4* __restrict__ out, uint32_t count)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count)
+        return;
+    pt::Xorshift32 rng{pt::pixel_seed(x, d.sample_number, d.random_number, d.freeze != 0)};
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 c = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+    if (!(rng() < 0.0f))
+    {
+        const LightSample ls = sample_emissive_with(d, MIPT_LSS_BASE_POWER, rng);
+        if (ls.area_pdf > 0.0f)
+        {
+            a = st3(ls.point, __int_as_float(ls.tri));
+            b = st3(ls.normal, ls.area_pdf * 1.0f);
+            c = stc(ls.emission, __int_as_float(0));
+        }
+    }
+    out[3 * (size_t)x] = a;
+    out[3 * (size_t)x + 1] = b;
+    out[3 * (size_t)x + 2] = c;
+}
+
+// ---------------------------------------------------------------------------
 // INITIAL CANDIDATES
 // ---------------------------------------------------------------------------
 
@@ -357,8 +408,24 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
         f3 ls_normal = pt::mk(0.0f, 0.0f, 0.0f);
         Color ls_emission = pt::col(0.0f);
         int32_t tri = -1;
+        if (r.presampled)
+        {
+            // use_presampled_light_candidate (InitialCandidates.h:33-90): the
+            // pixel's tile picks a subset with its own stream, the pixel a
+            // light of the subset
+            const float4* pl = r.presampled + 3 * (size_t)presampled_index(d, r, pix, rng);
+            point = ld3(pl[0]);
+            tri = __float_as_int(pl[0].w);
+            ls_normal = ld3(pl[1]);
+            pdf = pl[1].w;
+            ls_emission = ldc(pl[2]);
+            const f3 to = point - s.p;
+            dist = sqrtf(pt::dot(to, to));
+            dir = pt::vdiv(to, dist);
+            cosine = pt::dot(s.sn, dir);
+        }
         // sample_fresh_light_candidate (:99-178): envmap candidate probability 0
-        if (rng() > 0.0f)
+        else if (rng() > 0.0f)
         {
             const LightSample ls = sample_emissive(d, rng);
             tri = ls.tri;
@@ -827,6 +894,297 @@ __global__ void k_sp_finish(RsData r)
     res_store(r.out, pix, out);
 }
 
+// ---------------------------------------------------------------------------
+// FUSED SPATIOTEMPORAL REUSE (FusedSpatiotemporalReuse.h:152-566), static
+// camera: the temporal neighbour is the pixel itself
+// (find_temporal_neighbor_index, UtilsTemporal.h:37-41) and the spatial
+// neighbours are drawn around it. Temporal neighbour and spatial neighbours
+// come from last frame's output reservoirs; neighbour surfaces and the
+// similarity tests read the previous frame's G-buffer
+// (use_prev_frame_g_buffer(), RenderSettings.h:263-272). Pairwise MIS
+// (SpatiotemporalMISWeight.h:139-224) with the temporal neighbour resampled
+// first, the initial candidates last.
+//
+// Three kernels around two ray launches: k_fst_start evaluates every target
+// function that needs visibility (the temporal sample at the centre; the
+// centre's initial sample at the temporal and at each spatial neighbour's
+// surface) and writes their shadow rays; k_fst_combine applies the
+// visibility and resamples in the reference's order, then writes the
+// visibility-reuse ray; k_fst_finish kills or keeps the sample.
+// Per-pixel aux layout (stride fst_aux_stride(K)): [0] temporal target at the
+// centre (unoccluded), [1] neighbour seed, [2] valid count, [3] valid M sum,
+// [4] heuristics cache, [5] ray mask (bit 0 centre, bit 1 temporal neighbour),
+// [6] -, [7] -, then (target, confidence multiplier) for the temporal
+// neighbour and for each spatial neighbour (multiplier < 0: no entry).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t fst_aux_stride(int32_t K) { return 8 + 2 * (size_t)(K + 1); }
+
+// The fused pass seeds with wang_hash(((pix + 1) * (sample + 1)) ^ random)
+// (FusedSpatiotemporalReuse.h:173), an XOR where the other passes multiply.
+__device__ __forceinline__ uint32_t fused_seed(uint32_t pix, uint32_t sample_number, uint32_t random_number, bool freeze)
+{
+    return freeze ? pt::wang_hash(pix + 1u) : pt::wang_hash(((pix + 1u) * (sample_number + 1u)) ^ random_number);
+}
+
+// check_neighbor_similarity_heuristics(..., previous_frame = true)
+// (NeighborSimilarity.h:52-107): primary hit and position of the previous
+// frame's G-buffer, the neighbour's normal from the current one.
+__device__ __forceinline__ bool similar_prev(const PtData& d, const RsData& r, int32_t npix, uint32_t cpix, f3 cpoint,
+                                             f3 cnormal)
+{
+    if ((uint32_t)npix == cpix)
+        return true;
+    if (r.pg_prim[npix] == -1)
+        return false;
+    const f3 np = ld3(r.pg_pos[npix]);
+    const f3 nn = r.s.reject_using_geometric_normals ? gb_normal(d.gb_gn[npix]) : gb_normal(d.gb_sn[npix]);
+    bool plane = true, normal = true;
+    if (r.s.use_plane_distance_heuristic)
+        plane = fabsf(pt::dot(np - cpoint, cnormal)) < r.s.plane_distance_threshold;
+    if (r.s.use_normal_similarity_heuristic)
+        normal = pt::dot(cnormal, nn) > r.s.normal_similarity_angle_precomp;
+    return plane && normal;
+}
+
+__global__ __launch_bounds__(256, 2) void k_fst_start(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    const int32_t K = r.s.spatial_reuse_neighbor_count;
+    for (int32_t k = 0; k < K + 2; k++)
+        write_inactive(r.rays, (size_t)k * r.n + pix);
+    if (d.gb_prim[pix] == -1)
+    {
+        state_store(r, pix, res_empty(), 0u, PHASE_SKIP, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const int32_t cx = (int32_t)(pix % (uint32_t)d.width), cy = (int32_t)(pix / (uint32_t)d.width);
+    pt::Xorshift32 rng{fused_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
+    const uint32_t nseed = rng.next();
+    const Surface cs = surface_current(d, pix);
+    // load_spatiotemporal_neighbor_data (:71-101): frozen random leaves the
+    // temporal reservoir empty
+    const Res tn = d.freeze ? res_empty() : res_load(r.in, pix);
+    const f3 cnormal = r.s.reject_using_geometric_normals ? cs.gn : cs.sn;
+    // count_valid_spatiotemporal_neighbors (:103-148)
+    int32_t valid_count = 0, valid_M = 0;
+    uint32_t cache = 0u;
+    {
+        pt::Xorshift32 nrng{nseed};
+        for (int32_t i = 0; i < K; i++)
+        {
+            const int32_t np = spatial_neighbor(d, r, cx, cy, nrng);
+            if (np == -1)
+                continue;
+            if (!similar_prev(d, r, np, pix, cs.p, cnormal))
+                continue;
+            valid_M += __float_as_int(r.in[2 * (size_t)np + 1].x);
+            valid_count++;
+            cache |= 1u << i;
+        }
+    }
+    if (tn.M > 0)
+    {
+        valid_count++;
+        valid_M += tn.M;
+    }
+    const Res init = res_load(r.init, pix);
+    const bool cw = r.s.use_confidence_weights != 0;
+    const float conf_sum = cw ? (float)valid_M : 1.0f;
+    const bool update_mc = init.M > 0 && init.ucw > 0.0f;
+    float* aux = r.aux + fst_aux_stride(K) * pix;
+    for (int32_t i = 0; i < K + 1; i++)
+    {
+        aux[8 + 2 * i] = 0.0f;
+        aux[8 + 2 * i + 1] = -1.0f;
+    }
+    uint32_t rays = 0u;
+    float tf_center_t = 0.0f;
+    if (tn.M > 0)
+    {
+        f3 dir;
+        float dist;
+        if (tn.ucw > 0.0f)
+        {
+            tf_center_t = target_function(d, cs, tn.point, tn.tri, tn.flags, dir, dist);
+            if (tf_center_t != 0.0f)
+            {
+                write_ray(r.rays, pix, cs.p, cs.prim, dir, dist - 1.0e-4f);
+                rays |= 1u;
+            }
+        }
+        if (update_mc)
+        {
+            const Surface ts = surface_previous(d, r, pix);
+            const float tf_csn = target_function(d, ts, init.point, init.tri, init.flags, dir, dist);
+            const float tn_M = cw ? (float)tn.M : 1.0f;
+            aux[8] = tf_csn;
+            aux[9] = cw ? tn_M / conf_sum : 1.0f;
+            if (tf_csn != 0.0f)
+            {
+                write_ray(r.rays, (size_t)r.n + pix, ts.p, ts.prim, dir, dist - 1.0e-4f);
+                rays |= 2u;
+            }
+        }
+    }
+    if (update_mc)
+    {
+        pt::Xorshift32 nrng{nseed};
+        const int32_t start = valid_M == 0 ? K : 0;
+        for (int32_t i = start; i < K; i++)
+        {
+            if (!(cache & (1u << i)))
+            {
+                nrng.next();
+                nrng.next();
+                continue;
+            }
+            const int32_t np = spatial_neighbor(d, r, cx, cy, nrng);
+            if (np == -1)
+                continue;
+            const Surface ns = surface_previous(d, r, (uint32_t)np);
+            f3 dir;
+            float dist;
+            const float tf_csn = target_function(d, ns, init.point, init.tri, init.flags, dir, dist);
+            const float res_M = cw ? (float)__float_as_int(r.in[2 * (size_t)np + 1].x) : 1.0f;
+            aux[8 + 2 * (i + 1)] = tf_csn;
+            aux[8 + 2 * (i + 1) + 1] = cw ? res_M / conf_sum : 1.0f;
+            if (tf_csn != 0.0f)
+                write_ray(r.rays, (size_t)(i + 2) * r.n + pix, ns.p, ns.prim, dir, dist - 1.0e-4f);
+        }
+    }
+    aux[0] = tf_center_t;
+    aux[1] = __uint_as_float(nseed);
+    aux[2] = __int_as_float(valid_count);
+    aux[3] = __int_as_float(valid_M);
+    aux[4] = __uint_as_float(cache);
+    aux[5] = __uint_as_float(rays);
+    state_store(r, pix, res_empty(), rng.state, PHASE_PENDING, 0.0f, 0.0f, 0u);
+}
+
+__global__ __launch_bounds__(256, 2) void k_fst_combine(PtData d, RsData r)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= r.n)
+        return;
+    Res unused;
+    uint32_t rng_state, x2;
+    int32_t phase;
+    float x0, x1;
+    state_load(r, pix, unused, rng_state, phase, x0, x1, x2);
+    write_inactive(r.rays, pix);
+    if (phase == PHASE_SKIP)
+        return;
+    const int32_t K = r.s.spatial_reuse_neighbor_count;
+    const int32_t cx = (int32_t)(pix % (uint32_t)d.width), cy = (int32_t)(pix / (uint32_t)d.width);
+    const float* aux = r.aux + fst_aux_stride(K) * pix;
+    const uint32_t nseed = __float_as_uint(aux[1]);
+    const int32_t valid_count = __float_as_int(aux[2]);
+    const int32_t valid_M = __float_as_int(aux[3]);
+    const uint32_t cache = __float_as_uint(aux[4]);
+    const uint32_t rays = __float_as_uint(aux[5]);
+    pt::Xorshift32 rng{rng_state};
+    const Surface cs = surface_current(d, pix);
+    const Res tn = d.freeze ? res_empty() : res_load(r.in, pix);
+    const Res init = res_load(r.init, pix);
+    const bool cw = r.s.use_confidence_weights != 0;
+    const float center_M = cw ? (float)init.M : 1.0f;
+    const float conf_sum = cw ? (float)valid_M : 1.0f;
+    const float vdt = cw ? 1.0f : (float)valid_count;
+    const bool update_mc = init.M > 0 && init.ucw > 0.0f;
+    const float nume_mc = init.target / vdt * center_M;
+    Res out = res_empty();
+    float mc = 0.0f;
+    if (tn.M > 0)
+    {
+        float tf_center = aux[0];
+        if (rays & 1u)
+            tf_center = apply_visibility(tf_center, r.occ[pix] != 0);
+        const float tn_M = cw ? (float)tn.M : 1.0f;
+        const float nume = tn.target * tn_M;
+        const float denom = tn.target * conf_sum + tf_center / vdt * center_M;
+        const float mi = denom == 0.0f ? 0.0f : nume / denom;
+        if (update_mc)
+        {
+            float tf_csn = aux[8];
+            if (rays & 2u)
+                tf_csn = apply_visibility(tf_csn, r.occ[(size_t)r.n + pix] != 0);
+            const float denom_mc = tf_csn * conf_sum + nume_mc;
+            if (denom_mc != 0.0f)
+                mc += nume_mc / denom_mc / vdt * aux[9];
+        }
+        if (combine_with(out, tn, mi / vdt, tf_center, 1.0f, rng))
+            out.flags |= pt::RESTIR_FLAG_UNOCCLUDED;
+    }
+    pt::Xorshift32 nrng{nseed};
+    const int32_t start = valid_M == 0 ? K : 0;
+    for (int32_t i = start; i < K + 1; i++)
+    {
+        if (i < K && !(cache & (1u << i)))
+        {
+            nrng.next();
+            nrng.next();
+            continue;
+        }
+        const int32_t np = i == K ? (int32_t)pix : spatial_neighbor(d, r, cx, cy, nrng);
+        if (np == -1)
+            continue;
+        const Res nres = i == K ? init : res_load(r.in, (uint32_t)np);
+        float tf_center = 0.0f;
+        if (nres.ucw > 0.0f)
+        {
+            if (i == K)
+                tf_center = nres.target;
+            else
+            {
+                f3 dir;
+                float dist;
+                tf_center = target_function(d, cs, nres.point, nres.tri, nres.flags, dir, dist);
+            }
+        }
+        float w;
+        if (i < K)
+        {
+            const float res_M = cw ? (float)nres.M : 1.0f;
+            const float nume = nres.target * res_M;
+            const float denom = nres.target * conf_sum + tf_center / vdt * center_M;
+            const float mi = denom == 0.0f ? 0.0f : nume / denom;
+            if (update_mc)
+            {
+                float tf_csn = aux[8 + 2 * (i + 1)];
+                if (tf_csn != 0.0f)
+                    tf_csn = apply_visibility(tf_csn, r.occ[(size_t)(i + 2) * r.n + pix] != 0);
+                const float denom_mc = tf_csn * conf_sum + nume_mc;
+                if (denom_mc != 0.0f)
+                    mc += nume_mc / denom_mc / vdt * aux[8 + 2 * (i + 1) + 1];
+            }
+            w = mi / vdt;
+        }
+        else
+            w = mc == 0.0f ? 1.0f : mc;
+        if (combine_with(out, nres, w, tf_center, 1.0f, rng))
+        {
+            if (i == K)
+                out.flags |= nres.flags & pt::RESTIR_FLAG_UNOCCLUDED;
+            else
+                out.flags &= ~pt::RESTIR_FLAG_UNOCCLUDED;
+        }
+    }
+    res_end_normalized(out, 1.0f, 1.0f);
+    uint32_t need_vis = 0u;
+    if (r.kill_after && !(out.ucw <= 0.0f && out.wsum <= 0.0f) && !(out.flags & pt::RESTIR_FLAG_UNOCCLUDED))
+    {
+        const f3 p = ld3(d.gb_pos[pix]);
+        const f3 to = out.point - p;
+        const float dist = sqrtf(pt::dot(to, to));
+        const f3 dir = pt::vdiv(to, dist);
+        write_ray(r.rays, pix, p, d.gb_prim[pix], dir, dist - 1.0e-4f);
+        need_vis = 1u;
+    }
+    state_store(r, pix, out, rng.state, PHASE_DONE, 0.0f, 0.0f, need_vis);
+}
+
 __global__ void k_reservoir_clear(float4* buf, uint32_t n)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -892,7 +1250,8 @@ void allocate_restir_buffers(Ctx& ctx)
     if (R.settings.direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI || n == 0)
         return;
     const int32_t K = R.restir.spatial_reuse_neighbor_count;
-    const size_t slots = (size_t)(K > 2 ? K : 2);
+    // ray slots per pixel: K spatial, 2 temporal, K + 2 fused
+    const size_t slots = (size_t)K + 2;
     for (int i = 0; i < 3; i++)
         R.reservoirs[i].reserve(32 * n);
     if (restir_needs_prev_gbuffer(R))
@@ -907,7 +1266,13 @@ void allocate_restir_buffers(Ctx& ctx)
     R.rs_rays.reserve(32 * n * slots);
     R.rs_occ.reserve(n * slots);
     R.rs_hits.reserve(16 * n);
-    R.rs_aux.reserve(8 * n * (size_t)(K > 0 ? K : 1));
+    R.rs_aux.reserve(4 * n * std::max<size_t>(2 * (size_t)(K > 0 ? K : 1), 8 + 2 * ((size_t)K + 1)));
+    if (R.restir.do_light_presampling)
+    {
+        R.presampled.reserve(48 * (size_t)R.restir.light_presampling_number_of_subsets *
+                             (size_t)R.restir.light_presampling_subset_size);
+        R.presampled_ready = false;
+    }
 }
 
 void restir_reset(Ctx& ctx, hipStream_t s)
@@ -933,6 +1298,14 @@ void restir_initial_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_numbe
         PtData d = make_pt_data(ctx, sample_number, random_number);
         RsData r = make_rs_data(ctx);
         r.out = R.reservoirs[0].as<float4>();
+        if (st.do_light_presampling)
+        {
+            // the records hold triangle indices: never read them unfilled
+            if (!R.presampled_ready)
+                fail(MIPT_ERR_INVALID_ARGUMENT, "light presampling is on: launch "
+                                                "MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING before the initial candidates");
+            r.presampled = R.presampled.as<float4>();
+        }
         const unsigned g = blocks_for(r.n, 256);
         hipLaunchKernelGGL(k_ic_start, dim3(g), dim3(256), 0, s, d, r);
         for (int32_t j = 0; j < st.number_of_initial_bsdf_candidates; j++)
@@ -945,8 +1318,58 @@ void restir_initial_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_numbe
         hipLaunchKernelGGL(k_ic_finish, dim3(g), dim3(256), 0, s, r);
         HIP_CHECK(hipGetLastError());
     }
-    if (!st.do_temporal_reuse_pass && !st.do_spatial_reuse_pass)
+    if (!st.do_temporal_reuse_pass && !st.do_spatial_reuse_pass && !st.do_fused_spatiotemporal)
         end_frame(R, 0);
+}
+
+// ReSTIR_DI_LightsPresampling (ReSTIRDIRenderPass.cpp:411-447): subsets x
+// subset_size presampled lights, one thread each.
+void restir_presampling_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+{
+    check_ready(ctx);
+    RenderState& R = ctx.render;
+    const mipt_restir_di_settings& st = R.restir;
+    if (!st.do_light_presampling)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "light presampling is off in the ReSTIR DI settings");
+    if (ctx.emissive_count == 0)
+        return; // LightsPresampling.h:80-82: nothing to presample
+    const uint32_t count = (uint32_t)st.light_presampling_number_of_subsets * (uint32_t)st.light_presampling_subset_size;
+    PtData d = make_pt_data(ctx, sample_number, random_number);
+    hipLaunchKernelGGL(k_presample_lights, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, R.presampled.as<float4>(),
+                       count);
+    HIP_CHECK(hipGetLastError());
+    R.presampled_ready = true;
+}
+
+// ReSTIR_DI_SpatiotemporalReuse (ReSTIRDIRenderPass.cpp:624-657 for the
+// buffers): reads last frame's output, writes the other spatial buffer.
+void restir_spatiotemporal_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, hipStream_t s)
+{
+    check_ready(ctx);
+    RenderState& R = ctx.render;
+    const mipt_restir_di_settings& st = R.restir;
+    if (!st.do_fused_spatiotemporal)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "the fused spatiotemporal pass is off in the ReSTIR DI settings");
+    const int32_t in = R.last_output;
+    const int32_t out = in == 1 ? 2 : 1;
+    PtData d = make_pt_data(ctx, sample_number, random_number);
+    RsData r = make_rs_data(ctx);
+    r.in = R.reservoirs[in].as<float4>();
+    r.init = R.reservoirs[0].as<float4>();
+    r.out = R.reservoirs[out].as<float4>();
+    r.pass_index = 0;
+    r.kill_after = st.do_temporal_reuse_pass || st.spatial_number_of_passes - 1 != 0;
+    const int32_t K = st.spatial_reuse_neighbor_count;
+    const unsigned g = blocks_for(r.n, 256);
+    hipLaunchKernelGGL(k_fst_start, dim3(g), dim3(256), 0, s, d, r);
+    trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_occ.as<uint8_t>(), (uint32_t)(K + 2) * r.n, s);
+    hipLaunchKernelGGL(k_fst_combine, dim3(g), dim3(256), 0, s, d, r);
+    trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_occ.as<uint8_t>(), r.n, s);
+    hipLaunchKernelGGL(k_sp_finish, dim3(g), dim3(256), 0, s, r);
+    HIP_CHECK(hipGetLastError());
+    R.fused_output = out;
+    if (st.spatial_number_of_passes == 1)
+        end_frame(R, out);
 }
 
 // ReSTIR_DI_TemporalReuse (ReSTIRDIRenderPass.cpp:466-503 for the buffers).
@@ -980,12 +1403,24 @@ void restir_spatial_pass(Ctx& ctx, int32_t pass_index, uint32_t sample_number, u
     check_ready(ctx);
     RenderState& R = ctx.render;
     const mipt_restir_di_settings& st = R.restir;
-    if (!st.do_spatial_reuse_pass)
+    // the fused pass is followed by spatial passes 1 .. P-1 whatever
+    // do_spatial_reuse_pass says (ReSTIRDIRenderPass.cpp:634-656)
+    if (!st.do_spatial_reuse_pass && !st.do_fused_spatiotemporal)
         fail(MIPT_ERR_INVALID_ARGUMENT, "spatial reuse is disabled in the ReSTIR DI settings");
     if (pass_index < 0 || pass_index >= st.spatial_number_of_passes)
         fail(MIPT_ERR_INVALID_ARGUMENT, "spatial pass index out of range");
     int32_t in, out;
-    if (pass_index == 0)
+    if (st.do_fused_spatiotemporal)
+    {
+        // configure_spatial_pass_for_fused_spatiotemporal
+        // (ReSTIRDIRenderPass.cpp:573-604): pass 0 is the fused pass; later
+        // passes read the previous one's output and write the other buffer
+        if (pass_index == 0)
+            fail(MIPT_ERR_INVALID_ARGUMENT, "with do_fused_spatiotemporal, spatial pass 0 is the fused pass");
+        in = R.fused_output;
+        out = in == 1 ? 2 : 1;
+    }
+    else if (pass_index == 0)
     {
         in = st.do_temporal_reuse_pass ? R.temporal_output : 0;
         out = 1;
@@ -1018,6 +1453,8 @@ void restir_spatial_pass(Ctx& ctx, int32_t pass_index, uint32_t sample_number, u
     trace_any(ctx, reinterpret_cast<const mipt_ray*>(r.rays), R.rs_occ.as<uint8_t>(), r.n, s);
     hipLaunchKernelGGL(k_sp_finish, dim3(g), dim3(256), 0, s, r);
     HIP_CHECK(hipGetLastError());
+    if (st.do_fused_spatiotemporal)
+        R.fused_output = out;
     if (last)
         end_frame(R, out);
 }

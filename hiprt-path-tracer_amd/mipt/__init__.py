@@ -139,7 +139,9 @@ class RestirDISettings(C.Structure):
         ("use_normal_similarity_heuristic", C.c_int32), ("normal_similarity_angle_precomp", C.c_float),
         ("use_plane_distance_heuristic", C.c_int32), ("plane_distance_threshold", C.c_float),
         ("reject_using_geometric_normals", C.c_int32), ("do_final_shading_visibility", C.c_int32),
-        ("guard_backfacing_bsdf_candidates", C.c_int32),
+        ("guard_backfacing_bsdf_candidates", C.c_int32), ("do_fused_spatiotemporal", C.c_int32),
+        ("do_light_presampling", C.c_int32), ("light_presampling_number_of_subsets", C.c_int32),
+        ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
     ]
 
 
@@ -161,6 +163,8 @@ MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES = 2
 MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE = 3
 MIPT_PASS_RADIANCE_CACHE_TRAIN = 4
 MIPT_PASS_RADIANCE_CACHE_WARMUP = 5
+MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING = 6
+MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE = 7
 
 
 def MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i: int) -> int:
@@ -612,8 +616,8 @@ class Context:
         return out
 
     def set_restir_di_settings(self, settings: RestirDISettings):
-        self.restir = settings
         self._check(self.lib.mipt_set_restir_di_settings(self.h, C.byref(settings)), "mipt_set_restir_di_settings")
+        self.restir = settings
 
     def draws_per_sample(self) -> int:
         return int(self.lib.mipt_draws_per_sample(self.h))

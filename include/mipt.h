@@ -411,6 +411,20 @@ typedef struct mipt_restir_di_settings {
      * (InitialCandidates.h:402-410), which poisons the reservoir and then its
      * spatial and temporal neighbours. 1 gives such a candidate weight 0. */
     int32_t guard_backfacing_bsdf_candidates;
+    /* ReSTIRDISettings::do_fused_spatiotemporal (ReSTIRDISettings.h:141),
+     * default 0: temporal reuse and the first spatial pass in one pass
+     * (MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE). */
+    int32_t do_fused_spatiotemporal;
+    /* Kernel option ReSTIR_DI_DoLightPresampling (ReSTIRDIOptions.h:151),
+     * default 0: light candidates are drawn from subsets of lights presampled
+     * by power each frame (MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING), all pixels
+     * of a tile_size x tile_size tile from one random subset
+     * (ReSTIRDILightPresamplingSettings, ReSTIRDISettings.h:45-56:
+     * 128 subsets of 1024 lights, tiles of 8). */
+    int32_t do_light_presampling;
+    int32_t light_presampling_number_of_subsets;
+    int32_t light_presampling_subset_size;
+    int32_t light_presampling_tile_size;
 } mipt_restir_di_settings;
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s);
@@ -437,6 +451,16 @@ int mipt_set_camera(mipt_ctx* ctx, const mipt_camera* cam);
 /* ReSTIR_DI_SpatialReuse (DI/SpatialReuse.h:43) as spatial pass number i
  * (common_spatial_pass.spatial_pass_index), i in [0, spatial_number_of_passes). */
 #define MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i) (16 + (i))
+/* ReSTIR_DI_LightsPresampling (DI/LightsPresampling.h:74): the presampled
+ * light subsets, with the camera pass's random number (the reference launches
+ * it before drawing the initial candidates' number). */
+#define MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING 6
+/* ReSTIR_DI_SpatiotemporalReuse (DI/FusedSpatiotemporalReuse.h:152):
+ * temporal reuse and spatial pass 0 in one pass; further spatial passes
+ * (MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i), i >= 1) follow it. Its random
+ * number is the reference's third draw of the pass (after the temporal
+ * pass's number and permutation bits, ReSTIRDIRenderPass.cpp:515-604). */
+#define MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE 7
 
 /* One pass with the given sample_number (render_settings.sample_number) and
  * random_number (render_data.random_number). Asynchronous on `stream`.

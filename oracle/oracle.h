@@ -204,6 +204,11 @@ typedef struct {
     int32_t reject_using_geometric_normals;
     int32_t do_final_shading_visibility;
     int32_t guard_backfacing_bsdf_candidates; /* 0: the reference's 0/0 weight */
+    int32_t do_fused_spatiotemporal;
+    int32_t do_light_presampling;
+    int32_t light_presampling_number_of_subsets;
+    int32_t light_presampling_subset_size;
+    int32_t light_presampling_tile_size;
 } orc_restir_settings;
 
 /* The BSDF tables (the .f32 files of hiprt-path-tracer_amd/data, converted from the
@@ -266,6 +271,10 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t ran
 void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
 void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_number, uint32_t random_number,
                               int32_t nthreads);
+/* ReSTIR_DI_LightsPresampling and ReSTIR_DI_SpatiotemporalReuse (with
+ * do_light_presampling / do_fused_spatiotemporal in the settings). */
+void orc_frame_restir_presample(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
+void orc_frame_restir_spatiotemporal(orc_frame* f, uint32_t sample_number, uint32_t random_number, int32_t nthreads);
 /* Path pass: accumulates into fb/fb64 (sum), returns the dropped samples. */
 int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t random_number, float* fb, double* fb64,
                              int32_t nthreads);

@@ -53,7 +53,9 @@ class RestirSettings(C.Structure):
         ("use_normal_similarity_heuristic", C.c_int32), ("normal_similarity_angle_precomp", C.c_float),
         ("use_plane_distance_heuristic", C.c_int32), ("plane_distance_threshold", C.c_float),
         ("reject_using_geometric_normals", C.c_int32), ("do_final_shading_visibility", C.c_int32),
-        ("guard_backfacing_bsdf_candidates", C.c_int32),
+        ("guard_backfacing_bsdf_candidates", C.c_int32), ("do_fused_spatiotemporal", C.c_int32),
+        ("do_light_presampling", C.c_int32), ("light_presampling_number_of_subsets", C.c_int32),
+        ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
     ]
 
 
@@ -108,6 +110,8 @@ _SIG = {
     "orc_frame_restir_initial": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_temporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_spatial": (None, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_restir_presample": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_restir_spatiotemporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_path_trace": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, _P, _P, C.c_int32]),
     "orc_frame_read_reservoirs": (None, [C.c_void_p, C.c_int32, _P]),
     "orc_frame_gbuffer_prims": (None, [C.c_void_p, _P]),
@@ -375,6 +379,12 @@ class Renderer:
     def restir_spatial(self, pass_index, sample_number, rn, nthreads=0):
         lib().orc_frame_restir_spatial(self.h, pass_index, sample_number, rn, nthreads)
 
+    def restir_presample(self, sample_number, rn, nthreads=0):
+        lib().orc_frame_restir_presample(self.h, sample_number, rn, nthreads)
+
+    def restir_spatiotemporal(self, sample_number, rn, nthreads=0):
+        lib().orc_frame_restir_spatiotemporal(self.h, sample_number, rn, nthreads)
+
     def path_trace(self, sample_number, rn, nthreads=0):
         self.rejected += lib().orc_frame_path_trace(self.h, sample_number, rn, _p(self.fb), _p(self.fb64), nthreads)
 
@@ -389,16 +399,26 @@ class Renderer:
         stream): camera, [ReSTIR DI initial, temporal (two draws), spatial
         passes], path."""
         rn = iter(random_numbers)
-        self.camera_pass(sample_number, next(rn), nthreads)
+        rn_camera = next(rn)
+        self.camera_pass(sample_number, rn_camera, nthreads)
         if self.settings.direct_light_sampling_strategy == 5:
             r = self.restir
+            if r.do_light_presampling:
+                self.restir_presample(sample_number, rn_camera, nthreads)
             self.restir_initial(sample_number, next(rn), nthreads)
-            if r.do_temporal_reuse_pass:
-                self.restir_temporal(sample_number, next(rn), nthreads)
-                next(rn)
-            if r.do_spatial_reuse_pass:
-                for i in range(r.spatial_number_of_passes):
+            if r.do_fused_spatiotemporal:
+                next(rn)  # the temporal pass's number
+                next(rn)  # its permutation bits
+                self.restir_spatiotemporal(sample_number, next(rn), nthreads)
+                for i in range(1, r.spatial_number_of_passes):
                     self.restir_spatial(i, sample_number, next(rn), nthreads)
+            else:
+                if r.do_temporal_reuse_pass:
+                    self.restir_temporal(sample_number, next(rn), nthreads)
+                    next(rn)
+                if r.do_spatial_reuse_pass:
+                    for i in range(r.spatial_number_of_passes):
+                        self.restir_spatial(i, sample_number, next(rn), nthreads)
         self.path_trace(sample_number, next(rn), nthreads)
 
     def render_frames_in_window(self, frames, tile, random_numbers_of_frame, halo=None, nthreads=0):
@@ -415,7 +435,9 @@ class Renderer:
         per-pass margin. Restores the whole-image window at the end."""
         r = self.restir
         restir_on = self.settings.direct_light_sampling_strategy == 5
-        P = r.spatial_number_of_passes if (restir_on and r.do_spatial_reuse_pass) else 0
+        fused = restir_on and r.do_fused_spatiotemporal
+        # the fused pass reads neighbours as spatial pass 0 does
+        P = r.spatial_number_of_passes if (restir_on and (r.do_spatial_reuse_pass or fused)) else 0
         h = r.spatial_reuse_radius + 1 if halo is None else halo
         x0, x1, y0, y1 = tile
 
@@ -426,16 +448,28 @@ class Renderer:
             last = (frames - 1 - f) * P * h
             rn = iter(random_numbers_of_frame(f))
             win(last + P * h)
-            self.camera_pass(f, next(rn), nthreads)
+            rn_camera = next(rn)
+            self.camera_pass(f, rn_camera, nthreads)
             if restir_on:
+                if r.do_light_presampling:
+                    self.restir_presample(f, rn_camera, nthreads)
                 self.restir_initial(f, next(rn), nthreads)
-                if r.do_temporal_reuse_pass:
-                    self.restir_temporal(f, next(rn), nthreads)
+                if fused:
                     next(rn)
-                if r.do_spatial_reuse_pass:
-                    for i in range(P):
+                    next(rn)
+                    win(last + (P - 1) * h)
+                    self.restir_spatiotemporal(f, next(rn), nthreads)
+                    for i in range(1, P):
                         win(last + (P - 1 - i) * h)
                         self.restir_spatial(i, f, next(rn), nthreads)
+                else:
+                    if r.do_temporal_reuse_pass:
+                        self.restir_temporal(f, next(rn), nthreads)
+                        next(rn)
+                    if r.do_spatial_reuse_pass:
+                        for i in range(P):
+                            win(last + (P - 1 - i) * h)
+                            self.restir_spatial(i, f, next(rn), nthreads)
             win(0)
             self.path_trace(f, next(rn), nthreads)
         self.set_window(0, self.settings.width, 0, self.settings.height)

@@ -1702,11 +1702,11 @@ static rgb dispersion_filter(float* w, float scale)
 
 typedef struct { int32_t tri; v3 normal; float area; rgb emission; v3 point; float area_pdf; } olight;
 
-static olight sample_emissive(const rctx* R, uint32_t* st)
+static olight sample_emissive_with(const rctx* R, int32_t strategy, uint32_t* st)
 {
     olight L;
     int32_t e = orc_random_index(st, R->ne);
-    if (R->s.direct_light_sampling_base_strategy == 1)
+    if (strategy == 1)
     {
         float p = R->alias_probas[e];
         if (orc_rand01(st) > p)
@@ -1726,11 +1726,16 @@ static olight sample_emissive(const rctx* R, uint32_t* st)
     L.area = 0.5f * len;
     L.emission = R->mats[R->mat_idx[L.tri]].emission;
     L.area_pdf = 1.0f / L.area;
-    if (R->s.direct_light_sampling_base_strategy == 1)
+    if (strategy == 1)
         L.area_pdf *= (lum(L.emission) * L.area) / R->alias_sum;
     else
         L.area_pdf /= (float)R->ne;
     return L;
+}
+
+static olight sample_emissive(const rctx* R, uint32_t* st)
+{
+    return sample_emissive_with(R, R->s.direct_light_sampling_base_strategy, st);
 }
 
 static float emissive_hit_area_pdf(const rctx* R, float area, rgb emission)
@@ -2277,6 +2282,19 @@ struct orc_frame {
     /* pixel window [x0, x1) x [y0, y1) that the passes compute (the whole
      * image by default); pixels outside it keep their previous state */
     int32_t win[4];
+    /* light presampling: subsets x subset_size presampled lights */
+    struct opresampled* presampled;
+    size_t presampled_count;
+    int32_t fused_output; /* buffer the fused pass (or a spatial pass after it) wrote */
+};
+
+/* ReSTIRDIPresampledLight (ReSTIR/DI/PresampledLight.h:14-30) */
+struct opresampled {
+    int32_t tri;
+    v3 point, normal;
+    rgb radiance;
+    float pdf;
+    uint32_t flags;
 };
 
 static int in_window(const orc_frame* f, size_t pix)
@@ -2409,7 +2427,15 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
     f->shade_buffer = 1;
     f->temporal_output = 0;
     f->odd_frame = 0;
+    f->fused_output = 1;
     orc_frame_set_window(f, 0, s->width, 0, s->height);
+    if (rs && rs->do_light_presampling)
+    {
+        f->presampled_count = (size_t)rs->light_presampling_number_of_subsets * (size_t)rs->light_presampling_subset_size;
+        f->presampled = (struct opresampled*)calloc(f->presampled_count, sizeof(struct opresampled));
+        for (size_t i = 0; i < f->presampled_count; i++)
+            f->presampled[i].tri = -1;
+    }
     return f;
 }
 
@@ -2423,6 +2449,7 @@ void orc_frame_free(orc_frame* f)
     free(f->alias_alias);
     free(f->cur);
     free(f->prev);
+    free(f->presampled);
     for (int k = 0; k < 3; k++)
         free(f->res[k]);
     free(f);
@@ -2646,7 +2673,27 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                 float cosine = 0.0f, pdf = 0.0f, dist = 0.0f;
                 v3 dir = V(0, 0, 0), point = V(0, 0, 0);
                 int32_t tri_i = -1;
-                if (orc_rand01(&st) > 0.0f)
+                if (f->presampled)
+                {
+                    /* use_presampled_light_candidate (InitialCandidates.h:33-90) */
+                    const int32_t w_ = R->s.width, ts = f->rs.light_presampling_tile_size;
+                    const int32_t x = (int32_t)(pix % (size_t)w_), y = (int32_t)(pix / (size_t)w_);
+                    const int32_t tx = x / ts, ty = y / ts;
+                    const int32_t tile = (tx + ty + 1) * (tx + ty) / 2 + ty; /* Cantor pairing */
+                    uint32_t sst = rn * (uint32_t)(tile + 1);
+                    int32_t subset = orc_random_index(&sst, f->rs.light_presampling_number_of_subsets);
+                    int32_t in_subset = orc_random_index(&st, f->rs.light_presampling_subset_size);
+                    const struct opresampled* P =
+                        f->presampled + (size_t)subset * (size_t)f->rs.light_presampling_subset_size + (size_t)in_subset;
+                    tri_i = P->tri;
+                    point = P->point;
+                    pdf = P->pdf;
+                    v3 to = vsub(point, s.p);
+                    dist = vlen(to);
+                    dir = vdivs(to, dist);
+                    cosine = vdot(s.sn, dir);
+                }
+                else if (orc_rand01(&st) > 0.0f)
                 {
                     olight L = sample_emissive(R, &st);
                     tri_i = L.tri;
@@ -2730,13 +2777,46 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
             f->res[0][pix] = res;
         }
     }
-    if (!f->rs.do_temporal_reuse_pass && !f->rs.do_spatial_reuse_pass)
+    if (!f->rs.do_temporal_reuse_pass && !f->rs.do_spatial_reuse_pass && !f->rs.do_fused_spatiotemporal)
         end_frame(f, 0);
 }
 
 /* ReSTIR_DI_TemporalReuse (DI/TemporalReuse.h:50-296), accumulation mode
  * (the temporal neighbour is the pixel itself, UtilsTemporal.h:37-41), with
  * the pairwise MIS weights (TemporalMISWeight.h:164-254). */
+/* ReSTIR_DI_LightsPresampling (DI/LightsPresampling.h:74-112): one light
+ * per slot, seeded like a pixel by its slot index; an envmap draw with
+ * probability 0, then an emissive triangle by power
+ * (ReSTIR_DI_LightPresamplingStrategy, ReSTIRDIOptions.h:162). */
+void orc_frame_restir_presample(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
+{
+    if (!f->presampled || f->R.ne == 0)
+        return;
+    const rctx* R = &f->R;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int64_t x = 0; x < (int64_t)f->presampled_count; x++)
+    {
+        uint32_t st = orc_pixel_seed((uint32_t)x, sample_number, rn, R->s.freeze_random);
+        struct opresampled P;
+        memset(&P, 0, sizeof(P));
+        P.tri = -1;
+        if (!(orc_rand01(&st) < 0.0f))
+        {
+            olight L = sample_emissive_with(R, 1, &st);
+            if (L.area_pdf > 0.0f)
+            {
+                P.point = L.point;
+                P.normal = L.normal;
+                P.tri = L.tri;
+                P.pdf = L.area_pdf * 1.0f;
+                P.radiance = L.emission;
+            }
+        }
+        f->presampled[x] = P;
+    }
+}
+
 void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
 {
     f->R.sample_number = sample_number;
@@ -2848,7 +2928,15 @@ void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
     f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     int32_t in_i, out_i;
-    if (pass_index == 0)
+    if (f->rs.do_fused_spatiotemporal)
+    {
+        /* configure_spatial_pass_for_fused_spatiotemporal
+         * (ReSTIRDIRenderPass.cpp:573-604): after the fused pass, each pass
+         * reads the previous output and writes the other buffer */
+        in_i = f->fused_output;
+        out_i = in_i == 1 ? 2 : 1;
+    }
+    else if (pass_index == 0)
     {
         in_i = f->rs.do_temporal_reuse_pass ? f->temporal_output : 0;
         out_i = 1;
@@ -2957,7 +3045,166 @@ void orc_frame_restir_spatial(orc_frame* f, int32_t pass_index, uint32_t sample_
             o.M = f->rs.m_cap;
         out[pix] = o;
     }
+    if (f->rs.do_fused_spatiotemporal)
+        f->fused_output = out_i;
     if (last)
+        end_frame(f, out_i);
+}
+
+/* check_neighbor_similarity_heuristics(..., previous_frame = 1)
+ * (NeighborSimilarity.h:52-107): the neighbour's primary hit and position
+ * from the previous frame's G-buffer, its normal from the current one */
+static int similar_prev(const orc_frame* f, size_t np, size_t cp, v3 cpoint, v3 cnormal)
+{
+    if (np == cp)
+        return 1;
+    if (f->prev[np].prim == -1)
+        return 0;
+    v3 npt = f->prev[np].pos;
+    v3 nn = f->rs.reject_using_geometric_normals ? vnorm(oct_unpack(f->cur[np].gn)) : vnorm(oct_unpack(f->cur[np].sn));
+    int plane = 1, normal = 1;
+    if (f->rs.use_plane_distance_heuristic)
+        plane = fabsf(vdot(vsub(npt, cpoint), cnormal)) < f->rs.plane_distance_threshold;
+    if (f->rs.use_normal_similarity_heuristic)
+        normal = vdot(cnormal, nn) > f->rs.normal_similarity_angle_precomp;
+    return plane && normal;
+}
+
+/* ReSTIR_DI_SpatiotemporalReuse (DI/FusedSpatiotemporalReuse.h:152-566) with
+ * a static camera: the temporal neighbour is the pixel itself
+ * (UtilsTemporal.h:37-41), its reservoir and the spatial neighbours' come from
+ * last frame's output, neighbour surfaces and similarity tests from the
+ * previous frame's G-buffer (use_prev_frame_g_buffer()), pairwise MIS
+ * (SpatiotemporalMISWeight.h:139-224): the temporal neighbour resampled
+ * first, the spatial neighbours around it, the initial candidates last.
+ * Seeded with wang_hash(((pix + 1) * (sample + 1)) ^ random) (:173). */
+void orc_frame_restir_spatiotemporal(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
+{
+    f->R.sample_number = sample_number;
+    f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
+    f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
+    const int32_t in_i = f->last_output, out_i = in_i == 1 ? 2 : 1;
+    ores* in = f->res[in_i];
+    ores* init_buf = f->res[0];
+    ores* out = f->res[out_i];
+    const int32_t K = f->rs.spatial_reuse_neighbor_count;
+    const int cw = f->rs.use_confidence_weights != 0;
+    const int kill_after = f->rs.do_temporal_reuse_pass || f->rs.spatial_number_of_passes - 1 != 0;
+    const int w = f->R.s.width;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t pi = 0; pi < (int64_t)f->n; pi++)
+    {
+        size_t pix = (size_t)pi;
+        if (f->cur[pix].prim == -1 || !in_window(f, pix))
+            continue;
+        int cx = (int)(pix % (size_t)w), cy = (int)(pix / (size_t)w);
+        uint32_t st = f->R.s.freeze_random ? orc_wang_hash((uint32_t)pix + 1u)
+                                           : orc_wang_hash((((uint32_t)pix + 1u) * (sample_number + 1u)) ^ rn);
+        uint32_t nseed = orc_xorshift32(&st);
+        osurf cs = surf_of(f, f->cur, pix);
+        ores tn = f->R.s.freeze_random ? ores_empty() : in[pix];
+        v3 cnormal = f->rs.reject_using_geometric_normals ? cs.gn : cs.sn;
+        int32_t valid_count = 0, valid_M = 0;
+        uint32_t cache = 0u;
+        uint32_t nst = nseed;
+        for (int32_t i = 0; i < K; i++)
+        {
+            int64_t np = spatial_neighbor(f, cx, cy, &nst);
+            if (np < 0)
+                continue;
+            if (!similar_prev(f, (size_t)np, pix, cs.p, cnormal))
+                continue;
+            valid_M += in[np].M;
+            valid_count++;
+            cache |= 1u << i;
+        }
+        if (tn.M > 0)
+        {
+            valid_count++;
+            valid_M += tn.M;
+        }
+        ores init = init_buf[pix];
+        float center_M = cw ? (float)init.M : 1.0f;
+        float conf_sum = cw ? (float)valid_M : 1.0f;
+        float vdt = cw ? 1.0f : (float)valid_count;
+        int update_mc = init.M > 0 && init.ucw > 0.0f;
+        float nume_mc = init.target / vdt * center_M;
+        float mc = 0.0f;
+        ores o = ores_empty();
+        if (tn.M > 0)
+        {
+            float tf_center = tn.ucw > 0.0f ? target_fn(f, &cs, tn.point, tn.tri, tn.flags, 1) : 0.0f;
+            float tn_M = cw ? (float)tn.M : 1.0f;
+            float nume = tn.target * tn_M;
+            float denom = tn.target * conf_sum + tf_center / vdt * center_M;
+            float mi = denom == 0.0f ? 0.0f : nume / denom;
+            if (update_mc)
+            {
+                osurf ts = surf_of(f, f->prev, pix);
+                float tf_csn = target_fn(f, &ts, init.point, init.tri, init.flags, 1);
+                float denom_mc = tf_csn * conf_sum + nume_mc;
+                float conf_mult = cw ? tn_M / conf_sum : 1.0f;
+                if (denom_mc != 0.0f)
+                    mc += nume_mc / denom_mc / vdt * conf_mult;
+            }
+            if (combine_with(&o, &tn, mi / vdt, tf_center, 1.0f, &st))
+                o.flags |= RESTIR_UNOCCLUDED;
+        }
+        nst = nseed;
+        int32_t start = valid_M == 0 ? K : 0;
+        for (int32_t i = start; i < K + 1; i++)
+        {
+            if (i < K && !(cache & (1u << i)))
+            {
+                orc_xorshift32(&nst);
+                orc_xorshift32(&nst);
+                continue;
+            }
+            int64_t np = i == K ? (int64_t)pix : spatial_neighbor(f, cx, cy, &nst);
+            if (np < 0)
+                continue;
+            ores nr = i == K ? init : in[np];
+            float tf_center = 0.0f;
+            if (nr.ucw > 0.0f)
+                tf_center = i == K ? nr.target : target_fn(f, &cs, nr.point, nr.tri, nr.flags, 0);
+            float mis;
+            if (i < K)
+            {
+                float res_M = cw ? (float)nr.M : 1.0f;
+                float nume = nr.target * res_M;
+                float denom = nr.target * conf_sum + tf_center / vdt * center_M;
+                float mi = denom == 0.0f ? 0.0f : nume / denom;
+                if (update_mc)
+                {
+                    osurf ns = surf_of(f, f->prev, (size_t)np);
+                    float tf_csn = target_fn(f, &ns, init.point, init.tri, init.flags, 1);
+                    float denom_mc = tf_csn * conf_sum + nume_mc;
+                    float conf_mult = cw ? res_M / conf_sum : 1.0f;
+                    if (denom_mc != 0.0f)
+                        mc += nume_mc / denom_mc / vdt * conf_mult;
+                }
+                mis = mi / vdt;
+            }
+            else
+                mis = mc == 0.0f ? 1.0f : mc;
+            if (combine_with(&o, &nr, mis, tf_center, 1.0f, &st))
+            {
+                if (i == K)
+                    o.flags |= nr.flags & RESTIR_UNOCCLUDED;
+                else
+                    o.flags &= ~RESTIR_UNOCCLUDED;
+            }
+        }
+        res_end_normalized(&o, 1.0f, 1.0f);
+        if (kill_after)
+            visibility_kill(f, &o, cs.p, cs.prim);
+        if (f->rs.m_cap > 0 && o.M > f->rs.m_cap)
+            o.M = f->rs.m_cap;
+        out[pix] = o;
+    }
+    f->fused_output = out_i;
+    if (f->rs.spatial_number_of_passes == 1)
         end_frame(f, out_i);
 }
 

@@ -1,5 +1,6 @@
-"""GPU parity of the ReSTIR DI passes (initial candidates, temporal reuse,
-spatial reuse, final shading at bounce 0) and of power light sampling,
+"""GPU parity of the ReSTIR DI passes (light presampling, initial
+candidates, temporal reuse, spatial reuse, fused spatiotemporal reuse, final
+shading at bounce 0) and of power light sampling,
 through the C ABI, against the oracle's per-pixel restatement.
 
 Bar: first-hit primitive IDs bit-exact; reservoirs (sample point, light
@@ -112,6 +113,20 @@ VARIANTS = {
                           reject_using_geometric_normals=0),
     "no_final_visibility": dict(do_final_shading_visibility=0),
     "one_neighbor_radius3": dict(spatial_reuse_neighbor_count=1, spatial_reuse_radius=3),
+    "fused": dict(do_fused_spatiotemporal=1),
+    "fused_two_spatial": dict(do_fused_spatiotemporal=1, spatial_number_of_passes=2),
+    "fused_three_no_confidence": dict(do_fused_spatiotemporal=1, spatial_number_of_passes=3,
+                                      use_confidence_weights=0),
+    "fused_spatial_flag_off": dict(do_fused_spatiotemporal=1, do_spatial_reuse_pass=0, spatial_number_of_passes=2),
+    "fused_no_heuristics": dict(do_fused_spatiotemporal=1, use_normal_similarity_heuristic=0,
+                                use_plane_distance_heuristic=0),
+    "presampling": dict(do_light_presampling=1, light_presampling_number_of_subsets=16,
+                        light_presampling_subset_size=64),
+    "presampling_one_subset": dict(do_light_presampling=1, light_presampling_number_of_subsets=1,
+                                   light_presampling_subset_size=7, light_presampling_tile_size=3),
+    "presampling_fused": dict(do_light_presampling=1, light_presampling_number_of_subsets=8,
+                              light_presampling_subset_size=32, do_fused_spatiotemporal=1,
+                              spatial_number_of_passes=2),
 }
 
 
@@ -121,6 +136,22 @@ def test_restir_variants(mipt_mod, oracle_mod, torch_mod, variant):
     s = mipt_mod.default_render_settings(width=64, height=64, direct_light_sampling_strategy=5, nb_bounces=2)
     rs = mipt_mod.default_restir_di_settings(**VARIANTS[variant])
     render_both(mipt_mod, oracle_mod, scene, s, rs, 4)
+
+
+@pytest.mark.parametrize("variant", ["power", "freeze"])
+def test_fused_and_presampling_light_sampling(mipt_mod, oracle_mod, torch_mod, variant):
+    """Presampled power-sampled lights into the fused pass; frozen random
+    numbers (the fused pass then skips its temporal neighbour)."""
+    scene = mipt_mod.generate_scene(1, 64, 64)
+    s = mipt_mod.default_render_settings(width=64, height=64, direct_light_sampling_strategy=5, nb_bounces=2)
+    if variant == "power":
+        s.direct_light_sampling_base_strategy = 1
+    else:
+        s.freeze_random = 1
+    rs = mipt_mod.default_restir_di_settings(do_fused_spatiotemporal=1, do_light_presampling=1,
+                                             light_presampling_number_of_subsets=16,
+                                             light_presampling_subset_size=64)
+    render_both(mipt_mod, oracle_mod, scene, s, rs, 4, check_every_frame=True)
 
 
 @pytest.mark.parametrize("variant", ["power", "freeze", "direct_only", "power_mis"])
@@ -151,6 +182,22 @@ def test_restir_config2_geometry(mipt_mod, oracle_mod, torch_mod, guard):
                                          direct_light_sampling_base_strategy=1)
     rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
                                              spatial_number_of_passes=2, guard_backfacing_bsdf_candidates=guard)
+    fb, R = render_both(mipt_mod, oracle_mod, scene, s, rs, 3)
+    assert fb.mean() > 0.0
+
+
+def test_restir_config2_fused_presampling(mipt_mod, oracle_mod, torch_mod):
+    """The at-scale ReSTIR DI parameters with the fused spatiotemporal pass
+    and the reference's default light presampling (128 subsets of 1024
+    power-sampled lights, 8x8 pixel tiles per subset) on the 1M-triangle
+    scene with 4096 emitters, at 240x135."""
+    scene = simplify_materials(mipt_mod.generate_scene(2, 240, 135))
+    s = mipt_mod.default_render_settings(width=240, height=135, nb_bounces=5, direct_light_sampling_strategy=5,
+                                         direct_light_sampling_base_strategy=1)
+    rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
+                                             spatial_number_of_passes=2, do_fused_spatiotemporal=1,
+                                             do_light_presampling=1)
+    assert rs.light_presampling_number_of_subsets == 128 and rs.light_presampling_subset_size == 1024
     fb, R = render_both(mipt_mod, oracle_mod, scene, s, rs, 3)
     assert fb.mean() > 0.0
 
@@ -208,6 +255,28 @@ def test_restir_pass_errors(mipt_mod, torch_mod):
     bad = mipt_mod.default_render_settings(width=32, height=32, direct_light_sampling_strategy=4)
     with pytest.raises(mipt_mod.MiptError):
         ctx.set_render_settings(bad)
+    # the fused and presampling passes only run when their settings ask
+    for p in (mipt_mod.MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE, mipt_mod.MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING):
+        with pytest.raises(mipt_mod.MiptError):
+            ctx.launch_pass(p, 0, 1)
+    with pytest.raises(mipt_mod.MiptError):  # the fused pass reads last frame's reservoirs
+        ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(do_fused_spatiotemporal=1,
+                                                                       do_temporal_reuse_pass=0))
+    with pytest.raises(mipt_mod.MiptError):
+        ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(do_light_presampling=1,
+                                                                       light_presampling_subset_size=0))
+    ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(do_fused_spatiotemporal=1))
+    with pytest.raises(mipt_mod.MiptError):  # spatial pass 0 is the fused pass
+        ctx.launch_pass(mipt_mod.MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(0), 0, 1)
+    assert ctx.draws_per_sample() == 6
+    # presampled records are read only after the presampling pass filled them
+    ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(do_light_presampling=1))
+    ctx.set_render_settings(mipt_mod.default_render_settings(width=32, height=32, direct_light_sampling_strategy=5))
+    ctx.launch_pass(mipt_mod.MIPT_PASS_CAMERA_RAYS, 0, 1)
+    with pytest.raises(mipt_mod.MiptError):
+        ctx.launch_pass(mipt_mod.MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, 0, 1)
+    ctx.launch_pass(mipt_mod.MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING, 0, 1)
+    ctx.launch_pass(mipt_mod.MIPT_PASS_RESTIR_DI_INITIAL_CANDIDATES, 0, 1)
     ctx.close()
 
 

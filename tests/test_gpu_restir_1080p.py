@@ -2,7 +2,9 @@
 power-sampled emitters at 1920x1080, 32 light + 1 BSDF initial candidates,
 temporal reuse (M-cap 20), 2 spatial passes of 5 neighbours in 16 px,
 pairwise MIS, 5 bounces, 4 frames (frames 2 and 3 hold pixels whose camera
-ray went hit -> miss -> hit under jitter).
+ray went hit -> miss -> hit under jitter); once more with the fused
+spatiotemporal pass fed by the default light presampling (128 subsets of
+1024 lights, 8x8 pixel tiles per subset).
 
 The GPU renders the whole frame. Checked:
 - over the whole frame, the reservoir invariants (M in [1, M-cap] on the
@@ -51,8 +53,10 @@ def _boundary_tile(prims):
     return (tx * TILE, tx * TILE + TILE, ty * TILE, ty * TILE + TILE), int(per[ty, tx])
 
 
-@pytest.mark.parametrize("materials", ["diffuse_metal", "layered"])
-def test_restir_config2_1080p(mipt_mod, oracle_mod, torch_mod, materials):
+@pytest.mark.parametrize("case", ["diffuse_metal", "layered", "fused_presampling"])
+def test_restir_config2_1080p(mipt_mod, oracle_mod, torch_mod, case):
+    materials = "layered" if case == "layered" else "diffuse_metal"
+    fused = case == "fused_presampling"
     scene = mipt_mod.generate_scene(2, W, H)
     for m in scene.materials:
         m.specular_transmission = 0.0
@@ -61,7 +65,8 @@ def test_restir_config2_1080p(mipt_mod, oracle_mod, torch_mod, materials):
     s = mipt_mod.default_render_settings(width=W, height=H, nb_bounces=5, direct_light_sampling_strategy=5,
                                          direct_light_sampling_base_strategy=1)
     rs = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
-                                             spatial_number_of_passes=2)
+                                             spatial_number_of_passes=2, do_fused_spatiotemporal=int(fused),
+                                             do_light_presampling=int(fused))
     ctx = mipt_mod.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
@@ -70,7 +75,7 @@ def test_restir_config2_1080p(mipt_mod, oracle_mod, torch_mod, materials):
     ctx.set_camera(scene.camera)
     draws = ctx.draws_per_sample()
     for f in range(FRAMES):
-        print(f"restir 1080p {materials}: frame {f}", flush=True)
+        print(f"restir 1080p {case}: frame {f}", flush=True)
         ctx.render_sample(f)
         torch_mod.cuda.synchronize()
     prims = ctx.read_gbuffer_prims()
@@ -96,7 +101,7 @@ def test_restir_config2_1080p(mipt_mod, oracle_mod, torch_mod, materials):
     edge_tile, edges = _boundary_tile(prims)
     assert edges > 0
     for tile in (centre, edge_tile):
-        print(f"restir 1080p {materials}: oracle on tile {tile}", flush=True)
+        print(f"restir 1080p {case}: oracle on tile {tile}", flush=True)
         R = oracle_mod.Renderer(scene, s, rs)
         R.render_frames_in_window(FRAMES, tile, numbers)
         sel = np.zeros((H, W), bool)

@@ -57,14 +57,21 @@ def test_alias_table_empty(mipt_mod, oracle_mod):
     assert p.size == 0 and a.size == 0
 
 
+def _draws(s, rs):
+    """Host random numbers per sample, as mipt_draws_per_sample counts them."""
+    if s.direct_light_sampling_strategy != 5:
+        return 2
+    if rs.do_fused_spatiotemporal:
+        return 3 + 3 + rs.spatial_number_of_passes - 1
+    return 3 + 2 * rs.do_temporal_reuse_pass + rs.spatial_number_of_passes * rs.do_spatial_reuse_pass
+
+
 def _render(mipt_mod, oracle_mod, scene, frames, strategy=3, base=0, **restir):
     s = mipt_mod.default_render_settings(width=W, height=H, nb_bounces=0, direct_light_sampling_strategy=strategy,
                                          direct_light_sampling_base_strategy=base)
     rs = mipt_mod.default_restir_di_settings(**restir)
     R = oracle_mod.Renderer(scene, s, rs)
-    draws = 2
-    if strategy == 5:
-        draws += 1 + 2 * rs.do_temporal_reuse_pass + rs.spatial_number_of_passes * rs.do_spatial_reuse_pass
+    draws = _draws(s, rs)
     for f in range(frames):
         R.render_frame(f, [mipt_mod.host_random_number(f * draws + k) for k in range(draws)])
     return R.fb64.reshape(-1, 3) / frames, R
@@ -193,9 +200,6 @@ def _frame_numbers(mipt_mod, draws):
     return lambda f: [mipt_mod.host_random_number(f * draws + k) for k in range(draws)]
 
 
-def _draws(s, rs):
-    return 2 + (1 + 2 * rs.do_temporal_reuse_pass + rs.spatial_number_of_passes * rs.do_spatial_reuse_pass
-                if s.direct_light_sampling_strategy == 5 else 0)
 
 
 def test_windowed_frames_equal_full_frames_inside_the_tile(mipt_mod, oracle_mod):
@@ -234,3 +238,86 @@ def test_windowed_frames_equal_full_frames_inside_the_tile(mipt_mod, oracle_mod)
         diffs[halo] = n
     assert diffs[None] == 0
     assert diffs[rs.spatial_reuse_radius - 1] > 0
+
+
+PRESAMPLED = dict(do_light_presampling=1, light_presampling_number_of_subsets=16, light_presampling_subset_size=64)
+
+
+@pytest.mark.parametrize("scene_name", ["open", "full"])
+def test_restir_fused_spatiotemporal_unbiased(mipt_mod, oracle_mod, scene48, open_scene48, scene_name):
+    """The fused spatiotemporal pass (temporal neighbour, spatial neighbours
+    of last frame's reservoirs, initial candidates, pairwise MIS with
+    confidence weights) over 256 frames averages to the MIS light + BSDF
+    image within 1.5 % per channel, with and without occluders. Dropping the
+    temporal neighbour's canonical-weight term moves the mean by 5-30 %;
+    leaving its M out of the confidence sum makes it diverge."""
+    scene = open_scene48 if scene_name == "open" else scene48
+    ref, _ = _render(mipt_mod, oracle_mod, scene, 512)
+    img, R = _render(mipt_mod, oracle_mod, scene, 256, strategy=5, do_fused_spatiotemporal=1)
+    assert np.allclose(img.mean(axis=0), ref.mean(axis=0), rtol=0.015)
+    out = R.read_reservoirs(3)
+    hit = R.gb_prims >= 0
+    assert (out["M"][hit] >= 1).all() and (out["M"] <= 20).all()
+    sel = hit & (out["ucw"] > 0)
+    assert np.isin(out["tri"][sel], R.emissive).all()
+
+
+def test_light_presampling_unbiased(mipt_mod, oracle_mod, scene48, mis_reference):
+    """Initial candidates drawn from 16 presampled subsets of 64 lights
+    (tile 8x8 per subset) average to the MIS light + BSDF image within 1 %."""
+    img, R = _render(mipt_mod, oracle_mod, scene48, 256, strategy=5, do_temporal_reuse_pass=0,
+                     do_spatial_reuse_pass=0, **PRESAMPLED)
+    assert np.allclose(img.mean(axis=0), mis_reference, rtol=0.01)
+    assert (R.read_reservoirs(3)["M"][R.gb_prims >= 0] == 1).all()
+
+
+def test_fused_first_frame_keeps_the_initial_candidates(mipt_mod, oracle_mod, scene48):
+    """On the first frame last frame's reservoirs are empty (M = 0), so the
+    fused pass resamples only the initial candidates: the output reservoir
+    holds the initial one's sample, M, target and UCW, except that an initial
+    reservoir that visibility reuse emptied (UCW -1) is not selected (sample
+    cleared, M kept) and the final visibility reuse empties the occluded
+    ones (none here: the initial pass's own visibility reuse already did)."""
+    s = mipt_mod.default_render_settings(width=W, height=H, nb_bounces=0, direct_light_sampling_strategy=5)
+    rs = mipt_mod.default_restir_di_settings(do_fused_spatiotemporal=1)
+    R = oracle_mod.Renderer(scene48, s, rs)
+    R.render_frame(0, _frame_numbers(mipt_mod, _draws(s, rs))(0))
+    init, out = R.read_reservoirs(0), R.read_reservoirs(3)
+    hit = R.gb_prims >= 0
+    live = hit & (init["ucw"] > 0)
+    dead = hit & ~live
+    assert dead.any() and live.sum() > 0.5 * hit.sum()
+    assert np.array_equal(init["M"][hit], out["M"][hit])
+    for k in ("tri", "target", "point"):
+        assert np.array_equal(init[k][live], out[k][live]), k
+    assert (out["tri"][dead] == -1).all() and (out["target"][dead] == 0).all()
+    # the initial pass's visibility reuse left only unoccluded samples, so
+    # the final one keeps them all; UCW = (1 * target * UCW) / target
+    assert np.allclose(init["ucw"][live], out["ucw"][live], rtol=2e-7, atol=0)
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+def test_windowed_fused_frames_equal_full_frames(mipt_mod, oracle_mod, passes):
+    """The windowed renderer routes the presampling and fused passes as the
+    full-frame one does: inside the tile both agree bit for bit over 3
+    frames."""
+    Wt = Ht = 96
+    scene = mipt_mod.generate_scene(1, Wt, Ht)
+    s = mipt_mod.default_render_settings(width=Wt, height=Ht, nb_bounces=1, direct_light_sampling_strategy=5)
+    rs = mipt_mod.default_restir_di_settings(spatial_reuse_radius=5, spatial_number_of_passes=passes, m_cap=20,
+                                             do_fused_spatiotemporal=1, **PRESAMPLED)
+    draws = _draws(s, rs)
+    tile = (40, 60, 36, 56)
+    full = oracle_mod.Renderer(scene, s, rs)
+    for f in range(3):
+        full.render_frame(f, _frame_numbers(mipt_mod, draws)(f))
+    win = oracle_mod.Renderer(scene, s, rs)
+    win.render_frames_in_window(3, tile, _frame_numbers(mipt_mod, draws))
+    sel = np.zeros((Ht, Wt), bool)
+    sel[tile[2]:tile[3], tile[0]:tile[1]] = True
+    sel = sel.reshape(-1)
+    assert (win.read_reservoirs(0)["M"][~sel] == 0).any()
+    for which in (0, 1, 2, 3):
+        a, b = win.read_reservoirs(which)[sel], full.read_reservoirs(which)[sel]
+        assert a.tobytes() == b.tobytes(), which
+    assert win.fb.reshape(-1, 3)[sel].tobytes() == full.fb.reshape(-1, 3)[sel].tobytes()
