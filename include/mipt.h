@@ -385,8 +385,9 @@ void mipt_render_settings_default(mipt_render_settings* s);
  * 17-163) and its ReSTIRCommonSettings (ReSTIRCommonSettings.h). The kernel
  * options of ReSTIRDIOptions.h are fixed at the reference defaults:
  * pairwise MIS bias correction with visibility, visibility reuse, no
- * visibility in the initial or spatial target functions, no light
- * presampling. Later bounces use MIS light + BSDF
+ * visibility in the initial or spatial target functions, presampled lights
+ * drawn by power (when do_light_presampling is on). Later bounces use MIS
+ * light + BSDF
  * (RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF). Defaults:
  * mipt_restir_di_settings_default (the reference's). */
 typedef struct mipt_restir_di_settings {
@@ -490,6 +491,59 @@ int mipt_get_power_alias_table(mipt_ctx* ctx, float* probas, int32_t* alias, flo
 /* Vose alias table of n non-negative weights, normalised in FP64
  * (Utils::compute_alias_table, Utils.cpp:63-131). Host utility. */
 int mipt_compute_alias_table(const float* weights, int32_t n, float total_sum, float* out_probas, int32_t* out_alias);
+
+/* ------------------------------------------------------------------ */
+/* Ambient light: WorldSettings (src/HostDeviceCommon/WorldSettings.h: */
+/* 14-53). A ray that leaves the scene adds the uniform colour at any  */
+/* bounce; an environment map is seen by camera rays and otherwise     */
+/* importance sampled at every vertex (alias table over the texels'    */
+/* luminance, Envmap.h:77-133) with BSDF MIS (Envmap.h:168-255).       */
+/* ------------------------------------------------------------------ */
+#define MIPT_AMBIENT_NONE 0    /* AmbientLightType::NONE */
+#define MIPT_AMBIENT_UNIFORM 1 /* AmbientLightType::UNIFORM (the default, black) */
+#define MIPT_AMBIENT_ENVMAP 2  /* AmbientLightType::ENVMAP (needs mipt_set_envmap) */
+
+typedef struct mipt_world_settings {
+    int32_t ambient_light_type;                /* :23, default MIPT_AMBIENT_UNIFORM */
+    float uniform_light_color[3];              /* :24, default 0 */
+    float envmap_intensity;                    /* :30, default 1 */
+    /* :35, default 0: camera rays see the map without the intensity */
+    int32_t envmap_scale_background_intensity;
+    /* :51-52: float3x3 as [column][row] (glm layout), default identity;
+     * world_to_envmap is the inverse the caller computed
+     * (RendererEnvmap::do_animation, RendererEnvmap.cpp:127-163). */
+    float envmap_to_world[3][3];
+    float world_to_envmap[3][3];
+} mipt_world_settings;
+
+void mipt_world_settings_default(mipt_world_settings* w);
+/* Takes effect at the next pass. MIPT_AMBIENT_ENVMAP needs an envmap set
+ * with mipt_set_envmap. ReSTIR DI and the radiance cache with an envmap are
+ * refused at launch (MIPT_ERR_UNSUPPORTED). */
+int mipt_set_world_settings(mipt_ctx* ctx, const mipt_world_settings* w);
+int mipt_get_world_settings(const mipt_ctx* ctx, mipt_world_settings* out);
+
+/* GPURenderer::set_envmap (GPURenderer.cpp:962-998) for an Image32Bit:
+ * `pixels` holds width x height texels of `channels` floats (3 or 4; only
+ * the first three are read), row y at v = y / height, the first row looking
+ * down (-Y). The texels are packed to R9G9B9E5 on the GPU
+ * (RGBE9995Envmap::pack_from, EnvmapRGBE9995.h:21-44); the alias table over
+ * the luminance of the float texels and their sum are built on the host
+ * (Image32Bit::compute_alias_table, Image.cpp:652-672). Sets the ambient
+ * light type to MIPT_AMBIENT_ENVMAP; an empty image (width or height 0)
+ * turns an ENVMAP ambient light to UNIFORM, as the reference does. */
+int mipt_set_envmap(mipt_ctx* ctx, const float* pixels, int32_t width, int32_t height, int32_t channels);
+/* Host copies of the envmap's alias table (width x height entries each,
+ * either may be NULL), its luminance sum, and the packed texels. */
+int mipt_get_envmap_tables(const mipt_ctx* ctx, float* probas, int32_t* alias, float* luminance_sum,
+                           uint32_t* packed_texels);
+
+/* Procedural HDR environment map (no counterpart in the reference, which
+ * loads .hdr/.exr files): a sky gradient over a dim ground, each texel
+ * scaled by a lognormal factor (sigma 0.35) from PCG32 stream 5, and a sun
+ * disk of 1.5 degrees at 35 degrees elevation; width x height x 3 floats in
+ * mipt_set_envmap's layout. 4096 x 2048 is the full-stack configuration. */
+int mipt_envmap_generate(int32_t width, int32_t height, float* out_rgb);
 
 /* ------------------------------------------------------------------ */
 /* Learned radiance cache. No counterpart in the reference: a neural   */
