@@ -225,6 +225,9 @@ void allocate_render_buffers(Ctx& ctx)
     HIP_CHECK(hipMemset(R.gb_sn.ptr, 0, 4 * n));
     HIP_CHECK(hipMemset(R.gb_pos.ptr, 0, 16 * n));
     allocate_restir_buffers(ctx);
+    // the shadow buffers hold one slot per pixel, more with envmap sampling
+    R.shadow_slots = 1;
+    allocate_env_path_buffers(ctx);
     if (restir_needs_prev_gbuffer(R))
     {
         HIP_CHECK(hipMemset(R.pg_prim.ptr, 0xff, 4 * n));
@@ -433,6 +436,7 @@ int mipt_create(int hip_device, mipt_ctx** out)
     mipt::noalloc_scratch_init(hip_device);
     mipt_render_settings_default(&ctx->render.settings);
     mipt_restir_di_settings_default(&ctx->render.restir);
+    mipt_world_settings_default(&ctx->world);
     *out = reinterpret_cast<mipt_ctx*>(ctx);
     return MIPT_OK;
 }
@@ -964,6 +968,11 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         // a dispersive glass draws a wavelength from the pass's stream at
         // every surface fetch (ReSTIR/Surface.h:25-43); those draws are not
         // implemented.
+        if (mipt::env_sampling_on(*ctx) &&
+            (ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI ||
+             ctx->render.settings.radiance_cache))
+            mipt::fail(MIPT_ERR_UNSUPPORTED, "an environment map with ReSTIR DI (envmap candidates) or with the "
+                                             "radiance cache is not implemented yet");
         if (ctx->render.scene_dispersive && ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
             mipt::fail(MIPT_ERR_UNSUPPORTED, "ReSTIR DI with a dispersive glass material (dispersion_scale > 0 and "
                                              "specular_transmission > 0) is not implemented yet");
@@ -1108,6 +1117,78 @@ int mipt_get_power_alias_table(mipt_ctx* c, float* probas, int32_t* alias, float
             std::memcpy(alias, R.alias_alias_host.data(), sizeof(int32_t) * R.alias_alias_host.size());
         if (sum)
             *sum = R.alias_sum;
+    });
+}
+
+void mipt_world_settings_default(mipt_world_settings* w)
+{
+    if (!w)
+        return;
+    std::memset(w, 0, sizeof(*w));
+    w->ambient_light_type = MIPT_AMBIENT_UNIFORM;
+    w->envmap_intensity = 1.0f;
+    for (int k = 0; k < 3; k++)
+    {
+        w->envmap_to_world[k][k] = 1.0f;
+        w->world_to_envmap[k][k] = 1.0f;
+    }
+}
+
+int mipt_set_world_settings(mipt_ctx* c, const mipt_world_settings* w)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !w || w->ambient_light_type < MIPT_AMBIENT_NONE || w->ambient_light_type > MIPT_AMBIENT_ENVMAP ||
+        !(w->envmap_intensity >= 0.0f))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (w->ambient_light_type == MIPT_AMBIENT_ENVMAP && ctx->env_width == 0)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "MIPT_AMBIENT_ENVMAP: call mipt_set_envmap first");
+        HIP_CHECK(hipDeviceSynchronize()); // queued passes read the previous settings by value only
+        ctx->world = *w;
+        ctx->render.graph_dirty = true;
+        mipt::allocate_env_path_buffers(*ctx);
+    });
+}
+
+int mipt_get_world_settings(const mipt_ctx* c, mipt_world_settings* out)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    if (!ctx || !out)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    *out = ctx->world;
+    return MIPT_OK;
+}
+
+int mipt_set_envmap(mipt_ctx* c, const float* pixels, int32_t width, int32_t height, int32_t channels)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        HIP_CHECK(hipDeviceSynchronize()); // queued passes may read the old texels
+        ctx->render.graph_dirty = true;
+        mipt::set_envmap(*ctx, pixels, width, height, channels);
+    });
+}
+
+int mipt_get_envmap_tables(const mipt_ctx* c, float* probas, int32_t* alias, float* luminance_sum,
+                           uint32_t* packed_texels)
+{
+    const Ctx* ctx = reinterpret_cast<const Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(const_cast<Ctx*>(ctx), [&] {
+        if (ctx->env_width == 0)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "no envmap: call mipt_set_envmap first");
+        const size_t count = (size_t)ctx->env_width * (size_t)ctx->env_height;
+        if (probas)
+            std::memcpy(probas, ctx->env_probas_host.data(), 4 * count);
+        if (alias)
+            std::memcpy(alias, ctx->env_alias_host.data(), 4 * count);
+        if (luminance_sum)
+            *luminance_sum = ctx->env_sum;
+        if (packed_texels)
+            HIP_CHECK(hipMemcpy(packed_texels, ctx->env_texels.ptr, 4 * count, hipMemcpyDeviceToHost));
     });
 }
 

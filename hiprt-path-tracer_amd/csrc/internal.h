@@ -134,6 +134,10 @@ struct RenderState {
     DeviceBuffer vol_a, vol_b;    // volume state of the paths that keep one (uint4 each)
     // rays of one bounce, and the queues of live slots the traversal takes
     DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
+    // envmap sampling: the light sample's and the BSDF sample's pending
+    // contributions; their shadow rays take shadow slots n + pix and 2n + pix
+    DeviceBuffer env_light, env_bsdf;
+    uint32_t shadow_slots = 1;  // shadow ray slots per pixel the buffers hold
     DeviceBuffer q_shadow, q_bsdf, q_cont, q_skip, q_live, q_counts; // q_skip, q_live: two lists back to back
     // framebuffer
     DeviceBuffer fb, fb64, counters;
@@ -204,6 +208,15 @@ struct Ctx {
     Bvh bvh;
     TraceConfig trace_cfg;
 
+    // Ambient light (mipt_set_world_settings / mipt_set_envmap)
+    mipt_world_settings world{};
+    int32_t env_width = 0, env_height = 0;
+    float env_sum = 0.0f;             // luminance sum of the float texels
+    DeviceBuffer env_texels;          // uint32 R9G9B9E5 [env_width * env_height]
+    DeviceBuffer env_probas, env_alias;
+    std::vector<float> env_probas_host;
+    std::vector<int32_t> env_alias_host;
+
     // Traversal spill stack (entries beyond the LDS stack), sized on demand.
     DeviceBuffer spill;
     DeviceBuffer autotune_hits; // hit records of the autotuner's timed launches
@@ -231,6 +244,11 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
                      bool warmup = false, const FrameParams* frame = nullptr);
 float consistent_tau(float tau_0, uint32_t sample_number);
 void write_frame_params(FrameParams* frame, const FrameParams& values, hipStream_t stream);
+
+// envmap.hip: the ambient light's envmap and its device view.
+void set_envmap(Ctx& ctx, const float* pixels, int32_t width, int32_t height, int32_t channels);
+bool env_sampling_on(const Ctx& ctx); // ENVMAP ambient light with intensity > 0
+void allocate_env_path_buffers(Ctx& ctx);
 
 // lights.cpp: Utils::compute_alias_table (src/Utils/Utils.cpp:63-131) and the
 // emissive power list (GPURenderer.cpp:221-295).

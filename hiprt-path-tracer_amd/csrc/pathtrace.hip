@@ -48,6 +48,12 @@ constexpr uint32_t FLAG_SKIP = 64u;      // continuation ray goes on past a skip
 // 1e-6 (Beer-Lambert); the reference evaluates it only when the shadow ray is
 // unoccluded, so the floor is applied after the shadow ray, at the next vertex.
 constexpr uint32_t FLAG_NEE_FLOOR = 128u;
+// Envmap sampling (Envmap.h:168-255): the envmap light sample's shadow ray
+// (slot n + pix) and the MIS BSDF sample's (slot 2n + pix) are pending; the
+// light sample's evaluation floored the medium distance (as FLAG_NEE_FLOOR).
+constexpr uint32_t FLAG_ENV_LIGHT = 256u;
+constexpr uint32_t FLAG_ENV_BSDF = 512u;
+constexpr uint32_t FLAG_ENV_FLOOR = 1024u;
 
 // Slots of the live rays of one bounce, per ray kind (segmented lists), and
 // the two lists the boundary-skip rounds alternate between.
@@ -154,14 +160,7 @@ __device__ __forceinline__ bool init_path(const PtData& d, uint32_t pix)
     d.throughput[pix] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     d.radiance[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const int32_t prim = d.gb_prim[pix];
-    if (prim < 0)
-    {
-        d.flags[pix] = 0u; // camera ray missed: the scenes have no envmap
-        return false;
-    }
     const f3 p = ld3(d.gb_pos[pix]);
-    const f3 gn = gb_normal(d.gb_gn[pix]);
-    const f3 sn = gb_normal(d.gb_sn[pix]);
     f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
     if (d.views)
     {
@@ -169,8 +168,19 @@ __device__ __forceinline__ bool init_path(const PtData& d, uint32_t pix)
         cam = pt::mk(v.position[0], v.position[1], v.position[2]);
     }
     // GBufferDevice.h:19-22: view = normalize(camera - hit); the path ray is
-    // normalize(-view) and the BSDF view direction is -ray.direction.
+    // normalize(-view) and the BSDF view direction is -ray.direction. A pixel
+    // whose camera ray missed holds origin + direction as its hit point.
     const f3 ray_dir = pt::normalize(-1.0f * pt::normalize(cam - p));
+    if (prim < 0)
+    {
+        // path_tracing_miss_gather_envmap at bounce 0 (PathTracing.h:121-170)
+        const Color bg = env_miss(d.env, ray_dir, 0) * pt::col(1.0f);
+        d.radiance[pix] = stc(bg, 0.0f);
+        d.flags[pix] = 0u;
+        return false;
+    }
+    const f3 gn = gb_normal(d.gb_gn[pix]);
+    const f3 sn = gb_normal(d.gb_sn[pix]);
     d.vtx_pos[pix] = st3(p, 0.0f);
     d.vtx_sn[pix] = st3(sn, __int_as_float(prim));
     d.vtx_gn[pix] = st3(gn, __int_as_float(d.gb_mat[pix]));
@@ -273,7 +283,8 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
 // Lights.h:123-226 (NEE with MIS), Lights.h:261-294 (ReSTIR DI at bounce 0)
 // and PathTracing.h:81-101 (next bounce + Russian roulette).
 __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, uint32_t pix, uint32_t flags,
-                                             bool& shadow_active, bool& bsdf_active, bool& cont)
+                                             bool& shadow_active, bool& bsdf_active, bool& cont, bool& env_light_active,
+                                             bool& env_bsdf_active)
 {
     const bool stored_volume = (flags & FLAG_VOL) != 0u;
     flags &= FLAG_ALIVE;
@@ -366,6 +377,55 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
             flags |= FLAG_NEE_BSDF;
         }
     }
+    // sample_environment_map (Envmap.h:168-276): after the emissive
+    // geometry, and at bounce 0 only without ReSTIR DI (which would sample
+    // the envmap among its candidates)
+    bool env_floor = false;
+    if ((bounce > 0 || d.enable_direct) && env_sampled(d.env) && !(bounce == 0 && d.restir))
+    {
+        const uint32_t n = (uint32_t)(d.width * d.height);
+        f3 edir;
+        float epdf;
+        const Color ecol = env_sample(d.env, edir, epdf, rng);
+        // can_do_light_sampling is true with microfacet regularisation
+        // (MaterialUtils.h:71-78)
+        const float ecos = pt::dot(sn, edir);
+        Color env_light = pt::col(0.0f);
+        if (epdf > 0.0f && ecos > 0.0f)
+        {
+            pt::VolumeState vs_env = vs;
+            pt::BsdfCtx c{view, sn, gn, edir, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness, &vs_env, 0};
+            float bsdf_pdf;
+            const Color f = pt::principled_eval(frame_globals(d), m, c, bsdf_pdf);
+            const float w = balance_heuristic(epdf, 1.0f, bsdf_pdf, 1.0f);
+            env_light = f * ecos * w * ecol / epdf / 1.0f;
+            env_floor = vs_env.distance != vs.distance;
+            write_ray(d.shadow_rays, n + pix, p, prim, edir, 1.0e35f - 1.0e-4f);
+            env_light_active = true;
+            flags |= FLAG_ENV_LIGHT;
+        }
+        d.env_light[pix] = stc(env_light, 0.0f);
+        pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness, &vs,
+                      0};
+        f3 bdir;
+        float bpdf;
+        const Color f = pt::principled_sample(frame_globals(d), m, c, bdir, bpdf, rng);
+        const float bcos = fabsf(pt::dot(sn, bdir));
+        if (bpdf > 0.0f)
+        {
+            // envmap_eval needs no ray: only its visibility is pending
+            float env_pdf;
+            const Color radiance = env_eval(d.env, bdir, env_pdf);
+            if (env_pdf > 0.0f)
+            {
+                const float w = balance_heuristic(bpdf, 1.0f, env_pdf, 1.0f);
+                d.env_bsdf[pix] = stc(radiance * w * bcos * f / bpdf, 0.0f);
+                write_ray(d.shadow_rays, 2 * n + pix, p, prim, bdir, 1.0e35f - 1.0e-4f);
+                env_bsdf_active = true;
+                flags |= FLAG_ENV_BSDF;
+            }
+        }
+    }
     if (!shadow_active)
         d.shadow_occ[pix] = 0; // a contribution without a shadow ray is unoccluded
 
@@ -428,7 +488,9 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
     // the canonical state and keep nothing in memory.
     if (light_floor && !vs.reset)
         flags |= FLAG_NEE_FLOOR;
-    if (!pt::vol_canonical(vs) || (flags & FLAG_NEE_FLOOR))
+    if (env_floor && !vs.reset)
+        flags |= FLAG_ENV_FLOOR;
+    if (!pt::vol_canonical(vs) || (flags & (FLAG_NEE_FLOOR | FLAG_ENV_FLOOR)))
     {
         vol_store(d, pix, vs);
         flags |= FLAG_VOL;
@@ -559,13 +621,16 @@ __global__ __launch_bounds__(256, 2) void k_shade(PtData d, int32_t bounce, RayQ
         return;
     const uint32_t pix = sorted && li.valid ? sorted[blockIdx.x * blockDim.x + threadIdx.x] : li.pix;
     const uint32_t flags = li.valid ? d.flags[pix] : 0u;
-    bool shadow_active = false, bsdf_active = false, cont = false;
+    bool shadow_active = false, bsdf_active = false, cont = false, env_light = false, env_bsdf = false;
     if (flags & FLAG_ALIVE)
     {
         thr_prev[pix] = d.throughput[pix];
-        shade_vertex(d, bounce, pix, flags, shadow_active, bsdf_active, cont);
+        shade_vertex(d, bounce, pix, flags, shadow_active, bsdf_active, cont, env_light, env_bsdf);
     }
+    const uint32_t n = (uint32_t)(d.width * d.height);
     seg_append(q.shadow, shadow_active, pix);
+    seg_append(q.shadow, env_light, n + pix);
+    seg_append(q.shadow, env_bsdf, 2 * n + pix);
     seg_append(q.bsdf, bsdf_active, pix);
     seg_append(q.cont, cont, pix);
 }
@@ -581,13 +646,20 @@ enum : int { PATH_ENDED = 0, PATH_AT_VERTEX = 1, PATH_SKIPPING = 2 };
 __device__ __forceinline__ int next_vertex(const PtData& d, uint32_t pix, uint32_t flags)
 {
     const int4 h = d.cont_hits[pix];
-    if (h.x < 0)
-    {
-        d.flags[pix] = 0u; // miss: no envmap contribution
-        return PATH_ENDED;
-    }
     const float4 ra = d.cont_rays[2 * (size_t)pix], rb = d.cont_rays[2 * (size_t)pix + 1];
     const f3 o = pt::mk(ra.x, ra.y, ra.z), dir = pt::mk(rb.x, rb.y, rb.z);
+    if (h.x < 0)
+    {
+        // path_tracing_miss_gather_envmap (PathTracing.h:121-170) at a bounce
+        // > 0: the uniform ambient colour (a sampled envmap adds nothing)
+        if (d.env.type == MIPT_AMBIENT_UNIFORM)
+        {
+            const Color add = env_miss(d.env, dir, 1) * ldc(d.throughput[pix]);
+            d.radiance[pix] = stc(ldc(d.radiance[pix]) + add, 0.0f);
+        }
+        d.flags[pix] = 0u;
+        return PATH_ENDED;
+    }
     const float t = __int_as_float(h.y);
     f3 p, gn, sn;
     hit_attributes(d, o, dir, h.x, t, __int_as_float(h.z), __int_as_float(h.w), p, gn, sn);
@@ -598,6 +670,8 @@ __device__ __forceinline__ int next_vertex(const PtData& d, uint32_t pix, uint32
     {
         vol_load(d, pix, vs);
         if ((flags & FLAG_NEE_FLOOR) && !d.shadow_occ[pix])
+            vs.distance = pt::maxf_(vs.distance, 1.0e-6f);
+        if ((flags & FLAG_ENV_FLOOR) && !d.shadow_occ[(size_t)(d.width * d.height) + pix])
             vs.distance = pt::maxf_(vs.distance, 1.0e-6f);
         const bool skip = pt::vol_push(vs, mi, d.mats[mi].dielectric_priority);
         if (vs.inside)
@@ -676,10 +750,18 @@ __device__ __forceinline__ int resolve_path(const PtData& d, int32_t bounce, con
             }
         }
         const Color direct = light_mis + bsdf_mis;
+        // the envmap's BSDF-sampled part, then its light-sampled part
+        // (Envmap.h:253)
+        const size_t n = (size_t)(d.width * d.height);
+        Color env_bsdf = pt::col(0.0f), env_light = pt::col(0.0f);
+        if ((flags & FLAG_ENV_BSDF) && !d.shadow_occ[2 * n + pix])
+            env_bsdf = ldc(d.env_bsdf[pix]);
+        if ((flags & FLAG_ENV_LIGHT) && !d.shadow_occ[n + pix])
+            env_light = ldc(d.env_light[pix]);
         Color total = pt::col(0.0f);
         if (bounce == 0)
             total = total + d.mats[__float_as_int(d.vtx_gn[pix].w)].emission;
-        total = total + ((direct / 1.0f) + pt::col(0.0f)) * ldc(thr_prev[pix]);
+        total = total + ((direct / 1.0f) + (env_bsdf + env_light)) * ldc(thr_prev[pix]);
         radiance = radiance + total;
         d.radiance[pix] = stc(radiance, 0.0f);
     }
@@ -1019,6 +1101,9 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.bsdf_hits = R.bsdf_hits.as<int4>();
     d.cont_rays = R.cont_rays.as<float4>();
     d.cont_hits = R.cont_hits.as<int4>();
+    d.env = make_env_dev(ctx);
+    d.env_light = R.env_light.as<float4>();
+    d.env_bsdf = R.env_bsdf.as<float4>();
     d.restir = R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI;
     d.final_shading_visibility = R.restir.do_final_shading_visibility;
     d.restir_out = d.restir ? restir_reservoir_buffer(ctx, R.restir_shade_buffer) : nullptr;
@@ -1100,7 +1185,12 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
     uint32_t* qc = R.q_counts.as<uint32_t>();
     const size_t qn = (size_t)SEG_COUNT * seg_cap;
     uint32_t* qskip = R.q_skip.as<uint32_t>();
-    const RayQueues q{SegList{R.q_shadow.as<uint32_t>(), qc, seg_cap},
+    // shadow rays: the light sample's, and with envmap sampling two more per
+    // path (k_shade appends up to three times per thread)
+    const uint32_t shadow_slots = env_sampling_on(ctx) ? 3u : 1u;
+    if (R.shadow_slots < shadow_slots)
+        fail(MIPT_ERR_INVALID_ARGUMENT, "envmap sampling: the shadow ray buffers were not sized for it");
+    const RayQueues q{SegList{R.q_shadow.as<uint32_t>(), qc, shadow_slots * seg_cap},
                       SegList{R.q_bsdf.as<uint32_t>(), qc + SEG_COUNTER_WORDS, seg_cap},
                       SegList{R.q_cont.as<uint32_t>(), qc + 2 * SEG_COUNTER_WORDS, seg_cap},
                       {SegList{qskip, qc + 3 * SEG_COUNTER_WORDS, seg_cap},
@@ -1157,7 +1247,8 @@ void path_trace_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, h
         const bool cont_rays = bounce < R.settings.nb_bounces;
         if (bounce == 0)
         {
-            trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow, n, s);
+            trace_any_queue(ctx, R.shadow_rays.as<mipt_ray>(), R.shadow_occ.as<uint8_t>(), q.shadow,
+                            shadow_slots * n, s);
             trace_closest_queue(ctx, R.bsdf_rays.as<mipt_ray>(), R.bsdf_hits.as<mipt_hit>(), q.bsdf, n, s);
             if (cont_rays)
                 trace_closest_queue(ctx, R.cont_rays.as<mipt_ray>(), R.cont_hits.as<mipt_hit>(), q.cont, n, s);

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "envmap.h"
 #include "internal.h"
 #include "shading.h"
 
@@ -82,6 +83,12 @@ struct PtData {
     int4* bsdf_hits;
     float4* cont_rays;
     int4* cont_hits;
+    // ambient light; with envmap sampling, the pending contributions of the
+    // envmap light sample and of its MIS BSDF sample (shadow slots n + pix,
+    // 2n + pix)
+    EnvDev env;
+    float4* env_light;
+    float4* env_bsdf;
     // ReSTIR DI at bounce 0 (LSS_RESTIR_DI): the reservoirs to shade
     int32_t restir;
     int32_t final_shading_visibility;

@@ -165,6 +165,59 @@ PT_HD float atan2_fixed(float y, float x)
     return signbit(y) ? -r : r;
 }
 
+// asin and acos as the angle of a right triangle with legs x and
+// sqrt((1 - x)(1 + x)), through atan2_fixed; the argument is clamped to
+// [-1, 1] first (a normalised direction can exceed 1 by an ulp).
+PT_HD float asin_fixed(float y)
+{
+    y = fminf(fmaxf(y, -1.0f), 1.0f);
+    return atan2_fixed(y, sqrtf((1.0f - y) * (1.0f + y)));
+}
+PT_HD float acos_fixed(float x)
+{
+    x = fminf(fmaxf(x, -1.0f), 1.0f);
+    return atan2_fixed(sqrtf((1.0f - x) * (1.0f + x)), x);
+}
+
+// ---------------------------------------------------------------------------
+// Shared-exponent RGB, R9G9B9E5 (RGBE9995Packed, src/HostDeviceCommon/
+// Packing.h:441-485): channel q in [0, 511] in bits 0-8 (r), 9-17 (g), 18-26
+// (b), exponent E in bits 27-31, value q 2^(E - 24). Channels are clamped to
+// [0, 511/256 * 2^15]; the largest, floored at 2^-16, has binary exponent e,
+// and E = e + 16, one more when that channel rounds up to 512 units; each
+// channel is then q = round-half-even(c 2^(24 - E)) (a scale by a power of
+// two, exact).
+// ---------------------------------------------------------------------------
+PT_HD uint32_t rgb9e5_pack(float r, float g, float b)
+{
+    const float hi = 0x1.ffp+15f;
+    r = fminf(fmaxf(r, 0.0f), hi);
+    g = fminf(fmaxf(g, 0.0f), hi);
+    b = fminf(fmaxf(b, 0.0f), hi);
+    const float m = fmaxf(fmaxf(0x1p-16f, r), fmaxf(g, b));
+    uint32_t bits;
+    __builtin_memcpy(&bits, &m, 4);
+    int32_t E = (int32_t)(bits >> 23) - 127 + 16;
+    // 0x7fc000: mantissa at or above 511.5 / 256 of the binade rounds to 512
+    if ((bits & 0x7fffffu) >= 0x7fc000u)
+        E += 1;
+    const uint32_t scale_bits = (uint32_t)(24 - E + 127) << 23;
+    float scale;
+    __builtin_memcpy(&scale, &scale_bits, 4);
+    const uint32_t qr = (uint32_t)rintf(r * scale), qg = (uint32_t)rintf(g * scale), qb = (uint32_t)rintf(b * scale);
+    return ((uint32_t)E << 27) | (qb << 18) | (qg << 9) | qr;
+}
+
+PT_HD void rgb9e5_unpack(uint32_t p, float& r, float& g, float& b)
+{
+    const uint32_t scale_bits = (uint32_t)((int32_t)(p >> 27) - 24 + 127) << 23;
+    float scale;
+    __builtin_memcpy(&scale, &scale_bits, 4);
+    r = (float)(p & 0x1ffu) * scale;
+    g = (float)((p >> 9) & 0x1ffu) * scale;
+    b = (float)((p >> 18) & 0x1ffu) * scale;
+}
+
 // ---------------------------------------------------------------------------
 // RNG: the reference's per-pixel streams (src/Device/includes/Hash.h:11-19,
 // src/HostDeviceCommon/Xorshift.h:18-66).

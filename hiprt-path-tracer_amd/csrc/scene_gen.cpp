@@ -473,6 +473,61 @@ extern "C" int mipt_views_generate(int32_t count, mipt_view* out)
     return MIPT_OK;
 }
 
+extern "C" int mipt_envmap_generate(int32_t width, int32_t height, float* out_rgb)
+{
+    if (width <= 0 || height <= 0 || !out_rgb || (int64_t)width * height > (1ll << 26))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    mipt::Pcg32 rng(0x2026120200000000ULL + 5, 5);
+    // the sun: 35 degrees above the horizon, azimuth 60 degrees
+    const double elev = 35.0 * M_PI / 180.0, azim = 60.0 * M_PI / 180.0;
+    const mipt::V3 sun{std::cos(elev) * std::cos(azim), std::sin(elev), std::cos(elev) * std::sin(azim)};
+    const double sun_cos = std::cos(1.5 * M_PI / 180.0);
+    for (int32_t y = 0; y < height; y++)
+    {
+        // the direction of the texel's centre in the sampler's convention:
+        // theta from straight down, phi around Y
+        const double theta = (y + 0.5) / height * M_PI;
+        for (int32_t x = 0; x < width; x++)
+        {
+            const double phi = (x + 0.5) / width * 2.0 * M_PI;
+            const mipt::V3 d{-std::sin(theta) * std::cos(phi), -std::cos(theta), -std::sin(theta) * std::sin(phi)};
+            double r, g, b;
+            if (d.y > 0.0)
+            {
+                // zenith blue to a pale horizon
+                const double t = std::pow(1.0 - d.y, 4.0);
+                r = 0.25 + 0.65 * t;
+                g = 0.45 + 0.45 * t;
+                b = 0.90 + 0.10 * t;
+            }
+            else
+            {
+                r = 0.20;
+                g = 0.17;
+                b = 0.14;
+            }
+            // lognormal texel factor exp(0.35 N), N by Box-Muller
+            const double u1 = 1.0 - rng.uniform(), u2 = rng.uniform();
+            const double n01 = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+            const double k = std::exp(0.35 * n01);
+            r *= k;
+            g *= k;
+            b *= k;
+            if (mipt::dot(d, sun) >= sun_cos)
+            {
+                r = 2000.0;
+                g = 1800.0;
+                b = 1500.0;
+            }
+            float* o = out_rgb + 3 * ((size_t)y * (size_t)width + (size_t)x);
+            o[0] = (float)r;
+            o[1] = (float)g;
+            o[2] = (float)b;
+        }
+    }
+    return MIPT_OK;
+}
+
 extern "C" void mipt_scene_free(mipt_generated_scene* g)
 {
     if (!g)

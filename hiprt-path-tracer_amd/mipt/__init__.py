@@ -176,6 +176,21 @@ MIPT_LSS_RESTIR_DI = 5
 MIPT_LSS_BASE_UNIFORM = 0
 MIPT_LSS_BASE_POWER = 1
 
+MIPT_AMBIENT_NONE = 0
+MIPT_AMBIENT_UNIFORM = 1
+MIPT_AMBIENT_ENVMAP = 2
+
+
+class WorldSettings(C.Structure):
+    """mipt_world_settings (WorldSettings.h:21-53)."""
+
+    _fields_ = [
+        ("ambient_light_type", C.c_int32), ("uniform_light_color", C.c_float * 3), ("envmap_intensity", C.c_float),
+        ("envmap_scale_background_intensity", C.c_int32), ("envmap_to_world", (C.c_float * 3) * 3),
+        ("world_to_envmap", (C.c_float * 3) * 3),
+    ]
+
+
 # ReSTIR DI reservoir record as read back by mipt_read_restir_reservoirs.
 RESERVOIR_DTYPE = np.dtype([("point", np.float32, 3), ("target", np.float32), ("M", np.int32),
                             ("tri", np.int32), ("ucw", np.float32), ("flags", np.int32)])
@@ -234,6 +249,12 @@ _SIGNATURES = {
     "mipt_set_restir_di_settings": (C.c_int, [C.c_void_p, C.POINTER(RestirDISettings)]),
     "mipt_draws_per_sample": (C.c_int32, [C.c_void_p]),
     "mipt_get_power_alias_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "mipt_world_settings_default": (None, [C.POINTER(WorldSettings)]),
+    "mipt_set_world_settings": (C.c_int, [C.c_void_p, C.POINTER(WorldSettings)]),
+    "mipt_get_world_settings": (C.c_int, [C.c_void_p, C.POINTER(WorldSettings)]),
+    "mipt_set_envmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "mipt_get_envmap_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
+    "mipt_envmap_generate": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p]),
     "mipt_compute_alias_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p]),
     "mipt_read_restir_reservoirs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mipt_radiance_cache_settings_default": (None, [C.POINTER(RadianceCacheSettings)]),
@@ -298,6 +319,30 @@ def default_restir_di_settings(**overrides) -> RestirDISettings:
     for k, v in overrides.items():
         setattr(rs, k, v)
     return rs
+
+
+def default_world_settings(**overrides) -> WorldSettings:
+    w = WorldSettings()
+    load_library().mipt_world_settings_default(C.byref(w))
+    for k, v in overrides.items():
+        if k == "uniform_light_color":
+            w.uniform_light_color[:] = [float(x) for x in v]
+        elif k in ("envmap_to_world", "world_to_envmap"):
+            m = np.asarray(v, np.float32).reshape(3, 3)
+            for c in range(3):
+                getattr(w, k)[c][:] = [float(x) for x in m[c]]
+        else:
+            setattr(w, k, v)
+    return w
+
+
+def generate_envmap(width: int, height: int) -> np.ndarray:
+    """The procedural sun + sky envmap (mipt_envmap_generate), height x width x 3."""
+    out = np.zeros((height, width, 3), np.float32)
+    rc = load_library().mipt_envmap_generate(width, height, out.ctypes.data_as(C.c_void_p))
+    if rc != MIPT_OK:
+        raise MiptError(rc, "mipt_envmap_generate")
+    return out
 
 
 def compute_alias_table(weights, total_sum=None):
@@ -630,6 +675,35 @@ class Context:
                                                         alias.ctypes.data_as(C.c_void_p), C.byref(total)),
                     "mipt_get_power_alias_table")
         return probas, alias, total.value
+
+    # Ambient light
+    def set_world_settings(self, world: WorldSettings):
+        self._check(self.lib.mipt_set_world_settings(self.h, C.byref(world)), "mipt_set_world_settings")
+
+    def world_settings(self) -> WorldSettings:
+        w = WorldSettings()
+        self._check(self.lib.mipt_get_world_settings(self.h, C.byref(w)), "mipt_get_world_settings")
+        return w
+
+    def set_envmap(self, image):
+        """`image`: height x width x channels float32 (3 or 4 channels), or None to clear."""
+        if image is None:
+            self._check(self.lib.mipt_set_envmap(self.h, None, 0, 0, 3), "mipt_set_envmap")
+            return
+        img = np.ascontiguousarray(image, np.float32)
+        h, w, c = img.shape
+        self._check(self.lib.mipt_set_envmap(self.h, img.ctypes.data_as(C.c_void_p), w, h, c), "mipt_set_envmap")
+
+    def envmap_tables(self, count: int):
+        """(alias probabilities, aliases, luminance sum, packed R9G9B9E5 texels)."""
+        probas = np.zeros(count, np.float32)
+        alias = np.zeros(count, np.int32)
+        texels = np.zeros(count, np.uint32)
+        total = C.c_float()
+        self._check(self.lib.mipt_get_envmap_tables(self.h, probas.ctypes.data_as(C.c_void_p),
+                                                    alias.ctypes.data_as(C.c_void_p), C.byref(total),
+                                                    texels.ctypes.data_as(C.c_void_p)), "mipt_get_envmap_tables")
+        return probas, alias, total.value, texels
 
     def read_reservoirs(self, which: int = 3) -> np.ndarray:
         n = self.settings.width * self.settings.height

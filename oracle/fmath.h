@@ -130,4 +130,18 @@ static inline float orc_atan2f(float y, float x)
     return signbit(y) ? -r : r;
 }
 
+/* asin and acos through atan2 of the legs x and sqrt((1 - x)(1 + x)), the
+ * argument clamped to [-1, 1]. */
+static inline float orc_asinf(float y)
+{
+    y = fminf(fmaxf(y, -1.0f), 1.0f);
+    return orc_atan2f(y, sqrtf((1.0f - y) * (1.0f + y)));
+}
+
+static inline float orc_acosf(float x)
+{
+    x = fminf(fmaxf(x, -1.0f), 1.0f);
+    return orc_atan2f(sqrtf((1.0f - x) * (1.0f + x)), x);
+}
+
 #endif

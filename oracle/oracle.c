@@ -733,7 +733,7 @@ void orc_bounce_rays(const float* verts, const int32_t* idx, const orc_ray* prim
 
 /* Element-wise evaluation of the fixed-sequence math functions of fmath.h,
  * for their accuracy tests: fn 0 = sin, 1 = cos, 2 = pow(x, y), 3 = log2,
- * 4 = exp2. */
+ * 4 = exp2, 5 = log, 6 = exp, 7 = atan2(x, y), 8 = asin, 9 = acos. */
 void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n)
 {
     for (int32_t i = 0; i < n; i++)
@@ -747,6 +747,8 @@ void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int3
         case 5: out[i] = orc_logf(x[i]); break;
         case 6: out[i] = orc_expf(x[i]); break;
         case 7: out[i] = orc_atan2f(x[i], y[i]); break;
+        case 8: out[i] = orc_asinf(x[i]); break;
+        case 9: out[i] = orc_acosf(x[i]); break;
         default: out[i] = orc_exp2f(x[i]); break;
         }
     }

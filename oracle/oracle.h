@@ -235,7 +235,33 @@ typedef struct {
     orc_luts luts;
     const uint8_t* has_vertex_normals; /* uint8[V] or NULL */
     const float* vertex_normals;       /* float[3V] or NULL */
+    const struct orc_world* world;     /* NULL: uniform black ambient light */
 } orc_render_scene;
+
+/* WorldSettings (WorldSettings.h:14-53) with the envmap image as the caller
+ * gives it (Image32Bit: width x height texels of `channels` floats, row y at
+ * v = y / height). */
+typedef struct orc_world {
+    int32_t ambient_light_type; /* 0 none, 1 uniform, 2 envmap */
+    float uniform_light_color[3];
+    float envmap_intensity;
+    int32_t envmap_scale_background_intensity;
+    float envmap_to_world[3][3]; /* [column][row] */
+    float world_to_envmap[3][3];
+    const float* envmap;
+    int32_t envmap_width, envmap_height, envmap_channels;
+} orc_world;
+
+/* RGBE9995Packed::pack / unpack (Packing.h:447-481). */
+uint32_t orc_rgb9e5_pack(float r, float g, float b);
+void orc_rgb9e5_unpack(uint32_t p, float* rgb);
+/* Envmap sampling of a frame's world (Envmap.h:77-166): a texel by the alias
+ * table from the xorshift state *st, its direction, radiance and pdf; the
+ * radiance and pdf of a direction. */
+struct orc_frame;
+void orc_frame_env_sample(struct orc_frame* f, uint32_t* st, float* dir, float* radiance, float* pdf);
+void orc_frame_env_eval(struct orc_frame* f, const float* dir, float* radiance, float* pdf);
+void orc_frame_env_tables(const struct orc_frame* f, float* probas, int32_t* alias, float* sum, uint32_t* texels);
 
 /* One sample of every pixel: CameraRays (CameraRays.h:83-213) then MegaKernel
  * (Megakernel.h:23-144) per pixel, accumulated into fb (float[3WH], sum) and

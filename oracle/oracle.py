@@ -72,7 +72,18 @@ class RenderScene(C.Structure):
         ("vertices", C.c_void_p), ("indices", C.c_void_p), ("triangle_count", C.c_int32),
         ("material_indices", C.c_void_p), ("materials", C.c_void_p), ("material_count", C.c_int32),
         ("emissive", C.c_void_p), ("emissive_count", C.c_int32), ("luts", Luts),
-        ("has_vertex_normals", C.c_void_p), ("vertex_normals", C.c_void_p),
+        ("has_vertex_normals", C.c_void_p), ("vertex_normals", C.c_void_p), ("world", C.c_void_p),
+    ]
+
+
+class World(C.Structure):
+    """orc_world: WorldSettings plus the envmap image."""
+
+    _fields_ = [
+        ("ambient_light_type", C.c_int32), ("uniform_light_color", C.c_float * 3), ("envmap_intensity", C.c_float),
+        ("envmap_scale_background_intensity", C.c_int32), ("envmap_to_world", C.c_float * 9),
+        ("world_to_envmap", C.c_float * 9), ("envmap", C.c_void_p), ("envmap_width", C.c_int32),
+        ("envmap_height", C.c_int32), ("envmap_channels", C.c_int32),
     ]
 
 
@@ -111,6 +122,11 @@ _SIG = {
     "orc_frame_restir_temporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_spatial": (None, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_presample": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_rgb9e5_pack": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
+    "orc_rgb9e5_unpack": (None, [C.c_uint32, _P]),
+    "orc_frame_env_sample": (None, [C.c_void_p, C.POINTER(C.c_uint32), _P, _P, _P]),
+    "orc_frame_env_eval": (None, [C.c_void_p, _P, _P, _P]),
+    "orc_frame_env_tables": (None, [C.c_void_p, _P, _P, _P, _P]),
     "orc_frame_restir_spatiotemporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_path_trace": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, _P, _P, C.c_int32]),
     "orc_frame_read_reservoirs": (None, [C.c_void_p, C.c_int32, _P]),
@@ -316,7 +332,9 @@ class Renderer:
     state (G-buffers, ReSTIR DI reservoirs): the GPU renderer's passes run in
     the order the caller gives, each with its random number."""
 
-    def __init__(self, scene, settings, restir=None, lut=None):
+    def __init__(self, scene, settings, restir=None, lut=None, world=None, envmap=None):
+        """`world`: the library's WorldSettings (or None: uniform black);
+        `envmap`: height x width x channels float32 image (or None)."""
         self.scene = scene
         self.settings = RenderSettings()
         C.memmove(C.byref(self.settings), C.byref(settings), C.sizeof(RenderSettings))
@@ -347,6 +365,29 @@ class Renderer:
             self.has_vnormals = np.ascontiguousarray(scene.has_vertex_normals, np.uint8)
             rs.has_vertex_normals = _p(self.has_vnormals)
             rs.vertex_normals = _p(self.vnormals)
+        self.world = None
+        if world is not None or envmap is not None:
+            wd = World()
+            wd.ambient_light_type = 1
+            wd.envmap_intensity = 1.0
+            for k in range(3):
+                wd.envmap_to_world[4 * k] = 1.0
+                wd.world_to_envmap[4 * k] = 1.0
+            if world is not None:
+                wd.ambient_light_type = world.ambient_light_type
+                wd.uniform_light_color[:] = list(world.uniform_light_color)
+                wd.envmap_intensity = world.envmap_intensity
+                wd.envmap_scale_background_intensity = world.envmap_scale_background_intensity
+                wd.envmap_to_world[:] = [world.envmap_to_world[c][r] for c in range(3) for r in range(3)]
+                wd.world_to_envmap[:] = [world.world_to_envmap[c][r] for c in range(3) for r in range(3)]
+            if envmap is not None:
+                self.envmap = np.ascontiguousarray(envmap, np.float32)
+                wd.envmap = _p(self.envmap)
+                wd.envmap_height, wd.envmap_width, wd.envmap_channels = self.envmap.shape
+                if world is None:
+                    wd.ambient_light_type = 2
+            self.world = wd
+            rs.world = C.cast(C.pointer(wd), C.c_void_p)
         self.rs = rs
         self.cam = _cam(scene.camera)
         self.n = settings.width * settings.height
@@ -378,6 +419,32 @@ class Renderer:
 
     def restir_spatial(self, pass_index, sample_number, rn, nthreads=0):
         lib().orc_frame_restir_spatial(self.h, pass_index, sample_number, rn, nthreads)
+
+    def env_tables(self):
+        """(alias probabilities, aliases, luminance sum, packed texels) of the envmap."""
+        n = int(self.world.envmap_width) * int(self.world.envmap_height)
+        probas = np.zeros(n, np.float32)
+        alias = np.zeros(n, np.int32)
+        texels = np.zeros(n, np.uint32)
+        total = np.zeros(1, np.float32)
+        lib().orc_frame_env_tables(self.h, _p(probas), _p(alias), _p(total), _p(texels))
+        return probas, alias, float(total[0]), texels
+
+    def env_sample(self, state):
+        """One envmap sample from xorshift state `state`: (direction, radiance, pdf, next state)."""
+        st = C.c_uint32(state)
+        d = np.zeros(3, np.float32)
+        r = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        lib().orc_frame_env_sample(self.h, C.byref(st), _p(d), _p(r), _p(pdf))
+        return d, r, float(pdf[0]), st.value
+
+    def env_eval(self, direction):
+        d = np.ascontiguousarray(direction, np.float32)
+        r = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        lib().orc_frame_env_eval(self.h, _p(d), _p(r), _p(pdf))
+        return r, float(pdf[0])
 
     def restir_presample(self, sample_number, rn, nthreads=0):
         lib().orc_frame_restir_presample(self.h, sample_number, rn, nthreads)
@@ -486,6 +553,17 @@ class Renderer:
         total = C.c_float()
         lib().orc_frame_alias_table(self.h, _p(probas), _p(alias), C.byref(total))
         return probas[:E], alias[:E], total.value
+
+
+def rgb9e5_pack(r, g, b):
+    """The oracle's RGBE9995Packed::pack of one texel."""
+    return int(lib().orc_rgb9e5_pack(float(r), float(g), float(b)))
+
+
+def rgb9e5_unpack(p):
+    out = np.zeros(3, np.float32)
+    lib().orc_rgb9e5_unpack(int(p), _p(out))
+    return out
 
 
 def shading_normal(indices, vertex_normals, has_vertex_normals, prim, u, v, geometric_normal):
@@ -623,7 +701,7 @@ def wavelength_rgb(w):
     return out
 
 
-FMATH = {"sin": 0, "cos": 1, "pow": 2, "log2": 3, "exp2": 4, "log": 5, "exp": 6, "atan2": 7}
+FMATH = {"sin": 0, "cos": 1, "pow": 2, "log2": 3, "exp2": 4, "log": 5, "exp": 6, "atan2": 7, "asin": 8, "acos": 9}
 
 
 def fmath(name, x, y=None):

@@ -251,6 +251,16 @@ typedef struct {
     /* LSS_RESTIR_DI: reservoirs shaded at bounce 0 (struct ores, restir part) */
     const void* restir_out;
     int32_t final_shading_visibility;
+    /* ambient light (WorldSettings.h:21-53): packed texels, alias table */
+    int32_t env_type;
+    float env_uniform[3];
+    int32_t env_scale_bg;
+    float env_intensity, env_sum;
+    int32_t env_w, env_h;
+    const uint32_t* env_texels;
+    const float* env_probas;
+    const int32_t* env_alias;
+    float env_to_world[3][3], env_to_env[3][3];
 } rctx;
 
 enum { NO_INFO = 0, FROM_COAT = 2, FROM_FIRST_METAL = 4, FROM_SECOND_METAL = 8, FROM_SPECULAR = 16,
@@ -1733,6 +1743,174 @@ static olight sample_emissive_with(const rctx* R, int32_t strategy, uint32_t* st
     return L;
 }
 
+/* ------------------------------------------------------------------ */
+/* Ambient light: R9G9B9E5 texels (Packing.h:441-485) and the envmap   */
+/* (Envmap.h:29-276, Texture.h:209-240, PathTracing.h:121-170)          */
+/* ------------------------------------------------------------------ */
+
+/* RGBE9995Packed::pack: the channels are clamped to [0, 511/256 2^15]; a
+ * power of two whose exponent is the largest channel's (floored at 2^-16)
+ * plus 15 -- plus 16 when that channel's top nine significant bits round
+ * up -- is added to each channel, so the float addition rounds each to nine
+ * bits below that power; those low nine bits of the sums and the power's
+ * exponent + 2 (mod 32) make the 32-bit word. */
+uint32_t orc_rgb9e5_pack(float r, float g, float b)
+{
+    const float hi = 65408.0f, lo = 1.52587890625e-05f; /* 511/256 2^15, 2^-16 */
+    r = hmin(hi, hmax(0.0f, r));
+    g = hmin(hi, hmax(0.0f, g));
+    b = hmin(hi, hmax(0.0f, b));
+    float m = hmax(hmax(lo, r), hmax(g, b));
+    uint32_t mb;
+    memcpy(&mb, &m, 4);
+    uint32_t bias_bits = (mb + (15u << 23) + (1u << 14)) & 0x7f800000u;
+    float bias;
+    memcpy(&bias, &bias_bits, 4);
+    float sr = r + bias, sg = g + bias, sb = b + bias;
+    uint32_t R9, G9, B9;
+    memcpy(&R9, &sr, 4);
+    memcpy(&G9, &sg, 4);
+    memcpy(&B9, &sb, 4);
+    uint32_t e = ((bias_bits >> 23) + 2u) & 31u;
+    return (e << 27) | ((B9 & 0x1ffu) << 18) | ((G9 & 0x1ffu) << 9) | (R9 & 0x1ffu);
+}
+
+void orc_rgb9e5_unpack(uint32_t p, float* rgb)
+{
+    int ex = (int)(p >> 27) - 24;
+    rgb[0] = ldexpf((float)(p & 0x1ffu), ex);
+    rgb[1] = ldexpf((float)((p >> 9) & 0x1ffu), ex);
+    rgb[2] = ldexpf((float)((p >> 18) & 0x1ffu), ex);
+}
+
+static int env_on(const rctx* R) { return R->env_type == 2 && R->env_intensity > 0.0f && R->env_w > 0; }
+
+/* matrix_X_vec (Math.h:636-649) */
+static v3 env_rot(const float m[3][3], v3 v)
+{
+    return V(m[0][0] * v.x + m[1][0] * v.y + m[2][0] * v.z, m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z,
+             m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z);
+}
+
+static int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* sample_environment_map_texture, nearest texel (Texture.h:234-239); the
+ * texel coordinates are kept in the image */
+static rgb env_texel(const rctx* R, float u, float v)
+{
+    float fx = u * (float)(uint32_t)(R->env_w - 1), fy = v * (float)(uint32_t)(R->env_h - 1);
+    int32_t x = isnan(fx) ? 0 : clampi((int32_t)hmax(-1.0f, hmin(fx, (float)R->env_w)), 0, R->env_w - 1);
+    int32_t y = isnan(fy) ? 0 : clampi((int32_t)hmax(-1.0f, hmin(fy, (float)R->env_h)), 0, R->env_h - 1);
+    float c[3];
+    orc_rgb9e5_unpack(R->env_texels[x + y * R->env_w], c);
+    return cscale(C3(c[0], c[1], c[2]), R->env_intensity);
+}
+
+/* eval_envmap_no_pdf (Envmap.h:29-38) */
+static rgb env_radiance(const rctx* R, v3 dir)
+{
+    v3 d = env_rot(R->env_to_env, dir);
+    float u = 0.5f + orc_atan2f(d.z, d.x) * 0.15915494309189533577f;
+    float v = 0.5f + orc_asinf(d.y) * 0.31830988618379067154f;
+    return env_texel(R, u, v);
+}
+
+/* Envmap.h:122-131 / :155-163: luminance pdf over the texels, in solid angle */
+static float env_pdf(const rctx* R, rgb radiance, float sin_theta)
+{
+    float pdf = lum(radiance) / (R->env_sum * R->env_intensity);
+    pdf *= (float)((uint32_t)R->env_w * (uint32_t)R->env_h);
+    return pdf / (19.73920880217871723767f * sin_theta);
+}
+
+/* envmap_sample (Envmap.h:77-133) with DeviceAliasTable::sample */
+static rgb env_sample(const rctx* R, v3* dir, float* pdf, uint32_t* st)
+{
+    int32_t i = orc_random_index(st, R->env_w * R->env_h);
+    if (orc_rand01(st) > R->env_probas[i])
+        i = R->env_alias[i];
+    int32_t y = (int32_t)floorf((float)i / (float)(uint32_t)R->env_w);
+    int32_t x = (int32_t)floorf((float)i - (float)y * (float)(uint32_t)R->env_w);
+    float u = (float)x / (float)(uint32_t)R->env_w;
+    float v = (float)y / (float)(uint32_t)R->env_h;
+    float phi = u * 6.28318530717958647693f;
+    float theta = hmax(1.0e-5f, v * 3.14159265358979323846f);
+    float ct = orc_cosf(theta), s_t = orc_sinf(theta);
+    *dir = env_rot(R->env_to_world, V(-s_t * orc_cosf(phi), -ct, -s_t * orc_sinf(phi)));
+    rgb rad = env_texel(R, u, v);
+    *pdf = env_pdf(R, rad, s_t);
+    return rad;
+}
+
+/* envmap_eval (Envmap.h:139-166) */
+static rgb env_eval(const rctx* R, v3 dir, float* pdf)
+{
+    rgb rad = env_radiance(R, dir);
+    float theta = orc_acosf(-dir.y);
+    *pdf = env_pdf(R, rad, orc_sinf(theta));
+    return rad;
+}
+
+/* path_tracing_miss_gather_envmap (PathTracing.h:121-170), clamps off */
+static rgb env_miss(const rctx* R, v3 dir, int bounce)
+{
+    if (R->env_type == 1)
+        return C3(R->env_uniform[0], R->env_uniform[1], R->env_uniform[2]);
+    if (R->env_type != 2 || R->env_intensity == 0.0f || R->env_w == 0 || bounce != 0)
+        return C1(0.0f);
+    rgb c = env_radiance(R, dir);
+    if (!R->env_scale_bg)
+        c = cdivs(c, R->env_intensity);
+    return c;
+}
+
+static float bal2(float a, float b) { return a == 0.0f ? 0.0f : a / (1.0f * a + 1.0f * b); }
+
+/* sample_environment_map_with_mis (Envmap.h:168-255): the envmap light
+ * sample, then a BSDF sample for MIS, each with its shadow ray */
+static rgb sample_envmap_mis(const rctx* R, const omat* m, const hitinfo* hi, v3 view, float path_r, uint32_t* st,
+                             ovol* vs)
+{
+    v3 edir;
+    float epdf;
+    rgb ecol = env_sample(R, &edir, &epdf, st);
+    rgb env_mis = C1(0.0f);
+    float ecos = vdot(hi->sn, edir);
+    if (epdf > 0.0f && ecos > 0.0f)
+    {
+        orc_hit sh = trace1(R, hi->p, edir, hi->prim, INFINITY);
+        if (!(sh.prim >= 0 && sh.t < 1.0e35f - 1.0e-4f))
+        {
+            bctx c = {view, hi->sn, hi->gn, edir, NO_INFO, REG_MIS, path_r, vs, 0};
+            float bpdf;
+            rgb f = principled_eval(R, m, &c, &bpdf);
+            float w = bal2(epdf, bpdf);
+            env_mis = cdivs(cdivs(cmul(cscale(cscale(f, ecos), w), ecol), epdf), 1.0f);
+        }
+    }
+    rgb bsdf_mis = C1(0.0f);
+    bctx c = {view, hi->sn, hi->gn, V(0, 0, 0), NO_INFO, REG_MIS, path_r, vs, 0};
+    v3 dir;
+    float bpdf;
+    rgb f = principled_sample(R, m, &c, &dir, &bpdf, st);
+    float bcos = fabsf(vdot(hi->sn, dir));
+    if (bpdf > 0.0f)
+    {
+        orc_hit sh = trace1(R, hi->p, dir, hi->prim, INFINITY);
+        if (!(sh.prim >= 0 && sh.t < 1.0e35f - 1.0e-4f))
+        {
+            float ep;
+            rgb rad = env_eval(R, dir, &ep);
+            if (ep > 0.0f)
+            {
+                float w = bal2(bpdf, ep);
+                bsdf_mis = cdivs(cmul(cscale(cscale(rad, w), bcos), f), bpdf);
+            }
+        }
+    }
+    return cadd(bsdf_mis, env_mis);
+}
+
 static olight sample_emissive(const rctx* R, uint32_t* st)
 {
     return sample_emissive_with(R, R->s.direct_light_sampling_base_strategy, st);
@@ -1934,7 +2112,8 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
         }
         if (!found)
         {
-            missed = 1; /* no envmap in these scenes: adds 0 */
+            missed = 1;
+            color = cadd(color, cmul(env_miss(R, dir, bounce), thr));
             continue;
         }
         const omat* m = &R->mats[mat];
@@ -1947,10 +2126,15 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
                 direct = restir_final_shading(R, m, &hi, view, pix, &vs);
             else if (R->ne > 0)
                 direct = cadd(C1(0.0f), sample_one_light_mis(R, m, &hi, view, path_r, &st, &vs));
+            /* sample_environment_map (Envmap.h:258-276): ReSTIR DI covers
+             * the envmap at bounce 0 */
+            rgb env = C1(0.0f);
+            if (env_on(R) && !(bounce == 0 && R->s.direct_light_sampling_strategy == 5))
+                env = sample_envmap_mis(R, m, &hi, view, path_r, &st, &vs);
             rgb total = C1(0.0f);
             if (bounce == 0)
                 total = cadd(total, m->emission);
-            total = cadd(total, cmul(cadd(cdivs(direct, 1.0f), C1(0.0f)), thr));
+            total = cadd(total, cmul(cadd(cdivs(direct, 1.0f), env), thr));
             color = cadd(color, total);
         }
         /* path_tracing_compute_next_indirect_bounce (PathTracing.h:26-101) */
@@ -2286,6 +2470,10 @@ struct orc_frame {
     struct opresampled* presampled;
     size_t presampled_count;
     int32_t fused_output; /* buffer the fused pass (or a spatial pass after it) wrote */
+    /* the envmap's packed texels and alias table */
+    uint32_t* env_texels;
+    float* env_probas;
+    int32_t* env_alias;
 };
 
 /* ReSTIRDIPresampledLight (ReSTIR/DI/PresampledLight.h:14-30) */
@@ -2406,6 +2594,52 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
     R.alias_probas = f->alias_probas;
     R.alias_alias = f->alias_alias;
     R.alias_sum = sum;
+    /* WorldSettings; the envmap as RendererEnvmap prepares it
+     * (RendererEnvmap.cpp:13-20, :80-97): packed texels, then the alias
+     * table over the float texels' luminance (Image.cpp:526-546, :652-672) */
+    R.env_type = 1;
+    R.env_intensity = 1.0f;
+    for (int k = 0; k < 3; k++)
+    {
+        R.env_to_world[k][k] = 1.0f;
+        R.env_to_env[k][k] = 1.0f;
+    }
+    if (sc->world)
+    {
+        const orc_world* w = sc->world;
+        R.env_type = w->ambient_light_type;
+        memcpy(R.env_uniform, w->uniform_light_color, sizeof(R.env_uniform));
+        R.env_scale_bg = w->envmap_scale_background_intensity;
+        R.env_intensity = w->envmap_intensity;
+        memcpy(R.env_to_world, w->envmap_to_world, sizeof(R.env_to_world));
+        memcpy(R.env_to_env, w->world_to_envmap, sizeof(R.env_to_env));
+        if (w->envmap && w->envmap_width > 0 && w->envmap_height > 0)
+        {
+            size_t count = (size_t)w->envmap_width * (size_t)w->envmap_height;
+            f->env_texels = (uint32_t*)malloc(4 * count);
+            f->env_probas = (float*)malloc(4 * count);
+            f->env_alias = (int32_t*)malloc(4 * count);
+            float* lumv = (float*)malloc(4 * count);
+            float esum = 0.0f;
+            for (size_t i = 0; i < count; i++)
+            {
+                const float* px = w->envmap + i * (size_t)w->envmap_channels;
+                f->env_texels[i] = orc_rgb9e5_pack(px[0], px[1], px[2]);
+                lumv[i] = lum(C3(px[0], px[1], px[2]));
+                esum += lumv[i];
+            }
+            orc_alias_table(lumv, (int32_t)count, esum, f->env_probas, f->env_alias);
+            free(lumv);
+            R.env_sum = esum;
+            R.env_w = w->envmap_width;
+            R.env_h = w->envmap_height;
+            R.env_texels = f->env_texels;
+            R.env_probas = f->env_probas;
+            R.env_alias = f->env_alias;
+        }
+        else if (R.env_type == 2)
+            R.env_type = 1; /* set_envmap with an empty image */
+    }
     f->R = R;
     if (rs)
         f->rs = *rs;
@@ -2450,9 +2684,39 @@ void orc_frame_free(orc_frame* f)
     free(f->cur);
     free(f->prev);
     free(f->presampled);
+    free(f->env_texels);
+    free(f->env_probas);
+    free(f->env_alias);
     for (int k = 0; k < 3; k++)
         free(f->res[k]);
     free(f);
+}
+
+void orc_frame_env_sample(orc_frame* f, uint32_t* st, float* dir, float* radiance, float* pdf)
+{
+    v3 d;
+    rgb r = env_sample(&f->R, &d, pdf, st);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    radiance[0] = r.r; radiance[1] = r.g; radiance[2] = r.b;
+}
+
+void orc_frame_env_eval(orc_frame* f, const float* dir, float* radiance, float* pdf)
+{
+    rgb r = env_eval(&f->R, V(dir[0], dir[1], dir[2]), pdf);
+    radiance[0] = r.r; radiance[1] = r.g; radiance[2] = r.b;
+}
+
+void orc_frame_env_tables(const orc_frame* f, float* probas, int32_t* alias, float* sum, uint32_t* texels)
+{
+    size_t count = (size_t)f->R.env_w * (size_t)f->R.env_h;
+    if (probas)
+        memcpy(probas, f->env_probas, 4 * count);
+    if (alias)
+        memcpy(alias, f->env_alias, 4 * count);
+    if (sum)
+        *sum = f->R.env_sum;
+    if (texels)
+        memcpy(texels, f->env_texels, 4 * count);
 }
 
 static void set_threads(int32_t nthreads)
@@ -3227,9 +3491,8 @@ int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t rn, 
             if (!in_window(f, pix))
                 continue;
             const gbuf* g = f->cur + pix;
-            rgb c = C1(0.0f);
-            if (g->prim != -1)
-                c = megakernel_pixel(&f->R, x, y, rn, g);
+            /* a camera ray that missed gathers the ambient light at bounce 0 */
+            rgb c = megakernel_pixel(&f->R, x, y, rn, g);
             int bad = 0;
             float comp[3] = {c.r, c.g, c.b};
             for (int k = 0; k < 3; k++)

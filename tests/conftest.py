@@ -60,3 +60,61 @@ def open_back_box(mipt, width, height):
 @pytest.fixture(scope="session")
 def open_back_box_scene(mipt_mod):
     return open_back_box
+
+
+def look_at(mipt, eye, target, vfov_deg, width, height):
+    """A jittered pinhole camera (HIPRTCamera matrices as the scene
+    generator builds them: OpenGL view and perspective, near 0.1, far 1000)."""
+    import numpy as np
+
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, [0.0, 1.0, 0.0])
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.array([[*s, -s @ eye], [*u, -u @ eye], [*-f, f @ eye], [0, 0, 0, 1]])
+    iview = np.array([[s[0], u[0], -f[0], eye[0]], [s[1], u[1], -f[1], eye[1]], [s[2], u[2], -f[2], eye[2]],
+                      [0, 0, 0, 1]])
+    aspect, n, fa = width / height, 0.1, 1000.0
+    ft = 1.0 / np.tan(np.radians(vfov_deg) / 2)
+    proj = np.array([[ft / aspect, 0, 0, 0], [0, ft, 0, 0], [0, 0, (fa + n) / (n - fa), 2 * fa * n / (n - fa)],
+                     [0, 0, -1, 0]])
+    A, B = proj[2, 2], proj[2, 3]
+    iproj = np.array([[aspect / ft, 0, 0, 0], [0, 1 / ft, 0, 0], [0, 0, 0, -1], [0, 0, 1 / B, A / B]])
+    cam = mipt.Camera()
+    for i in range(4):
+        cam.inverse_view[i][:] = [float(np.float32(x)) for x in iview[i]]
+        cam.inverse_projection[i][:] = [float(np.float32(x)) for x in iproj[i]]
+        cam.view_projection[i][:] = [float(np.float32(x)) for x in (proj @ view)[i]]
+    cam.position[:] = [float(np.float32(x)) for x in eye]
+    cam.vertical_fov = float(np.radians(vfov_deg))
+    cam.sensor_width, cam.sensor_height, cam.do_jittering = width, height, 1
+    return cam
+
+
+def sky_scene(mipt, width, height, lights=True):
+    """The config-1 scene without the Cornell box's walls and ceiling: its
+    floor (triangles 0-3) and the 20 spheres under the open sky, with the two
+    emissive ceiling quads (triangles 24-27) floating above them or not, seen
+    from outside. Camera rays that pass the floor and the spheres leave the
+    scene."""
+    import numpy as np
+
+    full = mipt.generate_scene(1, width, height)
+    keep = np.zeros(full.triangle_count, bool)
+    keep[0:4] = True
+    keep[28:] = True
+    if lights:
+        keep[24:28] = True
+    emissive = np.arange(4, 8, dtype=np.int32) if lights else np.zeros(0, np.int32)
+    # from above and behind the floor, looking down at it: floor, spheres,
+    # the sky above the horizon and the map's lower half beyond the floor
+    cam = look_at(mipt, (0.0, 0.3, 3.2), (0.0, -0.55, 0.0), 55.0, width, height)
+    return mipt.Scene(full.vertices, full.indices[keep], full.material_indices[keep], full.materials, emissive,
+                      cam, width, height)
+
+
+@pytest.fixture(scope="session")
+def sky_scene_fn(mipt_mod):
+    return sky_scene

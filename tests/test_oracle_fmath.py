@@ -103,3 +103,21 @@ def test_atan2(oracle_mod):
     want = np.arctan2(zs.astype(np.float64), xs.astype(np.float64)).astype(np.float32)
     got = oracle_mod.fmath("atan2", zs, xs)
     assert np.array_equal(np.signbit(got), np.signbit(want)) and np.allclose(got, want, atol=0, rtol=1e-7)
+
+
+@pytest.mark.parametrize("name,fn", [("asin", np.arcsin), ("acos", np.arccos)])
+def test_asin_acos_ulp(oracle_mod, name, fn):
+    """asin and acos through atan2 of the triangle's legs: within 4 ulp (or
+    2e-7 absolute near asin's zero) over [-1, 1], exact at the ends, and the
+    argument clamped to [-1, 1] (a normalised direction may exceed 1 by an
+    ulp)."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([np.linspace(-1, 1, 200001), rng.uniform(-1, 1, 100000),
+                        1 - rng.uniform(0, 1e-3, 10000), -1 + rng.uniform(0, 1e-3, 10000)]).astype(np.float32)
+    got = oracle_mod.fmath(name, x)
+    exact = fn(x.astype(np.float64))
+    ok = (ulp_err(got, exact) <= 4) | (np.abs(got - exact) <= 2e-7)
+    assert ok.all(), (x[~ok][:5], got[~ok][:5], exact[~ok][:5])
+    ends = oracle_mod.fmath(name, np.float32([-1.0, 0.0, 1.0, 1.0000001, -1.0000001]))
+    want = np.float32(fn(np.float64([-1.0, 0.0, 1.0, 1.0, -1.0])))
+    assert np.abs(ends - want).max() <= 2.4e-7
