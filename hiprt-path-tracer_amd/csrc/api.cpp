@@ -784,6 +784,7 @@ void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
     s->light_presampling_number_of_subsets = 128;
     s->light_presampling_subset_size = 1024;
     s->light_presampling_tile_size = 8;
+    s->envmap_candidate_probability = 0.5f; // ReSTIRDISettings.h:22
 }
 
 int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
@@ -794,6 +795,7 @@ int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
     if (!ctx || !s || s->number_of_initial_light_candidates < 0 || s->number_of_initial_bsdf_candidates < 0 ||
         s->spatial_number_of_passes < 1 || s->spatial_reuse_radius < 0 || s->spatial_reuse_neighbor_count < 0 ||
         s->spatial_reuse_neighbor_count > 32 || s->m_cap < 0 ||
+        !(s->envmap_candidate_probability >= 0.0f && s->envmap_candidate_probability <= 1.0f) ||
         (s->do_light_presampling &&
          (s->light_presampling_number_of_subsets < 1 || s->light_presampling_subset_size < 1 ||
           s->light_presampling_tile_size < 1 ||
@@ -968,11 +970,8 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         // a dispersive glass draws a wavelength from the pass's stream at
         // every surface fetch (ReSTIR/Surface.h:25-43); those draws are not
         // implemented.
-        if (mipt::env_sampling_on(*ctx) &&
-            (ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI ||
-             ctx->render.settings.radiance_cache))
-            mipt::fail(MIPT_ERR_UNSUPPORTED, "an environment map with ReSTIR DI (envmap candidates) or with the "
-                                             "radiance cache is not implemented yet");
+        if (mipt::env_sampling_on(*ctx) && ctx->render.settings.radiance_cache)
+            mipt::fail(MIPT_ERR_UNSUPPORTED, "an environment map with the radiance cache is not implemented yet");
         if (ctx->render.scene_dispersive && ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
             mipt::fail(MIPT_ERR_UNSUPPORTED, "ReSTIR DI with a dispersive glass material (dispersion_scale > 0 and "
                                              "specular_transmission > 0) is not implemented yet");

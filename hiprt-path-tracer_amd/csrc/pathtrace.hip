@@ -254,14 +254,26 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
     const float4 ra = d.restir_out[2 * (size_t)pix], rb = d.restir_out[2 * (size_t)pix + 1];
     const float ucw = rb.z;
     const int32_t tri = __float_as_int(rb.y);
-    // A NaN weight sum leaves UCW = NaN with no sample selected (triangle -1):
-    // nothing to shade.
-    if (ucw <= 0.0f || tri < 0)
-        return pt::col(0.0f);
     const uint32_t flags = (uint32_t)__float_as_int(rb.w);
-    const f3 to = ld3(ra) - p;
-    const float dist = sqrtf(pt::dot(to, to));
-    const f3 dir = pt::vdiv(to, dist);
+    const bool env = (flags & pt::RESTIR_FLAG_ENVMAP) != 0u;
+    // A NaN weight sum leaves UCW = NaN with no sample selected (triangle -1,
+    // no envmap flag): nothing to shade. validate_reservoir (:103-108): an
+    // envmap sample without the envmap contributes nothing.
+    if (ucw <= 0.0f || (tri < 0 && !env) || (env && !env_sampled(d.env)))
+        return pt::col(0.0f);
+    float dist;
+    f3 dir;
+    if (env)
+    {
+        dir = env_rotate(d.env.to_world, ld3(ra));
+        dist = 1.0e35f;
+    }
+    else
+    {
+        const f3 to = ld3(ra) - p;
+        dist = sqrtf(pt::dot(to, to));
+        dir = pt::vdiv(to, dist);
+    }
     dir_out = dir;
     dist_out = dist;
     if (!(flags & pt::RESTIR_FLAG_UNOCCLUDED) && d.final_shading_visibility)
@@ -274,9 +286,19 @@ __device__ __forceinline__ Color restir_final_shading(const PtData& d, uint32_t 
         cosine = fabsf(cosine);
     if (!(cosine > 0.0f))
         return pt::col(0.0f);
-    const Color emission = d.mats[d.mat_idx[tri]].emission;
-    float conv = cos_at_light(pt::normalize(tri_normal_unnormalized(d, tri)), -1.0f * dir);
-    conv /= dist * dist;
+    Color emission;
+    float conv = 1.0f;
+    if (env)
+    {
+        float epdf;
+        emission = env_eval(d.env, dir, epdf);
+    }
+    else
+    {
+        emission = d.mats[d.mat_idx[tri]].emission;
+        conv = cos_at_light(pt::normalize(tri_normal_unnormalized(d, tri)), -1.0f * dir);
+        conv /= dist * dist;
+    }
     return (((f * ucw) * emission) * cosine) * conv;
 }
 
@@ -310,7 +332,8 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
     }
     // Without emissive triangles sample_emissive_geometry returns before any
     // draw (Lights.h:313-321): neither the light nor the MIS BSDF sample runs.
-    if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0 && d.restir && bounce == 0)
+    // ReSTIR DI at bounce 0 also covers the envmap (Lights.h:318-324)
+    if ((bounce > 0 || d.enable_direct) && (d.emissive_count > 0 || env_sampled(d.env)) && d.restir && bounce == 0)
     {
         bool need_shadow;
         f3 dir;

@@ -371,14 +371,20 @@ __device__ __forceinline__ LightSample sample_emissive(const PtData& d, pt::Xors
 }
 
 // pdf_of_emissive_triangle_hit_area_measure (LightUtils.h:51-76)
-__device__ __forceinline__ float emissive_hit_area_pdf(const PtData& d, float light_area, Color emission)
+__device__ __forceinline__ float emissive_hit_area_pdf_with(const PtData& d, int32_t strategy, float light_area,
+                                                            Color emission)
 {
     float pdf = 1.0f / light_area;
-    if (d.light_base == MIPT_LSS_BASE_POWER)
+    if (strategy == MIPT_LSS_BASE_POWER)
         pdf *= (luminance(emission) * light_area) / d.alias_sum;
     else
         pdf /= (float)d.emissive_count;
     return pdf;
+}
+
+__device__ __forceinline__ float emissive_hit_area_pdf(const PtData& d, float light_area, Color emission)
+{
+    return emissive_hit_area_pdf_with(d, d.light_base, light_area, emission);
 }
 
 } // namespace mipt

@@ -241,19 +241,35 @@ __device__ __forceinline__ Surface surface_previous(const PtData& d, const RsDat
     return s;
 }
 
+// Direction and distance from p to a sample: an envmap sample stores its
+// direction in envmap space and lies at 1e35 (ReSTIR_DI_get_light_sample_
+// direction, DI/TargetFunction.h:13-29).
+__device__ __forceinline__ f3 sample_direction(const PtData& d, f3 p, f3 point, uint32_t flags, float& dist)
+{
+    if (flags & pt::RESTIR_FLAG_ENVMAP)
+    {
+        dist = 1.0e35f;
+        return env_rotate(d.env.to_world, point);
+    }
+    const f3 to = point - p;
+    dist = sqrtf(pt::dot(to, to));
+    return pt::vdiv(to, dist);
+}
+
 // ReSTIR_DI_evaluate_target_function<false> (DI/TargetFunction.h:49-98): the
-// luminance of f * Le * cos * G. The visibility factor, when wanted, is
-// applied by the caller from a shadow ray (origin s.p, direction dir_out,
+// luminance of f * Le * cos * G (G = 1 for an envmap sample, whose Le is the
+// envmap's radiance in that direction). The visibility factor, when wanted,
+// is applied by the caller from a shadow ray (origin s.p, direction dir_out,
 // tmax dist_out - 1e-4), traced only when the value is non-zero.
 __device__ __forceinline__ float target_function(const PtData& d, const Surface& s, f3 point, int32_t tri, uint32_t flags,
                                                  f3& dir_out, float& dist_out)
 {
     dist_out = 0.0f;
-    if (tri == -1 || s.mat < 0)
+    const bool env = (flags & pt::RESTIR_FLAG_ENVMAP) != 0u;
+    if ((tri == -1 && !env) || s.mat < 0)
         return 0.0f;
-    const f3 to = point - s.p;
-    const float dist = sqrtf(pt::dot(to, to));
-    const f3 dir = pt::vdiv(to, dist);
+    float dist;
+    const f3 dir = sample_direction(d, s.p, point, flags, dist);
     dir_out = dir;
     dist_out = dist;
     const float cosine = pt::dot(s.sn, dir);
@@ -264,10 +280,30 @@ __device__ __forceinline__ float target_function(const PtData& d, const Surface&
     pt::BsdfCtx c{s.view, s.sn, s.gn, dir, flags & pt::RESTIR_FLAG_LOBE_MASK, pt::NO_REGULARIZATION, 0.0f, &vs, 0};
     float bpdf;
     const Color f = pt::principled_eval(d.g, m, c, bpdf);
-    const Color emission = d.mats[d.mat_idx[tri]].emission;
-    float geometry = cos_at_light(pt::normalize(tri_normal_unnormalized(d, tri)), -1.0f * dir);
-    geometry /= dist * dist;
+    Color emission;
+    float geometry = 1.0f;
+    if (env)
+    {
+        float epdf;
+        emission = env_eval(d.env, dir, epdf);
+    }
+    else
+    {
+        emission = d.mats[d.mat_idx[tri]].emission;
+        geometry = cos_at_light(pt::normalize(tri_normal_unnormalized(d, tri)), -1.0f * dir);
+        geometry /= dist * dist;
+    }
     return luminance(((f * emission) * cosine) * geometry);
+}
+
+// The probability that a light candidate samples the envmap rather than an
+// emissive triangle (InitialCandidates.h:451-459, LightsPresampling.h:99-107);
+// an envmap of intensity 0 counts as no envmap.
+__device__ __forceinline__ float env_candidate_probability(const PtData& d, const RsData& r)
+{
+    if (!env_sampled(d.env))
+        return 0.0f;
+    return d.emissive_count == 0 ? 1.0f : r.s.envmap_candidate_probability;
 }
 
 // target with visibility from a traced shadow ray; NaN stays NaN as in
@@ -307,7 +343,7 @@ __device__ __forceinline__ uint32_t presampled_index(const PtData& d, const RsDa
 // One presampled light per thread: an envmap draw (probability 0 here), then
 // an emissive triangle by power (ReSTIR_DI_LightPresamplingStrategy,
 // ReSTIRDIOptions.h:162). A zero-pdf sample keeps the empty record.
-__global__ void k_presample_lights(PtData d, float4* __restrict__ out, uint32_t count)
+__global__ void k_presample_lights(PtData d, float env_prob, float4* __restrict__ out, uint32_t count)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= count)
@@ -316,13 +352,24 @@ __global__ void k_presample_lights(PtData d, float4* __restrict__ out, uint32_t 
     float4 a = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
     float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 c = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
-    if (!(rng() < 0.0f))
+    if (rng() < env_prob)
+    {
+        // presample_envmap (LightsPresampling.h:22-35): the direction in
+        // envmap space
+        f3 dir;
+        float pdf;
+        const Color radiance = env_sample(d.env, dir, pdf, rng);
+        a = st3(env_rotate(d.env.to_env, dir), __int_as_float(-1));
+        b = st3(pt::mk(0.0f, 0.0f, 0.0f), pdf * env_prob);
+        c = stc(radiance, __int_as_float((int32_t)pt::RESTIR_FLAG_ENVMAP));
+    }
+    else
     {
         const LightSample ls = sample_emissive_with(d, MIPT_LSS_BASE_POWER, rng);
         if (ls.area_pdf > 0.0f)
         {
             a = st3(ls.point, __int_as_float(ls.tri));
-            b = st3(ls.normal, ls.area_pdf * 1.0f);
+            b = st3(ls.normal, ls.area_pdf * (1.0f - env_prob));
             c = stc(ls.emission, __int_as_float(0));
         }
     }
@@ -363,8 +410,8 @@ __device__ __forceinline__ bool next_bsdf_candidate(const PtData& d, const RsDat
 
 // reservoir.end(), M = 1 (InitialCandidates.h:465-472), then the visibility
 // reuse test (ReSTIR/Utils.h:25-67) as a ray in slot 1.
-__device__ __forceinline__ void finish_initial(const RsData& r, uint32_t pix, const Surface& s, Res& res,
-                                               uint32_t& need_vis)
+__device__ __forceinline__ void finish_initial(const PtData& d, const RsData& r, uint32_t pix, const Surface& s,
+                                               Res& res, uint32_t& need_vis)
 {
     res_end(res);
     res.M = 1;
@@ -374,9 +421,8 @@ __device__ __forceinline__ void finish_initial(const RsData& r, uint32_t pix, co
         return;
     if (res.flags & pt::RESTIR_FLAG_UNOCCLUDED)
         return;
-    const f3 to = res.point - s.p;
-    const float dist = sqrtf(pt::dot(to, to));
-    const f3 dir = pt::vdiv(to, dist);
+    float dist;
+    const f3 dir = sample_direction(d, s.p, res.point, res.flags, dist);
     write_ray(r.rays, (size_t)r.n + pix, s.p, s.prim, dir, dist - 1.0e-4f);
     need_vis = 1u;
 }
@@ -399,6 +445,7 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
     Res res = res_empty();
     const int32_t nl = r.s.number_of_initial_light_candidates;
     const int32_t nb = r.s.number_of_initial_bsdf_candidates;
+    const float env_prob = env_candidate_probability(d, r);
     // sample_light_candidates (InitialCandidates.h:180-298)
     for (int32_t i = 0; i < nl; i++)
     {
@@ -408,6 +455,7 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
         f3 ls_normal = pt::mk(0.0f, 0.0f, 0.0f);
         Color ls_emission = pt::col(0.0f);
         int32_t tri = -1;
+        uint32_t sflags = 0u;
         if (r.presampled)
         {
             // use_presampled_light_candidate (InitialCandidates.h:33-90): the
@@ -419,13 +467,12 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
             ls_normal = ld3(pl[1]);
             pdf = pl[1].w;
             ls_emission = ldc(pl[2]);
-            const f3 to = point - s.p;
-            dist = sqrtf(pt::dot(to, to));
-            dir = pt::vdiv(to, dist);
+            sflags = (uint32_t)__float_as_int(pl[2].w);
+            dir = sample_direction(d, s.p, point, sflags, dist);
             cosine = pt::dot(s.sn, dir);
         }
-        // sample_fresh_light_candidate (:99-178): envmap candidate probability 0
-        else if (rng() > 0.0f)
+        // sample_fresh_light_candidate (:92-178)
+        else if (rng() > env_prob)
         {
             const LightSample ls = sample_emissive(d, rng);
             tri = ls.tri;
@@ -439,7 +486,7 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
                 dist = sqrtf(pt::dot(to, to));
                 dir = pt::vdiv(to, dist);
                 cosine = pt::maxf_(0.0f, pt::dot(s.sn, dir));
-                pdf *= (1.0f - 0.0f);
+                pdf *= (1.0f - env_prob);
             }
             else
             {
@@ -448,6 +495,22 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
                 dir = pt::vdiv(to, dist);
             }
         }
+        // A draw of exactly 0 (the zero xorshift state that a seed of
+        // wang_hash(61) = 0 gives under freeze_random) lands here with
+        // probability 0 too: without an envmap it stays no candidate.
+        else if (env_prob > 0.0f)
+        {
+            // an envmap candidate, stored as its direction in envmap space;
+            // the candidate's own direction is recomputed from that
+            f3 edir;
+            ls_emission = env_sample(d.env, edir, pdf, rng);
+            cosine = pt::maxf_(0.0f, pt::dot(edir, s.sn));
+            pdf *= env_prob;
+            point = env_rotate(d.env.to_env, edir);
+            sflags = pt::RESTIR_FLAG_ENVMAP;
+            dir = sample_direction(d, s.p, point, sflags, dist);
+        }
+        const bool env = (sflags & pt::RESTIR_FLAG_ENVMAP) != 0u;
         float weight = 0.0f, sample_target = 0.0f;
         if (cosine > 0.0f && pdf > 0.0f)
         {
@@ -459,20 +522,36 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
             // its BSDF evaluation has the same arguments as the one above (the
             // direction is recomputed from the same point by the same
             // operations), and the normalised triangle normal equals the light
-            // sample's normal bit for bit, so both are reused.
-            const float geometry = cos_at_light(ls_normal, -1.0f * dir) / (dist * dist);
-            const float target = luminance(((f * ls_emission) * cosine) * geometry);
+            // sample's normal bit for bit, so both are reused. An envmap
+            // sample's emission is the envmap evaluated in its direction.
+            float target;
+            if (env)
+            {
+                const float tcos = pt::dot(s.sn, dir);
+                float epdf;
+                const Color e = env_eval(d.env, dir, epdf);
+                target = tcos <= 0.0f ? 0.0f : luminance(((f * e) * tcos) * 1.0f);
+            }
+            else
+            {
+                const float geometry = cos_at_light(ls_normal, -1.0f * dir) / (dist * dist);
+                target = luminance(((f * ls_emission) * cosine) * geometry);
+            }
             if (!(bsdf_pdf <= 0.0f))
             {
                 const f3 ln = ls_normal;
-                float lpdf = area_to_solid_angle(pdf / (1.0f - 0.0f), dist, cos_at_light(ln, -1.0f * dir));
-                lpdf *= (1.0f - 0.0f);
+                float lpdf = pdf;
+                if (!env)
+                {
+                    lpdf = area_to_solid_angle(pdf / (1.0f - env_prob), dist, cos_at_light(ln, -1.0f * dir));
+                    lpdf *= (1.0f - env_prob);
+                }
                 const float mis = balance_heuristic(lpdf, (float)nl, bsdf_pdf, (float)nb);
                 weight = mis * target / pdf;
                 sample_target = target;
             }
         }
-        add_candidate(res, point, tri, sample_target, 0u, weight, rng);
+        add_candidate(res, point, tri, sample_target, sflags, weight, rng);
     }
     int32_t cand = 0;
     float bpdf = 0.0f;
@@ -484,7 +563,7 @@ __global__ __launch_bounds__(256, 2) void k_ic_start(PtData d, RsData r)
         return;
     }
     uint32_t need_vis;
-    finish_initial(r, pix, s, res, need_vis);
+    finish_initial(d, r, pix, s, res, need_vis);
     state_store(r, pix, res, rng.state, PHASE_DONE, 0.0f, 0.0f, need_vis);
 }
 
@@ -525,11 +604,15 @@ __global__ __launch_bounds__(256, 2) void k_ic_bsdf(PtData d, RsData r)
             const f3 hgn = pt::normalize(pt::normalize(tri_normal_unnormalized(d, hp)));
             const float cos_l = cos_at_light(hgn, -1.0f * dir);
             float lpdf = 0.0f;
+            // with presampling, the light sampler is the presampling's
+            // (pdf_of_emissive_triangle_hit_solid_angle<ReSTIR_DI_
+            // LightPresamplingStrategy>, :360-366): power
+            const int32_t strategy = r.presampled ? MIPT_LSS_BASE_POWER : d.light_base;
             if (!(pt::dot(dir, s.sn) < 0.0f))
-                lpdf = area_to_solid_angle(emissive_hit_area_pdf(d, d.areas[hp], emission), t, cos_l);
+                lpdf = area_to_solid_angle(emissive_hit_area_pdf_with(d, strategy, d.areas[hp], emission), t, cos_l);
             if (!(bpdf <= 0.0f))
             {
-                lpdf *= (1.0f - 0.0f);
+                lpdf *= (1.0f - env_candidate_probability(d, r));
                 const float mis = balance_heuristic(bpdf, (float)nb, lpdf, (float)nl);
                 float pdf_area = bpdf;
                 pdf_area /= (t * t);
@@ -541,6 +624,25 @@ __global__ __launch_bounds__(256, 2) void k_ic_bsdf(PtData d, RsData r)
             }
         }
     }
+    else if (env_sampled(d.env) && pt::dot(s.sn, dir) > 0.0f)
+    {
+        // an envmap hit becomes an envmap sample (:396-436); its light pdf is
+        // the envmap sampler's, refractions excluded
+        float epdf;
+        env_eval(d.env, dir, epdf);
+        const f3 point = env_rotate(d.env.to_env, dir);
+        const uint32_t flags = (pt::RESTIR_FLAG_UNOCCLUDED | pt::RESTIR_FLAG_ENVMAP | info) & 0xffu;
+        f3 tdir;
+        float tdist;
+        const float target = target_function(d, s, point, -1, flags, tdir, tdist);
+        if (!(bpdf <= 0.0f))
+        {
+            epdf *= env_candidate_probability(d, r);
+            const float mis = balance_heuristic(bpdf, (float)nb, epdf, (float)nl);
+            const float weight = mis * target / bpdf;
+            add_candidate(res, point, -1, target, flags, weight, rng);
+        }
+    }
     cand++;
     if (next_bsdf_candidate(d, r, pix, s, m, cand, rng, bpdf, info))
     {
@@ -549,7 +651,7 @@ __global__ __launch_bounds__(256, 2) void k_ic_bsdf(PtData d, RsData r)
     }
     write_inactive(r.rays, pix);
     uint32_t need_vis;
-    finish_initial(r, pix, s, res, need_vis);
+    finish_initial(d, r, pix, s, res, need_vis);
     state_store(r, pix, res, rng.state, PHASE_DONE, 0.0f, 0.0f, need_vis);
 }
 
@@ -861,9 +963,8 @@ __global__ void k_sp_center(PtData d, RsData r)
     if (r.kill_after && !(out.ucw <= 0.0f && out.wsum <= 0.0f) && !(out.flags & pt::RESTIR_FLAG_UNOCCLUDED))
     {
         const f3 p = ld3(d.gb_pos[pix]);
-        const f3 to = out.point - p;
-        const float dist = sqrtf(pt::dot(to, to));
-        const f3 dir = pt::vdiv(to, dist);
+        float dist;
+        const f3 dir = sample_direction(d, p, out.point, out.flags, dist);
         write_ray(r.rays, pix, p, d.gb_prim[pix], dir, dist - 1.0e-4f);
         need_vis = 1u;
     }
@@ -1176,9 +1277,8 @@ __global__ __launch_bounds__(256, 2) void k_fst_combine(PtData d, RsData r)
     if (r.kill_after && !(out.ucw <= 0.0f && out.wsum <= 0.0f) && !(out.flags & pt::RESTIR_FLAG_UNOCCLUDED))
     {
         const f3 p = ld3(d.gb_pos[pix]);
-        const f3 to = out.point - p;
-        const float dist = sqrtf(pt::dot(to, to));
-        const f3 dir = pt::vdiv(to, dist);
+        float dist;
+        const f3 dir = sample_direction(d, p, out.point, out.flags, dist);
         write_ray(r.rays, pix, p, d.gb_prim[pix], dir, dist - 1.0e-4f);
         need_vis = 1u;
     }
@@ -1293,7 +1393,9 @@ void restir_initial_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_numbe
     check_ready(ctx);
     RenderState& R = ctx.render;
     const mipt_restir_di_settings& st = R.restir;
-    if (ctx.emissive_count > 0)
+    // with an envmap, candidates exist without emissive triangles
+    // (InitialCandidates.h:451-459)
+    if (ctx.emissive_count > 0 || env_sampling_on(ctx))
     {
         PtData d = make_pt_data(ctx, sample_number, random_number);
         RsData r = make_rs_data(ctx);
@@ -1331,12 +1433,14 @@ void restir_presampling_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_n
     const mipt_restir_di_settings& st = R.restir;
     if (!st.do_light_presampling)
         fail(MIPT_ERR_INVALID_ARGUMENT, "light presampling is off in the ReSTIR DI settings");
-    if (ctx.emissive_count == 0)
+    const bool env = env_sampling_on(ctx);
+    if (ctx.emissive_count == 0 && !env)
         return; // LightsPresampling.h:80-82: nothing to presample
+    const float env_prob = env ? (ctx.emissive_count == 0 ? 1.0f : st.envmap_candidate_probability) : 0.0f;
     const uint32_t count = (uint32_t)st.light_presampling_number_of_subsets * (uint32_t)st.light_presampling_subset_size;
     PtData d = make_pt_data(ctx, sample_number, random_number);
-    hipLaunchKernelGGL(k_presample_lights, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, R.presampled.as<float4>(),
-                       count);
+    hipLaunchKernelGGL(k_presample_lights, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, env_prob,
+                       R.presampled.as<float4>(), count);
     HIP_CHECK(hipGetLastError());
     R.presampled_ready = true;
 }

@@ -123,6 +123,9 @@ enum LobeInfo : uint32_t {
 // (which FROM_DIFFUSE shares; FROM_DIFFUSE_TRANSMISSION does not fit).
 // flags_to_BSDF_incident_light_info keeps bits 1-6 (DI/Reservoir.h:45-48).
 constexpr uint32_t RESTIR_FLAG_UNOCCLUDED = 1u << 7;
+// RESTIR_DI_FLAGS_ENVMAP_SAMPLE (ReSTIR/DI/SampleFlags.h:16): the sample's
+// point is a direction in envmap space
+constexpr uint32_t RESTIR_FLAG_ENVMAP = 1u << 0;
 constexpr uint32_t RESTIR_FLAG_LOBE_MASK = 0x3fu << 1;
 
 enum RegMode : uint32_t { NO_REGULARIZATION = 0, REGULARIZATION_CLASSIC = 1, REGULARIZATION_MIS = 2 };

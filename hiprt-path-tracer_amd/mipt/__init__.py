@@ -142,6 +142,7 @@ class RestirDISettings(C.Structure):
         ("guard_backfacing_bsdf_candidates", C.c_int32), ("do_fused_spatiotemporal", C.c_int32),
         ("do_light_presampling", C.c_int32), ("light_presampling_number_of_subsets", C.c_int32),
         ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
+        ("envmap_candidate_probability", C.c_float),
     ]
 
 

@@ -426,6 +426,11 @@ typedef struct mipt_restir_di_settings {
     int32_t light_presampling_number_of_subsets;
     int32_t light_presampling_subset_size;
     int32_t light_presampling_tile_size;
+    /* ReSTIRDIInitialCandidatesSettings::envmap_candidate_probability
+     * (ReSTIRDISettings.h:22), default 0.5: with an envmap, the probability
+     * that a light candidate (or a presampled light) is an envmap sample; 1
+     * when the scene has no emissive triangle. */
+    float envmap_candidate_probability;
 } mipt_restir_di_settings;
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s);

@@ -209,6 +209,7 @@ typedef struct {
     int32_t light_presampling_number_of_subsets;
     int32_t light_presampling_subset_size;
     int32_t light_presampling_tile_size;
+    float envmap_candidate_probability;
 } orc_restir_settings;
 
 /* The BSDF tables (the .f32 files of hiprt-path-tracer_amd/data, converted from the

@@ -1916,14 +1916,19 @@ static olight sample_emissive(const rctx* R, uint32_t* st)
     return sample_emissive_with(R, R->s.direct_light_sampling_base_strategy, st);
 }
 
-static float emissive_hit_area_pdf(const rctx* R, float area, rgb emission)
+static float emissive_hit_area_pdf_with(const rctx* R, int32_t strategy, float area, rgb emission)
 {
     float pdf = 1.0f / area;
-    if (R->s.direct_light_sampling_base_strategy == 1)
+    if (strategy == 1)
         pdf *= (lum(emission) * area) / R->alias_sum;
     else
         pdf /= (float)R->ne;
     return pdf;
+}
+
+static float emissive_hit_area_pdf(const rctx* R, float area, rgb emission)
+{
+    return emissive_hit_area_pdf_with(R, R->s.direct_light_sampling_base_strategy, area, emission);
 }
 
 static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi, v3 view, float path_r, uint32_t* st,
@@ -2000,17 +2005,34 @@ static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi,
 typedef struct { v3 point; float target; int32_t M; int32_t tri; float ucw; uint32_t flags; float wsum; } ores;
 
 #define RESTIR_UNOCCLUDED (1u << 7)
+#define RESTIR_ENVMAP (1u << 0) /* RESTIR_DI_FLAGS_ENVMAP_SAMPLE (SampleFlags.h:16) */
 #define RESTIR_LOBE_MASK (0x3fu << 1) /* flags_to_BSDF_incident_light_info keeps bits 1-6 */
 #define FROM_GLASS_REFRACT_BIT (1u << 6)
+
+/* ReSTIR_DI_get_light_sample_direction (DI/TargetFunction.h:13-29): an
+ * envmap sample holds a direction in envmap space and lies at 1e35 */
+static v3 sample_dir(const rctx* R, v3 p, v3 point, uint32_t flags, float* dist)
+{
+    if (flags & RESTIR_ENVMAP)
+    {
+        *dist = 1.0e35f;
+        return env_rot(R->env_to_world, point);
+    }
+    v3 to = vsub(point, p);
+    *dist = vlen(to);
+    return vdivs(to, *dist);
+}
 
 static rgb restir_final_shading(const rctx* R, const omat* m, const hitinfo* hi, v3 view, size_t pix, ovol* vs)
 {
     const ores* r = (const ores*)R->restir_out + pix;
-    if (r->ucw <= 0.0f || r->tri < 0) /* NaN weight sum with no selected sample */
+    int env = (r->flags & RESTIR_ENVMAP) != 0;
+    /* NaN weight sum with no selected sample; validate_reservoir
+     * (FinalShading.h:103-108) for an envmap sample without the envmap */
+    if (r->ucw <= 0.0f || (r->tri < 0 && !env) || (env && !env_on(R)))
         return C1(0.0f);
-    v3 to = vsub(r->point, hi->p);
-    float dist = vlen(to);
-    v3 dir = vdivs(to, dist);
+    float dist;
+    v3 dir = sample_dir(R, hi->p, r->point, r->flags, &dist);
     int in_shadow = 0;
     if (r->flags & RESTIR_UNOCCLUDED)
         in_shadow = 0;
@@ -2029,11 +2051,21 @@ static rgb restir_final_shading(const rctx* R, const omat* m, const hitinfo* hi,
         cosine = fabsf(cosine);
     if (!(cosine > 0.0f))
         return C1(0.0f);
-    rgb em = R->mats[R->mat_idx[r->tri]].emission;
-    v3 a, b, cc;
-    tri(R, r->tri, &a, &b, &cc);
-    float conv = hmax(0.0f, vdot(vnorm(vcross(vsub(b, a), vsub(cc, a))), vneg(dir)));
-    conv /= dist * dist;
+    rgb em;
+    float conv = 1.0f;
+    if (env)
+    {
+        float epdf;
+        em = env_eval(R, dir, &epdf);
+    }
+    else
+    {
+        em = R->mats[R->mat_idx[r->tri]].emission;
+        v3 a, b, cc;
+        tri(R, r->tri, &a, &b, &cc);
+        conv = hmax(0.0f, vdot(vnorm(vcross(vsub(b, a), vsub(cc, a))), vneg(dir)));
+        conv /= dist * dist;
+    }
     return cscale(cscale(cmul(cscale(f, r->ucw), em), cosine), conv);
 }
 
@@ -2122,7 +2154,7 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
         {
             /* estimate_direct_lighting (Lights.h:364-398) */
             rgb direct = C1(0.0f);
-            if (R->ne > 0 && R->restir_out && bounce == 0)
+            if ((R->ne > 0 || env_on(R)) && R->restir_out && bounce == 0)
                 direct = restir_final_shading(R, m, &hi, view, pix, &vs);
             else if (R->ne > 0)
                 direct = cadd(C1(0.0f), sample_one_light_mis(R, m, &hi, view, path_r, &st, &vs));
@@ -2811,11 +2843,11 @@ static osurf surf_of(const orc_frame* f, const gbuf* g, size_t pix)
 static float target_fn(const orc_frame* f, const osurf* s, v3 point, int32_t tri_i, uint32_t flags, int with_vis)
 {
     const rctx* R = &f->R;
-    if (tri_i == -1 || s->mat < 0)
+    int env = (flags & RESTIR_ENVMAP) != 0;
+    if ((tri_i == -1 && !env) || s->mat < 0)
         return 0.0f;
-    v3 to = vsub(point, s->p);
-    float dist = vlen(to);
-    v3 dir = vdivs(to, dist);
+    float dist;
+    v3 dir = sample_dir(R, s->p, point, flags, &dist);
     float cosine = vdot(s->sn, dir);
     if (cosine <= 0.0f)
         return 0.0f;
@@ -2823,11 +2855,21 @@ static float target_fn(const orc_frame* f, const osurf* s, v3 point, int32_t tri
     bctx c = {s->view, s->sn, s->gn, dir, (int)(flags & RESTIR_LOBE_MASK), NO_REG, 0.0f, &vs, 0};
     float bpdf;
     rgb fc = principled_eval(R, &R->mats[s->mat], &c, &bpdf);
-    rgb em = R->mats[R->mat_idx[tri_i]].emission;
-    v3 a, b, cc;
-    tri(R, tri_i, &a, &b, &cc);
-    float geom = hmax(0.0f, vdot(vnorm(vcross(vsub(b, a), vsub(cc, a))), vneg(dir)));
-    geom /= dist * dist;
+    rgb em;
+    float geom = 1.0f;
+    if (env)
+    {
+        float epdf;
+        em = env_eval(R, dir, &epdf);
+    }
+    else
+    {
+        em = R->mats[R->mat_idx[tri_i]].emission;
+        v3 a, b, cc;
+        tri(R, tri_i, &a, &b, &cc);
+        geom = hmax(0.0f, vdot(vnorm(vcross(vsub(b, a), vsub(cc, a))), vneg(dir)));
+        geom /= dist * dist;
+    }
     float t = lum(cscale(cscale(cmul(fc, em), cosine), geom));
     if (t == 0.0f)
         return 0.0f;
@@ -2893,9 +2935,8 @@ static void visibility_kill(const orc_frame* f, ores* r, v3 p, int32_t prim)
         return;
     if (r->flags & RESTIR_UNOCCLUDED)
         return;
-    v3 to = vsub(r->point, p);
-    float dist = vlen(to);
-    v3 dir = vdivs(to, dist);
+    float dist;
+    v3 dir = sample_dir(&f->R, p, r->point, r->flags, &dist);
     orc_hit h = trace1(&f->R, p, dir, prim, INFINITY);
     if (h.prim >= 0 && h.t < dist - 1.0e-4f)
         r->ucw = -1.0f;
@@ -2919,7 +2960,10 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
     f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     const rctx* R = &f->R;
     const int32_t nl = f->rs.number_of_initial_light_candidates, nb = f->rs.number_of_initial_bsdf_candidates;
-    if (R->ne > 0)
+    /* InitialCandidates.h:451-459: envmap candidates with an envmap (an
+     * intensity of 0 counts as none), only those without emissive triangles */
+    const float env_prob = env_on(R) ? (R->ne == 0 ? 1.0f : f->rs.envmap_candidate_probability) : 0.0f;
+    if (R->ne > 0 || env_on(R))
     {
         set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic)
@@ -2937,6 +2981,7 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                 float cosine = 0.0f, pdf = 0.0f, dist = 0.0f;
                 v3 dir = V(0, 0, 0), point = V(0, 0, 0);
                 int32_t tri_i = -1;
+                uint32_t sflags = 0u;
                 if (f->presampled)
                 {
                     /* use_presampled_light_candidate (InitialCandidates.h:33-90) */
@@ -2952,12 +2997,11 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                     tri_i = P->tri;
                     point = P->point;
                     pdf = P->pdf;
-                    v3 to = vsub(point, s.p);
-                    dist = vlen(to);
-                    dir = vdivs(to, dist);
+                    sflags = P->flags;
+                    dir = sample_dir(R, s.p, point, sflags, &dist);
                     cosine = vdot(s.sn, dir);
                 }
-                else if (orc_rand01(&st) > 0.0f)
+                else if (orc_rand01(&st) > env_prob)
                 {
                     olight L = sample_emissive(R, &st);
                     tri_i = L.tri;
@@ -2969,8 +3013,20 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                     if (pdf > 0.0f)
                     {
                         cosine = hmax(0.0f, vdot(s.sn, dir));
-                        pdf *= (1.0f - 0.0f);
+                        pdf *= (1.0f - env_prob);
                     }
+                }
+                else if (env_prob > 0.0f) /* a draw of 0 with no envmap: no candidate */
+                {
+                    /* the envmap sample branch of sample_fresh_light_candidate
+                     * (InitialCandidates.h:146-175), stored in envmap space */
+                    v3 edir;
+                    env_sample(R, &edir, &pdf, &st);
+                    cosine = hmax(0.0f, vdot(edir, s.sn));
+                    pdf *= env_prob;
+                    point = env_rot(R->env_to_env, edir);
+                    sflags = RESTIR_ENVMAP;
+                    dir = sample_dir(R, s.p, point, sflags, &dist);
                 }
                 float w = 0.0f, sample_target = 0.0f;
                 if (cosine > 0.0f && pdf > 0.0f)
@@ -2979,23 +3035,27 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                     bctx c = {s.view, s.sn, s.gn, dir, NO_INFO, NO_REG, 0.0f, &vs, 0};
                     float bsdf_pdf;
                     principled_eval(R, m, &c, &bsdf_pdf);
-                    float target = target_fn(f, &s, point, tri_i, 0u, 0);
+                    float target = target_fn(f, &s, point, tri_i, sflags, 0);
                     if (!(bsdf_pdf <= 0.0f))
                     {
-                        v3 a, b, cc;
-                        tri(R, tri_i, &a, &b, &cc);
-                        v3 ln = vcross(vsub(b, a), vsub(cc, a));
-                        float nl_len = vlen(ln);
-                        ln = vdivs(ln, nl_len);
-                        float cl = hmax(0.0f, vdot(ln, vneg(dir)));
-                        float lpdf = cl < 1.0e-8f ? 0.0f : (pdf / (1.0f - 0.0f)) * (dist * dist) / cl;
-                        lpdf *= (1.0f - 0.0f);
+                        float lpdf = pdf; /* an envmap pdf is in solid angle already */
+                        if (!(sflags & RESTIR_ENVMAP))
+                        {
+                            v3 a, b, cc;
+                            tri(R, tri_i, &a, &b, &cc);
+                            v3 ln = vcross(vsub(b, a), vsub(cc, a));
+                            float nl_len = vlen(ln);
+                            ln = vdivs(ln, nl_len);
+                            float cl = hmax(0.0f, vdot(ln, vneg(dir)));
+                            lpdf = cl < 1.0e-8f ? 0.0f : (pdf / (1.0f - env_prob)) * (dist * dist) / cl;
+                            lpdf *= (1.0f - env_prob);
+                        }
                         float mis = lpdf == 0.0f ? 0.0f : lpdf / ((float)nl * lpdf + (float)nb * bsdf_pdf);
                         w = mis * target / pdf;
                         sample_target = target;
                     }
                 }
-                add_candidate(&res, point, tri_i, sample_target, 0u, w, &st);
+                add_candidate(&res, point, tri_i, sample_target, sflags, w, &st);
             }
             for (int32_t i = 0; i < nb; i++)
             {
@@ -3008,7 +3068,23 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                     continue;
                 orc_hit h = trace1(R, s.p, dir, s.prim, INFINITY);
                 if (!(h.prim >= 0 && h.t < 1.0e35f - 1.0e-4f))
+                {
+                    /* an envmap hit becomes an envmap sample
+                     * (InitialCandidates.h:396-436), refractions excluded */
+                    if (env_on(R) && vdot(s.sn, dir) > 0.0f)
+                    {
+                        float epdf;
+                        env_eval(R, dir, &epdf);
+                        v3 point = env_rot(R->env_to_env, dir);
+                        uint32_t flags = (RESTIR_UNOCCLUDED | RESTIR_ENVMAP | (uint32_t)c.info) & 0xffu;
+                        float target = target_fn(f, &s, point, -1, flags, 0);
+                        epdf *= env_prob;
+                        float mis = bpdf == 0.0f ? 0.0f : bpdf / ((float)nb * bpdf + (float)nl * epdf);
+                        float w = mis * target / bpdf;
+                        add_candidate(&res, point, -1, target, flags, w, &st);
+                    }
                     continue;
+                }
                 rgb em = R->mats[R->mat_idx[h.prim]].emission;
                 if (cblack(em))
                     continue;
@@ -3022,10 +3098,13 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
                 float lpdf = 0.0f;
                 if (!(vdot(dir, s.sn) < 0.0f))
                 {
-                    float pa = emissive_hit_area_pdf(R, R->areas[h.prim], em);
+                    /* with presampling, the presampling strategy's pdf:
+                     * power (InitialCandidates.h:360-366) */
+                    float pa = emissive_hit_area_pdf_with(R, f->presampled ? 1 : R->s.direct_light_sampling_base_strategy,
+                                                          R->areas[h.prim], em);
                     lpdf = cl < 1.0e-8f ? 0.0f : pa * (h.t * h.t) / cl;
                 }
-                lpdf *= (1.0f - 0.0f);
+                lpdf *= (1.0f - env_prob);
                 float mis = bpdf == 0.0f ? 0.0f : bpdf / ((float)nb * bpdf + (float)nl * lpdf);
                 float pdf_area = bpdf;
                 pdf_area /= (h.t * h.t);
@@ -3049,14 +3128,16 @@ void orc_frame_restir_initial(orc_frame* f, uint32_t sample_number, uint32_t rn,
  * (the temporal neighbour is the pixel itself, UtilsTemporal.h:37-41), with
  * the pairwise MIS weights (TemporalMISWeight.h:164-254). */
 /* ReSTIR_DI_LightsPresampling (DI/LightsPresampling.h:74-112): one light
- * per slot, seeded like a pixel by its slot index; an envmap draw with
- * probability 0, then an emissive triangle by power
+ * per slot, seeded like a pixel by its slot index; the envmap with the
+ * envmap candidate probability (presample_envmap, :22-35, direction in
+ * envmap space), else an emissive triangle by power
  * (ReSTIR_DI_LightPresamplingStrategy, ReSTIRDIOptions.h:162). */
 void orc_frame_restir_presample(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
 {
-    if (!f->presampled || f->R.ne == 0)
+    if (!f->presampled || (f->R.ne == 0 && !env_on(&f->R)))
         return;
     const rctx* R = &f->R;
+    const float env_prob = env_on(R) ? (R->ne == 0 ? 1.0f : f->rs.envmap_candidate_probability) : 0.0f;
     set_threads(nthreads);
 #pragma omp parallel for schedule(static)
     for (int64_t x = 0; x < (int64_t)f->presampled_count; x++)
@@ -3065,7 +3146,15 @@ void orc_frame_restir_presample(orc_frame* f, uint32_t sample_number, uint32_t r
         struct opresampled P;
         memset(&P, 0, sizeof(P));
         P.tri = -1;
-        if (!(orc_rand01(&st) < 0.0f))
+        if (orc_rand01(&st) < env_prob)
+        {
+            v3 d;
+            P.radiance = env_sample(R, &d, &P.pdf, &st);
+            P.point = env_rot(R->env_to_env, d);
+            P.pdf *= env_prob;
+            P.flags = RESTIR_ENVMAP;
+        }
+        else
         {
             olight L = sample_emissive_with(R, 1, &st);
             if (L.area_pdf > 0.0f)
@@ -3073,7 +3162,7 @@ void orc_frame_restir_presample(orc_frame* f, uint32_t sample_number, uint32_t r
                 P.point = L.point;
                 P.normal = L.normal;
                 P.tri = L.tri;
-                P.pdf = L.area_pdf * 1.0f;
+                P.pdf = L.area_pdf * (1.0f - env_prob);
                 P.radiance = L.emission;
             }
         }

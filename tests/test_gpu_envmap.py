@@ -11,7 +11,9 @@ against the oracle's restatement:
   colour or none: first-hit IDs and every pixel bit-identical (the bar is
   1e-4 relative on >= 99.9 % of pixels);
 - the same in a batch of views and under graph replay;
-- the combinations this build does not implement are refused.
+- ReSTIR DI with envmap candidates, reservoirs and frames bit-identical;
+- the combination this build does not implement (the radiance cache with
+  an envmap) is refused.
 """
 import numpy as np
 import pytest
@@ -177,6 +179,81 @@ def test_envmap_views_and_graph(mipt_mod, torch_mod, envmap):
         off += f1.size
 
 
+def render_restir_both(mipt_mod, oracle_mod, scene, settings, restir, world, env, frames):
+    ctx = _ctx(mipt_mod, scene, settings, world, env)
+    ctx.set_restir_di_settings(restir)
+    ctx.set_render_settings(settings)
+    ctx.set_camera(scene.camera)
+    draws = ctx.draws_per_sample()
+    R = oracle_mod.Renderer(scene, settings, restir, world=world, envmap=env)
+    for f in range(frames):
+        ctx.render_sample(f)
+        R.render_frame(f, [mipt_mod.host_random_number(f * draws + k) for k in range(draws)])
+    assert np.array_equal(ctx.read_gbuffer_prims(), R.gb_prims), "first-hit IDs differ"
+    for which in (0, 1, 2, 3):
+        g, c = ctx.read_reservoirs(which), R.read_reservoirs(which)
+        same = (g.view(np.uint32).reshape(len(g), -1) == c.view(np.uint32).reshape(len(c), -1)).all(axis=1)
+        assert same.all(), (which, np.nonzero(~same)[0][:5])
+    fb, fb64 = ctx.read_framebuffer(fp64=True)
+    rej = ctx.rejected_samples()
+    ctx.close()
+    identical = int((fb.reshape(-1, 3) == R.fb.reshape(-1, 3)).all(axis=1).sum())
+    envs = int(((R.read_reservoirs(3)["flags"] & 1) == 1).sum())
+    print(f"\n{settings.width}x{settings.height} x{frames}: {identical} of {fb.size // 3} pixels bit-identical; "
+          f"{envs} envmap reservoirs; rejected gpu {rej} cpu {R.rejected}")
+    assert identical == fb.size // 3 and np.array_equal(fb64, R.fb64) and rej == R.rejected
+    return fb, R, envs
+
+
+RESTIR_VARIANTS = {
+    "default": dict(),
+    "initial_only": dict(do_temporal_reuse_pass=0, do_spatial_reuse_pass=0),
+    "two_spatial_prob_02": dict(spatial_number_of_passes=2, envmap_candidate_probability=0.2),
+    "fused_presampled": dict(do_fused_spatiotemporal=1, do_light_presampling=1,
+                             light_presampling_number_of_subsets=16, light_presampling_subset_size=64),
+    "envmap_only": dict(lights=False),
+    "rotated": dict(rotated=True),
+}
+
+
+@pytest.mark.parametrize("variant", list(RESTIR_VARIANTS))
+def test_restir_with_envmap(mipt_mod, oracle_mod, torch_mod, envmap, variant):
+    """ReSTIR DI with envmap candidates (light candidates and presampled
+    lights drawn from the envmap with the envmap candidate probability, BSDF
+    candidates that leave the scene) through every pass: reservoirs and
+    frames bit-identical."""
+    kw = dict(RESTIR_VARIANTS[variant])
+    lights = kw.pop("lights", True)
+    rotated = kw.pop("rotated", False)
+    scene = sky_scene(mipt_mod, 64, 64, lights=lights)
+    settings = mipt_mod.default_render_settings(width=64, height=64, nb_bounces=2, direct_light_sampling_strategy=5)
+    restir = mipt_mod.default_restir_di_settings(**kw)
+    if rotated:
+        to_world, to_env = rotation_y(60.0)
+        world = mipt_mod.default_world_settings(ambient_light_type=2, envmap_to_world=to_world,
+                                                world_to_envmap=to_env)
+    else:
+        world = mipt_mod.default_world_settings(ambient_light_type=2)
+    fb, R, envs = render_restir_both(mipt_mod, oracle_mod, scene, settings, restir, world, envmap, 4)
+    assert envs > 0 and fb.mean() > 0.01
+
+
+def test_restir_with_envmap_config2(mipt_mod, oracle_mod, torch_mod, envmap):
+    """The at-scale ReSTIR DI parameters (32 + 1 candidates, M-cap 20, two
+    spatial passes, fused pass, default presampling) on the 1M-triangle scene
+    with its 4096 emitters and the envmap, 240x135."""
+    scene = mipt_mod.generate_scene(2, 240, 135)
+    for m in scene.materials:
+        m.specular_transmission = 0.0
+    settings = mipt_mod.default_render_settings(width=240, height=135, nb_bounces=5, direct_light_sampling_strategy=5,
+                                                direct_light_sampling_base_strategy=1)
+    restir = mipt_mod.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20,
+                                                 spatial_number_of_passes=2, do_fused_spatiotemporal=1,
+                                                 do_light_presampling=1)
+    world = mipt_mod.default_world_settings(ambient_light_type=2, envmap_intensity=0.5)
+    render_restir_both(mipt_mod, oracle_mod, scene, settings, restir, world, envmap, 3)
+
+
 def test_envmap_errors(mipt_mod, torch_mod, envmap):
     scene = sky_scene(mipt_mod, 32, 32)
     ctx = mipt_mod.Context(0)
@@ -187,9 +264,12 @@ def test_envmap_errors(mipt_mod, torch_mod, envmap):
     with pytest.raises(mipt_mod.MiptError):  # two channels
         ctx.set_envmap(np.ones((8, 16, 2), np.float32))
     ctx.set_envmap(envmap)
-    ctx.set_render_settings(mipt_mod.default_render_settings(width=32, height=32, direct_light_sampling_strategy=5))
+    with pytest.raises(mipt_mod.MiptError):  # a probability outside [0, 1]
+        ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(envmap_candidate_probability=1.5))
+    ctx.set_render_settings(mipt_mod.default_render_settings(width=32, height=32, radiance_cache=1))
+    ctx.set_radiance_cache(mipt_mod.default_radiance_cache_settings())
     ctx.set_camera(scene.camera)
-    with pytest.raises(mipt_mod.MiptError):  # ReSTIR DI envmap candidates: not implemented
+    with pytest.raises(mipt_mod.MiptError):  # the radiance cache with an envmap: not implemented
         ctx.render_sample(0)
     ctx.set_render_settings(mipt_mod.default_render_settings(width=32, height=32))
     ctx.set_camera(scene.camera)

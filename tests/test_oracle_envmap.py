@@ -250,3 +250,59 @@ def test_uniform_ambient_light(mipt_mod, oracle_mod):
     none = mipt_mod.default_world_settings(ambient_light_type=0, uniform_light_color=c)
     img_n, _ = render(mipt_mod, oracle_mod, scene, 1, world=none, nb_bounces=1)
     assert (img_n == 0).all()
+
+
+def restir_render(mipt_mod, oracle_mod, scene, frames, world, env, strategy=5, w=W, h=H, **restir):
+    s = mipt_mod.default_render_settings(width=w, height=h, nb_bounces=0, direct_light_sampling_strategy=strategy)
+    rs = mipt_mod.default_restir_di_settings(**restir)
+    R = oracle_mod.Renderer(scene, s, rs, world=world, envmap=env)
+    if strategy == 5:
+        draws = 3 + (3 + rs.spatial_number_of_passes - 1 if rs.do_fused_spatiotemporal else
+                     2 * rs.do_temporal_reuse_pass + rs.spatial_number_of_passes * rs.do_spatial_reuse_pass)
+    else:
+        draws = 2
+    for f in range(frames):
+        R.render_frame(f, [mipt_mod.host_random_number(f * draws + k) for k in range(draws)])
+    return R.fb64.reshape(-1, 3) / frames, R
+
+
+def test_restir_floor_under_a_two_tone_sky(mipt_mod, oracle_mod):
+    """ReSTIR DI initial candidates, all envmap samples (no emissive
+    triangle): the floor converges to albedo x L_up as with MIS."""
+    scene = floor_scene(mipt_mod, W, H)
+    albedo = np.float64(scene.materials[0].base_color[0])
+    world = mipt_mod.default_world_settings(ambient_light_type=2)
+    env = two_tone(64, 32, UP, DOWN)
+    img, R = restir_render(mipt_mod, oracle_mod, scene, 64, world, env, do_temporal_reuse_pass=0,
+                           do_spatial_reuse_pass=0)
+    floor = R.gb_prims >= 0
+    res = R.read_reservoirs(3)
+    assert ((res["flags"][floor] & 1) == 1)[res["ucw"][floor] > 0].all()  # envmap samples
+    assert np.allclose(img[interior(floor)].mean(axis=0), albedo * UP.astype(np.float64), rtol=0.01)
+
+
+@pytest.mark.parametrize("case", ["initial", "temporal_spatial", "fused_presampled", "envmap_only"])
+def test_restir_with_envmap_unbiased(mipt_mod, oracle_mod, case):
+    """ReSTIR DI over emissive triangles and the envmap (candidates drawn from
+    either, BSDF candidates that leave the scene become envmap samples)
+    averages to the MIS light + BSDF + envmap image at bounce 0 (1.5 %)."""
+    rng = np.random.default_rng(9)
+    env = (np.exp(rng.normal(0, 0.5, (32, 64, 3))) * 0.5).astype(np.float32)
+    scene = sky_scene(mipt_mod, W, H, lights=case != "envmap_only")
+    world = mipt_mod.default_world_settings(ambient_light_type=2)
+    ref, _ = restir_render(mipt_mod, oracle_mod, scene, 384, world, env, strategy=3)
+    kw = {"initial": dict(do_temporal_reuse_pass=0, do_spatial_reuse_pass=0),
+          "temporal_spatial": dict(),
+          "fused_presampled": dict(do_fused_spatiotemporal=1, do_light_presampling=1,
+                                   light_presampling_number_of_subsets=16, light_presampling_subset_size=64),
+          "envmap_only": dict(do_temporal_reuse_pass=0, do_spatial_reuse_pass=0)}[case]
+    img, R = restir_render(mipt_mod, oracle_mod, scene, 256, world, env, **kw)
+    assert np.allclose(img.mean(axis=0), ref.mean(axis=0), rtol=0.015), (img.mean(axis=0), ref.mean(axis=0))
+    res = R.read_reservoirs(3)
+    live = res["ucw"] > 0
+    envs = (res["flags"] & 1) == 1
+    assert (res["tri"][live & envs] == -1).all()
+    if case == "envmap_only":
+        assert envs[live].all()
+    else:
+        assert envs[live].any() and (~envs[live]).any()

@@ -321,3 +321,24 @@ def test_windowed_fused_frames_equal_full_frames(mipt_mod, oracle_mod, passes):
         a, b = win.read_reservoirs(which)[sel], full.read_reservoirs(which)[sel]
         assert a.tobytes() == b.tobytes(), which
     assert win.fb.reshape(-1, 3)[sel].tobytes() == full.fb.reshape(-1, 3)[sel].tobytes()
+
+
+def test_zero_xorshift_state_pixel_under_freeze(mipt_mod, oracle_mod):
+    """Regression: with freeze_random the seed of pixel 60 is wang_hash(61)
+    = 0, a xorshift state that stays 0, so each of its draws is 0.0. The light
+    candidates' `draw > envmap probability` test then fails with probability
+    0 too; without an envmap that draw must leave no candidate rather than
+    sample an envmap that does not exist (the read of an empty alias table
+    that faulted). Two frames of ReSTIR DI render, and that pixel's initial
+    reservoir holds no envmap sample."""
+    assert oracle_mod.lib().orc_wang_hash(61) == 0
+    scene = mipt_mod.generate_scene(1, 64, 64)
+    s = mipt_mod.default_render_settings(width=64, height=64, direct_light_sampling_strategy=5, freeze_random=1)
+    rs = mipt_mod.default_restir_di_settings()
+    R = oracle_mod.Renderer(scene, s, rs)
+    draws = _draws(s, rs)
+    for f in range(2):
+        R.render_frame(f, _frame_numbers(mipt_mod, draws)(f))
+    init = R.read_reservoirs(0)
+    assert R.gb_prims[60] >= 0 and (init["flags"][60] & 1) == 0
+    assert np.isfinite(R.fb).all()
