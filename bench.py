@@ -21,7 +21,9 @@ FP64 4096-spp reference: "cache_vs_plain" with the cache as configuration 4
 states it (queried at bounce 2, trained on 1/16 of the paths extended 3
 bounces), "cache_vs_plain_tuned" with the library's defaults (bounce 1,
 1/64 of the paths). "restir_render" times ReSTIR DI frames of the
-1M-triangle scene at 1920x1080.
+1M-triangle scene at 1920x1080. "full_stack_render" times the full stack at
+scale: the 10M-triangle scene with its 16,384 emitters and a 4096 x 2048
+envmap at 3840x2160, ReSTIR DI, 8 bounces and the radiance cache.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--bvh sah|lbvh] [--no-cpu] [--no-render] [--no-cache]
        [--cache-ref-spp 4096] [--cache-plain-spp 32] [--cache-warmup 2] [--no-restir] [--restir-layered]
@@ -78,6 +80,8 @@ def main():
                     help="with --frames: the path pass's shading order (default: the library's, auto)")
     ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI leg")
     ap.add_argument("--no-views", action="store_true", help="skip the batched-views leg (configuration 3)")
+    ap.add_argument("--no-full-stack", action="store_true",
+                    help="skip the full-stack leg (10M triangles, envmap, 3840x2160)")
     ap.add_argument("--views", type=int, default=96, help="views of the batched-views leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
@@ -233,6 +237,10 @@ def main():
         restir_layered = restir_render_leg(mipt, torch, np, layered=True)
     else:
         restir_layered = None
+    full_stack = None
+    if not (args.no_render or args.no_full_stack):
+        step("full-stack leg: 10M-triangle scene, 4096x2048 envmap, 3840x2160, ReSTIR DI, 8 bounces, cache")
+        full_stack = full_stack_leg(mipt, torch, np)
     views = None
     if not (args.no_render or args.no_views):
         step(f"batched views leg: {args.views} views of the config-1 scene, 4 spp, 5 bounces")
@@ -286,6 +294,7 @@ def main():
         "restir_render_layered": restir_layered,
         "restir_render_fused": restir_fused,
         "views_render": views,
+        "full_stack_render": full_stack,
         "cache_vs_plain": cache,
         "cache_vs_plain_tuned": cache_tuned,
     }
@@ -559,6 +568,76 @@ def restir_render_leg(mipt, torch, np, layered, fused=False):
                                   "M_max": int(res["M"].max()),
                                   "ucw_positive": int((res["ucw"] > 0).sum()),
                                   "ucw_nonfinite": int((~np.isfinite(res["ucw"])).sum())},
+            "mean_radiance": [round(float(x), 5) for x in (fb.reshape(-1, 3) / (2 + frames)).mean(axis=0)]}
+
+
+def full_stack_leg(mipt, torch, np):
+    """The full stack at scale: the 10,479,104-triangle scene (8 fBm grids,
+    3000 flattened instances, 16,384 emitters of power log-uniform over three
+    decades, every Principled layer per object) under the 4096 x 2048
+    procedural sun + sky envmap (alias-table sampled, ReSTIR DI envmap
+    candidates with probability 0.5) at 3840x2160: ReSTIR DI with 32 light +
+    1 BSDF candidates, temporal reuse (M-cap 20) and 2 spatial passes of 5
+    neighbours in 16 px, 8 bounces, the radiance cache queried at bounce 2
+    and trained on 1/16 of the paths. ReSTIR DI does not take dispersive
+    glass yet: the dispersion of transmissive materials is set to 0 (the
+    other layers stay). Times 3 frames after 2 warm-up frames and 8 cache
+    warm-up frames; reports the SAH build of the 10M triangles."""
+    W, H = 3840, 2160
+    t0 = time.perf_counter()
+    scene = mipt.generate_scene(5, W, H)
+    gen_s = time.perf_counter() - t0
+    dispersive = 0
+    for m in scene.materials:
+        if m.dispersion_scale > 0.0 and m.specular_transmission > 0.0:
+            m.dispersion_scale = 0.0
+            dispersive += 1
+    env = mipt.generate_envmap(4096, 2048)
+    ctx = mipt.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh(mipt.MIPT_BVH_SAH_BINNED)
+    info = ctx.bvh_info()
+    ctx.set_envmap(env)
+    ctx.set_world_settings(mipt.default_world_settings(ambient_light_type=mipt.MIPT_AMBIENT_ENVMAP))
+    ctx.set_radiance_cache(mipt.default_radiance_cache_settings(query_bounce=2, train_every=16, train_extension=3))
+    rs = mipt.default_restir_di_settings(number_of_initial_light_candidates=32, m_cap=20, spatial_number_of_passes=2)
+    ctx.set_restir_di_settings(rs)
+    s = mipt.default_render_settings(width=W, height=H, nb_bounces=8,
+                                     direct_light_sampling_strategy=mipt.MIPT_LSS_RESTIR_DI,
+                                     direct_light_sampling_base_strategy=mipt.MIPT_LSS_BASE_POWER, radiance_cache=1)
+    ctx.set_render_settings(s)
+    ctx.set_camera(scene.camera)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    for i in range(8):
+        ctx.cache_warmup(100000 + i, sh)
+    for f in range(2):
+        ctx.render_sample(f, sh)
+    stream.synchronize()
+    frames = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for f in range(frames):
+        ctx.render_sample(2 + f, sh)
+    e1.record(stream)
+    stream.synchronize()
+    wall_ms = (time.perf_counter() - t0) / frames * 1e3
+    gpu_ms = e0.elapsed_time(e1) / frames
+    queries, train, dropped, loss = ctx.cache_stats()
+    fb = ctx.read_framebuffer()
+    rej = ctx.rejected_samples()
+    ctx.close()
+    return {"workload": "config 5: 10M-triangle scene (8 fBm grids + 3000 flattened instances + 16,384 emitters), "
+                        "4096x2048 sun + sky envmap, 3840x2160, ReSTIR DI 32+1 candidates, temporal M-cap 20, "
+                        "2 spatial passes, envmap candidates p = 0.5, 8 bounces, radiance cache (bounce 2, 1/16); "
+                        f"dispersion off under ReSTIR DI ({dispersive} transmissive materials)",
+            "triangles": int(scene.triangle_count), "emitters": int(scene.emissive.size),
+            "scene_generation_s": round(gen_s, 2), "bvh_build_ms": round(info.build_ms, 2),
+            "bvh_nodes": int(info.node_count), "bvh_sah_cost": round(info.sah_cost, 2),
+            "frames_timed": frames, "ms_per_frame": round(gpu_ms, 2), "wall_ms_per_frame": round(wall_ms, 2),
+            "cache_queries": int(queries), "cache_training_records": int(train), "cache_loss": float(loss),
+            "rejected_samples": int(rej),
             "mean_radiance": [round(float(x), 5) for x in (fb.reshape(-1, 3) / (2 + frames)).mean(axis=0)]}
 
 

@@ -970,8 +970,6 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         // a dispersive glass draws a wavelength from the pass's stream at
         // every surface fetch (ReSTIR/Surface.h:25-43); those draws are not
         // implemented.
-        if (mipt::env_sampling_on(*ctx) && ctx->render.settings.radiance_cache)
-            mipt::fail(MIPT_ERR_UNSUPPORTED, "an environment map with the radiance cache is not implemented yet");
         if (ctx->render.scene_dispersive && ctx->render.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
             mipt::fail(MIPT_ERR_UNSUPPORTED, "ReSTIR DI with a dispersive glass material (dispersion_scale > 0 and "
                                              "specular_transmission > 0) is not implemented yet");

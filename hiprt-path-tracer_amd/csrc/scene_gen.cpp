@@ -16,6 +16,9 @@
 //             random scale/rotation/translation, 4096 emissive triangles of
 //             power log-uniform over [1, 1000]. 1,040,384 triangles. Principled
 //             parameters uniform per object.
+//   config 5: 8 fBm grids (320x320 quads each), 3000 instances of
+//             icospheres of 1280 / 5120 / 20480 tris, 16,384 emitters; every
+//             Principled layer (coat, thin film, dispersion) per object.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -399,6 +402,89 @@ void scene_config2(Builder& B, mipt_camera& cam, int32_t w, int32_t h)
     cam = make_camera({0, 14, 16}, {0, 0, 0}, {0, 1, 0}, 45.0 * M_PI / 180.0, w, h);
 }
 
+// Every layer of the Principled BSDF drawn per object, thin film and
+// dispersion included.
+mipt_material principled_full(Pcg32& rng)
+{
+    mipt_material m = principled_random(rng);
+    m.thin_film = (float)rng.uniform();
+    m.thin_film_thickness = (float)rng.uniform(100.0, 900.0);
+    m.thin_film_ior = (float)rng.uniform(1.2, 2.0);
+    m.dispersion_scale = (float)rng.uniform();
+    m.dispersion_abbe_number = (float)rng.uniform(20.0, 60.0);
+    return m;
+}
+
+// Full stack at scale: 8 displaced grids of 320 x 320 quads (1,638,400
+// triangles), 3000 instances of three base meshes (icospheres of 1280, 5120
+// and 20480 triangles, drawn with probabilities 0.70 / 0.27 / 0.03, random
+// rotation and scale) and 16,384 emissive triangles of power log-uniform
+// over three decades: about 10M triangles. Instances are flattened into the
+// one triangle list the single-level BVH takes.
+void scene_config5(Builder& B, mipt_camera& cam, int32_t w, int32_t h)
+{
+    Pcg32 rng(0x2026120200000000ULL + 5, 5);
+    const int N = 320;
+    const double ext = 24.0; // each grid spans 48 x 48, the 8 grids a 4 x 2 layout
+    uint32_t seeds[8][6];
+    for (int g = 0; g < 8; g++)
+        for (int o = 0; o < 6; o++) seeds[g][o] = rng.next();
+    for (int g = 0; g < 8; g++)
+    {
+        const double ox = -72.0 + 48.0 * (g % 4), oz = -24.0 + 48.0 * (g / 4);
+        const int32_t mat = B.add_material(principled_full(rng));
+        const int32_t base = (int32_t)(B.verts.size() / 3);
+        for (int j = 0; j <= N; j++)
+            for (int i = 0; i <= N; i++)
+            {
+                const double x = -ext + 2 * ext * i / N, z = -ext + 2 * ext * j / N;
+                B.add_vertex({ox + x, fbm(x, z, seeds[g]), oz + z});
+            }
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++)
+            {
+                const int32_t a = base + j * (N + 1) + i, b = a + 1, c = a + (N + 1), d = c + 1;
+                B.add_tri(a, c, b, mat);
+                B.add_tri(b, c, d, mat);
+            }
+    }
+    const Mesh meshes[3] = {icosphere(3), icosphere(4), icosphere(5)};
+    for (int s = 0; s < 3000; s++)
+    {
+        const double u = rng.uniform();
+        const Mesh& mesh = meshes[u < 0.70 ? 0 : (u < 0.97 ? 1 : 2)];
+        double R[3][3];
+        random_rotation(rng, R);
+        const double scale = rng.uniform(0.3, 1.8);
+        const int g = (int)rng.uniform(0, 8);
+        const double x = rng.uniform(-22, 22), z = rng.uniform(-22, 22);
+        const double y = fbm(x, z, seeds[g]) + scale * rng.uniform(0.5, 3.0);
+        B.add_mesh(mesh, R, scale, {-72.0 + 48.0 * (g % 4) + x, y, -24.0 + 48.0 * (g / 4) + z},
+                   B.add_material(principled_full(rng)));
+    }
+    for (int e = 0; e < 16384; e++)
+    {
+        const V3 c = {rng.uniform(-94, 94), rng.uniform(8, 14), rng.uniform(-46, 46)};
+        const double size = rng.uniform(0.2, 0.6);
+        const double phi = rng.uniform(0, 2 * M_PI);
+        const V3 p0 = c + size * V3{std::cos(phi), 0, std::sin(phi)};
+        const V3 p1 = c + size * V3{std::cos(phi + 2.0944), rng.uniform(-0.2, 0.2), std::sin(phi + 2.0944)};
+        const V3 p2 = c + size * V3{std::cos(phi + 4.1888), rng.uniform(-0.2, 0.2), std::sin(phi + 4.1888)};
+        const double area = 0.5 * std::sqrt(dot(cross(p1 - p0, p2 - p0), cross(p1 - p0, p2 - p0)));
+        const double power = std::pow(10.0, rng.uniform(0.0, 3.0));
+        mipt_material m = lambertian(0, 0, 0);
+        const V3 tint = {rng.uniform(0.5, 1.0), rng.uniform(0.5, 1.0), rng.uniform(0.5, 1.0)};
+        const double L = power / (area > 1e-12 ? area : 1e-12);
+        m.emission[0] = (float)(L * tint.x);
+        m.emission[1] = (float)(L * tint.y);
+        m.emission[2] = (float)(L * tint.z);
+        const int32_t mat = B.add_material(m);
+        const int32_t a = B.add_vertex(p0), b = B.add_vertex(p1), d = B.add_vertex(p2);
+        B.add_tri(a, b, d, mat);
+    }
+    cam = make_camera({0, 26, 58}, {0, 0, 0}, {0, 1, 0}, 50.0 * M_PI / 180.0, w, h);
+}
+
 template <class T> T* dup(const std::vector<T>& v)
 {
     T* p = new T[v.size() > 0 ? v.size() : 1];
@@ -423,6 +509,8 @@ extern "C" int mipt_scene_generate(int32_t config_id, int32_t width, int32_t hei
             mipt::scene_config1(B, cam, width, height);
         else if (config_id == 2)
             mipt::scene_config2(B, cam, width, height);
+        else if (config_id == 5)
+            mipt::scene_config5(B, cam, width, height);
         else
             return MIPT_ERR_INVALID_ARGUMENT;
         auto* g = new mipt_generated_scene;

@@ -523,8 +523,9 @@ typedef struct mipt_world_settings {
 
 void mipt_world_settings_default(mipt_world_settings* w);
 /* Takes effect at the next pass. MIPT_AMBIENT_ENVMAP needs an envmap set
- * with mipt_set_envmap. ReSTIR DI and the radiance cache with an envmap are
- * refused at launch (MIPT_ERR_UNSUPPORTED). */
+ * with mipt_set_envmap; ReSTIR DI then samples the envmap among its
+ * candidates (mipt_restir_di_settings.envmap_candidate_probability), and the
+ * radiance cache learns the envmap's light like any other. */
 int mipt_set_world_settings(mipt_ctx* ctx, const mipt_world_settings* w);
 int mipt_get_world_settings(const mipt_ctx* ctx, mipt_world_settings* out);
 
@@ -656,7 +657,9 @@ int mipt_rejected_samples(mipt_ctx* ctx, uint32_t* out_count);
 /* ------------------------------------------------------------------ */
 /* Procedural scenes (replace the ASSIMP SceneParser, which is out of   */
 /* scope). config 1: Cornell box + 20 icospheres (102,428 triangles);  */
-/* config 2: fBm terrain + 400 spheres + 4096 emitters (1,040,384).     */
+/* config 2: fBm terrain + 400 spheres + 4096 emitters (1,040,384);     */
+/* config 5: 8 fBm grids + 3000 instances of 3 meshes (flattened) +     */
+/* 16,384 emitters, every Principled layer (about 10M triangles).       */
 /* Seeded PCG32 (state 0x2026120200000000 + config, stream = config).  */
 /* ------------------------------------------------------------------ */
 typedef struct mipt_generated_scene {

@@ -254,6 +254,63 @@ def test_restir_with_envmap_config2(mipt_mod, oracle_mod, torch_mod, envmap):
     render_restir_both(mipt_mod, oracle_mod, scene, settings, restir, world, envmap, 3)
 
 
+def test_radiance_cache_with_envmap(mipt_mod, torch_mod, envmap):
+    """The learned radiance cache under the envmap (sky scene, 128 x 128, 4
+    bounces): queries and training records are made, no target is dropped,
+    and after warm-up its 16-spp image has the mean of an FP64 1024-spp
+    plain render (2 %) and an error within 1.25x the plain form's."""
+    w = h = 128
+    scene = sky_scene(mipt_mod, w, h)
+    world = mipt_mod.default_world_settings(ambient_light_type=2, envmap_intensity=0.5)
+
+    def ctx_for(cache):
+        s = mipt_mod.default_render_settings(width=w, height=h, nb_bounces=4, accumulate_fp64=1,
+                                             radiance_cache=1 if cache else 0)
+        ctx = mipt_mod.Context(0)
+        ctx.set_scene(scene)
+        ctx.build_bvh()
+        ctx.set_envmap(envmap)
+        ctx.set_world_settings(world)
+        if cache:
+            ctx.set_radiance_cache(mipt_mod.default_radiance_cache_settings(batch_size=1 << 14))
+        ctx.set_render_settings(s)
+        ctx.set_camera(scene.camera)
+        return ctx
+
+    def render(ctx, spp, first=1):
+        for i in range(spp):
+            ctx.render_sample(first + i - 1 if i else 0)  # sample 0 starts the sum
+        torch_mod.cuda.synchronize()
+        return ctx.read_framebuffer(fp64=True)[1] / spp
+
+    ref_ctx = ctx_for(False)
+    ref = render(ref_ctx, 1024, first=5000)
+    plain = render(ref_ctx, 16)  # a fresh 16-spp accumulation on the same context
+    ref_ctx.close()
+    cache = ctx_for(True)
+    cache.render_sample(0)
+    torch_mod.cuda.synchronize()
+    queries, train, dropped, loss = cache.cache_stats()
+    assert dropped == 0 and queries > 0 and train > 0 and np.isfinite(loss)
+    for i in range(64):
+        cache.cache_warmup(10000 + i)
+    img = render(cache, 16)
+    cache.close()
+
+    def relmse(x, r):
+        return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+    e_plain, e_cache = relmse(plain, ref), relmse(img, ref)
+    print(f"\nenvmap + cache: relMSE plain {e_plain:.4g} cache {e_cache:.4g}; mean plain {plain.mean():.4f} "
+          f"cache {img.mean():.4f} ref {ref.mean():.4f}")
+    assert np.isfinite(img).all() and (img >= 0).all()
+    # under an open sky most light arrives at the first two vertices, whose
+    # variance the cache does not touch: no gain to claim here (measured
+    # 0.00938 against 0.00913), but no loss either, and the same mean
+    assert e_cache < 1.25 * e_plain
+    assert abs(img.mean() - ref.mean()) < 0.02 * ref.mean()
+
+
 def test_envmap_errors(mipt_mod, torch_mod, envmap):
     scene = sky_scene(mipt_mod, 32, 32)
     ctx = mipt_mod.Context(0)
@@ -266,11 +323,6 @@ def test_envmap_errors(mipt_mod, torch_mod, envmap):
     ctx.set_envmap(envmap)
     with pytest.raises(mipt_mod.MiptError):  # a probability outside [0, 1]
         ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(envmap_candidate_probability=1.5))
-    ctx.set_render_settings(mipt_mod.default_render_settings(width=32, height=32, radiance_cache=1))
-    ctx.set_radiance_cache(mipt_mod.default_radiance_cache_settings())
-    ctx.set_camera(scene.camera)
-    with pytest.raises(mipt_mod.MiptError):  # the radiance cache with an envmap: not implemented
-        ctx.render_sample(0)
     ctx.set_render_settings(mipt_mod.default_render_settings(width=32, height=32))
     ctx.set_camera(scene.camera)
     ctx.render_sample(0)
