@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--no-full-stack", action="store_true",
                     help="skip the full-stack leg (10M triangles, envmap, 3840x2160)")
     ap.add_argument("--views", type=int, default=96, help="views of the batched-views leg")
+    ap.add_argument("--no-bake", action="store_true", help="skip the LUT baker leg")
     ap.add_argument("--restir-layered", action="store_true",
                     help="also time the ReSTIR DI leg with the config-2 scene's full layered materials")
     ap.add_argument("--bvh", choices=["lbvh", "sah"], default="sah",
@@ -245,6 +246,10 @@ def main():
     if not (args.no_render or args.no_views):
         step(f"batched views leg: {args.views} views of the config-1 scene, 4 spp, 5 bounces")
         views = views_render_leg(mipt, torch, np, args.views)
+    bake = None
+    if not args.no_bake:
+        step("LUT baker leg: the five energy-compensation tables at the reference's settings")
+        bake = lut_bake_leg(mipt, np, args.no_cpu)
     cache = cache_tuned = None
     if not args.no_cache:
         step(f"radiance cache vs plain legs: config 1, 1920x1080; FP64 reference of {args.cache_ref_spp} spp")
@@ -295,6 +300,7 @@ def main():
         "restir_render_fused": restir_fused,
         "views_render": views,
         "full_stack_render": full_stack,
+        "lut_bake": bake,
         "cache_vs_plain": cache,
         "cache_vs_plain_tuned": cache_tuned,
     }
@@ -639,6 +645,63 @@ def full_stack_leg(mipt, torch, np):
             "cache_queries": int(queries), "cache_training_records": int(train), "cache_loss": float(loss),
             "rejected_samples": int(rej),
             "mean_radiance": [round(float(x), 5) for x in (fb.reshape(-1, 3) / (2 + frames)).mean(axis=0)]}
+
+
+def lut_bake_leg(mipt, np, no_cpu):
+    """The energy-compensation LUT baker: the conductor, glossy dielectric,
+    glass (entering, exiting) and thin-glass tables at the reference's sizes
+    and sample counts (GPUBakerConstants.h, *DirectionalAlbedoSettings.h),
+    one lane per texel (the reference's summation order), each table baked
+    into HBM and timed with hipEvents around its kernel launches; compared
+    through the Radiance file's 8-bit truncation with the tables the
+    reference ships. The CPU figure is the oracle's restatement on 32 texels
+    of each table at the same sample counts, on every host core."""
+    ctx = mipt.Context(0)
+    tables = []
+    total_ms = total_samples = 0.0
+    for name, table in (("conductor", mipt.MIPT_LUT_GGX_CONDUCTOR),
+                        ("glossy_dielectric", mipt.MIPT_LUT_GLOSSY_DIELECTRIC),
+                        ("glass", mipt.MIPT_LUT_GGX_GLASS), ("glass_inverse", mipt.MIPT_LUT_GGX_GLASS_INVERSE),
+                        ("thin_glass", mipt.MIPT_LUT_GGX_THIN_GLASS)):
+        s = mipt.default_bake_settings(table)
+        arr, info = ctx.bake_lut(s)
+        shipped = mipt.shipped_lut(table)
+        # the reference writes its tables through stb_image_write's RGBE
+        # truncation (8-bit mantissa)
+        m, e = np.frexp(arr)
+        q = np.ldexp(np.floor((arr * (m.astype(np.float32) * np.float32(256.0) / arr)).astype(np.float32)),
+                     e - 8).astype(np.float32)
+        total_ms += info.ms
+        total_samples += info.samples
+        tables.append({"table": name, "size": [s.size_cos_theta, s.size_roughness, s.size_ior],
+                       "samples_per_texel": info.launches * info.iterations_per_launch, "ms": round(info.ms, 2),
+                       "gsamples_per_s": round(info.samples / info.ms * 1e-6, 2),
+                       "equal_to_shipped": round(float((q == shipped).mean()), 5)})
+    ctx.close()
+    out = {"workload": "5 energy-compensation tables (1.82M texels) at the reference's sample counts",
+           "tables": tables, "total_ms": round(total_ms, 1),
+           "gsamples_per_s": round(total_samples / total_ms * 1e-6, 2)}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        cores = os.cpu_count() or 1
+        cpu_samples = 0.0
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(5)
+        for table in range(5):
+            s = mipt.default_bake_settings(table)
+            n = s.size_cos_theta * s.size_roughness * s.size_ior
+            tex = rng.integers(0, n, 32)
+            ipk, launches, _ = oracle.bake_plan(table, s.size_cos_theta, s.size_roughness, s.size_ior,
+                                                s.integration_sample_count)
+            oracle.bake_texels(table, s.size_cos_theta, s.size_roughness, s.size_ior, s.integration_sample_count,
+                               tex, nthreads=cores)
+            cpu_samples += 32.0 * ipk * launches
+        cpu_s = time.perf_counter() - t0
+        out["cpu_oracle"] = {"gsamples_per_s": round(cpu_samples / cpu_s * 1e-9, 4), "cores": cores,
+                             "sample": "32 texels of each table at the reference's sample counts",
+                             "s": round(cpu_s, 2)}
+    return out
 
 
 def views_render_leg(mipt, torch, np, count):

@@ -139,6 +139,32 @@ void upload_materials(Ctx& ctx)
     }
 }
 
+// The renderer's table `t` (MIPT_LUT_GGX_CONDUCTOR .. MIPT_LUT_GGX_THIN_GLASS):
+// its device buffer, element count and the shipped data.
+struct LutSlot {
+    DeviceBuffer* buf;
+    size_t count;
+    const float* shipped;
+};
+LutSlot lut_slot(RenderState& R, int32_t t)
+{
+    switch (t)
+    {
+    case MIPT_LUT_GGX_CONDUCTOR: return {&R.lut_conductor, 128 * 128, conductor_lut_data()};
+    case MIPT_LUT_GLOSSY_DIELECTRIC: return {&R.lut_glossy, 128 * 64 * 128, glossy_dielectric_lut_data()};
+    case MIPT_LUT_GGX_GLASS: return {&R.lut_glass, 256 * 16 * 128, glass_lut_data()};
+    case MIPT_LUT_GGX_GLASS_INVERSE: return {&R.lut_glass_inv, 256 * 16 * 128, glass_inv_lut_data()};
+    default: return {&R.lut_thin_glass, 32 * 32 * 96, thin_glass_lut_data()};
+    }
+}
+
+void upload_lut(RenderState& R, int32_t t)
+{
+    const LutSlot sl = lut_slot(R, t);
+    const std::vector<float>& o = R.lut_override[t];
+    upload(*sl.buf, o.empty() ? sl.shipped : o.data(), sl.count);
+}
+
 // Scene data the passes need beyond the traversal: unpacked materials,
 // triangle areas (ThreadFunctions.cpp:221-242: |cross(AB, AC)| * 0.5) and the
 // energy-compensation LUT.
@@ -166,11 +192,8 @@ void prepare_scene_for_render(Ctx& ctx)
         areas[t] = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) * 0.5f;
     }
     upload(R.areas, areas.data(), areas.size());
-    upload(R.lut_conductor, conductor_lut_data(), 128 * 128);
-    upload(R.lut_glossy, glossy_dielectric_lut_data(), 128 * 64 * 128);
-    upload(R.lut_glass, glass_lut_data(), 256 * 16 * 128);
-    upload(R.lut_glass_inv, glass_inv_lut_data(), 256 * 16 * 128);
-    upload(R.lut_thin_glass, thin_glass_lut_data(), 32 * 32 * 96);
+    for (int32_t t = MIPT_LUT_GGX_CONDUCTOR; t <= MIPT_LUT_GGX_THIN_GLASS; t++)
+        upload_lut(R, t);
     upload(R.lut_glass_exponent, glass_exponent_data(), 2 * 10 * 11 * 4);
     upload(R.lut_sheen, sheen_ltc_data(), 32 * 32 * 3);
     build_power_alias_table(ctx);
@@ -1187,6 +1210,78 @@ int mipt_get_envmap_tables(const mipt_ctx* c, float* probas, int32_t* alias, flo
         if (packed_texels)
             HIP_CHECK(hipMemcpy(packed_texels, ctx->env_texels.ptr, 4 * count, hipMemcpyDeviceToHost));
     });
+}
+
+int mipt_bake_settings_default(int32_t table, mipt_bake_settings* out)
+{
+    if (!out)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::bake_default_settings(table, out);
+}
+
+int mipt_bake_lut(mipt_ctx* c, const mipt_bake_settings* settings, float* out_device, mipt_bake_info* info)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx || !settings)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] { mipt::bake_lut(*ctx, *settings, out_device, info); });
+}
+
+int mipt_set_lut(mipt_ctx* c, int32_t table, const float* values, int32_t size_cos_theta, int32_t size_roughness,
+                 int32_t size_ior)
+{
+    Ctx* ctx = reinterpret_cast<Ctx*>(c);
+    if (!ctx)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    return mipt::guarded(ctx, [&] {
+        if (table < MIPT_LUT_GGX_CONDUCTOR || table > MIPT_LUT_GGX_THIN_GLASS)
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_lut: the renderer reads tables 0 .. 4 only");
+        mipt_bake_settings d;
+        mipt::bake_default_settings(table, &d);
+        if (values && (size_cos_theta != d.size_cos_theta || size_roughness != d.size_roughness ||
+                       size_ior != d.size_ior))
+            mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_set_lut: sizes differ from the renderer's table (" +
+                                                      std::to_string(d.size_cos_theta) + " x " +
+                                                      std::to_string(d.size_roughness) + " x " +
+                                                      std::to_string(d.size_ior) + ")");
+        mipt::RenderState& R = ctx->render;
+        const size_t count = (size_t)d.size_cos_theta * (size_t)d.size_roughness * (size_t)d.size_ior;
+        HIP_CHECK(hipDeviceSynchronize()); // queued passes may read the old table
+        if (values)
+            R.lut_override[table].assign(values, values + count);
+        else
+            R.lut_override[table].clear();
+        if (R.scene_ready)
+            mipt::upload_lut(R, table);
+        R.graph_dirty = true;
+    });
+}
+
+int mipt_get_shipped_lut(int32_t table, int32_t masking_shadowing, float* out, int32_t count)
+{
+    mipt_bake_settings d;
+    if (!out || mipt::bake_default_settings(table, &d) != MIPT_OK || table > MIPT_LUT_GGX_THIN_GLASS)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    if (masking_shadowing != MIPT_GGX_HEIGHT_UNCORRELATED &&
+        !(masking_shadowing == MIPT_GGX_HEIGHT_CORRELATED && table == MIPT_LUT_GGX_CONDUCTOR))
+        return MIPT_ERR_INVALID_ARGUMENT;
+    const size_t n = (size_t)d.size_cos_theta * (size_t)d.size_roughness * (size_t)d.size_ior;
+    if (count < 0 || (size_t)count != n)
+        return MIPT_ERR_INVALID_ARGUMENT;
+    const float* src = nullptr;
+    switch (table)
+    {
+    case MIPT_LUT_GGX_CONDUCTOR:
+        src = masking_shadowing == MIPT_GGX_HEIGHT_CORRELATED ? mipt::conductor_correlated_lut_data()
+                                                               : mipt::conductor_lut_data();
+        break;
+    case MIPT_LUT_GLOSSY_DIELECTRIC: src = mipt::glossy_dielectric_lut_data(); break;
+    case MIPT_LUT_GGX_GLASS: src = mipt::glass_lut_data(); break;
+    case MIPT_LUT_GGX_GLASS_INVERSE: src = mipt::glass_inv_lut_data(); break;
+    default: src = mipt::thin_glass_lut_data(); break;
+    }
+    std::memcpy(out, src, 4 * n);
+    return MIPT_OK;
 }
 
 int mipt_compute_alias_table(const float* weights, int32_t n, float total_sum, float* out_probas, int32_t* out_alias)

@@ -124,6 +124,8 @@ struct RenderState {
     DeviceBuffer areas;         // float[T]
     DeviceBuffer lut_conductor; // float[128*128]
     DeviceBuffer lut_glossy, lut_glass, lut_glass_inv, lut_thin_glass, lut_glass_exponent, lut_sheen;
+    // tables set with mipt_set_lut (MIPT_LUT_GGX_CONDUCTOR .. MIPT_LUT_GGX_THIN_GLASS), empty: the shipped one
+    std::vector<float> lut_override[5];
     bool scene_ready = false;
     bool scene_volumes = false;    // some material transmits light
     bool scene_dispersive = false; // some material is dispersive glass
@@ -229,6 +231,7 @@ struct Ctx {
 // material uses a lobe this build does not implement.
 std::string unpack_material(const mipt_material& in, float out[64]);
 const float* conductor_lut_data(); // 128 x 128, luts.cpp
+const float* conductor_correlated_lut_data(); // 128 x 128, height-correlated Smith
 const float* glossy_dielectric_lut_data(); // [128][64][128]
 const float* glass_lut_data();             // [128][16][256]
 const float* glass_inv_lut_data();         // [128][16][256]
@@ -247,6 +250,10 @@ void write_frame_params(FrameParams* frame, const FrameParams& values, hipStream
 
 // envmap.hip: the ambient light's envmap and its device view.
 void set_envmap(Ctx& ctx, const float* pixels, int32_t width, int32_t height, int32_t channels);
+// baker.hip
+int32_t bake_default_settings(int32_t table, mipt_bake_settings* s);
+void bake_plan(const mipt_bake_settings& s, int32_t& ipk, int32_t& launches, int32_t& normalizer);
+void bake_lut(Ctx& ctx, const mipt_bake_settings& s, float* out, mipt_bake_info* info);
 bool env_sampling_on(const Ctx& ctx); // ENVMAP ambient light with intensity > 0
 void allocate_env_path_buffers(Ctx& ctx);
 

@@ -192,6 +192,39 @@ class WorldSettings(C.Structure):
     ]
 
 
+# Energy-compensation LUT baker (GPUBaker.cpp, src/Device/kernels/Baking/)
+MIPT_LUT_GGX_CONDUCTOR = 0
+MIPT_LUT_GLOSSY_DIELECTRIC = 1
+MIPT_LUT_GGX_GLASS = 2
+MIPT_LUT_GGX_GLASS_INVERSE = 3
+MIPT_LUT_GGX_THIN_GLASS = 4
+MIPT_LUT_GGX_FRESNEL = 5
+MIPT_GGX_HEIGHT_CORRELATED = 0
+MIPT_GGX_HEIGHT_UNCORRELATED = 1
+
+
+class BakeSettings(C.Structure):
+    """mipt_bake_settings (the *DirectionalAlbedoSettings structs)."""
+
+    _fields_ = [
+        ("table", C.c_int32), ("size_cos_theta", C.c_int32), ("size_roughness", C.c_int32),
+        ("size_ior", C.c_int32), ("integration_sample_count", C.c_int32), ("masking_shadowing", C.c_int32),
+        ("lanes_per_texel", C.c_int32),
+    ]
+
+    @property
+    def shape(self):
+        """(size_ior, size_roughness, size_cos_theta): the table's array shape."""
+        return (self.size_ior, self.size_roughness, self.size_cos_theta)
+
+
+class BakeInfo(C.Structure):
+    _fields_ = [
+        ("iterations_per_launch", C.c_int32), ("launches", C.c_int32), ("normalizer", C.c_int32),
+        ("lanes_per_texel", C.c_int32), ("kernel_launches", C.c_int32), ("ms", C.c_float), ("samples", C.c_double),
+    ]
+
+
 # ReSTIR DI reservoir record as read back by mipt_read_restir_reservoirs.
 RESERVOIR_DTYPE = np.dtype([("point", np.float32, 3), ("target", np.float32), ("M", np.int32),
                             ("tri", np.int32), ("ucw", np.float32), ("flags", np.int32)])
@@ -256,6 +289,10 @@ _SIGNATURES = {
     "mipt_set_envmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "mipt_get_envmap_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
     "mipt_envmap_generate": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p]),
+    "mipt_bake_settings_default": (C.c_int, [C.c_int32, C.POINTER(BakeSettings)]),
+    "mipt_bake_lut": (C.c_int, [C.c_void_p, C.POINTER(BakeSettings), C.c_void_p, C.POINTER(BakeInfo)]),
+    "mipt_set_lut": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "mipt_get_shipped_lut": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_int32]),
     "mipt_compute_alias_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p]),
     "mipt_read_restir_reservoirs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mipt_radiance_cache_settings_default": (None, [C.POINTER(RadianceCacheSettings)]),
@@ -335,6 +372,28 @@ def default_world_settings(**overrides) -> WorldSettings:
         else:
             setattr(w, k, v)
     return w
+
+
+def default_bake_settings(table: int, **overrides) -> BakeSettings:
+    """The reference's bake settings of a table at the renderer's size."""
+    b = BakeSettings()
+    rc = load_library().mipt_bake_settings_default(table, C.byref(b))
+    if rc != MIPT_OK:
+        raise MiptError(rc, "mipt_bake_settings_default")
+    for k, v in overrides.items():
+        setattr(b, k, v)
+    return b
+
+
+def shipped_lut(table: int, masking_shadowing: int = MIPT_GGX_HEIGHT_UNCORRELATED) -> np.ndarray:
+    """A table the library ships (the reference's baked files), [z][roughness
+    flipped][cos_theta]."""
+    b = default_bake_settings(table)
+    out = np.zeros(b.shape, np.float32)
+    rc = load_library().mipt_get_shipped_lut(table, masking_shadowing, out.ctypes.data_as(C.c_void_p), out.size)
+    if rc != MIPT_OK:
+        raise MiptError(rc, "mipt_get_shipped_lut")
+    return out
 
 
 def generate_envmap(width: int, height: int) -> np.ndarray:
@@ -694,6 +753,25 @@ class Context:
         img = np.ascontiguousarray(image, np.float32)
         h, w, c = img.shape
         self._check(self.lib.mipt_set_envmap(self.h, img.ctypes.data_as(C.c_void_p), w, h, c), "mipt_set_envmap")
+
+    def bake_lut(self, settings: BakeSettings):
+        """(table [z][roughness flipped][cos_theta] float32, BakeInfo) of a
+        bake on the GPU (mipt_bake_lut)."""
+        n = settings.size_cos_theta * settings.size_roughness * settings.size_ior
+        dev = empty_device(4 * n)
+        info = BakeInfo()
+        self._check(self.lib.mipt_bake_lut(self.h, C.byref(settings), ptr(dev), C.byref(info)), "mipt_bake_lut")
+        return from_device(dev, np.float32)[:n].reshape(settings.shape).copy(), info
+
+    def set_lut(self, table: int, values=None):
+        """Installs a table for the passes from then on (None: the shipped one)."""
+        if values is None:
+            self._check(self.lib.mipt_set_lut(self.h, table, None, 0, 0, 0), "mipt_set_lut")
+            return
+        v = np.ascontiguousarray(values, np.float32)
+        shape = v.shape if v.ndim == 3 else (1,) + v.shape
+        self._check(self.lib.mipt_set_lut(self.h, table, v.ctypes.data_as(C.c_void_p), shape[2], shape[1], shape[0]),
+                    "mipt_set_lut")
 
     def envmap_tables(self, count: int):
         """(alias probabilities, aliases, luminance sum, packed R9G9B9E5 texels)."""

@@ -552,6 +552,79 @@ int mipt_get_envmap_tables(const mipt_ctx* ctx, float* probas, int32_t* alias, f
 int mipt_envmap_generate(int32_t width, int32_t height, float* out_rgb);
 
 /* ------------------------------------------------------------------ */
+/* Energy-compensation LUT baker: GPUBaker (src/Renderer/Baker/        */
+/* GPUBaker.cpp:37-93) and its kernels (src/Device/kernels/Baking/).   */
+/* A table holds the Monte Carlo directional albedo E(cos_theta_o,     */
+/* roughness[, F0]) of a single-scattering GGX BSDF, isotropic:        */
+/*   cos_theta_o = max(1e-3, x / (size_cos_theta - 1)) (^2.5 for the   */
+/*     glossy dielectric, glass and Fresnel tables);                   */
+/*   roughness = max(1e-4, y / (size_roughness - 1));                  */
+/*   F0 = (z / (size_ior - 1))^4, relative IOR (1 + sqrt F0) /          */
+/*     (1 - sqrt F0) with F0 clamped to [0, 0.99].                      */
+/* Samples per texel: launches x iterations_per_launch as the          */
+/* reference's host loop runs them (GPUBakerKernel.cpp:77-92, 1e8      */
+/* texel-samples a launch), launch i seeding xorshift32 with           */
+/* wang_hash(texel + 1) * i, so a bake draws the reference's samples.  */
+/* ------------------------------------------------------------------ */
+#define MIPT_LUT_GGX_CONDUCTOR 0       /* GGXConductorDirectionalAlbedo.h, 128 x 128 */
+#define MIPT_LUT_GLOSSY_DIELECTRIC 1   /* GlossyDielectricDirectionalAlbedo.h, 128 x 64 x 128 */
+#define MIPT_LUT_GGX_GLASS 2           /* GGXGlassDirectionalAlbedo.h entering, 256 x 16 x 128 */
+#define MIPT_LUT_GGX_GLASS_INVERSE 3   /* the same, exiting (relative IOR inverted) */
+#define MIPT_LUT_GGX_THIN_GLASS 4      /* GGXThinGlassDirectionalAlbedo.h, 32 x 32 x 96 */
+#define MIPT_LUT_GGX_FRESNEL 5         /* GGXFresnelDirectionalAlbedo.h, 256 x 256 x 256 (not read by the renderer) */
+
+#define MIPT_GGX_HEIGHT_CORRELATED 0   /* GGXMaskingShadowingFlavor (BSDFsData.h:20-24) */
+#define MIPT_GGX_HEIGHT_UNCORRELATED 1 /* the renderer's flavour */
+
+/* The *DirectionalAlbedoSettings structs (src/Renderer/Baker/). */
+typedef struct mipt_bake_settings {
+    int32_t table;                    /* MIPT_LUT_* */
+    int32_t size_cos_theta;           /* texture_size_cos_theta(_o) */
+    int32_t size_roughness;           /* texture_size_roughness */
+    int32_t size_ior;                 /* texture_size_ior; 1 for the conductor table */
+    int32_t integration_sample_count; /* 262144 conductor, 131072 glossy, 65536 others */
+    int32_t masking_shadowing;        /* default MIPT_GGX_HEIGHT_UNCORRELATED */
+    /* partial sums per texel, 1, 2, 4, ..., 64: launch i goes to partial
+     * (i - 1) % lanes, the partials add pairwise ((p0 + p1) + (p2 + p3)) + ...
+     * 1 (the default) is the reference's summation order, whose rounding
+     * drift (~0.1 %) is in the shipped tables; more lanes run small tables
+     * faster and round less; 0 picks lanes for occupancy. */
+    int32_t lanes_per_texel;
+} mipt_bake_settings;
+
+typedef struct mipt_bake_info {
+    int32_t iterations_per_launch; /* samples per texel in one reference launch */
+    int32_t launches;              /* reference launches (seeds 1 .. launches) */
+    int32_t normalizer;            /* nb_samples dividing each sample (the Fresnel kernel's own count) */
+    int32_t lanes_per_texel;       /* partial sums used */
+    int32_t kernel_launches;       /* GPU launches the bake took */
+    float ms;                      /* GPU time of the bake (hipEvents) */
+    double samples;                /* samples drawn, texels x launches x iterations_per_launch */
+} mipt_bake_info;
+
+/* Defaults of a table: the reference's settings and the renderer's size. */
+int mipt_bake_settings_default(int32_t table, mipt_bake_settings* out);
+/* Bakes a table into `out_device` (device memory, size_cos_theta *
+ * size_roughness * size_ior floats) in the renderer's layout
+ * [z][size_roughness - 1 - y][x]: the reference writes rows in y order and
+ * reads them back flipped (GPUBakerKernel.cpp:102-121, Image32Bit::
+ * read_image_hdr with flipY). A sample whose value is NaN is skipped like a
+ * zero-pdf one. Synchronous; info may be NULL. */
+int mipt_bake_lut(mipt_ctx* ctx, const mipt_bake_settings* settings, float* out_device, mipt_bake_info* info);
+/* Replaces one of the renderer's tables (MIPT_LUT_GGX_CONDUCTOR ..
+ * MIPT_LUT_GGX_THIN_GLASS, the conductor's flavour the uncorrelated one)
+ * from host memory in the same layout, for the passes from then on
+ * (CPURenderer::setup_brdfs_data, CPURenderer.cpp:155-196, loads the baked
+ * files); the sizes must be the renderer's. values NULL restores the table
+ * the library ships. */
+int mipt_set_lut(mipt_ctx* ctx, int32_t table, const float* values, int32_t size_cos_theta, int32_t size_roughness,
+                 int32_t size_ior);
+/* Host copy of a table the library ships (the reference's baked .hdr files,
+ * RGBE-decoded), in the renderer's layout: MIPT_LUT_GGX_CONDUCTOR with
+ * masking_shadowing either flavour, the other tables uncorrelated. */
+int mipt_get_shipped_lut(int32_t table, int32_t masking_shadowing, float* out, int32_t count);
+
+/* ------------------------------------------------------------------ */
 /* Learned radiance cache. No counterpart in the reference: a neural   */
 /* radiance cache trained online from the tracer's own paths. The      */
 /* position goes through a multiresolution hash grid (FP16 features,   */

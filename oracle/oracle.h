@@ -364,6 +364,15 @@ uint32_t orc_oct_pack(const float* n);
 /* fmath.h functions element-wise: fn 0 sin, 1 cos, 2 pow(x, y), 3 log2, 4 exp2. */
 void orc_fmath_eval(int32_t fn, const float* x, const float* y, float* out, int32_t n);
 void orc_oct_unpack(uint32_t p, float* out);
+/* directional-albedo LUT baker: table 0 conductor, 1 glossy dielectric,
+ * 2 glass, 3 glass with the inverse IOR, 4 thin glass, 5 Fresnel-weighted
+ * conductor; texels are x + y * nx + z * nx * ny (y = roughness index) */
+void orc_bake_plan(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t count, int32_t* ipk, int32_t* launches,
+                   int32_t* normaliser);
+float orc_bake_texel(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t count, int32_t correlated,
+                     int32_t lanes, int32_t texel);
+void orc_bake_texels(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t count, int32_t correlated,
+                     int32_t lanes, const int32_t* texels, int32_t n, float* out, int32_t nthreads);
 
 #ifdef __cplusplus
 }

@@ -143,6 +143,11 @@ _SIG = {
     "orc_dispersion_ior": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
     "orc_thin_film_fresnel": (None, [_P, C.c_float, C.c_float, _P]),
     "orc_hue_shift": (None, [_P, C.c_float, _P]),
+    "orc_bake_plan": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P]),
+    "orc_bake_texel": (C.c_float, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_int32]),
+    "orc_bake_texels": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P,
+                               C.c_int32, _P, C.c_int32]),
 }
 
 _lib = None
@@ -307,12 +312,17 @@ _LUTS = {}
 
 
 def load_luts(conductor=None):
-    """{name: float32 array} of the BSDF tables (conductor replaceable)."""
+    """{name: float32 array} of the BSDF tables. `conductor` replaces the
+    conductor table (an array), or any tables (a dict name -> array)."""
     if not _LUTS:
         for k, (f, shape) in LUT_FILES.items():
             _LUTS[k] = np.fromfile(os.path.join(DATA_DIR, f), "<f4").reshape(shape)
     out = dict(_LUTS)
-    if conductor is not None:
+    if isinstance(conductor, dict):
+        for k, v in conductor.items():
+            assert k in LUT_FILES and np.shape(v) == _LUTS[k].shape, k
+            out[k] = np.ascontiguousarray(v, np.float32)
+    elif conductor is not None:
         out["conductor"] = np.ascontiguousarray(conductor, np.float32)
     return out
 
@@ -759,3 +769,26 @@ class Bsdf:
         lib().orc_bsdf_sample(C.cast(self.mat, C.c_void_p), C.byref(self.luts), self.tau, self.reg, self.path_r,
                               *[_p(x) for x in a], C.byref(st), _p(d), _p(f), C.byref(pdf))
         return d, f, pdf.value, int(st.value)
+
+
+# ---------------------------------------------------------------------------
+# Directional-albedo LUT baker (src/Device/kernels/Baking/, GPUBakerKernel.cpp)
+# ---------------------------------------------------------------------------
+BAKE_CONDUCTOR, BAKE_GLOSSY_DIELECTRIC, BAKE_GLASS, BAKE_GLASS_INVERSE, BAKE_THIN_GLASS, BAKE_FRESNEL = range(6)
+
+
+def bake_plan(table, nx, ny, nz, count):
+    """(iterations per launch, launches, normalising sample count) of a bake."""
+    a = np.zeros(3, np.int32)
+    lib().orc_bake_plan(table, nx, ny, nz, count, _p(a[0:1]), _p(a[1:2]), _p(a[2:3]))
+    return int(a[0]), int(a[1]), int(a[2])
+
+
+def bake_texels(table, nx, ny, nz, count, texels, correlated=False, lanes=1, nthreads=0):
+    """Baked values of the given texels (x + y * nx + z * nx * ny, y the
+    roughness index), launches split over `lanes` partial sums."""
+    idx = np.ascontiguousarray(texels, np.int32).reshape(-1)
+    out = np.zeros(idx.size, np.float32)
+    lib().orc_bake_texels(table, nx, ny, nz, count, int(bool(correlated)), lanes, _p(idx), idx.size, _p(out),
+                          nthreads)
+    return out

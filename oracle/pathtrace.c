@@ -3664,3 +3664,302 @@ static int32_t render_sample_via_frame(const orc_render_scene* sc, const orc_oct
     orc_frame_free(f);
     return rejected;
 }
+
+/* ------------------------------------------------------------------ */
+/* Directional-albedo LUT baker (src/Device/kernels/Baking/ kernels,   */
+/* src/Renderer/Baker/GPUBakerKernel.cpp:52-100, GPUBakerConstants.h)    */
+/* ------------------------------------------------------------------ */
+/* Tables: 0 GGX conductor E(cos, roughness); 1 glossy dielectric; 2 GGX
+ * glass entering; 3 GGX glass exiting (relative IOR inverted); 4 GGX thin
+ * glass; 5 GGX conductor weighted by the dielectric Fresnel of the
+ * sampled direction. One texel integrates `launches` reference launches of
+ * `ipk` samples each; launch i (1-based) seeds xorshift32 with
+ * wang_hash(texel + 1) * i. Launch i is taken by lane (i - 1) % lanes of
+ * the texel, each lane summing its launches in increasing order, and the
+ * lanes' sums combine pairwise ((l0 + l1) + (l2 + l3)) + ...; one lane is
+ * the reference's own order. */
+#define BAKE_ELEMENTS_PER_LAUNCH 100000000.0f /* GPUBakerConstants.h:38 */
+
+void orc_bake_plan(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t count, int32_t* ipk, int32_t* launches,
+                   int32_t* normaliser)
+{
+    /* GPUBakerKernel.cpp:77-78: the launch count from the table's texels */
+    float texels = (float)(nx * ny * nz);
+    int32_t k = (int32_t)floorf(hmax(1.0f, BAKE_ELEMENTS_PER_LAUNCH / texels));
+    int32_t n = (int32_t)ceilf((float)count / (float)k);
+    /* the kernels' own nb_samples; the Fresnel kernel derives its
+     * iterations per launch from the 2D size (GGXFresnelDirectionalAlbedo.h:56) */
+    int32_t kk = table == 5 ? (int32_t)floorf(hmax(1.0f, BAKE_ELEMENTS_PER_LAUNCH / (float)(nx * ny))) : k;
+    *ipk = k;
+    *launches = n;
+    *normaliser = (int32_t)ceilf((float)count / (float)kk) * kk;
+}
+
+/* torrance_sparrow_GGX_eval_reflect<0> (Microfacet.h:103-178), isotropic,
+ * the peak sampled, with the masking-shadowing flavour of the bake */
+static float bake_reflect(float r, float F, v3 lv, v3 ll, v3 lh, float* pdf, int correlated)
+{
+    *pdf = 0.0f;
+    if (smooth(r))
+    {
+        if (vdot(reflect(lv, V(0, 0, 1)), ll) < 0.999999f)
+            return 0.0f;
+        *pdf = 1.0e9f;
+        return 1.0e9f * F / fabsf(ll.z);
+    }
+    if (ll.z < 0.0f)
+        return 0.0f;
+    float ax, ay;
+    alphas(r, 0.0f, &ax, &ay);
+    float D = ggx_D(ax, ay, lh);
+    float lam_v = lambda_(ax, ay, lv);
+    float G1V = 1.0f / (1.0f + lam_v);
+    float Dv = vndf(D, G1V, lv, lh);
+    float NoV = hmax(1.0e-3f, fabsf(lv.z)), NoL = hmax(1.0e-3f, fabsf(ll.z));
+    *pdf = Dv / (4.0f * vdot(lv, lh));
+    if (*pdf == 0.0f)
+        return 0.0f;
+    float lam_l = lambda_(ax, ay, ll);
+    float G2 = correlated ? 1.0f / (1.0f + lam_v + lam_l) : G1V * (1.0f / (1.0f + lam_l));
+    return F * D * G2 / (4.0f * NoL * NoV);
+}
+
+/* GGX_anisotropic_sample_microfacet (Microfacet.h:460-476) */
+static v3 bake_microfacet(v3 lv, float r, uint32_t* st)
+{
+    float ax, ay;
+    alphas(r, 0.0f, &ax, &ay);
+    if (ax <= 1.0e-4f && ay <= 1.0e-4f)
+        return V(0, 0, 1);
+    return vndf_sample(lv, ax, ay, st);
+}
+
+/* GGX_glass_E_sample / thin_glass_sample (GGXGlassDirectionalAlbedo.h:123-160,
+ * GGXThinGlassDirectionalAlbedo.h:32-86) */
+static v3 bake_glass_sample(float eta, float r, v3 lv, int thin, uint32_t* st)
+{
+    if (fabsf(eta - 1.0f) < 1.0e-5f)
+        eta = 1.0f + 1.0e-5f;
+    v3 m = bake_microfacet(lv, r, st);
+    float F = fresnel_d(vdot(lv, m), eta);
+    if (thin && r < 0.1f)
+        F += (1.0f - F) * (1.0f - F) * F / (1.0f - F * F);
+    if (orc_rand01(st) < F)
+        return reflect(lv, m);
+    if (vdot(m, lv) < 0.0f)
+        m = vneg(m);
+    if (thin)
+    {
+        v3 d = reflect(lv, m);
+        d.z *= -1.0f;
+        return d;
+    }
+    return refract_dir(lv, m, eta);
+}
+
+/* GGX_glass_E_eval / thin_glass_eval (GGXGlassDirectionalAlbedo.h:36-118,
+ * GGXThinGlassDirectionalAlbedo.h:88-189) */
+static float bake_glass_eval(float eta, float r, v3 lv, v3 ll, float* pdf, int thin, int correlated)
+{
+    *pdf = 0.0f;
+    float NoV = lv.z, NoL = ll.z;
+    if (fabsf(NoL) < 1.0e-8f)
+        return 0.0f;
+    int reflecting = NoL * NoV > 0.0f;
+    if (fabsf(eta - 1.0f) < 1.0e-5f)
+        eta = 1.0f + 1.0e-5f;
+    v3 h;
+    if (reflecting)
+        h = vadd(ll, lv);
+    else if (thin)
+        h = vadd(V(ll.x, ll.y, -ll.z), lv);
+    else
+        h = vadd(vscale(ll, eta), lv);
+    h = vnorm(h);
+    if (h.z < 0.0f)
+        h = vneg(h);
+    float HoL = vdot(ll, h), HoV = vdot(lv, h);
+    if (HoL * NoL < 0.0f || HoV * NoV < 0.0f)
+        return 0.0f;
+    float F = fresnel_d(HoV, eta);
+    if (thin && r < 0.1f)
+        F += (1.0f - F) * (1.0f - F) * F / (1.0f - F * F);
+    if (reflecting)
+    {
+        float v = bake_reflect(r, F, lv, ll, h, pdf, correlated);
+        *pdf *= F;
+        return v;
+    }
+    float dp = HoL + HoV / eta;
+    float dp2 = dp * dp;
+    float denom = dp2 * NoL * NoV;
+    float ax, ay;
+    alphas(r, 0.0f, &ax, &ay);
+    float D = ggx_D(ax, ay, h);
+    float G1V = 1.0f / (1.0f + lambda_(ax, ay, lv));
+    float G1L = 1.0f / (1.0f + lambda_(ax, ay, ll));
+    float G2 = G1V * G1L;
+    float dwm_dwi = fabsf(HoL) / dp2;
+    float D_pdf = G1V / fabsf(NoV) * D * fabsf(HoV);
+    *pdf = dwm_dwi * D_pdf;
+    *pdf *= 1.0f - F;
+    return D * (1.0f - F) * G2 * fabsf(HoL * HoV / denom);
+}
+
+/* one sample of a table's integrand; 0 with *skip set for the samples the
+ * reference `continue`s past */
+static float bake_sample(int32_t table, float r, float eta, v3 lv, int correlated, uint32_t* st, int* skip)
+{
+    *skip = 0;
+    float pdf;
+    if (table == 0 || table == 5)
+    {
+        /* GGXConductorDirectionalAlbedo.h:60-78, GGXFresnelDirectionalAlbedo.h:60-77 */
+        v3 ll = ggx_sample(r, 0.0f, lv, st);
+        if (ll.z < 0.0f)
+        {
+            *skip = 1;
+            return 0.0f;
+        }
+        float F = table == 5 ? fresnel_d(ll.z, eta) : 1.0f;
+        float v = bake_reflect(r, F, lv, ll, vnorm(vadd(lv, ll)), &pdf, correlated);
+        v /= pdf;
+        v *= ll.z;
+        return v;
+    }
+    if (table == 1)
+    {
+        /* GlossyDielectricDirectionalAlbedo.h:64-119 */
+        float lobe = orc_rand01(st);
+        v3 ll;
+        if (lobe < 0.5f)
+        {
+            ll = ggx_sample(r, 0.0f, lv, st);
+            if (ll.z < 0.0f)
+            {
+                *skip = 1;
+                return 0.0f;
+            }
+        }
+        else
+        {
+            /* cosine_weighted_sample_z_up_frame (Sampling.h:210-220) */
+            float r1 = orc_rand01(st), r2 = orc_rand01(st);
+            float phi = M_TWO_PI_F * r1, ct = sqrtf(r2), stt = sqrtf(1.0f - ct * ct);
+            float s, c;
+            orc_sincos(phi, &s, &c);
+            ll = vnorm(V(c * stt, s * stt, ct));
+        }
+        v3 h = vnorm(vadd(lv, ll));
+        float F = fresnel_d(vdot(h, ll), eta);
+        float pdf_s;
+        float spec = bake_reflect(r, F, lv, ll, h, &pdf_s, correlated);
+        float total = pdf_s * 0.5f;
+        float thr = 1.0f;
+        thr *= 1.0f - fresnel_d(ll.z, eta);
+        thr *= 1.0f - fresnel_d(lv.z, eta);
+        /* lambertian_brdf_eval (Lambertian.h:14-23) of a white base colour */
+        float pdf_d = 0.0f, diff = 0.0f;
+        if (ll.z > 0.0f)
+        {
+            pdf_d = ll.z * M_INV_PI_F;
+            diff = M_INV_PI_F;
+        }
+        total += pdf_d * 0.5f;
+        diff *= thr;
+        float v = spec + diff;
+        v *= ll.z;
+        v /= total;
+        return v;
+    }
+    /* glass, its inverse and thin glass: GGXGlassDirectionalAlbedo.h:191-203,
+     * GGXThinGlassDirectionalAlbedo.h:231-244 */
+    int thin = table == 4;
+    float rr = thin ? thin_rough(1, r, eta) : r;
+    v3 ll = bake_glass_sample(eta, rr, lv, thin, st);
+    float v = bake_glass_eval(eta, rr, lv, ll, &pdf, thin, correlated);
+    if (pdf == 0.0f)
+    {
+        *skip = 1;
+        return 0.0f;
+    }
+    v /= pdf;
+    v *= fabsf(ll.z);
+    return v;
+}
+
+/* the texel parameters (GGXConductorDirectionalAlbedo.h:44-52,
+ * GlossyDielectricDirectionalAlbedo.h:46-63): cos_theta_o, roughness and
+ * the relative IOR from F0 = (z / (nz - 1))^4 */
+static void bake_texel_params(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t x, int32_t y, int32_t z,
+                              float* r_out, float* eta_out, v3* lv_out)
+{
+    float cos_o = 1.0f / ((float)nx - 1.0f) * (float)x;
+    cos_o = hmax(1.0e-3f, cos_o);
+    if (table != 0 && table != 4)
+        cos_o = cos_o * cos_o * sqrtf(cos_o); /* powf(cos_o, 2.5f) */
+    float sin_o = orc_sinf(orc_acosf(cos_o));
+    float r = 1.0f / ((float)ny - 1.0f) * (float)y;
+    r = hmax(r, 1.0e-4f);
+    float eta = 1.0f;
+    if (table != 0)
+    {
+        float F0 = 1.0f / ((float)nz - 1.0f) * (float)z;
+        F0 *= F0;
+        F0 *= F0;
+        float sq = sqrtf(hclamp(0.0f, 0.99f, F0));
+        eta = (1.0f + sq) / (1.0f - sq);
+        if (table == 3)
+            eta = 1.0f / eta;
+    }
+    *r_out = r;
+    *eta_out = eta;
+    *lv_out = vnorm(V(sin_o, 0.0f, cos_o));
+}
+
+float orc_bake_texel(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t count, int32_t correlated,
+                     int32_t lanes, int32_t texel)
+{
+    int32_t ipk, launches, norm;
+    orc_bake_plan(table, nx, ny, nz, count, &ipk, &launches, &norm);
+    int32_t x = texel % nx, y = (texel / nx) % ny, z = texel / (nx * ny);
+    float r, eta;
+    v3 lv;
+    bake_texel_params(table, nx, ny, nz, x, y, z, &r, &eta, &lv);
+    float part[64];
+    for (int32_t j = 0; j < lanes; j++)
+    {
+        float acc = 0.0f;
+        for (int32_t i = j + 1; i <= launches; i += lanes)
+        {
+            uint32_t st = orc_wang_hash((uint32_t)texel + 1u) * (uint32_t)i;
+            for (int32_t s = 0; s < ipk; s++)
+            {
+                int skip;
+                float v = bake_sample(table, r, eta, lv, correlated, &st, &skip);
+                /* a NaN sample (thin-glass transmission whose generalized
+                 * half-vector term HoL + HoV / eta cancels to exactly 0 in
+                 * binary32: inf / inf) is skipped like a zero-pdf one */
+                if (!skip && v == v)
+                    acc += v / (float)norm;
+            }
+        }
+        part[j] = acc;
+    }
+    for (int32_t off = 1; off < lanes; off <<= 1)
+        for (int32_t j = 0; j < lanes; j += 2 * off)
+            part[j] = part[j] + part[j + off];
+    return part[0];
+}
+
+void orc_bake_texels(int32_t table, int32_t nx, int32_t ny, int32_t nz, int32_t count, int32_t correlated,
+                     int32_t lanes, const int32_t* texels, int32_t n, float* out, int32_t nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int32_t i = 0; i < n; i++)
+        out[i] = orc_bake_texel(table, nx, ny, nz, count, correlated, lanes, texels[i]);
+}

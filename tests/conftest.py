@@ -3,6 +3,7 @@ and the `gpu` marker (tests that need an MI355X; run with -m gpu)."""
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -118,3 +119,24 @@ def sky_scene(mipt, width, height, lights=True):
 @pytest.fixture(scope="session")
 def sky_scene_fn(mipt_mod):
     return sky_scene
+
+
+def rgbe_truncate(v):
+    """The value a float takes through a Radiance .hdr file as the reference
+    writes its baked tables (stb_image_write's stbiw__linear_to_rgbe, one
+    channel: mantissa byte = (unsigned char)(v * frexp(v) * 256 / v),
+    truncated; 0 below 1e-32) and reads them back (r + g + b) * 2^(e - 136) / 3."""
+    v = np.asarray(v, np.float32)
+    out = np.zeros_like(v)
+    ok = v >= np.float32(1e-32)
+    m, e = np.frexp(v[ok])
+    norm = (m.astype(np.float32) * np.float32(256.0) / v[ok]).astype(np.float32)
+    byte = np.floor((v[ok] * norm).astype(np.float32))
+    out[ok] = np.ldexp(byte, e - 8).astype(np.float32)
+    return out
+
+
+def rgbe_steps(a, b):
+    """(a - b) in units of the 8-bit mantissa step at b (b > 0)."""
+    step = np.ldexp(1.0, np.frexp(np.maximum(np.asarray(b, np.float64), 1e-30))[1] - 8)
+    return (np.asarray(a, np.float64) - np.asarray(b, np.float64)) / step
