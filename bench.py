@@ -654,7 +654,7 @@ def lut_bake_leg(mipt, np, no_cpu):
     one lane per texel (the reference's summation order), each table baked
     into HBM and timed with hipEvents around its kernel launches; compared
     through the Radiance file's 8-bit truncation with the tables the
-    reference ships. The CPU figure is the oracle's restatement on 32 texels
+    reference ships. The CPU figure is the oracle's restatement on 256 texels
     of each table at the same sample counts, on every host core."""
     ctx = mipt.Context(0)
     tables = []
@@ -691,15 +691,15 @@ def lut_bake_leg(mipt, np, no_cpu):
         for table in range(5):
             s = mipt.default_bake_settings(table)
             n = s.size_cos_theta * s.size_roughness * s.size_ior
-            tex = rng.integers(0, n, 32)
+            tex = rng.integers(0, n, 256)
             ipk, launches, _ = oracle.bake_plan(table, s.size_cos_theta, s.size_roughness, s.size_ior,
                                                 s.integration_sample_count)
             oracle.bake_texels(table, s.size_cos_theta, s.size_roughness, s.size_ior, s.integration_sample_count,
                                tex, nthreads=cores)
-            cpu_samples += 32.0 * ipk * launches
+            cpu_samples += 256.0 * ipk * launches
         cpu_s = time.perf_counter() - t0
         out["cpu_oracle"] = {"gsamples_per_s": round(cpu_samples / cpu_s * 1e-9, 4), "cores": cores,
-                             "sample": "32 texels of each table at the reference's sample counts",
+                             "sample": "256 texels of each table at the reference's sample counts",
                              "s": round(cpu_s, 2)}
     return out
 

@@ -158,7 +158,7 @@ def test_installed_tables_render(mipt_mod, oracle_mod, baked, torch_mod):
 def test_errors(mipt_mod, torch_mod):
     ctx = mipt_mod.Context(0)
     bad = [
-        dict(table=6),
+        dict(table=-1),
         dict(size_ior=2),  # conductor is 2D
         dict(size_cos_theta=1),
         dict(integration_sample_count=0),
@@ -167,7 +167,9 @@ def test_errors(mipt_mod, torch_mod):
         dict(lanes_per_texel=128),
     ]
     for kw in bad:
-        s = mipt_mod.default_bake_settings(0, **kw)
+        s = mipt_mod.default_bake_settings(0)
+        for k, v in kw.items():
+            setattr(s, k, v)
         with pytest.raises(mipt_mod.MiptError):
             ctx.bake_lut(s)
     with pytest.raises(mipt_mod.MiptError):
