@@ -359,6 +359,12 @@ int create_fail(Ctx* ctx, const char* what, hipError_t e)
 }
 } // namespace
 
+namespace mipt {
+// Message of a failed call that has no context (mipt_last_error(NULL)).
+void set_no_ctx_error(const std::string& msg) { g_no_ctx_error = msg; }
+} // namespace mipt
+
+
 extern "C" {
 
 const char* mipt_version(void) { return "mipt 0.1 (gfx950)"; }

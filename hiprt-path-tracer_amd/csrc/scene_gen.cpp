@@ -494,6 +494,15 @@ template <class T> T* dup(const std::vector<T>& v)
 }
 
 } // namespace
+
+// The look-at camera of the scene loaders (glTF ingest).
+mipt_camera look_at_camera(const double eye[3], const double target[3], const double up[3], double vfov, int32_t w,
+                           int32_t h)
+{
+    return make_camera({eye[0], eye[1], eye[2]}, {target[0], target[1], target[2]}, {up[0], up[1], up[2]}, vfov, w,
+                       h);
+}
+
 } // namespace mipt
 
 extern "C" int mipt_scene_generate(int32_t config_id, int32_t width, int32_t height, mipt_generated_scene** out)
@@ -622,6 +631,8 @@ extern "C" void mipt_scene_free(mipt_generated_scene* g)
         return;
     delete[] g->desc.triangles_vertex_indices;
     delete[] g->desc.vertices_positions;
+    delete[] g->desc.has_vertex_normals;
+    delete[] g->desc.vertex_normals;
     delete[] g->desc.material_indices;
     delete[] g->desc.materials;
     delete[] g->desc.emissive_triangles_primitive_indices;

@@ -268,6 +268,7 @@ _SIGNATURES = {
                                    C.c_void_p]),
     "mipt_scene_generate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.POINTER(GeneratedScene))]),
     "mipt_scene_free": (None, [C.POINTER(GeneratedScene)]),
+    "mipt_scene_load_gltf": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.POINTER(GeneratedScene))]),
     "mipt_host_random_number": (C.c_uint32, [C.c_uint32]),
     "mipt_render_settings_default": (None, [C.POINTER(RenderSettings)]),
     "mipt_set_render_settings": (C.c_int, [C.c_void_p, C.POINTER(RenderSettings)]),
@@ -500,12 +501,28 @@ def generate_views(count: int):
 
 
 def generate_scene(config_id: int, width: int, height: int) -> Scene:
-    """Procedural scene of configuration 1 or 2, generated by libmipt."""
+    """Procedural scene of configuration 1, 2 or 5, generated by libmipt."""
     lib = load_library()
     out = C.POINTER(GeneratedScene)()
     rc = lib.mipt_scene_generate(config_id, width, height, C.byref(out))
     if rc != MIPT_OK:
         raise MiptError(rc, f"mipt_scene_generate({config_id})")
+    return _take_scene(lib, out)
+
+
+def load_gltf(path: str, width: int, height: int) -> Scene:
+    """A glTF 2.0 scene (.gltf / .glb) read by libmipt (mipt_scene_load_gltf),
+    with its vertex normals and camera."""
+    lib = load_library()
+    out = C.POINTER(GeneratedScene)()
+    rc = lib.mipt_scene_load_gltf(os.fsencode(path), width, height, C.byref(out))
+    if rc != MIPT_OK:
+        raise MiptError(rc, "mipt_scene_load_gltf: " + lib.mipt_last_error(None).decode())
+    return _take_scene(lib, out)
+
+
+def _take_scene(lib, out) -> Scene:
+    """Copies a mipt_generated_scene into a Scene and frees it."""
     try:
         g = out.contents
         d = g.desc
@@ -516,9 +533,14 @@ def generate_scene(config_id: int, width: int, height: int) -> Scene:
         mats = (Material * M)()
         C.memmove(mats, d.materials, C.sizeof(Material) * M)
         emissive = np.ctypeslib.as_array(d.emissive_triangles_primitive_indices, shape=(E,)).copy() if E else None
+        normals = has_normals = None
+        if d.vertex_normals and d.has_vertex_normals:
+            normals = np.ctypeslib.as_array(d.vertex_normals, shape=(3 * V,)).copy()
+            has_normals = np.ctypeslib.as_array(d.has_vertex_normals, shape=(V,)).copy()
         cam = Camera()
         C.memmove(C.byref(cam), C.byref(g.camera), C.sizeof(Camera))
-        return Scene(verts, idx, mat_idx, mats, emissive, cam, g.width, g.height)
+        return Scene(verts, idx, mat_idx, mats, emissive, cam, g.width, g.height, vertex_normals=normals,
+                     has_vertex_normals=has_normals)
     finally:
         lib.mipt_scene_free(out)
 

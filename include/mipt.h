@@ -744,6 +744,25 @@ typedef struct mipt_generated_scene {
 int mipt_scene_generate(int32_t config_id, int32_t width, int32_t height, mipt_generated_scene** out);
 void mipt_scene_free(mipt_generated_scene* scene);
 
+/* glTF 2.0 ingest: SceneParser::parse_scene_file (src/Scene/SceneParser.cpp:
+ * 19-226) for a .gltf (external or base64 data: buffers) or .glb file, which
+ * the reference reads through ASSIMP with aiProcess_PreTransformVertices |
+ * aiProcess_Triangulate. Vertices and normals are pre-transformed to world
+ * space (double precision, rounded once); triangles are grouped material by
+ * material, within a material in depth-first node order; materials follow
+ * SceneParser::read_material_properties (:392-436) over the importer's keys
+ * (base colour, emission x KHR_materials_emissive_strength, metallic,
+ * roughness, KHR_materials_anisotropy / sheen / specular / clearcoat / ior /
+ * transmission / volume), plus the default glTF material, last, for
+ * primitives without one; every triangle of a material with a non-zero
+ * emission colour is an emitter. The camera is the first one, at its node's
+ * origin looking down the node's -Z with +Y up, with its yfov at
+ * width / height (without a camera: the origin looking down -Z, 40 degrees
+ * horizontal). Textures, texture coordinates, skins, morph targets,
+ * animations, point and line primitives, sparse and quantised accessors are
+ * not read. Freed with mipt_scene_free; errors in mipt_last_error(NULL). */
+int mipt_scene_load_gltf(const char* path, int32_t width, int32_t height, mipt_generated_scene** out);
+
 /* Host xorshift32 seeded with 42, one draw per pass: the reference's
  * per-pass random_number (src/Renderer/GPURenderer.cpp:750-751). */
 uint32_t mipt_host_random_number(uint32_t pass_index);
