@@ -205,6 +205,8 @@ void allocate_render_buffers(Ctx& ctx)
     RenderState& R = ctx.render;
     const size_t n = (size_t)R.settings.width * (size_t)R.settings.height;
     R.pixels = (uint32_t)n;
+    R.frame_camera_valid = false;
+    R.frame_count = 0;
     R.gb_prim.reserve(4 * n);
     R.gb_pos.reserve(16 * n);
     R.gb_gn.reserve(4 * n);
@@ -784,6 +786,7 @@ void mipt_render_settings_default(mipt_render_settings* s)
     s->direct_light_sampling_strategy = MIPT_LSS_MIS_LIGHT_BSDF;
     s->direct_light_sampling_base_strategy = MIPT_LSS_BASE_UNIFORM;
     s->do_normal_mapping = 1;
+    s->accumulate = 1;
 }
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
@@ -814,6 +817,11 @@ void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
     s->light_presampling_subset_size = 1024;
     s->light_presampling_tile_size = 8;
     s->envmap_candidate_probability = 0.5f; // ReSTIRDISettings.h:22
+    // ReSTIRCommonTemporalPassSettings (ReSTIRDISettings.h:65-69)
+    s->use_permutation_sampling = 0;
+    s->permutation_sampling_random_bits = 42;
+    s->max_neighbor_search_count = 8;
+    s->neighbor_search_radius = 4;
 }
 
 int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
@@ -828,7 +836,9 @@ int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
         (s->do_light_presampling &&
          (s->light_presampling_number_of_subsets < 1 || s->light_presampling_subset_size < 1 ||
           s->light_presampling_tile_size < 1 ||
-          (int64_t)s->light_presampling_number_of_subsets * s->light_presampling_subset_size > (1ll << 24))))
+          (int64_t)s->light_presampling_number_of_subsets * s->light_presampling_subset_size > (1ll << 24))) ||
+        s->max_neighbor_search_count < 0 || s->max_neighbor_search_count > 64 || s->neighbor_search_radius < 0 ||
+        s->neighbor_search_radius > 4096)
         return MIPT_ERR_INVALID_ARGUMENT;
     if (s->do_fused_spatiotemporal && !s->do_temporal_reuse_pass)
     {
@@ -837,8 +847,13 @@ int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
         return MIPT_ERR_UNSUPPORTED;
     }
     return mipt::guarded(ctx, [&] {
+        // New permutation bits alone (the per-frame value of the temporal
+        // passes) keep the buffers and the reservoirs in them.
+        mipt_restir_di_settings same = *s;
+        same.permutation_sampling_random_bits = ctx->render.restir.permutation_sampling_random_bits;
+        const bool bits_only = std::memcmp(&same, &ctx->render.restir, sizeof(same)) == 0;
         ctx->render.restir = *s;
-        if (ctx->render.configured)
+        if (ctx->render.configured && !bits_only)
             mipt::allocate_render_buffers(*ctx);
     });
 }
@@ -995,6 +1010,9 @@ int mipt_launch_pass(mipt_ctx* c, int32_t pass, uint32_t sample_number, uint32_t
         if (!ctx->render.camera_set)
             mipt::fail(MIPT_ERR_INVALID_ARGUMENT, "mipt_launch_pass: call mipt_set_camera first");
         mipt::prepare_scene_for_render(*ctx);
+        // Not accumulating, every frame is sample 0 (MegaKernelRenderPass.cpp:42-43).
+        if (!ctx->render.settings.accumulate)
+            sample_number = 0;
         // The ReSTIR DI passes rebuild each surface's volume state, which for
         // a dispersive glass draws a wavelength from the pass's stream at
         // every surface fetch (ReSTIR/Surface.h:25-43); those draws are not
@@ -1062,14 +1080,16 @@ int render_sample_passes(mipt_ctx* c, uint32_t sample_number, void* stream, bool
         mipt::RenderState& R = ctx->render;
         const bool plain = R.settings.direct_light_sampling_strategy == MIPT_LSS_MIS_LIGHT_BSDF &&
                            !R.settings.radiance_cache && !warmup;
-        if (R.graph_enabled && plain && sample_number > 0 && R.configured && R.camera_set)
+        if (R.graph_enabled && plain && sample_number > 0 && R.configured && R.camera_set && R.settings.accumulate)
             return mipt::guarded(ctx, [&] {
                 mipt::render_sample_graph(*ctx, sample_number, reinterpret_cast<hipStream_t>(stream));
             });
         R.graph_replays = 0;
     }
     const uint32_t draws = (uint32_t)mipt_draws_per_sample(c);
-    uint32_t k = sample_number * draws;
+    // Not accumulating, the host stream runs on from frame to frame
+    // (GPURenderer::reset leaves it alone then).
+    uint32_t k = (ctx->render.settings.accumulate ? sample_number : ctx->render.frame_count++) * draws;
     int rc = mipt_launch_pass(c, MIPT_PASS_CAMERA_RAYS, sample_number, mipt_host_random_number(k++), stream);
     const mipt::RenderState& R = ctx->render;
     if (rc == MIPT_OK && R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
@@ -1085,7 +1105,9 @@ int render_sample_passes(mipt_ctx* c, uint32_t sample_number, void* stream, bool
         if (rc == MIPT_OK && R.restir.do_fused_spatiotemporal)
         {
             // the temporal pass's number and permutation bits are drawn, then
-            // the fused kernel's (configure_spatiotemporal_pass)
+            // the fused kernel's (configure_temporal_pass_for_fused_spatiotemporal,
+            // configure_spatial_pass)
+            ctx->render.restir.permutation_sampling_random_bits = (int32_t)mipt_host_random_number(k + 1);
             k += 2;
             rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE, sample_number,
                                   mipt_host_random_number(k++), stream);
@@ -1097,7 +1119,8 @@ int render_sample_passes(mipt_ctx* c, uint32_t sample_number, void* stream, bool
         {
             if (rc == MIPT_OK && R.restir.do_temporal_reuse_pass)
             {
-                // the second draw is the permutation-sampling bits, unused here
+                // the second draw is the permutation-sampling bits (configure_temporal_pass)
+                ctx->render.restir.permutation_sampling_random_bits = (int32_t)mipt_host_random_number(k + 1);
                 rc = mipt_launch_pass(c, MIPT_PASS_RESTIR_DI_TEMPORAL_REUSE, sample_number,
                                       mipt_host_random_number(k), stream);
                 k += 2;

@@ -102,6 +102,11 @@ struct RenderState {
     bool camera_set = false;
     mipt_render_settings settings{};
     mipt_camera camera{};
+    // the camera of the last camera-ray pass, and the one before it (the
+    // reprojection of ReSTIR DI temporal reuse when not accumulating)
+    mipt_camera frame_camera{}, prev_camera{};
+    bool frame_camera_valid = false;
+    uint32_t frame_count = 0; // mipt_render_sample calls since the buffers were allocated (not accumulating)
     uint32_t pixels = 0;
     // scene-derived
     DeviceBuffer materials;     // pt::Material[material_count]

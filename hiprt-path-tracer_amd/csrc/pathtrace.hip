@@ -1158,9 +1158,14 @@ void camera_rays_pass(Ctx& ctx, uint32_t sample_number, uint32_t random_number, 
                     R.gb_mat.as<int32_t>()};
     GBufferView prev{R.pg_prim.as<int32_t>(), R.pg_pos.as<float4>(), R.pg_gn.as<uint32_t>(), R.pg_sn.as<uint32_t>(),
                      R.pg_mat.as<int32_t>()};
-    if (sample_number == 0 && !frame)
+    // prev_camera: the camera of the previous frame (the first frame's own)
+    R.prev_camera = R.frame_camera_valid ? R.frame_camera : R.camera;
+    R.frame_camera = R.camera;
+    R.frame_camera_valid = true;
+    if (sample_number == 0 && !frame && R.settings.accumulate)
     {
-        // need_to_reset: G-buffers and ReSTIR DI reservoirs start over.
+        // need_to_reset: G-buffers and ReSTIR DI reservoirs start over; not
+        // accumulating, a moving camera does not reset them (GPURenderer::reset).
         hipLaunchKernelGGL(k_gbuffer_reset, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur, n);
         if (keep_prev)
             hipLaunchKernelGGL(k_gbuffer_reset, dim3(blocks_for(n, 256)), dim3(256), 0, s, prev, n);

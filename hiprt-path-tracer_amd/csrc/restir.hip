@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "pt_device.h"
 
@@ -104,6 +105,10 @@ struct RsData {
     mipt_restir_di_settings s;
     int32_t pass_index;
     int32_t kill_after; // spatial: visibility reuse after this pass
+    // the previous frame's camera (temporal reprojection when not accumulating)
+    int32_t accumulate;
+    float prev_vp[4][4];
+    float prev_cam_pos[3];
 };
 
 __device__ __forceinline__ void state_store(const RsData& r, uint32_t pix, const Res& res, uint32_t rng, int32_t phase,
@@ -228,12 +233,15 @@ __device__ __forceinline__ Surface surface_current(const PtData& d, uint32_t pix
     return s;
 }
 
+// get_pixel_surface_previous_frame (Surface.h:55-71): the previous G-buffer,
+// seen from the previous frame's camera through the current frame's hit
+// position at that pixel.
 __device__ __forceinline__ Surface surface_previous(const PtData& d, const RsData& r, uint32_t pix)
 {
     Surface s;
     s.mat = r.pg_mat[pix];
     s.prim = r.pg_prim[pix];
-    const f3 cam = pt::mk(d.cam_pos[0], d.cam_pos[1], d.cam_pos[2]);
+    const f3 cam = pt::mk(r.prev_cam_pos[0], r.prev_cam_pos[1], r.prev_cam_pos[2]);
     s.view = pt::normalize(cam - ld3(d.gb_pos[pix]));
     s.sn = gb_normal(r.pg_sn[pix]);
     s.gn = gb_normal(r.pg_gn[pix]);
@@ -677,9 +685,104 @@ __global__ void k_ic_finish(RsData r)
     res_store(r.out, pix, res);
 }
 
+// check_neighbor_similarity_heuristics(..., previous_frame = true)
+// (NeighborSimilarity.h:52-107): primary hit and position of the previous
+// frame's G-buffer, the neighbour's normal from the current one.
+__device__ __forceinline__ bool similar_prev(const PtData& d, const RsData& r, int32_t npix, uint32_t cpix, f3 cpoint,
+                                             f3 cnormal)
+{
+    if ((uint32_t)npix == cpix)
+        return true;
+    if (r.pg_prim[npix] == -1)
+        return false;
+    const f3 np = ld3(r.pg_pos[npix]);
+    const f3 nn = r.s.reject_using_geometric_normals ? gb_normal(d.gb_gn[npix]) : gb_normal(d.gb_sn[npix]);
+    bool plane = true, normal = true;
+    if (r.s.use_plane_distance_heuristic)
+        plane = fabsf(pt::dot(np - cpoint, cnormal)) < r.s.plane_distance_threshold;
+    if (r.s.use_normal_similarity_heuristic)
+        normal = pt::dot(cnormal, nn) > r.s.normal_similarity_angle_precomp;
+    return plane && normal;
+}
+
+// A pixel coordinate from a rounded float: values out of any viewport's
+// range go to a coordinate far outside (their int conversion is undefined in
+// the reference).
+__device__ __forceinline__ int32_t pixel_coord(float v) { return (v > -1.0e9f && v < 1.0e9f) ? (int32_t)v : -1000000000; }
+
+// find_temporal_neighbor_index (UtilsTemporal.h:37-99): (index, x, y) of the
+// temporal neighbour. Accumulating, the pixel itself. Else the primary hit
+// reprojected by the previous frame's view-projection (matrix_X_point,
+// Math.h:617-633), rounded, optionally permuted within 4 x 4 blocks
+// (apply_permutation_sampling, :11-22), then up to max_neighbor_search_count
+// random pixels within neighbor_search_radius (x offset drawn first), the
+// first that passes the similarity heuristics against the previous G-buffer;
+// index -1 without one. (x, y) is the rounded reprojected pixel.
+__device__ int3 temporal_neighbor(const PtData& d, const RsData& r, uint32_t pix, f3 cnormal, pt::Xorshift32& rng)
+{
+    const int32_t W = d.width, H = d.height;
+    if (r.accumulate)
+        return make_int3((int32_t)pix, (int32_t)(pix % (uint32_t)W), (int32_t)(pix / (uint32_t)W));
+    const f3 p = ld3(d.gb_pos[pix]);
+    const float(*m)[4] = r.prev_vp;
+    const float xt = m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3];
+    const float yt = m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3];
+    const float wt = m[3][0] * p.x + m[3][1] * p.y + m[3][2] * p.z + m[3][3];
+    float inv_w = 1.0f;
+    if (!(wt < 1.0e-10f && wt > -1.0e-10f))
+        inv_w = 1.0f / wt;
+    float sx = xt * inv_w, sy = yt * inv_w;
+    sx += 1.0f;
+    sy += 1.0f;
+    sx *= 0.5f;
+    sy *= 0.5f;
+    float fx = sx * (float)W, fy = sy * (float)H;
+    fx -= 0.5f;
+    fy -= 0.5f;
+    int32_t idx = -1;
+    for (int32_t i = 0; i < r.s.max_neighbor_search_count + 1; i++)
+    {
+        float ox = 0.0f, oy = 0.0f;
+        if (i > 0)
+        {
+            const float rx = rng() - 0.5f;
+            const float ry = rng() - 0.5f;
+            ox = rx * (float)r.s.neighbor_search_radius;
+            oy = ry * (float)r.s.neighbor_search_radius;
+        }
+        int32_t x = pixel_coord(roundf(fx + ox)), y = pixel_coord(roundf(fy + oy));
+        if (r.s.use_permutation_sampling && i == 0)
+        {
+            const int32_t bits = r.s.permutation_sampling_random_bits;
+            const int32_t dx = bits & 3, dy = (bits >> 2) & 3;
+            x += dx;
+            y += dy;
+            x ^= 3;
+            y ^= 3;
+            x -= dx;
+            y -= dy;
+        }
+        if (x < 0 || x >= W || y < 0 || y >= H)
+            continue;
+        idx = x + y * W;
+        if (similar_prev(d, r, idx, pix, p, cnormal))
+            break;
+        idx = -1;
+    }
+    return make_int3(idx, pixel_coord(roundf(fx)), pixel_coord(roundf(fy)));
+}
+
 // ---------------------------------------------------------------------------
-// TEMPORAL REUSE (accumulation: the temporal neighbour is the pixel itself)
+// TEMPORAL REUSE (TemporalReuse.h:50-296). The temporal neighbour is the
+// pixel itself when accumulating, the reprojected pixel otherwise; k_tp_start
+// keeps its index in the pass state (slot 3.z) with the pixel's random state
+// after the neighbour search, which k_tp_finish resumes.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void state_store_neighbor(const RsData& r, uint32_t pix, int32_t neighbor)
+{
+    r.state[4 * (size_t)pix + 3].z = __int_as_float(neighbor);
+}
+
 __global__ void k_tp_start(PtData d, RsData r)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -692,19 +795,26 @@ __global__ void k_tp_start(PtData d, RsData r)
         state_store(r, pix, res_empty(), 0u, PHASE_SKIP, 0.0f, 0.0f, 0u);
         return;
     }
-    const Res tn = res_load(r.in, pix);
-    if (d.freeze || tn.M == 0)
+    pt::Xorshift32 rng{pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
+    const Surface center = surface_current(d, pix);
+    const int32_t tix = temporal_neighbor(d, r, pix, center.sn, rng).x;
+    if (tix == -1 || d.freeze)
     {
         state_store(r, pix, res_empty(), 0u, PHASE_COPY, 0.0f, 0.0f, 0u);
         return;
     }
-    const Surface prev = surface_previous(d, r, pix);
+    const Res tn = res_load(r.in, (uint32_t)tix);
+    if (tn.M == 0)
+    {
+        state_store(r, pix, res_empty(), 0u, PHASE_COPY, 0.0f, 0.0f, 0u);
+        return;
+    }
+    const Surface prev = surface_previous(d, r, (uint32_t)tix);
     if (is_emissive(d, prev.mat))
     {
         state_store(r, pix, res_empty(), 0u, PHASE_COPY, 0.0f, 0.0f, 0u);
         return;
     }
-    const Surface center = surface_current(d, pix);
     const Res init = res_load(r.init, pix);
     uint32_t rays = 0u;
     float tf_center = 0.0f;
@@ -727,7 +837,8 @@ __global__ void k_tp_start(PtData d, RsData r)
         write_ray(r.rays, (size_t)r.n + pix, prev.p, prev.prim, dir, dist - 1.0e-4f);
         rays |= 2u;
     }
-    state_store(r, pix, res_empty(), 0u, PHASE_DONE, tf_center, tf_csn, rays);
+    state_store(r, pix, res_empty(), rng.state, PHASE_DONE, tf_center, tf_csn, rays);
+    state_store_neighbor(r, pix, tix);
 }
 
 // TemporalReuse.h:100-296 with the pairwise MIS weights of
@@ -750,12 +861,13 @@ __global__ void k_tp_finish(PtData d, RsData r)
         res_store(r.out, pix, init);
         return;
     }
-    const Res tn = res_load(r.in, pix);
+    const int32_t tix = __float_as_int(r.state[4 * (size_t)pix + 3].z);
+    const Res tn = res_load(r.in, (uint32_t)tix);
     if (rays & 1u)
         tf_center = apply_visibility(tf_center, r.occ[pix] != 0);
     if (rays & 2u)
         tf_csn = apply_visibility(tf_csn, r.occ[(size_t)r.n + pix] != 0);
-    pt::Xorshift32 rng{pt::pixel_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
+    pt::Xorshift32 rng{rng_state};
     Res out = res_empty();
     const bool cw = r.s.use_confidence_weights != 0;
     const float tn_M = cw ? (float)tn.M : 1.0f;
@@ -996,10 +1108,10 @@ __global__ void k_sp_finish(RsData r)
 }
 
 // ---------------------------------------------------------------------------
-// FUSED SPATIOTEMPORAL REUSE (FusedSpatiotemporalReuse.h:152-566), static
-// camera: the temporal neighbour is the pixel itself
-// (find_temporal_neighbor_index, UtilsTemporal.h:37-41) and the spatial
-// neighbours are drawn around it. Temporal neighbour and spatial neighbours
+// FUSED SPATIOTEMPORAL REUSE (FusedSpatiotemporalReuse.h:152-566): the
+// temporal neighbour is found by find_temporal_neighbor_index (the pixel
+// itself when accumulating) and the spatial neighbours are drawn around its
+// reprojected position. Temporal neighbour and spatial neighbours
 // come from last frame's output reservoirs; neighbour surfaces and the
 // similarity tests read the previous frame's G-buffer
 // (use_prev_frame_g_buffer(), RenderSettings.h:263-272). Pairwise MIS
@@ -1015,8 +1127,10 @@ __global__ void k_sp_finish(RsData r)
 // Per-pixel aux layout (stride fst_aux_stride(K)): [0] temporal target at the
 // centre (unoccluded), [1] neighbour seed, [2] valid count, [3] valid M sum,
 // [4] heuristics cache, [5] ray mask (bit 0 centre, bit 1 temporal neighbour),
-// [6] -, [7] -, then (target, confidence multiplier) for the temporal
-// neighbour and for each spatial neighbour (multiplier < 0: no entry).
+// [6], [7] the reprojected pixel (the spatial neighbours' centre), then
+// (target, confidence multiplier) for the temporal neighbour and for each
+// spatial neighbour (multiplier < 0: no entry). The temporal neighbour's
+// index is in the pass state (slot 3.z).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ size_t fst_aux_stride(int32_t K) { return 8 + 2 * (size_t)(K + 1); }
 
@@ -1025,26 +1139,6 @@ __device__ __forceinline__ size_t fst_aux_stride(int32_t K) { return 8 + 2 * (si
 __device__ __forceinline__ uint32_t fused_seed(uint32_t pix, uint32_t sample_number, uint32_t random_number, bool freeze)
 {
     return freeze ? pt::wang_hash(pix + 1u) : pt::wang_hash(((pix + 1u) * (sample_number + 1u)) ^ random_number);
-}
-
-// check_neighbor_similarity_heuristics(..., previous_frame = true)
-// (NeighborSimilarity.h:52-107): primary hit and position of the previous
-// frame's G-buffer, the neighbour's normal from the current one.
-__device__ __forceinline__ bool similar_prev(const PtData& d, const RsData& r, int32_t npix, uint32_t cpix, f3 cpoint,
-                                             f3 cnormal)
-{
-    if ((uint32_t)npix == cpix)
-        return true;
-    if (r.pg_prim[npix] == -1)
-        return false;
-    const f3 np = ld3(r.pg_pos[npix]);
-    const f3 nn = r.s.reject_using_geometric_normals ? gb_normal(d.gb_gn[npix]) : gb_normal(d.gb_sn[npix]);
-    bool plane = true, normal = true;
-    if (r.s.use_plane_distance_heuristic)
-        plane = fabsf(pt::dot(np - cpoint, cnormal)) < r.s.plane_distance_threshold;
-    if (r.s.use_normal_similarity_heuristic)
-        normal = pt::dot(cnormal, nn) > r.s.normal_similarity_angle_precomp;
-    return plane && normal;
 }
 
 __global__ __launch_bounds__(256, 2) void k_fst_start(PtData d, RsData r)
@@ -1060,13 +1154,15 @@ __global__ __launch_bounds__(256, 2) void k_fst_start(PtData d, RsData r)
         state_store(r, pix, res_empty(), 0u, PHASE_SKIP, 0.0f, 0.0f, 0u);
         return;
     }
-    const int32_t cx = (int32_t)(pix % (uint32_t)d.width), cy = (int32_t)(pix / (uint32_t)d.width);
     pt::Xorshift32 rng{fused_seed(pix, d.sample_number, d.random_number, d.freeze != 0)};
     const uint32_t nseed = rng.next();
     const Surface cs = surface_current(d, pix);
-    // load_spatiotemporal_neighbor_data (:71-101): frozen random leaves the
-    // temporal reservoir empty
-    const Res tn = d.freeze ? res_empty() : res_load(r.in, pix);
+    // load_spatiotemporal_neighbor_data (:71-101): no temporal neighbour or
+    // frozen random leaves the temporal reservoir empty; the spatial
+    // neighbours are drawn around the reprojected pixel
+    const int3 tnb = temporal_neighbor(d, r, pix, cs.sn, rng);
+    const Res tn = (tnb.x == -1 || d.freeze) ? res_empty() : res_load(r.in, (uint32_t)tnb.x);
+    const int32_t cx = tnb.y, cy = tnb.z;
     const f3 cnormal = r.s.reject_using_geometric_normals ? cs.gn : cs.sn;
     // count_valid_spatiotemporal_neighbors (:103-148)
     int32_t valid_count = 0, valid_M = 0;
@@ -1117,7 +1213,7 @@ __global__ __launch_bounds__(256, 2) void k_fst_start(PtData d, RsData r)
         }
         if (update_mc)
         {
-            const Surface ts = surface_previous(d, r, pix);
+            const Surface ts = surface_previous(d, r, (uint32_t)tnb.x);
             const float tf_csn = target_function(d, ts, init.point, init.tri, init.flags, dir, dist);
             const float tn_M = cw ? (float)tn.M : 1.0f;
             aux[8] = tf_csn;
@@ -1161,7 +1257,10 @@ __global__ __launch_bounds__(256, 2) void k_fst_start(PtData d, RsData r)
     aux[3] = __int_as_float(valid_M);
     aux[4] = __uint_as_float(cache);
     aux[5] = __uint_as_float(rays);
+    aux[6] = __int_as_float(cx);
+    aux[7] = __int_as_float(cy);
     state_store(r, pix, res_empty(), rng.state, PHASE_PENDING, 0.0f, 0.0f, 0u);
+    state_store_neighbor(r, pix, tnb.x);
 }
 
 __global__ __launch_bounds__(256, 2) void k_fst_combine(PtData d, RsData r)
@@ -1178,8 +1277,9 @@ __global__ __launch_bounds__(256, 2) void k_fst_combine(PtData d, RsData r)
     if (phase == PHASE_SKIP)
         return;
     const int32_t K = r.s.spatial_reuse_neighbor_count;
-    const int32_t cx = (int32_t)(pix % (uint32_t)d.width), cy = (int32_t)(pix / (uint32_t)d.width);
     const float* aux = r.aux + fst_aux_stride(K) * pix;
+    const int32_t cx = __float_as_int(aux[6]), cy = __float_as_int(aux[7]);
+    const int32_t tix = __float_as_int(r.state[4 * (size_t)pix + 3].z);
     const uint32_t nseed = __float_as_uint(aux[1]);
     const int32_t valid_count = __float_as_int(aux[2]);
     const int32_t valid_M = __float_as_int(aux[3]);
@@ -1187,7 +1287,7 @@ __global__ __launch_bounds__(256, 2) void k_fst_combine(PtData d, RsData r)
     const uint32_t rays = __float_as_uint(aux[5]);
     pt::Xorshift32 rng{rng_state};
     const Surface cs = surface_current(d, pix);
-    const Res tn = d.freeze ? res_empty() : res_load(r.in, pix);
+    const Res tn = (tix == -1 || d.freeze) ? res_empty() : res_load(r.in, (uint32_t)tix);
     const Res init = res_load(r.init, pix);
     const bool cw = r.s.use_confidence_weights != 0;
     const float center_M = cw ? (float)init.M : 1.0f;
@@ -1310,6 +1410,9 @@ RsData make_rs_data(Ctx& ctx)
     r.pg_mat = R.pg_mat.as<int32_t>();
     r.n = R.pixels;
     r.s = R.restir;
+    r.accumulate = R.settings.accumulate;
+    std::memcpy(r.prev_vp, R.prev_camera.view_projection, sizeof(r.prev_vp));
+    std::memcpy(r.prev_cam_pos, R.prev_camera.position, sizeof(r.prev_cam_pos));
     return r;
 }
 

@@ -124,7 +124,7 @@ class RenderSettings(C.Structure):
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
         ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
-        ("do_normal_mapping", C.c_int32),
+        ("do_normal_mapping", C.c_int32), ("accumulate", C.c_int32),
     ]
 
 
@@ -142,7 +142,9 @@ class RestirDISettings(C.Structure):
         ("guard_backfacing_bsdf_candidates", C.c_int32), ("do_fused_spatiotemporal", C.c_int32),
         ("do_light_presampling", C.c_int32), ("light_presampling_number_of_subsets", C.c_int32),
         ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
-        ("envmap_candidate_probability", C.c_float),
+        ("envmap_candidate_probability", C.c_float), ("use_permutation_sampling", C.c_int32),
+        ("permutation_sampling_random_bits", C.c_int32), ("max_neighbor_search_count", C.c_int32),
+        ("neighbor_search_radius", C.c_int32),
     ]
 
 

@@ -372,6 +372,16 @@ typedef struct mipt_render_settings {
      * (get_shading_normal, Intersect.h:114-133; normal maps need textures,
      * which this build does not upload); 0 keeps the geometric normal. */
     int32_t do_normal_mapping;
+    /* RenderSettings::accumulate (RenderSettings.h:71-73), default 1. 0 is
+     * the interactive mode: every frame is sample 0 whatever sample_number
+     * the call passes (MegaKernelRenderPass.cpp:42-43: the framebuffer is
+     * overwritten, the per-pixel seeds use sample 0), the G-buffers and
+     * ReSTIR DI reservoirs are not reset between frames, mipt_render_sample
+     * draws its random numbers from a running frame count, and ReSTIR DI
+     * temporal reuse reprojects each pixel's primary hit into the previous
+     * frame's camera to find its temporal neighbour
+     * (find_temporal_neighbor_index, UtilsTemporal.h:37-99). */
+    int32_t accumulate;
 } mipt_render_settings;
 
 #define MIPT_LSS_MIS_LIGHT_BSDF 3 /* LSS_MIS_LIGHT_BSDF: one light sample + one BSDF sample, balance heuristic */
@@ -431,6 +441,21 @@ typedef struct mipt_restir_di_settings {
      * that a light candidate (or a presampled light) is an envmap sample; 1
      * when the scene has no emissive triangle. */
     float envmap_candidate_probability;
+    /* ReSTIRCommonTemporalPassSettings (ReSTIRCommonSettings.h:14-34,
+     * defaults ReSTIRDISettings.h:65-69), read when render_settings.accumulate
+     * is 0: the reprojected pixel is taken if it passes the similarity
+     * heuristics against the previous frame's G-buffer, else up to
+     * max_neighbor_search_count (8) random pixels within
+     * neighbor_search_radius (4) px of it. use_permutation_sampling (0)
+     * shuffles the reprojected pixel within 4 x 4 blocks by
+     * permutation_sampling_random_bits, which the reference draws from its
+     * host stream each frame (ReSTIRDIRenderPass.cpp:470, :518);
+     * mipt_render_sample sets it the same way, mipt_launch_pass callers set
+     * it here before the temporal or fused pass. */
+    int32_t use_permutation_sampling;
+    int32_t permutation_sampling_random_bits;
+    int32_t max_neighbor_search_count;
+    int32_t neighbor_search_radius;
 } mipt_restir_di_settings;
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s);

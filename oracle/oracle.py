@@ -40,7 +40,7 @@ class RenderSettings(C.Structure):
         ("microfacet_regularization_min_roughness", C.c_float), ("do_microfacet_regularization", C.c_int32),
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
         ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
-        ("do_normal_mapping", C.c_int32),
+        ("do_normal_mapping", C.c_int32), ("accumulate", C.c_int32),
     ]
 
 
@@ -56,7 +56,9 @@ class RestirSettings(C.Structure):
         ("guard_backfacing_bsdf_candidates", C.c_int32), ("do_fused_spatiotemporal", C.c_int32),
         ("do_light_presampling", C.c_int32), ("light_presampling_number_of_subsets", C.c_int32),
         ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
-        ("envmap_candidate_probability", C.c_float),
+        ("envmap_candidate_probability", C.c_float), ("use_permutation_sampling", C.c_int32),
+        ("permutation_sampling_random_bits", C.c_int32), ("max_neighbor_search_count", C.c_int32),
+        ("neighbor_search_radius", C.c_int32),
     ]
 
 
@@ -119,6 +121,9 @@ _SIG = {
     "orc_frame_set_window": (None, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "orc_shading_normal": (None, [_P, _P, _P, C.c_int32, C.c_float, C.c_float, _P, _P]),
     "orc_frame_camera_pass": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
+    "orc_frame_set_camera": (None, [C.c_void_p, _P]),
+    "orc_frame_set_permutation_bits": (None, [C.c_void_p, C.c_int32]),
+    "orc_frame_temporal_neighbors": (None, [C.c_void_p, C.c_uint32, C.c_uint32, _P]),
     "orc_frame_restir_initial": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_temporal": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32]),
     "orc_frame_restir_spatial": (None, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]),
@@ -418,18 +423,39 @@ class Renderer:
         """Later passes compute only the pixels of [x0, x1) x [y0, y1)."""
         lib().orc_frame_set_window(self.h, int(x0), int(x1), int(y0), int(y1))
 
+    def set_camera(self, camera):
+        """The camera of the next frames (the library's Camera); the camera
+        of the last camera pass becomes the previous frame's."""
+        self.cam = Camera()
+        C.memmove(C.byref(self.cam), C.byref(camera), C.sizeof(Camera))
+        lib().orc_frame_set_camera(self.h, C.byref(self.cam))
+
     def camera_pass(self, sample_number, rn, nthreads=0):
-        lib().orc_frame_camera_pass(self.h, sample_number, rn, nthreads)
+        lib().orc_frame_camera_pass(self.h, self._sample(sample_number), rn, nthreads)
         lib().orc_frame_gbuffer_prims(self.h, _p(self.gb_prims))
 
+    def _sample(self, sample_number):
+        """Not accumulating, every frame is sample 0 (MegaKernelRenderPass.cpp:42-43)."""
+        return sample_number if self.settings.accumulate else 0
+
     def restir_initial(self, sample_number, rn, nthreads=0):
-        lib().orc_frame_restir_initial(self.h, sample_number, rn, nthreads)
+        lib().orc_frame_restir_initial(self.h, self._sample(sample_number), rn, nthreads)
 
     def restir_temporal(self, sample_number, rn, nthreads=0):
-        lib().orc_frame_restir_temporal(self.h, sample_number, rn, nthreads)
+        lib().orc_frame_restir_temporal(self.h, self._sample(sample_number), rn, nthreads)
+
+    def temporal_neighbors(self, sample_number, rn):
+        """(n, 3) int32: each pixel's temporal neighbour (index, -1 none, -2
+        no hit) and reprojected (x, y), as the temporal pass would find them."""
+        out = np.zeros(3 * self.n, np.int32)
+        lib().orc_frame_temporal_neighbors(self.h, self._sample(sample_number), rn, _p(out))
+        return out.reshape(-1, 3)
+
+    def set_permutation_bits(self, bits):
+        lib().orc_frame_set_permutation_bits(self.h, C.c_int32(bits & 0xffffffff).value)
 
     def restir_spatial(self, pass_index, sample_number, rn, nthreads=0):
-        lib().orc_frame_restir_spatial(self.h, pass_index, sample_number, rn, nthreads)
+        lib().orc_frame_restir_spatial(self.h, pass_index, self._sample(sample_number), rn, nthreads)
 
     def env_tables(self):
         """(alias probabilities, aliases, luminance sum, packed texels) of the envmap."""
@@ -458,13 +484,14 @@ class Renderer:
         return r, float(pdf[0])
 
     def restir_presample(self, sample_number, rn, nthreads=0):
-        lib().orc_frame_restir_presample(self.h, sample_number, rn, nthreads)
+        lib().orc_frame_restir_presample(self.h, self._sample(sample_number), rn, nthreads)
 
     def restir_spatiotemporal(self, sample_number, rn, nthreads=0):
-        lib().orc_frame_restir_spatiotemporal(self.h, sample_number, rn, nthreads)
+        lib().orc_frame_restir_spatiotemporal(self.h, self._sample(sample_number), rn, nthreads)
 
     def path_trace(self, sample_number, rn, nthreads=0):
-        self.rejected += lib().orc_frame_path_trace(self.h, sample_number, rn, _p(self.fb), _p(self.fb64), nthreads)
+        self.rejected += lib().orc_frame_path_trace(self.h, self._sample(sample_number), rn, _p(self.fb), _p(self.fb64),
+                                                    nthreads)
 
     def render_sample(self, sample_number, rn_camera, rn_path, nthreads=0):
         """CAMERA_RAYS then PATH_TRACE."""
@@ -486,14 +513,15 @@ class Renderer:
             self.restir_initial(sample_number, next(rn), nthreads)
             if r.do_fused_spatiotemporal:
                 next(rn)  # the temporal pass's number
-                next(rn)  # its permutation bits
+                self.set_permutation_bits(next(rn))
                 self.restir_spatiotemporal(sample_number, next(rn), nthreads)
                 for i in range(1, r.spatial_number_of_passes):
                     self.restir_spatial(i, sample_number, next(rn), nthreads)
             else:
                 if r.do_temporal_reuse_pass:
-                    self.restir_temporal(sample_number, next(rn), nthreads)
-                    next(rn)
+                    rn_t = next(rn)
+                    self.set_permutation_bits(next(rn))
+                    self.restir_temporal(sample_number, rn_t, nthreads)
                 if r.do_spatial_reuse_pass:
                     for i in range(r.spatial_number_of_passes):
                         self.restir_spatial(i, sample_number, next(rn), nthreads)

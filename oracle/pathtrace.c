@@ -2506,6 +2506,10 @@ struct orc_frame {
     uint32_t* env_texels;
     float* env_probas;
     int32_t* env_alias;
+    /* the camera of the next camera pass (R.cam points here), the camera of
+     * the last camera pass and the one before it */
+    orc_camera cam, frame_cam, prev_cam;
+    int frame_cam_valid;
 };
 
 /* ReSTIRDIPresampledLight (ReSTIR/DI/PresampledLight.h:14-30) */
@@ -2621,7 +2625,7 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
     R.lut_thin_glass = sc->luts.thin_glass;
     R.glass_exp = sc->luts.glass_exponent;
     R.tree = tree;
-    R.cam = cam;
+    R.cam = NULL;
     R.s = *s;
     R.alias_probas = f->alias_probas;
     R.alias_alias = f->alias_alias;
@@ -2673,6 +2677,8 @@ orc_frame* orc_frame_create(const orc_render_scene* sc, const orc_octree* tree, 
             R.env_type = 1; /* set_envmap with an empty image */
     }
     f->R = R;
+    f->cam = *cam;
+    f->R.cam = &f->cam;
     if (rs)
         f->rs = *rs;
     f->n = (size_t)s->width * (size_t)s->height;
@@ -2763,13 +2769,23 @@ static void set_threads(int32_t nthreads)
 
 /* CameraRays (CameraRays.h:83-213): reset at sample 0 (reset_render :20-59),
  * previous-frame copy (:117-124), G-buffer write. */
+void orc_frame_set_camera(orc_frame* f, const orc_camera* cam) { f->cam = *cam; }
+
+void orc_frame_set_permutation_bits(orc_frame* f, int32_t bits) { f->rs.permutation_sampling_random_bits = bits; }
+
 void orc_frame_camera_pass(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
 {
     f->R.sample_number = sample_number;
     f->R.consistent_tau = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number);
     f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     int kp = keep_prev(f);
-    if (sample_number == 0)
+    /* prev_camera: the previous frame's camera (the first frame's own) */
+    f->prev_cam = f->frame_cam_valid ? f->frame_cam : f->cam;
+    f->frame_cam = f->cam;
+    f->frame_cam_valid = 1;
+    /* need_to_reset: not accumulating, a moving camera keeps the G-buffers
+     * and reservoirs (GPURenderer::reset, GPURenderer.cpp:740-761) */
+    if (sample_number == 0 && f->R.s.accumulate)
     {
         for (size_t i = 0; i < f->n; i++)
         {
@@ -2826,7 +2842,10 @@ static ovol surf_vol(const orc_frame* f, const osurf* s)
 static osurf surf_of(const orc_frame* f, const gbuf* g, size_t pix)
 {
     osurf s;
-    v3 cam = V(f->R.cam->position[0], f->R.cam->position[1], f->R.cam->position[2]);
+    /* a previous-frame surface looks from the previous frame's camera
+     * (get_pixel_surface_previous_frame, Surface.h:55-71) */
+    const orc_camera* c = g == f->prev ? &f->prev_cam : f->R.cam;
+    v3 cam = V(c->position[0], c->position[1], c->position[2]);
     s.mat = g[pix].mat;
     s.prim = g[pix].prim;
     s.p = g[pix].pos;
@@ -3170,6 +3189,97 @@ void orc_frame_restir_presample(orc_frame* f, uint32_t sample_number, uint32_t r
     }
 }
 
+static int similar_prev(const orc_frame* f, size_t np, size_t cp, v3 cpoint, v3 cnormal);
+
+/* float -> pixel coordinate, out-of-range values to a coordinate far outside
+ * any viewport (the reference's int conversion of them is undefined) */
+static int pixel_coord(float v) { return (v > -1.0e9f && v < 1.0e9f) ? (int)v : -1000000000; }
+
+/* find_temporal_neighbor_index (UtilsTemporal.h:37-99): the pixel itself
+ * when accumulating; else the primary hit reprojected by the previous
+ * frame's view-projection (matrix_X_point, Math.h:617-633), then up to
+ * max_neighbor_search_count random pixels within neighbor_search_radius
+ * (x offset drawn first), the first that passes the similarity heuristics
+ * against the previous G-buffer. Returns the index (-1: none) and the
+ * rounded reprojected pixel in *px, *py. */
+static int64_t temporal_neighbor(const orc_frame* f, size_t pix, v3 cnormal, uint32_t* st, int* px, int* py)
+{
+    const int W = f->R.s.width, H = f->R.s.height;
+    if (f->R.s.accumulate)
+    {
+        *px = (int)(pix % (size_t)W);
+        *py = (int)(pix / (size_t)W);
+        return (int64_t)pix;
+    }
+    const v3 p = f->cur[pix].pos;
+    const float(*m)[4] = f->prev_cam.view_projection;
+    float xt = m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3];
+    float yt = m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3];
+    float wt = m[3][0] * p.x + m[3][1] * p.y + m[3][2] * p.z + m[3][3];
+    float inv_w = 1.0f;
+    if (!(wt < 1.0e-10f && wt > -1.0e-10f))
+        inv_w = 1.0f / wt;
+    float sx = xt * inv_w, sy = yt * inv_w;
+    sx += 1.0f;
+    sy += 1.0f;
+    sx *= 0.5f;
+    sy *= 0.5f;
+    float fx = sx * (float)W, fy = sy * (float)H;
+    fx -= 0.5f;
+    fy -= 0.5f;
+    int64_t idx = -1;
+    for (int i = 0; i < f->rs.max_neighbor_search_count + 1; i++)
+    {
+        float ox = 0.0f, oy = 0.0f;
+        if (i > 0)
+        {
+            float rx = orc_rand01(st) - 0.5f;
+            float ry = orc_rand01(st) - 0.5f;
+            ox = rx * (float)f->rs.neighbor_search_radius;
+            oy = ry * (float)f->rs.neighbor_search_radius;
+        }
+        int x = pixel_coord(roundf(fx + ox)), y = pixel_coord(roundf(fy + oy));
+        if (f->rs.use_permutation_sampling && i == 0)
+        {
+            /* apply_permutation_sampling (UtilsTemporal.h:11-22) */
+            int bits = f->rs.permutation_sampling_random_bits;
+            int dx = bits & 3, dy = (bits >> 2) & 3;
+            x += dx;
+            y += dy;
+            x ^= 3;
+            y ^= 3;
+            x -= dx;
+            y -= dy;
+        }
+        if (x < 0 || x >= W || y < 0 || y >= H)
+            continue;
+        idx = (int64_t)x + (int64_t)y * W;
+        if (similar_prev(f, (size_t)idx, pix, p, cnormal))
+            break;
+        idx = -1;
+    }
+    *px = pixel_coord(roundf(fx));
+    *py = pixel_coord(roundf(fy));
+    return idx;
+}
+
+void orc_frame_temporal_neighbors(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t* out)
+{
+    for (size_t pix = 0; pix < f->n; pix++)
+    {
+        int32_t* o = out + 3 * pix;
+        o[0] = -2;
+        o[1] = o[2] = 0;
+        if (f->cur[pix].prim == -1)
+            continue;
+        uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, f->R.s.freeze_random);
+        int x, y;
+        o[0] = (int32_t)temporal_neighbor(f, pix, vnorm(oct_unpack(f->cur[pix].sn)), &st, &x, &y);
+        o[1] = x;
+        o[2] = y;
+    }
+}
+
 void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t rn, int32_t nthreads)
 {
     f->R.sample_number = sample_number;
@@ -3188,14 +3298,17 @@ void orc_frame_restir_temporal(orc_frame* f, uint32_t sample_number, uint32_t rn
         if (f->cur[pix].prim == -1 || !in_window(f, pix))
             continue;
         uint32_t st = orc_pixel_seed((uint32_t)pix, sample_number, rn, f->R.s.freeze_random);
-        ores tn = in[pix];
         ores ini = init[pix];
-        if (f->R.s.freeze_random || tn.M == 0)
+        int tx, ty;
+        v3 csn = vnorm(oct_unpack(f->cur[pix].sn));
+        int64_t tix = temporal_neighbor(f, pix, csn, &st, &tx, &ty);
+        if (tix < 0 || f->R.s.freeze_random || in[tix].M == 0)
         {
             out[pix] = ini;
             continue;
         }
-        osurf prev = surf_of(f, f->prev, pix);
+        ores tn = in[tix];
+        osurf prev = surf_of(f, f->prev, (size_t)tix);
         if (prev.mat >= 0)
         {
             rgb e = f->R.mats[prev.mat].emission;
@@ -3456,7 +3569,13 @@ void orc_frame_restir_spatiotemporal(orc_frame* f, uint32_t sample_number, uint3
                                            : orc_wang_hash((((uint32_t)pix + 1u) * (sample_number + 1u)) ^ rn);
         uint32_t nseed = orc_xorshift32(&st);
         osurf cs = surf_of(f, f->cur, pix);
-        ores tn = f->R.s.freeze_random ? ores_empty() : in[pix];
+        /* load_spatiotemporal_neighbor_data (:71-101); the spatial
+         * neighbours are drawn around the reprojected pixel */
+        int tx, ty;
+        int64_t tix = temporal_neighbor(f, pix, cs.sn, &st, &tx, &ty);
+        ores tn = (tix < 0 || f->R.s.freeze_random) ? ores_empty() : in[tix];
+        cx = tx;
+        cy = ty;
         v3 cnormal = f->rs.reject_using_geometric_normals ? cs.gn : cs.sn;
         int32_t valid_count = 0, valid_M = 0;
         uint32_t cache = 0u;
@@ -3494,7 +3613,7 @@ void orc_frame_restir_spatiotemporal(orc_frame* f, uint32_t sample_number, uint3
             float mi = denom == 0.0f ? 0.0f : nume / denom;
             if (update_mc)
             {
-                osurf ts = surf_of(f, f->prev, pix);
+                osurf ts = surf_of(f, f->prev, (size_t)tix);
                 float tf_csn = target_fn(f, &ts, init.point, init.tri, init.flags, 1);
                 float denom_mc = tf_csn * conf_sum + nume_mc;
                 float conf_mult = cw ? tn_M / conf_sum : 1.0f;

@@ -63,9 +63,10 @@ def open_back_box_scene(mipt_mod):
     return open_back_box
 
 
-def look_at(mipt, eye, target, vfov_deg, width, height):
-    """A jittered pinhole camera (HIPRTCamera matrices as the scene
-    generator builds them: OpenGL view and perspective, near 0.1, far 1000)."""
+def look_at(mipt, eye, target, vfov_deg, width, height, jitter=True):
+    """A pinhole camera, jittered by default (HIPRTCamera matrices as the
+    scene generator builds them: OpenGL view and perspective, near 0.1, far
+    1000)."""
     import numpy as np
 
     eye = np.asarray(eye, np.float64)
@@ -90,7 +91,7 @@ def look_at(mipt, eye, target, vfov_deg, width, height):
         cam.view_projection[i][:] = [float(np.float32(x)) for x in (proj @ view)[i]]
     cam.position[:] = [float(np.float32(x)) for x in eye]
     cam.vertical_fov = float(np.radians(vfov_deg))
-    cam.sensor_width, cam.sensor_height, cam.do_jittering = width, height, 1
+    cam.sensor_width, cam.sensor_height, cam.do_jittering = width, height, int(jitter)
     return cam
 
 
