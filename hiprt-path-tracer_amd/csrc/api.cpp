@@ -218,6 +218,9 @@ void allocate_render_buffers(Ctx& ctx)
     R.rng.reserve(4 * n);
     R.flags.reserve(4 * n);
     R.vol_a.reserve(16 * n);
+    if (R.settings.direct_light_sampling_strategy == MIPT_LSS_RIS_BSDF_AND_LIGHT ||
+        R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI)
+        R.ris_state.reserve(48 * n);
     R.vol_b.reserve(16 * n);
     R.shadow_rays.reserve(32 * n);
     R.shadow_occ.reserve(n);
@@ -787,6 +790,8 @@ void mipt_render_settings_default(mipt_render_settings* s)
     s->direct_light_sampling_base_strategy = MIPT_LSS_BASE_UNIFORM;
     s->do_normal_mapping = 1;
     s->accumulate = 1;
+    s->ris_number_of_light_candidates = 4; // RISSettings.h:12-15
+    s->ris_number_of_bsdf_candidates = 1;
 }
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
@@ -822,6 +827,7 @@ void mipt_restir_di_settings_default(mipt_restir_di_settings* s)
     s->permutation_sampling_random_bits = 42;
     s->max_neighbor_search_count = 8;
     s->neighbor_search_radius = 4;
+    s->later_bounces_sampling_strategy = MIPT_RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT; // ReSTIRDIOptions.h:140
 }
 
 int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
@@ -838,7 +844,9 @@ int mipt_set_restir_di_settings(mipt_ctx* c, const mipt_restir_di_settings* s)
           s->light_presampling_tile_size < 1 ||
           (int64_t)s->light_presampling_number_of_subsets * s->light_presampling_subset_size > (1ll << 24))) ||
         s->max_neighbor_search_count < 0 || s->max_neighbor_search_count > 64 || s->neighbor_search_radius < 0 ||
-        s->neighbor_search_radius > 4096)
+        s->neighbor_search_radius > 4096 ||
+        (s->later_bounces_sampling_strategy != MIPT_RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF &&
+         s->later_bounces_sampling_strategy != MIPT_RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT))
         return MIPT_ERR_INVALID_ARGUMENT;
     if (s->do_fused_spatiotemporal && !s->do_temporal_reuse_pass)
     {
@@ -866,7 +874,10 @@ int mipt_set_render_settings(mipt_ctx* c, const mipt_render_settings* s)
     if (!ctx || !s || s->width <= 0 || s->height <= 0 || s->nb_bounces < 0 ||
         (int64_t)s->width * (int64_t)s->height > (1ll << 28) ||
         (s->direct_light_sampling_strategy != MIPT_LSS_MIS_LIGHT_BSDF &&
+         s->direct_light_sampling_strategy != MIPT_LSS_RIS_BSDF_AND_LIGHT &&
          s->direct_light_sampling_strategy != MIPT_LSS_RESTIR_DI) ||
+        s->ris_number_of_light_candidates < 0 || s->ris_number_of_light_candidates > 64 ||
+        s->ris_number_of_bsdf_candidates < 0 || s->ris_number_of_bsdf_candidates > 1 ||
         (s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_UNIFORM &&
          s->direct_light_sampling_base_strategy != MIPT_LSS_BASE_POWER) ||
         (s->radiance_cache != 0 && s->radiance_cache != 1))

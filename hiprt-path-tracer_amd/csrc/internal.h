@@ -139,6 +139,7 @@ struct RenderState {
     // path state
     DeviceBuffer throughput, thr_prev, radiance, vtx_pos, vtx_sn, vtx_gn, vtx_view, nee_light, nee_bsdf, rng, flags;
     DeviceBuffer vol_a, vol_b;    // volume state of the paths that keep one (uint4 each)
+    DeviceBuffer ris_state;       // 3 float4 per path with RIS of the lights
     // rays of one bounce, and the queues of live slots the traversal takes
     DeviceBuffer shadow_rays, shadow_occ, bsdf_rays, bsdf_hits, cont_rays, cont_hits;
     // envmap sampling: the light sample's and the BSDF sample's pending

@@ -54,6 +54,10 @@ constexpr uint32_t FLAG_NEE_FLOOR = 128u;
 constexpr uint32_t FLAG_ENV_LIGHT = 256u;
 constexpr uint32_t FLAG_ENV_BSDF = 512u;
 constexpr uint32_t FLAG_ENV_FLOOR = 1024u;
+// Direct light by RIS this bounce: the light candidates' reservoir is in
+// ris_state, its kept light sample's shadow ray under FLAG_NEE_LIGHT and the
+// BSDF candidate's ray under FLAG_NEE_BSDF.
+constexpr uint32_t FLAG_RIS = 2048u;
 
 // Slots of the live rays of one bounce, per ray kind (segmented lists), and
 // the two lists the boundary-skip rounds alternate between.
@@ -346,6 +350,84 @@ __device__ __forceinline__ void shade_vertex(const PtData& d, int32_t bounce, ui
             write_ray(d.shadow_rays, pix, p, prim, dir, dist - 1.0e-4f);
             shadow_active = true;
         }
+    }
+    else if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0 && (d.ris_all || (d.ris_later && bounce > 0)))
+    {
+        // ---- sample_bsdf_and_lights_RIS_reservoir (RIS/RIS.h:83-244): the
+        // light candidates stream through the reservoir here; the BSDF
+        // candidate's weight needs its ray's hit and joins in resolve_path.
+        // can_do_light_sampling is true with microfacet regularisation
+        // (MaterialUtils.h:71-78). The candidates' evaluations update the
+        // path's volume state, as the reference's do.
+        const float nl = (float)d.ris_light, nb = (float)d.ris_bsdf;
+        float wsum = 0.0f, target_l = 0.0f, cos_l = 0.0f, dist_l = 0.0f;
+        bool has_light = false;
+        Color f_l = pt::col(0.0f), em_l = pt::col(0.0f);
+        f3 dir_l = pt::mk(0.0f, 0.0f, 0.0f);
+        for (int32_t i = 0; i < d.ris_light; i++)
+        {
+            const LightSample ls = sample_emissive(d, rng);
+            float w = 0.0f, target = 0.0f, cos_p = 0.0f, dist = 0.0f;
+            Color f = pt::col(0.0f);
+            f3 dirn = pt::mk(0.0f, 0.0f, 0.0f);
+            if (ls.area_pdf > 0.0f)
+            {
+                const f3 to = ls.point - p;
+                dist = sqrtf(pt::dot(to, to));
+                dirn = pt::vdiv(to, dist);
+                const float cos_at_light = pt::maxf_(0.0f, pt::dot(ls.normal, -1.0f * dirn));
+                cos_p = fabsf(pt::dot(sn, dirn));
+                if (cos_p > 0.0f && cos_at_light > 1.0e-6f)
+                {
+                    pt::BsdfCtx c{view, sn, gn, dirn, pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness, &vs, 0};
+                    float bsdf_pdf = 0.0f;
+                    f = pt::principled_eval(frame_globals(d), m, c, bsdf_pdf);
+                    target = pt::luminance(f * ls.emission * cos_p);
+                    const float spdf =
+                        cos_at_light < 1.0e-8f ? 0.0f : ls.area_pdf * (dist * dist) / cos_at_light;
+                    w = balance_heuristic(spdf, nl, bsdf_pdf, nb) * target / spdf;
+                }
+            }
+            wsum += w;
+            if (rng() < w / wsum)
+            {
+                has_light = true;
+                f_l = f;
+                em_l = ls.emission;
+                target_l = target;
+                cos_l = cos_p;
+                dir_l = dirn;
+                dist_l = dist;
+            }
+        }
+        float u_b = 0.0f;
+        if (d.ris_bsdf > 0)
+        {
+            pt::BsdfCtx c{view, sn, gn, pt::mk(0.0f, 0.0f, 0.0f), pt::NO_INFO, pt::REGULARIZATION_MIS, path_roughness,
+                          &vs, 0};
+            f3 dir;
+            float bpdf;
+            const Color f = pt::principled_sample(frame_globals(d), m, c, dir, bpdf, rng);
+            if (bpdf > 0.0f)
+            {
+                d.nee_bsdf[pix] = stc(f, bpdf);
+                write_ray(d.bsdf_rays, pix, p, prim, dir, 1.0e35f - 1.0e-4f);
+                bsdf_active = true;
+                flags |= FLAG_NEE_BSDF;
+            }
+            u_b = rng();
+        }
+        if (has_light)
+        {
+            write_ray(d.shadow_rays, pix, p, prim, dir_l, dist_l - 1.0e-4f);
+            shadow_active = true;
+            flags |= FLAG_NEE_LIGHT;
+        }
+        float4* rs = d.ris_state + 3 * (size_t)pix;
+        rs[0] = stc(f_l, cos_l);
+        rs[1] = stc(em_l, target_l);
+        rs[2] = make_float4(wsum, u_b, has_light ? 1.0f : 0.0f, 0.0f);
+        flags |= FLAG_RIS;
     }
     else if ((bounce > 0 || d.enable_direct) && d.emissive_count > 0)
     {
@@ -733,6 +815,67 @@ __device__ __forceinline__ int next_vertex(const PtData& d, uint32_t pix, uint32
 
 // Direct lighting of the bounce (Lights.h:364-398) and the next vertex;
 // returns the path's state after the continuation ray (PATH_*).
+// The BSDF candidate joins the RIS reservoir (RIS/RIS.h:196-241), then the
+// kept sample is shaded (RIS.h:18-80): a light sample through its shadow ray,
+// a BSDF sample with the contribution and cosine of its candidate.
+__device__ __forceinline__ Color resolve_ris(const PtData& d, uint32_t pix, uint32_t flags)
+{
+    const float4* rs = d.ris_state + 3 * (size_t)pix;
+    const float4 s2 = rs[2];
+    const float nl = (float)d.ris_light, nb = (float)d.ris_bsdf;
+    float wsum = s2.x;
+    bool bsdf_kept = false;
+    Color f_b = pt::col(0.0f), em_b = pt::col(0.0f);
+    float cos_b = 0.0f, target_b = 0.0f;
+    if (d.ris_bsdf > 0)
+    {
+        float w = 0.0f;
+        bool hit_light = false;
+        if (flags & FLAG_NEE_BSDF)
+        {
+            const int4 h = d.bsdf_hits[pix];
+            if (h.x >= 0)
+            {
+                const int32_t hp = h.x;
+                const Color emission = d.mats[d.mat_idx[hp]].emission;
+                const float4 rb = d.bsdf_rays[2 * (size_t)pix + 1];
+                const f3 dir = pt::mk(rb.x, rb.y, rb.z);
+                f3 a, b, c;
+                tri_verts(d, hp, a, b, c);
+                const f3 hgn = pt::normalize(pt::normalize(pt::cross(b - a, c - a)));
+                const float cos_at_light = pt::maxf_(0.0f, pt::dot(hgn, -1.0f * dir));
+                if (!pt::is_black(emission) && cos_at_light > 0.0f)
+                {
+                    const float4 nbv = d.nee_bsdf[pix];
+                    const float bpdf = nbv.w;
+                    f_b = ldc(nbv);
+                    em_b = emission;
+                    cos_b = fabsf(pt::dot(ld3(d.vtx_sn[pix]), dir));
+                    target_b = pt::luminance(f_b * emission * cos_b);
+                    const float pdf_area = emissive_hit_area_pdf(d, d.areas[hp], emission);
+                    const float t = __int_as_float(h.y);
+                    const float lpdf = cos_at_light < 1.0e-8f ? 0.0f : pdf_area * (t * t) / cos_at_light;
+                    w = balance_heuristic(bpdf, nb, lpdf, nl) * target_b / bpdf;
+                    hit_light = true;
+                }
+            }
+        }
+        wsum += w;
+        if (s2.y < w / wsum)
+            bsdf_kept = hit_light;
+    }
+    const float4 s0 = rs[0], s1 = rs[1];
+    const float target = bsdf_kept ? target_b : s1.w;
+    const float ucw = wsum == 0.0f ? 0.0f : 1.0f / target * wsum;
+    if (!(ucw > 0.0f))
+        return pt::col(0.0f);
+    if (bsdf_kept)
+        return f_b * ucw * em_b * cos_b;
+    if (s2.z == 0.0f || d.shadow_occ[pix])
+        return pt::col(0.0f);
+    return ldc(s0) * ucw * ldc(s1) * s0.w;
+}
+
 __device__ __forceinline__ int resolve_path(const PtData& d, int32_t bounce, const float4* __restrict__ thr_prev,
                                             uint32_t pix)
 {
@@ -747,7 +890,9 @@ __device__ __forceinline__ int resolve_path(const PtData& d, int32_t bounce, con
         if ((flags & FLAG_NEE_LIGHT) && !d.shadow_occ[pix])
             light_mis = ldc(d.nee_light[pix]);
         Color bsdf_mis = pt::col(0.0f);
-        if (flags & FLAG_NEE_BSDF)
+        if (flags & FLAG_RIS)
+            light_mis = resolve_ris(d, pix, flags);
+        else if (flags & FLAG_NEE_BSDF)
         {
             const int4 h = d.bsdf_hits[pix];
             if (h.x >= 0)
@@ -1128,6 +1273,11 @@ PtData make_pt_data(Ctx& ctx, uint32_t sample_number, uint32_t random_number)
     d.env_light = R.env_light.as<float4>();
     d.env_bsdf = R.env_bsdf.as<float4>();
     d.restir = R.settings.direct_light_sampling_strategy == MIPT_LSS_RESTIR_DI;
+    d.ris_all = R.settings.direct_light_sampling_strategy == MIPT_LSS_RIS_BSDF_AND_LIGHT;
+    d.ris_later = d.restir && R.restir.later_bounces_sampling_strategy == MIPT_RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT;
+    d.ris_light = R.settings.ris_number_of_light_candidates;
+    d.ris_bsdf = R.settings.ris_number_of_bsdf_candidates;
+    d.ris_state = R.ris_state.as<float4>();
     d.final_shading_visibility = R.restir.do_final_shading_visibility;
     d.restir_out = d.restir ? restir_reservoir_buffer(ctx, R.restir_shade_buffer) : nullptr;
     d.fb = R.fb.as<float>();

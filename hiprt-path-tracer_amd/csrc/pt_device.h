@@ -89,6 +89,13 @@ struct PtData {
     EnvDev env;
     float4* env_light;
     float4* env_bsdf;
+    // RIS of the lights (sample_lights_RIS, RIS/RIS.h:252): at every bounce
+    // (LSS_RIS_BSDF_AND_LIGHT) or after bounce 0 under ReSTIR DI; candidate
+    // counts; per path, the light candidate kept and the reservoir's state
+    // until the BSDF candidate's ray is back (3 float4)
+    int32_t ris_all, ris_later;
+    int32_t ris_light, ris_bsdf;
+    float4* ris_state;
     // ReSTIR DI at bounce 0 (LSS_RESTIR_DI): the reservoirs to shade
     int32_t restir;
     int32_t final_shading_visibility;

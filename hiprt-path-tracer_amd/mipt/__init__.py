@@ -125,6 +125,7 @@ class RenderSettings(C.Structure):
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
         ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
         ("do_normal_mapping", C.c_int32), ("accumulate", C.c_int32),
+        ("ris_number_of_light_candidates", C.c_int32), ("ris_number_of_bsdf_candidates", C.c_int32),
     ]
 
 
@@ -144,7 +145,7 @@ class RestirDISettings(C.Structure):
         ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
         ("envmap_candidate_probability", C.c_float), ("use_permutation_sampling", C.c_int32),
         ("permutation_sampling_random_bits", C.c_int32), ("max_neighbor_search_count", C.c_int32),
-        ("neighbor_search_radius", C.c_int32),
+        ("neighbor_search_radius", C.c_int32), ("later_bounces_sampling_strategy", C.c_int32),
     ]
 
 
@@ -175,7 +176,10 @@ def MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(i: int) -> int:
 
 
 MIPT_LSS_MIS_LIGHT_BSDF = 3
+MIPT_LSS_RIS_BSDF_AND_LIGHT = 4
 MIPT_LSS_RESTIR_DI = 5
+MIPT_RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF = 2
+MIPT_RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT = 3
 MIPT_LSS_BASE_UNIFORM = 0
 MIPT_LSS_BASE_POWER = 1
 

@@ -382,10 +382,17 @@ typedef struct mipt_render_settings {
      * frame's camera to find its temporal neighbour
      * (find_temporal_neighbor_index, UtilsTemporal.h:37-99). */
     int32_t accumulate;
+    /* RISSettings (RIS/RISSettings.h:9-16, RenderSettings.h:206): candidates
+     * of resampled importance sampling of the lights (MIPT_LSS_RIS_BSDF_AND_LIGHT,
+     * and ReSTIR DI's later bounces), defaults 4 light and 1 BSDF candidate
+     * (0 or 1 BSDF candidate in this build). */
+    int32_t ris_number_of_light_candidates;
+    int32_t ris_number_of_bsdf_candidates;
 } mipt_render_settings;
 
 #define MIPT_LSS_MIS_LIGHT_BSDF 3 /* LSS_MIS_LIGHT_BSDF: one light sample + one BSDF sample, balance heuristic */
-#define MIPT_LSS_RESTIR_DI 5      /* LSS_RESTIR_DI: ReSTIR DI reservoir at bounce 0, MIS light + BSDF after */
+#define MIPT_LSS_RIS_BSDF_AND_LIGHT 4 /* LSS_RIS_BSDF_AND_LIGHT: RIS over light + BSDF candidates (sample_lights_RIS, RIS/RIS.h:252) */
+#define MIPT_LSS_RESTIR_DI 5      /* LSS_RESTIR_DI: ReSTIR DI reservoir at bounce 0, later_bounces_sampling_strategy after */
 #define MIPT_LSS_BASE_UNIFORM 0   /* LSS_BASE_UNIFORM: emissive triangle picked uniformly */
 #define MIPT_LSS_BASE_POWER 1     /* LSS_BASE_POWER: picked by emitted power (alias table) */
 
@@ -396,9 +403,8 @@ void mipt_render_settings_default(mipt_render_settings* s);
  * options of ReSTIRDIOptions.h are fixed at the reference defaults:
  * pairwise MIS bias correction with visibility, visibility reuse, no
  * visibility in the initial or spatial target functions, presampled lights
- * drawn by power (when do_light_presampling is on). Later bounces use MIS
- * light + BSDF
- * (RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF). Defaults:
+ * drawn by power (when do_light_presampling is on). Later bounces use
+ * later_bounces_sampling_strategy (RIS by default, as the reference). Defaults:
  * mipt_restir_di_settings_default (the reference's). */
 typedef struct mipt_restir_di_settings {
     int32_t number_of_initial_light_candidates; /* 4 */
@@ -456,7 +462,15 @@ typedef struct mipt_restir_di_settings {
     int32_t permutation_sampling_random_bits;
     int32_t max_neighbor_search_count;
     int32_t neighbor_search_radius;
+    /* Kernel option ReSTIR_DI_LaterBouncesSamplingStrategy (ReSTIRDIOptions.h:
+     * 124-140): the direct light of the bounces after the first,
+     * MIPT_RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT (the default: RIS with
+     * render_settings.ris_* candidates) or ..._MIS_LIGHT_BSDF. */
+    int32_t later_bounces_sampling_strategy;
 } mipt_restir_di_settings;
+
+#define MIPT_RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF 2    /* RESTIR_DI_LATER_BOUNCES_MIS_LIGHT_BSDF */
+#define MIPT_RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT 3 /* RESTIR_DI_LATER_BOUNCES_RIS_BSDF_AND_LIGHT */
 
 void mipt_restir_di_settings_default(mipt_restir_di_settings* s);
 /* Takes effect at the next pass; allocates the reservoir buffers when the

@@ -184,6 +184,8 @@ typedef struct {
     int32_t radiance_cache;                      /* not read by the oracle */
     int32_t do_normal_mapping;                   /* RenderSettings.h:203: smooth vertex normals */
     int32_t accumulate;                          /* RenderSettings.h:73: 0 = the camera may move */
+    int32_t ris_number_of_light_candidates;      /* RISSettings (RIS/RISSettings.h:9-16) */
+    int32_t ris_number_of_bsdf_candidates;
 } orc_render_settings;
 
 /* ReSTIRDISettings subset (ReSTIRDISettings.h:17-163), same order as the
@@ -216,6 +218,7 @@ typedef struct {
     int32_t permutation_sampling_random_bits;
     int32_t max_neighbor_search_count;
     int32_t neighbor_search_radius;
+    int32_t later_bounces_sampling_strategy;     /* 2 MIS light + BSDF, 3 RIS */
 } orc_restir_settings;
 
 /* The BSDF tables (the .f32 files of hiprt-path-tracer_amd/data, converted from the

@@ -41,6 +41,7 @@ class RenderSettings(C.Structure):
         ("accumulate_fp64", C.c_int32), ("direct_light_sampling_strategy", C.c_int32),
         ("direct_light_sampling_base_strategy", C.c_int32), ("radiance_cache", C.c_int32),
         ("do_normal_mapping", C.c_int32), ("accumulate", C.c_int32),
+        ("ris_number_of_light_candidates", C.c_int32), ("ris_number_of_bsdf_candidates", C.c_int32),
     ]
 
 
@@ -58,7 +59,7 @@ class RestirSettings(C.Structure):
         ("light_presampling_subset_size", C.c_int32), ("light_presampling_tile_size", C.c_int32),
         ("envmap_candidate_probability", C.c_float), ("use_permutation_sampling", C.c_int32),
         ("permutation_sampling_random_bits", C.c_int32), ("max_neighbor_search_count", C.c_int32),
-        ("neighbor_search_radius", C.c_int32),
+        ("neighbor_search_radius", C.c_int32), ("later_bounces_sampling_strategy", C.c_int32),
     ]
 
 

@@ -251,6 +251,8 @@ typedef struct {
     /* LSS_RESTIR_DI: reservoirs shaded at bounce 0 (struct ores, restir part) */
     const void* restir_out;
     int32_t final_shading_visibility;
+    /* ReSTIR DI's later bounces by RIS (ReSTIR_DI_LaterBouncesSamplingStrategy) */
+    int32_t ris_later;
     /* ambient light (WorldSettings.h:21-53): packed texels, alias table */
     int32_t env_type;
     float env_uniform[3];
@@ -1998,6 +2000,116 @@ static rgb sample_one_light_mis(const rctx* R, const omat* m, const hitinfo* hi,
     return cadd(light_mis, bsdf_mis);
 }
 
+/* Sampling.h:103-116 */
+static float balance_heuristic(float pa, float na, float pb, float nb)
+{
+    return pa == 0.0f ? 0.0f : pa / (na * pa + nb * pb);
+}
+
+/* sample_lights_RIS (RIS/RIS.h:83-258): light candidates, then the BSDF
+ * candidate, streamed through one reservoir (RIS_Reservoir.h:29-45: every
+ * candidate draws its selection number, selected when u < w / wsum), then the
+ * kept sample shaded (RIS.h:18-80). can_do_light_sampling is true with
+ * microfacet regularisation (MaterialUtils.h:71-78). The candidates' BSDF
+ * evaluations take the path's volume state by reference, as in the reference. */
+static rgb sample_lights_ris(const rctx* R, const omat* m, const hitinfo* hi, v3 view, float path_r, uint32_t* st,
+                             ovol* vs)
+{
+    const int32_t nlc = R->s.ris_number_of_light_candidates, nbc = R->s.ris_number_of_bsdf_candidates;
+    const float nl = (float)nlc, nb = (float)nbc;
+    float wsum = 0.0f;
+    /* the kept sample */
+    int is_bsdf = 0;
+    v3 point = V(0, 0, 0);
+    rgb emission = C1(0.0f), fb = C1(0.0f);
+    float target = 0.0f, cos_b = 0.0f;
+    for (int32_t i = 0; i < nlc; i++)
+    {
+        olight L = sample_emissive(R, st);
+        float tf = 0.0f, w = 0.0f;
+        if (L.area_pdf > 0.0f)
+        {
+            v3 to = vsub(L.point, hi->p);
+            float dist = vlen(to);
+            v3 dn = vdivs(to, dist);
+            float cl = hmax(0.0f, vdot(L.normal, vneg(dn)));
+            float cp = fabsf(vdot(hi->sn, dn));
+            if (cp > 0.0f && cl > 1.0e-6f)
+            {
+                bctx c = {view, hi->sn, hi->gn, dn, NO_INFO, REG_MIS, path_r, vs, 0};
+                float bpdf = 0.0f;
+                rgb f = principled_eval(R, m, &c, &bpdf);
+                tf = lum(cscale(cmul(f, L.emission), cp));
+                float spdf = cl < 1.0e-8f ? 0.0f : L.area_pdf * (dist * dist) / cl;
+                w = balance_heuristic(spdf, nl, bpdf, nb) * tf / spdf;
+            }
+        }
+        wsum += w;
+        if (orc_rand01(st) < w / wsum)
+        {
+            is_bsdf = 0;
+            point = L.point;
+            emission = L.emission;
+            target = tf;
+        }
+    }
+    for (int32_t i = 0; i < nbc; i++)
+    {
+        bctx c = {view, hi->sn, hi->gn, V(0, 0, 0), NO_INFO, REG_MIS, path_r, vs, 0};
+        v3 dir;
+        float bpdf = 0.0f, w = 0.0f, tf = 0.0f, cp = 0.0f;
+        rgb f = principled_sample(R, m, &c, &dir, &bpdf, st);
+        rgb em = C1(0.0f);
+        int hit_light = 0;
+        if (bpdf > 0.0f)
+        {
+            /* evaluate_bsdf_light_sample_ray, CPU branch (Intersect.h:484-535) */
+            orc_hit h = trace1(R, hi->p, dir, hi->prim, INFINITY);
+            if (h.prim >= 0 && h.t < 1.0e35f - 1.0e-4f)
+            {
+                em = R->mats[R->mat_idx[h.prim]].emission;
+                v3 a, b, cc;
+                tri(R, h.prim, &a, &b, &cc);
+                v3 hgn = vnorm(vnorm(vcross(vsub(b, a), vsub(cc, a))));
+                float cl = hmax(0.0f, vdot(hgn, vneg(dir)));
+                if (!cblack(em) && cl > 0.0f)
+                {
+                    cp = fabsf(vdot(hi->sn, dir));
+                    tf = lum(cscale(cmul(f, em), cp));
+                    float pa = emissive_hit_area_pdf(R, R->areas[h.prim], em);
+                    float lpdf = cl < 1.0e-8f ? 0.0f : pa * (h.t * h.t) / cl;
+                    w = balance_heuristic(bpdf, nb, lpdf, nl) * tf / bpdf;
+                    hit_light = 1;
+                }
+            }
+        }
+        wsum += w;
+        if (orc_rand01(st) < w / wsum)
+        {
+            is_bsdf = hit_light;
+            emission = hit_light ? em : C1(0.0f);
+            fb = f;
+            cos_b = cp;
+            target = tf;
+        }
+    }
+    float ucw = wsum == 0.0f ? 0.0f : 1.0f / target * wsum;
+    if (!(ucw > 0.0f))
+        return C1(0.0f);
+    if (is_bsdf)
+        return cscale(cmul(cscale(fb, ucw), emission), cos_b);
+    v3 to = vsub(point, hi->p);
+    float dist = vlen(to);
+    v3 dn = vdivs(to, dist);
+    orc_hit sh = trace1(R, hi->p, dn, hi->prim, INFINITY);
+    if (sh.prim >= 0 && sh.t < dist - 1.0e-4f)
+        return C1(0.0f);
+    bctx c = {view, hi->sn, hi->gn, dn, NO_INFO, REG_MIS, path_r, vs, 0};
+    float bpdf;
+    rgb f = principled_eval(R, m, &c, &bpdf);
+    return cscale(cmul(cscale(f, ucw), emission), fabsf(vdot(hi->sn, dn)));
+}
+
 /* ------------------------------------------------------------------ */
 /* ReSTIR DI reservoir (DI/Reservoir.h:52-187) and its final shading    */
 /* (DI/FinalShading.h:16-125)                                           */
@@ -2156,6 +2268,8 @@ static rgb megakernel_pixel(const rctx* R, int x, int y, uint32_t rn, const gbuf
             rgb direct = C1(0.0f);
             if ((R->ne > 0 || env_on(R)) && R->restir_out && bounce == 0)
                 direct = restir_final_shading(R, m, &hi, view, pix, &vs);
+            else if (R->ne > 0 && (R->s.direct_light_sampling_strategy == 4 || (R->ris_later && bounce > 0)))
+                direct = cadd(C1(0.0f), sample_lights_ris(R, m, &hi, view, path_r, &st, &vs));
             else if (R->ne > 0)
                 direct = cadd(C1(0.0f), sample_one_light_mis(R, m, &hi, view, path_r, &st, &vs));
             /* sample_environment_map (Envmap.h:258-276): ReSTIR DI covers
@@ -3688,6 +3802,7 @@ int32_t orc_frame_path_trace(orc_frame* f, uint32_t sample_number, uint32_t rn, 
     f->R.consistent_tau_refract = orc_consistent_tau(f->R.s.microfacet_regularization_tau_0, sample_number + 1);
     f->R.restir_out = restir_on(f) ? f->res[f->shade_buffer] : NULL;
     f->R.final_shading_visibility = f->rs.do_final_shading_visibility;
+    f->R.ris_later = restir_on(f) && f->rs.later_bounces_sampling_strategy == 3;
     int32_t rejected = 0;
     const int w = f->R.s.width;
     set_threads(nthreads);
@@ -3774,7 +3889,8 @@ static int32_t render_sample_via_frame(const orc_render_scene* sc, const orc_oct
                                        uint32_t rn_path, float* fb, double* fb64, int32_t* gb_prims, int32_t nthreads)
 {
     orc_render_settings plain = *s;
-    plain.direct_light_sampling_strategy = 3;
+    if (plain.direct_light_sampling_strategy != 4)
+        plain.direct_light_sampling_strategy = 3;
     orc_frame* f = orc_frame_create(sc, tree, cam, &plain, NULL);
     orc_frame_camera_pass(f, sample_number, rn_camera, nthreads);
     if (gb_prims)

@@ -252,9 +252,10 @@ def test_restir_pass_errors(mipt_mod, torch_mod):
         ctx.launch_pass(mipt_mod.MIPT_PASS_RESTIR_DI_SPATIAL_REUSE(1), 0, 1)
     with pytest.raises(mipt_mod.MiptError):
         ctx.set_restir_di_settings(mipt_mod.default_restir_di_settings(spatial_reuse_neighbor_count=33))
-    bad = mipt_mod.default_render_settings(width=32, height=32, direct_light_sampling_strategy=4)
-    with pytest.raises(mipt_mod.MiptError):
-        ctx.set_render_settings(bad)
+    for strategy in (0, 1, 2, 6):  # no light sampling, one light without MIS, BSDF only: not built; 6: none
+        bad = mipt_mod.default_render_settings(width=32, height=32, direct_light_sampling_strategy=strategy)
+        with pytest.raises(mipt_mod.MiptError):
+            ctx.set_render_settings(bad)
     # the fused and presampling passes only run when their settings ask
     for p in (mipt_mod.MIPT_PASS_RESTIR_DI_SPATIOTEMPORAL_REUSE, mipt_mod.MIPT_PASS_RESTIR_DI_LIGHTS_PRESAMPLING):
         with pytest.raises(mipt_mod.MiptError):
